@@ -1,0 +1,217 @@
+/*
+ * meshfemsparse_amd.h -- C ABI of the MI355X (gfx950) per-Newton-step linear-solve pipeline.
+ *
+ * Three stages, each a hand-written HIP kernel family behind this boundary:
+ *   1. block-CSC Hessian assembly (3x3-block scatter-add through precomputed slot maps),
+ *   2. supernodal Cholesky numeric factorization (ordering + symbolic analysis on the host, once),
+ *   3. supernodal forward/back solves (+ symmetric block SpMV and FP64 iterative refinement).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: MFS_OK (0) or a negative MFS_ERR_* code;
+ *     mfs_last_error() returns a thread-local message for the most recent failure.
+ *   - Host arrays are caller-owned and only read during the call.
+ *   - Arguments named *_dev are device pointers (hipMalloc'd / torch CUDA tensors) on the
+ *     current HIP device; `stream` is a hipStream_t (NULL = default stream).  Device work is
+ *     enqueued on `stream`; calls that must report a result to the host (numeric factorization
+ *     pivot status, refinement residuals) synchronize that stream before returning.
+ *   - Sparse pattern arrays follow the reference's SuiteSparse_long convention (int64).
+ *   - Block-CSC value layout ("ContiguousBlocks"): upper-triangle block pattern Ap/Ai over block
+ *     indices; block k occupies Ax[9k .. 9k+8] as a column-major 3x3; the diagonal block is the
+ *     last entry of its column and only its upper triangle is meaningful.
+ *   - A handle must not be used by two host threads at once (it owns mutable scratch), matching
+ *     the reference factorizer's threading contract.
+ */
+#ifndef MESHFEMSPARSE_AMD_H
+#define MESHFEMSPARSE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* Status codes                                                                                */
+/* ------------------------------------------------------------------------------------------ */
+#define MFS_OK               0
+#define MFS_ERR_ARG         -1  /* invalid argument / size mismatch      (std::invalid_argument) */
+#define MFS_ERR_STATE       -2  /* missing factorization / wrong phase   (std::runtime_error)    */
+#define MFS_ERR_NOT_SPD     -3  /* pivot failure; see mfs_num_successful_pivots                  */
+#define MFS_ERR_DEVICE      -4  /* HIP runtime error                                             */
+#define MFS_ERR_ALLOC       -5  /* host or device allocation failure                             */
+#define MFS_ERR_UNSUPPORTED -6  /* e.g. partially pinned blocks, non-uniform block sizes         */
+
+/* Factor precisions */
+#define MFS_PREC_FP64 0         /* plain FP64 path: the drop-in                                   */
+#define MFS_PREC_FP32 1         /* FP32 factor after symmetric diagonal scaling, FP64 refinement  */
+
+/* Element-Hessian layouts accepted by mfs_assemble_hessian */
+#define MFS_HE_FULL12   0       /* 144 doubles per element: 12x12 column-major (eval_He(ei))      */
+#define MFS_HE_UPPER78  1       /* 78 doubles per element: local-upper blocks, packed (see below) */
+
+typedef struct mfs_pattern_s*    mfs_pattern;
+typedef struct mfs_assembler_s*  mfs_assembler;
+typedef struct mfs_factorizer_s* mfs_factorizer;
+
+const char* mfs_last_error(void);
+int         mfs_version(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Block sparsity pattern                                                                      */
+/* Replaces SystemAssembler<3>::blockSparsityPattern (src/lib/MeshFEMSparse/SystemAssembler.hh:251-465): */
+/* per-element sorted stencils -> column buckets (rows <= column) -> dedupe + sort -> Ap/Ai.   */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_pattern_build(const int32_t* elem_nodes, int64_t num_elems, int32_t nodes_per_elem,
+                      int64_t num_blocks, mfs_pattern* out);
+/* nz = number of stored upper blocks; Ap has num_blocks+1 entries, Ai has nz entries.       */
+int mfs_pattern_get(mfs_pattern p, int64_t* num_blocks, int64_t* nz,
+                    const int64_t** Ap, const int64_t** Ai);
+int mfs_pattern_destroy(mfs_pattern p);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Hessian assembly                                                                            */
+/* Replaces SystemAssembler<3>::assembleHessian -> assembleHessianBlockAccelerated ->          */
+/* ElementHessianContribAssembler<true>::run (SystemAssembler.hh:60-97, 524-565).             */
+/* Accumulates: Ax += sum_e scatter(He_e), reading only the local-upper blocks of He_e        */
+/* (off-diagonal (a,b) with a<b, transposed when the global order is reversed; upper triangle */
+/* of the 4 diagonal blocks).  Contributions to each 3x3 slot are added in ascending element  */
+/* order, so the result equals a serial element-order run of the reference bit for bit.       */
+/* MFS_HE_UPPER78 packing per element: the 6 off-diagonal blocks (a,b), a<b in order           */
+/* (0,1),(0,2),(0,3),(1,2),(1,3),(2,3) as column-major 3x3 (54 doubles), then the 4 diagonal  */
+/* blocks' upper triangles in column-major order (0,0),(0,1),(1,1),(0,2),(1,2),(2,2) (24).    */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_assembler_create(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
+                         const int32_t* elem_nodes, int64_t num_elems, int32_t nodes_per_elem,
+                         mfs_assembler* out);
+int mfs_assemble_hessian(mfs_assembler a, const double* He_dev, int he_layout,
+                         double* Ax_dev, void* stream);
+/* Ax = 0 over the 9*nz values (BlockCSCHessianBase::setZero, BlockCSCHessian.hh:638-649).    */
+int mfs_assembler_zero(mfs_assembler a, double* Ax_dev, void* stream);
+int mfs_assembler_destroy(mfs_assembler a);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Factorizer                                                                                  */
+/* Replaces CholeskyFactorizerBase / CatamariFactorizer                                        */
+/* (src/lib/MeshFEMSparse/Solvers/CholeskyFactorizerBase.hh:50-443,                            */
+/*  src/lib/MeshFEMSparse/Solvers/CatamariFactorizer.cc:136-948).                              */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_factorizer_create(int precision, mfs_factorizer* out);
+int mfs_factorizer_destroy(mfs_factorizer f);
+
+/* factorizeSymbolic(const BlockCSCHessianBase&, pinnedScalarVars) (CatamariFactorizer.cc:136-616):
+ * whole-block pins -> block row/col removal (SparseMatrices.hh:2020-2076) -> nested-dissection
+ * ordering -> elimination tree, relaxed supernodes, row structures, extend-add maps, injection
+ * plan, level schedule.  Host arrays.  block_size must be 3.  Partially pinned blocks return
+ * MFS_ERR_UNSUPPORTED (the reference falls back to a scalar factorization there).            */
+int mfs_factorize_symbolic(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai,
+                           int64_t num_blocks, int32_t block_size,
+                           const int64_t* pinned_scalar_vars, int64_t num_pins);
+
+/* factorizeNumeric / factorizeNumericWithShift(A, sigma[, B]) (CholeskyFactorizerBase.hh:93-110,
+ * CatamariFactorizer.cc:618-714): factor P(A + sigma*I)P^T, or P(A + sigma*B)P^T when Bx_dev is
+ * non-NULL (B has A's pattern and layout).  Ax_dev holds the unreduced 9*nz block values.
+ * On a pivot failure returns MFS_ERR_NOT_SPD and drops back to the symbolic state.           */
+int mfs_factorize_numeric(mfs_factorizer f, const double* Ax_dev, double sigma,
+                          const double* Bx_dev, void* stream);
+
+/* CholeskyFactorizerBase::solve (CholeskyFactorizerBase.hh:262-287) with the fused pin removal
+ * and permutation (removeFixedEntries/extractFullSolution, :298-363): b, x are full-length
+ * (num_blocks*3 per column, column-major, nrhs columns); pinned entries of x are set to 0.
+ * solveMultiRHS (CatamariFactorizer.cc:862-907) is the nrhs > 1 case.  b and x may alias.   */
+int mfs_solve(mfs_factorizer f, const double* b_dev, double* x_dev, int32_t nrhs, void* stream);
+
+/* solveRawReduced / solveRawReducedInPlace (CatamariFactorizer.cc:788-860): reduced-length
+ * vectors; already_permuted = vectors are in factor order.  b and x may alias (in place).    */
+int mfs_solve_raw_reduced(mfs_factorizer f, const double* b_dev, double* x_dev, int32_t nrhs,
+                          int32_t already_permuted, void* stream);
+
+/* removeFixedEntries / extractFullSolution (CholeskyFactorizerBase.hh:298-363).              */
+int mfs_remove_fixed_entries(mfs_factorizer f, const double* x_full_dev, double* x_reduced_dev,
+                             int32_t nrhs, int32_t permute, void* stream);
+int mfs_extract_full_solution(mfs_factorizer f, const double* x_reduced_dev, double* x_full_dev,
+                              int32_t nrhs, int32_t permute, void* stream);
+
+/* Symmetric block SpMV from the upper blocks: y = A x over the full (unreduced) pattern the
+ * factorizer was analysed with (BlockCSCHessian::applyRaw, BlockCSCHessian.hh:1059-1120).
+ * Each output row sums its blocks in a fixed order (deterministic).                          */
+int mfs_spmv(mfs_factorizer f, const double* Ax_dev, const double* x_dev, double* y_dev,
+             void* stream);
+
+/* Solve A x = b (full-length, one RHS) to ||b - A x||/||b|| <= tol over the free rows, by the
+ * factor plus FP64 iterative refinement with the FP64 SpMV residual.  For an FP32 factor this
+ * is the mixed-precision path; for FP64 it verifies (and if needed refines) the direct solve.
+ * iters = refinement corrections applied after the first solve; relres = final residual.   */
+int mfs_solve_refined(mfs_factorizer f, const double* Ax_dev, const double* b_dev, double* x_dev,
+                      double tol, int32_t max_iters, int32_t* iters, double* relres, void* stream);
+
+/* State / statistics (CholeskyFactorizerBase.hh:55-59,193-260; CatamariFactorizer.cc:716-948). */
+int mfs_factorization_type(mfs_factorizer f, int32_t* type);   /* 0 none, 1 symbolic, 2 numeric */
+int mfs_num_successful_pivots(mfs_factorizer f, int64_t* n);
+int mfs_sizes(mfs_factorizer f, int64_t* n_full, int64_t* n_reduced);
+int mfs_factor_nnz(mfs_factorizer f, int64_t* nnz);             /* scalar nnz(L), incl. diagonal */
+int mfs_flop_estimate(mfs_factorizer f, double* flops);         /* POTRF+TRSM+SYRK, per supernode */
+int mfs_get_inverse_permutation(mfs_factorizer f, int64_t* iperm_scalar); /* new -> old reduced */
+int mfs_clear_factors(mfs_factorizer f);
+int mfs_stash_factorization(mfs_factorizer f);
+int mfs_has_stashed_factorization(mfs_factorizer f, int32_t* has);
+int mfs_swap_stashed_factorization(mfs_factorizer f);
+int mfs_clear_stashed_factorization(mfs_factorizer f);
+
+/* Supernode statistics of the symbolic analysis: number of supernodes, elimination-tree
+ * levels, max supernode width and max row count below a supernode (scalars).               */
+int mfs_supernode_stats(mfs_factorizer f, int64_t* num_supernodes, int64_t* num_levels,
+                        int64_t* max_width, int64_t* max_degree);
+
+/* Host-only symbolic analysis (no device work): the ordering and supernode statistics the
+ * factorizer would use.  iperm_blocks[n_reduced_blocks] = reduced block at factor position k;
+ * stats[8] = {n_reduced_blocks, num_supernodes, num_levels, factor_nnz (scalar), max_width,
+ * max_degree, L doubles, U arena doubles}; flops[2] = {supernodal POTRF+TRSM+SYRK flops,
+ * sum over scalar columns of c_j^2 for the unrelaxed column counts c_j}.  Any output may be NULL. */
+int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
+                     const int64_t* pinned_scalar_vars, int64_t num_pins, int64_t* iperm_blocks,
+                     int64_t* stats, double* flops);
+
+/* Copy the numeric factor L (lower, factor order, incl. diagonal; explicit zeros from supernode
+ * relaxation included) to host CSC arrays: Lp[n_reduced+1], Li[nnz], Lx[nnz].  For testing.  */
+int mfs_factor_export_csc(mfs_factorizer f, int64_t* Lp, int64_t* Li, double* Lx);
+
+/* Per-stage device time of the most recent numeric factorization / solve, milliseconds, from
+ * HIP events on the caller's stream: [0] injection, [1] extend-add, [2] dense panels, [3] total.*/
+int mfs_last_timings(mfs_factorizer f, double* ms4);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Workload generator (the reference has no energy code; the caller supplies He).             */
+/* Unit cube, N^3 hexes, each split into 6 Kuhn tets; node id = x + (N+1)(y + (N+1) z).       */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_cube_mesh(int32_t N, int32_t* tets /* 6N^3 x 4 */, double* X /* (N+1)^3 x 3 */);
+/* X += sigma_rel * h * N(0,1) per coordinate, from a splitmix64 stream seeded by `seed`.    */
+int mfs_perturb_nodes(int32_t N, uint64_t seed, double sigma_rel, double* X);
+/* Compressible neo-Hookean psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2 per linear tet:
+ * element Hessians (144 doubles each, col-major 12x12, dof 3a+i) and/or the assembled
+ * gradient (num_nodes*3).  E_per_elem/nu_per_elem may be NULL to use E, nu.  Host, threaded.*/
+int mfs_neohookean_host(const double* X_rest, const double* X, const int32_t* tets, int64_t ne,
+                        const double* E_per_elem, const double* nu_per_elem, double E, double nu,
+                        double* He_out, double* grad_out, double* energy_out, int32_t num_threads);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Test hooks: the device kernels' building blocks exercised directly.                        */
+/* ------------------------------------------------------------------------------------------ */
+/* C(MxN) = alpha * A(MxK) * B(NxK)^T + beta * C, column-major, through the batched MFMA tile
+ * kernel used by the factorization.                                                         */
+int mfs_test_gemm_nt(int32_t M, int32_t N, int32_t K, double alpha, const double* A_dev,
+                     int32_t lda, const double* B_dev, int32_t ldb, double beta, double* C_dev,
+                     int32_t ldc, void* stream);
+/* trs_kernels::MultiplyLowerBlock semantics (block-compressed row indices, BS=1..3):
+ *   B(row_i, :) -= A(i, :) * B(supernode_start : supernode_start + size, :),
+ *   row_i = indices[i / BS] + i % BS  for i < degree,
+ * through the row-dot device routine of the forward-solve update kernel.                    */
+int mfs_test_multiply_lower_block(int32_t BS, const int64_t* indices_dev, int64_t supernode_start,
+                                  int64_t supernode_size, int64_t degree, const double* A_dev,
+                                  int64_t lda, int32_t nrhs, double* B_dev, int64_t ldb,
+                                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MESHFEMSPARSE_AMD_H */

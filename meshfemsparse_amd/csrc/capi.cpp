@@ -1,0 +1,362 @@
+// extern "C" boundary: status codes + thread-local error message around the C++ objects.
+#include "../../include/meshfemsparse_amd.h"
+
+#include <cstring>
+#include <memory>
+
+#include "device.h"
+
+namespace mfs {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+} // namespace mfs
+
+struct mfs_pattern_s { mfs::Pattern P; };
+struct mfs_assembler_s { std::unique_ptr<mfs::AssemblerDev> dev; };
+struct mfs_factorizer_s { std::unique_ptr<mfs::FactorizerDev> dev; };
+
+#define MFS_TRY try {
+#define MFS_CATCH                                                                             \
+    }                                                                                         \
+    catch (const mfs::Error& e) {                                                             \
+        mfs::set_last_error(e.what());                                                        \
+        return e.code;                                                                        \
+    }                                                                                         \
+    catch (const std::bad_alloc&) {                                                           \
+        mfs::set_last_error("host allocation failed");                                        \
+        return MFS_ERR_ALLOC;                                                                 \
+    }                                                                                         \
+    catch (const std::exception& e) {                                                         \
+        mfs::set_last_error(e.what());                                                        \
+        return MFS_ERR_ARG;                                                                   \
+    }                                                                                         \
+    return MFS_OK;
+
+#define CHECK_PTR(p)                                                                          \
+    if (!(p)) throw mfs::Error(MFS_ERR_ARG, std::string("null argument: ") + #p);
+
+extern "C" {
+
+const char* mfs_last_error(void) { return mfs::g_last_error.c_str(); }
+int mfs_version(void) { return 100; }
+
+int mfs_pattern_build(const int32_t* en, int64_t ne, int32_t npe, int64_t nb, mfs_pattern* out) {
+    MFS_TRY
+    CHECK_PTR(out);
+    if (ne > 0) CHECK_PTR(en);
+    if (nb <= 0 || ne < 0) throw mfs::Error(MFS_ERR_ARG, "bad pattern sizes");
+    auto p = std::make_unique<mfs_pattern_s>();
+    p->P = mfs::build_block_pattern(en, ne, npe, nb);
+    *out = p.release();
+    MFS_CATCH
+}
+
+int mfs_pattern_get(mfs_pattern p, int64_t* nb, int64_t* nz, const int64_t** Ap, const int64_t** Ai) {
+    MFS_TRY
+    CHECK_PTR(p);
+    if (nb) *nb = p->P.n;
+    if (nz) *nz = p->P.nz();
+    if (Ap) *Ap = p->P.Ap.data();
+    if (Ai) *Ai = p->P.Ai.data();
+    MFS_CATCH
+}
+
+int mfs_pattern_destroy(mfs_pattern p) {
+    delete p;
+    return MFS_OK;
+}
+
+int mfs_assembler_create(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int32_t* en,
+                         int64_t ne, int32_t npe, mfs_assembler* out) {
+    MFS_TRY
+    CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(out);
+    if (ne > 0) CHECK_PTR(en);
+    if (nb <= 0 || ne < 0) throw mfs::Error(MFS_ERR_ARG, "bad assembler sizes");
+    mfs::Pattern P;
+    P.n = nb;
+    P.Ap.assign(Ap, Ap + nb + 1);
+    P.Ai.assign(Ai, Ai + Ap[nb]);
+    for (int64_t j = 0; j < nb; ++j)
+        for (int64_t k = P.Ap[j]; k < P.Ap[j + 1]; ++k)
+            if (P.Ai[k] < 0 || P.Ai[k] > j || (k > P.Ap[j] && P.Ai[k] <= P.Ai[k - 1]))
+                throw mfs::Error(MFS_ERR_ARG, "pattern must be sorted upper-triangular block CSC");
+    auto a = std::make_unique<mfs_assembler_s>();
+    a->dev = std::make_unique<mfs::AssemblerDev>(P, en, ne, npe);
+    *out = a.release();
+    MFS_CATCH
+}
+
+int mfs_assemble_hessian(mfs_assembler a, const double* He, int layout, double* Ax, void* stream) {
+    MFS_TRY
+    CHECK_PTR(a); CHECK_PTR(Ax);
+    if (a->dev->nslots() > 0) CHECK_PTR(He);
+    a->dev->assemble(He, layout, Ax, stream);
+    MFS_CATCH
+}
+
+int mfs_assembler_zero(mfs_assembler a, double* Ax, void* stream) {
+    MFS_TRY
+    CHECK_PTR(a); CHECK_PTR(Ax);
+    a->dev->zero(Ax, stream);
+    MFS_CATCH
+}
+
+int mfs_assembler_destroy(mfs_assembler a) {
+    delete a;
+    return MFS_OK;
+}
+
+int mfs_factorizer_create(int precision, mfs_factorizer* out) {
+    MFS_TRY
+    CHECK_PTR(out);
+    if (precision != MFS_PREC_FP64 && precision != MFS_PREC_FP32)
+        throw mfs::Error(MFS_ERR_ARG, "unknown precision");
+    auto f = std::make_unique<mfs_factorizer_s>();
+    f->dev = std::make_unique<mfs::FactorizerDev>(precision);
+    *out = f.release();
+    MFS_CATCH
+}
+
+int mfs_factorizer_destroy(mfs_factorizer f) {
+    delete f;
+    return MFS_OK;
+}
+
+int mfs_factorize_symbolic(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t nb,
+                           int32_t bs, const int64_t* pins, int64_t npins) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ap); CHECK_PTR(Ai);
+    if (npins > 0) CHECK_PTR(pins);
+    f->dev->symbolic(Ap, Ai, nb, bs, pins, npins < 0 ? 0 : npins);
+    MFS_CATCH
+}
+
+int mfs_factorize_numeric(mfs_factorizer f, const double* Ax, double sigma, const double* Bx, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ax);
+    f->dev->numeric(Ax, sigma, Bx, stream);
+    MFS_CATCH
+}
+
+int mfs_solve(mfs_factorizer f, const double* b, double* x, int32_t nrhs, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(b); CHECK_PTR(x);
+    f->dev->solve(b, x, nrhs, stream);
+    MFS_CATCH
+}
+
+int mfs_solve_raw_reduced(mfs_factorizer f, const double* b, double* x, int32_t nrhs,
+                          int32_t already_permuted, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(b); CHECK_PTR(x);
+    f->dev->solve_raw_reduced(b, x, nrhs, already_permuted != 0, stream);
+    MFS_CATCH
+}
+
+int mfs_remove_fixed_entries(mfs_factorizer f, const double* xf, double* xr, int32_t nrhs,
+                             int32_t permute, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(xf); CHECK_PTR(xr);
+    if (nrhs < 1) throw mfs::Error(MFS_ERR_ARG, "nrhs must be >= 1");
+    f->dev->remove_fixed(xf, xr, nrhs, permute != 0, stream);
+    MFS_CATCH
+}
+
+int mfs_extract_full_solution(mfs_factorizer f, const double* xr, double* xf, int32_t nrhs,
+                              int32_t permute, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(xf); CHECK_PTR(xr);
+    if (nrhs < 1) throw mfs::Error(MFS_ERR_ARG, "nrhs must be >= 1");
+    f->dev->extract_full(xr, xf, nrhs, permute != 0, stream);
+    MFS_CATCH
+}
+
+int mfs_spmv(mfs_factorizer f, const double* Ax, const double* x, double* y, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ax); CHECK_PTR(x); CHECK_PTR(y);
+    f->dev->spmv(Ax, x, y, stream);
+    MFS_CATCH
+}
+
+int mfs_solve_refined(mfs_factorizer f, const double* Ax, const double* b, double* x, double tol,
+                      int32_t max_iters, int32_t* iters, double* relres, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ax); CHECK_PTR(b); CHECK_PTR(x);
+    int it = 0;
+    f->dev->solve_refined(Ax, b, x, tol, max_iters, &it, relres, stream);
+    if (iters) *iters = it;
+    MFS_CATCH
+}
+
+int mfs_factorization_type(mfs_factorizer f, int32_t* t) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(t);
+    *t = f->dev->ftype;
+    MFS_CATCH
+}
+
+int mfs_num_successful_pivots(mfs_factorizer f, int64_t* n) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(n);
+    *n = f->dev->num_successful;
+    MFS_CATCH
+}
+
+int mfs_sizes(mfs_factorizer f, int64_t* nf, int64_t* nr) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->require(1);
+    if (nf) *nf = 3 * f->dev->sym->n_full_blocks;
+    if (nr) *nr = 3 * f->dev->sym->n_red;
+    MFS_CATCH
+}
+
+int mfs_factor_nnz(mfs_factorizer f, int64_t* nnz) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(nnz);
+    f->dev->require(1);
+    *nnz = f->dev->sym->factor_nnz;
+    MFS_CATCH
+}
+
+int mfs_flop_estimate(mfs_factorizer f, double* flops) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(flops);
+    f->dev->require(1);
+    *flops = f->dev->sym->flops;
+    MFS_CATCH
+}
+
+int mfs_get_inverse_permutation(mfs_factorizer f, int64_t* ip) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(ip);
+    f->dev->require(1);
+    const auto& S = *f->dev->sym;
+    for (int64_t k = 0; k < S.n_red; ++k)
+        for (int c = 0; c < 3; ++c) ip[3 * k + c] = 3 * (int64_t)S.iperm[k] + c;
+    MFS_CATCH
+}
+
+int mfs_clear_factors(mfs_factorizer f) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->clear_factors();
+    MFS_CATCH
+}
+
+int mfs_stash_factorization(mfs_factorizer f) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->stash();
+    MFS_CATCH
+}
+
+int mfs_has_stashed_factorization(mfs_factorizer f, int32_t* has) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(has);
+    *has = f->dev->has_stash() ? 1 : 0;
+    MFS_CATCH
+}
+
+int mfs_swap_stashed_factorization(mfs_factorizer f) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->swap_stash();
+    MFS_CATCH
+}
+
+int mfs_clear_stashed_factorization(mfs_factorizer f) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->clear_stash();
+    MFS_CATCH
+}
+
+int mfs_supernode_stats(mfs_factorizer f, int64_t* ns, int64_t* nl, int64_t* mw, int64_t* md) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->require(1);
+    const auto& S = *f->dev->sym;
+    if (ns) *ns = S.ns;
+    if (nl) *nl = S.num_levels;
+    if (mw) *mw = S.max_width;
+    if (md) *md = S.max_degree;
+    MFS_CATCH
+}
+
+int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
+                     int64_t npins, int64_t* iperm, int64_t* stats, double* flops) {
+    MFS_TRY
+    CHECK_PTR(Ap); CHECK_PTR(Ai);
+    if (npins > 0) CHECK_PTR(pins);
+    mfs::SymbolicOptions opt;
+    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, opt);
+    if (iperm)
+        for (int64_t k = 0; k < S.n_red; ++k) iperm[k] = S.iperm[k];
+    if (stats) {
+        stats[0] = S.n_red; stats[1] = S.ns; stats[2] = S.num_levels; stats[3] = S.factor_nnz;
+        stats[4] = S.max_width; stats[5] = S.max_degree; stats[6] = S.L_size; stats[7] = S.U_size;
+    }
+    if (flops) { flops[0] = S.flops; flops[1] = S.flops_unrelaxed; }
+    MFS_CATCH
+}
+
+int mfs_factor_export_csc(mfs_factorizer f, int64_t* Lp, int64_t* Li, double* Lx) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Lp); CHECK_PTR(Li); CHECK_PTR(Lx);
+    f->dev->export_csc(Lp, Li, Lx);
+    MFS_CATCH
+}
+
+int mfs_last_timings(mfs_factorizer f, double* ms4) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(ms4);
+    for (int i = 0; i < 4; ++i) ms4[i] = f->dev->timings[i];
+    MFS_CATCH
+}
+
+int mfs_cube_mesh(int32_t N, int32_t* tets, double* X) {
+    MFS_TRY
+    if (N <= 0 || N > 1000) throw mfs::Error(MFS_ERR_ARG, "N out of range");
+    mfs::cube_mesh(N, tets, X);
+    MFS_CATCH
+}
+
+int mfs_perturb_nodes(int32_t N, uint64_t seed, double sigma_rel, double* X) {
+    MFS_TRY
+    CHECK_PTR(X);
+    if (N <= 0) throw mfs::Error(MFS_ERR_ARG, "N out of range");
+    mfs::perturb_nodes(N, seed, sigma_rel, X);
+    MFS_CATCH
+}
+
+int mfs_neohookean_host(const double* Xr, const double* X, const int32_t* tets, int64_t ne,
+                        const double* Ee, const double* nue, double E, double nu, double* He,
+                        double* grad, double* energy, int32_t nthreads) {
+    MFS_TRY
+    CHECK_PTR(Xr); CHECK_PTR(X); CHECK_PTR(tets);
+    if (grad) {
+        int64_t nn = 0;
+        for (int64_t q = 0; q < 4 * ne; ++q) nn = std::max<int64_t>(nn, tets[q] + 1);
+        std::memset(grad, 0, sizeof(double) * 3 * nn);
+    }
+    mfs::neohookean_host(Xr, X, tets, ne, Ee, nue, E, nu, He, grad, energy, nthreads);
+    MFS_CATCH
+}
+
+int mfs_test_gemm_nt(int32_t M, int32_t N, int32_t K, double alpha, const double* A, int32_t lda,
+                     const double* B, int32_t ldb, double beta, double* C, int32_t ldc, void* stream) {
+    MFS_TRY
+    mfs::test_gemm_nt(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+    MFS_CATCH
+}
+
+int mfs_test_multiply_lower_block(int32_t BS, const int64_t* indices, int64_t start, int64_t size,
+                                  int64_t degree, const double* A, int64_t lda, int32_t nrhs,
+                                  double* B, int64_t ldb, void* stream) {
+    MFS_TRY
+    mfs::test_multiply_lower_block(BS, indices, start, size, degree, A, lda, nrhs, B, ldb, stream);
+    MFS_CATCH
+}
+
+} // extern "C"

@@ -1,0 +1,88 @@
+// Device-side objects behind the C ABI (implemented in kernels.hip).
+#pragma once
+#include "mfs_internal.h"
+
+#include <memory>
+
+namespace mfs {
+
+struct DeviceArray {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray();
+    void alloc(size_t b);
+    void release();
+    template <class T> void upload(const std::vector<T>& v) {
+        alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+        upload_raw(v.data(), v.size() * sizeof(T));
+    }
+    void upload_raw(const void* src, size_t b);
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+class AssemblerDev {
+public:
+    AssemblerDev(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);
+    void assemble(const double* He, int layout, double* Ax, void* stream);
+    void zero(double* Ax, void* stream);
+    int64_t nslots() const { return nslots_; }
+private:
+    int64_t nslots_ = 0, ne_ = 0;
+    DeviceArray seg_, contrib_, diag_;
+};
+
+class FactorizerDev {
+public:
+    explicit FactorizerDev(int precision);
+    ~FactorizerDev();
+    void symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs, const int64_t* pins,
+                  int64_t npins);
+    void numeric(const double* Ax, double sigma, const double* Bx, void* stream);
+    void solve(const double* b, double* x, int nrhs, void* stream);
+    void solve_raw_reduced(const double* b, double* x, int nrhs, bool already_permuted, void* stream);
+    void remove_fixed(const double* xf, double* xr, int nrhs, bool permute, void* stream);
+    void extract_full(const double* xr, double* xf, int nrhs, bool permute, void* stream);
+    void spmv(const double* Ax, const double* x, double* y, void* stream);
+    void solve_refined(const double* Ax, const double* b, double* x, double tol, int max_iters,
+                       int* iters, double* relres, void* stream);
+    void export_csc(int64_t* Lp, int64_t* Li, double* Lx);
+    void clear_factors();
+    void stash();
+    void swap_stash();
+    void clear_stash();
+    bool has_stash() const { return stash_valid_; }
+
+    int ftype = 0;                 // 0 none, 1 symbolic, 2 numeric
+    int64_t num_successful = 0;
+    double timings[4] = {0, 0, 0, 0};
+    std::unique_ptr<Symbolic> sym;
+    int precision;
+
+    void require(int t) const;
+
+private:
+    void solve_permuted_inplace(double* xp, int nrhs, void* stream);   // factor-order vectors
+    void ensure_vec(DeviceArray& a, size_t bytes);
+
+    DeviceArray L_, U_, inv_, status_;
+    DeviceArray planent_, rel_, ext_, potrf_, gemm_, zr_;
+    DeviceArray fg_, fd_, fv_, bd_, bv_, children_, usoff_, rows_, sdeg_, reloff_;
+    DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;
+    DeviceArray usolve_, bpart_, scratch_b_, scratch_c_;
+    DeviceArray Lstash_;
+    bool stash_valid_ = false;
+    int stash_type_ = 0;
+    int64_t stash_succ_ = 0;
+};
+
+// test hooks
+void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+                  int ldb, double beta, double* C, int ldc, void* stream);
+void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, int64_t size,
+                               int64_t degree, const double* A, int64_t lda, int nrhs, double* B,
+                               int64_t ldb, void* stream);
+
+} // namespace mfs

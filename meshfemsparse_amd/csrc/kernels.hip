@@ -1,0 +1,1064 @@
+// gfx950 (MI355X) kernels of the per-Newton-step pipeline and the device-side driver objects.
+//
+//   assembly   : slot-map gather, fixed element order per 3x3 slot (HBM-bound)
+//   injection  : plan gather of Ax blocks into supernode panels (+ sigma I / sigma B)
+//   factor     : multifrontal supernodal LL^T over a static level schedule:
+//                extend-add of children's update matrices, then per 64-column step
+//                diagonal-tile POTRF (+ explicit tile inverse), TRSM as a GEMM with the inverse,
+//                in-panel trailing update, and the Schur update U -= L21 L21^T;
+//                all dense tile work runs through one batched FP64 MFMA (16x16x4) tile kernel
+//   solves     : forward (children update vectors gathered in fixed order) and backward
+//                (partial sums per row chunk, reduced in fixed order) supernodal sweeps
+//   SpMV / IR  : block-CSR symmetric SpMV, deterministic norms, refinement driver
+// Every reduction has a fixed order, so results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "device.h"
+
+namespace mfs {
+
+#define HIP_CHECK(x)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            throw Error(-4, std::string("HIP error: ") + hipGetErrorString(e_) + " at " +    \
+                                __FILE__ + ":" + std::to_string(__LINE__));                 \
+    } while (0)
+
+static inline hipStream_t S_(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------------------
+// Device memory
+// ------------------------------------------------------------------------------------------
+DeviceArray::~DeviceArray() { release(); }
+void DeviceArray::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+void DeviceArray::alloc(size_t b) {
+    if (p && bytes >= b) return;
+    release();
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw Error(-5, "hipMalloc of " + std::to_string(b) + " bytes failed: " + hipGetErrorString(e));
+    }
+    bytes = b;
+}
+void DeviceArray::upload_raw(const void* src, size_t b) {
+    if (b) HIP_CHECK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------------------------------
+// Generic small kernels
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_zero_ranges(double* __restrict__ base, const int64_t* __restrict__ ranges) {
+    const int64_t off = ranges[2 * blockIdx.y], cnt = ranges[2 * blockIdx.y + 1];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * blockDim.x)
+        base[off + i] = 0.0;
+}
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+static void fill(double* p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>(nblk(n, 256), 8192);
+    k_fill<<<g, 256, 0, s>>>(p, n, v);
+    LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------------------------------
+// Assembly: one thread per destination 3x3 slot; contributions in ascending element order.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double he_entry(const double* __restrict__ He, int layout, int64_t e,
+                                           int pair, int r, int c) {
+    // local-upper block `pair` of element e, entry (r, c) of the 3x3 block
+    if (layout == 0) {
+        const int A[10] = {0, 0, 0, 1, 1, 2, 0, 1, 2, 3};
+        const int B[10] = {1, 2, 3, 2, 3, 3, 0, 1, 2, 3};
+        return He[e * 144 + (3 * B[pair] + c) * 12 + 3 * A[pair] + r];
+    }
+    if (pair < 6) return He[e * 78 + 9 * pair + 3 * c + r];
+    return He[e * 78 + 54 + 6 * (pair - 6) + (c * (c + 1)) / 2 + r];   // r <= c
+}
+
+__global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ seg,
+                                                  const uint32_t* __restrict__ contrib,
+                                                  const uint8_t* __restrict__ is_diag,
+                                                  const double* __restrict__ He, int layout,
+                                                  double* __restrict__ Ax, int64_t nslots) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double v[9];
+    double* dst = Ax + 9 * s;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] = dst[q];
+    const int32_t b = seg[s], e_ = seg[s + 1];
+    if (is_diag[s]) {
+        for (int32_t p = b; p < e_; ++p) {
+            const uint32_t w = contrib[p];
+            const int64_t el = w >> 5;
+            const int pair = (w >> 1) & 15;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r <= c; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, r, c);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r <= c; ++r) dst[3 * c + r] = v[3 * c + r];
+    } else {
+        for (int32_t p = b; p < e_; ++p) {
+            const uint32_t w = contrib[p];
+            const int64_t el = w >> 5;
+            const int pair = (w >> 1) & 15;
+            if (w & 1u) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, c, r);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, r, c);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dst[q] = v[q];
+    }
+}
+
+AssemblerDev::AssemblerDev(const Pattern& P, const int32_t* en, int64_t ne, int npe) {
+    SlotMap M = build_slot_map(P, en, ne, npe);
+    nslots_ = M.nslots;
+    ne_ = ne;
+    seg_.upload(M.seg);
+    contrib_.upload(M.contrib);
+    diag_.upload(M.is_diag);
+}
+
+void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stream) {
+    if (layout != 0 && layout != 1) throw Error(-1, "unknown element Hessian layout");
+    if (nslots_ == 0) return;
+    k_assemble<<<nblk(nslots_, 256), 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(),
+                                                          diag_.as<uint8_t>(), He, layout, Ax, nslots_);
+    LAUNCH_CHECK();
+}
+
+void AssemblerDev::zero(double* Ax, void* stream) { fill(Ax, 9 * nslots_, 0.0, S_(stream)); }
+
+// ------------------------------------------------------------------------------------------
+// Value injection: L[dst] = block (as-is / transposed / lower of diagonal) + shift.
+// ------------------------------------------------------------------------------------------
+struct PlanEnt {
+    int64_t dstmode;
+    int32_t src;
+    int32_t ld;
+};
+
+__global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan, int64_t n,
+                                                const double* __restrict__ Ax,
+                                                const double* __restrict__ Bx, double sigma,
+                                                double* __restrict__ L) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const PlanEnt pe = plan[q];
+    const int mode = (int)(pe.dstmode & 3);
+    double* d = L + (pe.dstmode >> 2);
+    const double* a = Ax + 9 * (int64_t)pe.src;
+    const int64_t ld = pe.ld;
+    if (mode == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) d[c * ld + r] = a[3 * c + r];
+    } else if (mode == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) d[c * ld + r] = a[3 * r + c];
+    } else {
+        const double* bb = Bx ? Bx + 9 * (int64_t)pe.src : nullptr;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = c; r < 3; ++r) {
+                double v = a[3 * r + c];
+                if (bb) v += sigma * bb[3 * r + c];
+                else if (r == c) v += sigma;
+                d[c * ld + r] = v;
+            }
+    }
+    if (mode != 2 && Bx) {
+        const double* bb = Bx + 9 * (int64_t)pe.src;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                d[c * ld + r] += sigma * (mode == 0 ? bb[3 * c + r] : bb[3 * r + c]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Extend-add: child's U (lower 3x3 blocks of column block kj) into the parent front.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ tasks,
+                                               const int32_t* __restrict__ rel,
+                                               double* __restrict__ U, double* __restrict__ L) {
+    const ExtendTask t = tasks[blockIdx.x];
+    const int32_t* relc = rel + t.rel_off;
+    const int kj = t.col_block;
+    const int64_t Dc = 3 * (int64_t)t.dc;
+    const int tj = relc[kj];
+    const double* uc = U + t.u_child;
+    for (int ki = kj + (int)threadIdx.x; ki < t.dc; ki += 64) {
+        const int ti = relc[ki];
+        double* dst;
+        int64_t ldd;
+        if (tj < t.wp) {
+            dst = L + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti;
+            ldd = t.ldp;
+        } else {
+            const int64_t Dp = 3 * (int64_t)t.dp;
+            dst = U + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
+            ldd = Dp;
+        }
+        const double* src = uc + (3 * (int64_t)kj) * Dc + 3 * (int64_t)ki;
+        if (ki == kj) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = c; r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Diagonal tile POTRF (kb <= 64) + explicit inverse of the tile's Cholesky factor.
+// ------------------------------------------------------------------------------------------
+constexpr int TS = 64;   // tile size (scalars)
+
+__global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
+                                               double* __restrict__ L, double* __restrict__ inv,
+                                               int* __restrict__ status) {
+    const PotrfTask t = tasks[blockIdx.x];
+    __shared__ double a[TS][TS + 1];    // a[c][r]
+    __shared__ double iv[TS][TS + 1];
+    const int tid = threadIdx.x, kb = t.kb;
+    const int64_t ld = t.ld;
+    double* D = L + t.diag_off;
+    for (int idx = tid; idx < kb * kb; idx += 256) {
+        const int r = idx % kb, c = idx / kb;
+        a[c][r] = (r >= c) ? D[c * ld + r] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (tid == 0) {
+            double d = a[j][j];
+            if (!(d > 0.0)) {
+                atomicMin(status, t.col0 + j);
+                d = 1.0;
+            }
+            a[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const double dj = a[j][j];
+        for (int r = j + 1 + tid; r < kb; r += 256) a[j][r] /= dj;
+        __syncthreads();
+        const int m = kb - j - 1;
+        for (int idx = tid; idx < m * m; idx += 256) {
+            const int c = j + 1 + idx / m, r = j + 1 + idx % m;
+            if (r >= c) a[c][r] -= a[j][r] * a[j][c];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < kb * kb; idx += 256) {
+        const int r = idx % kb, c = idx / kb;
+        if (r >= c) D[c * ld + r] = a[c][r];
+    }
+    // inverse: column c of L^{-1} by forward substitution (one thread per column)
+    if (tid < kb) {
+        const int c = tid;
+        iv[c][c] = 1.0 / a[c][c];
+        for (int i = c + 1; i < kb; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += a[k][i] * iv[c][k];
+            iv[c][i] = -s / a[i][i];
+        }
+    }
+    __syncthreads();
+    double* out = inv + (int64_t)t.slot * TS * TS;
+    for (int idx = tid; idx < TS * TS; idx += 256) {
+        const int r = idx % TS, c = idx / TS;
+        out[idx] = (c < kb && r < kb && r >= c) ? iv[c][r] : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched tile GEMM, C = alpha A B^T + beta C, 64x64 output tile per workgroup (4 waves of
+// 32x32), FP64 MFMA 16x16x4, K staged through LDS in chunks of 16.
+// MFMA f64 16x16x4 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16 (i / 4), register i % 4.
+// ------------------------------------------------------------------------------------------
+constexpr int KC = 16;
+
+__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks,
+                                              double* __restrict__ L, double* __restrict__ U,
+                                              const double* __restrict__ inv) {
+    const GemmTask t = tasks[blockIdx.x];
+    const double* A = L + t.a_off;
+    const double* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
+    double* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
+    __shared__ double As[KC][TS];
+    __shared__ double Bs[KC][TS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    const int lr = tid & 63, lk = tid >> 6;
+    const int64_t lda = t.lda, ldb = t.ldb;
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = lk + 4 * q, kk = k0 + k;
+            const bool kin = kk < t.K;
+            As[k][lr] = (kin && lr < t.M) ? A[kk * lda + lr] : 0.0;
+            Bs[k][lr] = (kin && lr < t.N) ? B[kk * ldb + lr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            const int kr = ks + (lane >> 4), cc = lane & 15;
+            const double a0 = As[kr][wm + cc], a1 = As[kr][wm + 16 + cc];
+            const double b0 = Bs[kr][wn + cc], b1 = Bs[kr][wn + 16 + cc];
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
+    const bool beta0 = (t.flags & GT_BETA0) != 0;
+    const int64_t ldc = t.ldc;
+    const int col_l = lane & 15, row_l = 4 * (lane >> 4);
+    auto store = [&](const d4& acc, int i0, int j0) {
+        const int col = j0 + col_l;
+        if (col >= t.N) return;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = i0 + row_l + r;
+            if (row < t.M) {
+                double* p = C + col * ldc + row;
+                const double v = alpha * acc[r];
+                *p = beta0 ? v : (*p + v);
+            }
+        }
+    };
+    store(acc00, wm, wn);
+    store(acc01, wm, wn + 16);
+    store(acc10, wm + 16, wn);
+    store(acc11, wm + 16, wn + 16);
+}
+
+// ------------------------------------------------------------------------------------------
+// Solve kernels (x = factor-order reduced vector; one RHS per blockIdx.y)
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restrict__ tasks,
+                                                 const int32_t* __restrict__ children,
+                                                 const int32_t* __restrict__ sdeg,
+                                                 const int64_t* __restrict__ rel_off,
+                                                 const int32_t* __restrict__ rel,
+                                                 const int64_t* __restrict__ usoff,
+                                                 double* __restrict__ xall, int64_t ldx,
+                                                 double* __restrict__ uall, int64_t ldu) {
+    const SolveGatherTask t = tasks[blockIdx.x];
+    double* x = xall + blockIdx.y * ldx;
+    double* u = uall + blockIdx.y * ldu;
+    double* us = u + t.u_off;
+    for (int i = threadIdx.x; i < t.d; i += blockDim.x) us[i] = 0.0;
+    __syncthreads();
+    for (int q = t.child_begin; q < t.child_end; ++q) {
+        const int c = children[q];
+        const int Dc = 3 * sdeg[c];
+        const double* uc = u + usoff[c];
+        const int32_t* relc = rel + rel_off[c];
+        for (int k = threadIdx.x; k < Dc; k += blockDim.x) {
+            const int tgt = 3 * relc[k / 3] + k % 3;
+            const double v = uc[k];
+            if (tgt < t.w) x[t.x_off + tgt] -= v;
+            else us[tgt - t.w] += v;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_fdiag(const SolveDiagTask* __restrict__ tasks,
+                                              const double* __restrict__ L,
+                                              double* __restrict__ xall, int64_t ldx) {
+    const SolveDiagTask t = tasks[blockIdx.x];
+    double* x = xall + blockIdx.y * ldx + t.x_off;
+    __shared__ double a[TS][TS + 1];
+    __shared__ double xs[TS];
+    const int tid = threadIdx.x, kb = t.kb;
+    const double* D = L + t.diag_off;
+    for (int c = 0; c < kb; ++c)
+        if (tid >= c && tid < kb) a[c][tid] = D[(int64_t)c * t.ld + tid];
+    if (tid < kb) xs[tid] = x[tid];
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (tid == j) xs[j] /= a[j][j];
+        __syncthreads();
+        if (tid > j && tid < kb) xs[tid] -= a[j][tid] * xs[j];
+        __syncthreads();
+    }
+    if (tid < kb) x[tid] = xs[tid];
+}
+
+// dot of panel row r (columns 0..kb-1, leading dimension ld) with y held in LDS
+__device__ __forceinline__ double panel_row_dot(const double* __restrict__ P, int64_t ld, int kb,
+                                                const double* y, int r) {
+    double s = 0.0;
+    for (int c = 0; c < kb; ++c) s += P[c * ld + r] * y[c];
+    return s;
+}
+
+__global__ __launch_bounds__(64) void k_fgemv(const SolveGemvTask* __restrict__ tasks,
+                                              const double* __restrict__ L,
+                                              double* __restrict__ xall, int64_t ldx,
+                                              double* __restrict__ uall, int64_t ldu) {
+    const SolveGemvTask t = tasks[blockIdx.x];
+    double* x = xall + blockIdx.y * ldx;
+    double* u = uall + blockIdx.y * ldu;
+    __shared__ double y[TS];
+    const int tid = threadIdx.x;
+    if (tid < t.kb) y[tid] = x[t.x_cols + tid];
+    __syncthreads();
+    if (tid >= t.nr) return;
+    const double v = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
+    const int row = t.r0 + tid;
+    if (row < t.w) x[t.x_rows + row] -= v;
+    else u[t.u_off + row - t.w] += v;
+}
+
+__global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__ tasks,
+                                               const double* __restrict__ L,
+                                               const int32_t* __restrict__ rows,
+                                               const double* __restrict__ xall, int64_t ldx,
+                                               double* __restrict__ partall, int64_t ldp) {
+    const SolveGemvTask t = tasks[blockIdx.x];
+    const double* x = xall + blockIdx.y * ldx;
+    double* part = partall + blockIdx.y * ldp;
+    __shared__ double T[TS][TS + 1];   // T[c][i]
+    __shared__ double xv[TS];
+    __shared__ double red[4][TS];
+    const int tid = threadIdx.x, ci = tid & 63, g = tid >> 6;
+    double acc = 0.0;
+    const double* P = L + t.l_off;
+    for (int rb = 0; rb < t.nr; rb += TS) {
+        const int nrb = min(TS, t.nr - rb);
+        for (int q = 0; q < 16; ++q) {
+            const int c = g + 4 * q;
+            if (c < t.kb) T[c][ci] = (ci < nrb) ? P[(int64_t)c * t.ld + rb + ci] : 0.0;
+        }
+        if (tid < TS) {
+            double v = 0.0;
+            if (tid < nrb) {
+                const int row = t.r0 + rb + tid;
+                if (row < t.w) v = x[t.x_rows + row];
+                else {
+                    const int q = row - t.w;
+                    v = x[3 * (int64_t)rows[t.rows_off + q / 3] + q % 3];
+                }
+            }
+            xv[tid] = v;
+        }
+        __syncthreads();
+        if (ci < t.kb) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc += T[ci][g * 16 + i] * xv[g * 16 + i];
+        }
+        __syncthreads();
+    }
+    red[g][ci] = acc;
+    __syncthreads();
+    if (tid < t.kb) part[(int64_t)t.part * TS + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+}
+
+__global__ __launch_bounds__(64) void k_bdiag(const SolveDiagTask* __restrict__ tasks,
+                                              const double* __restrict__ L,
+                                              const double* __restrict__ partall, int64_t ldp,
+                                              double* __restrict__ xall, int64_t ldx) {
+    const SolveDiagTask t = tasks[blockIdx.x];
+    double* x = xall + blockIdx.y * ldx + t.x_off;
+    const double* part = partall + blockIdx.y * ldp;
+    __shared__ double a[TS][TS + 1];
+    __shared__ double xs[TS];
+    const int tid = threadIdx.x, kb = t.kb;
+    const double* D = L + t.diag_off;
+    for (int c = 0; c < kb; ++c)
+        if (tid >= c && tid < kb) a[c][tid] = D[(int64_t)c * t.ld + tid];
+    if (tid < kb) {
+        double v = x[tid];
+        for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + tid];
+        xs[tid] = v;
+    }
+    __syncthreads();
+    for (int j = kb - 1; j >= 0; --j) {
+        if (tid == j) xs[j] /= a[j][j];
+        __syncthreads();
+        if (tid < j) xs[tid] -= a[tid][j] * xs[j];
+        __syncthreads();
+    }
+    if (tid < kb) x[tid] = xs[tid];
+}
+
+// ------------------------------------------------------------------------------------------
+// Permutation / pin maps, SpMV, norms, refinement helpers
+// ------------------------------------------------------------------------------------------
+__global__ void k_gather_map(const int64_t* __restrict__ map, int64_t n, const double* __restrict__ src,
+                             int64_t lds, double* __restrict__ dst, int64_t ldd) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = map[i];
+    if (t >= 0) dst[blockIdx.y * ldd + t] = src[blockIdx.y * lds + i];
+}
+__global__ void k_scatter_map(const int64_t* __restrict__ map, int64_t n, const double* __restrict__ src,
+                              int64_t lds, double* __restrict__ dst, int64_t ldd) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = map[i];
+    dst[blockIdx.y * ldd + i] = (t >= 0) ? src[blockIdx.y * lds + t] : 0.0;
+}
+__global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+__global__ __launch_bounds__(256) void k_spmv(const int64_t* __restrict__ ptr,
+                                              const int64_t* __restrict__ ent,
+                                              const int32_t* __restrict__ col,
+                                              const double* __restrict__ Ax,
+                                              const double* __restrict__ x, double* __restrict__ y,
+                                              int64_t nb) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    double y0 = 0, y1 = 0, y2 = 0;
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+        const int64_t e = ent[p];
+        const int mode = (int)(e & 3);
+        const double* B = Ax + 9 * (e >> 2);
+        const double* xj = x + 3 * (int64_t)col[p];
+        const double x0 = xj[0], x1 = xj[1], x2 = xj[2];
+        if (mode == 0) {
+            y0 += B[0] * x0 + B[3] * x1 + B[6] * x2;
+            y1 += B[1] * x0 + B[4] * x1 + B[7] * x2;
+            y2 += B[2] * x0 + B[5] * x1 + B[8] * x2;
+        } else if (mode == 1) {
+            y0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
+            y1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
+            y2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
+        } else {   // symmetric from the upper triangle: (0,0)=B0 (0,1)=B3 (1,1)=B4 (0,2)=B6 (1,2)=B7 (2,2)=B8
+            y0 += B[0] * x0 + B[3] * x1 + B[6] * x2;
+            y1 += B[3] * x0 + B[4] * x1 + B[7] * x2;
+            y2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
+        }
+    }
+    y[3 * i] = y0;
+    y[3 * i + 1] = y1;
+    y[3 * i + 2] = y2;
+}
+
+__global__ void k_residual(const double* __restrict__ b, const double* __restrict__ y,
+                           const int64_t* __restrict__ mask, double* __restrict__ r, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    r[i] = mask[i] >= 0 ? b[i] - y[i] : 0.0;
+}
+__global__ void k_axpy1(double* __restrict__ x, const double* __restrict__ d, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+constexpr int NRED = 256;
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ a,
+                                                       const int64_t* __restrict__ mask, int64_t n,
+                                                       double* __restrict__ part) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)NRED * 256)
+        if (!mask || mask[i] >= 0) acc += a[i] * a[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void k_sum_final(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double s[256];
+    s[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// trs_kernels::MultiplyLowerBlock semantics on the device, sharing the forward-solve row dot.
+__global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start, int64_t size,
+                      int64_t degree, const double* __restrict__ A, int64_t lda,
+                      double* __restrict__ B, int64_t ldb) {
+    extern __shared__ double y[];
+    double* Bc = B + blockIdx.y * ldb;
+    for (int64_t c = threadIdx.x; c < size; c += blockDim.x) y[c] = Bc[start + c];
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < degree; i += blockDim.x) {
+        const double v = panel_row_dot(A, lda, (int)size, y, (int)i);
+        Bc[indices[i / BS] + i % BS] -= v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// FactorizerDev
+// ------------------------------------------------------------------------------------------
+FactorizerDev::FactorizerDev(int prec) : precision(prec) {
+    if (prec != 0) throw Error(-6, "only the FP64 factorization is available in this build");
+}
+FactorizerDev::~FactorizerDev() = default;
+
+void FactorizerDev::require(int t) const {
+    if (ftype < t)
+        throw Error(-2, t >= 2 ? "Numeric factorization does not exist" : "Symbolic factorization does not exist");
+}
+
+void FactorizerDev::ensure_vec(DeviceArray& a, size_t bytes) { a.alloc(bytes); }
+
+void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
+                             const int64_t* pins, int64_t npins) {
+    if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
+    SymbolicOptions opt;
+    auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, opt));
+    ftype = 0;
+    stash_valid_ = false;
+    // injection plan entries with per-entry leading dimension
+    std::vector<PlanEnt> pe(S->plan_src.size());
+    for (size_t q = 0; q < pe.size(); ++q) {
+        const int64_t dst = S->plan_dst[q] >> 2;
+        const int32_t s = (int32_t)(std::upper_bound(S->panel_off.begin(), S->panel_off.end(), dst) -
+                                    S->panel_off.begin()) - 1;
+        pe[q].dstmode = S->plan_dst[q];
+        pe[q].src = S->plan_src[q];
+        pe[q].ld = 3 * (S->swidth[s] + S->sdeg[s]);
+    }
+    planent_.upload(pe);
+    rel_.upload(S->rel);
+    ext_.upload(S->extend_tasks);
+    potrf_.upload(S->potrf_tasks);
+    gemm_.upload(S->gemm_tasks);
+    {
+        std::vector<int64_t> zr;
+        for (auto& p : S->zero_ranges) { zr.push_back(p.first); zr.push_back(p.second); }
+        zr_.upload(zr);
+    }
+    fg_.upload(S->fgather_tasks);
+    fd_.upload(S->fdiag_tasks);
+    fv_.upload(S->fgemv_tasks);
+    bd_.upload(S->bdiag_tasks);
+    bv_.upload(S->bgemv_tasks);
+    children_.upload(S->children);
+    usoff_.upload(S->u_solve_off);
+    rows_.upload(S->rows);
+    sdeg_.upload(S->sdeg);
+    reloff_.upload(S->rel_off);
+    spmv_ptr_.upload(S->spmv_ptr);
+    spmv_ent_.upload(S->spmv_ent);
+    spmv_col_.upload(S->spmv_col);
+    f2p_.upload(S->full_to_perm);
+    f2r_.upload(S->full_to_red);
+    {
+        std::vector<int64_t> r2p(3 * S->n_red);
+        for (int64_t r = 0; r < S->n_red; ++r)
+            for (int c = 0; c < 3; ++c) r2p[3 * r + c] = 3 * (int64_t)S->pos[r] + c;
+        r2p_.upload(r2p);
+    }
+    L_.alloc(std::max<int64_t>(S->L_size, 1) * sizeof(double));
+    U_.alloc(std::max<int64_t>(S->U_size, 1) * sizeof(double));
+    inv_.alloc(S->inv_slots * TS * TS * sizeof(double));
+    status_.alloc(sizeof(int) * 4);
+    sym = std::move(S);
+    ftype = 1;
+}
+
+void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, void* stream) {
+    require(1);
+    const Symbolic& S = *sym;
+    hipStream_t s = S_(stream);
+    ftype = 1;
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    double* L = L_.as<double>();
+    double* U = U_.as<double>();
+    fill(L, S.L_size, 0.0, s);
+    k_set_int<<<1, 1, 0, s>>>(status_.as<int>(), INT_MAX);
+    LAUNCH_CHECK();
+    const int64_t np = (int64_t)S.plan_src.size();
+    if (np) {
+        k_inject<<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, L);
+        LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    const int64_t* zr = zr_.as<int64_t>();
+    for (const LevelPlan& P : S.levels) {
+        const int64_t nz = P.zero_u.second - P.zero_u.first;
+        if (nz > 0) {
+            dim3 g(64, (unsigned)nz);
+            k_zero_ranges<<<g, 256, 0, s>>>(U, zr + 2 * P.zero_u.first);
+            LAUNCH_CHECK();
+        }
+        for (auto [a, b] : P.extend_passes)
+            if (b > a) {
+                k_extend<<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
+                LAUNCH_CHECK();
+            }
+        for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
+            auto [p0, p1] = P.potrf_steps[t];
+            if (p1 > p0) {
+                k_potrf<<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
+                                                           status_.as<int>());
+                LAUNCH_CHECK();
+            }
+            auto [t0, t1] = P.trsm_steps[t];
+            if (t1 > t0) {
+                k_gemm<<<(unsigned)(t1 - t0), 256, 0, s>>>(gemm_.as<GemmTask>() + t0, L, U, inv_.as<double>());
+                LAUNCH_CHECK();
+            }
+            auto [u0, u1] = P.update_steps[t];
+            if (u1 > u0) {
+                k_gemm<<<(unsigned)(u1 - u0), 256, 0, s>>>(gemm_.as<GemmTask>() + u0, L, U, inv_.as<double>());
+                LAUNCH_CHECK();
+            }
+        }
+        if (P.syrk.second > P.syrk.first) {
+            k_gemm<<<(unsigned)(P.syrk.second - P.syrk.first), 256, 0, s>>>(gemm_.as<GemmTask>() + P.syrk.first,
+                                                                           L, U, inv_.as<double>());
+            LAUNCH_CHECK();
+        }
+    }
+    HIP_CHECK(hipEventRecord(ev[2], s));
+    int st = 0;
+    HIP_CHECK(hipMemcpyAsync(&st, status_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float m01 = 0, m12 = 0;
+    HIP_CHECK(hipEventElapsedTime(&m01, ev[0], ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&m12, ev[1], ev[2]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    timings[0] = m01;
+    timings[1] = 0;
+    timings[2] = m12;
+    timings[3] = m01 + m12;
+    if (st != INT_MAX) {
+        num_successful = st;
+        ftype = 1;
+        throw Error(-3, std::to_string(st) + "/" + std::to_string(3 * S.n_red) +
+                            "  pivots successful in numeric factorization (non-positive definite?)");
+    }
+    num_successful = 3 * S.n_red;
+    ftype = 2;
+}
+
+void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
+    const Symbolic& S = *sym;
+    hipStream_t s = S_(stream);
+    const int64_t ldx = 3 * S.n_red;
+    const int64_t ldu = std::max<int64_t>(S.u_solve_size, 1);
+    const int64_t ldp = S.bpart_size * TS;
+    usolve_.alloc((size_t)ldu * nrhs * sizeof(double));
+    bpart_.alloc((size_t)ldp * nrhs * sizeof(double));
+    double* u = usolve_.as<double>();
+    double* part = bpart_.as<double>();
+    const double* L = L_.as<double>();
+    const int32_t* sdeg = sdeg_.as<int32_t>();
+    const int64_t* reloff = reloff_.as<int64_t>();
+    for (const LevelPlan& P : S.levels) {
+        if (P.fgather.second > P.fgather.first) {
+            dim3 g((unsigned)(P.fgather.second - P.fgather.first), (unsigned)nrhs);
+            k_fgather<<<g, 256, 0, s>>>(fg_.as<SolveGatherTask>() + P.fgather.first, children_.as<int32_t>(),
+                                        sdeg, reloff, rel_.as<int32_t>(), usoff_.as<int64_t>(), xp, ldx, u, ldu);
+            LAUNCH_CHECK();
+        }
+        for (size_t t = 0; t < P.fdiag_steps.size(); ++t) {
+            auto [d0, d1] = P.fdiag_steps[t];
+            if (d1 > d0) {
+                dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
+                k_fdiag<<<g, 64, 0, s>>>(fd_.as<SolveDiagTask>() + d0, L, xp, ldx);
+                LAUNCH_CHECK();
+            }
+            auto [v0, v1] = P.fgemv_steps[t];
+            if (v1 > v0) {
+                dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+                k_fgemv<<<g, 64, 0, s>>>(fv_.as<SolveGemvTask>() + v0, L, xp, ldx, u, ldu);
+                LAUNCH_CHECK();
+            }
+        }
+    }
+    for (int lv = (int)S.levels.size() - 1; lv >= 0; --lv) {
+        const LevelPlan& P = S.levels[lv];
+        for (size_t t = 0; t < P.bdiag_steps.size(); ++t) {
+            auto [v0, v1] = P.bgemv_steps[t];
+            if (v1 > v0) {
+                dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+                k_bgemv<<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), xp, ldx, part, ldp);
+                LAUNCH_CHECK();
+            }
+            auto [d0, d1] = P.bdiag_steps[t];
+            if (d1 > d0) {
+                dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
+                k_bdiag<<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, L, part, ldp, xp, ldx);
+                LAUNCH_CHECK();
+            }
+        }
+    }
+}
+
+void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
+    require(2);
+    if (nrhs < 1) throw Error(-1, "Must specify at least one rhs.");
+    const Symbolic& S = *sym;
+    hipStream_t s = S_(stream);
+    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
+    scratch_b_.alloc((size_t)nr * nrhs * sizeof(double));
+    double* xp = scratch_b_.as<double>();
+    dim3 g(nblk(nf, 256), (unsigned)nrhs);
+    k_gather_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, b, nf, xp, nr);
+    LAUNCH_CHECK();
+    solve_permuted_inplace(xp, nrhs, stream);
+    k_scatter_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, xp, nr, x, nf);
+    LAUNCH_CHECK();
+}
+
+void FactorizerDev::solve_raw_reduced(const double* b, double* x, int nrhs, bool already_permuted,
+                                      void* stream) {
+    require(2);
+    if (nrhs < 1) throw Error(-1, "Must specify at least one rhs.");
+    const Symbolic& S = *sym;
+    hipStream_t s = S_(stream);
+    const int64_t nr = 3 * S.n_red;
+    scratch_b_.alloc((size_t)nr * nrhs * sizeof(double));
+    double* xp = scratch_b_.as<double>();
+    dim3 g(nblk(nr, 256), (unsigned)nrhs);
+    if (already_permuted) {
+        k_copy<<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(b, xp, nr * nrhs);
+        LAUNCH_CHECK();
+    } else {
+        k_gather_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, b, nr, xp, nr);
+        LAUNCH_CHECK();
+    }
+    solve_permuted_inplace(xp, nrhs, stream);
+    if (already_permuted) {
+        k_copy<<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(xp, x, nr * nrhs);
+        LAUNCH_CHECK();
+    } else {
+        k_scatter_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, xp, nr, x, nr);
+        LAUNCH_CHECK();
+    }
+}
+
+void FactorizerDev::remove_fixed(const double* xf, double* xr, int nrhs, bool permute, void* stream) {
+    require(1);
+    const Symbolic& S = *sym;
+    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
+    dim3 g(nblk(nf, 256), (unsigned)nrhs);
+    k_gather_map<<<g, 256, 0, S_(stream)>>>((permute ? f2p_ : f2r_).as<int64_t>(), nf, xf, nf, xr, nr);
+    LAUNCH_CHECK();
+}
+
+void FactorizerDev::extract_full(const double* xr, double* xf, int nrhs, bool permute, void* stream) {
+    require(1);
+    const Symbolic& S = *sym;
+    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
+    dim3 g(nblk(nf, 256), (unsigned)nrhs);
+    k_scatter_map<<<g, 256, 0, S_(stream)>>>((permute ? f2p_ : f2r_).as<int64_t>(), nf, xr, nr, xf, nf);
+    LAUNCH_CHECK();
+}
+
+void FactorizerDev::spmv(const double* Ax, const double* x, double* y, void* stream) {
+    require(1);
+    const int64_t nb = sym->n_full_blocks;
+    k_spmv<<<nblk(nb, 256), 256, 0, S_(stream)>>>(spmv_ptr_.as<int64_t>(), spmv_ent_.as<int64_t>(),
+                                                 spmv_col_.as<int32_t>(), Ax, x, y, nb);
+    LAUNCH_CHECK();
+}
+
+void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, double tol,
+                                  int max_iters, int* iters, double* relres, void* stream) {
+    require(2);
+    hipStream_t s = S_(stream);
+    const int64_t nf = 3 * sym->n_full_blocks;
+    scratch_c_.alloc((size_t)(3 * nf + 2 * NRED + 8) * sizeof(double));
+    double* y = scratch_c_.as<double>();
+    double* r = y + nf;
+    double* dx = r + nf;
+    double* part = dx + nf;
+    double* nrm = part + NRED;   // [0] = |b|^2, [1] = |r|^2
+    const int64_t* mask = f2r_.as<int64_t>();
+    auto sumsq = [&](const double* v, double* out) {
+        k_sumsq_partial<<<NRED, 256, 0, s>>>(v, mask, nf, part);
+        LAUNCH_CHECK();
+        k_sum_final<<<1, 256, 0, s>>>(part, out);
+        LAUNCH_CHECK();
+    };
+    sumsq(b, nrm);
+    solve(b, x, 1, stream);
+    int it = 0;
+    double h[2] = {0, 0};
+    double rel = 0;
+    while (true) {
+        spmv(Ax, x, y, stream);
+        k_residual<<<nblk(nf, 256), 256, 0, s>>>(b, y, mask, r, nf);
+        LAUNCH_CHECK();
+        sumsq(r, nrm + 1);
+        HIP_CHECK(hipMemcpyAsync(h, nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        rel = h[0] > 0 ? std::sqrt(h[1] / h[0]) : std::sqrt(h[1]);
+        if (rel <= tol || it >= max_iters) break;
+        solve(r, dx, 1, stream);
+        k_axpy1<<<nblk(nf, 256), 256, 0, s>>>(x, dx, nf);
+        LAUNCH_CHECK();
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+}
+
+void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
+    require(2);
+    const Symbolic& S = *sym;
+    std::vector<double> h(S.L_size);
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipMemcpy(h.data(), L_.p, S.L_size * sizeof(double), hipMemcpyDeviceToHost));
+    const int64_t n = 3 * S.n_red;
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int32_t sn = 0; sn < S.ns; ++sn) {
+        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn];
+        for (int64_t c = 0; c < W; ++c) cnt[3 * (int64_t)S.sfirst[sn] + c + 1] = W + D - c;
+    }
+    for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+    for (int64_t j = 0; j <= n; ++j) Lp[j] = cnt[j];
+    for (int32_t sn = 0; sn < S.ns; ++sn) {
+        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn], ld = W + D;
+        const int64_t f = 3 * (int64_t)S.sfirst[sn];
+        for (int64_t c = 0; c < W; ++c) {
+            int64_t q = Lp[f + c];
+            for (int64_t r = c; r < W + D; ++r) {
+                const int64_t grow = r < W ? f + r
+                                           : 3 * (int64_t)S.rows[S.srows_off[sn] + (r - W) / 3] + (r - W) % 3;
+                Li[q] = grow;
+                Lx[q] = h[S.panel_off[sn] + c * ld + r];
+                ++q;
+            }
+        }
+    }
+}
+
+void FactorizerDev::clear_factors() {
+    if (ftype > 0) ftype = 0;
+}
+
+void FactorizerDev::stash() {
+    require(1);
+    Lstash_.alloc(L_.bytes);
+    HIP_CHECK(hipMemcpy(Lstash_.p, L_.p, L_.bytes, hipMemcpyDeviceToDevice));
+    stash_valid_ = true;
+    stash_type_ = ftype;
+    stash_succ_ = num_successful;
+}
+
+void FactorizerDev::swap_stash() {
+    if (!stash_valid_) throw Error(-2, "No stashed factorization");
+    std::swap(L_.p, Lstash_.p);
+    std::swap(L_.bytes, Lstash_.bytes);
+    std::swap(ftype, stash_type_);
+    std::swap(num_successful, stash_succ_);
+    if (ftype == 0) ftype = 1;   // the symbolic analysis is shared by both factors
+}
+
+void FactorizerDev::clear_stash() {
+    Lstash_.release();
+    stash_valid_ = false;
+}
+
+// ------------------------------------------------------------------------------------------
+// test hooks
+// ------------------------------------------------------------------------------------------
+void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, const double* B,
+                  int ldb, double beta, double* C, int ldc, void* stream) {
+    if (!(alpha == 1.0 || alpha == -1.0) || !(beta == 0.0 || beta == 1.0))
+        throw Error(-1, "the tile kernel supports alpha in {+1,-1} and beta in {0,1}");
+    if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, N) || ldc < std::max(1, M))
+        throw Error(-1, "bad GEMM dimensions");
+    std::vector<GemmTask> tasks;
+    for (int c0 = 0; c0 < N; c0 += TS)
+        for (int r0 = 0; r0 < M; r0 += TS) {
+            GemmTask g{};
+            g.a_off = r0;
+            g.lda = lda;
+            g.b_off = c0;
+            g.ldb = ldb;
+            g.c_off = (int64_t)c0 * ldc + r0;
+            g.ldc = ldc;
+            g.M = std::min(TS, M - r0);
+            g.N = std::min(TS, N - c0);
+            g.K = K;
+            g.flags = GT_C_IN_U | GT_B_IN_INV | (beta == 0.0 ? GT_BETA0 : 0) | (alpha > 0 ? GT_ALPHA_POS : 0);
+            tasks.push_back(g);
+        }
+    if (tasks.empty()) return;
+    DeviceArray d;
+    d.upload(tasks);
+    hipStream_t s = S_(stream);
+    k_gemm<<<(unsigned)tasks.size(), 256, 0, s>>>(d.as<GemmTask>(), const_cast<double*>(A), C, B);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, int64_t size,
+                               int64_t degree, const double* A, int64_t lda, int nrhs, double* B,
+                               int64_t ldb, void* stream) {
+    if (BS < 1 || BS > 3 || size < 0 || degree < 0 || nrhs < 1 || degree % BS)
+        throw Error(-1, "bad MultiplyLowerBlock arguments");
+    if (size == 0 || degree == 0) return;
+    hipStream_t s = S_(stream);
+    dim3 g(1, (unsigned)nrhs);
+    k_mlb<<<g, 256, (size_t)size * sizeof(double), s>>>(BS, indices, start, size, degree, A, lda, B, ldb);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+} // namespace mfs

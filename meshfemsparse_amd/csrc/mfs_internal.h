@@ -1,0 +1,211 @@
+// Internal host-side structures shared by the C ABI, the symbolic analysis and the device code.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+namespace mfs {
+
+// Error raised inside the library; the C ABI turns it into a status code + message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+// ------------------------------------------------------------------------------------------
+// Block sparsity pattern (upper triangle, CSC over block indices, diagonal last per column).
+// ------------------------------------------------------------------------------------------
+struct Pattern {
+    int64_t n = 0;
+    std::vector<int64_t> Ap, Ai;
+    int64_t nz() const { return n ? Ap[n] : 0; }
+};
+
+Pattern build_block_pattern(const int32_t* elem_nodes, int64_t ne, int npe, int64_t nblocks);
+
+// ------------------------------------------------------------------------------------------
+// Assembly slot map: for every destination block slot, the list of (element, local pair,
+// transpose) contributions sorted by element.  Entry = (e << 5) | (pair << 1) | transpose.
+// Pairs 0..5 are the off-diagonal local-upper blocks (0,1),(0,2),(0,3),(1,2),(1,3),(2,3);
+// pairs 6..9 are the diagonal blocks 0..3 (upper triangle only).
+// ------------------------------------------------------------------------------------------
+struct SlotMap {
+    int64_t nslots = 0;
+    std::vector<int32_t> seg;        // nslots + 1
+    std::vector<uint32_t> contrib;   // seg[nslots]
+    std::vector<uint8_t> is_diag;    // per slot
+};
+SlotMap build_slot_map(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);
+
+// ------------------------------------------------------------------------------------------
+// Nested-dissection ordering of a symmetric graph (CSR, no self loops).
+// perm[new] = old.
+// ------------------------------------------------------------------------------------------
+struct NDOptions {
+    int leaf_size = 96;
+    int coarse_size = 120;
+    double imbalance = 0.06;
+    uint64_t seed = 0x5eed;
+};
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
+                                       const std::vector<int32_t>& adj, const NDOptions& opt);
+
+// ------------------------------------------------------------------------------------------
+// Symbolic analysis result (all indices in *block* units unless noted "scalar").
+// ------------------------------------------------------------------------------------------
+struct GemmTask {           // C(MxN) = alpha * A(MxK) * B(NxK)^T + beta * C ; column-major
+    int64_t a_off, b_off, c_off;
+    int32_t lda, ldb, ldc;
+    int32_t M, N, K;
+    int32_t flags;          // bit0: C lives in the update arena; bit1: beta = 0 (else 1);
+                            // bit2: alpha = +1 (else -1); bit3: A/B live in the inverse scratch
+    int32_t pad;
+};
+static_assert(sizeof(GemmTask) == 56, "GemmTask layout");
+enum : int32_t { GT_C_IN_U = 1, GT_BETA0 = 2, GT_ALPHA_POS = 4, GT_B_IN_INV = 8 };
+
+struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], ld; inverse -> slot
+    int64_t diag_off;
+    int32_t ld, kb;
+    int32_t slot;
+    int32_t col0;           // scalar column (factor order) of the tile's first column
+};
+
+struct ExtendTask {         // extend-add of child's update matrix into the parent front
+    int64_t u_child;        // offset of child's U (D_c x D_c, ld = D_c) in the update arena
+    int64_t p_panel;        // parent panel offset in L
+    int64_t u_parent;       // parent U offset in arena (or -1 if parent has no U)
+    int32_t dc;             // child degree (blocks)
+    int32_t ldp;            // parent panel leading dimension (scalars)
+    int32_t wp;             // parent width (blocks)
+    int32_t dp;             // parent degree (blocks)
+    int64_t rel_off;        // offset of child's relative indices (dc entries) in rel array
+    int32_t col_block;      // column block kj of the child's U handled by this task
+    int32_t pad;
+};
+
+struct SolveDiagTask {      // triangular solve on one diagonal tile of a supernode
+    int64_t diag_off;       // L offset of the tile
+    int32_t ld, kb;
+    int64_t x_off;          // scalar offset in the permuted solution vector
+    int32_t part_begin, part_end;   // backward: partial-sum rows to add first
+};
+
+struct SolveGemvTask {      // forward: rows [r0, r0+nr) of a panel, column block [k0, k0+kb)
+    int64_t l_off;          // L offset of (r0, k0)
+    int32_t ld, nr, kb;
+    int32_t w;              // supernode width (scalars): rows < w go to x, rows >= w go to u
+    int32_t r0;             // first row (scalar, within panel)
+    int32_t pad;
+    int64_t x_cols;         // x offset of column block (input y)
+    int64_t x_rows;         // x offset of panel row 0 (for rows < w)
+    int64_t u_off;          // u buffer offset of struct row 0 (for rows >= w)
+    int64_t rows_off;       // offset into the scalar row-index array for rows >= w (backward)
+    int32_t part;           // backward: output partial slot
+    int32_t pad2;
+};
+
+struct SolveGatherTask {    // forward: gather children update vectors into x / u of parent
+    int32_t sn;             // supernode
+    int32_t child_begin, child_end;   // range in child list
+    int32_t w, d;           // scalars
+    int64_t x_off;          // x offset of the supernode's first column
+    int64_t u_off;          // u offset of the supernode (d scalars)
+};
+
+struct LevelPlan {
+    // factorization
+    std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks
+    std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
+    std::vector<std::pair<int64_t,int64_t>> trsm_steps;      // per step: range in gemm tasks
+    std::vector<std::pair<int64_t,int64_t>> update_steps;    // per step: range in gemm tasks
+    std::pair<int64_t,int64_t> syrk{0, 0};                   // range in gemm tasks
+    std::pair<int64_t,int64_t> zero_u{0, 0};                 // U arena range to clear (doubles)
+    // forward solve
+    std::pair<int64_t,int64_t> fgather{0, 0};
+    std::vector<std::pair<int64_t,int64_t>> fdiag_steps, fgemv_steps;
+    // backward solve (steps in execution order)
+    std::vector<std::pair<int64_t,int64_t>> bgemv_steps, bdiag_steps;
+};
+
+struct Symbolic {
+    // sizes
+    int64_t n_full_blocks = 0;     // unreduced block count
+    int64_t n_red = 0;             // reduced blocks
+    int64_t nz_full = 0;           // unreduced stored upper blocks
+    // pins
+    std::vector<int64_t> fixed_scalar;          // sorted unique scalar vars
+    std::vector<int64_t> red_of_full;           // block: full -> reduced or -1
+    // ordering: pos[reduced block] = factor position; iperm[pos] = reduced block
+    std::vector<int32_t> pos, iperm;
+    // supernodes (factor order)
+    int32_t ns = 0;
+    std::vector<int32_t> sfirst, swidth, sdeg, sparent, slevel;    // blocks
+    std::vector<int64_t> srows_off;            // offset of struct rows in `rows`
+    std::vector<int32_t> rows;                 // struct rows (block, factor order)
+    std::vector<int64_t> panel_off;            // scalar offset of panel in L
+    std::vector<int64_t> u_off;                // scalar offset of U in arena (-1 if d == 0)
+    std::vector<int64_t> rel_off;              // offset of relative indices in `rel`
+    std::vector<int32_t> rel;                  // relative positions in parent's row list
+    std::vector<int32_t> children;             // child lists (CSR by parent)
+    std::vector<int32_t> child_ptr;
+    std::vector<int32_t> super_of;             // block column -> supernode
+    int64_t L_size = 0;                        // doubles
+    int64_t U_size = 0;                        // doubles (arena)
+    int64_t inv_slots = 0;                     // 64x64 inverse scratch slots
+    int64_t u_solve_size = 0;                  // doubles per RHS for solve update vectors
+    std::vector<int64_t> u_solve_off;          // per supernode
+    int64_t factor_nnz = 0;                    // scalar lower nnz incl. diag of the panels
+    double flops = 0.0;                        // POTRF+TRSM+SYRK over supernodes (scalar)
+    double flops_unrelaxed = 0.0;              // sum_j c_j^2 of the unrelaxed column counts
+    int32_t num_levels = 0;
+    int64_t max_width = 0, max_degree = 0;
+    // injection plan: per reduced stored block (in reduced-CSC order)
+    std::vector<int32_t> plan_src;             // original block entry index (Ax block)
+    std::vector<int64_t> plan_dst;             // (scalar L offset << 2) | mode
+    // schedule
+    std::vector<LevelPlan> levels;
+    std::vector<ExtendTask> extend_tasks;
+    std::vector<PotrfTask> potrf_tasks;
+    std::vector<GemmTask> gemm_tasks;
+    std::vector<SolveGatherTask> fgather_tasks;
+    std::vector<SolveDiagTask> fdiag_tasks, bdiag_tasks;
+    std::vector<SolveGemvTask> fgemv_tasks, bgemv_tasks;
+    std::vector<std::pair<int64_t,int64_t>> zero_ranges;   // U arena (offset, count) to clear
+    std::vector<int64_t> solve_rows;           // scalar x offsets for struct rows (per sn)
+    std::vector<int64_t> solve_rows_off;       // per supernode offset into solve_rows
+    std::vector<int32_t> solve_child_list;     // forward gather: child supernode ids
+    int64_t bpart_size = 0;                    // partial-sum slots (x64 doubles)
+    // full-pattern SpMV (block-CSR over both triangles): for block row i, entries
+    // (col block, Ax block index, transpose flag); diagonal flagged with 2.
+    std::vector<int64_t> spmv_ptr;
+    std::vector<int64_t> spmv_ent;             // (blockidx << 2) | mode, mode 0 as-is, 1 transpose, 2 diag
+    std::vector<int32_t> spmv_col;
+    // pin maps (scalar): full index -> permuted reduced scalar index or -1
+    std::vector<int64_t> full_to_perm;
+    std::vector<int64_t> full_to_red;
+};
+
+struct SymbolicOptions {
+    NDOptions nd;
+    // relaxed amalgamation thresholds (scalar columns)
+    int nrelax0 = 12, nrelax1 = 48, nrelax2 = 192;
+    double zrelax0 = 0.8, zrelax1 = 0.2, zrelax2 = 0.05;
+    int tile = 64;          // dense tile / step width (scalars)
+    int solve_chunk = 512;  // backward partial-sum rows per task
+};
+
+Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
+                 const int64_t* pins, int64_t npins, const SymbolicOptions& opt);
+
+// Workload (host)
+void cube_mesh(int N, int32_t* tets, double* X);
+void perturb_nodes(int N, uint64_t seed, double sigma_rel, double* X);
+void neohookean_host(const double* Xr, const double* X, const int32_t* tets, int64_t ne,
+                     const double* Ee, const double* nue, double E, double nu,
+                     double* He, double* grad, double* energy, int nthreads);
+
+} // namespace mfs

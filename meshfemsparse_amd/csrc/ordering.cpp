@@ -1,0 +1,412 @@
+// Nested-dissection ordering for the reduced block graph (host, once per sparsity pattern).
+//
+// The reference orders with CHOLMOD/METIS nested dissection (Solvers/cholmod_ordering.hh:71-96,
+// ParallelNestedDissection/), which is unavailable here; any fill-reducing ND ordering is
+// acceptable for the path (the factor is unique for a fixed ordering).  This is a multilevel
+// bisection: heavy-edge-matching coarsening, greedy graph growing on the coarsest graph,
+// boundary Fiduccia-Mattheyses refinement while uncoarsening, then a minimum vertex cover of
+// the cut edges (Koenig's theorem on the bipartite boundary graph) as the vertex separator.
+// Parts are ordered first, the separator last; small parts are leaves in their given order.
+#include "mfs_internal.h"
+
+#include <algorithm>
+#include <numeric>
+#include <queue>
+
+namespace mfs {
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    int32_t below(int32_t n) { return (int32_t)(next() % (uint64_t)n); }
+};
+
+struct G {
+    int32_t n = 0;
+    std::vector<int64_t> xadj;
+    std::vector<int32_t> adj, ew, vw;
+    int64_t total_vw() const { return std::accumulate(vw.begin(), vw.end(), int64_t(0)); }
+};
+
+G coarsen(const G& g, std::vector<int32_t>& cmap, Rng& rng, int32_t maxvw) {
+    const int32_t n = g.n;
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    for (int32_t i = n - 1; i > 0; --i) std::swap(order[i], order[rng.below(i + 1)]);
+    std::vector<int32_t> match(n, -1);
+    for (int32_t v : order) {
+        if (match[v] != -1) continue;
+        int32_t best = -1, bw = -1;
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+            const int32_t u = g.adj[e];
+            if (match[u] == -1 && u != v && g.vw[v] + g.vw[u] <= maxvw && g.ew[e] > bw) {
+                best = u;
+                bw = g.ew[e];
+            }
+        }
+        if (best >= 0) { match[v] = best; match[best] = v; }
+        else match[v] = v;
+    }
+    cmap.assign(n, -1);
+    std::vector<int32_t> rep;
+    rep.reserve(n / 2 + 1);
+    int32_t cn = 0;
+    for (int32_t v = 0; v < n; ++v)
+        if (cmap[v] == -1) {
+            cmap[v] = cn;
+            cmap[match[v]] = cn;
+            rep.push_back(v);
+            ++cn;
+        }
+    G c;
+    c.n = cn;
+    c.xadj.assign(cn + 1, 0);
+    c.vw.assign(cn, 0);
+    c.adj.reserve(g.adj.size() / 2);
+    c.ew.reserve(g.adj.size() / 2);
+    std::vector<int64_t> marker(cn, -1);
+    for (int32_t ci = 0; ci < cn; ++ci) {
+        const int64_t start = (int64_t)c.adj.size();
+        const int32_t v0 = rep[ci], v1 = match[v0];
+        for (int k = 0; k < 2; ++k) {
+            const int32_t u = k == 0 ? v0 : v1;
+            if (k == 1 && v1 == v0) break;
+            c.vw[ci] += g.vw[u];
+            for (int64_t e = g.xadj[u]; e < g.xadj[u + 1]; ++e) {
+                const int32_t cu = cmap[g.adj[e]];
+                if (cu == ci) continue;
+                const int64_t m = marker[cu];
+                if (m >= start && c.adj[m] == cu) c.ew[m] += g.ew[e];
+                else {
+                    marker[cu] = (int64_t)c.adj.size();
+                    c.adj.push_back(cu);
+                    c.ew.push_back(g.ew[e]);
+                }
+            }
+        }
+        c.xadj[ci + 1] = (int64_t)c.adj.size();
+    }
+    return c;
+}
+
+int64_t edge_cut(const G& g, const std::vector<int8_t>& where) {
+    int64_t cut = 0;
+    for (int32_t v = 0; v < g.n; ++v)
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e)
+            if (where[g.adj[e]] != where[v]) cut += g.ew[e];
+    return cut / 2;
+}
+
+// Boundary FM refinement of a 2-way edge cut under a balance constraint.
+void fm_refine(const G& g, std::vector<int8_t>& where, double ub, int npasses) {
+    const int32_t n = g.n;
+    const int64_t tw = g.total_vw();
+    int64_t maxvw = 0;
+    for (int32_t v = 0; v < n; ++v) maxvw = std::max<int64_t>(maxvw, g.vw[v]);
+    const int64_t maxw = std::max<int64_t>((int64_t)((1.0 + ub) * 0.5 * tw), (tw + 1) / 2 + maxvw);
+    std::vector<int64_t> id(n), ed(n);
+    std::vector<char> locked(n);
+    int64_t pw[2] = {0, 0};
+    for (int32_t v = 0; v < n; ++v) pw[where[v]] += g.vw[v];
+    for (int pass = 0; pass < npasses; ++pass) {
+        for (int32_t v = 0; v < n; ++v) {
+            id[v] = ed[v] = 0;
+            locked[v] = 0;
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e)
+                (where[g.adj[e]] == where[v] ? id[v] : ed[v]) += g.ew[e];
+        }
+        using Ent = std::pair<int64_t, int32_t>;
+        std::priority_queue<Ent> heap[2];
+        for (int32_t v = 0; v < n; ++v)
+            if (ed[v] > 0) heap[where[v]].push({ed[v] - id[v], v});
+        int64_t cut = edge_cut(g, where);
+        const int64_t cut0 = cut;
+        int64_t best_cut = cut;
+        int64_t best_imb = std::abs(pw[0] - pw[1]);
+        bool best_ok = pw[0] <= maxw && pw[1] <= maxw;
+        std::vector<int32_t> moves;
+        size_t best_len = 0;
+        const size_t limit = (size_t)std::max<int32_t>(40, std::min<int32_t>(n / 20, 400));
+        while (true) {
+            int32_t cand[2] = {-1, -1};
+            int64_t cg[2] = {0, 0};
+            for (int s = 0; s < 2; ++s) {
+                while (!heap[s].empty()) {
+                    const auto [gn, v] = heap[s].top();
+                    if (locked[v] || where[v] != s || gn != ed[v] - id[v]) { heap[s].pop(); continue; }
+                    cand[s] = v;
+                    cg[s] = gn;
+                    break;
+                }
+            }
+            int side = -1;
+            const bool ok0 = cand[0] >= 0 && pw[1] + g.vw[cand[0]] <= maxw;
+            const bool ok1 = cand[1] >= 0 && pw[0] + g.vw[cand[1]] <= maxw;
+            if (pw[0] > maxw && cand[0] >= 0) side = 0;
+            else if (pw[1] > maxw && cand[1] >= 0) side = 1;
+            else if (ok0 && ok1) side = (cg[0] > cg[1] || (cg[0] == cg[1] && pw[0] >= pw[1])) ? 0 : 1;
+            else if (ok0) side = 0;
+            else if (ok1) side = 1;
+            if (side < 0) break;
+            const int32_t v = cand[side];
+            heap[side].pop();
+            locked[v] = 1;
+            where[v] = (int8_t)(1 - side);
+            pw[side] -= g.vw[v];
+            pw[1 - side] += g.vw[v];
+            cut -= ed[v] - id[v];
+            std::swap(id[v], ed[v]);
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+                const int32_t u = g.adj[e];
+                if (where[u] == where[v]) { id[u] += g.ew[e]; ed[u] -= g.ew[e]; }
+                else { id[u] -= g.ew[e]; ed[u] += g.ew[e]; }
+                if (!locked[u] && ed[u] > 0) heap[where[u]].push({ed[u] - id[u], u});
+            }
+            moves.push_back(v);
+            const bool ok = pw[0] <= maxw && pw[1] <= maxw;
+            const int64_t imb = std::abs(pw[0] - pw[1]);
+            if ((ok && !best_ok) || (ok == best_ok && (cut < best_cut || (cut == best_cut && imb < best_imb)))) {
+                best_cut = cut;
+                best_imb = imb;
+                best_ok = ok;
+                best_len = moves.size();
+            }
+            if (moves.size() - best_len > limit) break;
+        }
+        for (size_t k = moves.size(); k > best_len; --k) {
+            const int32_t v = moves[k - 1];
+            const int s = where[v];
+            where[v] = (int8_t)(1 - s);
+            pw[s] -= g.vw[v];
+            pw[1 - s] += g.vw[v];
+        }
+        if (best_cut >= cut0 && best_len == 0) break;
+    }
+}
+
+// Greedy graph growing bisection of a (small) coarse graph; best of several trials.
+std::vector<int8_t> initial_bisection(const G& g, Rng& rng, double ub) {
+    const int32_t n = g.n;
+    const int64_t tw = g.total_vw();
+    std::vector<int8_t> best;
+    int64_t best_cut = -1;
+    const int trials = 8;
+    for (int t = 0; t < trials; ++t) {
+        std::vector<int8_t> where(n, 1);
+        std::vector<int64_t> conn0(n, 0);   // edge weight to part 0
+        std::vector<int64_t> deg(n, 0);
+        for (int32_t v = 0; v < n; ++v)
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) deg[v] += g.ew[e];
+        int64_t w0 = 0;
+        auto move = [&](int32_t v) {
+            where[v] = 0;
+            w0 += g.vw[v];
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) conn0[g.adj[e]] += g.ew[e];
+        };
+        move(rng.below(n));
+        while (2 * w0 < tw) {
+            int32_t bv = -1;
+            int64_t bg = INT64_MIN;
+            for (int32_t v = 0; v < n; ++v) {
+                if (where[v] != 1 || conn0[v] == 0) continue;
+                const int64_t gain = 2 * conn0[v] - deg[v];
+                if (gain > bg) { bg = gain; bv = v; }
+            }
+            if (bv < 0) {   // disconnected: restart growth from a random vertex of part 1
+                std::vector<int32_t> rest;
+                for (int32_t v = 0; v < n; ++v) if (where[v] == 1) rest.push_back(v);
+                if (rest.empty()) break;
+                bv = rest[rng.below((int32_t)rest.size())];
+            }
+            move(bv);
+        }
+        fm_refine(g, where, ub, 4);
+        const int64_t cut = edge_cut(g, where);
+        if (best_cut < 0 || cut < best_cut) { best_cut = cut; best = where; }
+    }
+    return best;
+}
+
+// Minimum vertex cover of the cut edges -> separator (where = 2).
+void edge_to_vertex_separator(const G& g, std::vector<int8_t>& where) {
+    const int32_t n = g.n;
+    std::vector<int32_t> bidx(n, -1);
+    std::vector<int32_t> L, R;
+    for (int32_t v = 0; v < n; ++v) {
+        bool bnd = false;
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1] && !bnd; ++e) bnd = where[g.adj[e]] != where[v];
+        if (!bnd) continue;
+        if (where[v] == 0) { bidx[v] = (int32_t)L.size(); L.push_back(v); }
+        else { bidx[v] = (int32_t)R.size(); R.push_back(v); }
+    }
+    if (L.empty()) return;
+    // Maximum bipartite matching L-R over cut edges (greedy + augmenting DFS).
+    std::vector<int32_t> mL(L.size(), -1), mR(R.size(), -1);
+    for (size_t i = 0; i < L.size(); ++i) {
+        const int32_t v = L[i];
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+            const int32_t u = g.adj[e];
+            if (where[u] == 1 && mR[bidx[u]] == -1) { mL[i] = bidx[u]; mR[bidx[u]] = (int32_t)i; break; }
+        }
+    }
+    std::vector<int32_t> stamp(L.size(), -1);
+    std::vector<std::pair<int32_t, int64_t>> stack;
+    for (size_t i0 = 0; i0 < L.size(); ++i0) {
+        if (mL[i0] != -1) continue;
+        // iterative DFS for an augmenting path from L[i0]
+        stack.clear();
+        stack.push_back({(int32_t)i0, g.xadj[L[i0]]});
+        stamp[i0] = (int32_t)i0;
+        std::vector<int32_t> pathR;
+        bool found = false;
+        while (!stack.empty() && !found) {
+            auto& [li, ep] = stack.back();
+            const int32_t v = L[li];
+            bool advanced = false;
+            while (ep < g.xadj[v + 1]) {
+                const int32_t u = g.adj[ep++];
+                if (where[u] != 1) continue;
+                const int32_t ri = bidx[u];
+                const int32_t lj = mR[ri];
+                if (lj == -1) {
+                    pathR.push_back(ri);
+                    found = true;
+                    advanced = true;
+                    break;
+                }
+                if (stamp[lj] == (int32_t)i0) continue;
+                stamp[lj] = (int32_t)i0;
+                pathR.push_back(ri);
+                stack.push_back({lj, g.xadj[L[lj]]});
+                advanced = true;
+                break;
+            }
+            if (!advanced) {
+                stack.pop_back();
+                if (!pathR.empty() && !stack.empty()) pathR.pop_back();
+            }
+        }
+        if (found) {
+            // stack holds L vertices of the path, pathR the R vertices (same length)
+            for (size_t k = 0; k < stack.size(); ++k) {
+                const int32_t li = stack[k].first, ri = pathR[k];
+                mL[li] = ri;
+                mR[ri] = li;
+            }
+        }
+    }
+    // Koenig: Z = reachable from unmatched L via alternating paths; cover = (L \ Z) u (R n Z)
+    std::vector<char> zL(L.size(), 0), zR(R.size(), 0);
+    std::vector<int32_t> q;
+    for (size_t i = 0; i < L.size(); ++i)
+        if (mL[i] == -1) { zL[i] = 1; q.push_back((int32_t)i); }
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int32_t v = L[q[h]];
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+            const int32_t u = g.adj[e];
+            if (where[u] != 1) continue;
+            const int32_t ri = bidx[u];
+            if (zR[ri]) continue;
+            zR[ri] = 1;
+            const int32_t lj = mR[ri];
+            if (lj >= 0 && !zL[lj]) { zL[lj] = 1; q.push_back(lj); }
+        }
+    }
+    for (size_t i = 0; i < L.size(); ++i) if (!zL[i]) where[L[i]] = 2;
+    for (size_t i = 0; i < R.size(); ++i) if (zR[i]) where[R[i]] = 2;
+}
+
+std::vector<int8_t> bisect(const G& g0, Rng& rng, const NDOptions& opt) {
+    std::vector<G> graphs;
+    std::vector<std::vector<int32_t>> cmaps;
+    graphs.push_back(g0);
+    const int64_t tw = g0.total_vw();
+    const int32_t maxvw = (int32_t)std::max<int64_t>(1, (int64_t)(1.5 * tw / opt.coarse_size));
+    while (graphs.back().n > opt.coarse_size) {
+        std::vector<int32_t> cmap;
+        G c = coarsen(graphs.back(), cmap, rng, maxvw);
+        if (c.n > 0.92 * graphs.back().n) break;
+        cmaps.push_back(std::move(cmap));
+        graphs.push_back(std::move(c));
+    }
+    std::vector<int8_t> where = initial_bisection(graphs.back(), rng, opt.imbalance);
+    for (int lvl = (int)graphs.size() - 2; lvl >= 0; --lvl) {
+        const auto& cm = cmaps[lvl];
+        std::vector<int8_t> fw(graphs[lvl].n);
+        for (int32_t v = 0; v < graphs[lvl].n; ++v) fw[v] = where[cm[v]];
+        where.swap(fw);
+        fm_refine(graphs[lvl], where, opt.imbalance, 6);
+    }
+    edge_to_vertex_separator(g0, where);
+    return where;
+}
+
+G induced(const G& g, const std::vector<int8_t>& where, int8_t part, std::vector<int32_t>& local,
+          std::vector<int32_t>& members) {
+    members.clear();
+    for (int32_t v = 0; v < g.n; ++v)
+        if (where[v] == part) { local[v] = (int32_t)members.size(); members.push_back(v); }
+    G s;
+    s.n = (int32_t)members.size();
+    s.xadj.assign(s.n + 1, 0);
+    s.vw.assign(s.n, 1);
+    for (int32_t i = 0; i < s.n; ++i) {
+        const int32_t v = members[i];
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e)
+            if (where[g.adj[e]] == part) { s.adj.push_back(local[g.adj[e]]); s.ew.push_back(1); }
+        s.xadj[i + 1] = (int64_t)s.adj.size();
+    }
+    return s;
+}
+
+void nd_recurse(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
+                const NDOptions& opt) {
+    if (g.n <= opt.leaf_size) {
+        out.insert(out.end(), gid.begin(), gid.end());
+        return;
+    }
+    std::vector<int8_t> where = bisect(g, rng, opt);
+    int32_t cnt[3] = {0, 0, 0};
+    for (int8_t w : where) cnt[w]++;
+    if (cnt[0] == 0 || cnt[1] == 0) {   // could not split (e.g. clique-like): leaf
+        out.insert(out.end(), gid.begin(), gid.end());
+        return;
+    }
+    std::vector<int32_t> local(g.n, -1), members;
+    for (int8_t part = 0; part < 2; ++part) {
+        G s = induced(g, where, part, local, members);
+        std::vector<int32_t> sg(members.size());
+        for (size_t i = 0; i < members.size(); ++i) sg[i] = gid[members[i]];
+        nd_recurse(s, sg, out, rng, opt);
+    }
+    for (int32_t v = 0; v < g.n; ++v)
+        if (where[v] == 2) out.push_back(gid[v]);
+}
+
+} // namespace
+
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
+                                       const std::vector<int32_t>& adj, const NDOptions& opt) {
+    G g;
+    g.n = n;
+    g.xadj = xadj;
+    g.adj = adj;
+    g.ew.assign(adj.size(), 1);
+    g.vw.assign(n, 1);
+    std::vector<int32_t> gid(n);
+    std::iota(gid.begin(), gid.end(), 0);
+    std::vector<int32_t> out;
+    out.reserve(n);
+    Rng rng{opt.seed};
+    nd_recurse(g, gid, out, rng, opt);
+    return out;
+}
+
+} // namespace mfs

@@ -1,0 +1,713 @@
+// Host symbolic analysis, computed once per (pattern, pins) and reused across Newton steps.
+//
+// Mirrors the work CatamariFactorizer::m_factorizeSymbolic does (CatamariFactorizer.cc:177-616):
+// whole-block pin removal (:187-246, SparseMatrices.hh:2020-2076), fill-reducing ordering
+// (:290-325), symbolic factorization into relaxed supernodes (:516, :102-107 relax_supernodes),
+// and the value-injection ("conversion") plan that maps each stored input block to its place in
+// the supernodal factor (CatamariConversionPlan.hh:402-541).  Everything is in 3x3 block units:
+// all scalar columns of a block share one row structure, so the block analysis is exact for
+// uniform block size 3 (the reference's IndexBlockSize compression, CatamariConversionPlan.hh:101-141).
+//
+// The numeric phase is a multifrontal supernodal LL^T: supernode s owns a column panel
+// [L11; L21] of (W+D) x W scalars (W = 3 width, D = 3 degree) and an update matrix U_s (D x D)
+// that receives its children's update matrices (extend-add) and then -L21 L21^T.  The level
+// schedule (supernode heights in the assembly tree) and every dense tile task are built here,
+// so the device side only replays fixed task lists.
+#include "mfs_internal.h"
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+
+namespace mfs {
+
+namespace {
+
+void etree(int32_t n, const std::vector<int64_t>& Up, const std::vector<int32_t>& Ui,
+           std::vector<int32_t>& parent) {
+    // Liu's algorithm with path compression; column k of the upper pattern holds rows i < k.
+    parent.assign(n, -1);
+    std::vector<int32_t> anc(n, -1);
+    for (int32_t k = 0; k < n; ++k)
+        for (int64_t p = Up[k]; p < Up[k + 1]; ++p) {
+            int32_t r = Ui[p];
+            while (anc[r] != -1 && anc[r] != k) {
+                const int32_t nx = anc[r];
+                anc[r] = k;
+                r = nx;
+            }
+            if (anc[r] == -1) { anc[r] = k; parent[r] = k; }
+        }
+}
+
+std::vector<int32_t> postorder(int32_t n, const std::vector<int32_t>& parent) {
+    std::vector<int32_t> head(n, -1), next(n, -1), post;
+    post.reserve(n);
+    for (int32_t j = n - 1; j >= 0; --j)
+        if (parent[j] != -1) { next[j] = head[parent[j]]; head[parent[j]] = j; }
+    std::vector<int32_t> stack;
+    for (int32_t j = 0; j < n; ++j) {
+        if (parent[j] != -1) continue;
+        stack.push_back(j);
+        while (!stack.empty()) {
+            const int32_t p = stack.back();
+            const int32_t c = head[p];
+            if (c == -1) { stack.pop_back(); post.push_back(p); }
+            else { head[p] = next[c]; stack.push_back(c); }
+        }
+    }
+    return post;
+}
+
+// Column counts of L (incl. diagonal) for a postordered etree (parent[j] > j), following the
+// row-subtree / least-common-ancestor characterisation (Gilbert, Ng, Peyton).
+std::vector<int64_t> column_counts(int32_t n, const std::vector<int32_t>& parent,
+                                   const std::vector<int64_t>& Lp, const std::vector<int32_t>& Li) {
+    std::vector<int64_t> delta(n, 0);
+    std::vector<int32_t> first(n, -1), maxfirst(n, -1), prevleaf(n, -1), ancestor(n);
+    for (int32_t k = 0; k < n; ++k) {
+        int32_t j = k;
+        delta[j] = (first[j] == -1) ? 1 : 0;
+        for (; j != -1 && first[j] == -1; j = parent[j]) first[j] = k;
+    }
+    std::iota(ancestor.begin(), ancestor.end(), 0);
+    for (int32_t j = 0; j < n; ++j) {
+        if (parent[j] != -1) delta[parent[j]]--;
+        for (int64_t p = Lp[j]; p < Lp[j + 1]; ++p) {
+            const int32_t i = Li[p];            // i > j
+            if (first[j] <= maxfirst[i]) continue;
+            maxfirst[i] = first[j];
+            const int32_t jprev = prevleaf[i];
+            prevleaf[i] = j;
+            delta[j]++;
+            if (jprev == -1) continue;           // first leaf of row subtree i
+            int32_t q = jprev;
+            while (q != ancestor[q]) q = ancestor[q];
+            for (int32_t s = jprev; s != q;) { const int32_t sp = ancestor[s]; ancestor[s] = q; s = sp; }
+            delta[q]--;                          // q = lca(jprev, j)
+        }
+        if (parent[j] != -1) ancestor[j] = parent[j];
+    }
+    for (int32_t j = 0; j < n; ++j)
+        if (parent[j] != -1) delta[parent[j]] += delta[j];
+    return delta;
+}
+
+// First-fit allocator over a growing arena (U buffers of the multifrontal schedule).
+struct Arena {
+    std::map<int64_t, int64_t> free_;   // offset -> size
+    int64_t top = 0, peak = 0;
+    int64_t alloc(int64_t sz) {
+        sz = (sz + 31) & ~int64_t(31);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= sz) {
+                const int64_t off = it->first, rest = it->second - sz;
+                free_.erase(it);
+                if (rest > 0) free_[off + sz] = rest;
+                return off;
+            }
+        const int64_t off = top;
+        top += sz;
+        peak = std::max(peak, top);
+        return off;
+    }
+    void release(int64_t off, int64_t sz) {
+        sz = (sz + 31) & ~int64_t(31);
+        auto it = free_.emplace(off, sz).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+                it = pv;
+            }
+        }
+        if (it->first + it->second == top) { top = it->first; free_.erase(it); }
+    }
+};
+
+} // namespace
+
+Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
+                 int64_t npins, const SymbolicOptions& opt) {
+    Symbolic S;
+    if (nb <= 0) throw Error(-1, "empty matrix");
+    if (nb >= (int64_t(1) << 30)) throw Error(-6, "matrix too large");
+    S.n_full_blocks = nb;
+    S.nz_full = Ap[nb];
+    for (int64_t j = 0; j < nb; ++j) {
+        if (Ap[j + 1] <= Ap[j]) throw Error(-1, "column " + std::to_string(j) + " has no diagonal block");
+        for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+            if (Ai[k] < 0 || Ai[k] > j) throw Error(-1, "block pattern must be upper triangular");
+            if (k > Ap[j] && Ai[k] <= Ai[k - 1]) throw Error(-1, "block rows must be sorted and unique");
+        }
+        if (Ai[Ap[j + 1] - 1] != j) throw Error(-1, "diagonal block must be last in its column");
+    }
+
+    // ---- whole-block pins (CatamariFactorizer.cc:187-246) ----
+    std::vector<int8_t> pinned_comp(nb, 0);
+    {
+        std::vector<char> seen((size_t)nb * 3, 0);
+        for (int64_t q = 0; q < npins; ++q) {
+            const int64_t v = pins[q];
+            if (v < 0 || v >= 3 * nb) throw Error(-1, "Fixed variable index out of range");
+            if (seen[v]) continue;
+            seen[v] = 1;
+            S.fixed_scalar.push_back(v);
+            pinned_comp[v / 3]++;
+        }
+        std::sort(S.fixed_scalar.begin(), S.fixed_scalar.end());
+        for (int64_t b = 0; b < nb; ++b)
+            if (pinned_comp[b] != 0 && pinned_comp[b] != 3)
+                throw Error(-6, "Partially-pinned block variables are not supported by the block factorization");
+    }
+    S.red_of_full.assign(nb, -1);
+    int64_t nr = 0;
+    for (int64_t b = 0; b < nb; ++b)
+        if (!pinned_comp[b]) S.red_of_full[b] = nr++;
+    S.n_red = nr;
+    if (nr == 0) throw Error(-1, "every variable is pinned");
+    const int32_t n = (int32_t)nr;
+
+    // reduced entries (reduced-CSC order) and the adjacency graph
+    struct RE { int32_t i, j; int32_t src; };
+    std::vector<RE> red;
+    red.reserve(S.nz_full);
+    for (int64_t j = 0; j < nb; ++j) {
+        if (S.red_of_full[j] < 0) continue;
+        for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+            const int64_t i = Ai[k];
+            if (S.red_of_full[i] < 0) continue;
+            red.push_back({(int32_t)S.red_of_full[i], (int32_t)S.red_of_full[j], (int32_t)k});
+        }
+    }
+    std::vector<int64_t> xadj(n + 1, 0);
+    for (const RE& e : red)
+        if (e.i != e.j) { xadj[e.i + 1]++; xadj[e.j + 1]++; }
+    for (int32_t v = 0; v < n; ++v) xadj[v + 1] += xadj[v];
+    std::vector<int32_t> adj(xadj[n]);
+    {
+        std::vector<int64_t> bk(xadj.begin(), xadj.end() - 1);
+        for (const RE& e : red)
+            if (e.i != e.j) { adj[bk[e.i]++] = e.j; adj[bk[e.j]++] = e.i; }
+    }
+
+    // ---- ordering: ND, then etree postorder ----
+    std::vector<int32_t> perm = nested_dissection(n, xadj, adj, opt.nd);   // perm[new] = old
+    auto build_patterns = [&](const std::vector<int32_t>& pos_, std::vector<int64_t>& Lp,
+                              std::vector<int32_t>& Li, std::vector<int64_t>& Up,
+                              std::vector<int32_t>& Ui) {
+        Lp.assign(n + 1, 0);
+        Up.assign(n + 1, 0);
+        for (const RE& e : red) {
+            if (e.i == e.j) continue;
+            const int32_t a = pos_[e.i], b = pos_[e.j];
+            Lp[std::min(a, b) + 1]++;
+            Up[std::max(a, b) + 1]++;
+        }
+        for (int32_t v = 0; v < n; ++v) { Lp[v + 1] += Lp[v]; Up[v + 1] += Up[v]; }
+        Li.resize(Lp[n]);
+        Ui.resize(Up[n]);
+        std::vector<int64_t> bl(Lp.begin(), Lp.end() - 1), bu(Up.begin(), Up.end() - 1);
+        for (const RE& e : red) {
+            if (e.i == e.j) continue;
+            const int32_t a = pos_[e.i], b = pos_[e.j];
+            Li[bl[std::min(a, b)]++] = std::max(a, b);
+            Ui[bu[std::max(a, b)]++] = std::min(a, b);
+        }
+        for (int32_t v = 0; v < n; ++v) {
+            std::sort(Li.begin() + Lp[v], Li.begin() + Lp[v + 1]);
+            std::sort(Ui.begin() + Up[v], Ui.begin() + Up[v + 1]);
+        }
+    };
+    std::vector<int32_t> posv(n);
+    for (int32_t k = 0; k < n; ++k) posv[perm[k]] = k;
+    std::vector<int64_t> Lp, Up;
+    std::vector<int32_t> Li, Ui, parent;
+    build_patterns(posv, Lp, Li, Up, Ui);
+    etree(n, Up, Ui, parent);
+    {
+        std::vector<int32_t> post = postorder(n, parent);
+        S.iperm.resize(n);
+        for (int32_t k = 0; k < n; ++k) S.iperm[k] = perm[post[k]];
+        S.pos.resize(n);
+        for (int32_t k = 0; k < n; ++k) S.pos[S.iperm[k]] = k;
+    }
+    build_patterns(S.pos, Lp, Li, Up, Ui);
+    etree(n, Up, Ui, parent);
+    for (int32_t j = 0; j < n; ++j)
+        if (parent[j] != -1 && parent[j] <= j) throw Error(-2, "internal: etree not postordered");
+    std::vector<int64_t> cc = column_counts(n, parent, Lp, Li);
+    for (int32_t j = 0; j < n; ++j) {
+        const double c = 3.0 * cc[j];
+        S.flops_unrelaxed += c * c + (c - 1) * (c - 1) + (c - 2) * (c - 2);
+    }
+
+    // ---- fundamental supernodes ----
+    std::vector<int32_t> nchild(n, 0);
+    for (int32_t j = 0; j < n; ++j) if (parent[j] != -1) nchild[parent[j]]++;
+    struct SN { int32_t f, w; int64_t d; double nz; };
+    std::vector<SN> fund;
+    for (int32_t j = 0; j < n; ++j) {
+        const bool cont = j > 0 && parent[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1;
+        if (cont) fund.back().w++;
+        else fund.push_back({j, 1, 0, 0.0});
+    }
+    for (SN& s : fund) {
+        s.d = cc[s.f] - s.w;
+        s.nz = 0.5 * s.w * (s.w + 1.0) + (double)s.w * s.d;
+    }
+
+    // ---- relaxed amalgamation (contiguous last child into parent) ----
+    std::vector<SN> merged;
+    for (const SN& s0 : fund) {
+        SN cur = s0;
+        while (!merged.empty()) {
+            const SN& c = merged.back();
+            const int32_t clast = c.f + c.w - 1;
+            if (clast != cur.f - 1) break;
+            const int32_t pcol = parent[clast];
+            if (pcol < cur.f || pcol >= cur.f + cur.w) break;
+            const int64_t w = (int64_t)c.w + cur.w;
+            const double tot = 0.5 * w * (w + 1.0) + (double)w * cur.d;
+            const double z = (tot - (c.nz + cur.nz)) / tot;
+            const int64_t ncol = 3 * w;
+            const bool ok = ncol <= opt.nrelax0 || (ncol <= opt.nrelax1 && z < opt.zrelax0) ||
+                            (ncol <= opt.nrelax2 && z < opt.zrelax1) || z < opt.zrelax2;
+            if (!ok) break;
+            cur.f = c.f;
+            cur.w = (int32_t)w;
+            cur.nz = c.nz + cur.nz;
+            merged.pop_back();
+        }
+        merged.push_back(cur);
+    }
+    const int32_t ns = (int32_t)merged.size();
+    S.ns = ns;
+    S.sfirst.resize(ns);
+    S.swidth.resize(ns);
+    S.super_of.resize(n);
+    for (int32_t s = 0; s < ns; ++s) {
+        S.sfirst[s] = merged[s].f;
+        S.swidth[s] = merged[s].w;
+        for (int32_t j = merged[s].f; j < merged[s].f + merged[s].w; ++j) S.super_of[j] = s;
+    }
+    S.sparent.assign(ns, -1);
+    for (int32_t s = 0; s < ns; ++s) {
+        const int32_t last = S.sfirst[s] + S.swidth[s] - 1;
+        if (parent[last] != -1) S.sparent[s] = S.super_of[parent[last]];
+    }
+    // children lists (ascending supernode id)
+    S.child_ptr.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) if (S.sparent[s] >= 0) S.child_ptr[S.sparent[s] + 1]++;
+    for (int32_t s = 0; s < ns; ++s) S.child_ptr[s + 1] += S.child_ptr[s];
+    S.children.resize(S.child_ptr[ns]);
+    {
+        std::vector<int32_t> bk(S.child_ptr.begin(), S.child_ptr.end() - 1);
+        for (int32_t s = 0; s < ns; ++s) if (S.sparent[s] >= 0) S.children[bk[S.sparent[s]]++] = s;
+    }
+
+    // ---- supernode row structures ----
+    S.sdeg.assign(ns, 0);
+    S.srows_off.assign(ns + 1, 0);
+    {
+        std::vector<int32_t> mark(n, -1), tmp;
+        std::vector<std::vector<int32_t>> st(ns);
+        for (int32_t s = 0; s < ns; ++s) {
+            const int32_t f = S.sfirst[s], last = f + S.swidth[s] - 1;
+            tmp.clear();
+            for (int32_t j = f; j <= last; ++j)
+                for (int64_t p = Lp[j]; p < Lp[j + 1]; ++p) {
+                    const int32_t i = Li[p];
+                    if (i > last && mark[i] != s) { mark[i] = s; tmp.push_back(i); }
+                }
+            for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q)
+                for (int32_t i : st[S.children[q]])
+                    if (i > last && mark[i] != s) { mark[i] = s; tmp.push_back(i); }
+            std::sort(tmp.begin(), tmp.end());
+            st[s] = tmp;
+        }
+        for (int32_t s = 0; s < ns; ++s) {
+            S.sdeg[s] = (int32_t)st[s].size();
+            if (S.sdeg[s] != merged[s].d) throw Error(-2, "internal: supernode structure size mismatch");
+            S.srows_off[s + 1] = S.srows_off[s] + S.sdeg[s];
+        }
+        S.rows.resize(S.srows_off[ns]);
+        for (int32_t s = 0; s < ns; ++s)
+            std::copy(st[s].begin(), st[s].end(), S.rows.begin() + S.srows_off[s]);
+    }
+
+    // ---- levels (height in the assembly tree), sizes, offsets ----
+    S.slevel.assign(ns, 0);
+    for (int32_t s = 0; s < ns; ++s)
+        if (S.sparent[s] >= 0) S.slevel[S.sparent[s]] = std::max(S.slevel[S.sparent[s]], S.slevel[s] + 1);
+    S.num_levels = 0;
+    for (int32_t s = 0; s < ns; ++s) S.num_levels = std::max(S.num_levels, S.slevel[s] + 1);
+    S.panel_off.resize(ns + 1);
+    S.panel_off[0] = 0;
+    S.factor_nnz = 0;
+    S.flops = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        const int64_t W = 3 * (int64_t)S.swidth[s], D = 3 * (int64_t)S.sdeg[s];
+        S.panel_off[s + 1] = S.panel_off[s] + ((W + D) * W + 31) / 32 * 32;
+        S.factor_nnz += W * (W + 1) / 2 + W * D;
+        S.flops += (double)W * W * W / 3.0 + (double)D * W * W + (double)D * D * W;
+        S.max_width = std::max(S.max_width, W);
+        S.max_degree = std::max(S.max_degree, D);
+    }
+    S.L_size = S.panel_off[ns];
+
+    // relative indices of each child's structure in its parent's row list
+    S.rel_off.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) S.rel_off[s + 1] = S.rel_off[s] + (S.sparent[s] >= 0 ? S.sdeg[s] : 0);
+    S.rel.resize(S.rel_off[ns]);
+    for (int32_t c = 0; c < ns; ++c) {
+        const int32_t p = S.sparent[c];
+        if (p < 0) continue;
+        const int32_t fp = S.sfirst[p], wp = S.swidth[p];
+        const int32_t* prow = S.rows.data() + S.srows_off[p];
+        const int32_t* pend = prow + S.sdeg[p];
+        for (int32_t k = 0; k < S.sdeg[c]; ++k) {
+            const int32_t r = S.rows[S.srows_off[c] + k];
+            int32_t rp;
+            if (r < fp) throw Error(-2, "internal: child row above parent");
+            if (r < fp + wp) rp = r - fp;
+            else {
+                const int32_t* it = std::lower_bound(prow, pend, r);
+                if (it == pend || *it != r) throw Error(-2, "internal: child row missing in parent");
+                rp = wp + (int32_t)(it - prow);
+            }
+            S.rel[S.rel_off[c] + k] = rp;
+        }
+    }
+
+    // ---- injection plan ----
+    S.plan_src.resize(red.size());
+    S.plan_dst.resize(red.size());
+    for (size_t q = 0; q < red.size(); ++q) {
+        const RE& e = red[q];
+        const int32_t a = S.pos[e.i], b = S.pos[e.j];
+        const int32_t col = std::min(a, b), row = std::max(a, b);
+        const int32_t s = S.super_of[col];
+        const int32_t f = S.sfirst[s], w = S.swidth[s];
+        int64_t rp;
+        if (row < f + w) rp = row - f;
+        else {
+            const int32_t* prow = S.rows.data() + S.srows_off[s];
+            const int32_t* it = std::lower_bound(prow, prow + S.sdeg[s], row);
+            if (it == prow + S.sdeg[s] || *it != row) throw Error(-2, "internal: injection row missing");
+            rp = w + (it - prow);
+        }
+        const int64_t ld = 3 * ((int64_t)w + S.sdeg[s]);
+        const int64_t dst = S.panel_off[s] + 3 * (int64_t)(col - f) * ld + 3 * rp;
+        const int64_t mode = (a == b) ? 2 : (a > b ? 0 : 1);
+        S.plan_src[q] = e.src;
+        S.plan_dst[q] = (dst << 2) | mode;
+    }
+
+    // ---- level schedule: U arena, tile tasks, solve tasks ----
+    const int T = opt.tile;
+    std::vector<std::vector<int32_t>> bylevel(S.num_levels);
+    for (int32_t s = 0; s < ns; ++s) bylevel[S.slevel[s]].push_back(s);
+    S.u_off.assign(ns, -1);
+    Arena arena;
+    S.levels.resize(S.num_levels);
+    S.u_solve_off.assign(ns, 0);
+    {
+        int64_t acc = 0;
+        for (int32_t s = 0; s < ns; ++s) { S.u_solve_off[s] = acc; acc += 3 * (int64_t)S.sdeg[s]; }
+        S.u_solve_size = acc;
+    }
+    int64_t max_potrf = 0, max_part = 0;
+    std::vector<std::pair<int64_t,int64_t>> zero_ranges;
+    for (int32_t lv = 0; lv < S.num_levels; ++lv) {
+        LevelPlan& P = S.levels[lv];
+        const auto& sl = bylevel[lv];
+        // allocate U for this level
+        for (int32_t s : sl) {
+            const int64_t D = 3 * (int64_t)S.sdeg[s];
+            if (D > 0) S.u_off[s] = arena.alloc(D * D);
+        }
+        // extend-add passes by child rank
+        int32_t maxc = 0;
+        for (int32_t s : sl) maxc = std::max(maxc, S.child_ptr[s + 1] - S.child_ptr[s]);
+        for (int32_t r = 0; r < maxc; ++r) {
+            const int64_t b0 = (int64_t)S.extend_tasks.size();
+            for (int32_t p : sl) {
+                if (S.child_ptr[p] + r >= S.child_ptr[p + 1]) continue;
+                const int32_t c = S.children[S.child_ptr[p] + r];
+                if (S.sdeg[c] == 0) continue;
+                for (int32_t kj = 0; kj < S.sdeg[c]; ++kj) {
+                    ExtendTask t;
+                    t.u_child = S.u_off[c];
+                    t.p_panel = S.panel_off[p];
+                    t.u_parent = S.u_off[p];
+                    t.dc = S.sdeg[c];
+                    t.ldp = 3 * (S.swidth[p] + S.sdeg[p]);
+                    t.wp = S.swidth[p];
+                    t.dp = S.sdeg[p];
+                    t.rel_off = S.rel_off[c];
+                    t.col_block = kj;
+                    t.pad = 0;
+                    S.extend_tasks.push_back(t);
+                }
+            }
+            P.extend_passes.push_back({b0, (int64_t)S.extend_tasks.size()});
+        }
+        // release children's U (consumed by this level's extend-add)
+        for (int32_t p : sl)
+            for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
+                const int32_t c = S.children[q];
+                if (S.u_off[c] >= 0) {
+                    const int64_t D = 3 * (int64_t)S.sdeg[c];
+                    arena.release(S.u_off[c], D * D);
+                }
+            }
+        // zero ranges for this level's U (cleared before the extend-add)
+        {
+            const int64_t z0 = (int64_t)zero_ranges.size();
+            for (int32_t s : sl)
+                if (S.u_off[s] >= 0) {
+                    const int64_t D = 3 * (int64_t)S.sdeg[s];
+                    zero_ranges.push_back({S.u_off[s], D * D});
+                }
+            P.zero_u = {z0, (int64_t)zero_ranges.size()};
+        }
+        // dense steps
+        int32_t maxsteps = 0;
+        for (int32_t s : sl) maxsteps = std::max<int32_t>(maxsteps, (3 * S.swidth[s] + T - 1) / T);
+        for (int32_t t = 0; t < maxsteps; ++t) {
+            const int64_t p0 = (int64_t)S.potrf_tasks.size();
+            int32_t slot = 0;
+            std::vector<std::pair<int32_t,int32_t>> active;   // (sn, slot)
+            for (int32_t s : sl) {
+                const int32_t W = 3 * S.swidth[s];
+                if (t * T >= W) continue;
+                const int32_t D = 3 * S.sdeg[s], ld = W + D, k0 = t * T, kb = std::min(T, W - k0);
+                PotrfTask pt;
+                pt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
+                pt.ld = ld;
+                pt.kb = kb;
+                pt.slot = slot;
+                pt.col0 = 3 * S.sfirst[s] + k0;
+                S.potrf_tasks.push_back(pt);
+                active.push_back({s, slot});
+                ++slot;
+            }
+            max_potrf = std::max<int64_t>(max_potrf, slot);
+            P.potrf_steps.push_back({p0, (int64_t)S.potrf_tasks.size()});
+            const int64_t g0 = (int64_t)S.gemm_tasks.size();
+            for (auto [s, sl_] : active) {
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                const int64_t off = S.panel_off[s];
+                for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
+                    GemmTask g{};
+                    g.a_off = off + (int64_t)k0 * ld + r0;
+                    g.lda = ld;
+                    g.b_off = (int64_t)sl_ * T * T;
+                    g.ldb = T;
+                    g.c_off = g.a_off;
+                    g.ldc = ld;
+                    g.M = std::min(T, W + D - r0);
+                    g.N = kb;
+                    g.K = kb;
+                    g.flags = GT_BETA0 | GT_ALPHA_POS | GT_B_IN_INV;
+                    S.gemm_tasks.push_back(g);
+                }
+            }
+            P.trsm_steps.push_back({g0, (int64_t)S.gemm_tasks.size()});
+            const int64_t u0 = (int64_t)S.gemm_tasks.size();
+            for (auto [s, sl_] : active) {
+                (void)sl_;
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                const int64_t off = S.panel_off[s];
+                for (int32_t c0 = k0 + kb; c0 < W; c0 += T)
+                    for (int32_t r0 = c0; r0 < W + D; r0 += T) {
+                        GemmTask g{};
+                        g.a_off = off + (int64_t)k0 * ld + r0;
+                        g.lda = ld;
+                        g.b_off = off + (int64_t)k0 * ld + c0;
+                        g.ldb = ld;
+                        g.c_off = off + (int64_t)c0 * ld + r0;
+                        g.ldc = ld;
+                        g.M = std::min(T, W + D - r0);
+                        g.N = std::min(T, W - c0);
+                        g.K = kb;
+                        g.flags = 0;
+                        S.gemm_tasks.push_back(g);
+                    }
+            }
+            P.update_steps.push_back({u0, (int64_t)S.gemm_tasks.size()});
+        }
+        // Schur update into U
+        {
+            const int64_t s0 = (int64_t)S.gemm_tasks.size();
+            for (int32_t s : sl) {
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                if (D == 0) continue;
+                const int64_t off = S.panel_off[s];
+                for (int32_t c0 = 0; c0 < D; c0 += T)
+                    for (int32_t r0 = c0; r0 < D; r0 += T) {
+                        GemmTask g{};
+                        g.a_off = off + W + r0;
+                        g.lda = ld;
+                        g.b_off = off + W + c0;
+                        g.ldb = ld;
+                        g.c_off = S.u_off[s] + (int64_t)c0 * D + r0;
+                        g.ldc = D;
+                        g.M = std::min(T, D - r0);
+                        g.N = std::min(T, D - c0);
+                        g.K = W;
+                        g.flags = GT_C_IN_U;
+                        S.gemm_tasks.push_back(g);
+                    }
+            }
+            P.syrk = {s0, (int64_t)S.gemm_tasks.size()};
+        }
+        // forward solve: gather, then per step diag + gemv
+        {
+            const int64_t f0 = (int64_t)S.fgather_tasks.size();
+            for (int32_t s : sl) {
+                SolveGatherTask g{};
+                g.sn = s;
+                g.child_begin = S.child_ptr[s];
+                g.child_end = S.child_ptr[s + 1];
+                g.w = 3 * S.swidth[s];
+                g.d = 3 * S.sdeg[s];
+                g.x_off = 3 * (int64_t)S.sfirst[s];
+                g.u_off = S.u_solve_off[s];
+                S.fgather_tasks.push_back(g);
+            }
+            P.fgather = {f0, (int64_t)S.fgather_tasks.size()};
+            for (int32_t t = 0; t < maxsteps; ++t) {
+                const int64_t d0 = (int64_t)S.fdiag_tasks.size();
+                const int64_t v0 = (int64_t)S.fgemv_tasks.size();
+                for (int32_t s : sl) {
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                    if (t * T >= W) continue;
+                    const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                    SolveDiagTask dt{};
+                    dt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
+                    dt.ld = ld;
+                    dt.kb = kb;
+                    dt.x_off = 3 * (int64_t)S.sfirst[s] + k0;
+                    S.fdiag_tasks.push_back(dt);
+                    for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
+                        SolveGemvTask gv{};
+                        gv.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
+                        gv.ld = ld;
+                        gv.nr = std::min(T, W + D - r0);
+                        gv.kb = kb;
+                        gv.w = W;
+                        gv.r0 = r0;
+                        gv.x_cols = 3 * (int64_t)S.sfirst[s] + k0;
+                        gv.x_rows = 3 * (int64_t)S.sfirst[s];
+                        gv.u_off = S.u_solve_off[s];
+                        gv.rows_off = S.srows_off[s];
+                        S.fgemv_tasks.push_back(gv);
+                    }
+                }
+                P.fdiag_steps.push_back({d0, (int64_t)S.fdiag_tasks.size()});
+                P.fgemv_steps.push_back({v0, (int64_t)S.fgemv_tasks.size()});
+            }
+        }
+        // backward solve: per step (last column block first): partial gemv^T, then diag
+        {
+            const int C = opt.solve_chunk;
+            for (int32_t t = 0; t < maxsteps; ++t) {
+                const int64_t v0 = (int64_t)S.bgemv_tasks.size();
+                const int64_t d0 = (int64_t)S.bdiag_tasks.size();
+                int32_t part = 0;
+                for (int32_t s : sl) {
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                    const int32_t nblk = (W + T - 1) / T;
+                    if (t >= nblk) continue;
+                    const int32_t kblk = nblk - 1 - t;
+                    const int32_t k0 = kblk * T, kb = std::min(T, W - k0);
+                    SolveDiagTask dt{};
+                    dt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
+                    dt.ld = ld;
+                    dt.kb = kb;
+                    dt.x_off = 3 * (int64_t)S.sfirst[s] + k0;
+                    dt.part_begin = part;
+                    for (int32_t r0 = k0 + kb; r0 < W + D; r0 += C) {
+                        SolveGemvTask gv{};
+                        gv.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
+                        gv.ld = ld;
+                        gv.nr = std::min(C, W + D - r0);
+                        gv.kb = kb;
+                        gv.w = W;
+                        gv.r0 = r0;
+                        gv.x_cols = 3 * (int64_t)S.sfirst[s] + k0;
+                        gv.x_rows = 3 * (int64_t)S.sfirst[s];
+                        gv.u_off = 0;
+                        gv.rows_off = S.srows_off[s];
+                        gv.part = part++;
+                        S.bgemv_tasks.push_back(gv);
+                    }
+                    dt.part_end = part;
+                    S.bdiag_tasks.push_back(dt);
+                }
+                max_part = std::max<int64_t>(max_part, part);
+                P.bgemv_steps.push_back({v0, (int64_t)S.bgemv_tasks.size()});
+                P.bdiag_steps.push_back({d0, (int64_t)S.bdiag_tasks.size()});
+            }
+        }
+    }
+    S.U_size = std::max<int64_t>(arena.peak, 32);
+    S.inv_slots = std::max<int64_t>(max_potrf, 1);
+    S.bpart_size = std::max<int64_t>(max_part, 1);
+    S.zero_ranges = std::move(zero_ranges);
+
+    // ---- symmetric SpMV map over the full pattern (block-CSR, both triangles) ----
+    {
+        std::vector<int64_t> cnt(nb + 1, 0);
+        for (int64_t j = 0; j < nb; ++j)
+            for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+                cnt[j + 1]++;                          // (Ai[k], j) contributes to row j
+                if (Ai[k] != j) cnt[Ai[k] + 1]++;      // and to row Ai[k]
+            }
+        for (int64_t i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
+        S.spmv_ptr = cnt;
+        S.spmv_ent.resize(cnt[nb]);
+        S.spmv_col.resize(cnt[nb]);
+        std::vector<int64_t> bk(cnt.begin(), cnt.end() - 1);
+        // row i: first its own column's blocks (rows j <= i, transposed / diagonal), ascending j
+        for (int64_t i = 0; i < nb; ++i)
+            for (int64_t k = Ap[i]; k < Ap[i + 1]; ++k) {
+                const int64_t j = Ai[k];
+                S.spmv_ent[bk[i]] = (k << 2) | (j == i ? 2 : 1);
+                S.spmv_col[bk[i]++] = (int32_t)j;
+            }
+        // then the blocks (i, j), j > i, ascending j
+        for (int64_t j = 0; j < nb; ++j)
+            for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+                const int64_t i = Ai[k];
+                if (i == j) continue;
+                S.spmv_ent[bk[i]] = (k << 2) | 0;
+                S.spmv_col[bk[i]++] = (int32_t)j;
+            }
+    }
+
+    // ---- pin / permutation maps (scalar) ----
+    S.full_to_perm.assign(3 * nb, -1);
+    S.full_to_red.assign(3 * nb, -1);
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t r = S.red_of_full[b];
+        if (r < 0) continue;
+        for (int c = 0; c < 3; ++c) {
+            S.full_to_red[3 * b + c] = 3 * r + c;
+            S.full_to_perm[3 * b + c] = 3 * (int64_t)S.pos[r] + c;
+        }
+    }
+    return S;
+}
+
+} // namespace mfs

@@ -1,0 +1,297 @@
+"""Host mirror of the reference's factorizer interface over the GPU supernodal factorizer.
+
+CholeskyFactorizerBase (src/lib/MeshFEMSparse/Solvers/CholeskyFactorizerBase.hh:50-443) and the
+factory make_cholesky_factorizer(provider, singlePrecision)
+(src/lib/MeshFEMSparse/Solvers/make_cholesky_factorizer.hh:8-10) with a new provider value,
+CholeskyProvider.GpuSupernodal.  Method names, argument meaning and failure behaviour follow the
+reference; vectors are float64 CUDA tensors (column-major n x nrhs for multi-RHS calls).
+Failures raise the same kinds of errors: a missing factorization or a non-SPD matrix raises
+RuntimeError (MFSError subclasses), bad sizes raise ValueError.
+"""
+import enum
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MFSError, NotPositiveDefiniteError, c_vp, ptr
+from .sparse import _dptr, _stream
+
+
+class CholeskyProvider(enum.Enum):
+    """CholeskyFactorizerBase.hh:19-24 plus the MI355X provider."""
+    CHOLMOD = 0
+    Catamari = 1
+    CatamariNesdis = 2
+    CatamariNesdisParallel = 3
+    CatamariMetis = 4
+    CatamariLegacy = 5
+    CatamariAMD = 6
+    CatamariScotch = 7
+    CatamariAdaptive = 8
+    PARDISO = 9
+    Accelerate = 10
+    CatamariNesdisReuse = 11
+    CatamariNative = 12
+    CHOLMODNative = 13
+    GpuSupernodal = 100
+
+
+class CholeskySys(enum.Enum):
+    A = 0
+    L = 1
+    Lt = 2
+    P = 3
+    Pt = 4
+
+
+class FactorizationType(enum.IntEnum):
+    None_ = 0
+    Symbolic = 1
+    Numeric = 2
+
+
+class GpuSupernodalFactorizer:
+    """Supernodal LL^T on one MI355X: host ND ordering + symbolic analysis (once per pattern),
+    device value injection, numeric factorization and solves (per Newton step)."""
+
+    def __init__(self, singlePrecision=False):
+        self.singlePrecision = bool(singlePrecision)
+        self._h = c_vp()
+        _lib.call("mfs_factorizer_create", _lib.MFS_PREC_FP32 if singlePrecision else _lib.MFS_PREC_FP64,
+                  self._h)
+        self._fixed = []
+        self._n_full = 0
+        self._n_red = 0
+        self._Hpattern = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.mfs_factorizer_destroy(self._h)
+            self._h = None
+
+    # -- state ----------------------------------------------------------------------------------
+    def _ftype(self):
+        t = _lib.c_i32()
+        _lib.call("mfs_factorization_type", self._h, t)
+        return t.value
+
+    def hasFactorization(self, t=FactorizationType.Numeric):
+        if isinstance(t, CholeskySys):
+            t = FactorizationType.Numeric if t in (CholeskySys.A, CholeskySys.L, CholeskySys.Lt) \
+                else FactorizationType.Symbolic
+        return self._ftype() >= int(t)
+
+    def hasNumericFactorization(self):
+        return self.hasFactorization(FactorizationType.Numeric)
+
+    def checkPosDef(self):
+        return self._ftype() == FactorizationType.Numeric
+
+    def assertFactorization(self, t=FactorizationType.Numeric):
+        if not self.hasFactorization(t):
+            raise RuntimeError(("Numeric" if t == FactorizationType.Numeric else "Symbolic") +
+                               " factorization does not exist")
+
+    def provider(self):
+        return CholeskyProvider.GpuSupernodal
+
+    def preferInPlaceSolve(self):
+        return True
+
+    def supportsPrePermutation(self):
+        return True
+
+    def m(self):
+        return self._n_full
+
+    def n(self):
+        return self._n_full
+
+    def m_reduced(self):
+        self.assertFactorization(FactorizationType.Symbolic)
+        return self._n_red
+
+    def n_reduced(self):
+        return self.m_reduced()
+
+    def hasFixedVars(self):
+        return len(self._fixed) > 0
+
+    def getFixedVars(self):
+        return list(self._fixed)
+
+    def getFactorNNZ(self):
+        v = _lib.c_i64()
+        _lib.call("mfs_factor_nnz", self._h, v)
+        return v.value
+
+    def getFlopEstimate(self):
+        v = _lib.c_double()
+        _lib.call("mfs_flop_estimate", self._h, v)
+        return v.value
+
+    def getInversePermutation(self):
+        out = np.empty(self.n_reduced(), dtype=np.int64)
+        _lib.call("mfs_get_inverse_permutation", self._h, ptr(out, _lib.c_i64))
+        return out
+
+    def supernodeStats(self):
+        v = [_lib.c_i64() for _ in range(4)]
+        _lib.call("mfs_supernode_stats", self._h, *v)
+        return dict(num_supernodes=v[0].value, num_levels=v[1].value, max_width=v[2].value,
+                    max_degree=v[3].value)
+
+    def numSuccessfulPivots(self):
+        v = _lib.c_i64()
+        _lib.call("mfs_num_successful_pivots", self._h, v)
+        return v.value
+
+    def lastTimings(self):
+        a = np.zeros(4)
+        _lib.call("mfs_last_timings", self._h, ptr(a, _lib.c_double))
+        return dict(inject_ms=a[0], levels_ms=a[2], total_ms=a[3])
+
+    # -- factorization --------------------------------------------------------------------------
+    def factorizeSymbolic(self, H, pinnedVars=()):
+        pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
+        _lib.call("mfs_factorize_symbolic", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
+                  H.n, 3, ptr(pins, _lib.c_i64), len(pins))
+        self._fixed = [int(p) for p in pins]
+        self._n_full = 3 * H.n
+        self._n_red = 3 * H.n - len(self._fixed)
+        self._Hpattern = (H.n, H.nz)
+
+    def _check_pattern(self, H):
+        if self._Hpattern is not None and self._Hpattern != (H.n, H.nz):
+            raise ValueError("matrix pattern differs from the symbolic factorization")
+        if H.isSparsityOnly():
+            raise ValueError("cannot factorize a sparsity-only matrix")
+
+    def factorizeNumeric(self, H, isInTryCatch=False):
+        self.assertFactorization(FactorizationType.Symbolic)
+        self._check_pattern(H)
+        _lib.call("mfs_factorize_numeric", self._h, _dptr(H.Ax), 0.0, None, _stream())
+
+    def factorizeNumericWithShift(self, A, sigma, B=None, isInTryCatch=False):
+        self.assertFactorization(FactorizationType.Symbolic)
+        self._check_pattern(A)
+        Bp = None
+        if B is not None:
+            if B.n != A.n or B.nz != A.nz or not np.array_equal(B.Ap, A.Ap) or not np.array_equal(B.Ai, A.Ai):
+                raise ValueError("Shift matrix must have the same sparsity and value layout as A")
+            Bp = _dptr(B.Ax)
+        _lib.call("mfs_factorize_numeric", self._h, _dptr(A.Ax), float(sigma), Bp, _stream())
+
+    def factorize(self, H, fixedVars=()):
+        self.factorizeSymbolic(H, fixedVars)
+        self.factorizeNumeric(H)
+
+    def clearFactors(self):
+        _lib.call("mfs_clear_factors", self._h)
+
+    def stashFactorization(self):
+        _lib.call("mfs_stash_factorization", self._h)
+
+    def hasStashedFactorization(self):
+        v = _lib.c_i32()
+        _lib.call("mfs_has_stashed_factorization", self._h, v)
+        return bool(v.value)
+
+    def swapStashedFactorization(self):
+        _lib.call("mfs_swap_stashed_factorization", self._h)
+
+    def clearStashedFactorization(self):
+        _lib.call("mfs_clear_stashed_factorization", self._h)
+
+    # -- solves ---------------------------------------------------------------------------------
+    def _colmajor(self, B, rows):
+        if B.dim() == 1:
+            if B.numel() != rows:
+                raise RuntimeError("CholeskyFactorizerBase::solve: incorrect b size")
+            return B.contiguous(), 1
+        if B.shape[0] != rows:
+            raise RuntimeError("Incorrect RHS size")
+        if B.shape[1] < 1:
+            raise RuntimeError("Must specify at least one rhs.")
+        return B.t().contiguous(), B.shape[1]
+
+    def solve(self, b, sys=CholeskySys.A):
+        if sys != CholeskySys.A:
+            raise RuntimeError("Alternative CholeskySys not yet wrapped")
+        self.assertFactorization(FactorizationType.Numeric)
+        bc, nrhs = self._colmajor(b, self.m())
+        x = torch.empty_like(bc)
+        _lib.call("mfs_solve", self._h, _dptr(bc), _dptr(x), nrhs, _stream())
+        return x if b.dim() == 1 else x.t()
+
+    def solveMultiRHS(self, B):
+        self.assertFactorization(FactorizationType.Numeric)
+        if B.dim() != 2:
+            raise RuntimeError("Incorrect RHS size")
+        return self.solve(B)
+
+    def solveRawReduced(self, b, x, sys=CholeskySys.A, alreadyPermuted=False):
+        if sys != CholeskySys.A:
+            raise RuntimeError("Alternative CholeskySys not yet wrapped")
+        self.assertFactorization(FactorizationType.Numeric)
+        nrhs = 1 if b.dim() == 1 else b.shape[0]
+        _lib.call("mfs_solve_raw_reduced", self._h, _dptr(b), _dptr(x), nrhs, int(bool(alreadyPermuted)),
+                  _stream())
+
+    def solveRawReducedInPlace(self, bx, sys=CholeskySys.A, alreadyPermuted=False):
+        self.solveRawReduced(bx, bx, sys, alreadyPermuted)
+
+    def removeFixedEntries(self, x, permute=False):
+        self.assertFactorization(FactorizationType.Symbolic)
+        if x.numel() != self.n():
+            raise RuntimeError("Invalid x size")
+        xr = torch.empty(self.n_reduced(), dtype=torch.float64, device=x.device)
+        _lib.call("mfs_remove_fixed_entries", self._h, _dptr(x.contiguous()), _dptr(xr), 1,
+                  int(bool(permute)), _stream())
+        return xr
+
+    def extractFullSolution(self, xr, permute=False):
+        self.assertFactorization(FactorizationType.Symbolic)
+        if xr.numel() != self.n_reduced():
+            raise RuntimeError("Invalid xReduced size")
+        x = torch.empty(self.n(), dtype=torch.float64, device=xr.device)
+        _lib.call("mfs_extract_full_solution", self._h, _dptr(xr.contiguous()), _dptr(x), 1,
+                  int(bool(permute)), _stream())
+        return x
+
+    # -- extras: SpMV, refinement, factor export ------------------------------------------------
+    def apply(self, H, x):
+        y = torch.empty_like(x)
+        _lib.call("mfs_spmv", self._h, _dptr(H.Ax), _dptr(x.contiguous()), _dptr(y), _stream())
+        return y
+
+    def solveRefined(self, H, b, tol=1e-10, maxIters=10):
+        x = torch.empty_like(b)
+        it, rr = _lib.c_i32(), _lib.c_double()
+        _lib.call("mfs_solve_refined", self._h, _dptr(H.Ax), _dptr(b.contiguous()), _dptr(x), float(tol),
+                  int(maxIters), it, rr, _stream())
+        return x, it.value, rr.value
+
+    def exportFactorCSC(self):
+        n = self.n_reduced()
+        nnz = self.getFactorNNZ()
+        Lp = np.empty(n + 1, np.int64)
+        Li = np.empty(nnz, np.int64)
+        Lx = np.empty(nnz, np.float64)
+        _lib.call("mfs_factor_export_csc", self._h, ptr(Lp, _lib.c_i64), ptr(Li, _lib.c_i64),
+                  ptr(Lx, _lib.c_double))
+        return Lp, Li, Lx
+
+
+def make_cholesky_factorizer(provider=CholeskyProvider.GpuSupernodal, singlePrecision=False):
+    """make_cholesky_factorizer.cc:14-87 with the GPU provider; other providers are not built."""
+    if provider == CholeskyProvider.CatamariLegacy and singlePrecision:
+        raise ValueError("Single precision is unavailable for legacy Catamari")
+    if provider != CholeskyProvider.GpuSupernodal:
+        raise RuntimeError(f"provider {provider.name} is not available in this build")
+    return GpuSupernodalFactorizer(singlePrecision)
+
+
+__all__ = ["CholeskyProvider", "CholeskySys", "FactorizationType", "GpuSupernodalFactorizer",
+           "make_cholesky_factorizer", "MFSError", "NotPositiveDefiniteError"]

@@ -1,0 +1,162 @@
+"""Host mirror of the reference's block-CSC Hessian and assembler interfaces.
+
+BlockCSCHessian  <- BlockCSCHessian<OptimizationVarStructure<3>, ContiguousBlocks=true>
+                    (src/lib/MeshFEMSparse/BlockCSCHessian.hh:560-571, 798-875): upper block
+                    pattern Ap/Ai (int64, host) and the value array Ax (9 doubles per block,
+                    column-major 3x3, diagonal block last per column, upper triangle meaningful),
+                    held in device memory.
+SystemAssembler  <- SystemAssembler<3> (src/lib/MeshFEMSparse/SystemAssembler.hh):
+                    blockSparsityPattern (:251-465) and assembleHessian (:559-565), which
+                    accumulates into H.Ax and allocates it when H is sparsity-only.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MFS_HE_FULL12, MFS_HE_UPPER78, P_i32, P_i64, c_i64, c_vp, ptr
+
+
+def _stream():
+    return c_vp(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a CUDA (HIP) torch tensor")
+    if t.dtype != torch.float64:
+        raise TypeError("expected a float64 tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return c_vp(t.data_ptr())
+
+
+class BlockCSCHessian:
+    """Uniform 3x3-block upper-triangle CSC matrix with contiguous blocks."""
+
+    blockSize = 3
+
+    def __init__(self, Ap, Ai, device="cuda"):
+        self.Ap = np.ascontiguousarray(Ap, dtype=np.int64)
+        self.Ai = np.ascontiguousarray(Ai, dtype=np.int64)
+        self.n = len(self.Ap) - 1
+        self.m = self.n
+        self.nz = int(self.Ap[-1])
+        self.Ax = None
+        self.device = device
+
+    # -- BlockCSCHessianBase interface --------------------------------------------------------
+    def numScalarCols(self):
+        return 3 * self.n
+
+    def scalarNNZ(self):
+        return 9 * self.nz
+
+    def isSparsityOnly(self):
+        return self.Ax is None
+
+    def maxBlockSize(self):
+        return 3
+
+    def uniformBlockSize(self):
+        return True
+
+    def setZero(self):
+        """BlockCSCHessian.hh:638-649: allocate Ax on first use, otherwise zero it."""
+        if self.Ax is None or self.Ax.numel() != self.scalarNNZ():
+            self.Ax = torch.zeros(self.scalarNNZ(), dtype=torch.float64, device=self.device)
+        else:
+            self.Ax.zero_()
+
+    def data(self):
+        return self.Ax
+
+    def diagBlockLoc(self, j):
+        return 9 * (int(self.Ap[j + 1]) - 1)
+
+    def toDense(self):
+        """Dense symmetric scalar matrix (host, float64); for small tests."""
+        Ax = self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor) else np.asarray(self.Ax)
+        A = np.zeros((3 * self.n, 3 * self.n))
+        for j in range(self.n):
+            for k in range(self.Ap[j], self.Ap[j + 1]):
+                i = self.Ai[k]
+                B = Ax[9 * k:9 * k + 9].reshape(3, 3).T    # column-major block
+                if i == j:
+                    U = np.triu(B)
+                    A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = U + np.triu(U, 1).T
+                else:
+                    A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = B
+                    A[3 * j:3 * j + 3, 3 * i:3 * i + 3] = B.T
+        return A
+
+
+class SystemAssembler:
+    """SystemAssembler<3> over `numBlockVars` 3-vector variables."""
+
+    def __init__(self, numBlockVars):
+        self.nb = int(numBlockVars)
+        self._asm = {}
+
+    def numBlockVars(self):
+        return self.nb
+
+    def numVars(self):
+        return 3 * self.nb
+
+    def blockSparsityPattern(self, elem_nodes):
+        en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        ne, npe = en.shape
+        lib = _lib.load()
+        h = c_vp()
+        _lib.check(lib.mfs_pattern_build(ptr(en, _lib.c_i32), ne, npe, self.nb, h))
+        try:
+            nb, nz = c_i64(), c_i64()
+            Ap, Ai = P_i64(), P_i64()
+            _lib.check(lib.mfs_pattern_get(h, nb, nz, Ap, Ai))
+            Apn = np.ctypeslib.as_array(Ap, shape=(self.nb + 1,)).copy()
+            Ain = np.ctypeslib.as_array(Ai, shape=(max(nz.value, 1),))[:nz.value].copy()
+        finally:
+            lib.mfs_pattern_destroy(h)
+        return BlockCSCHessian(Apn, Ain)
+
+    def _assembler(self, H, en):
+        key = (id(H), H.nz, en.ctypes.data, en.shape)
+        a = self._asm.get(key)
+        if a is None:
+            a = DeviceAssembler(H, en)
+            self._asm = {key: a}
+        return a
+
+    def assembleHessian(self, H, elem_nodes, He, layout=MFS_HE_FULL12):
+        """Accumulate element Hessians He (device, ne x 144 or ne x 78 float64) into H.Ax."""
+        if H is None:
+            raise RuntimeError("Attempted to assemble into a BorderedSparseHessian with no block sparsity pattern.")
+        if H.isSparsityOnly():
+            H.setZero()
+        en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        self._assembler(H, en).assemble(He, H.Ax, layout)
+
+
+class DeviceAssembler:
+    """Slot map of (pattern, connectivity), resident on the device."""
+
+    def __init__(self, H, elem_nodes):
+        en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        self.ne, npe = en.shape
+        self._h = c_vp()
+        _lib.call("mfs_assembler_create", ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64), H.n,
+                  ptr(en, _lib.c_i32), self.ne, npe, self._h)
+
+    def assemble(self, He, Ax, layout=MFS_HE_FULL12):
+        per = 144 if layout == MFS_HE_FULL12 else 78
+        if He.numel() != per * self.ne:
+            raise ValueError(f"He must hold {per} doubles per element")
+        _lib.call("mfs_assemble_hessian", self._h, _dptr(He), layout, _dptr(Ax), _stream())
+
+    def zero(self, Ax):
+        _lib.call("mfs_assembler_zero", self._h, _dptr(Ax), _stream())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.mfs_assembler_destroy(self._h)
+            self._h = None

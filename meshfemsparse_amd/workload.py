@@ -1,0 +1,49 @@
+"""Synthetic Newton-step workloads: the unit-cube Kuhn-tet mesh with a seeded perturbation and
+compressible neo-Hookean element Hessians / gradient (computed by the library's host code)."""
+import numpy as np
+
+from . import _lib
+from ._lib import ptr
+
+
+def cube_mesh(N):
+    """(tets[6N^3, 4] int32, X_rest[(N+1)^3, 3] float64); node id = x + (N+1)(y + (N+1)z)."""
+    tets = np.empty((6 * N ** 3, 4), np.int32)
+    X = np.empty(((N + 1) ** 3, 3), np.float64)
+    _lib.call("mfs_cube_mesh", N, ptr(tets, _lib.c_i32), ptr(X, _lib.c_double))
+    return tets, X
+
+
+def perturb(N, X_rest, seed, sigma_rel=1e-3):
+    X = np.array(X_rest, dtype=np.float64, copy=True)
+    _lib.call("mfs_perturb_nodes", N, int(seed) & 0xFFFFFFFFFFFFFFFF, float(sigma_rel), ptr(X, _lib.c_double))
+    return X
+
+
+def lame(E, nu):
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def neohookean(X_rest, X, tets, E=1e5, nu=0.3, E_per_elem=None, nu_per_elem=None, hessians=True,
+               gradient=True, energy=False, threads=0):
+    """Per-element 12x12 Hessians (ne x 144, column-major), assembled gradient, total energy."""
+    ne = len(tets)
+    Xr = np.ascontiguousarray(X_rest, np.float64)
+    Xc = np.ascontiguousarray(X, np.float64)
+    tt = np.ascontiguousarray(tets, np.int32)
+    He = np.empty((ne, 144), np.float64) if hessians else None
+    g = np.zeros(Xc.size, np.float64) if gradient else None
+    en = np.zeros(1) if energy else None
+    Ee = None if E_per_elem is None else np.ascontiguousarray(E_per_elem, np.float64)
+    nue = None if nu_per_elem is None else np.ascontiguousarray(nu_per_elem, np.float64)
+    P = lambda a: None if a is None else ptr(a, _lib.c_double)
+    _lib.call("mfs_neohookean_host", ptr(Xr, _lib.c_double), ptr(Xc, _lib.c_double), ptr(tt, _lib.c_i32), ne,
+              P(Ee), P(nue), float(E), float(nu), P(He), P(g), P(en), int(threads))
+    return He, g, (en[0] if energy else None)
+
+
+def z0_pins(N):
+    """Scalar variables of all z = 0 nodes (whole 3-blocks)."""
+    M = N + 1
+    nodes = np.arange(M * M, dtype=np.int64)
+    return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
