@@ -152,6 +152,9 @@ int mfs_sizes(mfs_factorizer f, int64_t* n_full, int64_t* n_reduced);
 int mfs_factor_nnz(mfs_factorizer f, int64_t* nnz);             /* scalar nnz(L), incl. diagonal */
 int mfs_flop_estimate(mfs_factorizer f, double* flops);         /* POTRF+TRSM+SYRK, per supernode */
 int mfs_get_inverse_permutation(mfs_factorizer f, int64_t* iperm_scalar); /* new -> old reduced */
+/* out3 = {supernodal flops (= mfs_flop_estimate), the diagonal-tile POTRF share of it,
+ *         sum over scalar columns of c_j^2 for the exact (unrelaxed) column counts}          */
+int mfs_flop_breakdown(mfs_factorizer f, double* out3);
 int mfs_clear_factors(mfs_factorizer f);
 int mfs_stash_factorization(mfs_factorizer f);
 int mfs_has_stashed_factorization(mfs_factorizer f, int32_t* has);
@@ -172,6 +175,13 @@ int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
                      const int64_t* pinned_scalar_vars, int64_t num_pins, int64_t* iperm_blocks,
                      int64_t* stats, double* flops);
 
+/* Host replay of the device factorization schedule (same analysis, same task lists, plain CPU
+ * loops): factor of P(A + sigma I)P^T as CSC like mfs_factor_export_csc (sizes from
+ * mfs_analyze_host stats[3]); failed_col = first non-positive pivot column or -1.  Testing.  */
+int mfs_debug_emulate_factor(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
+                             const int64_t* pinned_scalar_vars, int64_t num_pins, const double* Ax,
+                             double sigma, int64_t* Lp, int64_t* Li, double* Lx, int64_t* failed_col);
+
 /* Copy the numeric factor L (lower, factor order, incl. diagonal; explicit zeros from supernode
  * relaxation included) to host CSC arrays: Lp[n_reduced+1], Li[nnz], Lx[nnz].  For testing.  */
 int mfs_factor_export_csc(mfs_factorizer f, int64_t* Lp, int64_t* Li, double* Lx);
@@ -179,6 +189,14 @@ int mfs_factor_export_csc(mfs_factorizer f, int64_t* Lp, int64_t* Li, double* Lx
 /* Per-stage device time of the most recent numeric factorization / solve, milliseconds, from
  * HIP events on the caller's stream: [0] injection, [1] extend-add, [2] dense panels, [3] total.*/
 int mfs_last_timings(mfs_factorizer f, double* ms4);
+
+/* Per-kernel-class device time, accumulated from HIP event pairs placed around every launch while
+ * enabled (numeric factorization and mfs_solve_refined; the events cost a few microseconds each).
+ * Classes (16): fill L, inject, zero U, extend-add, POTRF tiles, TRSM GEMM tiles, in-panel update
+ * GEMM tiles, Schur-update GEMM tiles, forward gather, forward diagonal, forward GEMV, backward
+ * GEMV^T, backward diagonal, SpMV, permutation maps, assembly.  ms[16] and counts[16].        */
+int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
+int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Workload generator (the reference has no energy code; the caller supplies He).             */

@@ -41,6 +41,7 @@ _SIGS = {
     "mfs_sizes": [c_vp, P_i64, P_i64],
     "mfs_factor_nnz": [c_vp, P_i64],
     "mfs_flop_estimate": [c_vp, P_dbl],
+    "mfs_flop_breakdown": [c_vp, P_dbl],
     "mfs_get_inverse_permutation": [c_vp, P_i64],
     "mfs_clear_factors": [c_vp],
     "mfs_stash_factorization": [c_vp],
@@ -50,7 +51,10 @@ _SIGS = {
     "mfs_supernode_stats": [c_vp, P_i64, P_i64, P_i64, P_i64],
     "mfs_factor_export_csc": [c_vp, P_i64, P_i64, P_dbl],
     "mfs_analyze_host": [P_i64, P_i64, c_i64, P_i64, c_i64, P_i64, P_i64, P_dbl],
+    "mfs_debug_emulate_factor": [P_i64, P_i64, c_i64, P_i64, c_i64, P_dbl, c_dbl, P_i64, P_i64, P_dbl, P_i64],
     "mfs_last_timings": [c_vp, P_dbl],
+    "mfs_kernel_profiling": [c_vp, c_i32, c_i32],
+    "mfs_kernel_profile": [c_vp, P_dbl, P_i64],
     "mfs_cube_mesh": [c_i32, P_i32, P_dbl],
     "mfs_perturb_nodes": [c_i32, ctypes.c_uint64, c_dbl, P_dbl],
     "mfs_neohookean_host": [P_dbl, P_dbl, P_i32, c_i64, P_dbl, P_dbl, c_dbl, c_dbl, P_dbl, P_dbl,

@@ -227,6 +227,17 @@ int mfs_flop_estimate(mfs_factorizer f, double* flops) {
     MFS_CATCH
 }
 
+int mfs_flop_breakdown(mfs_factorizer f, double* out3) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(out3);
+    f->dev->require(1);
+    const auto& S = *f->dev->sym;
+    out3[0] = S.flops;
+    out3[1] = S.flops_potrf;
+    out3[2] = S.flops_unrelaxed;
+    MFS_CATCH
+}
+
 int mfs_get_inverse_permutation(mfs_factorizer f, int64_t* ip) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(ip);
@@ -301,6 +312,21 @@ int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int
     MFS_CATCH
 }
 
+int mfs_debug_emulate_factor(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
+                             int64_t npins, const double* Ax, double sigma, int64_t* Lp, int64_t* Li,
+                             double* Lx, int64_t* failed_col) {
+    MFS_TRY
+    CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(Ax);
+    if (npins > 0) CHECK_PTR(pins);
+    mfs::SymbolicOptions opt;
+    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, opt);
+    std::vector<double> L;
+    const int64_t fc = mfs::emulate_numeric(S, Ax, sigma, L);
+    if (failed_col) *failed_col = fc;
+    if (Lp && Li && Lx) mfs::panels_to_csc(S, L, Lp, Li, Lx);
+    MFS_CATCH
+}
+
 int mfs_factor_export_csc(mfs_factorizer f, int64_t* Lp, int64_t* Li, double* Lx) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(Lp); CHECK_PTR(Li); CHECK_PTR(Lx);
@@ -312,6 +338,25 @@ int mfs_last_timings(mfs_factorizer f, double* ms4) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(ms4);
     for (int i = 0; i < 4; ++i) ms4[i] = f->dev->timings[i];
+    MFS_CATCH
+}
+
+int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->prof.on = enable != 0;
+    if (reset) f->dev->prof.reset();
+    MFS_CATCH
+}
+
+int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts) {
+    MFS_TRY
+    CHECK_PTR(f);
+    auto& P = f->dev->prof;
+    for (int c = 0; c < mfs::KernelProfile::NCLASS; ++c) {
+        if (ms) ms[c] = P.ms[c];
+        if (counts) counts[c] = P.count[c];
+    }
     MFS_CATCH
 }
 

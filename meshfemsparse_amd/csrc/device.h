@@ -34,6 +34,23 @@ private:
     DeviceArray seg_, contrib_, diag_;
 };
 
+// Optional per-kernel-class device timing (HIP event pairs around each launch).
+struct KernelProfile {
+    enum { FILL_L, INJECT, ZERO_U, EXTEND, POTRF, TRSM, UPDATE, SYRK, FGATHER, FDIAG, FGEMV, BGEMV,
+           BDIAG, SPMV, MAPS, ASSEMBLE, NCLASS };
+    bool on = false;
+    double ms[NCLASS] = {};
+    int64_t count[NCLASS] = {};
+    void begin(int cls, void* stream);
+    void end(void* stream);
+    void collect();            // after the stream has been synchronized
+    void reset();
+    ~KernelProfile();
+private:
+    std::vector<void*> pool_;
+    std::vector<std::pair<int, size_t>> marks_;
+};
+
 class FactorizerDev {
 public:
     explicit FactorizerDev(int precision);
@@ -55,6 +72,7 @@ public:
     void clear_stash();
     bool has_stash() const { return stash_valid_; }
 
+    KernelProfile prof;
     int ftype = 0;                 // 0 none, 1 symbolic, 2 numeric
     int64_t num_successful = 0;
     double timings[4] = {0, 0, 0, 0};

@@ -1,0 +1,140 @@
+// Host replay of the device factorization schedule: the same task lists (injection plan,
+// extend-add tasks, diagonal-tile POTRF + inverse, tile GEMMs) executed by plain loops.  Used by
+// CPU tests to check the symbolic schedule independently of the GPU kernels.
+#include <algorithm>
+#include <cmath>
+#include <climits>
+
+#include "mfs_internal.h"
+
+namespace mfs {
+
+int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L) {
+    const int TS = 64;
+    L.assign(S.L_size, 0.0);
+    std::vector<double> U(S.U_size, 0.0), inv((size_t)S.inv_slots * TS * TS, 0.0);
+    int64_t status = INT64_MAX;
+    // injection
+    for (size_t q = 0; q < S.plan_src.size(); ++q) {
+        const int64_t dst = S.plan_dst[q] >> 2;
+        const int mode = (int)(S.plan_dst[q] & 3);
+        const int32_t s = (int32_t)(std::upper_bound(S.panel_off.begin(), S.panel_off.end(), dst) -
+                                    S.panel_off.begin()) - 1;
+        const int64_t ld = 3 * ((int64_t)S.swidth[s] + S.sdeg[s]);
+        const double* a = Ax + 9 * (int64_t)S.plan_src[q];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                if (mode == 2 && r < c) continue;
+                double v = mode == 0 ? a[3 * c + r] : a[3 * r + c];
+                if (mode == 2 && r == c) v += sigma;
+                L[dst + c * ld + r] = v;
+            }
+    }
+    auto gemm = [&](const GemmTask& t) {
+        const double* A = L.data() + t.a_off;
+        const double* B = (t.flags & GT_B_IN_INV) ? inv.data() + t.b_off : L.data() + t.b_off;
+        double* C = (t.flags & GT_C_IN_U) ? U.data() + t.c_off : L.data() + t.c_off;
+        const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
+        std::vector<double> tmp((size_t)t.M * t.N, 0.0);
+        for (int n = 0; n < t.N; ++n)
+            for (int k = 0; k < t.K; ++k) {
+                const double b = B[(int64_t)k * t.ldb + n];
+                for (int m = 0; m < t.M; ++m) tmp[(size_t)n * t.M + m] += A[(int64_t)k * t.lda + m] * b;
+            }
+        for (int n = 0; n < t.N; ++n)
+            for (int m = 0; m < t.M; ++m) {
+                double& c = C[(int64_t)n * t.ldc + m];
+                const double v = alpha * tmp[(size_t)n * t.M + m];
+                c = (t.flags & GT_BETA0) ? v : c + v;
+            }
+    };
+    for (const LevelPlan& P : S.levels) {
+        for (int64_t z = P.zero_u.first; z < P.zero_u.second; ++z)
+            std::fill(U.begin() + S.zero_ranges[z].first,
+                      U.begin() + S.zero_ranges[z].first + S.zero_ranges[z].second, 0.0);
+        for (auto [a, b] : P.extend_passes)
+            for (int64_t q = a; q < b; ++q) {
+                const ExtendTask& t = S.extend_tasks[q];
+                const int32_t* relc = S.rel.data() + t.rel_off;
+                const int kj = t.col_block, tj = relc[kj];
+                const int64_t Dc = 3 * (int64_t)t.dc;
+                for (int ki = kj; ki < t.dc; ++ki) {
+                    const int ti = relc[ki];
+                    double* dst;
+                    int64_t ldd;
+                    if (tj < t.wp) { dst = L.data() + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti; ldd = t.ldp; }
+                    else {
+                        const int64_t Dp = 3 * (int64_t)t.dp;
+                        dst = U.data() + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
+                        ldd = Dp;
+                    }
+                    const double* src = U.data() + t.u_child + 3 * (int64_t)kj * Dc + 3 * (int64_t)ki;
+                    for (int c = 0; c < 3; ++c)
+                        for (int r = (ki == kj ? c : 0); r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+                }
+            }
+        for (size_t st = 0; st < P.potrf_steps.size(); ++st) {
+            for (int64_t q = P.potrf_steps[st].first; q < P.potrf_steps[st].second; ++q) {
+                const PotrfTask& t = S.potrf_tasks[q];
+                double* D = L.data() + t.diag_off;
+                const int kb = t.kb;
+                const int64_t ld = t.ld;
+                for (int j = 0; j < kb; ++j) {
+                    double d = D[j * ld + j];
+                    for (int k = 0; k < j; ++k) d -= D[k * ld + j] * D[k * ld + j];
+                    if (!(d > 0.0)) { status = std::min<int64_t>(status, t.col0 + j); d = 1.0; }
+                    d = std::sqrt(d);
+                    D[j * ld + j] = d;
+                    for (int r = j + 1; r < kb; ++r) {
+                        double s = D[j * ld + r];
+                        for (int k = 0; k < j; ++k) s -= D[k * ld + r] * D[k * ld + j];
+                        D[j * ld + r] = s / d;
+                    }
+                }
+                double* out = inv.data() + (int64_t)t.slot * TS * TS;
+                std::fill(out, out + TS * TS, 0.0);
+                for (int c = 0; c < kb; ++c) {
+                    out[c * TS + c] = 1.0 / D[c * ld + c];
+                    for (int i = c + 1; i < kb; ++i) {
+                        double s = 0;
+                        for (int k = c; k < i; ++k) s += D[k * ld + i] * out[c * TS + k];
+                        out[c * TS + i] = -s / D[i * ld + i];
+                    }
+                }
+            }
+            for (int64_t q = P.trsm_steps[st].first; q < P.trsm_steps[st].second; ++q) gemm(S.gemm_tasks[q]);
+            for (int64_t q = P.update_steps[st].first; q < P.update_steps[st].second; ++q) gemm(S.gemm_tasks[q]);
+        }
+        for (int64_t q = P.syrk.first; q < P.syrk.second; ++q) gemm(S.gemm_tasks[q]);
+    }
+    return status == INT64_MAX ? -1 : status;
+}
+
+} // namespace mfs
+
+namespace mfs {
+
+void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx) {
+    const int64_t n = 3 * S.n_red;
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int32_t sn = 0; sn < S.ns; ++sn) {
+        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn];
+        for (int64_t c = 0; c < W; ++c) cnt[3 * (int64_t)S.sfirst[sn] + c + 1] = W + D - c;
+    }
+    for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+    for (int64_t j = 0; j <= n; ++j) Lp[j] = cnt[j];
+    for (int32_t sn = 0; sn < S.ns; ++sn) {
+        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn], ld = W + D;
+        const int64_t f = 3 * (int64_t)S.sfirst[sn];
+        for (int64_t c = 0; c < W; ++c) {
+            int64_t q = Lp[f + c];
+            for (int64_t r = c; r < W + D; ++r) {
+                Li[q] = r < W ? f + r : 3 * (int64_t)S.rows[S.srows_off[sn] + (r - W) / 3] + (r - W) % 3;
+                Lx[q] = h[S.panel_off[sn] + c * ld + r];
+                ++q;
+            }
+        }
+    }
+}
+
+} // namespace mfs

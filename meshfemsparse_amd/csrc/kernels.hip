@@ -60,6 +60,39 @@ void DeviceArray::upload_raw(const void* src, size_t b) {
     if (b) HIP_CHECK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
 }
 
+void KernelProfile::begin(int cls, void* stream) {
+    if (!on) return;
+    const size_t i = marks_.size() * 2;
+    while (pool_.size() < i + 2) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        pool_.push_back(e);
+    }
+    HIP_CHECK(hipEventRecord((hipEvent_t)pool_[i], (hipStream_t)stream));
+    marks_.push_back({cls, i});
+}
+void KernelProfile::end(void* stream) {
+    if (!on || marks_.empty()) return;
+    HIP_CHECK(hipEventRecord((hipEvent_t)pool_[marks_.back().second + 1], (hipStream_t)stream));
+}
+void KernelProfile::collect() {
+    for (auto [cls, i] : marks_) {
+        float m = 0;
+        HIP_CHECK(hipEventSynchronize((hipEvent_t)pool_[i + 1]));
+        HIP_CHECK(hipEventElapsedTime(&m, (hipEvent_t)pool_[i], (hipEvent_t)pool_[i + 1]));
+        ms[cls] += m;
+        count[cls] += 1;
+    }
+    marks_.clear();
+}
+void KernelProfile::reset() {
+    for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; }
+    marks_.clear();
+}
+KernelProfile::~KernelProfile() {
+    for (void* e : pool_) (void)hipEventDestroy((hipEvent_t)e);
+}
+
 // ------------------------------------------------------------------------------------------
 // Generic small kernels
 // ------------------------------------------------------------------------------------------
@@ -320,8 +353,9 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
 // ------------------------------------------------------------------------------------------
 // Batched tile GEMM, C = alpha A B^T + beta C, 64x64 output tile per workgroup (4 waves of
 // 32x32), FP64 MFMA 16x16x4, K staged through LDS in chunks of 16.
-// MFMA f64 16x16x4 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// D[i][j] in lane j + 16 (i / 4), register i % 4.
+// MFMA f64 16x16x4 operand layout on gfx950 (measured): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k, D[i][j] in lane j + 16 (i % 4), register i / 4 (rows interleaved across the four
+// 16-lane groups; the f32 16x16x4 variant instead uses lane j + 16 (i / 4), register i % 4).
 // ------------------------------------------------------------------------------------------
 constexpr int KC = 16;
 
@@ -363,13 +397,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
     const bool beta0 = (t.flags & GT_BETA0) != 0;
     const int64_t ldc = t.ldc;
-    const int col_l = lane & 15, row_l = 4 * (lane >> 4);
+    const int col_l = lane & 15, row_l = lane >> 4;
     auto store = [&](const d4& acc, int i0, int j0) {
         const int col = j0 + col_l;
         if (col >= t.N) return;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = i0 + row_l + r;
+            const int row = i0 + 4 * r + row_l;
             if (row < t.M) {
                 double* p = C + col * ldc + row;
                 const double v = alpha * acc[r];
@@ -725,13 +759,17 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
     HIP_CHECK(hipEventRecord(ev[0], s));
     double* L = L_.as<double>();
     double* U = U_.as<double>();
+    prof.begin(KernelProfile::FILL_L, s);
     fill(L, S.L_size, 0.0, s);
+    prof.end(s);
     k_set_int<<<1, 1, 0, s>>>(status_.as<int>(), INT_MAX);
     LAUNCH_CHECK();
     const int64_t np = (int64_t)S.plan_src.size();
     if (np) {
+        prof.begin(KernelProfile::INJECT, s);
         k_inject<<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, L);
         LAUNCH_CHECK();
+        prof.end(s);
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
     const int64_t* zr = zr_.as<int64_t>();
@@ -739,42 +777,55 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
         const int64_t nz = P.zero_u.second - P.zero_u.first;
         if (nz > 0) {
             dim3 g(64, (unsigned)nz);
+            prof.begin(KernelProfile::ZERO_U, s);
             k_zero_ranges<<<g, 256, 0, s>>>(U, zr + 2 * P.zero_u.first);
             LAUNCH_CHECK();
+            prof.end(s);
         }
         for (auto [a, b] : P.extend_passes)
             if (b > a) {
+                prof.begin(KernelProfile::EXTEND, s);
                 k_extend<<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
                 LAUNCH_CHECK();
+                prof.end(s);
             }
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];
             if (p1 > p0) {
+                prof.begin(KernelProfile::POTRF, s);
                 k_potrf<<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
                                                            status_.as<int>());
                 LAUNCH_CHECK();
+                prof.end(s);
             }
             auto [t0, t1] = P.trsm_steps[t];
             if (t1 > t0) {
+                prof.begin(KernelProfile::TRSM, s);
                 k_gemm<<<(unsigned)(t1 - t0), 256, 0, s>>>(gemm_.as<GemmTask>() + t0, L, U, inv_.as<double>());
                 LAUNCH_CHECK();
+                prof.end(s);
             }
             auto [u0, u1] = P.update_steps[t];
             if (u1 > u0) {
+                prof.begin(KernelProfile::UPDATE, s);
                 k_gemm<<<(unsigned)(u1 - u0), 256, 0, s>>>(gemm_.as<GemmTask>() + u0, L, U, inv_.as<double>());
                 LAUNCH_CHECK();
+                prof.end(s);
             }
         }
         if (P.syrk.second > P.syrk.first) {
+            prof.begin(KernelProfile::SYRK, s);
             k_gemm<<<(unsigned)(P.syrk.second - P.syrk.first), 256, 0, s>>>(gemm_.as<GemmTask>() + P.syrk.first,
                                                                            L, U, inv_.as<double>());
             LAUNCH_CHECK();
+            prof.end(s);
         }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
     int st = 0;
     HIP_CHECK(hipMemcpyAsync(&st, status_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    prof.collect();
     float m01 = 0, m12 = 0;
     HIP_CHECK(hipEventElapsedTime(&m01, ev[0], ev[1]));
     HIP_CHECK(hipEventElapsedTime(&m12, ev[1], ev[2]));
@@ -809,22 +860,28 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
     for (const LevelPlan& P : S.levels) {
         if (P.fgather.second > P.fgather.first) {
             dim3 g((unsigned)(P.fgather.second - P.fgather.first), (unsigned)nrhs);
+            prof.begin(KernelProfile::FGATHER, s);
             k_fgather<<<g, 256, 0, s>>>(fg_.as<SolveGatherTask>() + P.fgather.first, children_.as<int32_t>(),
                                         sdeg, reloff, rel_.as<int32_t>(), usoff_.as<int64_t>(), xp, ldx, u, ldu);
             LAUNCH_CHECK();
+            prof.end(s);
         }
         for (size_t t = 0; t < P.fdiag_steps.size(); ++t) {
             auto [d0, d1] = P.fdiag_steps[t];
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
+                prof.begin(KernelProfile::FDIAG, s);
                 k_fdiag<<<g, 64, 0, s>>>(fd_.as<SolveDiagTask>() + d0, L, xp, ldx);
                 LAUNCH_CHECK();
+                prof.end(s);
             }
             auto [v0, v1] = P.fgemv_steps[t];
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+                prof.begin(KernelProfile::FGEMV, s);
                 k_fgemv<<<g, 64, 0, s>>>(fv_.as<SolveGemvTask>() + v0, L, xp, ldx, u, ldu);
                 LAUNCH_CHECK();
+                prof.end(s);
             }
         }
     }
@@ -834,14 +891,18 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             auto [v0, v1] = P.bgemv_steps[t];
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+                prof.begin(KernelProfile::BGEMV, s);
                 k_bgemv<<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), xp, ldx, part, ldp);
                 LAUNCH_CHECK();
+                prof.end(s);
             }
             auto [d0, d1] = P.bdiag_steps[t];
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
+                prof.begin(KernelProfile::BDIAG, s);
                 k_bdiag<<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, L, part, ldp, xp, ldx);
                 LAUNCH_CHECK();
+                prof.end(s);
             }
         }
     }
@@ -911,9 +972,11 @@ void FactorizerDev::extract_full(const double* xr, double* xf, int nrhs, bool pe
 void FactorizerDev::spmv(const double* Ax, const double* x, double* y, void* stream) {
     require(1);
     const int64_t nb = sym->n_full_blocks;
+    prof.begin(KernelProfile::SPMV, stream);
     k_spmv<<<nblk(nb, 256), 256, 0, S_(stream)>>>(spmv_ptr_.as<int64_t>(), spmv_ent_.as<int64_t>(),
                                                  spmv_col_.as<int32_t>(), Ax, x, y, nb);
     LAUNCH_CHECK();
+    prof.end(stream);
 }
 
 void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, double tol,
@@ -955,6 +1018,10 @@ void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, 
     }
     if (iters) *iters = it;
     if (relres) *relres = rel;
+    if (prof.on) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        prof.collect();
+    }
 }
 
 void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {

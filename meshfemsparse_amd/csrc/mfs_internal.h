@@ -161,6 +161,7 @@ struct Symbolic {
     int64_t factor_nnz = 0;                    // scalar lower nnz incl. diag of the panels
     double flops = 0.0;                        // POTRF+TRSM+SYRK over supernodes (scalar)
     double flops_unrelaxed = 0.0;              // sum_j c_j^2 of the unrelaxed column counts
+    double flops_potrf = 0.0;                  // diagonal-tile POTRF share of `flops` (sum kb^3/3)
     int32_t num_levels = 0;
     int64_t max_width = 0, max_degree = 0;
     // injection plan: per reduced stored block (in reduced-CSC order)
@@ -200,6 +201,11 @@ struct SymbolicOptions {
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
                  const int64_t* pins, int64_t npins, const SymbolicOptions& opt);
+
+// Host replay of the device schedule (returns the first failing pivot column or -1) and the
+// supernodal-panel -> CSC conversion.
+int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L);
+void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx);
 
 // Workload (host)
 void cube_mesh(int N, int32_t* tets, double* X);

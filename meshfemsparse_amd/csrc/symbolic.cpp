@@ -496,6 +496,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 pt.slot = slot;
                 pt.col0 = 3 * S.sfirst[s] + k0;
                 S.potrf_tasks.push_back(pt);
+                S.flops_potrf += (double)kb * kb * kb / 3.0;
                 active.push_back({s, slot});
                 ++slot;
             }

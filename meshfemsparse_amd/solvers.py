@@ -131,6 +131,11 @@ class GpuSupernodalFactorizer:
         _lib.call("mfs_flop_estimate", self._h, v)
         return v.value
 
+    def flopBreakdown(self):
+        a = np.zeros(3)
+        _lib.call("mfs_flop_breakdown", self._h, ptr(a, _lib.c_double))
+        return dict(total=a[0], potrf=a[1], unrelaxed=a[2])
+
     def getInversePermutation(self):
         out = np.empty(self.n_reduced(), dtype=np.int64)
         _lib.call("mfs_get_inverse_permutation", self._h, ptr(out, _lib.c_i64))

@@ -128,3 +128,48 @@ def test_partial_pins_are_rejected():
     H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
     with pytest.raises(_lib.UnsupportedError):
         analyze(H, np.array([1]))
+
+
+def emulate(H, pins, Ax, sigma=0.0):
+    iperm, st, fl = analyze(H, pins)
+    nred, nnz = st[0], st[3]
+    Lp = np.zeros(3 * nred + 1, np.int64)
+    Li = np.zeros(nnz, np.int64)
+    Lx = np.zeros(nnz)
+    fc = np.zeros(1, np.int64)
+    pins = np.ascontiguousarray(pins, np.int64)
+    _lib.call("mfs_debug_emulate_factor", _lib.ptr(H.Ap, _lib.c_i64), _lib.ptr(H.Ai, _lib.c_i64), H.n,
+              _lib.ptr(pins, _lib.c_i64), len(pins), _lib.ptr(Ax, _lib.c_double), sigma, _lib.ptr(Lp, _lib.c_i64),
+              _lib.ptr(Li, _lib.c_i64), _lib.ptr(Lx, _lib.c_double), _lib.ptr(fc, _lib.c_i64))
+    return iperm, (Lp, Li, Lx), fc[0]
+
+
+@pytest.mark.parametrize("N,sigma", [(3, 0.0), (5, 0.0), (7, 2.5)])
+def test_device_schedule_replay_matches_oracle(N, sigma):
+    """The device task lists (injection plan, extend-add, tile POTRF/TRSM/updates, Schur
+    updates), replayed on the host, reproduce the oracle's unique Cholesky factor."""
+    tets, Xr = wl.cube_mesh(N)
+    X = wl.perturb(N, Xr, 0)
+    He, _, _ = wl.neohookean(Xr, X, tets, gradient=False)
+    nb = len(Xr)
+    H = mfs.SystemAssembler(nb).blockSparsityPattern(tets)
+    Ax = orc.assemble(H.Ap, H.Ai, nb, tets, He)
+    pins = wl.z0_pins(N)
+    iperm, Le, fc = emulate(H, pins, Ax, sigma)
+    assert fc == -1
+    nred = len(iperm)
+    pos = np.empty(nred, np.int32)
+    pos[iperm] = np.arange(nred)
+    red = -np.ones(nb, np.int64)
+    red[(N + 1) ** 2:] = np.arange(nred)
+    Lo = orc.dense_L(*orc.chol(*orc.permuted_lower(H.Ap, H.Ai, nb, Ax, red, pos, sigma)))
+    assert np.abs(orc.dense_L(*Le) - Lo).max() <= 1e-12 * np.abs(Lo).max()
+
+
+def test_device_schedule_replay_reports_first_bad_pivot():
+    tets, Xr = wl.cube_mesh(3)
+    He, _, _ = wl.neohookean(Xr, wl.perturb(3, Xr, 0), tets, gradient=False)
+    H = mfs.SystemAssembler(len(Xr)).blockSparsityPattern(tets)
+    Ax = -orc.assemble(H.Ap, H.Ai, len(Xr), tets, He)
+    _, _, fc = emulate(H, wl.z0_pins(3), Ax)
+    assert fc == 0
