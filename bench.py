@@ -81,7 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="hexes per cube side (64 = 811,200 free DOFs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-n", type=int, default=20, help="cube side of the CPU baseline sample")
+    ap.add_argument("--cpu-n", type=int, default=24, help="cube side of the CPU baseline sample")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -94,6 +96,13 @@ def main():
     from meshfemsparse_amd import _lib, workload as wl
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib * 131072, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB done")
+        del src, dst
     N = args.n
     nseeds = args.steps + args.warmup
     t0 = time.perf_counter()
