@@ -175,12 +175,15 @@ int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
                      const int64_t* pinned_scalar_vars, int64_t num_pins, int64_t* iperm_blocks,
                      int64_t* stats, double* flops);
 
-/* Host replay of the device factorization schedule (same analysis, same task lists, plain CPU
- * loops): factor of P(A + sigma I)P^T as CSC like mfs_factor_export_csc (sizes from
- * mfs_analyze_host stats[3]); failed_col = first non-positive pivot column or -1.  Testing.  */
+/* Host replay of the device factorization and solve schedules (same analysis, same task lists,
+ * plain CPU loops): factor of P(A + sigma I)P^T as CSC like mfs_factor_export_csc (sizes from
+ * mfs_analyze_host stats[3]); failed_col = first non-positive pivot column or -1; if x_perm is
+ * non-NULL it holds a factor-order right-hand side (3 n_reduced) and is overwritten by the
+ * solution of the replayed forward/backward sweeps.  Testing.                               */
 int mfs_debug_emulate_factor(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
                              const int64_t* pinned_scalar_vars, int64_t num_pins, const double* Ax,
-                             double sigma, int64_t* Lp, int64_t* Li, double* Lx, int64_t* failed_col);
+                             double sigma, int64_t* Lp, int64_t* Li, double* Lx, int64_t* failed_col,
+                             double* x_perm);
 
 /* Copy the numeric factor L (lower, factor order, incl. diagonal; explicit zeros from supernode
  * relaxation included) to host CSC arrays: Lp[n_reduced+1], Li[nnz], Lx[nnz].  For testing.  */

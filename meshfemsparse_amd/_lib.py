@@ -51,7 +51,8 @@ _SIGS = {
     "mfs_supernode_stats": [c_vp, P_i64, P_i64, P_i64, P_i64],
     "mfs_factor_export_csc": [c_vp, P_i64, P_i64, P_dbl],
     "mfs_analyze_host": [P_i64, P_i64, c_i64, P_i64, c_i64, P_i64, P_i64, P_dbl],
-    "mfs_debug_emulate_factor": [P_i64, P_i64, c_i64, P_i64, c_i64, P_dbl, c_dbl, P_i64, P_i64, P_dbl, P_i64],
+    "mfs_debug_emulate_factor": [P_i64, P_i64, c_i64, P_i64, c_i64, P_dbl, c_dbl, P_i64, P_i64, P_dbl, P_i64,
+                                 P_dbl],
     "mfs_last_timings": [c_vp, P_dbl],
     "mfs_kernel_profiling": [c_vp, c_i32, c_i32],
     "mfs_kernel_profile": [c_vp, P_dbl, P_i64],

@@ -41,7 +41,7 @@ struct KernelProfile {
     bool on = false;
     double ms[NCLASS] = {};
     int64_t count[NCLASS] = {};
-    void begin(int cls, void* stream);
+    void begin(int cls, void* stream, double work = 0.0, int64_t items = 0);
     void end(void* stream);
     void collect();            // after the stream has been synchronized
     void reset();
@@ -49,6 +49,7 @@ struct KernelProfile {
 private:
     std::vector<void*> pool_;
     std::vector<std::pair<int, size_t>> marks_;
+    std::vector<std::pair<double, int64_t>> work_;   // per mark: flops (or bytes), tasks
 };
 
 class FactorizerDev {
@@ -87,10 +88,10 @@ private:
 
     DeviceArray L_, U_, inv_, status_;
     DeviceArray planent_, rel_, ext_, potrf_, gemm_, zr_;
-    DeviceArray fg_, fd_, fv_, bd_, bv_, children_, usoff_, rows_, sdeg_, reloff_;
+    DeviceArray fg_, fs_, bd_, bv_, children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;
-    DeviceArray usolve_, bpart_, scratch_b_, scratch_c_;
-    DeviceArray Lstash_;
+    DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
+    DeviceArray Lstash_, invstash_;
     bool stash_valid_ = false;
     int stash_type_ = 0;
     int64_t stash_succ_ = 0;

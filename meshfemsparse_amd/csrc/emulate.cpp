@@ -9,10 +9,12 @@
 
 namespace mfs {
 
-int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L) {
+int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L,
+                        std::vector<double>& inv) {
     const int TS = 64;
     L.assign(S.L_size, 0.0);
-    std::vector<double> U(S.U_size, 0.0), inv((size_t)S.inv_slots * TS * TS, 0.0);
+    inv.assign((size_t)S.inv_slots * TS * TS, 0.0);
+    std::vector<double> U(S.U_size, 0.0);
     int64_t status = INT64_MAX;
     // injection
     for (size_t q = 0; q < S.plan_src.size(); ++q) {
@@ -102,10 +104,10 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                     }
                 }
             }
-            for (int64_t q = P.trsm_steps[st].first; q < P.trsm_steps[st].second; ++q) gemm(S.gemm_tasks[q]);
-            for (int64_t q = P.update_steps[st].first; q < P.update_steps[st].second; ++q) gemm(S.gemm_tasks[q]);
+            for (int64_t q = P.trsm_steps[st].b; q < P.trsm_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
+            for (int64_t q = P.update_steps[st].b; q < P.update_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
         }
-        for (int64_t q = P.syrk.first; q < P.syrk.second; ++q) gemm(S.gemm_tasks[q]);
+        for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) gemm(S.gemm_tasks[q]);
     }
     return status == INT64_MAX ? -1 : status;
 }
@@ -135,6 +137,98 @@ void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp,
             }
         }
     }
+}
+
+} // namespace mfs
+
+namespace mfs {
+
+// Host replay of the device solve schedule on a factor-order vector x (one RHS), using the
+// panels L and the stored tile inverses `inv` produced by emulate_numeric_full.
+void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+                   std::vector<double>& x) {
+    const int TS = 64;
+    std::vector<double> u(std::max<int64_t>(S.u_solve_size, 1), 0.0), Y(x.size(), 0.0);
+    std::vector<double> part((size_t)S.bpart_size * TS, 0.0);
+    for (const LevelPlan& P : S.levels) {
+        for (int64_t q = P.fgather.first; q < P.fgather.second; ++q) {
+            const SolveGatherTask& t = S.fgather_tasks[q];
+            std::fill(u.begin() + t.u_off, u.begin() + t.u_off + t.d, 0.0);
+            for (int32_t k = t.child_begin; k < t.child_end; ++k) {
+                const int32_t c = S.children[k];
+                const int Dc = 3 * S.sdeg[c];
+                for (int i = 0; i < Dc; ++i) {
+                    const int tgt = 3 * S.rel[S.rel_off[c] + i / 3] + i % 3;
+                    const double v = u[S.u_solve_off[c] + i];
+                    if (tgt < t.w) x[t.x_off + tgt] -= v;
+                    else u[t.u_off + tgt - t.w] += v;
+                }
+            }
+        }
+        for (auto [v0, v1] : P.fstep_steps) {
+            // all tasks of a step read x_k before any writes: compute y per task first
+            std::vector<std::vector<double>> ys(v1 - v0);
+            for (int64_t q = v0; q < v1; ++q) {
+                const FStepTask& t = S.fstep_tasks[q];
+                std::vector<double> y(TS, 0.0);
+                for (int i = 0; i < TS; ++i)
+                    for (int j = 0; j < t.kb; ++j) y[i] += inv[t.inv_off + j * TS + i] * x[t.x_blk + j];
+                ys[q - v0] = y;
+            }
+            for (int64_t q = v0; q < v1; ++q) {
+                const FStepTask& t = S.fstep_tasks[q];
+                const std::vector<double>& y = ys[q - v0];
+                if (t.nr == 0) {
+                    for (int i = 0; i < t.kb; ++i) Y[t.x_blk + i] = y[i];
+                    continue;
+                }
+                for (int r = 0; r < t.nr; ++r) {
+                    double d = 0;
+                    for (int c = 0; c < t.kb; ++c) d += L[t.l_off + (int64_t)c * t.ld + r] * y[c];
+                    const int row = t.r0 + r;
+                    if (row < t.w) x[t.x_rows + row] -= d;
+                    else u[t.u_off + row - t.w] += d;
+                }
+            }
+        }
+    }
+    for (int lv = (int)S.levels.size() - 1; lv >= 0; --lv) {
+        const LevelPlan& P = S.levels[lv];
+        for (size_t st = 0; st < P.bdiag_steps.size(); ++st) {
+            for (int64_t q = P.bgemv_steps[st].first; q < P.bgemv_steps[st].second; ++q) {
+                const SolveGemvTask& t = S.bgemv_tasks[q];
+                for (int c = 0; c < t.kb; ++c) {
+                    double acc = 0;
+                    for (int r = 0; r < t.nr; ++r) {
+                        const int row = t.r0 + r;
+                        double xv;
+                        if (row < t.w) xv = Y[t.x_rows + row];
+                        else {
+                            const int qq = row - t.w;
+                            xv = Y[3 * (int64_t)S.rows[t.rows_off + qq / 3] + qq % 3];
+                        }
+                        acc += L[t.l_off + (int64_t)c * t.ld + r] * xv;
+                    }
+                    part[(int64_t)t.part * TS + c] = acc;
+                }
+            }
+            for (int64_t q = P.bdiag_steps[st].first; q < P.bdiag_steps[st].second; ++q) {
+                const SolveDiagTask& t = S.bdiag_tasks[q];
+                std::vector<double> z(TS, 0.0);
+                for (int i = 0; i < t.kb; ++i) {
+                    double v = Y[t.x_off + i];
+                    for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + i];
+                    z[i] = v;
+                }
+                for (int i = 0; i < t.kb; ++i) {
+                    double s = 0;
+                    for (int j = 0; j < t.kb; ++j) s += inv[t.inv_off + i * TS + j] * z[j];
+                    Y[t.x_off + i] = s;
+                }
+            }
+        }
+    }
+    x = Y;
 }
 
 } // namespace mfs

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "device.h"
@@ -60,8 +62,9 @@ void DeviceArray::upload_raw(const void* src, size_t b) {
     if (b) HIP_CHECK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
 }
 
-void KernelProfile::begin(int cls, void* stream) {
+void KernelProfile::begin(int cls, void* stream, double work, int64_t items) {
     if (!on) return;
+    work_.push_back({work, items});
     const size_t i = marks_.size() * 2;
     while (pool_.size() < i + 2) {
         hipEvent_t e;
@@ -76,18 +79,26 @@ void KernelProfile::end(void* stream) {
     HIP_CHECK(hipEventRecord((hipEvent_t)pool_[marks_.back().second + 1], (hipStream_t)stream));
 }
 void KernelProfile::collect() {
-    for (auto [cls, i] : marks_) {
+    // MFS_PROFILE_DUMP=<path>: append one line per launch: class, tasks, work, milliseconds
+    static const char* dump = getenv("MFS_PROFILE_DUMP");
+    FILE* fp = dump ? fopen(dump, "a") : nullptr;
+    for (size_t q = 0; q < marks_.size(); ++q) {
+        auto [cls, i] = marks_[q];
         float m = 0;
         HIP_CHECK(hipEventSynchronize((hipEvent_t)pool_[i + 1]));
         HIP_CHECK(hipEventElapsedTime(&m, (hipEvent_t)pool_[i], (hipEvent_t)pool_[i + 1]));
         ms[cls] += m;
         count[cls] += 1;
+        if (fp) fprintf(fp, "%d %lld %.6e %.6f\n", cls, (long long)work_[q].second, work_[q].first, m);
     }
+    if (fp) fclose(fp);
     marks_.clear();
+    work_.clear();
 }
 void KernelProfile::reset() {
     for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; }
     marks_.clear();
+    work_.clear();
 }
 KernelProfile::~KernelProfile() {
     for (void* e : pool_) (void)hipEventDestroy((hipEvent_t)e);
@@ -291,130 +302,188 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 
 // ------------------------------------------------------------------------------------------
 // Diagonal tile POTRF (kb <= 64) + explicit inverse of the tile's Cholesky factor.
+// One wave: lane r owns row r of the tile (kept in LDS, a[c][r]); right-looking column steps.
+// The inverse is kept for the TRSM of this step and for both solve sweeps.
 // ------------------------------------------------------------------------------------------
-constexpr int TS = 64;   // tile size (scalars)
+constexpr int TS = 64;   // diagonal tile size (scalars)
 
-__global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
-                                               double* __restrict__ L, double* __restrict__ inv,
-                                               int* __restrict__ status) {
+__global__ __launch_bounds__(64) void k_potrf(const PotrfTask* __restrict__ tasks,
+                                              double* __restrict__ L, double* __restrict__ inv,
+                                              int* __restrict__ status) {
     const PotrfTask t = tasks[blockIdx.x];
-    __shared__ double a[TS][TS + 1];    // a[c][r]
-    __shared__ double iv[TS][TS + 1];
-    const int tid = threadIdx.x, kb = t.kb;
+    __shared__ double col[TS];          // broadcast column (and diagonal) of the current step
+    __shared__ double lt[TS][TS + 1];   // final factor, lt[c][i] = L(i, c); reused for the inverse
+    const int r = threadIdx.x, kb = t.kb;
     const int64_t ld = t.ld;
     double* D = L + t.diag_off;
-    for (int idx = tid; idx < kb * kb; idx += 256) {
-        const int r = idx % kb, c = idx / kb;
-        a[c][r] = (r >= c) ? D[c * ld + r] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        if (tid == 0) {
-            double d = a[j][j];
+    // lane r keeps row r of the tile in registers (static indices: every loop is unrolled)
+    double row[TS];
+#pragma unroll
+    for (int c = 0; c < TS; ++c) row[c] = (c < kb && r < kb && c <= r) ? D[c * ld + r] : 0.0;
+#pragma unroll
+    for (int j = 0; j < TS; ++j) {
+        if (j < kb) {
+            col[r] = row[j];                       // column j (rows >= j valid)
+            __syncthreads();
+            double d = col[j];
             if (!(d > 0.0)) {
-                atomicMin(status, t.col0 + j);
+                if (r == 0) atomicMin(status, t.col0 + j);
                 d = 1.0;
             }
-            a[j][j] = sqrt(d);
+            d = sqrt(d);
+            const double l = (r > j) ? row[j] / d : (r == j ? d : row[j]);
+            row[j] = l;
+            __syncthreads();
+            col[r] = l;                            // l_rj for r > j
+            __syncthreads();
+#pragma unroll
+            for (int c = j + 1; c < TS; ++c)
+                if (c <= r && c < kb) row[c] -= l * col[c];
+            __syncthreads();
         }
-        __syncthreads();
-        const double dj = a[j][j];
-        for (int r = j + 1 + tid; r < kb; r += 256) a[j][r] /= dj;
-        __syncthreads();
-        const int m = kb - j - 1;
-        for (int idx = tid; idx < m * m; idx += 256) {
-            const int c = j + 1 + idx / m, r = j + 1 + idx % m;
-            if (r >= c) a[c][r] -= a[j][r] * a[j][c];
-        }
-        __syncthreads();
     }
-    for (int idx = tid; idx < kb * kb; idx += 256) {
-        const int r = idx % kb, c = idx / kb;
-        if (r >= c) D[c * ld + r] = a[c][r];
-    }
-    // inverse: column c of L^{-1} by forward substitution (one thread per column)
-    if (tid < kb) {
-        const int c = tid;
-        iv[c][c] = 1.0 / a[c][c];
-        for (int i = c + 1; i < kb; ++i) {
-            double s = 0.0;
-            for (int k = c; k < i; ++k) s += a[k][i] * iv[c][k];
-            iv[c][i] = -s / a[i][i];
-        }
+#pragma unroll
+    for (int c = 0; c < TS; ++c) {
+        if (c < kb && r < kb && c <= r) D[c * ld + r] = row[c];
+        lt[c][r] = (c <= r) ? row[c] : 0.0;
     }
     __syncthreads();
-    double* out = inv + (int64_t)t.slot * TS * TS;
-    for (int idx = tid; idx < TS * TS; idx += 256) {
-        const int r = idx % TS, c = idx / TS;
-        out[idx] = (c < kb && r < kb && r >= c) ? iv[c][r] : 0.0;
+    // inverse X = L^{-1}; lane c owns column c: X[i][c] = (delta_ic - sum_{k<i} L(i,k) X[k][c]) / L(i,i)
+    double x[TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i) {
+        double sacc = (i == r) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) sacc -= lt[k][i] * x[k];
+        x[i] = (i < kb) ? sacc / lt[i][i] : 0.0;
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TS; ++i) lt[r][i] = (r < kb) ? x[i] : 0.0;     // lt[c][i] = X[i][c]
+    __syncthreads();
+    double* out = inv + (int64_t)t.slot * TS * TS;
+    for (int c = 0; c < TS; ++c) out[c * TS + r] = lt[c][r];
 }
 
 // ------------------------------------------------------------------------------------------
-// Batched tile GEMM, C = alpha A B^T + beta C, 64x64 output tile per workgroup (4 waves of
-// 32x32), FP64 MFMA 16x16x4, K staged through LDS in chunks of 16.
+// Batched tile GEMM, C = alpha A B^T + beta C, TM x TN output tile per 256-thread workgroup
+// (2x2 waves), FP64 MFMA 16x16x4, K staged through double-buffered LDS in chunks of 16.
 // MFMA f64 16x16x4 operand layout on gfx950 (measured): A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k, D[i][j] in lane j + 16 (i % 4), register i / 4 (rows interleaved across the four
 // 16-lane groups; the f32 16x16x4 variant instead uses lane j + 16 (i / 4), register i % 4).
 // ------------------------------------------------------------------------------------------
 constexpr int KC = 16;
 
-__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks,
+// (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
+template <int TM, int TN>
+__global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
                                               double* __restrict__ L, double* __restrict__ U,
                                               const double* __restrict__ inv) {
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int AST = 256 / TM, BST = 256 / TN;           // k stride of the loaders
+    constexpr int AL = KC / AST, BL = KC / BST;              // loads per thread per chunk
     const GemmTask t = tasks[blockIdx.x];
     const double* A = L + t.a_off;
     const double* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
     double* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
-    __shared__ double As[KC][TS];
-    __shared__ double Bs[KC][TS];
+    __shared__ double As[2][KC][TM];
+    __shared__ double Bs[2][KC][TN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    const int lr = tid & 63, lk = tid >> 6;
-    const int64_t lda = t.lda, ldb = t.ldb;
-    for (int k0 = 0; k0 < t.K; k0 += KC) {
+    const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+    d4 acc[FM][FN];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = lk + 4 * q, kk = k0 + k;
-            const bool kin = kk < t.K;
-            As[k][lr] = (kin && lr < t.M) ? A[kk * lda + lr] : 0.0;
-            Bs[k][lr] = (kin && lr < t.N) ? B[kk * ldb + lr] : 0.0;
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+    const int ar = tid % TM, ak = tid / TM, br = tid % TN, bk = tid / TN;
+    const int64_t lda = t.lda, ldb = t.ldb;
+    const bool arow = ar < t.M, brow = br < t.N;
+    double ra[AL], rb[BL];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < AL; ++q) {
+            const int kk = k0 + ak + AST * q;
+            ra[q] = (arow && kk < t.K) ? A[kk * lda + ar] : 0.0;
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BL; ++q) {
+            const int kk = k0 + bk + BST * q;
+            rb[q] = (brow && kk < t.K) ? B[kk * ldb + br] : 0.0;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < AL; ++q) As[buf][ak + AST * q][ar] = ra[q];
+#pragma unroll
+        for (int q = 0; q < BL; ++q) Bs[buf][bk + BST * q][br] = rb[q];
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+        const bool more = k0 + KC < t.K;
+        if (more) gload(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kr = ks + (lane >> 4), cc = lane & 15;
-            const double a0 = As[kr][wm + cc], a1 = As[kr][wm + 16 + cc];
-            const double b0 = Bs[kr][wn + cc], b1 = Bs[kr][wn + 16 + cc];
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+            double af[FM], bf[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = As[buf][kr][wm + 16 * i + cc];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][kr][wn + 16 * j + cc];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
         }
+        if (more) sstore(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
     const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
     const bool beta0 = (t.flags & GT_BETA0) != 0;
     const int64_t ldc = t.ldc;
     const int col_l = lane & 15, row_l = lane >> 4;
-    auto store = [&](const d4& acc, int i0, int j0) {
-        const int col = j0 + col_l;
-        if (col >= t.N) return;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = i0 + 4 * r + row_l;
-            if (row < t.M) {
-                double* p = C + col * ldc + row;
-                const double v = alpha * acc[r];
-                *p = beta0 ? v : (*p + v);
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int col = wn + 16 * j + col_l;
+            if (col >= t.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wm + 16 * i + 4 * r + row_l;
+                if (row < t.M) {
+                    double* p = C + col * ldc + row;
+                    const double v = alpha * acc[i][j][r];
+                    *p = beta0 ? v : (*p + v);
+                }
             }
         }
-    };
-    store(acc00, wm, wn);
-    store(acc01, wm, wn + 16);
-    store(acc10, wm + 16, wn);
-    store(acc11, wm + 16, wn + 16);
+}
+
+// executed flops of a task range (2 M N K per tile task)
+static double gemm_work(const Symbolic& S, const TaskRange& r) {
+    double w = 0.0;
+    for (int64_t q = r.b; q < r.e; ++q) {
+        const GemmTask& g = S.gemm_tasks[q];
+        w += 2.0 * g.M * g.N * g.K;
+    }
+    return w;
+}
+
+static void launch_gemm(const TaskRange& r, const GemmTask* tasks, double* L, double* U, const double* inv,
+                        hipStream_t s) {
+    if (r.m > r.b) {
+        k_gemm<128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv);
+        LAUNCH_CHECK();
+    }
+    if (r.e > r.m) {
+        k_gemm<64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv);
+        LAUNCH_CHECK();
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -449,28 +518,6 @@ __global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restri
     }
 }
 
-__global__ __launch_bounds__(64) void k_fdiag(const SolveDiagTask* __restrict__ tasks,
-                                              const double* __restrict__ L,
-                                              double* __restrict__ xall, int64_t ldx) {
-    const SolveDiagTask t = tasks[blockIdx.x];
-    double* x = xall + blockIdx.y * ldx + t.x_off;
-    __shared__ double a[TS][TS + 1];
-    __shared__ double xs[TS];
-    const int tid = threadIdx.x, kb = t.kb;
-    const double* D = L + t.diag_off;
-    for (int c = 0; c < kb; ++c)
-        if (tid >= c && tid < kb) a[c][tid] = D[(int64_t)c * t.ld + tid];
-    if (tid < kb) xs[tid] = x[tid];
-    __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        if (tid == j) xs[j] /= a[j][j];
-        __syncthreads();
-        if (tid > j && tid < kb) xs[tid] -= a[j][tid] * xs[j];
-        __syncthreads();
-    }
-    if (tid < kb) x[tid] = xs[tid];
-}
-
 // dot of panel row r (columns 0..kb-1, leading dimension ld) with y held in LDS
 __device__ __forceinline__ double panel_row_dot(const double* __restrict__ P, int64_t ld, int kb,
                                                 const double* y, int r) {
@@ -479,22 +526,35 @@ __device__ __forceinline__ double panel_row_dot(const double* __restrict__ P, in
     return s;
 }
 
-__global__ __launch_bounds__(64) void k_fgemv(const SolveGemvTask* __restrict__ tasks,
+// Forward step for column block k of a supernode: y_k = L_kk^{-1} x_k (stored tile inverse),
+// then rows [r0, r0 + nr) below: x (rows < w) -= L(rows, k) y_k, u (rows >= w) += L(rows, k) y_k.
+// The writer task (nr == 0) stores y_k into Y; no task of the launch writes x_k.
+__global__ __launch_bounds__(64) void k_fstep(const FStepTask* __restrict__ tasks,
                                               const double* __restrict__ L,
-                                              double* __restrict__ xall, int64_t ldx,
-                                              double* __restrict__ uall, int64_t ldu) {
-    const SolveGemvTask t = tasks[blockIdx.x];
+                                              const double* __restrict__ inv,
+                                              double* __restrict__ xall, double* __restrict__ yall,
+                                              int64_t ldx, double* __restrict__ uall, int64_t ldu) {
+    const FStepTask t = tasks[blockIdx.x];
     double* x = xall + blockIdx.y * ldx;
     double* u = uall + blockIdx.y * ldu;
-    __shared__ double y[TS];
+    __shared__ double xs[TS], y[TS];
     const int tid = threadIdx.x;
-    if (tid < t.kb) y[tid] = x[t.x_cols + tid];
+    xs[tid] = tid < t.kb ? x[t.x_blk + tid] : 0.0;
     __syncthreads();
+    const double* Vi = inv + t.inv_off;
+    double v = 0.0;
+    for (int j = 0; j < t.kb; ++j) v += Vi[j * TS + tid] * xs[j];
+    y[tid] = v;
+    __syncthreads();
+    if (t.nr == 0) {
+        if (tid < t.kb) yall[blockIdx.y * ldx + t.x_blk + tid] = v;
+        return;
+    }
     if (tid >= t.nr) return;
-    const double v = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
+    const double d = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
     const int row = t.r0 + tid;
-    if (row < t.w) x[t.x_rows + row] -= v;
-    else u[t.u_off + row - t.w] += v;
+    if (row < t.w) x[t.x_rows + row] -= d;
+    else u[t.u_off + row - t.w] += d;
 }
 
 __global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__ tasks,
@@ -541,32 +601,31 @@ __global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__
     if (tid < t.kb) part[(int64_t)t.part * TS + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
 }
 
+// Backward diagonal block: x_k = L_kk^{-T} (y_k - sum of the partial sums), via the inverse.
 __global__ __launch_bounds__(64) void k_bdiag(const SolveDiagTask* __restrict__ tasks,
-                                              const double* __restrict__ L,
+                                              const double* __restrict__ inv,
                                               const double* __restrict__ partall, int64_t ldp,
                                               double* __restrict__ xall, int64_t ldx) {
     const SolveDiagTask t = tasks[blockIdx.x];
     double* x = xall + blockIdx.y * ldx + t.x_off;
     const double* part = partall + blockIdx.y * ldp;
-    __shared__ double a[TS][TS + 1];
-    __shared__ double xs[TS];
+    __shared__ double vi[TS][TS + 1];   // vi[c][r] = inverse(r, c)
+    __shared__ double z[TS];
     const int tid = threadIdx.x, kb = t.kb;
-    const double* D = L + t.diag_off;
-    for (int c = 0; c < kb; ++c)
-        if (tid >= c && tid < kb) a[c][tid] = D[(int64_t)c * t.ld + tid];
+    const double* Vi = inv + t.inv_off;
+    for (int c = 0; c < kb; ++c) vi[c][tid] = Vi[c * TS + tid];
+    double v = 0.0;
     if (tid < kb) {
-        double v = x[tid];
+        v = x[tid];
         for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + tid];
-        xs[tid] = v;
     }
+    z[tid] = v;
     __syncthreads();
-    for (int j = kb - 1; j >= 0; --j) {
-        if (tid == j) xs[j] /= a[j][j];
-        __syncthreads();
-        if (tid < j) xs[tid] -= a[tid][j] * xs[j];
-        __syncthreads();
+    if (tid < kb) {
+        double s = 0.0;
+        for (int j = 0; j < kb; ++j) s += vi[tid][j] * z[j];   // (L^{-1})^T row tid = column tid of L^{-1}
+        x[tid] = s;
     }
-    if (tid < kb) x[tid] = xs[tid];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -721,8 +780,7 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
         zr_.upload(zr);
     }
     fg_.upload(S->fgather_tasks);
-    fd_.upload(S->fdiag_tasks);
-    fv_.upload(S->fgemv_tasks);
+    fs_.upload(S->fstep_tasks);
     bd_.upload(S->bdiag_tasks);
     bv_.upload(S->bgemv_tasks);
     children_.upload(S->children);
@@ -793,31 +851,27 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
             auto [p0, p1] = P.potrf_steps[t];
             if (p1 > p0) {
                 prof.begin(KernelProfile::POTRF, s);
-                k_potrf<<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
+                k_potrf<<<(unsigned)(p1 - p0), 64, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
                                                            status_.as<int>());
                 LAUNCH_CHECK();
                 prof.end(s);
             }
-            auto [t0, t1] = P.trsm_steps[t];
-            if (t1 > t0) {
-                prof.begin(KernelProfile::TRSM, s);
-                k_gemm<<<(unsigned)(t1 - t0), 256, 0, s>>>(gemm_.as<GemmTask>() + t0, L, U, inv_.as<double>());
-                LAUNCH_CHECK();
+            const TaskRange& tr = P.trsm_steps[t];
+            if (tr.e > tr.b) {
+                prof.begin(KernelProfile::TRSM, s, prof.on ? gemm_work(S, tr) : 0.0, tr.e - tr.b);
+                launch_gemm(tr, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
                 prof.end(s);
             }
-            auto [u0, u1] = P.update_steps[t];
-            if (u1 > u0) {
-                prof.begin(KernelProfile::UPDATE, s);
-                k_gemm<<<(unsigned)(u1 - u0), 256, 0, s>>>(gemm_.as<GemmTask>() + u0, L, U, inv_.as<double>());
-                LAUNCH_CHECK();
+            const TaskRange& ur = P.update_steps[t];
+            if (ur.e > ur.b) {
+                prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b);
+                launch_gemm(ur, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
                 prof.end(s);
             }
         }
-        if (P.syrk.second > P.syrk.first) {
-            prof.begin(KernelProfile::SYRK, s);
-            k_gemm<<<(unsigned)(P.syrk.second - P.syrk.first), 256, 0, s>>>(gemm_.as<GemmTask>() + P.syrk.first,
-                                                                           L, U, inv_.as<double>());
-            LAUNCH_CHECK();
+        if (P.syrk.e > P.syrk.b) {
+            prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
+            launch_gemm(P.syrk, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
             prof.end(s);
         }
     }
@@ -857,6 +911,8 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
     const double* L = L_.as<double>();
     const int32_t* sdeg = sdeg_.as<int32_t>();
     const int64_t* reloff = reloff_.as<int64_t>();
+    ysolve_.alloc((size_t)ldx * nrhs * sizeof(double));
+    double* Y = ysolve_.as<double>();
     for (const LevelPlan& P : S.levels) {
         if (P.fgather.second > P.fgather.first) {
             dim3 g((unsigned)(P.fgather.second - P.fgather.first), (unsigned)nrhs);
@@ -866,20 +922,12 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             LAUNCH_CHECK();
             prof.end(s);
         }
-        for (size_t t = 0; t < P.fdiag_steps.size(); ++t) {
-            auto [d0, d1] = P.fdiag_steps[t];
-            if (d1 > d0) {
-                dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
-                prof.begin(KernelProfile::FDIAG, s);
-                k_fdiag<<<g, 64, 0, s>>>(fd_.as<SolveDiagTask>() + d0, L, xp, ldx);
-                LAUNCH_CHECK();
-                prof.end(s);
-            }
-            auto [v0, v1] = P.fgemv_steps[t];
+        for (size_t t = 0; t < P.fstep_steps.size(); ++t) {
+            auto [v0, v1] = P.fstep_steps[t];
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
                 prof.begin(KernelProfile::FGEMV, s);
-                k_fgemv<<<g, 64, 0, s>>>(fv_.as<SolveGemvTask>() + v0, L, xp, ldx, u, ldu);
+                k_fstep<<<g, 64, 0, s>>>(fs_.as<FStepTask>() + v0, L, inv_.as<double>(), xp, Y, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -892,7 +940,7 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BGEMV, s);
-                k_bgemv<<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), xp, ldx, part, ldp);
+                k_bgemv<<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), Y, ldx, part, ldp);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -900,12 +948,16 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BDIAG, s);
-                k_bdiag<<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, L, part, ldp, xp, ldx);
+                k_bdiag<<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, inv_.as<double>(), part, ldp, Y, ldx);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
         }
     }
+    prof.begin(KernelProfile::MAPS, s);
+    k_copy<<<std::min<unsigned>(nblk(ldx * nrhs, 256), 8192), 256, 0, s>>>(Y, xp, ldx * nrhs);
+    LAUNCH_CHECK();
+    prof.end(s);
 }
 
 void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
@@ -1062,6 +1114,8 @@ void FactorizerDev::stash() {
     require(1);
     Lstash_.alloc(L_.bytes);
     HIP_CHECK(hipMemcpy(Lstash_.p, L_.p, L_.bytes, hipMemcpyDeviceToDevice));
+    invstash_.alloc(inv_.bytes);
+    HIP_CHECK(hipMemcpy(invstash_.p, inv_.p, inv_.bytes, hipMemcpyDeviceToDevice));
     stash_valid_ = true;
     stash_type_ = ftype;
     stash_succ_ = num_successful;
@@ -1071,6 +1125,8 @@ void FactorizerDev::swap_stash() {
     if (!stash_valid_) throw Error(-2, "No stashed factorization");
     std::swap(L_.p, Lstash_.p);
     std::swap(L_.bytes, Lstash_.bytes);
+    std::swap(inv_.p, invstash_.p);            // tile inverses belong to the factor (solves use them)
+    std::swap(inv_.bytes, invstash_.bytes);
     std::swap(ftype, stash_type_);
     std::swap(num_successful, stash_succ_);
     if (ftype == 0) ftype = 1;   // the symbolic analysis is shared by both factors
@@ -1078,6 +1134,7 @@ void FactorizerDev::swap_stash() {
 
 void FactorizerDev::clear_stash() {
     Lstash_.release();
+    invstash_.release();
     stash_valid_ = false;
 }
 
@@ -1090,9 +1147,11 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
         throw Error(-1, "the tile kernel supports alpha in {+1,-1} and beta in {0,1}");
     if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, N) || ldc < std::max(1, M))
         throw Error(-1, "bad GEMM dimensions");
+    // 128x128 tiles when both dimensions reach 128 (as the factorization schedule does), else 64x64
+    const int TT = (M >= 128 && N >= 128) ? 128 : 64;
     std::vector<GemmTask> tasks;
-    for (int c0 = 0; c0 < N; c0 += TS)
-        for (int r0 = 0; r0 < M; r0 += TS) {
+    for (int c0 = 0; c0 < N; c0 += TT)
+        for (int r0 = 0; r0 < M; r0 += TT) {
             GemmTask g{};
             g.a_off = r0;
             g.lda = lda;
@@ -1100,8 +1159,8 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
             g.ldb = ldb;
             g.c_off = (int64_t)c0 * ldc + r0;
             g.ldc = ldc;
-            g.M = std::min(TS, M - r0);
-            g.N = std::min(TS, N - c0);
+            g.M = std::min(TT, M - r0);
+            g.N = std::min(TT, N - c0);
             g.K = K;
             g.flags = GT_C_IN_U | GT_B_IN_INV | (beta == 0.0 ? GT_BETA0 : 0) | (alpha > 0 ? GT_ALPHA_POS : 0);
             tasks.push_back(g);
@@ -1110,8 +1169,11 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
     DeviceArray d;
     d.upload(tasks);
     hipStream_t s = S_(stream);
-    k_gemm<<<(unsigned)tasks.size(), 256, 0, s>>>(d.as<GemmTask>(), const_cast<double*>(A), C, B);
-    LAUNCH_CHECK();
+    TaskRange r;
+    r.b = 0;
+    r.m = TT == 128 ? (int64_t)tasks.size() : 0;
+    r.e = (int64_t)tasks.size();
+    launch_gemm(r, d.as<GemmTask>(), const_cast<double*>(A), C, B, s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -87,11 +87,28 @@ struct ExtendTask {         // extend-add of child's update matrix into the pare
     int32_t pad;
 };
 
-struct SolveDiagTask {      // triangular solve on one diagonal tile of a supernode
-    int64_t diag_off;       // L offset of the tile
+struct SolveDiagTask {      // backward: x_k = L_kk^{-T} (y_k - sum of partials), via the stored inverse
+    int64_t inv_off;        // offset of the tile inverse (64x64, column-major)
     int32_t ld, kb;
     int64_t x_off;          // scalar offset in the permuted solution vector
-    int32_t part_begin, part_end;   // backward: partial-sum rows to add first
+    int32_t part_begin, part_end;   // partial-sum slots to subtract first
+};
+
+struct FStepTask {          // forward step: y_k = L_kk^{-1} x_k, then rows [r0, r0+nr) -= L(rows,k) y_k
+    int64_t inv_off;        // tile inverse
+    int64_t l_off;          // L offset of (r0, k0)  (unused by the writer task)
+    int32_t ld, kb;
+    int32_t nr;             // rows of this task (0 = writer task: stores y_k)
+    int32_t w;              // supernode width (scalars): rows < w go to x, rows >= w go to u
+    int32_t r0;             // first row (scalar, within panel)
+    int32_t pad;
+    int64_t x_blk;          // x / y offset of the column block
+    int64_t x_rows;         // x offset of panel row 0 (rows < w)
+    int64_t u_off;          // u offset of structure row 0 (rows >= w)
+};
+
+struct TaskRange {          // [b, m): 128x128-tile GEMM tasks, [m, e): 64x64-tile tasks
+    int64_t b = 0, m = 0, e = 0;
 };
 
 struct SolveGemvTask {      // forward: rows [r0, r0+nr) of a panel, column block [k0, k0+kb)
@@ -120,13 +137,13 @@ struct LevelPlan {
     // factorization
     std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks
     std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
-    std::vector<std::pair<int64_t,int64_t>> trsm_steps;      // per step: range in gemm tasks
-    std::vector<std::pair<int64_t,int64_t>> update_steps;    // per step: range in gemm tasks
-    std::pair<int64_t,int64_t> syrk{0, 0};                   // range in gemm tasks
+    std::vector<TaskRange> trsm_steps;                       // per step: gemm tasks
+    std::vector<TaskRange> update_steps;                     // per step: gemm tasks
+    TaskRange syrk;                                          // gemm tasks
     std::pair<int64_t,int64_t> zero_u{0, 0};                 // U arena range to clear (doubles)
     // forward solve
     std::pair<int64_t,int64_t> fgather{0, 0};
-    std::vector<std::pair<int64_t,int64_t>> fdiag_steps, fgemv_steps;
+    std::vector<std::pair<int64_t,int64_t>> fstep_steps;     // per step: range in fstep tasks
     // backward solve (steps in execution order)
     std::vector<std::pair<int64_t,int64_t>> bgemv_steps, bdiag_steps;
 };
@@ -173,8 +190,9 @@ struct Symbolic {
     std::vector<PotrfTask> potrf_tasks;
     std::vector<GemmTask> gemm_tasks;
     std::vector<SolveGatherTask> fgather_tasks;
-    std::vector<SolveDiagTask> fdiag_tasks, bdiag_tasks;
-    std::vector<SolveGemvTask> fgemv_tasks, bgemv_tasks;
+    std::vector<FStepTask> fstep_tasks;
+    std::vector<SolveDiagTask> bdiag_tasks;
+    std::vector<SolveGemvTask> bgemv_tasks;
     std::vector<std::pair<int64_t,int64_t>> zero_ranges;   // U arena (offset, count) to clear
     std::vector<int64_t> solve_rows;           // scalar x offsets for struct rows (per sn)
     std::vector<int64_t> solve_rows_off;       // per supernode offset into solve_rows
@@ -195,7 +213,9 @@ struct SymbolicOptions {
     // relaxed amalgamation thresholds (scalar columns)
     int nrelax0 = 12, nrelax1 = 48, nrelax2 = 192;
     double zrelax0 = 0.8, zrelax1 = 0.2, zrelax2 = 0.05;
-    int tile = 64;          // dense tile / step width (scalars)
+    int tile = 64;          // diagonal tile / inner step width (scalars)
+    int outer = 256;        // outer panel block: trailing update with K = outer
+    int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
     int solve_chunk = 512;  // backward partial-sum rows per task
 };
 
@@ -204,7 +224,10 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
 
 // Host replay of the device schedule (returns the first failing pivot column or -1) and the
 // supernodal-panel -> CSC conversion.
-int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L);
+int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L,
+                        std::vector<double>& inv);
+void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+                   std::vector<double>& x);
 void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx);
 
 // Workload (host)

@@ -423,7 +423,45 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         for (int32_t s = 0; s < ns; ++s) { S.u_solve_off[s] = acc; acc += 3 * (int64_t)S.sdeg[s]; }
         S.u_solve_size = acc;
     }
-    int64_t max_potrf = 0, max_part = 0;
+    int64_t max_part = 0;
+    const int OB = std::max(opt.outer / T, 1) * T;
+    const int BT = opt.big_tile;
+    // Tiles of the lower part (rows >= cols, in region coordinates) of the GEMM region
+    // C(r, c) (+)= -A(r, :) B(c, :)^T, r in [max(c0,rlo), rhi), c in [c0, c1), with A(r,k) at
+    // a_base + r + k*lda, B(c,k) at b_base + c + k*ldb and C(r,c) at c_base + r + c*ldc.
+    // 128x128 tiles when the region is large in both dimensions, else 64x64.
+    auto emit_lower = [&](std::vector<GemmTask>& big, std::vector<GemmTask>& small, int64_t a_base, int lda,
+                          int64_t b_base, int ldb, int64_t c_base, int ldc, int c0, int c1, int rhi, int K,
+                          int flags, bool allow_big, bool c_is_u = false) {
+        (void)c_is_u;
+        if (c1 <= c0 || K <= 0) return;
+        const bool use_big = allow_big && (c1 - c0) >= BT && (rhi - c0) >= BT;
+        const int TT = use_big ? BT : T;
+        for (int cc = c0; cc < c1; cc += TT)
+            for (int rr = cc; rr < rhi; rr += TT) {
+                GemmTask g{};
+                g.a_off = a_base + rr;
+                g.lda = lda;
+                g.b_off = b_base + cc;
+                g.ldb = ldb;
+                g.c_off = c_base + (int64_t)cc * ldc + rr;
+                g.ldc = ldc;
+                g.M = std::min(TT, rhi - rr);
+                g.N = std::min(TT, c1 - cc);
+                g.K = K;
+                g.flags = flags;
+                (use_big ? big : small).push_back(g);
+            }
+    };
+    auto push_tasks = [](std::vector<GemmTask>& all, std::vector<GemmTask>& big, std::vector<GemmTask>& small) {
+        TaskRange r;
+        r.b = (int64_t)all.size();
+        all.insert(all.end(), big.begin(), big.end());
+        r.m = (int64_t)all.size();
+        all.insert(all.end(), small.begin(), small.end());
+        r.e = (int64_t)all.size();
+        return r;
+    };
     std::vector<std::pair<int64_t,int64_t>> zero_ranges;
     for (int32_t lv = 0; lv < S.num_levels; ++lv) {
         LevelPlan& P = S.levels[lv];
@@ -478,13 +516,15 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 }
             P.zero_u = {z0, (int64_t)zero_ranges.size()};
         }
-        // dense steps
+        // dense steps: per supernode, outer blocks of `outer` columns; inside, `tile`-wide steps
+        // (diagonal POTRF + tile inverse, TRSM of all rows below, update of the outer block's
+        // remaining columns); after an outer block, one trailing update with K = outer.
         int32_t maxsteps = 0;
         for (int32_t s : sl) maxsteps = std::max<int32_t>(maxsteps, (3 * S.swidth[s] + T - 1) / T);
+        std::vector<int64_t> slot_of_step;   // per (supernode at this level) current step's inverse slot
         for (int32_t t = 0; t < maxsteps; ++t) {
             const int64_t p0 = (int64_t)S.potrf_tasks.size();
-            int32_t slot = 0;
-            std::vector<std::pair<int32_t,int32_t>> active;   // (sn, slot)
+            std::vector<std::pair<int32_t,int64_t>> active;   // (sn, inverse slot)
             for (int32_t s : sl) {
                 const int32_t W = 3 * S.swidth[s];
                 if (t * T >= W) continue;
@@ -493,86 +533,70 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 pt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
                 pt.ld = ld;
                 pt.kb = kb;
-                pt.slot = slot;
+                pt.slot = (int32_t)S.potrf_tasks.size();
                 pt.col0 = 3 * S.sfirst[s] + k0;
                 S.potrf_tasks.push_back(pt);
                 S.flops_potrf += (double)kb * kb * kb / 3.0;
-                active.push_back({s, slot});
-                ++slot;
+                active.push_back({s, pt.slot});
             }
-            max_potrf = std::max<int64_t>(max_potrf, slot);
             P.potrf_steps.push_back({p0, (int64_t)S.potrf_tasks.size()});
-            const int64_t g0 = (int64_t)S.gemm_tasks.size();
-            for (auto [s, sl_] : active) {
-                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                const int32_t k0 = t * T, kb = std::min(T, W - k0);
-                const int64_t off = S.panel_off[s];
-                for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
-                    GemmTask g{};
-                    g.a_off = off + (int64_t)k0 * ld + r0;
-                    g.lda = ld;
-                    g.b_off = (int64_t)sl_ * T * T;
-                    g.ldb = T;
-                    g.c_off = g.a_off;
-                    g.ldc = ld;
-                    g.M = std::min(T, W + D - r0);
-                    g.N = kb;
-                    g.K = kb;
-                    g.flags = GT_BETA0 | GT_ALPHA_POS | GT_B_IN_INV;
-                    S.gemm_tasks.push_back(g);
-                }
-            }
-            P.trsm_steps.push_back({g0, (int64_t)S.gemm_tasks.size()});
-            const int64_t u0 = (int64_t)S.gemm_tasks.size();
-            for (auto [s, sl_] : active) {
-                (void)sl_;
-                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                const int32_t k0 = t * T, kb = std::min(T, W - k0);
-                const int64_t off = S.panel_off[s];
-                for (int32_t c0 = k0 + kb; c0 < W; c0 += T)
-                    for (int32_t r0 = c0; r0 < W + D; r0 += T) {
+            {   // TRSM: X := X L_kk^{-T} on all rows below the diagonal tile (64x64 tiles, in place)
+                std::vector<GemmTask> big, small;
+                for (auto [s, slot] : active) {
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                    const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                    const int64_t off = S.panel_off[s];
+                    for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
                         GemmTask g{};
                         g.a_off = off + (int64_t)k0 * ld + r0;
                         g.lda = ld;
-                        g.b_off = off + (int64_t)k0 * ld + c0;
-                        g.ldb = ld;
-                        g.c_off = off + (int64_t)c0 * ld + r0;
+                        g.b_off = slot * (int64_t)T * T;
+                        g.ldb = T;
+                        g.c_off = g.a_off;
                         g.ldc = ld;
                         g.M = std::min(T, W + D - r0);
-                        g.N = std::min(T, W - c0);
+                        g.N = kb;
                         g.K = kb;
-                        g.flags = 0;
-                        S.gemm_tasks.push_back(g);
+                        g.flags = GT_BETA0 | GT_ALPHA_POS | GT_B_IN_INV;
+                        small.push_back(g);
                     }
+                }
+                P.trsm_steps.push_back(push_tasks(S.gemm_tasks, big, small));
             }
-            P.update_steps.push_back({u0, (int64_t)S.gemm_tasks.size()});
+            {   // inner updates (within the outer block) and outer trailing updates
+                std::vector<GemmTask> big, small;
+                for (auto [s, slot] : active) {
+                    (void)slot;
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                    const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                    const int32_t J0 = (k0 / OB) * OB, Jend = std::min(W, J0 + OB);
+                    const int64_t off = S.panel_off[s];
+                    // inner: columns [k0+kb, Jend), rows [c, W+D), K = kb
+                    emit_lower(big, small, off + (int64_t)k0 * ld, ld, off + (int64_t)k0 * ld, ld, off, ld,
+                               k0 + kb, Jend, W + D, kb, 0, /*allow_big=*/false);
+                    // outer: after the last inner step of the outer block
+                    if (k0 + kb == Jend && Jend < W)
+                        emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
+                                   Jend, W, W + D, Jend - J0, 0, true);
+                }
+                P.update_steps.push_back(push_tasks(S.gemm_tasks, big, small));
+            }
         }
-        // Schur update into U
+        // Schur update U -= L21 L21^T (lower tiles), K = W
         {
-            const int64_t s0 = (int64_t)S.gemm_tasks.size();
+            std::vector<GemmTask> big, small;
             for (int32_t s : sl) {
                 const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
                 if (D == 0) continue;
                 const int64_t off = S.panel_off[s];
-                for (int32_t c0 = 0; c0 < D; c0 += T)
-                    for (int32_t r0 = c0; r0 < D; r0 += T) {
-                        GemmTask g{};
-                        g.a_off = off + W + r0;
-                        g.lda = ld;
-                        g.b_off = off + W + c0;
-                        g.ldb = ld;
-                        g.c_off = S.u_off[s] + (int64_t)c0 * D + r0;
-                        g.ldc = D;
-                        g.M = std::min(T, D - r0);
-                        g.N = std::min(T, D - c0);
-                        g.K = W;
-                        g.flags = GT_C_IN_U;
-                        S.gemm_tasks.push_back(g);
-                    }
+                // region rows/cols are indices into L21 (0..D); A = P[W + r, 0..W), C = U[r, c]
+                emit_lower(big, small, off + W, ld, off + W, ld, S.u_off[s], D, 0, D, D, W, GT_C_IN_U, true,
+                           /*c_is_u=*/true);
             }
-            P.syrk = {s0, (int64_t)S.gemm_tasks.size()};
+            P.syrk = push_tasks(S.gemm_tasks, big, small);
         }
-        // forward solve: gather, then per step diag + gemv
+        // forward solve: gather children's update vectors, then per step one fused launch
+        // (y_k = L_kk^{-1} x_k by the stored inverse; rows below -= L(rows, k) y_k)
         {
             const int64_t f0 = (int64_t)S.fgather_tasks.size();
             for (int32_t s : sl) {
@@ -587,36 +611,35 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 S.fgather_tasks.push_back(g);
             }
             P.fgather = {f0, (int64_t)S.fgather_tasks.size()};
+            const int64_t pbase = P.potrf_steps.empty() ? 0 : P.potrf_steps[0].first;
+            (void)pbase;
             for (int32_t t = 0; t < maxsteps; ++t) {
-                const int64_t d0 = (int64_t)S.fdiag_tasks.size();
-                const int64_t v0 = (int64_t)S.fgemv_tasks.size();
+                const int64_t v0 = (int64_t)S.fstep_tasks.size();
+                int64_t slot = P.potrf_steps[t].first;
                 for (int32_t s : sl) {
                     const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
                     if (t * T >= W) continue;
                     const int32_t k0 = t * T, kb = std::min(T, W - k0);
-                    SolveDiagTask dt{};
-                    dt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
-                    dt.ld = ld;
-                    dt.kb = kb;
-                    dt.x_off = 3 * (int64_t)S.sfirst[s] + k0;
-                    S.fdiag_tasks.push_back(dt);
+                    FStepTask ft{};
+                    ft.inv_off = slot * (int64_t)T * T;
+                    ft.ld = ld;
+                    ft.kb = kb;
+                    ft.w = W;
+                    ft.x_blk = 3 * (int64_t)S.sfirst[s] + k0;
+                    ft.x_rows = 3 * (int64_t)S.sfirst[s];
+                    ft.u_off = S.u_solve_off[s];
+                    ft.nr = 0;                    // writer
+                    S.fstep_tasks.push_back(ft);
                     for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
-                        SolveGemvTask gv{};
-                        gv.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
-                        gv.ld = ld;
-                        gv.nr = std::min(T, W + D - r0);
-                        gv.kb = kb;
-                        gv.w = W;
-                        gv.r0 = r0;
-                        gv.x_cols = 3 * (int64_t)S.sfirst[s] + k0;
-                        gv.x_rows = 3 * (int64_t)S.sfirst[s];
-                        gv.u_off = S.u_solve_off[s];
-                        gv.rows_off = S.srows_off[s];
-                        S.fgemv_tasks.push_back(gv);
+                        FStepTask g = ft;
+                        g.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
+                        g.nr = std::min(T, W + D - r0);
+                        g.r0 = r0;
+                        S.fstep_tasks.push_back(g);
                     }
+                    ++slot;
                 }
-                P.fdiag_steps.push_back({d0, (int64_t)S.fdiag_tasks.size()});
-                P.fgemv_steps.push_back({v0, (int64_t)S.fgemv_tasks.size()});
+                P.fstep_steps.push_back({v0, (int64_t)S.fstep_tasks.size()});
             }
         }
         // backward solve: per step (last column block first): partial gemv^T, then diag
@@ -632,8 +655,14 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     if (t >= nblk) continue;
                     const int32_t kblk = nblk - 1 - t;
                     const int32_t k0 = kblk * T, kb = std::min(T, W - k0);
+                    // inverse slot of this block: the potrf task of step kblk for supernode s
+                    int64_t slot = P.potrf_steps[kblk].first;
+                    for (int32_t s2 : sl) {
+                        if (s2 == s) break;
+                        if (kblk * T < 3 * S.swidth[s2]) ++slot;
+                    }
                     SolveDiagTask dt{};
-                    dt.diag_off = S.panel_off[s] + (int64_t)k0 * ld + k0;
+                    dt.inv_off = slot * (int64_t)T * T;
                     dt.ld = ld;
                     dt.kb = kb;
                     dt.x_off = 3 * (int64_t)S.sfirst[s] + k0;
@@ -663,7 +692,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         }
     }
     S.U_size = std::max<int64_t>(arena.peak, 32);
-    S.inv_slots = std::max<int64_t>(max_potrf, 1);
+    S.inv_slots = std::max<int64_t>((int64_t)S.potrf_tasks.size(), 1);
     S.bpart_size = std::max<int64_t>(max_part, 1);
     S.zero_ranges = std::move(zero_ranges);
 

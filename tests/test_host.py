@@ -130,7 +130,7 @@ def test_partial_pins_are_rejected():
         analyze(H, np.array([1]))
 
 
-def emulate(H, pins, Ax, sigma=0.0):
+def emulate(H, pins, Ax, sigma=0.0, x_perm=None):
     iperm, st, fl = analyze(H, pins)
     nred, nnz = st[0], st[3]
     Lp = np.zeros(3 * nred + 1, np.int64)
@@ -138,9 +138,13 @@ def emulate(H, pins, Ax, sigma=0.0):
     Lx = np.zeros(nnz)
     fc = np.zeros(1, np.int64)
     pins = np.ascontiguousarray(pins, np.int64)
+    xp = None if x_perm is None else np.ascontiguousarray(x_perm, np.float64)
     _lib.call("mfs_debug_emulate_factor", _lib.ptr(H.Ap, _lib.c_i64), _lib.ptr(H.Ai, _lib.c_i64), H.n,
               _lib.ptr(pins, _lib.c_i64), len(pins), _lib.ptr(Ax, _lib.c_double), sigma, _lib.ptr(Lp, _lib.c_i64),
-              _lib.ptr(Li, _lib.c_i64), _lib.ptr(Lx, _lib.c_double), _lib.ptr(fc, _lib.c_i64))
+              _lib.ptr(Li, _lib.c_i64), _lib.ptr(Lx, _lib.c_double), _lib.ptr(fc, _lib.c_i64),
+              None if xp is None else _lib.ptr(xp, _lib.c_double))
+    if xp is not None:
+        return iperm, (Lp, Li, Lx), fc[0], xp
     return iperm, (Lp, Li, Lx), fc[0]
 
 
@@ -173,3 +177,49 @@ def test_device_schedule_replay_reports_first_bad_pivot():
     Ax = -orc.assemble(H.Ap, H.Ai, len(Xr), tets, He)
     _, _, fc = emulate(H, wl.z0_pins(3), Ax)
     assert fc == 0
+
+
+def chain_pattern(nb=20):
+    en = np.array([[i, i + 1] for i in range(nb - 1)], np.int32)
+    H = mfs.SystemAssembler(nb).blockSparsityPattern(en)
+    Ax = np.zeros(9 * H.nz)
+    for j in range(nb):
+        for k in range(H.Ap[j], H.Ap[j + 1]):
+            bi = H.Ai[k]
+            for c in range(3):
+                for r in range(3):
+                    if bi == j and r > c:
+                        continue
+                    i_, j_ = 3 * bi + r, 3 * j + c
+                    Ax[9 * k + 3 * c + r] = 10.1 if i_ == j_ else -0.1 * (1 + (i_ + j_) % 5)
+    return H, Ax
+
+
+@pytest.mark.parametrize("case", ["chain", "mesh5", "mesh7"])
+def test_device_solve_schedule_replay(case):
+    """Forward/backward solve task lists (with the stored tile inverses), replayed on the host,
+    solve P A P^T y = P b to 1e-12."""
+    if case == "chain":
+        H, Ax = chain_pattern()
+        pins = np.zeros(0, np.int64)
+    else:
+        N = int(case[-1])
+        tets, Xr = wl.cube_mesh(N)
+        He, _, _ = wl.neohookean(Xr, wl.perturb(N, Xr, 0), tets, gradient=False)
+        H = mfs.SystemAssembler(len(Xr)).blockSparsityPattern(tets)
+        Ax = orc.assemble(H.Ap, H.Ai, len(Xr), tets, He)
+        pins = wl.z0_pins(N)
+    iperm = analyze(H, pins)[0]
+    nred = len(iperm)
+    pos = np.empty(nred, np.int32)
+    pos[iperm] = np.arange(nred)
+    pinned = np.zeros(H.n, bool)
+    pinned[np.asarray(pins, np.int64) // 3] = True
+    red = -np.ones(H.n, np.int64)
+    red[~pinned] = np.arange(nred)
+    A = orc.dense_from_lower(*orc.permuted_lower(H.Ap, H.Ai, H.n, Ax, red, pos))
+    rng = np.random.default_rng(0)
+    xt = rng.standard_normal(3 * nred)
+    _, _, fc, xs = emulate(H, pins, Ax, 0.0, A @ xt)
+    assert fc == -1
+    assert np.linalg.norm(xs - xt) <= 1e-12 * np.linalg.norm(xt) * 100
