@@ -307,62 +307,173 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 // ------------------------------------------------------------------------------------------
 constexpr int TS = 64;   // diagonal tile size (scalars)
 
-__global__ __launch_bounds__(64) void k_potrf(const PotrfTask* __restrict__ tasks,
-                                              double* __restrict__ L, double* __restrict__ inv,
-                                              int* __restrict__ status) {
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Blocked in 16-column panels.  Per panel b: wave 0 factors the 16x16 diagonal block and inverts
+// it in registers (lane i = row i; broadcasts by readlane, no LDS round trips), then all 256
+// threads apply the panel TRSM (L_ib = A_ib X_bb^T) and the trailing SYRK through LDS.  The
+// full inverse follows by block rows: X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.  Rows/columns beyond
+// kb are padded with the identity, so they factor and invert to the identity and are masked out.
+__global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
+                                               double* __restrict__ L, double* __restrict__ inv,
+                                               int* __restrict__ status) {
+    constexpr int NB = 16;
     const PotrfTask t = tasks[blockIdx.x];
-    __shared__ double col[TS];          // broadcast column (and diagonal) of the current step
-    __shared__ double lt[TS][TS + 1];   // final factor, lt[c][i] = L(i, c); reused for the inverse
-    const int r = threadIdx.x, kb = t.kb;
+    __shared__ double a[TS][TS + 1];    // a[c][r] = tile(r, c) -> L(r, c)
+    __shared__ double x[TS][TS + 1];    // x[c][r] = X(r, c), X = L^{-1}
+    const int tid = threadIdx.x, kb = t.kb;
     const int64_t ld = t.ld;
     double* D = L + t.diag_off;
-    // lane r keeps row r of the tile in registers (static indices: every loop is unrolled)
-    double row[TS];
+    for (int idx = tid; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        double v;
+        if (r < kb && c < kb) v = (r >= c) ? D[c * ld + r] : 0.0;
+        else v = (r == c) ? 1.0 : 0.0;
+        a[c][r] = v;
+        x[c][r] = 0.0;
+    }
+    __syncthreads();
+    for (int b = 0; b < TS / NB; ++b) {
+        const int b0 = NB * b;
+        if (tid < 64) {   // wave 0: diagonal block factor + inverse in registers
+            const int i = tid & (NB - 1);
+            double p[NB];
 #pragma unroll
-    for (int c = 0; c < TS; ++c) row[c] = (c < kb && r < kb && c <= r) ? D[c * ld + r] : 0.0;
+            for (int c = 0; c < NB; ++c) p[c] = (c <= i) ? a[b0 + c][b0 + i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < TS; ++j) {
-        if (j < kb) {
-            col[r] = row[j];                       // column j (rows >= j valid)
-            __syncthreads();
-            double d = col[j];
-            if (!(d > 0.0)) {
-                if (r == 0) atomicMin(status, t.col0 + j);
-                d = 1.0;
+            for (int j = 0; j < NB; ++j) {
+                double d = readlane_d(p[j], j);
+                if (!(d > 0.0)) {
+                    if (tid == 0) atomicMin(status, t.col0 + b0 + j);
+                    d = 1.0;
+                }
+                const double sd = sqrt(d);
+                const double l = (i > j) ? p[j] / sd : (i == j ? sd : 0.0);
+                p[j] = l;
+#pragma unroll
+                for (int c = j + 1; c < NB; ++c) {
+                    const double lc = readlane_d(l, c);
+                    if (i >= c) p[c] -= l * lc;
+                }
             }
-            d = sqrt(d);
-            const double l = (r > j) ? row[j] / d : (r == j ? d : row[j]);
-            row[j] = l;
-            __syncthreads();
-            col[r] = l;                            // l_rj for r > j
-            __syncthreads();
+            // X_bb column i: X(r, i) = (delta_ri - sum_{k<r} L(r, k) X(k, i)) / L(r, r)
+            double xc[NB];
 #pragma unroll
-            for (int c = j + 1; c < TS; ++c)
-                if (c <= r && c < kb) row[c] -= l * col[c];
-            __syncthreads();
+            for (int r = 0; r < NB; ++r) {
+                double sacc = (r == i) ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < r; ++k) sacc -= readlane_d(p[k], r) * xc[k];
+                xc[r] = sacc / readlane_d(p[r], r);
+            }
+            if (tid < NB) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) a[b0 + c][b0 + i] = (c <= i) ? p[c] : 0.0;
+#pragma unroll
+                for (int r = 0; r < NB; ++r) x[b0 + i][b0 + r] = (r >= i) ? xc[r] : 0.0;
+            }
         }
+        __syncthreads();
+        // panel TRSM: L(r, b0+c) = sum_k A(r, b0+k) X(b0+c, b0+k), rows r >= b0+16
+        const int nrow = TS - b0 - NB;
+        double tv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int idx = tid + 256 * q;
+            tv[q] = 0.0;
+            if (idx < nrow * NB) {
+                const int r = b0 + NB + (idx % nrow), c = idx / nrow;
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * x[b0 + k][b0 + c];
+                tv[q] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int idx = tid + 256 * q;
+            if (idx < nrow * NB) {
+                const int r = b0 + NB + (idx % nrow), c = idx / nrow;
+                a[b0 + c][r] = tv[q];
+            }
+        }
+        __syncthreads();
+        // trailing SYRK: A(r, c) -= sum_k L(r, b0+k) L(c, b0+k) for b0+16 <= c <= r
+        for (int idx = tid; idx < nrow * nrow; idx += 256) {
+            const int r = b0 + NB + (idx % nrow), c = b0 + NB + (idx / nrow);
+            if (r < c) continue;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * a[b0 + k][c];
+            a[c][r] -= acc;
+        }
+        __syncthreads();
     }
+    // off-diagonal blocks of X by block rows bi = 1..3: T = sum_{k=bj}^{bi-1} L_{bi,k} X_{k,bj},
+    // X_{bi,bj} = -X_{bi,bi} T  (all bj < bi at once)
+    for (int bi = 1; bi < TS / NB; ++bi) {
+        const int n = bi * NB * NB;           // bj < bi blocks of 16x16
+        double tv[3];
 #pragma unroll
-    for (int c = 0; c < TS; ++c) {
-        if (c < kb && r < kb && c <= r) D[c * ld + r] = row[c];
-        lt[c][r] = (c <= r) ? row[c] : 0.0;
+        for (int q = 0; q < 3; ++q) {
+            const int idx = tid + 256 * q;
+            tv[q] = 0.0;
+            if (idx < n) {
+                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
+                const int r = bi * NB + rr, c = bj * NB + cc;
+                double acc = 0.0;
+                for (int k = bj * NB; k < bi * NB; ++k) acc += a[k][r] * x[c][k];
+                tv[q] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {       // stash T in the (still zero) X block
+            const int idx = tid + 256 * q;
+            if (idx < n) {
+                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
+                x[bj * NB + cc][bi * NB + rr] = tv[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int idx = tid + 256 * q;
+            tv[q] = 0.0;
+            if (idx < n) {
+                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
+                const int r = bi * NB + rr, c = bj * NB + cc;
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) acc += x[bi * NB + k][r] * x[c][bi * NB + k];
+                tv[q] = -acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int idx = tid + 256 * q;
+            if (idx < n) {
+                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
+                x[bj * NB + cc][bi * NB + rr] = tv[q];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    // inverse X = L^{-1}; lane c owns column c: X[i][c] = (delta_ic - sum_{k<i} L(i,k) X[k][c]) / L(i,i)
-    double x[TS];
-#pragma unroll
-    for (int i = 0; i < TS; ++i) {
-        double sacc = (i == r) ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = 0; k < i; ++k) sacc -= lt[k][i] * x[k];
-        x[i] = (i < kb) ? sacc / lt[i][i] : 0.0;
+    for (int idx = tid; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        if (r < kb && c < kb && r >= c) D[c * ld + r] = a[c][r];
     }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < TS; ++i) lt[r][i] = (r < kb) ? x[i] : 0.0;     // lt[c][i] = X[i][c]
-    __syncthreads();
     double* out = inv + (int64_t)t.slot * TS * TS;
-    for (int c = 0; c < TS; ++c) out[c * TS + r] = lt[c][r];
+    for (int idx = tid; idx < TS * TS; idx += 256) {
+        const int r = idx & 63, c = idx >> 6;
+        out[idx] = (r < kb && c < kb) ? x[c][r] : 0.0;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -756,6 +867,11 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
                              const int64_t* pins, int64_t npins) {
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
     SymbolicOptions opt;
+    // tuning knobs: MFS_OUTER (outer panel block, multiple of 64), MFS_SOLVE_CHUNK (backward
+    // partial-sum rows per task, multiple of 64), MFS_LEAF (nested-dissection leaf size)
+    if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
+    if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::max(64, atoi(e) / 64 * 64);
+    if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
     auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, opt));
     ftype = 0;
     stash_valid_ = false;
@@ -850,8 +966,8 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];
             if (p1 > p0) {
-                prof.begin(KernelProfile::POTRF, s);
-                k_potrf<<<(unsigned)(p1 - p0), 64, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
+                prof.begin(KernelProfile::POTRF, s, 0.0, p1 - p0);
+                k_potrf<<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
                                                            status_.as<int>());
                 LAUNCH_CHECK();
                 prof.end(s);

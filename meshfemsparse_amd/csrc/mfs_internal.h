@@ -214,9 +214,9 @@ struct SymbolicOptions {
     int nrelax0 = 12, nrelax1 = 48, nrelax2 = 192;
     double zrelax0 = 0.8, zrelax1 = 0.2, zrelax2 = 0.05;
     int tile = 64;          // diagonal tile / inner step width (scalars)
-    int outer = 256;        // outer panel block: trailing update with K = outer
+    int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
-    int solve_chunk = 512;  // backward partial-sum rows per task
+    int solve_chunk = 256;  // backward partial-sum rows per task
 };
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
