@@ -181,9 +181,14 @@ def main():
         achieved = flops_per_launch / avg_launch_s / 1e12
     else:
         avg_launch_s, achieved = None, factor_tflops
+    # HBM bytes per k_gemm launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes on this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_gemm_traffic_64cube_fp64.json")
+    if N == 64 and os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     roofline = {"bound": "mfma", "kernel": "k_gemm (FP64 MFMA tile GEMM: TRSM, in-panel update, Schur update)",
                 "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "avg_launch_ms": None if avg_launch_s is None else avg_launch_s * 1e3,
                 "launches_per_step": gemm_launches / max(args.steps, 1)}
     kernel_ms = {k: round(float(m) / args.steps, 3) for k, m in zip(KCLASSES, ms) if cnt[KCLASSES.index(k)]}
