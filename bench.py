@@ -7,7 +7,13 @@ covers Ax := 0, assembly, value injection + numeric factorization, solve + refin
 analysis (ordering, supernodes, schedule) runs once before the timed region and is reported
 separately.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--no-cpu]
+With --precision fp32 the factorization runs in FP32 on MFMA after symmetric diagonal scaling and
+FP64 iterative refinement brings the residual to <= 1e-10; --stress draws per-element E
+log-uniform in [1e3, 1e7] and nu uniform in [0.2, 0.45] (the conditioning stress case), and the
+forward error against the FP64 factor's solution is reported per step.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--precision fp64|fp32] [--stress]
+                  [--no-cpu]
 """
 import argparse
 import json
@@ -20,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 PEAK_HBM_GBS = 8000.0
 # kernel classes of mfs_kernel_profile
 KCLASSES = ["fill_L", "inject", "zero_U", "extend", "potrf", "trsm", "update", "syrk", "fgather",
@@ -85,6 +91,11 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--stress", action="store_true",
+                    help="per-element E log-uniform [1e3, 1e7], nu uniform [0.2, 0.45]")
+    ap.add_argument("--forward-error", action="store_true",
+                    help="also factor in FP64 and report the forward error of every step (fp32)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding of the factorization "
@@ -116,15 +127,21 @@ def main():
     # inputs for every step, resident in HBM
     He_d, b_d = [], []
     t0 = time.perf_counter()
+    Ee = nue = None
+    if args.stress:
+        rng = np.random.default_rng(12345)
+        Ee = 10 ** rng.uniform(3, 7, len(tets))
+        nue = rng.uniform(0.2, 0.45, len(tets))
     for s in range(nseeds):
         X = wl.perturb(N, Xr, seed=s)
-        He, g, _ = wl.neohookean(Xr, X, tets)
+        He, g, _ = wl.neohookean(Xr, X, tets, E_per_elem=Ee, nu_per_elem=nue)
         He_d.append(torch.from_numpy(He.ravel()).to(dev))
         b_d.append(torch.from_numpy(-g).to(dev))
         del He, g
     torch.cuda.synchronize()
     log(f"[bench] generated + uploaded {nseeds} seeds in {time.perf_counter() - t0:.1f}s")
-    f = mfs.make_cholesky_factorizer()
+    single = args.precision == "fp32"
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
     t0 = time.perf_counter()
     f.factorizeSymbolic(H, pins)
     t_sym = time.perf_counter() - t0
@@ -142,9 +159,9 @@ def main():
         e[1].record()
         f.factorizeNumeric(H)
         e[2].record()
-        x, iters, rr = f.solveRefined(H, b_d[s], tol=1e-10, maxIters=10)
+        x, iters, rr = f.solveRefined(H, b_d[s], tol=1e-10, maxIters=30 if single else 10)
         e[3].record()
-        return e, iters, rr
+        return e, iters, rr, x
 
     for s in range(args.warmup):
         step(s)
@@ -168,6 +185,22 @@ def main():
     t_sol = np.mean([r[0][2].elapsed_time(r[0][3]) for r in recs]) / 1e3
     iters = [r[1] for r in recs]
     rrs = [r[2] for r in recs]
+    fwd_err = None
+    if single and args.forward_error:
+        # forward error of each step's refined FP32-factor solution against the FP64 factor's solve
+        xs = [r[3].clone() for r in recs]
+        del recs
+        f64 = mfs.make_cholesky_factorizer()
+        f64.factorizeSymbolic(H, pins)
+        fwd_err = []
+        for k in range(args.steps):
+            dasm.zero(H.Ax)
+            dasm.assemble(He_d[args.warmup + k], H.Ax)
+            f64.factorizeNumeric(H)
+            x64, _, _ = f64.solveRefined(H, b_d[args.warmup + k], tol=1e-12, maxIters=5)
+            fwd_err.append(float(torch.linalg.norm(xs[k] - x64) / torch.linalg.norm(x64)))
+        del f64
+        log(f"[bench] forward errors vs FP64 factor: {fwd_err}")
     s_per_step = wall / args.steps
     factor_tflops = flops / t_fac / 1e12
     # roofline of the dominant kernel (tile GEMM: TRSM + in-panel updates + Schur updates)
@@ -183,12 +216,14 @@ def main():
         avg_launch_s, achieved = None, factor_tflops
     # HBM bytes per k_gemm launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes on this workload
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_gemm_traffic_64cube_fp64.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_gemm_traffic_64cube_{args.precision}.json")
     if N == 64 and os.path.exists(tpath):
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
-    roofline = {"bound": "mfma", "kernel": "k_gemm (FP64 MFMA tile GEMM: TRSM, in-panel update, Schur update)",
-                "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+    peak = PEAK_TFLOPS[args.precision]
+    roofline = {"bound": "mfma",
+                "kernel": f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)",
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": traffic,
                 "avg_launch_ms": None if avg_launch_s is None else avg_launch_s * 1e3,
                 "launches_per_step": gemm_launches / max(args.steps, 1)}
     kernel_ms = {k: round(float(m) / args.steps, 3) for k, m in zip(KCLASSES, ms) if cnt[KCLASSES.index(k)]}
@@ -205,13 +240,18 @@ def main():
         "metric": "Newton linear-solve s/step @811k DOF to <=1e-10 rel. residual; factor TFLOP/s",
         "value": s_per_step, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": s_per_step * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He on host)",
-        "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean E=1e5 nu=0.3, sigma=1e-3 h, z=0 pinned; "
-                               "assembly + FP64 supernodal Cholesky + solve/refine to 1e-10",
+        "dtype": "f64" if not single else "f32 factor/solve, f64 residual+refinement",
+        "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He on host)",
+        "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean "
+                               + ("E log-uniform [1e3,1e7] nu U[0.2,0.45]" if args.stress else "E=1e5 nu=0.3")
+                               + ", sigma=1e-3 h, z=0 pinned; assembly + "
+                               + ("FP32 scaled supernodal Cholesky + FP64 refinement to 1e-10" if single
+                                  else "FP64 supernodal Cholesky + solve/refine to 1e-10"),
                    "free_dofs": nred, "tets": len(tets), "factor_nnz": f.getFactorNNZ(), "factor_flops": flops,
                    "supernodes": st["num_supernodes"], "levels": st["num_levels"]},
         "factor_tflops": factor_tflops, "t_assembly_s": t_asm, "t_factor_s": t_fac, "t_solve_s": t_sol,
-        "refine_iters": iters, "relres_max": max(rrs), "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms,
+        "refine_iters": iters, "relres_max": max(rrs), "forward_error_vs_fp64": fwd_err,
+        "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

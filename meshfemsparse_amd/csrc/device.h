@@ -84,14 +84,17 @@ public:
 
 private:
     void solve_permuted_inplace(double* xp, int nrhs, void* stream);   // factor-order vectors
+    template <typename T> void numeric_t(const double* Ax, double sigma, const double* Bx, void* stream);
+    template <typename T> void solve_t(T* xp, int nrhs, void* stream);
     void ensure_vec(DeviceArray& a, size_t bytes);
+    size_t fsize() const;
 
     DeviceArray L_, U_, inv_, status_;
-    DeviceArray planent_, rel_, ext_, potrf_, gemm_, zr_;
+    DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, zr_;
     DeviceArray fg_, fs_, bd_, bv_, children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;
     DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
-    DeviceArray Lstash_, invstash_;
+    DeviceArray Lstash_, invstash_, scalestash_;
     bool stash_valid_ = false;
     int stash_type_ = 0;
     int64_t stash_succ_ = 0;

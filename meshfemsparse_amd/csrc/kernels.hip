@@ -107,23 +107,26 @@ KernelProfile::~KernelProfile() {
 // ------------------------------------------------------------------------------------------
 // Generic small kernels
 // ------------------------------------------------------------------------------------------
-__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-__global__ void k_zero_ranges(double* __restrict__ base, const int64_t* __restrict__ ranges) {
+template <typename T>
+__global__ void k_zero_ranges(T* __restrict__ base, const int64_t* __restrict__ ranges) {
     const int64_t off = ranges[2 * blockIdx.y], cnt = ranges[2 * blockIdx.y + 1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt;
          i += (int64_t)gridDim.x * blockDim.x)
-        base[off + i] = 0.0;
+        base[off + i] = T(0);
 }
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
-static void fill(double* p, int64_t n, double v, hipStream_t s) {
+template <typename T>
+static void fill(T* p, int64_t n, T v, hipStream_t s) {
     if (n <= 0) return;
     const unsigned g = (unsigned)std::min<int64_t>(nblk(n, 256), 8192);
-    k_fill<<<g, 256, 0, s>>>(p, n, v);
+    k_fill<T><<<g, 256, 0, s>>>(p, n, v);
     LAUNCH_CHECK();
 }
 
@@ -210,72 +213,80 @@ void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stre
 void AssemblerDev::zero(double* Ax, void* stream) { fill(Ax, 9 * nslots_, 0.0, S_(stream)); }
 
 // ------------------------------------------------------------------------------------------
-// Value injection: L[dst] = block (as-is / transposed / lower of diagonal) + shift.
+// Value injection: L[dst] = block (as-is / transposed / lower of diagonal) + shift, optionally
+// scaled symmetrically, s_i s_j (A + shift)_ij, and converted to the factor type.
 // ------------------------------------------------------------------------------------------
 struct PlanEnt {
     int64_t dstmode;
     int32_t src;
     int32_t ld;
+    int32_t rb, cb;     // factor-order block row / column of the lower-triangle block
 };
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan, int64_t n,
                                                 const double* __restrict__ Ax,
                                                 const double* __restrict__ Bx, double sigma,
-                                                double* __restrict__ L) {
+                                                const double* __restrict__ scale,
+                                                T* __restrict__ L) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= n) return;
     const PlanEnt pe = plan[q];
     const int mode = (int)(pe.dstmode & 3);
-    double* d = L + (pe.dstmode >> 2);
+    T* d = L + (pe.dstmode >> 2);
     const double* a = Ax + 9 * (int64_t)pe.src;
+    const double* bb = Bx ? Bx + 9 * (int64_t)pe.src : nullptr;
     const int64_t ld = pe.ld;
-    if (mode == 0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) d[c * ld + r] = a[3 * c + r];
-    } else if (mode == 1) {
+        for (int r = 0; r < 3; ++r) {
+            if (mode == 2 && r < c) continue;
+            const int srcidx = (mode == 0) ? 3 * c + r : 3 * r + c;
+            double v = a[srcidx];
+            if (bb) v += sigma * bb[srcidx];
+            else if (mode == 2 && r == c) v += sigma;
+            if (scale) v *= scale[3 * (int64_t)pe.rb + r] * scale[3 * (int64_t)pe.cb + c];
+            d[c * ld + r] = (T)v;
+        }
+}
+
+// s_k = 1 / sqrt((A + shift)_kk) in factor order; a non-positive diagonal is a pivot failure
+__global__ void k_diag_scale(const int32_t* __restrict__ diag_src, int64_t nblocks,
+                             const double* __restrict__ Ax, const double* __restrict__ Bx,
+                             double sigma, double* __restrict__ scale, int* __restrict__ status) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= nblocks) return;
+    const int64_t src = 9 * (int64_t)diag_src[k];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int r = 0; r < 3; ++r) d[c * ld + r] = a[3 * r + c];
-    } else {
-        const double* bb = Bx ? Bx + 9 * (int64_t)pe.src : nullptr;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int r = c; r < 3; ++r) {
-                double v = a[3 * r + c];
-                if (bb) v += sigma * bb[3 * r + c];
-                else if (r == c) v += sigma;
-                d[c * ld + r] = v;
-            }
-    }
-    if (mode != 2 && Bx) {
-        const double* bb = Bx + 9 * (int64_t)pe.src;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                d[c * ld + r] += sigma * (mode == 0 ? bb[3 * c + r] : bb[3 * r + c]);
+    for (int c = 0; c < 3; ++c) {
+        double d = 0.0;
+        if (src >= 0) d = Ax[src + 4 * c] + (Bx ? sigma * Bx[src + 4 * c] : sigma);
+        else if (!Bx) d = sigma;
+        if (!(d > 0.0)) {
+            atomicMin(status, (int)(3 * k + c));
+            d = 1.0;
+        }
+        scale[3 * k + c] = 1.0 / sqrt(d);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // Extend-add: child's U (lower 3x3 blocks of column block kj) into the parent front.
 // ------------------------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ tasks,
                                                const int32_t* __restrict__ rel,
-                                               double* __restrict__ U, double* __restrict__ L) {
+                                               T* __restrict__ U, T* __restrict__ L) {
     const ExtendTask t = tasks[blockIdx.x];
     const int32_t* relc = rel + t.rel_off;
     const int kj = t.col_block;
     const int64_t Dc = 3 * (int64_t)t.dc;
     const int tj = relc[kj];
-    const double* uc = U + t.u_child;
+    const T* uc = U + t.u_child;
     for (int ki = kj + (int)threadIdx.x; ki < t.dc; ki += 64) {
         const int ti = relc[ki];
-        double* dst;
+        T* dst;
         int64_t ldd;
         if (tj < t.wp) {
             dst = L + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti;
@@ -285,7 +296,7 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
             dst = U + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
             ldd = Dp;
         }
-        const double* src = uc + (3 * (int64_t)kj) * Dc + 3 * (int64_t)ki;
+        const T* src = uc + (3 * (int64_t)kj) * Dc + 3 * (int64_t)ki;
         if (ki == kj) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -307,87 +318,88 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 // ------------------------------------------------------------------------------------------
 constexpr int TS = 64;   // diagonal tile size (scalars)
 
-__device__ __forceinline__ double readlane_d(double v, int lane) {
+__device__ __forceinline__ double readlane_t(double v, int lane) {
     const long long bits = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), lane);
     const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ float readlane_t(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
+__device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 
-// Blocked in 16-column panels.  Per panel b: wave 0 factors the 16x16 diagonal block and inverts
-// it in registers (lane i = row i; broadcasts by readlane, no LDS round trips), then all 256
-// threads apply the panel TRSM (L_ib = A_ib X_bb^T) and the trailing SYRK through LDS.  The
-// full inverse follows by block rows: X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.  Rows/columns beyond
-// kb are padded with the identity, so they factor and invert to the identity and are masked out.
+template <typename T>
 __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
-                                               double* __restrict__ L, double* __restrict__ inv,
+                                               T* __restrict__ L, T* __restrict__ inv,
                                                int* __restrict__ status) {
     constexpr int NB = 16;
     const PotrfTask t = tasks[blockIdx.x];
-    __shared__ double a[TS][TS + 1];    // a[c][r] = tile(r, c) -> L(r, c)
-    __shared__ double x[TS][TS + 1];    // x[c][r] = X(r, c), X = L^{-1}
+    __shared__ T a[TS][TS + 1];         // a[c][r] = tile(r, c) -> L(r, c)
+    __shared__ T x[TS][TS + 1];         // x[c][r] = X(r, c), X = L^{-1}
     const int tid = threadIdx.x, kb = t.kb;
     const int64_t ld = t.ld;
-    double* D = L + t.diag_off;
+    T* D = L + t.diag_off;
     for (int idx = tid; idx < TS * TS; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
-        double v;
-        if (r < kb && c < kb) v = (r >= c) ? D[c * ld + r] : 0.0;
-        else v = (r == c) ? 1.0 : 0.0;
+        T v;
+        if (r < kb && c < kb) v = (r >= c) ? D[c * ld + r] : T(0);
+        else v = (r == c) ? T(1) : T(0);
         a[c][r] = v;
-        x[c][r] = 0.0;
+        x[c][r] = T(0);
     }
     __syncthreads();
     for (int b = 0; b < TS / NB; ++b) {
         const int b0 = NB * b;
         if (tid < 64) {   // wave 0: diagonal block factor + inverse in registers
             const int i = tid & (NB - 1);
-            double p[NB];
+            T p[NB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) p[c] = (c <= i) ? a[b0 + c][b0 + i] : 0.0;
+            for (int c = 0; c < NB; ++c) p[c] = (c <= i) ? a[b0 + c][b0 + i] : T(0);
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                double d = readlane_d(p[j], j);
-                if (!(d > 0.0)) {
+                T d = readlane_t(p[j], j);
+                if (!(d > T(0))) {
                     if (tid == 0) atomicMin(status, t.col0 + b0 + j);
-                    d = 1.0;
+                    d = T(1);
                 }
-                const double sd = sqrt(d);
-                const double l = (i > j) ? p[j] / sd : (i == j ? sd : 0.0);
+                const T sd = sqrt_t(d);
+                const T l = (i > j) ? p[j] / sd : (i == j ? sd : T(0));
                 p[j] = l;
 #pragma unroll
                 for (int c = j + 1; c < NB; ++c) {
-                    const double lc = readlane_d(l, c);
+                    const T lc = readlane_t(l, c);
                     if (i >= c) p[c] -= l * lc;
                 }
             }
             // X_bb column i: X(r, i) = (delta_ri - sum_{k<r} L(r, k) X(k, i)) / L(r, r)
-            double xc[NB];
+            T xc[NB];
 #pragma unroll
             for (int r = 0; r < NB; ++r) {
-                double sacc = (r == i) ? 1.0 : 0.0;
+                T sacc = (r == i) ? T(1) : T(0);
 #pragma unroll
-                for (int k = 0; k < r; ++k) sacc -= readlane_d(p[k], r) * xc[k];
-                xc[r] = sacc / readlane_d(p[r], r);
+                for (int k = 0; k < r; ++k) sacc -= readlane_t(p[k], r) * xc[k];
+                xc[r] = sacc / readlane_t(p[r], r);
             }
             if (tid < NB) {
 #pragma unroll
-                for (int c = 0; c < NB; ++c) a[b0 + c][b0 + i] = (c <= i) ? p[c] : 0.0;
+                for (int c = 0; c < NB; ++c) a[b0 + c][b0 + i] = (c <= i) ? p[c] : T(0);
 #pragma unroll
-                for (int r = 0; r < NB; ++r) x[b0 + i][b0 + r] = (r >= i) ? xc[r] : 0.0;
+                for (int r = 0; r < NB; ++r) x[b0 + i][b0 + r] = (r >= i) ? xc[r] : T(0);
             }
         }
         __syncthreads();
         // panel TRSM: L(r, b0+c) = sum_k A(r, b0+k) X(b0+c, b0+k), rows r >= b0+16
         const int nrow = TS - b0 - NB;
-        double tv[3];
+        T tv[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int idx = tid + 256 * q;
-            tv[q] = 0.0;
+            tv[q] = T(0);
             if (idx < nrow * NB) {
                 const int r = b0 + NB + (idx % nrow), c = idx / nrow;
-                double acc = 0.0;
+                T acc = T(0);
 #pragma unroll
                 for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * x[b0 + k][b0 + c];
                 tv[q] = acc;
@@ -407,7 +419,7 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
         for (int idx = tid; idx < nrow * nrow; idx += 256) {
             const int r = b0 + NB + (idx % nrow), c = b0 + NB + (idx / nrow);
             if (r < c) continue;
-            double acc = 0.0;
+            T acc = T(0);
 #pragma unroll
             for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * a[b0 + k][c];
             a[c][r] -= acc;
@@ -418,15 +430,15 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
     // X_{bi,bj} = -X_{bi,bi} T  (all bj < bi at once)
     for (int bi = 1; bi < TS / NB; ++bi) {
         const int n = bi * NB * NB;           // bj < bi blocks of 16x16
-        double tv[3];
+        T tv[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int idx = tid + 256 * q;
-            tv[q] = 0.0;
+            tv[q] = T(0);
             if (idx < n) {
                 const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
                 const int r = bi * NB + rr, c = bj * NB + cc;
-                double acc = 0.0;
+                T acc = T(0);
                 for (int k = bj * NB; k < bi * NB; ++k) acc += a[k][r] * x[c][k];
                 tv[q] = acc;
             }
@@ -444,11 +456,11 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int idx = tid + 256 * q;
-            tv[q] = 0.0;
+            tv[q] = T(0);
             if (idx < n) {
                 const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
                 const int r = bi * NB + rr, c = bj * NB + cc;
-                double acc = 0.0;
+                T acc = T(0);
 #pragma unroll
                 for (int k = 0; k < NB; ++k) acc += x[bi * NB + k][r] * x[c][bi * NB + k];
                 tv[q] = -acc;
@@ -469,57 +481,75 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
         const int r = idx & 63, c = idx >> 6;
         if (r < kb && c < kb && r >= c) D[c * ld + r] = a[c][r];
     }
-    double* out = inv + (int64_t)t.slot * TS * TS;
+    T* out = inv + (int64_t)t.slot * TS * TS;
     for (int idx = tid; idx < TS * TS; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
-        out[idx] = (r < kb && c < kb) ? x[c][r] : 0.0;
+        out[idx] = (r < kb && c < kb) ? x[c][r] : T(0);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // Batched tile GEMM, C = alpha A B^T + beta C, TM x TN output tile per 256-thread workgroup
-// (2x2 waves), FP64 MFMA 16x16x4, K staged through double-buffered LDS in chunks of 16.
-// MFMA f64 16x16x4 operand layout on gfx950 (measured): A[i][k] in lane i + 16k, B[k][j] in
-// lane j + 16k, D[i][j] in lane j + 16 (i % 4), register i / 4 (rows interleaved across the four
-// 16-lane groups; the f32 16x16x4 variant instead uses lane j + 16 (i / 4), register i % 4).
+// (2x2 waves), MFMA 16x16x4 (FP64 or FP32), K staged through double-buffered LDS in chunks of 16.
+// Operand layouts on gfx950 (measured): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k for both;
+// D[i][j] in lane j + 16 (i % 4), register i / 4 for f64_16x16x4 (rows interleaved across the four
+// 16-lane groups), and in lane j + 16 (i / 4), register i % 4 for f32_16x16x4.
 // ------------------------------------------------------------------------------------------
 constexpr int KC = 16;
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
 // (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
-template <int TM, int TN>
+template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
-                                              double* __restrict__ L, double* __restrict__ U,
-                                              const double* __restrict__ inv) {
+                                                 T* __restrict__ L, T* __restrict__ U,
+                                                 const T* __restrict__ inv) {
+    typedef typename Mfma<T>::acc_t acc_t;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int AST = 256 / TM, BST = 256 / TN;           // k stride of the loaders
     constexpr int AL = KC / AST, BL = KC / BST;              // loads per thread per chunk
     const GemmTask t = tasks[blockIdx.x];
-    const double* A = L + t.a_off;
-    const double* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
-    double* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
-    __shared__ double As[2][KC][TM];
-    __shared__ double Bs[2][KC][TN];
+    const T* A = L + t.a_off;
+    const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
+    T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
+    __shared__ T As[2][KC][TM];
+    __shared__ T Bs[2][KC][TN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
-    d4 acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     const int ar = tid % TM, ak = tid / TM, br = tid % TN, bk = tid / TN;
     const int64_t lda = t.lda, ldb = t.ldb;
     const bool arow = ar < t.M, brow = br < t.N;
-    double ra[AL], rb[BL];
+    T ra[AL], rb[BL];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < AL; ++q) {
             const int kk = k0 + ak + AST * q;
-            ra[q] = (arow && kk < t.K) ? A[kk * lda + ar] : 0.0;
+            ra[q] = (arow && kk < t.K) ? A[kk * lda + ar] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < BL; ++q) {
             const int kk = k0 + bk + BST * q;
-            rb[q] = (brow && kk < t.K) ? B[kk * ldb + br] : 0.0;
+            rb[q] = (brow && kk < t.K) ? B[kk * ldb + br] : T(0);
         }
     };
     auto sstore = [&](int buf) {
@@ -538,7 +568,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kr = ks + (lane >> 4), cc = lane & 15;
-            double af[FM], bf[FN];
+            T af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = As[buf][kr][wm + 16 * i + cc];
 #pragma unroll
@@ -546,17 +576,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
         }
         if (more) sstore(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
-    const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
+    const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
     const bool beta0 = (t.flags & GT_BETA0) != 0;
     const int64_t ldc = t.ldc;
-    const int col_l = lane & 15, row_l = lane >> 4;
+    const int col_l = lane & 15;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -565,10 +594,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
             if (col >= t.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = wm + 16 * i + 4 * r + row_l;
+                const int row = wm + 16 * i + Mfma<T>::row(lane, r);
                 if (row < t.M) {
-                    double* p = C + col * ldc + row;
-                    const double v = alpha * acc[i][j][r];
+                    T* p = C + col * ldc + row;
+                    const T v = alpha * acc[i][j][r];
                     *p = beta0 ? v : (*p + v);
                 }
             }
@@ -585,43 +614,44 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     return w;
 }
 
-static void launch_gemm(const TaskRange& r, const GemmTask* tasks, double* L, double* U, const double* inv,
-                        hipStream_t s) {
+template <typename T>
+static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
     if (r.m > r.b) {
-        k_gemm<128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv);
+        k_gemm<T, 128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv);
         LAUNCH_CHECK();
     }
     if (r.e > r.m) {
-        k_gemm<64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv);
+        k_gemm<T, 64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv);
         LAUNCH_CHECK();
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// Solve kernels (x = factor-order reduced vector; one RHS per blockIdx.y)
+// Solve kernels (x = factor-order reduced vector of the factor type; one RHS per blockIdx.y)
 // ------------------------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restrict__ tasks,
                                                  const int32_t* __restrict__ children,
                                                  const int32_t* __restrict__ sdeg,
                                                  const int64_t* __restrict__ rel_off,
                                                  const int32_t* __restrict__ rel,
                                                  const int64_t* __restrict__ usoff,
-                                                 double* __restrict__ xall, int64_t ldx,
-                                                 double* __restrict__ uall, int64_t ldu) {
+                                                 T* __restrict__ xall, int64_t ldx,
+                                                 T* __restrict__ uall, int64_t ldu) {
     const SolveGatherTask t = tasks[blockIdx.x];
-    double* x = xall + blockIdx.y * ldx;
-    double* u = uall + blockIdx.y * ldu;
-    double* us = u + t.u_off;
-    for (int i = threadIdx.x; i < t.d; i += blockDim.x) us[i] = 0.0;
+    T* x = xall + blockIdx.y * ldx;
+    T* u = uall + blockIdx.y * ldu;
+    T* us = u + t.u_off;
+    for (int i = threadIdx.x; i < t.d; i += blockDim.x) us[i] = T(0);
     __syncthreads();
     for (int q = t.child_begin; q < t.child_end; ++q) {
         const int c = children[q];
         const int Dc = 3 * sdeg[c];
-        const double* uc = u + usoff[c];
+        const T* uc = u + usoff[c];
         const int32_t* relc = rel + rel_off[c];
         for (int k = threadIdx.x; k < Dc; k += blockDim.x) {
             const int tgt = 3 * relc[k / 3] + k % 3;
-            const double v = uc[k];
+            const T v = uc[k];
             if (tgt < t.w) x[t.x_off + tgt] -= v;
             else us[tgt - t.w] += v;
         }
@@ -630,9 +660,9 @@ __global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restri
 }
 
 // dot of panel row r (columns 0..kb-1, leading dimension ld) with y held in LDS
-__device__ __forceinline__ double panel_row_dot(const double* __restrict__ P, int64_t ld, int kb,
-                                                const double* y, int r) {
-    double s = 0.0;
+template <typename T>
+__device__ __forceinline__ T panel_row_dot(const T* __restrict__ P, int64_t ld, int kb, const T* y, int r) {
+    T s = T(0);
     for (int c = 0; c < kb; ++c) s += P[c * ld + r] * y[c];
     return s;
 }
@@ -640,20 +670,20 @@ __device__ __forceinline__ double panel_row_dot(const double* __restrict__ P, in
 // Forward step for column block k of a supernode: y_k = L_kk^{-1} x_k (stored tile inverse),
 // then rows [r0, r0 + nr) below: x (rows < w) -= L(rows, k) y_k, u (rows >= w) += L(rows, k) y_k.
 // The writer task (nr == 0) stores y_k into Y; no task of the launch writes x_k.
+template <typename T>
 __global__ __launch_bounds__(64) void k_fstep(const FStepTask* __restrict__ tasks,
-                                              const double* __restrict__ L,
-                                              const double* __restrict__ inv,
-                                              double* __restrict__ xall, double* __restrict__ yall,
-                                              int64_t ldx, double* __restrict__ uall, int64_t ldu) {
+                                              const T* __restrict__ L, const T* __restrict__ inv,
+                                              T* __restrict__ xall, T* __restrict__ yall,
+                                              int64_t ldx, T* __restrict__ uall, int64_t ldu) {
     const FStepTask t = tasks[blockIdx.x];
-    double* x = xall + blockIdx.y * ldx;
-    double* u = uall + blockIdx.y * ldu;
-    __shared__ double xs[TS], y[TS];
+    T* x = xall + blockIdx.y * ldx;
+    T* u = uall + blockIdx.y * ldu;
+    __shared__ T xs[TS], y[TS];
     const int tid = threadIdx.x;
-    xs[tid] = tid < t.kb ? x[t.x_blk + tid] : 0.0;
+    xs[tid] = tid < t.kb ? x[t.x_blk + tid] : T(0);
     __syncthreads();
-    const double* Vi = inv + t.inv_off;
-    double v = 0.0;
+    const T* Vi = inv + t.inv_off;
+    T v = T(0);
     for (int j = 0; j < t.kb; ++j) v += Vi[j * TS + tid] * xs[j];
     y[tid] = v;
     __syncthreads();
@@ -662,34 +692,36 @@ __global__ __launch_bounds__(64) void k_fstep(const FStepTask* __restrict__ task
         return;
     }
     if (tid >= t.nr) return;
-    const double d = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
+    const T d = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
     const int row = t.r0 + tid;
     if (row < t.w) x[t.x_rows + row] -= d;
     else u[t.u_off + row - t.w] += d;
 }
 
+// Backward partial sums: part[p][c] = sum over the task's rows of L(row, k0 + c) x(row)
+template <typename T>
 __global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__ tasks,
-                                               const double* __restrict__ L,
+                                               const T* __restrict__ L,
                                                const int32_t* __restrict__ rows,
-                                               const double* __restrict__ xall, int64_t ldx,
-                                               double* __restrict__ partall, int64_t ldp) {
+                                               const T* __restrict__ xall, int64_t ldx,
+                                               T* __restrict__ partall, int64_t ldp) {
     const SolveGemvTask t = tasks[blockIdx.x];
-    const double* x = xall + blockIdx.y * ldx;
-    double* part = partall + blockIdx.y * ldp;
-    __shared__ double T[TS][TS + 1];   // T[c][i]
-    __shared__ double xv[TS];
-    __shared__ double red[4][TS];
+    const T* x = xall + blockIdx.y * ldx;
+    T* part = partall + blockIdx.y * ldp;
+    __shared__ T Tt[TS][TS + 1];   // Tt[c][i]
+    __shared__ T xv[TS];
+    __shared__ T red[4][TS];
     const int tid = threadIdx.x, ci = tid & 63, g = tid >> 6;
-    double acc = 0.0;
-    const double* P = L + t.l_off;
+    T acc = T(0);
+    const T* P = L + t.l_off;
     for (int rb = 0; rb < t.nr; rb += TS) {
         const int nrb = min(TS, t.nr - rb);
         for (int q = 0; q < 16; ++q) {
             const int c = g + 4 * q;
-            if (c < t.kb) T[c][ci] = (ci < nrb) ? P[(int64_t)c * t.ld + rb + ci] : 0.0;
+            if (c < t.kb) Tt[c][ci] = (ci < nrb) ? P[(int64_t)c * t.ld + rb + ci] : T(0);
         }
         if (tid < TS) {
-            double v = 0.0;
+            T v = T(0);
             if (tid < nrb) {
                 const int row = t.r0 + rb + tid;
                 if (row < t.w) v = x[t.x_rows + row];
@@ -703,7 +735,7 @@ __global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__
         __syncthreads();
         if (ci < t.kb) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc += T[ci][g * 16 + i] * xv[g * 16 + i];
+            for (int i = 0; i < 16; ++i) acc += Tt[ci][g * 16 + i] * xv[g * 16 + i];
         }
         __syncthreads();
     }
@@ -713,19 +745,20 @@ __global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__
 }
 
 // Backward diagonal block: x_k = L_kk^{-T} (y_k - sum of the partial sums), via the inverse.
+template <typename T>
 __global__ __launch_bounds__(64) void k_bdiag(const SolveDiagTask* __restrict__ tasks,
-                                              const double* __restrict__ inv,
-                                              const double* __restrict__ partall, int64_t ldp,
-                                              double* __restrict__ xall, int64_t ldx) {
+                                              const T* __restrict__ inv,
+                                              const T* __restrict__ partall, int64_t ldp,
+                                              T* __restrict__ xall, int64_t ldx) {
     const SolveDiagTask t = tasks[blockIdx.x];
-    double* x = xall + blockIdx.y * ldx + t.x_off;
-    const double* part = partall + blockIdx.y * ldp;
-    __shared__ double vi[TS][TS + 1];   // vi[c][r] = inverse(r, c)
-    __shared__ double z[TS];
+    T* x = xall + blockIdx.y * ldx + t.x_off;
+    const T* part = partall + blockIdx.y * ldp;
+    __shared__ T vi[TS][TS + 1];   // vi[c][r] = inverse(r, c)
+    __shared__ T z[TS];
     const int tid = threadIdx.x, kb = t.kb;
-    const double* Vi = inv + t.inv_off;
+    const T* Vi = inv + t.inv_off;
     for (int c = 0; c < kb; ++c) vi[c][tid] = Vi[c * TS + tid];
-    double v = 0.0;
+    T v = T(0);
     if (tid < kb) {
         v = x[tid];
         for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + tid];
@@ -733,10 +766,28 @@ __global__ __launch_bounds__(64) void k_bdiag(const SolveDiagTask* __restrict__ 
     z[tid] = v;
     __syncthreads();
     if (tid < kb) {
-        double s = 0.0;
+        T s = T(0);
         for (int j = 0; j < kb; ++j) s += vi[tid][j] * z[j];   // (L^{-1})^T row tid = column tid of L^{-1}
         x[tid] = s;
     }
+}
+
+// factor-order vector conversions around a scaled (or lower-precision) solve: y = (T)(x * s)
+template <typename T>
+__global__ void k_to_factor(const double* __restrict__ x, const double* __restrict__ s, T* __restrict__ y,
+                            int64_t n, int64_t ld) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[blockIdx.y * ld + i];
+    y[blockIdx.y * ld + i] = (T)(s ? v * s[i] : v);
+}
+template <typename T>
+__global__ void k_from_factor(const T* __restrict__ y, const double* __restrict__ s, double* __restrict__ x,
+                              int64_t n, int64_t ld) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = (double)y[blockIdx.y * ld + i];
+    x[blockIdx.y * ld + i] = s ? v * s[i] : v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -756,7 +807,8 @@ __global__ void k_scatter_map(const int64_t* __restrict__ map, int64_t n, const 
     const int64_t t = map[i];
     dst[blockIdx.y * ldd + i] = (t >= 0) ? src[blockIdx.y * lds + t] : 0.0;
 }
-__global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int64_t n) {
+template <typename T>
+__global__ void k_copy(const T* __restrict__ s, T* __restrict__ d, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         d[i] = s[i];
@@ -852,7 +904,7 @@ __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start
 // FactorizerDev
 // ------------------------------------------------------------------------------------------
 FactorizerDev::FactorizerDev(int prec) : precision(prec) {
-    if (prec != 0) throw Error(-6, "only the FP64 factorization is available in this build");
+    if (prec != 0 && prec != 1) throw Error(-1, "unknown precision");
 }
 FactorizerDev::~FactorizerDev() = default;
 
@@ -862,6 +914,8 @@ void FactorizerDev::require(int t) const {
 }
 
 void FactorizerDev::ensure_vec(DeviceArray& a, size_t bytes) { a.alloc(bytes); }
+
+size_t FactorizerDev::fsize() const { return precision == 1 ? sizeof(float) : sizeof(double); }
 
 void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
                              const int64_t* pins, int64_t npins) {
@@ -875,7 +929,7 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, opt));
     ftype = 0;
     stash_valid_ = false;
-    // injection plan entries with per-entry leading dimension
+    // injection plan entries with per-entry leading dimension and factor-order block row/column
     std::vector<PlanEnt> pe(S->plan_src.size());
     for (size_t q = 0; q < pe.size(); ++q) {
         const int64_t dst = S->plan_dst[q] >> 2;
@@ -884,8 +938,11 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
         pe[q].dstmode = S->plan_dst[q];
         pe[q].src = S->plan_src[q];
         pe[q].ld = 3 * (S->swidth[s] + S->sdeg[s]);
+        pe[q].rb = S->plan_row[q];
+        pe[q].cb = S->plan_col[q];
     }
     planent_.upload(pe);
+    diagsrc_.upload(S->diag_src);
     rel_.upload(S->rel);
     ext_.upload(S->extend_tasks);
     potrf_.upload(S->potrf_tasks);
@@ -915,51 +972,61 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
             for (int c = 0; c < 3; ++c) r2p[3 * r + c] = 3 * (int64_t)S->pos[r] + c;
         r2p_.upload(r2p);
     }
-    L_.alloc(std::max<int64_t>(S->L_size, 1) * sizeof(double));
-    U_.alloc(std::max<int64_t>(S->U_size, 1) * sizeof(double));
-    inv_.alloc(S->inv_slots * TS * TS * sizeof(double));
+    const size_t fs = fsize();
+    L_.alloc(std::max<int64_t>(S->L_size, 1) * fs);
+    U_.alloc(std::max<int64_t>(S->U_size, 1) * fs);
+    inv_.alloc(S->inv_slots * TS * TS * fs);
+    if (precision == 1) scale_.alloc(3 * S->n_red * sizeof(double));
     status_.alloc(sizeof(int) * 4);
     sym = std::move(S);
     ftype = 1;
 }
 
-void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, void* stream) {
-    require(1);
+template <typename T>
+void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, void* stream) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
-    ftype = 1;
     hipEvent_t ev[3];
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipEventRecord(ev[0], s));
-    double* L = L_.as<double>();
-    double* U = U_.as<double>();
+    T* L = L_.as<T>();
+    T* U = U_.as<T>();
+    T* inv = inv_.as<T>();
     prof.begin(KernelProfile::FILL_L, s);
-    fill(L, S.L_size, 0.0, s);
+    fill<T>(L, S.L_size, T(0), s);
     prof.end(s);
     k_set_int<<<1, 1, 0, s>>>(status_.as<int>(), INT_MAX);
     LAUNCH_CHECK();
+    const double* scale = nullptr;
+    if (precision == 1) {   // symmetric diagonal scaling of the factored matrix
+        k_diag_scale<<<nblk(S.n_red, 256), 256, 0, s>>>(diagsrc_.as<int32_t>(), S.n_red, Ax, Bx, sigma,
+                                                       scale_.as<double>(), status_.as<int>());
+        LAUNCH_CHECK();
+        scale = scale_.as<double>();
+    }
     const int64_t np = (int64_t)S.plan_src.size();
     if (np) {
         prof.begin(KernelProfile::INJECT, s);
-        k_inject<<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, L);
+        k_inject<T><<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, scale, L);
         LAUNCH_CHECK();
         prof.end(s);
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
     const int64_t* zr = zr_.as<int64_t>();
+    const GemmTask* gt = gemm_.as<GemmTask>();
     for (const LevelPlan& P : S.levels) {
         const int64_t nz = P.zero_u.second - P.zero_u.first;
         if (nz > 0) {
             dim3 g(64, (unsigned)nz);
             prof.begin(KernelProfile::ZERO_U, s);
-            k_zero_ranges<<<g, 256, 0, s>>>(U, zr + 2 * P.zero_u.first);
+            k_zero_ranges<T><<<g, 256, 0, s>>>(U, zr + 2 * P.zero_u.first);
             LAUNCH_CHECK();
             prof.end(s);
         }
         for (auto [a, b] : P.extend_passes)
             if (b > a) {
                 prof.begin(KernelProfile::EXTEND, s);
-                k_extend<<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
+                k_extend<T><<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -967,27 +1034,26 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
             auto [p0, p1] = P.potrf_steps[t];
             if (p1 > p0) {
                 prof.begin(KernelProfile::POTRF, s, 0.0, p1 - p0);
-                k_potrf<<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv_.as<double>(),
-                                                           status_.as<int>());
+                k_potrf<T><<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv, status_.as<int>());
                 LAUNCH_CHECK();
                 prof.end(s);
             }
             const TaskRange& tr = P.trsm_steps[t];
             if (tr.e > tr.b) {
                 prof.begin(KernelProfile::TRSM, s, prof.on ? gemm_work(S, tr) : 0.0, tr.e - tr.b);
-                launch_gemm(tr, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
+                launch_gemm<T>(tr, gt, L, U, inv, s);
                 prof.end(s);
             }
             const TaskRange& ur = P.update_steps[t];
             if (ur.e > ur.b) {
                 prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b);
-                launch_gemm(ur, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
+                launch_gemm<T>(ur, gt, L, U, inv, s);
                 prof.end(s);
             }
         }
         if (P.syrk.e > P.syrk.b) {
             prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
-            launch_gemm(P.syrk, gemm_.as<GemmTask>(), L, U, inv_.as<double>(), s);
+            launch_gemm<T>(P.syrk, gt, L, U, inv, s);
             prof.end(s);
         }
     }
@@ -1014,27 +1080,37 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
     ftype = 2;
 }
 
-void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
+void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, void* stream) {
+    require(1);
+    ftype = 1;
+    if (precision == 1) numeric_t<float>(Ax, sigma, Bx, stream);
+    else numeric_t<double>(Ax, sigma, Bx, stream);
+}
+
+template <typename T>
+void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
     const int64_t ldx = 3 * S.n_red;
     const int64_t ldu = std::max<int64_t>(S.u_solve_size, 1);
     const int64_t ldp = S.bpart_size * TS;
-    usolve_.alloc((size_t)ldu * nrhs * sizeof(double));
-    bpart_.alloc((size_t)ldp * nrhs * sizeof(double));
-    double* u = usolve_.as<double>();
-    double* part = bpart_.as<double>();
-    const double* L = L_.as<double>();
+    const size_t fs = sizeof(T);
+    usolve_.alloc((size_t)ldu * nrhs * fs);
+    bpart_.alloc((size_t)ldp * nrhs * fs);
+    ysolve_.alloc((size_t)ldx * nrhs * fs);
+    T* u = usolve_.as<T>();
+    T* part = bpart_.as<T>();
+    T* Y = ysolve_.as<T>();
+    const T* L = L_.as<T>();
+    const T* inv = inv_.as<T>();
     const int32_t* sdeg = sdeg_.as<int32_t>();
     const int64_t* reloff = reloff_.as<int64_t>();
-    ysolve_.alloc((size_t)ldx * nrhs * sizeof(double));
-    double* Y = ysolve_.as<double>();
     for (const LevelPlan& P : S.levels) {
         if (P.fgather.second > P.fgather.first) {
             dim3 g((unsigned)(P.fgather.second - P.fgather.first), (unsigned)nrhs);
             prof.begin(KernelProfile::FGATHER, s);
-            k_fgather<<<g, 256, 0, s>>>(fg_.as<SolveGatherTask>() + P.fgather.first, children_.as<int32_t>(),
-                                        sdeg, reloff, rel_.as<int32_t>(), usoff_.as<int64_t>(), xp, ldx, u, ldu);
+            k_fgather<T><<<g, 256, 0, s>>>(fg_.as<SolveGatherTask>() + P.fgather.first, children_.as<int32_t>(),
+                                           sdeg, reloff, rel_.as<int32_t>(), usoff_.as<int64_t>(), xp, ldx, u, ldu);
             LAUNCH_CHECK();
             prof.end(s);
         }
@@ -1043,7 +1119,7 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
                 prof.begin(KernelProfile::FGEMV, s);
-                k_fstep<<<g, 64, 0, s>>>(fs_.as<FStepTask>() + v0, L, inv_.as<double>(), xp, Y, ldx, u, ldu);
+                k_fstep<T><<<g, 64, 0, s>>>(fs_.as<FStepTask>() + v0, L, inv, xp, Y, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -1056,7 +1132,7 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             if (v1 > v0) {
                 dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BGEMV, s);
-                k_bgemv<<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), Y, ldx, part, ldp);
+                k_bgemv<T><<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), Y, ldx, part, ldp);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -1064,16 +1140,35 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BDIAG, s);
-                k_bdiag<<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, inv_.as<double>(), part, ldp, Y, ldx);
+                k_bdiag<T><<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, inv, part, ldp, Y, ldx);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
         }
     }
     prof.begin(KernelProfile::MAPS, s);
-    k_copy<<<std::min<unsigned>(nblk(ldx * nrhs, 256), 8192), 256, 0, s>>>(Y, xp, ldx * nrhs);
+    k_copy<T><<<std::min<unsigned>(nblk(ldx * nrhs, 256), 8192), 256, 0, s>>>(Y, xp, ldx * nrhs);
     LAUNCH_CHECK();
     prof.end(s);
+}
+
+// factor-order double vectors in/out; FP32 factors solve the scaled system in single precision
+void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
+    const Symbolic& S = *sym;
+    if (precision == 0) {
+        solve_t<double>(xp, nrhs, stream);
+        return;
+    }
+    hipStream_t s = S_(stream);
+    const int64_t n = 3 * S.n_red;
+    xf_.alloc((size_t)n * nrhs * sizeof(float));
+    float* xf = xf_.as<float>();
+    dim3 g(nblk(n, 256), (unsigned)nrhs);
+    k_to_factor<float><<<g, 256, 0, s>>>(xp, scale_.as<double>(), xf, n, n);
+    LAUNCH_CHECK();
+    solve_t<float>(xf, nrhs, stream);
+    k_from_factor<float><<<g, 256, 0, s>>>(xf, scale_.as<double>(), xp, n, n);
+    LAUNCH_CHECK();
 }
 
 void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
@@ -1103,7 +1198,7 @@ void FactorizerDev::solve_raw_reduced(const double* b, double* x, int nrhs, bool
     double* xp = scratch_b_.as<double>();
     dim3 g(nblk(nr, 256), (unsigned)nrhs);
     if (already_permuted) {
-        k_copy<<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(b, xp, nr * nrhs);
+        k_copy<double><<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(b, xp, nr * nrhs);
         LAUNCH_CHECK();
     } else {
         k_gather_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, b, nr, xp, nr);
@@ -1111,7 +1206,7 @@ void FactorizerDev::solve_raw_reduced(const double* b, double* x, int nrhs, bool
     }
     solve_permuted_inplace(xp, nrhs, stream);
     if (already_permuted) {
-        k_copy<<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(xp, x, nr * nrhs);
+        k_copy<double><<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(xp, x, nr * nrhs);
         LAUNCH_CHECK();
     } else {
         k_scatter_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, xp, nr, x, nr);
@@ -1192,33 +1287,26 @@ void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, 
     }
 }
 
+// L of P A P^T (for an FP32 factor: the factor of S P A P^T S, unscaled to L = S^{-1} L~)
 void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
     require(2);
     const Symbolic& S = *sym;
     std::vector<double> h(S.L_size);
     HIP_CHECK(hipDeviceSynchronize());
-    HIP_CHECK(hipMemcpy(h.data(), L_.p, S.L_size * sizeof(double), hipMemcpyDeviceToHost));
-    const int64_t n = 3 * S.n_red;
-    std::vector<int64_t> cnt(n + 1, 0);
-    for (int32_t sn = 0; sn < S.ns; ++sn) {
-        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn];
-        for (int64_t c = 0; c < W; ++c) cnt[3 * (int64_t)S.sfirst[sn] + c + 1] = W + D - c;
+    if (precision == 1) {
+        std::vector<float> hf(S.L_size);
+        HIP_CHECK(hipMemcpy(hf.data(), L_.p, S.L_size * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < S.L_size; ++i) h[i] = hf[i];
+    } else {
+        HIP_CHECK(hipMemcpy(h.data(), L_.p, S.L_size * sizeof(double), hipMemcpyDeviceToHost));
     }
-    for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
-    for (int64_t j = 0; j <= n; ++j) Lp[j] = cnt[j];
-    for (int32_t sn = 0; sn < S.ns; ++sn) {
-        const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn], ld = W + D;
-        const int64_t f = 3 * (int64_t)S.sfirst[sn];
-        for (int64_t c = 0; c < W; ++c) {
-            int64_t q = Lp[f + c];
-            for (int64_t r = c; r < W + D; ++r) {
-                const int64_t grow = r < W ? f + r
-                                           : 3 * (int64_t)S.rows[S.srows_off[sn] + (r - W) / 3] + (r - W) % 3;
-                Li[q] = grow;
-                Lx[q] = h[S.panel_off[sn] + c * ld + r];
-                ++q;
-            }
-        }
+    panels_to_csc(S, h, Lp, Li, Lx);
+    if (precision == 1) {
+        std::vector<double> sc(3 * S.n_red);
+        HIP_CHECK(hipMemcpy(sc.data(), scale_.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const int64_t n = 3 * S.n_red;
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t q = Lp[j]; q < Lp[j + 1]; ++q) Lx[q] /= sc[Li[q]];
     }
 }
 
@@ -1232,6 +1320,10 @@ void FactorizerDev::stash() {
     HIP_CHECK(hipMemcpy(Lstash_.p, L_.p, L_.bytes, hipMemcpyDeviceToDevice));
     invstash_.alloc(inv_.bytes);
     HIP_CHECK(hipMemcpy(invstash_.p, inv_.p, inv_.bytes, hipMemcpyDeviceToDevice));
+    if (precision == 1) {
+        scalestash_.alloc(scale_.bytes);
+        HIP_CHECK(hipMemcpy(scalestash_.p, scale_.p, scale_.bytes, hipMemcpyDeviceToDevice));
+    }
     stash_valid_ = true;
     stash_type_ = ftype;
     stash_succ_ = num_successful;
@@ -1239,10 +1331,15 @@ void FactorizerDev::stash() {
 
 void FactorizerDev::swap_stash() {
     if (!stash_valid_) throw Error(-2, "No stashed factorization");
+    // the panels, the tile inverses and (FP32) the scaling all belong to the factor
     std::swap(L_.p, Lstash_.p);
     std::swap(L_.bytes, Lstash_.bytes);
-    std::swap(inv_.p, invstash_.p);            // tile inverses belong to the factor (solves use them)
+    std::swap(inv_.p, invstash_.p);
     std::swap(inv_.bytes, invstash_.bytes);
+    if (precision == 1) {
+        std::swap(scale_.p, scalestash_.p);
+        std::swap(scale_.bytes, scalestash_.bytes);
+    }
     std::swap(ftype, stash_type_);
     std::swap(num_successful, stash_succ_);
     if (ftype == 0) ftype = 1;   // the symbolic analysis is shared by both factors
@@ -1251,6 +1348,7 @@ void FactorizerDev::swap_stash() {
 void FactorizerDev::clear_stash() {
     Lstash_.release();
     invstash_.release();
+    scalestash_.release();
     stash_valid_ = false;
 }
 
@@ -1289,7 +1387,7 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
     r.b = 0;
     r.m = TT == 128 ? (int64_t)tasks.size() : 0;
     r.e = (int64_t)tasks.size();
-    launch_gemm(r, d.as<GemmTask>(), const_cast<double*>(A), C, B, s);
+    launch_gemm<double>(r, d.as<GemmTask>(), const_cast<double*>(A), C, B, s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
 

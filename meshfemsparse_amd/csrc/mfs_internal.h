@@ -184,6 +184,8 @@ struct Symbolic {
     // injection plan: per reduced stored block (in reduced-CSC order)
     std::vector<int32_t> plan_src;             // original block entry index (Ax block)
     std::vector<int64_t> plan_dst;             // (scalar L offset << 2) | mode
+    std::vector<int32_t> plan_row, plan_col;   // factor-order block row / column of the lower block
+    std::vector<int32_t> diag_src;             // per factor position: Ax block index of its diagonal
     // schedule
     std::vector<LevelPlan> levels;
     std::vector<ExtendTask> extend_tasks;

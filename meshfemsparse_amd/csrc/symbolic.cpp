@@ -389,6 +389,12 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     // ---- injection plan ----
     S.plan_src.resize(red.size());
     S.plan_dst.resize(red.size());
+    S.plan_row.resize(red.size());
+    S.plan_col.resize(red.size());
+    S.diag_src.assign(n, -1);
+    for (int64_t j = 0; j < nb; ++j)
+        if (S.red_of_full[j] >= 0 && Ap[j + 1] > Ap[j] && Ai[Ap[j + 1] - 1] == j)
+            S.diag_src[S.pos[S.red_of_full[j]]] = (int32_t)(Ap[j + 1] - 1);   // -1: no stored diagonal
     for (size_t q = 0; q < red.size(); ++q) {
         const RE& e = red[q];
         const int32_t a = S.pos[e.i], b = S.pos[e.j];
@@ -408,6 +414,8 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         const int64_t mode = (a == b) ? 2 : (a > b ? 0 : 1);
         S.plan_src[q] = e.src;
         S.plan_dst[q] = (dst << 2) | mode;
+        S.plan_row[q] = row;
+        S.plan_col[q] = col;
     }
 
     // ---- level schedule: U arena, tile tasks, solve tasks ----

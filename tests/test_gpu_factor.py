@@ -5,7 +5,12 @@ with the oracle's up-looking factor under the ordering the device analysis chose
 the reference's checkSolves (tests/test_catamari_factorizer.cc:13-98): known solutions
 x_ij = sin(0.2 (i+1)(j+1)), multi-RHS and single solves, raw/permuted/in-place reduced solves,
 shifts, shift-by-matrix, stash/swap, rescaling, failure on -A and recovery; tolerance 1e-11
-relative to ||x|| (double precision).
+relative to ||x|| in double precision and 2e-5 in single precision, the reference's own bounds.
+
+Single precision factors the symmetrically scaled matrix S A S (S = diag(A)^-1/2) in FP32 and
+solves in FP32; the reference's single-precision factorizers cast A to float and factor that
+(CatamariFactorizer.cc:640-660), so both lose definiteness on the same rounding case
+(test_catamari_factorizer.cc:197-212).
 """
 import numpy as np
 import pytest
@@ -17,12 +22,13 @@ from tests import oracle_lib as orc
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-11
+TOL_SINGLE = 2e-5
 
 
-def mesh_problem(dev, N, seed=0, E=1e5, nu=0.3, Ee=None):
+def mesh_problem(dev, N, seed=0, E=1e5, nu=0.3, Ee=None, nue=None):
     tets, Xr = wl.cube_mesh(N)
     X = wl.perturb(N, Xr, seed)
-    He, g, _ = wl.neohookean(Xr, X, tets, E=E, nu=nu, E_per_elem=Ee)
+    He, g, _ = wl.neohookean(Xr, X, tets, E=E, nu=nu, E_per_elem=Ee, nu_per_elem=nue)
     nb = len(Xr)
     asm = mfs.SystemAssembler(nb)
     H = asm.blockSparsityPattern(tets)
@@ -84,8 +90,9 @@ def dense_of(H):
     return H.toDense()
 
 
-def check_solves(dev, H, pins):
-    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, False)
+def check_solves(dev, H, pins, single=False):
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    TOL = TOL_SINGLE if single else globals()["TOL"]
     A = dense_of(H)
     n = A.shape[0]
     expected = np.array([[np.sin(0.2 * (i + 1) * (j + 1)) for j in range(3)] for i in range(n)])
@@ -159,16 +166,96 @@ def check_solves(dev, H, pins):
     f.clearStashedFactorization()
 
 
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("pin_mode", [0, 1])
-def test_check_solves_chain(dev, pin_mode):
+def test_check_solves_chain(dev, pin_mode, single):
     H = chain_problem(dev)
     pins = [0, 1, 2] if pin_mode == 1 else []
-    check_solves(dev, H, pins)
+    check_solves(dev, H, pins, single)
 
 
-def test_check_solves_mesh(dev):
+@pytest.mark.parametrize("single", [False, True])
+def test_check_solves_mesh(dev, single):
     H, pins, _, _, _ = mesh_problem(dev, 4, seed=3)
-    check_solves(dev, H, pins)
+    check_solves(dev, H, pins, single)
+
+
+@pytest.mark.parametrize("N", [5, 8])
+def test_single_factor_matches_unique_cholesky(dev, N):
+    """The FP32 factor, unscaled, against the FP64 oracle factor: agreement to FP32 accuracy,
+    and bitwise run-to-run determinism."""
+    H, pins, _, _, _ = mesh_problem(dev, N)
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, True)
+    f.factorizeSymbolic(H, pins)
+    f.factorizeNumeric(H)
+    (Lp, Li, Lx), _, _ = oracle_factor(H, f)
+    Lo = orc.dense_L(Lp, Li, Lx)
+    Lg = orc.dense_L(*f.exportFactorCSC())
+    # column-relative error: each column of L scales with sqrt of its pivot
+    cn = np.abs(Lo).max(axis=0)
+    assert (np.abs(Lg - Lo).max(axis=0) <= 2e-5 * cn).all()
+    assert not np.array_equal(Lg, Lo)          # really computed in single precision
+    f.factorizeNumeric(H)
+    assert np.array_equal(orc.dense_L(*f.exportFactorCSC()), Lg)
+
+
+def one_block(dev, M):
+    """A single 3x3 diagonal block holding the upper triangle of M (column-major)."""
+    H = mfs.BlockCSCHessian(np.array([0, 1]), np.array([0]), dev)
+    H.Ax = torch.tensor(np.asarray(M, np.float64).T.ravel(), device=dev)
+    return H
+
+
+def test_single_rounding_loss_of_definiteness(dev):
+    """[[1, 1], [1, 1 + 1e-8]] is SPD in double but singular once rounded to float
+    (test_catamari_factorizer.cc:197-212), embedded as the leading 2x2 of one 3x3 block."""
+    H = one_block(dev, [[1, 1, 0], [1, 1 + 1e-8, 0], [0, 0, 1]])
+    dbl = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, False)
+    single = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, True)
+    dbl.factorize(H)
+    assert dbl.hasNumericFactorization()
+    with pytest.raises(mfs.NotPositiveDefiniteError):
+        single.factorize(H)
+    assert not single.hasNumericFactorization()
+    b = torch.ones(3, dtype=torch.float64, device=dev)
+    x = torch.empty(3, dtype=torch.float64, device=dev)
+    with pytest.raises(RuntimeError):
+        single.solveRawReduced(b, x)
+    with pytest.raises(RuntimeError):
+        single.solveMultiRHS(torch.ones(3, 2, dtype=torch.float64, device=dev))
+
+
+def test_single_non_positive_diagonal_fails(dev):
+    """A non-positive diagonal entry cannot be scaled: reported as a pivot failure."""
+    H = one_block(dev, [[4, 0, 0], [0, -1, 0], [0, 0, 1]])
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, True)
+    with pytest.raises(mfs.NotPositiveDefiniteError):
+        f.factorize(H)
+    assert f.numSuccessfulPivots() == 1
+
+
+def test_single_refined_newton_step_16cube(dev):
+    """FP32 factor + FP64 iterative refinement on a stiffness field spread over four decades
+    (per-element E log-uniform in [1e3, 1e7], nu uniform in [0.2, 0.45]): relative residual
+    <= 1e-10, and the solution agrees with the FP64 factor's."""
+    N = 16
+    rng = np.random.default_rng(7)
+    ne = 6 * N ** 3
+    Ee, nue = 10 ** rng.uniform(3, 7, ne), rng.uniform(0.2, 0.45, ne)
+    H, pins, g, _, _ = mesh_problem(dev, N, seed=1, Ee=Ee, nue=nue)
+    b = torch.from_numpy(-g).to(dev)
+    f64 = mfs.make_cholesky_factorizer()
+    f64.factorize(H, pins)
+    x64, it64, rr64 = f64.solveRefined(H, b, tol=1e-10, maxIters=5)
+    f32 = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, True)
+    f32.factorize(H, pins)
+    x0 = f32.solve(b)                     # unrefined single-precision solve
+    x32, it32, rr32 = f32.solveRefined(H, b, tol=1e-10, maxIters=30)
+    assert rr64 <= 1e-10 and rr32 <= 1e-10
+    assert 1 <= it32 <= 30
+    e0 = torch.linalg.norm(x0 - x64) / torch.linalg.norm(x64)
+    e1 = torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)
+    assert e1 < 1e-6 and e1 < e0
 
 
 def test_partially_pinned_block_is_rejected(dev):
