@@ -30,7 +30,8 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 PEAK_HBM_GBS = 8000.0
 # kernel classes of mfs_kernel_profile
 KCLASSES = ["fill_L", "inject", "zero_U", "extend", "potrf", "trsm", "update", "syrk", "fgather",
-            "fdiag", "fgemv", "bgemv", "bdiag", "spmv", "maps", "assemble"]
+            "fdiag", "fupdate", "bpartial", "bdiag", "spmv", "maps", "assemble", "blkinv", "bgather",
+            "convert", "refine"]
 
 
 def log(*a):
@@ -177,8 +178,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     _lib.call("mfs_kernel_profiling", f._h, 0, 0)
-    ms = np.zeros(16)
-    cnt = np.zeros(16, np.int64)
+    ms = np.zeros(len(KCLASSES))
+    cnt = np.zeros(len(KCLASSES), np.int64)
     _lib.call("mfs_kernel_profile", f._h, _lib.ptr(ms, _lib.c_double), _lib.ptr(cnt, _lib.c_i64))
     t_asm = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
     t_fac = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3

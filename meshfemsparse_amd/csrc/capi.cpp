@@ -318,15 +318,15 @@ int mfs_debug_emulate_factor(const int64_t* Ap, const int64_t* Ai, int64_t nb, c
     MFS_TRY
     CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(Ax);
     if (npins > 0) CHECK_PTR(pins);
-    mfs::SymbolicOptions opt;
-    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, opt);
+    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, mfs::options_from_env());
     std::vector<double> L, inv;
     const int64_t fc = mfs::emulate_numeric(S, Ax, sigma, L, inv);
     if (failed_col) *failed_col = fc;
     if (Lp && Li && Lx) mfs::panels_to_csc(S, L, Lp, Li, Lx);
     if (x_perm && fc < 0) {
-        std::vector<double> x(x_perm, x_perm + 3 * S.n_red);
-        mfs::emulate_solve(S, L, inv, x);
+        std::vector<double> x(x_perm, x_perm + 3 * S.n_red), Xb;
+        mfs::emulate_block_inverses(S, L, inv, Xb);
+        mfs::emulate_solve(S, L, Xb, x);
         std::copy(x.begin(), x.end(), x_perm);
     }
     MFS_CATCH

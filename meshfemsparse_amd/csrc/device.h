@@ -37,7 +37,7 @@ private:
 // Optional per-kernel-class device timing (HIP event pairs around each launch).
 struct KernelProfile {
     enum { FILL_L, INJECT, ZERO_U, EXTEND, POTRF, TRSM, UPDATE, SYRK, FGATHER, FDIAG, FGEMV, BGEMV,
-           BDIAG, SPMV, MAPS, ASSEMBLE, NCLASS };
+           BDIAG, SPMV, MAPS, ASSEMBLE, BLKINV, BGATHER, CONVERT, REFINE, NCLASS };
     bool on = false;
     double ms[NCLASS] = {};
     int64_t count[NCLASS] = {};
@@ -91,10 +91,11 @@ private:
 
     DeviceArray L_, U_, inv_, status_;
     DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, zr_;
-    DeviceArray fg_, fs_, bd_, bv_, children_, usoff_, rows_, sdeg_, reloff_;
+    DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
+    DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;
     DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
-    DeviceArray Lstash_, invstash_, scalestash_;
+    DeviceArray Lstash_, invstash_, scalestash_, xbstash_;
     bool stash_valid_ = false;
     int stash_type_ = 0;
     int64_t stash_succ_ = 0;

@@ -145,14 +145,54 @@ namespace mfs {
 
 // Host replay of the device solve schedule on a factor-order vector x (one RHS), using the
 // panels L and the stored tile inverses `inv` produced by emulate_numeric_full.
-void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+// Block inverses X_b = L_bb^{-1} by the tile recurrence the device uses: X_jj = (tile inverse),
+// X_ij = -X_ii sum_{m=j}^{i-1} L_im X_mj.
+void emulate_block_inverses(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+                            std::vector<double>& Xb) {
+    const int TS = 64;
+    Xb.assign(S.xb_size, 0.0);
+    for (const SolveBlock& b : S.sblocks) {
+        const int nb = b.nb, nt = (nb + TS - 1) / TS;
+        double* X = Xb.data() + b.xb_off;
+        const double* Lb = L.data() + b.l_off;
+        for (int j = 0; j < nt; ++j) {
+            const int cj = std::min(TS, nb - TS * j);
+            const double* Vj = inv.data() + (int64_t)S.tile_slot[b.ts_off + j] * TS * TS;
+            for (int c = 0; c < cj; ++c)
+                for (int r = c; r < cj; ++r) X[(int64_t)(TS * j + c) * nb + TS * j + r] = Vj[c * TS + r];
+            for (int i = j + 1; i < nt; ++i) {
+                const int ri = std::min(TS, nb - TS * i);
+                std::vector<double> acc((size_t)TS * TS, 0.0);   // acc[c * TS + r]
+                for (int m = j; m < i; ++m) {
+                    const int cm = std::min(TS, nb - TS * m);
+                    for (int c = 0; c < cj; ++c)
+                        for (int k = 0; k < cm; ++k) {
+                            const double xv = X[(int64_t)(TS * j + c) * nb + TS * m + k];
+                            for (int r = 0; r < ri; ++r) acc[c * TS + r] += Lb[(int64_t)(TS * m + k) * b.ld + TS * i + r] * xv;
+                        }
+                }
+                const double* Vi = inv.data() + (int64_t)S.tile_slot[b.ts_off + i] * TS * TS;
+                for (int c = 0; c < cj; ++c)
+                    for (int r = 0; r < ri; ++r) {
+                        double v = 0;
+                        for (int k = 0; k <= r; ++k) v += Vi[k * TS + r] * acc[c * TS + k];
+                        X[(int64_t)(TS * j + c) * nb + TS * i + r] = -v;
+                    }
+            }
+        }
+    }
+}
+
+// Host replay of the device solve schedule (one right-hand side, factor order, in place).
+void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& Xb,
                    std::vector<double>& x) {
     const int TS = 64;
+    const int SB = S.solve_block;
     std::vector<double> u(std::max<int64_t>(S.u_solve_size, 1), 0.0), Y(x.size(), 0.0);
-    std::vector<double> part((size_t)S.bpart_size * TS, 0.0);
+    std::vector<double> part((size_t)S.bpart_size * SB, 0.0);
     for (const LevelPlan& P : S.levels) {
-        for (int64_t q = P.fgather.first; q < P.fgather.second; ++q) {
-            const SolveGatherTask& t = S.fgather_tasks[q];
+        for (int64_t q = P.gather.first; q < P.gather.second; ++q) {
+            const SolveGatherTask& t = S.gather_tasks[q];
             std::fill(u.begin() + t.u_off, u.begin() + t.u_off + t.d, 0.0);
             for (int32_t k = t.child_begin; k < t.child_end; ++k) {
                 const int32_t c = S.children[k];
@@ -165,70 +205,60 @@ void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::v
                 }
             }
         }
-        for (auto [v0, v1] : P.fstep_steps) {
-            // all tasks of a step read x_k before any writes: compute y per task first
-            std::vector<std::vector<double>> ys(v1 - v0);
-            for (int64_t q = v0; q < v1; ++q) {
-                const FStepTask& t = S.fstep_tasks[q];
-                std::vector<double> y(TS, 0.0);
-                for (int i = 0; i < TS; ++i)
-                    for (int j = 0; j < t.kb; ++j) y[i] += inv[t.inv_off + j * TS + i] * x[t.x_blk + j];
-                ys[q - v0] = y;
-            }
-            for (int64_t q = v0; q < v1; ++q) {
-                const FStepTask& t = S.fstep_tasks[q];
-                const std::vector<double>& y = ys[q - v0];
-                if (t.nr == 0) {
-                    for (int i = 0; i < t.kb; ++i) Y[t.x_blk + i] = y[i];
-                    continue;
+        for (size_t st = 0; st < P.fdiag_steps.size(); ++st) {
+            for (int64_t q = P.fdiag_steps[st].first; q < P.fdiag_steps[st].second; ++q) {
+                const SolveTile& tl = S.fdiag_tiles[q];
+                const SolveBlock& b = S.sblocks[tl.blk];
+                const int r0 = TS * tl.t, nr = std::min(TS, b.nb - r0), nc = std::min(b.nb, r0 + TS);
+                for (int r = 0; r < nr; ++r) {
+                    double v = 0;
+                    for (int c = 0; c < nc; ++c) v += Xb[b.xb_off + (int64_t)c * b.nb + r0 + r] * x[b.x_off + c];
+                    Y[b.x_off + r0 + r] = v;
                 }
+            }
+            for (int64_t q = P.fupd_steps[st].first; q < P.fupd_steps[st].second; ++q) {
+                const SolveRowTask& t = S.fupd_tasks[q];
                 for (int r = 0; r < t.nr; ++r) {
-                    double d = 0;
-                    for (int c = 0; c < t.kb; ++c) d += L[t.l_off + (int64_t)c * t.ld + r] * y[c];
-                    const int row = t.r0 + r;
-                    if (row < t.w) x[t.x_rows + row] -= d;
-                    else u[t.u_off + row - t.w] += d;
+                    double v = 0;
+                    for (int c = 0; c < t.nc; ++c) v += L[t.l_off + (int64_t)c * t.ld + r] * Y[t.v_off + c];
+                    if (t.mode == 0) x[t.d_off + r] -= v;
+                    else u[t.d_off + r] += v;
                 }
             }
         }
     }
     for (int lv = (int)S.levels.size() - 1; lv >= 0; --lv) {
         const LevelPlan& P = S.levels[lv];
+        for (int64_t q = P.gather.first; q < P.gather.second; ++q) {
+            const SolveGatherTask& t = S.gather_tasks[q];
+            for (int i = 0; i < t.d; ++i) u[t.u_off + i] = x[3 * (int64_t)S.rows[t.rows_off + i / 3] + i % 3];
+        }
         for (size_t st = 0; st < P.bdiag_steps.size(); ++st) {
-            for (int64_t q = P.bgemv_steps[st].first; q < P.bgemv_steps[st].second; ++q) {
-                const SolveGemvTask& t = S.bgemv_tasks[q];
-                for (int c = 0; c < t.kb; ++c) {
+            for (int64_t q = P.bpart_steps[st].first; q < P.bpart_steps[st].second; ++q) {
+                const SolveRowTask& t = S.bpart_tasks[q];
+                const double* src = (t.mode == 0 ? x.data() : u.data()) + t.v_off;
+                for (int c = 0; c < t.nc; ++c) {
                     double acc = 0;
-                    for (int r = 0; r < t.nr; ++r) {
-                        const int row = t.r0 + r;
-                        double xv;
-                        if (row < t.w) xv = Y[t.x_rows + row];
-                        else {
-                            const int qq = row - t.w;
-                            xv = Y[3 * (int64_t)S.rows[t.rows_off + qq / 3] + qq % 3];
-                        }
-                        acc += L[t.l_off + (int64_t)c * t.ld + r] * xv;
-                    }
-                    part[(int64_t)t.part * TS + c] = acc;
+                    for (int r = 0; r < t.nr; ++r) acc += L[t.l_off + (int64_t)c * t.ld + r] * src[r];
+                    part[t.d_off + c] = acc;
                 }
             }
             for (int64_t q = P.bdiag_steps[st].first; q < P.bdiag_steps[st].second; ++q) {
-                const SolveDiagTask& t = S.bdiag_tasks[q];
-                std::vector<double> z(TS, 0.0);
-                for (int i = 0; i < t.kb; ++i) {
-                    double v = Y[t.x_off + i];
-                    for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + i];
-                    z[i] = v;
-                }
-                for (int i = 0; i < t.kb; ++i) {
-                    double s = 0;
-                    for (int j = 0; j < t.kb; ++j) s += inv[t.inv_off + i * TS + j] * z[j];
-                    Y[t.x_off + i] = s;
+                const SolveTile& tl = S.bdiag_tiles[q];
+                const SolveBlock& b = S.sblocks[tl.blk];
+                const int c0 = TS * tl.t, nc = std::min(TS, b.nb - c0);
+                for (int c = c0; c < c0 + nc; ++c) {
+                    double v = 0;
+                    for (int r = c0; r < b.nb; ++r) {
+                        double z = Y[b.x_off + r];
+                        for (int p = b.part_begin; p < b.part_end; ++p) z -= part[(int64_t)p * SB + r];
+                        v += Xb[b.xb_off + (int64_t)c * b.nb + r] * z;
+                    }
+                    x[b.x_off + c] = v;
                 }
             }
         }
     }
-    x = Y;
 }
 
 } // namespace mfs

@@ -659,116 +659,267 @@ __global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restri
     }
 }
 
-// dot of panel row r (columns 0..kb-1, leading dimension ld) with y held in LDS
+// Backward gather: u = x at the supernode's structure rows (final values of its ancestors)
 template <typename T>
-__device__ __forceinline__ T panel_row_dot(const T* __restrict__ P, int64_t ld, int kb, const T* y, int r) {
-    T s = T(0);
-    for (int c = 0; c < kb; ++c) s += P[c * ld + r] * y[c];
-    return s;
-}
-
-// Forward step for column block k of a supernode: y_k = L_kk^{-1} x_k (stored tile inverse),
-// then rows [r0, r0 + nr) below: x (rows < w) -= L(rows, k) y_k, u (rows >= w) += L(rows, k) y_k.
-// The writer task (nr == 0) stores y_k into Y; no task of the launch writes x_k.
-template <typename T>
-__global__ __launch_bounds__(64) void k_fstep(const FStepTask* __restrict__ tasks,
-                                              const T* __restrict__ L, const T* __restrict__ inv,
-                                              T* __restrict__ xall, T* __restrict__ yall,
-                                              int64_t ldx, T* __restrict__ uall, int64_t ldu) {
-    const FStepTask t = tasks[blockIdx.x];
-    T* x = xall + blockIdx.y * ldx;
-    T* u = uall + blockIdx.y * ldu;
-    __shared__ T xs[TS], y[TS];
-    const int tid = threadIdx.x;
-    xs[tid] = tid < t.kb ? x[t.x_blk + tid] : T(0);
-    __syncthreads();
-    const T* Vi = inv + t.inv_off;
-    T v = T(0);
-    for (int j = 0; j < t.kb; ++j) v += Vi[j * TS + tid] * xs[j];
-    y[tid] = v;
-    __syncthreads();
-    if (t.nr == 0) {
-        if (tid < t.kb) yall[blockIdx.y * ldx + t.x_blk + tid] = v;
-        return;
-    }
-    if (tid >= t.nr) return;
-    const T d = panel_row_dot(L + t.l_off, t.ld, t.kb, y, tid);
-    const int row = t.r0 + tid;
-    if (row < t.w) x[t.x_rows + row] -= d;
-    else u[t.u_off + row - t.w] += d;
-}
-
-// Backward partial sums: part[p][c] = sum over the task's rows of L(row, k0 + c) x(row)
-template <typename T>
-__global__ __launch_bounds__(256) void k_bgemv(const SolveGemvTask* __restrict__ tasks,
-                                               const T* __restrict__ L,
-                                               const int32_t* __restrict__ rows,
-                                               const T* __restrict__ xall, int64_t ldx,
-                                               T* __restrict__ partall, int64_t ldp) {
-    const SolveGemvTask t = tasks[blockIdx.x];
+__global__ __launch_bounds__(256) void k_bgather(const SolveGatherTask* __restrict__ tasks,
+                                                 const int32_t* __restrict__ rows,
+                                                 const T* __restrict__ xall, int64_t ldx,
+                                                 T* __restrict__ uall, int64_t ldu) {
+    const SolveGatherTask t = tasks[blockIdx.x];
     const T* x = xall + blockIdx.y * ldx;
-    T* part = partall + blockIdx.y * ldp;
-    __shared__ T Tt[TS][TS + 1];   // Tt[c][i]
-    __shared__ T xv[TS];
-    __shared__ T red[4][TS];
-    const int tid = threadIdx.x, ci = tid & 63, g = tid >> 6;
-    T acc = T(0);
-    const T* P = L + t.l_off;
-    for (int rb = 0; rb < t.nr; rb += TS) {
-        const int nrb = min(TS, t.nr - rb);
-        for (int q = 0; q < 16; ++q) {
-            const int c = g + 4 * q;
-            if (c < t.kb) Tt[c][ci] = (ci < nrb) ? P[(int64_t)c * t.ld + rb + ci] : T(0);
-        }
-        if (tid < TS) {
-            T v = T(0);
-            if (tid < nrb) {
-                const int row = t.r0 + rb + tid;
-                if (row < t.w) v = x[t.x_rows + row];
-                else {
-                    const int q = row - t.w;
-                    v = x[3 * (int64_t)rows[t.rows_off + q / 3] + q % 3];
-                }
-            }
-            xv[tid] = v;
-        }
-        __syncthreads();
-        if (ci < t.kb) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) acc += Tt[ci][g * 16 + i] * xv[g * 16 + i];
-        }
-        __syncthreads();
-    }
-    red[g][ci] = acc;
-    __syncthreads();
-    if (tid < t.kb) part[(int64_t)t.part * TS + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    T* u = uall + blockIdx.y * ldu + t.u_off;
+    const int32_t* rw = rows + t.rows_off;
+    for (int i = threadIdx.x; i < t.d; i += blockDim.x) u[i] = x[3 * (int64_t)rw[i / 3] + i % 3];
 }
 
-// Backward diagonal block: x_k = L_kk^{-T} (y_k - sum of the partial sums), via the inverse.
+// deterministic wave-wide sum (fixed butterfly); every lane ends with the same value
 template <typename T>
-__global__ __launch_bounds__(64) void k_bdiag(const SolveDiagTask* __restrict__ tasks,
-                                              const T* __restrict__ inv,
-                                              const T* __restrict__ partall, int64_t ldp,
-                                              T* __restrict__ xall, int64_t ldx) {
-    const SolveDiagTask t = tasks[blockIdx.x];
-    T* x = xall + blockIdx.y * ldx + t.x_off;
-    const T* part = partall + blockIdx.y * ldp;
-    __shared__ T vi[TS][TS + 1];   // vi[c][r] = inverse(r, c)
-    __shared__ T z[TS];
-    const int tid = threadIdx.x, kb = t.kb;
-    const T* Vi = inv + t.inv_off;
-    for (int c = 0; c < kb; ++c) vi[c][tid] = Vi[c * TS + tid];
-    T v = T(0);
-    if (tid < kb) {
-        v = x[tid];
-        for (int p = t.part_begin; p < t.part_end; ++p) v -= part[(int64_t)p * TS + tid];
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block inverses X = L_bb^{-1} (lower triangular, nb x nb, column-major, ld nb), one workgroup per
+// 64-column tile j: X_jj = stored tile inverse, then for i > j
+//   X_ij = -X_ii * sum_{m=j}^{i-1} L_im X_mj
+// with 64x64x64 tile products on MFMA (2x2 waves, each a 32x32 quadrant of 16x16 fragments).
+// The tiles X_mj this workgroup wrote are read back after a barrier (same workgroup, same CU).
+template <typename T>
+__global__ __launch_bounds__(256) void k_blkinv(const SolveTile* __restrict__ tiles,
+                                                const SolveBlock* __restrict__ blocks,
+                                                const int32_t* __restrict__ tile_slot,
+                                                const T* __restrict__ L, const T* __restrict__ inv,
+                                                T* __restrict__ Xb) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    const SolveTile st = tiles[blockIdx.x];
+    const SolveBlock b = blocks[st.blk];
+    const int j = st.t, nb = b.nb, nt = (nb + TS - 1) / TS;
+    const int64_t ld = b.ld;
+    T* X = Xb + b.xb_off;
+    const T* Lb = L + b.l_off;
+    __shared__ T As[TS][TS + 1];   // As[k][r] = A(r, k)
+    __shared__ T Bs[TS][TS + 1];   // Bs[k][c] = B(k, c)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+    const int cj = min(TS, nb - TS * j);
+    {
+        const T* Vj = inv + (int64_t)tile_slot[b.ts_off + j] * TS * TS;
+        for (int idx = tid; idx < TS * TS; idx += 256) {
+            const int r = idx & 63, c = idx >> 6;
+            if (r < cj && c < cj) X[(int64_t)(TS * j + c) * nb + TS * j + r] = (r >= c) ? Vj[idx] : T(0);
+        }
     }
-    z[tid] = v;
+    auto mma_tile = [&](acc_t (&acc)[2][2]) {
+#pragma unroll 4
+        for (int k = 0; k < TS; k += 4) {
+            T a[2], bb[2];
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                a[f] = As[k + (lane >> 4)][wr + 16 * f + (lane & 15)];
+                bb[f] = Bs[k + (lane >> 4)][wc + 16 * f + (lane & 15)];
+            }
+#pragma unroll
+            for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+                for (int fj = 0; fj < 2; ++fj) acc[fi][fj] = Mfma<T>::mma(a[fi], bb[fj], acc[fi][fj]);
+        }
+    };
+    for (int i = j + 1; i < nt; ++i) {
+        const int ri = min(TS, nb - TS * i);
+        acc_t acc[2][2];
+#pragma unroll
+        for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[fi][fj][q] = T(0);
+        for (int m = j; m < i; ++m) {
+            const int cm = min(TS, nb - TS * m);
+            __syncthreads();
+            for (int idx = tid; idx < TS * TS; idx += 256) {
+                const int r = idx & 63, k = idx >> 6;      // A = L_im
+                As[k][r] = (r < ri && k < cm) ? Lb[(int64_t)(TS * m + k) * ld + TS * i + r] : T(0);
+                const int kk = idx & 63, c = idx >> 6;     // B = X_mj
+                Bs[kk][c] = (kk < cm && c < cj) ? X[(int64_t)(TS * j + c) * nb + TS * m + kk] : T(0);
+            }
+            __syncthreads();
+            mma_tile(acc);
+        }
+        __syncthreads();
+        // B = the sum (rows k), A = X_ii (lower triangle of the stored tile inverse)
+#pragma unroll
+        for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Bs[wr + 16 * fi + Mfma<T>::row(lane, q)][wc + 16 * fj + (lane & 15)] = acc[fi][fj][q];
+        {
+            const T* Vi = inv + (int64_t)tile_slot[b.ts_off + i] * TS * TS;
+            for (int idx = tid; idx < TS * TS; idx += 256) {
+                const int r = idx & 63, k = idx >> 6;
+                As[k][r] = (r >= k) ? Vi[idx] : T(0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[fi][fj][q] = T(0);
+        mma_tile(acc);
+#pragma unroll
+        for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = wr + 16 * fi + Mfma<T>::row(lane, q), c = wc + 16 * fj + (lane & 15);
+                    if (r < ri && c < cj) X[(int64_t)(TS * j + c) * nb + TS * i + r] = -acc[fi][fj][q];
+                }
+    }
+}
+
+// Forward diagonal: y(rows of tile t) = X(rows, 0 .. 64(t+1)) x_b; lanes over rows, the four waves
+// over interleaved columns, partial sums combined in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sfdiag(const SolveTile* __restrict__ tiles,
+                                                const SolveBlock* __restrict__ blocks,
+                                                const T* __restrict__ Xb,
+                                                const T* __restrict__ xall, T* __restrict__ yall,
+                                                int64_t ldx) {
+    const SolveTile st = tiles[blockIdx.x];
+    const SolveBlock b = blocks[st.blk];
+    const T* x = xall + blockIdx.y * ldx + b.x_off;
+    T* y = yall + blockIdx.y * ldx + b.x_off;
+    __shared__ T z[1024];
+    __shared__ T red[4][TS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = TS * st.t, nr = min(TS, b.nb - r0), nc = min(b.nb, r0 + TS);
+    for (int c = tid; c < nc; c += 256) z[c] = x[c];
     __syncthreads();
-    if (tid < kb) {
-        T s = T(0);
-        for (int j = 0; j < kb; ++j) s += vi[tid][j] * z[j];   // (L^{-1})^T row tid = column tid of L^{-1}
-        x[tid] = s;
+    const T* Xr = Xb + b.xb_off + r0 + lane;
+    T acc = T(0);
+    if (lane < nr) {
+        // batches of 8 independent loads (columns wave, wave + 4, ...) keep the memory pipe full
+        int c = wave;
+        for (; c + 28 < nc; c += 32) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = Xr[(int64_t)(c + 4 * q) * b.nb];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += v[q] * z[c + 4 * q];
+        }
+        for (; c < nc; c += 4) acc += Xr[(int64_t)c * b.nb] * z[c];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (tid < nr) y[r0 + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+}
+
+// Forward update: rows [r0, r0 + nr) below block b: x -= L(rows, b) y_b (rows of the supernode)
+// or u += L(rows, b) y_b (structure rows, the update vector passed to the parent).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sfupd(const SolveRowTask* __restrict__ tasks,
+                                               const T* __restrict__ L,
+                                               const T* __restrict__ yall, T* __restrict__ xall,
+                                               int64_t ldx, T* __restrict__ uall, int64_t ldu) {
+    const SolveRowTask t = tasks[blockIdx.x];
+    __shared__ T yv[1024];
+    __shared__ T red[4][TS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T* y = yall + blockIdx.y * ldx + t.v_off;
+    for (int c = tid; c < t.nc; c += 256) yv[c] = y[c];
+    __syncthreads();
+    const T* P = L + t.l_off + lane;
+    T acc = T(0);
+    if (lane < t.nr) {
+        int c = wave;
+        for (; c + 28 < t.nc; c += 32) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = P[(int64_t)(c + 4 * q) * t.ld];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += v[q] * yv[c + 4 * q];
+        }
+        for (; c < t.nc; c += 4) acc += P[(int64_t)c * t.ld] * yv[c];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (tid < t.nr) {
+        const T v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        if (t.mode == 0) xall[blockIdx.y * ldx + t.d_off + tid] -= v;
+        else uall[blockIdx.y * ldu + t.d_off + tid] += v;
+    }
+}
+
+// Backward partial sums: part[d_off + c] = sum over rows [r0, r0 + nr) of L(row, c) x(row) for the
+// task's nc (<= 16) columns; one wave per column (lanes over rows), x held in registers.
+constexpr int SOLVE_RC = 1024;   // maximum rows per backward partial task
+template <typename T>
+__global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__ tasks,
+                                                const T* __restrict__ L,
+                                                const T* __restrict__ xall, int64_t ldx,
+                                                const T* __restrict__ uall, int64_t ldu,
+                                                T* __restrict__ partall, int64_t ldp) {
+    const SolveRowTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T* src = (t.mode == 0 ? xall + blockIdx.y * ldx : uall + blockIdx.y * ldu) + t.v_off;
+    T xv[SOLVE_RC / 64];
+#pragma unroll
+    for (int m = 0; m < SOLVE_RC / 64; ++m) {
+        const int r = lane + 64 * m;
+        xv[m] = r < t.nr ? src[r] : T(0);
+    }
+    T* part = partall + blockIdx.y * ldp + t.d_off;
+    for (int c = wave; c < t.nc; c += 4) {
+        const T* P = L + t.l_off + (int64_t)c * t.ld + lane;
+        T v[SOLVE_RC / 64];
+#pragma unroll
+        for (int m = 0; m < SOLVE_RC / 64; ++m) v[m] = (lane + 64 * m < t.nr) ? P[64 * m] : T(0);
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < SOLVE_RC / 64; ++m) acc += v[m] * xv[m];
+        acc = wave_sum(acc);
+        if (lane == 0) part[c] = acc;
+    }
+}
+
+// Backward diagonal: x(columns of tile t) = X(rows >= 64t, columns)^T z, z = y_b - sum of the
+// block's partial sums (fixed order); one wave per column, lanes over rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sbdiag(const SolveTile* __restrict__ tiles,
+                                                const SolveBlock* __restrict__ blocks,
+                                                const T* __restrict__ Xb,
+                                                const T* __restrict__ yall, T* __restrict__ xall,
+                                                int64_t ldx, const T* __restrict__ partall, int64_t ldp,
+                                                int sb) {
+    const SolveTile st = tiles[blockIdx.x];
+    const SolveBlock b = blocks[st.blk];
+    const T* y = yall + blockIdx.y * ldx + b.x_off;
+    T* x = xall + blockIdx.y * ldx + b.x_off;
+    const T* part = partall + blockIdx.y * ldp;
+    __shared__ T z[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = TS * st.t, ncol = min(TS, b.nb - c0), nz = b.nb - c0;
+    for (int r = tid; r < nz; r += 256) {
+        T v = y[c0 + r];
+        for (int p = b.part_begin; p < b.part_end; ++p) v -= part[(int64_t)p * sb + c0 + r];
+        z[r] = v;
+    }
+    __syncthreads();
+    for (int c = wave; c < ncol; c += 4) {
+        const T* Xc = Xb + b.xb_off + (int64_t)(c0 + c) * b.nb + c0;
+        T v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = (lane + 64 * m < nz) ? Xc[lane + 64 * m] : T(0);
+        T acc = T(0);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc += v[m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
+        acc = wave_sum(acc);
+        if (lane == 0) x[c0 + c] = acc;
     }
 }
 
@@ -886,7 +1037,8 @@ __global__ void k_sum_final(const double* __restrict__ part, double* __restrict_
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
-// trs_kernels::MultiplyLowerBlock semantics on the device, sharing the forward-solve row dot.
+// trs_kernels::MultiplyLowerBlock semantics on the device: the forward update's row dot
+// (lanes over rows, columns in order) with the rows scattered through `indices`.
 __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start, int64_t size,
                       int64_t degree, const double* __restrict__ A, int64_t lda,
                       double* __restrict__ B, int64_t ldb) {
@@ -895,7 +1047,8 @@ __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start
     for (int64_t c = threadIdx.x; c < size; c += blockDim.x) y[c] = Bc[start + c];
     __syncthreads();
     for (int64_t i = threadIdx.x; i < degree; i += blockDim.x) {
-        const double v = panel_row_dot(A, lda, (int)size, y, (int)i);
+        double v = 0.0;
+        for (int64_t c = 0; c < size; ++c) v += A[c * lda + i] * y[c];
         Bc[indices[i / BS] + i % BS] -= v;
     }
 }
@@ -920,13 +1073,7 @@ size_t FactorizerDev::fsize() const { return precision == 1 ? sizeof(float) : si
 void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
                              const int64_t* pins, int64_t npins) {
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
-    SymbolicOptions opt;
-    // tuning knobs: MFS_OUTER (outer panel block, multiple of 64), MFS_SOLVE_CHUNK (backward
-    // partial-sum rows per task, multiple of 64), MFS_LEAF (nested-dissection leaf size)
-    if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
-    if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::max(64, atoi(e) / 64 * 64);
-    if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
-    auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, opt));
+    auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, options_from_env()));
     ftype = 0;
     stash_valid_ = false;
     // injection plan entries with per-entry leading dimension and factor-order block row/column
@@ -952,10 +1099,14 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
         for (auto& p : S->zero_ranges) { zr.push_back(p.first); zr.push_back(p.second); }
         zr_.upload(zr);
     }
-    fg_.upload(S->fgather_tasks);
-    fs_.upload(S->fstep_tasks);
-    bd_.upload(S->bdiag_tasks);
-    bv_.upload(S->bgemv_tasks);
+    gath_.upload(S->gather_tasks);
+    sblk_.upload(S->sblocks);
+    invt_.upload(S->inv_tiles);
+    fdt_.upload(S->fdiag_tiles);
+    bdt_.upload(S->bdiag_tiles);
+    fupd_.upload(S->fupd_tasks);
+    bprt_.upload(S->bpart_tasks);
+    tslot_.upload(S->tile_slot);
     children_.upload(S->children);
     usoff_.upload(S->u_solve_off);
     rows_.upload(S->rows);
@@ -976,6 +1127,7 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     L_.alloc(std::max<int64_t>(S->L_size, 1) * fs);
     U_.alloc(std::max<int64_t>(S->U_size, 1) * fs);
     inv_.alloc(S->inv_slots * TS * TS * fs);
+    Xb_.alloc(S->xb_size * fs);
     if (precision == 1) scale_.alloc(3 * S->n_red * sizeof(double));
     status_.alloc(sizeof(int) * 4);
     sym = std::move(S);
@@ -1057,6 +1209,14 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
             prof.end(s);
         }
     }
+    // explicit inverses of the solve column blocks' diagonal parts
+    if (!S.inv_tiles.empty()) {
+        prof.begin(KernelProfile::BLKINV, s, 0.0, (int64_t)S.inv_tiles.size());
+        k_blkinv<T><<<(unsigned)S.inv_tiles.size(), 256, 0, s>>>(invt_.as<SolveTile>(), sblk_.as<SolveBlock>(),
+                                                               tslot_.as<int32_t>(), L, inv, Xb_.as<T>());
+        LAUNCH_CHECK();
+        prof.end(s);
+    }
     HIP_CHECK(hipEventRecord(ev[2], s));
     int st = 0;
     HIP_CHECK(hipMemcpyAsync(&st, status_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1087,13 +1247,18 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
     else numeric_t<double>(Ax, sigma, Bx, stream);
 }
 
+// Supernodal solves in factor order, in place on xp (nrhs columns, leading dimension 3 n_red).
+// Forward, level by level: gather the children's update vectors, then per column-block step
+// y_b = X_b x_b (into Y) and the update of every row below.  Backward, top level first: gather
+// the final x of each supernode's structure rows, then per step (last block first) partial sums
+// of L(rows below, b)^T x and x_b = X_b^T (y_b - partials) (into xp).
 template <typename T>
 void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
     const int64_t ldx = 3 * S.n_red;
     const int64_t ldu = std::max<int64_t>(S.u_solve_size, 1);
-    const int64_t ldp = S.bpart_size * TS;
+    const int64_t ldp = S.bpart_size * S.solve_block;
     const size_t fs = sizeof(T);
     usolve_.alloc((size_t)ldu * nrhs * fs);
     bpart_.alloc((size_t)ldp * nrhs * fs);
@@ -1102,24 +1267,33 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
     T* part = bpart_.as<T>();
     T* Y = ysolve_.as<T>();
     const T* L = L_.as<T>();
-    const T* inv = inv_.as<T>();
-    const int32_t* sdeg = sdeg_.as<int32_t>();
-    const int64_t* reloff = reloff_.as<int64_t>();
+    const T* Xb = Xb_.as<T>();
+    const SolveBlock* blk = sblk_.as<SolveBlock>();
+    const SolveGatherTask* gt = gath_.as<SolveGatherTask>();
     for (const LevelPlan& P : S.levels) {
-        if (P.fgather.second > P.fgather.first) {
-            dim3 g((unsigned)(P.fgather.second - P.fgather.first), (unsigned)nrhs);
+        if (P.gather.second > P.gather.first) {
+            dim3 g((unsigned)(P.gather.second - P.gather.first), (unsigned)nrhs);
             prof.begin(KernelProfile::FGATHER, s);
-            k_fgather<T><<<g, 256, 0, s>>>(fg_.as<SolveGatherTask>() + P.fgather.first, children_.as<int32_t>(),
-                                           sdeg, reloff, rel_.as<int32_t>(), usoff_.as<int64_t>(), xp, ldx, u, ldu);
+            k_fgather<T><<<g, 256, 0, s>>>(gt + P.gather.first, children_.as<int32_t>(), sdeg_.as<int32_t>(),
+                                           reloff_.as<int64_t>(), rel_.as<int32_t>(), usoff_.as<int64_t>(),
+                                           xp, ldx, u, ldu);
             LAUNCH_CHECK();
             prof.end(s);
         }
-        for (size_t t = 0; t < P.fstep_steps.size(); ++t) {
-            auto [v0, v1] = P.fstep_steps[t];
-            if (v1 > v0) {
-                dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+        for (size_t t = 0; t < P.fdiag_steps.size(); ++t) {
+            auto [d0, d1] = P.fdiag_steps[t];
+            if (d1 > d0) {
+                dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
+                prof.begin(KernelProfile::FDIAG, s);
+                k_sfdiag<T><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
+                LAUNCH_CHECK();
+                prof.end(s);
+            }
+            auto [u0, u1] = P.fupd_steps[t];
+            if (u1 > u0) {
+                dim3 g((unsigned)(u1 - u0), (unsigned)nrhs);
                 prof.begin(KernelProfile::FGEMV, s);
-                k_fstep<T><<<g, 64, 0, s>>>(fs_.as<FStepTask>() + v0, L, inv, xp, Y, ldx, u, ldu);
+                k_sfupd<T><<<g, 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -1127,12 +1301,19 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
     }
     for (int lv = (int)S.levels.size() - 1; lv >= 0; --lv) {
         const LevelPlan& P = S.levels[lv];
+        if (P.gather.second > P.gather.first) {
+            dim3 g((unsigned)(P.gather.second - P.gather.first), (unsigned)nrhs);
+            prof.begin(KernelProfile::BGATHER, s);
+            k_bgather<T><<<g, 256, 0, s>>>(gt + P.gather.first, rows_.as<int32_t>(), xp, ldx, u, ldu);
+            LAUNCH_CHECK();
+            prof.end(s);
+        }
         for (size_t t = 0; t < P.bdiag_steps.size(); ++t) {
-            auto [v0, v1] = P.bgemv_steps[t];
-            if (v1 > v0) {
-                dim3 g((unsigned)(v1 - v0), (unsigned)nrhs);
+            auto [p0, p1] = P.bpart_steps[t];
+            if (p1 > p0) {
+                dim3 g((unsigned)(p1 - p0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BGEMV, s);
-                k_bgemv<T><<<g, 256, 0, s>>>(bv_.as<SolveGemvTask>() + v0, L, rows_.as<int32_t>(), Y, ldx, part, ldp);
+                k_sbpart<T><<<g, 256, 0, s>>>(bprt_.as<SolveRowTask>() + p0, L, xp, ldx, u, ldu, part, ldp);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -1140,16 +1321,13 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BDIAG, s);
-                k_bdiag<T><<<g, 64, 0, s>>>(bd_.as<SolveDiagTask>() + d0, inv, part, ldp, Y, ldx);
+                k_sbdiag<T><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
+                                              S.solve_block);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
         }
     }
-    prof.begin(KernelProfile::MAPS, s);
-    k_copy<T><<<std::min<unsigned>(nblk(ldx * nrhs, 256), 8192), 256, 0, s>>>(Y, xp, ldx * nrhs);
-    LAUNCH_CHECK();
-    prof.end(s);
 }
 
 // factor-order double vectors in/out; FP32 factors solve the scaled system in single precision
@@ -1164,11 +1342,15 @@ void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
     xf_.alloc((size_t)n * nrhs * sizeof(float));
     float* xf = xf_.as<float>();
     dim3 g(nblk(n, 256), (unsigned)nrhs);
+    prof.begin(KernelProfile::CONVERT, s);
     k_to_factor<float><<<g, 256, 0, s>>>(xp, scale_.as<double>(), xf, n, n);
     LAUNCH_CHECK();
+    prof.end(s);
     solve_t<float>(xf, nrhs, stream);
+    prof.begin(KernelProfile::CONVERT, s);
     k_from_factor<float><<<g, 256, 0, s>>>(xf, scale_.as<double>(), xp, n, n);
     LAUNCH_CHECK();
+    prof.end(s);
 }
 
 void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
@@ -1267,16 +1449,20 @@ void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, 
     double rel = 0;
     while (true) {
         spmv(Ax, x, y, stream);
+        prof.begin(KernelProfile::REFINE, s);
         k_residual<<<nblk(nf, 256), 256, 0, s>>>(b, y, mask, r, nf);
         LAUNCH_CHECK();
         sumsq(r, nrm + 1);
+        prof.end(s);
         HIP_CHECK(hipMemcpyAsync(h, nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
         rel = h[0] > 0 ? std::sqrt(h[1] / h[0]) : std::sqrt(h[1]);
         if (rel <= tol || it >= max_iters) break;
         solve(r, dx, 1, stream);
+        prof.begin(KernelProfile::REFINE, s);
         k_axpy1<<<nblk(nf, 256), 256, 0, s>>>(x, dx, nf);
         LAUNCH_CHECK();
+        prof.end(s);
         ++it;
     }
     if (iters) *iters = it;
@@ -1320,6 +1506,8 @@ void FactorizerDev::stash() {
     HIP_CHECK(hipMemcpy(Lstash_.p, L_.p, L_.bytes, hipMemcpyDeviceToDevice));
     invstash_.alloc(inv_.bytes);
     HIP_CHECK(hipMemcpy(invstash_.p, inv_.p, inv_.bytes, hipMemcpyDeviceToDevice));
+    xbstash_.alloc(Xb_.bytes);
+    HIP_CHECK(hipMemcpy(xbstash_.p, Xb_.p, Xb_.bytes, hipMemcpyDeviceToDevice));
     if (precision == 1) {
         scalestash_.alloc(scale_.bytes);
         HIP_CHECK(hipMemcpy(scalestash_.p, scale_.p, scale_.bytes, hipMemcpyDeviceToDevice));
@@ -1331,11 +1519,13 @@ void FactorizerDev::stash() {
 
 void FactorizerDev::swap_stash() {
     if (!stash_valid_) throw Error(-2, "No stashed factorization");
-    // the panels, the tile inverses and (FP32) the scaling all belong to the factor
+    // the panels, the tile and block inverses and (FP32) the scaling all belong to the factor
     std::swap(L_.p, Lstash_.p);
     std::swap(L_.bytes, Lstash_.bytes);
     std::swap(inv_.p, invstash_.p);
     std::swap(inv_.bytes, invstash_.bytes);
+    std::swap(Xb_.p, xbstash_.p);
+    std::swap(Xb_.bytes, xbstash_.bytes);
     if (precision == 1) {
         std::swap(scale_.p, scalestash_.p);
         std::swap(scale_.bytes, scalestash_.bytes);
@@ -1348,6 +1538,7 @@ void FactorizerDev::swap_stash() {
 void FactorizerDev::clear_stash() {
     Lstash_.release();
     invstash_.release();
+    xbstash_.release();
     scalestash_.release();
     stash_valid_ = false;
 }

@@ -87,50 +87,47 @@ struct ExtendTask {         // extend-add of child's update matrix into the pare
     int32_t pad;
 };
 
-struct SolveDiagTask {      // backward: x_k = L_kk^{-T} (y_k - sum of partials), via the stored inverse
-    int64_t inv_off;        // offset of the tile inverse (64x64, column-major)
-    int32_t ld, kb;
-    int64_t x_off;          // scalar offset in the permuted solution vector
-    int32_t part_begin, part_end;   // partial-sum slots to subtract first
+// Supernodal solves work on column blocks of at most `solve_block` columns of a supernode.  Each
+// block's diagonal part is inverted explicitly after the numeric factorization (X = L_bb^{-1},
+// lower triangular, nb x nb column-major), so a forward step is y_b = X x_b followed by the
+// update of every row below, and a backward step is x_b = X^T (y_b - L(rows, b)^T x(rows)).
+struct SolveBlock {
+    int64_t l_off;          // L offset of the block's first diagonal entry (c0, c0)
+    int64_t xb_off;         // offset of the block inverse X (ld = nb)
+    int64_t x_off;          // factor-order scalar offset of the block's first column
+    int32_t ld, nb;         // panel leading dimension, block width (scalars)
+    int32_t ts_off;         // index of the block's first 64-column tile in the tile-slot list
+    int32_t part_begin, part_end;   // backward: partial-sum slots to subtract
+    int32_t pad;
+};
+static_assert(sizeof(SolveBlock) == 48, "SolveBlock layout");
+
+struct SolveTile {          // (block, 64-row or 64-column tile of the block)
+    int32_t blk, t;
 };
 
-struct FStepTask {          // forward step: y_k = L_kk^{-1} x_k, then rows [r0, r0+nr) -= L(rows,k) y_k
-    int64_t inv_off;        // tile inverse
-    int64_t l_off;          // L offset of (r0, k0)  (unused by the writer task)
-    int32_t ld, kb;
-    int32_t nr;             // rows of this task (0 = writer task: stores y_k)
-    int32_t w;              // supernode width (scalars): rows < w go to x, rows >= w go to u
-    int32_t r0;             // first row (scalar, within panel)
-    int32_t pad;
-    int64_t x_blk;          // x / y offset of the column block
-    int64_t x_rows;         // x offset of panel row 0 (rows < w)
-    int64_t u_off;          // u offset of structure row 0 (rows >= w)
+struct SolveRowTask {       // a row range [r0, r0 + nr) of a supernode panel against nc block columns
+    int64_t l_off;          // L offset of (first row, first column)
+    int64_t v_off;          // forward: offset of y_b in Y; backward: offset of x(first row) in its source
+    int64_t d_off;          // forward: x / u offset of the first row; backward: slot * solve_block + column
+    int32_t ld, nr, nc;
+    int32_t mode;           // forward: 0 x -= L y, 1 u += L y; backward: 0 rows from x, 1 rows from gathered u
 };
+static_assert(sizeof(SolveRowTask) == 40, "SolveRowTask layout");
 
 struct TaskRange {          // [b, m): 128x128-tile GEMM tasks, [m, e): 64x64-tile tasks
     int64_t b = 0, m = 0, e = 0;
 };
 
-struct SolveGemvTask {      // forward: rows [r0, r0+nr) of a panel, column block [k0, k0+kb)
-    int64_t l_off;          // L offset of (r0, k0)
-    int32_t ld, nr, kb;
-    int32_t w;              // supernode width (scalars): rows < w go to x, rows >= w go to u
-    int32_t r0;             // first row (scalar, within panel)
-    int32_t pad;
-    int64_t x_cols;         // x offset of column block (input y)
-    int64_t x_rows;         // x offset of panel row 0 (for rows < w)
-    int64_t u_off;          // u buffer offset of struct row 0 (for rows >= w)
-    int64_t rows_off;       // offset into the scalar row-index array for rows >= w (backward)
-    int32_t part;           // backward: output partial slot
-    int32_t pad2;
-};
-
-struct SolveGatherTask {    // forward: gather children update vectors into x / u of parent
+struct SolveGatherTask {    // forward: gather children update vectors into x / u of parent;
+                            // backward: gather x of the supernode's structure rows into u
     int32_t sn;             // supernode
     int32_t child_begin, child_end;   // range in child list
     int32_t w, d;           // scalars
+    int32_t pad;
     int64_t x_off;          // x offset of the supernode's first column
     int64_t u_off;          // u offset of the supernode (d scalars)
+    int64_t rows_off;       // offset of the structure rows (blocks) in the row list
 };
 
 struct LevelPlan {
@@ -141,11 +138,11 @@ struct LevelPlan {
     std::vector<TaskRange> update_steps;                     // per step: gemm tasks
     TaskRange syrk;                                          // gemm tasks
     std::pair<int64_t,int64_t> zero_u{0, 0};                 // U arena range to clear (doubles)
-    // forward solve
-    std::pair<int64_t,int64_t> fgather{0, 0};
-    std::vector<std::pair<int64_t,int64_t>> fstep_steps;     // per step: range in fstep tasks
-    // backward solve (steps in execution order)
-    std::vector<std::pair<int64_t,int64_t>> bgemv_steps, bdiag_steps;
+    // solves: gather tasks (forward: children -> x/u; backward: x -> u), and per column-block
+    // step the diagonal tiles and the row tasks (backward steps in execution order)
+    std::pair<int64_t,int64_t> gather{0, 0};
+    std::vector<std::pair<int64_t,int64_t>> fdiag_steps, fupd_steps;
+    std::vector<std::pair<int64_t,int64_t>> bpart_steps, bdiag_steps;
 };
 
 struct Symbolic {
@@ -191,15 +188,17 @@ struct Symbolic {
     std::vector<ExtendTask> extend_tasks;
     std::vector<PotrfTask> potrf_tasks;
     std::vector<GemmTask> gemm_tasks;
-    std::vector<SolveGatherTask> fgather_tasks;
-    std::vector<FStepTask> fstep_tasks;
-    std::vector<SolveDiagTask> bdiag_tasks;
-    std::vector<SolveGemvTask> bgemv_tasks;
     std::vector<std::pair<int64_t,int64_t>> zero_ranges;   // U arena (offset, count) to clear
-    std::vector<int64_t> solve_rows;           // scalar x offsets for struct rows (per sn)
-    std::vector<int64_t> solve_rows_off;       // per supernode offset into solve_rows
-    std::vector<int32_t> solve_child_list;     // forward gather: child supernode ids
-    int64_t bpart_size = 0;                    // partial-sum slots (x64 doubles)
+    // solves
+    int32_t solve_block = 512;                 // column-block width (scalars, multiple of 64)
+    std::vector<int32_t> tile_slot;            // per (supernode, 64-column tile): inverse slot
+    std::vector<SolveBlock> sblocks;           // all column blocks (per supernode, in order)
+    std::vector<SolveTile> inv_tiles;          // block-inverse work: (block, column tile)
+    std::vector<SolveGatherTask> gather_tasks;
+    std::vector<SolveTile> fdiag_tiles, bdiag_tiles;
+    std::vector<SolveRowTask> fupd_tasks, bpart_tasks;
+    int64_t xb_size = 0;                       // block-inverse storage (scalars)
+    int64_t bpart_size = 0;                    // partial-sum slots (x solve_block scalars)
     // full-pattern SpMV (block-CSR over both triangles): for block row i, entries
     // (col block, Ax block index, transpose flag); diagonal flagged with 2.
     std::vector<int64_t> spmv_ptr;
@@ -218,8 +217,14 @@ struct SymbolicOptions {
     int tile = 64;          // diagonal tile / inner step width (scalars)
     int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
-    int solve_chunk = 256;  // backward partial-sum rows per task
+    int solve_block = 512;  // solve column-block width (multiple of 64)
+    int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
 };
+
+// Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),
+// MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
+// (nested-dissection leaf size); values are rounded down to multiples of 64 where required.
+SymbolicOptions options_from_env();
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
                  const int64_t* pins, int64_t npins, const SymbolicOptions& opt);
@@ -228,7 +233,9 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
 // supernodal-panel -> CSC conversion.
 int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L,
                         std::vector<double>& inv);
-void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+void emulate_block_inverses(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& inv,
+                            std::vector<double>& Xb);
+void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::vector<double>& Xb,
                    std::vector<double>& x);
 void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx);
 

@@ -13,6 +13,7 @@
 // that receives its children's update matrices (extend-add) and then -L21 L21^T.  The level
 // schedule (supernode heights in the assembly tree) and every dense tile task are built here,
 // so the device side only replays fixed task lists.
+#include <cstdlib>
 #include "mfs_internal.h"
 
 #include <algorithm>
@@ -132,6 +133,15 @@ struct Arena {
 };
 
 } // namespace
+
+SymbolicOptions options_from_env() {
+    SymbolicOptions opt;
+    if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
+    if (const char* e = getenv("MFS_SOLVE_BLOCK")) opt.solve_block = std::min(1024, std::max(64, atoi(e) / 64 * 64));
+    if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
+    if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
+    return opt;
+}
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
                  int64_t npins, const SymbolicOptions& opt) {
@@ -433,6 +443,38 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     }
     int64_t max_part = 0;
     const int OB = std::max(opt.outer / T, 1) * T;
+    // solve column blocks: per supernode, blocks of SB columns with explicit diagonal inverses
+    const int SB = std::min(1024 / T, std::max(opt.solve_block / T, 1)) * T;   // kernels hold <= 1024
+    S.solve_block = SB;
+    std::vector<int32_t> tile_base(ns + 1, 0), sblock0(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+        const int32_t W = 3 * S.swidth[s];
+        tile_base[s + 1] = tile_base[s] + (W + T - 1) / T;
+        sblock0[s + 1] = sblock0[s] + (W + SB - 1) / SB;
+    }
+    S.tile_slot.assign(tile_base[ns], -1);
+    S.sblocks.resize(sblock0[ns]);
+    {
+        int64_t xb = 0;
+        for (int32_t s = 0; s < ns; ++s) {
+            const int32_t W = 3 * S.swidth[s], ld = W + 3 * S.sdeg[s];
+            for (int32_t k = sblock0[s]; k < sblock0[s + 1]; ++k) {
+                const int32_t c0 = (k - sblock0[s]) * SB, nbk = std::min(SB, W - c0);
+                SolveBlock& b = S.sblocks[k];
+                b.l_off = S.panel_off[s] + (int64_t)c0 * ld + c0;
+                b.xb_off = xb;
+                b.x_off = 3 * (int64_t)S.sfirst[s] + c0;
+                b.ld = ld;
+                b.nb = nbk;
+                b.ts_off = tile_base[s] + c0 / T;
+                b.part_begin = b.part_end = 0;
+                b.pad = 0;
+                xb += (int64_t)nbk * nbk;
+                for (int32_t j = 0; j < (nbk + T - 1) / T; ++j) S.inv_tiles.push_back({k, j});
+            }
+        }
+        S.xb_size = std::max<int64_t>(xb, 1);
+    }
     const int BT = opt.big_tile;
     // Tiles of the lower part (rows >= cols, in region coordinates) of the GEMM region
     // C(r, c) (+)= -A(r, :) B(c, :)^T, r in [max(c0,rlo), rhi), c in [c0, c1), with A(r,k) at
@@ -544,6 +586,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 pt.slot = (int32_t)S.potrf_tasks.size();
                 pt.col0 = 3 * S.sfirst[s] + k0;
                 S.potrf_tasks.push_back(pt);
+                S.tile_slot[tile_base[s] + t] = pt.slot;
                 S.flops_potrf += (double)kb * kb * kb / 3.0;
                 active.push_back({s, pt.slot});
             }
@@ -603,10 +646,9 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             }
             P.syrk = push_tasks(S.gemm_tasks, big, small);
         }
-        // forward solve: gather children's update vectors, then per step one fused launch
-        // (y_k = L_kk^{-1} x_k by the stored inverse; rows below -= L(rows, k) y_k)
+        // solves.  Gather tasks (one per supernode of the level) serve both directions.
         {
-            const int64_t f0 = (int64_t)S.fgather_tasks.size();
+            const int64_t g0 = (int64_t)S.gather_tasks.size();
             for (int32_t s : sl) {
                 SolveGatherTask g{};
                 g.sn = s;
@@ -616,86 +658,80 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 g.d = 3 * S.sdeg[s];
                 g.x_off = 3 * (int64_t)S.sfirst[s];
                 g.u_off = S.u_solve_off[s];
-                S.fgather_tasks.push_back(g);
+                g.rows_off = S.srows_off[s];
+                S.gather_tasks.push_back(g);
             }
-            P.fgather = {f0, (int64_t)S.fgather_tasks.size()};
-            const int64_t pbase = P.potrf_steps.empty() ? 0 : P.potrf_steps[0].first;
-            (void)pbase;
-            for (int32_t t = 0; t < maxsteps; ++t) {
-                const int64_t v0 = (int64_t)S.fstep_tasks.size();
-                int64_t slot = P.potrf_steps[t].first;
-                for (int32_t s : sl) {
-                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                    if (t * T >= W) continue;
-                    const int32_t k0 = t * T, kb = std::min(T, W - k0);
-                    FStepTask ft{};
-                    ft.inv_off = slot * (int64_t)T * T;
-                    ft.ld = ld;
-                    ft.kb = kb;
-                    ft.w = W;
-                    ft.x_blk = 3 * (int64_t)S.sfirst[s] + k0;
-                    ft.x_rows = 3 * (int64_t)S.sfirst[s];
-                    ft.u_off = S.u_solve_off[s];
-                    ft.nr = 0;                    // writer
-                    S.fstep_tasks.push_back(ft);
-                    for (int32_t r0 = k0 + kb; r0 < W + D; r0 += T) {
-                        FStepTask g = ft;
-                        g.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
-                        g.nr = std::min(T, W + D - r0);
-                        g.r0 = r0;
-                        S.fstep_tasks.push_back(g);
-                    }
-                    ++slot;
-                }
-                P.fstep_steps.push_back({v0, (int64_t)S.fstep_tasks.size()});
-            }
+            P.gather = {g0, (int64_t)S.gather_tasks.size()};
         }
-        // backward solve: per step (last column block first): partial gemv^T, then diag
-        {
-            const int C = opt.solve_chunk;
-            for (int32_t t = 0; t < maxsteps; ++t) {
-                const int64_t v0 = (int64_t)S.bgemv_tasks.size();
-                const int64_t d0 = (int64_t)S.bdiag_tasks.size();
-                int32_t part = 0;
-                for (int32_t s : sl) {
-                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                    const int32_t nblk = (W + T - 1) / T;
-                    if (t >= nblk) continue;
-                    const int32_t kblk = nblk - 1 - t;
-                    const int32_t k0 = kblk * T, kb = std::min(T, W - k0);
-                    // inverse slot of this block: the potrf task of step kblk for supernode s
-                    int64_t slot = P.potrf_steps[kblk].first;
-                    for (int32_t s2 : sl) {
-                        if (s2 == s) break;
-                        if (kblk * T < 3 * S.swidth[s2]) ++slot;
-                    }
-                    SolveDiagTask dt{};
-                    dt.inv_off = slot * (int64_t)T * T;
-                    dt.ld = ld;
-                    dt.kb = kb;
-                    dt.x_off = 3 * (int64_t)S.sfirst[s] + k0;
-                    dt.part_begin = part;
-                    for (int32_t r0 = k0 + kb; r0 < W + D; r0 += C) {
-                        SolveGemvTask gv{};
-                        gv.l_off = S.panel_off[s] + (int64_t)k0 * ld + r0;
-                        gv.ld = ld;
-                        gv.nr = std::min(C, W + D - r0);
-                        gv.kb = kb;
-                        gv.w = W;
-                        gv.r0 = r0;
-                        gv.x_cols = 3 * (int64_t)S.sfirst[s] + k0;
-                        gv.x_rows = 3 * (int64_t)S.sfirst[s];
-                        gv.u_off = 0;
-                        gv.rows_off = S.srows_off[s];
-                        gv.part = part++;
-                        S.bgemv_tasks.push_back(gv);
-                    }
-                    dt.part_end = part;
-                    S.bdiag_tasks.push_back(dt);
+        int32_t maxblk = 0;
+        for (int32_t s : sl) maxblk = std::max(maxblk, sblock0[s + 1] - sblock0[s]);
+        // row ranges [lo, hi) split into chunks of at most `ch` rows that do not straddle W
+        auto chunks = [](int32_t lo, int32_t W, int32_t hi, int32_t ch) {
+            std::vector<std::pair<int32_t,int32_t>> out;
+            for (int32_t r = lo; r < std::min(W, hi); r += ch) out.push_back({r, std::min(r + ch, std::min(W, hi))});
+            for (int32_t r = std::max(lo, W); r < hi; r += ch) out.push_back({r, std::min(r + ch, hi)});
+            return out;
+        };
+        // forward: per step k, y_b = X_b x_b for block k of every supernode that has one
+        // (one workgroup per 64-row tile), then every row below the block is updated
+        for (int32_t k = 0; k < maxblk; ++k) {
+            const int64_t d0 = (int64_t)S.fdiag_tiles.size(), u0 = (int64_t)S.fupd_tasks.size();
+            for (int32_t s : sl) {
+                if (k >= sblock0[s + 1] - sblock0[s]) continue;
+                const int32_t bi = sblock0[s] + k;
+                const SolveBlock& b = S.sblocks[bi];
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * SB;
+                for (int32_t t = 0; t < (b.nb + T - 1) / T; ++t) S.fdiag_tiles.push_back({bi, t});
+                for (auto [r0, r1] : chunks(c0 + b.nb, W, W + D, T)) {
+                    SolveRowTask rt{};
+                    rt.l_off = S.panel_off[s] + (int64_t)c0 * b.ld + r0;
+                    rt.v_off = b.x_off;
+                    rt.mode = r0 < W ? 0 : 1;
+                    rt.d_off = r0 < W ? 3 * (int64_t)S.sfirst[s] + r0 : S.u_solve_off[s] + (r0 - W);
+                    rt.ld = b.ld;
+                    rt.nr = r1 - r0;
+                    rt.nc = b.nb;
+                    S.fupd_tasks.push_back(rt);
                 }
-                max_part = std::max<int64_t>(max_part, part);
-                P.bgemv_steps.push_back({v0, (int64_t)S.bgemv_tasks.size()});
-                P.bdiag_steps.push_back({d0, (int64_t)S.bdiag_tasks.size()});
+            }
+            P.fdiag_steps.push_back({d0, (int64_t)S.fdiag_tiles.size()});
+            P.fupd_steps.push_back({u0, (int64_t)S.fupd_tasks.size()});
+        }
+        // backward: per step (last block first), partial sums of L(rows below, b)^T x over row
+        // chunks and 16-column groups, then x_b = X_b^T (y_b - sum of partials) per 64-column tile
+        {
+            const int32_t RC = std::min(1024, std::max(T, opt.solve_chunk / T * T));
+            const int32_t CG = 16;
+            for (int32_t st = 0; st < maxblk; ++st) {
+                const int64_t p0 = (int64_t)S.bpart_tasks.size(), d0 = (int64_t)S.bdiag_tiles.size();
+                int32_t slot = 0;
+                for (int32_t s : sl) {
+                    const int32_t nbs = sblock0[s + 1] - sblock0[s];
+                    if (st >= nbs) continue;
+                    const int32_t k = nbs - 1 - st, bi = sblock0[s] + k;
+                    SolveBlock& b = S.sblocks[bi];
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * SB;
+                    b.part_begin = slot;
+                    for (auto [r0, r1] : chunks(c0 + b.nb, W, W + D, RC)) {
+                        for (int32_t g = 0; g < b.nb; g += CG) {
+                            SolveRowTask rt{};
+                            rt.l_off = S.panel_off[s] + (int64_t)(c0 + g) * b.ld + r0;
+                            rt.mode = r0 < W ? 0 : 1;
+                            rt.v_off = r0 < W ? 3 * (int64_t)S.sfirst[s] + r0 : S.u_solve_off[s] + (r0 - W);
+                            rt.d_off = (int64_t)slot * SB + g;
+                            rt.ld = b.ld;
+                            rt.nr = r1 - r0;
+                            rt.nc = std::min(CG, b.nb - g);
+                            S.bpart_tasks.push_back(rt);
+                        }
+                        ++slot;
+                    }
+                    b.part_end = slot;
+                    for (int32_t j = 0; j < (b.nb + T - 1) / T; ++j) S.bdiag_tiles.push_back({bi, j});
+                }
+                max_part = std::max<int64_t>(max_part, slot);
+                P.bpart_steps.push_back({p0, (int64_t)S.bpart_tasks.size()});
+                P.bdiag_steps.push_back({d0, (int64_t)S.bdiag_tiles.size()});
             }
         }
     }

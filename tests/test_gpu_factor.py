@@ -180,6 +180,16 @@ def test_check_solves_mesh(dev, single):
     check_solves(dev, H, pins, single)
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("block,chunk", [(64, 64), (128, 64), (320, 128)])
+def test_solves_with_small_solve_blocks(dev, block, chunk, single, monkeypatch):
+    """Several column blocks per supernode and several backward partial slots per block."""
+    monkeypatch.setenv("MFS_SOLVE_BLOCK", str(block))
+    monkeypatch.setenv("MFS_SOLVE_CHUNK", str(chunk))
+    H, pins, _, _, _ = mesh_problem(dev, 7, seed=2)
+    check_solves(dev, H, pins, single)
+
+
 @pytest.mark.parametrize("N", [5, 8])
 def test_single_factor_matches_unique_cholesky(dev, N):
     """The FP32 factor, unscaled, against the FP64 oracle factor: agreement to FP32 accuracy,

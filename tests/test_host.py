@@ -195,10 +195,16 @@ def chain_pattern(nb=20):
     return H, Ax
 
 
-@pytest.mark.parametrize("case", ["chain", "mesh5", "mesh7"])
-def test_device_solve_schedule_replay(case):
-    """Forward/backward solve task lists (with the stored tile inverses), replayed on the host,
-    solve P A P^T y = P b to 1e-12."""
+@pytest.mark.parametrize("case,block,chunk", [("chain", None, None), ("mesh5", None, None),
+                                              ("mesh7", None, None), ("mesh7", 64, 64),
+                                              ("mesh7", 128, 64), ("mesh5", 192, 128)])
+def test_device_solve_schedule_replay(case, block, chunk, monkeypatch):
+    """Forward/backward solve task lists (column blocks with explicit diagonal-block inverses,
+    row tasks, backward partial sums), replayed on the host, solve P A P^T y = P b to 1e-12.
+    Small solve blocks / chunks force several blocks per supernode and several partial slots."""
+    if block:
+        monkeypatch.setenv("MFS_SOLVE_BLOCK", str(block))
+        monkeypatch.setenv("MFS_SOLVE_CHUNK", str(chunk))
     if case == "chain":
         H, Ax = chain_pattern()
         pins = np.zeros(0, np.int64)
