@@ -225,6 +225,10 @@ int mfs_neohookean_host(const double* X_rest, const double* X, const int32_t* te
 int mfs_test_gemm_nt(int32_t M, int32_t N, int32_t K, double alpha, const double* A_dev,
                      int32_t lda, const double* B_dev, int32_t ldb, double beta, double* C_dev,
                      int32_t ldc, void* stream);
+/* The same through the FP32 instantiation of the tile kernel (float operands).               */
+int mfs_test_gemm_nt_f32(int32_t M, int32_t N, int32_t K, float alpha, const float* A_dev,
+                         int32_t lda, const float* B_dev, int32_t ldb, float beta, float* C_dev,
+                         int32_t ldc, void* stream);
 /* trs_kernels::MultiplyLowerBlock semantics (block-compressed row indices, BS=1..3):
  *   B(row_i, :) -= A(i, :) * B(supernode_start : supernode_start + size, :),
  *   row_i = indices[i / BS] + i % BS  for i < degree,

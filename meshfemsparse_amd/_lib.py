@@ -62,6 +62,8 @@ _SIGS = {
                             P_dbl, c_i32],
     "mfs_test_gemm_nt": [c_i32, c_i32, c_i32, c_dbl, c_vp, c_i32, c_vp, c_i32, c_dbl, c_vp, c_i32,
                          c_vp],
+    "mfs_test_gemm_nt_f32": [c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i32, c_vp, c_i32, ctypes.c_float,
+                             c_vp, c_i32, c_vp],
     "mfs_test_multiply_lower_block": [c_i32, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp,
                                       c_i64, c_vp],
 }

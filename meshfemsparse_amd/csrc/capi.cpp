@@ -401,6 +401,13 @@ int mfs_test_gemm_nt(int32_t M, int32_t N, int32_t K, double alpha, const double
     MFS_CATCH
 }
 
+int mfs_test_gemm_nt_f32(int32_t M, int32_t N, int32_t K, float alpha, const float* A, int32_t lda,
+                         const float* B, int32_t ldb, float beta, float* C, int32_t ldc, void* stream) {
+    MFS_TRY
+    mfs::test_gemm_nt(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+    MFS_CATCH
+}
+
 int mfs_test_multiply_lower_block(int32_t BS, const int64_t* indices, int64_t start, int64_t size,
                                   int64_t degree, const double* A, int64_t lda, int32_t nrhs,
                                   double* B, int64_t ldb, void* stream) {

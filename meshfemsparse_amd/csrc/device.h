@@ -102,8 +102,9 @@ private:
 };
 
 // test hooks
-void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, const double* B,
-                  int ldb, double beta, double* C, int ldc, void* stream);
+template <typename T>
+void test_gemm_nt(int M, int N, int K, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
+                  int ldc, void* stream);
 void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, int64_t size,
                                int64_t degree, const double* A, int64_t lda, int nrhs, double* B,
                                int64_t ldb, void* stream);

@@ -495,8 +495,23 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
 // D[i][j] in lane j + 16 (i % 4), register i / 4 for f64_16x16x4 (rows interleaved across the four
 // 16-lane groups), and in lane j + 16 (i / 4), register i % 4 for f32_16x16x4.
 // ------------------------------------------------------------------------------------------
-constexpr int KC = 16;
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Workgroups are dispatched to the 8 XCDs round-robin by workgroup id.  Mode 1 gives XCD x the
+// x-th contiguous run of tasks; mode 2 deals runs of 8 consecutive tasks round-robin (tasks
+// beyond the last full round of 64 keep their position), so each run of 8 tiles shares one L2
+// while the XCDs still draw evenly from the whole launch.
+__device__ __forceinline__ int xcd_task(int b, int n, int mode) {
+    if (mode == 1) {
+        const int x = b & 7, j = b >> 3, q = n >> 3, r = n & 7;
+        return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+    }
+    if (mode == 2 && b < (n & ~63)) {
+        const int x = b & 7, j = b >> 3;
+        return (((j >> 3) << 3) + x) * 8 + (j & 7);
+    }
+    return b;
+}
 
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -515,15 +530,15 @@ template <> struct Mfma<float> {
 };
 
 // (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, int KC>
 __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
                                                  T* __restrict__ L, T* __restrict__ U,
-                                                 const T* __restrict__ inv) {
+                                                 const T* __restrict__ inv, int remap) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int AST = 256 / TM, BST = 256 / TN;           // k stride of the loaders
     constexpr int AL = KC / AST, BL = KC / BST;              // loads per thread per chunk
-    const GemmTask t = tasks[blockIdx.x];
+    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x, remap)];
     const T* A = L + t.a_off;
     const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
     T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
@@ -614,14 +629,28 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     return w;
 }
 
+// tuning knobs: MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (FP32 K
+// chunk, 16 or 32, default 16; FP64 uses 16 so that two workgroups share a CU's LDS)
+static int gemm_remap() {
+    static const int v = getenv("MFS_GEMM_REMAP") ? atoi(getenv("MFS_GEMM_REMAP")) : 0;
+    return v;
+}
+static int gemm_kc32() {
+    static const int v = getenv("MFS_GEMM_KC") ? (atoi(getenv("MFS_GEMM_KC")) >= 32) : 0;
+    return v;
+}
 template <typename T>
 static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
+    const int rm = gemm_remap();
+    const bool k32 = sizeof(T) == 4 && gemm_kc32();
     if (r.m > r.b) {
-        k_gemm<T, 128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv);
+        if (k32) k_gemm<T, 128, 128, 32><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+        else k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
         LAUNCH_CHECK();
     }
     if (r.e > r.m) {
-        k_gemm<T, 64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv);
+        if (k32) k_gemm<T, 64, 64, 32><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+        else k_gemm<T, 64, 64, 16><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
         LAUNCH_CHECK();
     }
 }
@@ -1546,9 +1575,10 @@ void FactorizerDev::clear_stash() {
 // ------------------------------------------------------------------------------------------
 // test hooks
 // ------------------------------------------------------------------------------------------
-void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, const double* B,
-                  int ldb, double beta, double* C, int ldc, void* stream) {
-    if (!(alpha == 1.0 || alpha == -1.0) || !(beta == 0.0 || beta == 1.0))
+template <typename T>
+void test_gemm_nt(int M, int N, int K, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
+                  int ldc, void* stream) {
+    if (!(alpha == T(1) || alpha == T(-1)) || !(beta == T(0) || beta == T(1)))
         throw Error(-1, "the tile kernel supports alpha in {+1,-1} and beta in {0,1}");
     if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, N) || ldc < std::max(1, M))
         throw Error(-1, "bad GEMM dimensions");
@@ -1567,7 +1597,7 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
             g.M = std::min(TT, M - r0);
             g.N = std::min(TT, N - c0);
             g.K = K;
-            g.flags = GT_C_IN_U | GT_B_IN_INV | (beta == 0.0 ? GT_BETA0 : 0) | (alpha > 0 ? GT_ALPHA_POS : 0);
+            g.flags = GT_C_IN_U | GT_B_IN_INV | (beta == T(0) ? GT_BETA0 : 0) | (alpha > T(0) ? GT_ALPHA_POS : 0);
             tasks.push_back(g);
         }
     if (tasks.empty()) return;
@@ -1578,9 +1608,13 @@ void test_gemm_nt(int M, int N, int K, double alpha, const double* A, int lda, c
     r.b = 0;
     r.m = TT == 128 ? (int64_t)tasks.size() : 0;
     r.e = (int64_t)tasks.size();
-    launch_gemm<double>(r, d.as<GemmTask>(), const_cast<double*>(A), C, B, s);
+    launch_gemm<T>(r, d.as<GemmTask>(), const_cast<T*>(A), C, B, s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
+template void test_gemm_nt<double>(int, int, int, double, const double*, int, const double*, int, double,
+                                   double*, int, void*);
+template void test_gemm_nt<float>(int, int, int, float, const float*, int, const float*, int, float,
+                                  float*, int, void*);
 
 void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, int64_t size,
                                int64_t degree, const double* A, int64_t lda, int nrhs, double* B,

@@ -217,13 +217,15 @@ struct SymbolicOptions {
     int tile = 64;          // diagonal tile / inner step width (scalars)
     int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
+    int tile_group = 8;     // GEMM tiles are emitted in groups of this many tile columns
     int solve_block = 512;  // solve column-block width (multiple of 64)
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
 };
 
 // Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
-// (nested-dissection leaf size); values are rounded down to multiples of 64 where required.
+// (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group); values are rounded
+// down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,

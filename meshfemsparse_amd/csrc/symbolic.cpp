@@ -140,6 +140,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_SOLVE_BLOCK")) opt.solve_block = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
+    if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
     return opt;
 }
 
@@ -487,8 +488,13 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         if (c1 <= c0 || K <= 0) return;
         const bool use_big = allow_big && (c1 - c0) >= BT && (rhi - c0) >= BT;
         const int TT = use_big ? BT : T;
-        for (int cc = c0; cc < c1; cc += TT)
-            for (int rr = cc; rr < rhi; rr += TT) {
+        // groups of G tile columns, row by row inside a group: a run of G*G consecutive tasks
+        // covers a G x G block of tiles that shares G row panels of A and G column panels of B
+        // (the kernel hands each XCD a contiguous run of tasks, so a run shares one L2)
+        const int G = std::max(1, opt.tile_group);
+        for (int cg = c0; cg < c1; cg += G * TT)
+            for (int rr = cg; rr < rhi; rr += TT)
+                for (int cc = cg; cc < std::min(c1, cg + G * TT) && cc <= rr; cc += TT) {
                 GemmTask g{};
                 g.a_off = a_base + rr;
                 g.lda = lda;
@@ -567,8 +573,9 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             P.zero_u = {z0, (int64_t)zero_ranges.size()};
         }
         // dense steps: per supernode, outer blocks of `outer` columns; inside, `tile`-wide steps
-        // (diagonal POTRF + tile inverse, TRSM of all rows below, update of the outer block's
-        // remaining columns); after an outer block, one trailing update with K = outer.
+        // (diagonal POTRF + tile inverse, TRSM of all rows below, then the left-looking update of
+        // the next column block from the outer block's finished columns); after an outer block,
+        // one right-looking trailing update of all later columns with K = outer.
         int32_t maxsteps = 0;
         for (int32_t s : sl) maxsteps = std::max<int32_t>(maxsteps, (3 * S.swidth[s] + T - 1) / T);
         std::vector<int64_t> slot_of_step;   // per (supernode at this level) current step's inverse slot
@@ -614,21 +621,24 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 }
                 P.trsm_steps.push_back(push_tasks(S.gemm_tasks, big, small));
             }
-            {   // inner updates (within the outer block) and outer trailing updates
+            {   // after TRSM(t): the next step's column block gets the left-looking update from the
+                // outer block's finished columns (K = k1 - J0), or, when step t closes its outer
+                // block, all later columns get the outer trailing update (K = outer)
                 std::vector<GemmTask> big, small;
                 for (auto [s, slot] : active) {
                     (void)slot;
                     const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                    const int32_t k0 = t * T, kb = std::min(T, W - k0);
+                    const int32_t k0 = t * T, kb = std::min(T, W - k0), k1 = k0 + kb;
                     const int32_t J0 = (k0 / OB) * OB, Jend = std::min(W, J0 + OB);
                     const int64_t off = S.panel_off[s];
-                    // inner: columns [k0+kb, Jend), rows [c, W+D), K = kb
-                    emit_lower(big, small, off + (int64_t)k0 * ld, ld, off + (int64_t)k0 * ld, ld, off, ld,
-                               k0 + kb, Jend, W + D, kb, 0, /*allow_big=*/false);
-                    // outer: after the last inner step of the outer block
-                    if (k0 + kb == Jend && Jend < W)
+                    if (k1 < Jend) {
+                        const int32_t kb1 = std::min(T, W - k1);
+                        emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
+                                   k1, k1 + kb1, W + D, k1 - J0, 0, true);
+                    } else if (Jend < W) {
                         emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
                                    Jend, W, W + D, Jend - J0, 0, true);
+                    }
                 }
                 P.update_steps.push_back(push_tasks(S.gemm_tasks, big, small));
             }

@@ -36,6 +36,28 @@ def test_gemm_nt_tile_kernel(dev, M, N, K, alpha, beta):
     assert np.array_equal(got[:, M:], C[:, M:])       # padding rows untouched
 
 
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (64, 64, 64), (70, 33, 17), (130, 200, 300),
+                                   (256, 256, 1000)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0)])
+def test_gemm_nt_tile_kernel_fp32(dev, M, N, K, alpha, beta):
+    """The FP32 tile kernel against an FP64 product of the same float inputs: FP32 rounding of a
+    K-term sum bounds the error by about K * 2^-24 * sum |a||b| per entry."""
+    rng = np.random.default_rng(M * 1000 + N + K + 7)
+    lda, ldb, ldc = M + 3, N + 5, M + 1
+    A = rng.standard_normal((K, lda)).astype(np.float32)
+    B = rng.standard_normal((K, ldb)).astype(np.float32)
+    C = rng.standard_normal((N, ldc)).astype(np.float32)
+    Ad, Bd, Cd = (torch.from_numpy(a.ravel().copy()).to(dev) for a in (A, B, C))
+    _lib.call("mfs_test_gemm_nt_f32", M, N, K, alpha, _lib.c_vp(Ad.data_ptr()), lda, _lib.c_vp(Bd.data_ptr()),
+              ldb, beta, _lib.c_vp(Cd.data_ptr()), ldc, _s())
+    Am, Bm, Cm = (x.astype(np.float64) for x in (A[:, :M].T, B[:, :N].T, C[:, :M].T))
+    exp = alpha * Am @ Bm.T + beta * Cm
+    bound = (K + 2) * 2.0 ** -24 * (np.abs(Am) @ np.abs(Bm).T + np.abs(Cm))
+    got = Cd.cpu().numpy().reshape(N, ldc)
+    assert (np.abs(got[:, :M].T - exp) <= bound).all()
+    assert np.array_equal(got[:, M:], C[:, M:])       # padding rows untouched
+
+
 def test_multiply_lower_block_reference_kat_on_device(dev):
     with open(os.path.join(GOLDEN, "multiply_lower_block_kat.json")) as f:
         kat = json.load(f)

@@ -71,8 +71,10 @@ def oracle_factor(H, f, sigma=0.0, Bx=None):
     return orc.chol(Cp, Ci, Cx), red, pos
 
 
-@pytest.mark.parametrize("N", [2, 5, 8])
-def test_factor_matches_unique_cholesky(dev, N):
+@pytest.mark.parametrize("N,outer", [(2, None), (5, None), (8, None), (8, 128), (9, 64)])
+def test_factor_matches_unique_cholesky(dev, N, outer, monkeypatch):
+    if outer:
+        monkeypatch.setenv("MFS_OUTER", str(outer))      # several outer blocks per supernode
     H, pins, _, _, _ = mesh_problem(dev, N)
     f = mfs.make_cholesky_factorizer()
     f.factorizeSymbolic(H, pins)

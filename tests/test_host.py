@@ -148,10 +148,14 @@ def emulate(H, pins, Ax, sigma=0.0, x_perm=None):
     return iperm, (Lp, Li, Lx), fc[0]
 
 
-@pytest.mark.parametrize("N,sigma", [(3, 0.0), (5, 0.0), (7, 2.5)])
-def test_device_schedule_replay_matches_oracle(N, sigma):
-    """The device task lists (injection plan, extend-add, tile POTRF/TRSM/updates, Schur
-    updates), replayed on the host, reproduce the oracle's unique Cholesky factor."""
+@pytest.mark.parametrize("N,sigma,outer", [(3, 0.0, None), (5, 0.0, None), (7, 2.5, None), (7, 0.0, 64),
+                                           (8, 0.0, 128)])
+def test_device_schedule_replay_matches_oracle(N, sigma, outer, monkeypatch):
+    """The device task lists (injection plan, extend-add, tile POTRF/TRSM, left-looking in-block
+    updates, outer trailing updates, Schur updates), replayed on the host, reproduce the oracle's
+    unique Cholesky factor.  Small outer blocks put several outer blocks in one supernode."""
+    if outer:
+        monkeypatch.setenv("MFS_OUTER", str(outer))
     tets, Xr = wl.cube_mesh(N)
     X = wl.perturb(N, Xr, 0)
     He, _, _ = wl.neohookean(Xr, X, tets, gradient=False)
