@@ -529,6 +529,33 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// C (+)= alpha * acc for one wave's FM x FN fragments: per fragment row, all loads of C are issued
+// before any store (one memory round trip per row instead of one per element).
+template <typename T, int FM, int FN>
+__device__ __forceinline__ void gemm_store_acc(const typename Mfma<T>::acc_t (&acc)[FM][FN], T* __restrict__ C,
+                                               int64_t ldc, int M, int N, int wm, int wn, int lane,
+                                               T alpha, bool beta0) {
+    const int col_l = lane & 15;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        T cv[FN][4];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = wn + 16 * j + col_l, row = wm + 16 * i + Mfma<T>::row(lane, r);
+                cv[j][r] = (!beta0 && row < M && col < N) ? C[col * ldc + row] : T(0);
+            }
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = wn + 16 * j + col_l, row = wm + 16 * i + Mfma<T>::row(lane, r);
+                if (row < M && col < N) C[col * ldc + row] = cv[j][r] + alpha * acc[i][j][r];
+            }
+    }
+}
+
 // (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
 template <typename T, int TM, int TN, int KC>
 __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
@@ -554,18 +581,37 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
     const int ar = tid % TM, ak = tid / TM, br = tid % TN, bk = tid / TN;
     const int64_t lda = t.lda, ldb = t.ldb;
     const bool arow = ar < t.M, brow = br < t.N;
+    // running operand pointers (rows beyond M / N read row 0 and are zeroed by a select); only
+    // a final partial chunk checks k
+    const T* pa = A + (arow ? ar : 0) + (int64_t)ak * lda;
+    const T* pb = B + (brow ? br : 0) + (int64_t)bk * ldb;
     T ra[AL], rb[BL];
     auto gload = [&](int k0) {
+        if (k0 + KC <= t.K) {
 #pragma unroll
-        for (int q = 0; q < AL; ++q) {
-            const int kk = k0 + ak + AST * q;
-            ra[q] = (arow && kk < t.K) ? A[kk * lda + ar] : T(0);
-        }
+            for (int q = 0; q < AL; ++q) {
+                const T v = pa[(int64_t)(AST * q) * lda];
+                ra[q] = arow ? v : T(0);
+            }
 #pragma unroll
-        for (int q = 0; q < BL; ++q) {
-            const int kk = k0 + bk + BST * q;
-            rb[q] = (brow && kk < t.K) ? B[kk * ldb + br] : T(0);
+            for (int q = 0; q < BL; ++q) {
+                const T v = pb[(int64_t)(BST * q) * ldb];
+                rb[q] = brow ? v : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < AL; ++q) {
+                const int kk = k0 + ak + AST * q;
+                ra[q] = (arow && kk < t.K) ? pa[(int64_t)(AST * q) * lda] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < BL; ++q) {
+                const int kk = k0 + bk + BST * q;
+                rb[q] = (brow && kk < t.K) ? pb[(int64_t)(BST * q) * ldb] : T(0);
+            }
         }
+        pa += KC * lda;
+        pb += KC * ldb;
     };
     auto sstore = [&](int buf) {
 #pragma unroll
@@ -599,27 +645,159 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
     }
     const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
     const bool beta0 = (t.flags & GT_BETA0) != 0;
-    const int64_t ldc = t.ldc;
-    const int col_l = lane & 15;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int col = wn + 16 * j + col_l;
-            if (col >= t.N) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = wm + 16 * i + Mfma<T>::row(lane, r);
-                if (row < t.M) {
-                    T* p = C + col * ldc + row;
-                    const T v = alpha * acc[i][j][r];
-                    *p = beta0 ? v : (*p + v);
-                }
-            }
-        }
+    gemm_store_acc<T, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0);
 }
 
 // executed flops of a task range (2 M N K per tile task)
+// Second-generation tile GEMM (same task format and semantics as k_gemm).  Operand tiles sit in
+// LDS k-contiguous per row (row stride LDK: 16 k values plus padding that keeps the 16-byte
+// accesses of eight consecutive rows in distinct banks).  The MFMA k index of step s is mapped to
+// chunk column 4 g + s for lane group g = lane / 16, so each lane fetches its four k values of a
+// fragment for the whole chunk with one 16-byte LDS read; A and B use the same mapping, so the
+// product is unchanged.  In FP32 the C tile leaves through LDS in two halves for coalesced row
+// access; in FP64 the accumulators are stored directly.
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<float> { static constexpr int KC = 16, LDK = 20; };
+template <> struct GemmCfg<double> { static constexpr int KC = 16, LDK = 18; };
+
+template <typename T, int TM, int TN>
+__global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ tasks,
+                                                  T* __restrict__ L, T* __restrict__ U,
+                                                  const T* __restrict__ inv, int remap) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    constexpr int KC = GemmCfg<T>::KC, LDK = GemmCfg<T>::LDK;
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int AG = TM * (KC / 4) / 256, BG = TN * (KC / 4) / 256;   // (row, 4-k group)s per thread
+    constexpr int OPS = 2 * (TM + TN) * LDK, EPI = WN * (TM + 1);
+    __shared__ __attribute__((aligned(16))) T smem[OPS > EPI ? OPS : EPI];
+    T* const As = smem;                       // [2][TM][LDK]
+    T* const Bs = smem + 2 * TM * LDK;        // [2][TN][LDK]
+    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x, remap)];
+    const T* A = L + t.a_off;
+    const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
+    T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+    const int64_t lda = t.lda, ldb = t.ldb;
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    // loader: thread -> row tid % TM, k groups tid / TM + (256 / TM) q.  Rows beyond M (N) read
+    // row 0 and are zeroed by a select; running pointers advance by KC columns per chunk; only a
+    // final partial chunk checks k.
+    const int ar = tid % TM, br = tid % TN;
+    const int akg = tid / TM, bkg = tid / TN;
+    const bool arow = ar < t.M, brow = br < t.N;
+    const T* pa = A + (arow ? ar : 0) + (int64_t)(4 * akg) * lda;
+    const T* pb = B + (brow ? br : 0) + (int64_t)(4 * bkg) * ldb;
+    const int64_t aq = (int64_t)(4 * (256 / TM)) * lda, bq = (int64_t)(4 * (256 / TN)) * ldb;
+    v4 ra[AG], rb[BG];
+    auto gload = [&](int k0) {
+        if (k0 + KC <= t.K) {
+#pragma unroll
+            for (int q = 0; q < AG; ++q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const T v = pa[q * aq + u * lda];
+                    ra[q][u] = arow ? v : T(0);
+                }
+#pragma unroll
+            for (int q = 0; q < BG; ++q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const T v = pb[q * bq + u * ldb];
+                    rb[q][u] = brow ? v : T(0);
+                }
+        } else {
+#pragma unroll
+            for (int q = 0; q < AG; ++q) {
+                const int kk = k0 + 4 * (akg + (256 / TM) * q);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) ra[q][u] = (arow && kk + u < t.K) ? pa[q * aq + u * lda] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < BG; ++q) {
+                const int kk = k0 + 4 * (bkg + (256 / TN) * q);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rb[q][u] = (brow && kk + u < t.K) ? pb[q * bq + u * ldb] : T(0);
+            }
+        }
+        pa += KC * lda;
+        pb += KC * ldb;
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < AG; ++q)
+            *(v4*)&As[(buf * TM + ar) * LDK + 4 * (akg + (256 / TM) * q)] = ra[q];
+#pragma unroll
+        for (int q = 0; q < BG; ++q)
+            *(v4*)&Bs[(buf * TN + br) * LDK + 4 * (bkg + (256 / TN) * q)] = rb[q];
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    const int fr = lane & 15, fg = 4 * (lane >> 4);
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+        const bool more = k0 + KC < t.K;
+        if (more) gload(k0 + KC);
+        v4 af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = *(const v4*)&As[(buf * TM + wm + 16 * i + fr) * LDK + fg];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = *(const v4*)&Bs[(buf * TN + wn + 16 * j + fr) * LDK + fg];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i][s], bf[j][s], acc[i][j]);
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
+    const bool beta0 = (t.flags & GT_BETA0) != 0;
+    const int64_t ldc = t.ldc;
+    if constexpr (sizeof(T) == 8) {
+        // FP64: straight from the accumulators (measured faster than the LDS round trip)
+        gemm_store_acc<T, FM, FN>(acc, C, ldc, t.M, t.N, wm, wn, lane, alpha, beta0);
+        return;
+    }
+    // FP32: column halves [h WN, (h + 1) WN) staged in LDS as Cs[col][row], then coalesced
+    T* const Cs = smem;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if ((wave >> 1) == h) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        Cs[(16 * j + fr) * (TM + 1) + wm + 16 * i + Mfma<T>::row(lane, q)] = acc[i][j][q];
+        }
+        __syncthreads();
+        // all loads of C first, then all stores: one memory round trip per half
+        constexpr int PER = WN * TM / 256;
+        T cv[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e, r = idx % TM, col = h * WN + idx / TM;
+            cv[e] = (!beta0 && r < t.M && col < t.N) ? C[col * ldc + r] : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e, r = idx % TM, cl = idx / TM, col = h * WN + cl;
+            if (r < t.M && col < t.N) C[col * ldc + r] = cv[e] + alpha * Cs[cl * (TM + 1) + r];
+        }
+        __syncthreads();
+    }
+}
+
 static double gemm_work(const Symbolic& S, const TaskRange& r) {
     double w = 0.0;
     for (int64_t q = r.b; q < r.e; ++q) {
@@ -629,10 +807,15 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     return w;
 }
 
-// tuning knobs: MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (FP32 K
-// chunk, 16 or 32, default 16; FP64 uses 16 so that two workgroups share a CU's LDS)
+// tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 2 for FP32, 1 for FP64),
+// MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (k_gemm's FP32 K chunk,
+// 16 or 32, default 16; FP64 uses 16 so that two workgroups share a CU's LDS)
 static int gemm_remap() {
     static const int v = getenv("MFS_GEMM_REMAP") ? atoi(getenv("MFS_GEMM_REMAP")) : 0;
+    return v;
+}
+static int gemm_version() {
+    static const int v = getenv("MFS_GEMM_V") ? atoi(getenv("MFS_GEMM_V")) : 0;
     return v;
 }
 static int gemm_kc32() {
@@ -642,6 +825,19 @@ static int gemm_kc32() {
 template <typename T>
 static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
     const int rm = gemm_remap();
+    // default: the k-contiguous kernel for FP32, the k-major kernel for FP64 (measured best)
+    const int ver = gemm_version() ? gemm_version() : (sizeof(T) == 4 ? 2 : 1);
+    if (ver == 2) {
+        if (r.m > r.b) {
+            k_gemm2<T, 128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+            LAUNCH_CHECK();
+        }
+        if (r.e > r.m) {
+            k_gemm2<T, 64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+            LAUNCH_CHECK();
+        }
+        return;
+    }
     const bool k32 = sizeof(T) == 4 && gemm_kc32();
     if (r.m > r.b) {
         if (k32) k_gemm<T, 128, 128, 32><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
