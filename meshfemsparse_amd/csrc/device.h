@@ -90,7 +90,7 @@ private:
     size_t fsize() const;
 
     DeviceArray L_, U_, inv_, status_;
-    DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, zr_;
+    DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_;
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;

@@ -14,7 +14,9 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
     const int TS = 64;
     L.assign(S.L_size, 0.0);
     inv.assign((size_t)S.inv_slots * TS * TS, 0.0);
-    std::vector<double> U(S.U_size, 0.0);
+    // the update arena is never cleared on the device: start from NaNs so that any read of an
+    // entry that was not written first shows up in the result
+    std::vector<double> U(S.U_size, std::nan(""));
     int64_t status = INT64_MAX;
     // injection
     for (size_t q = 0; q < S.plan_src.size(); ++q) {
@@ -51,11 +53,8 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
             }
     };
     for (const LevelPlan& P : S.levels) {
-        for (int64_t z = P.zero_u.first; z < P.zero_u.second; ++z)
-            std::fill(U.begin() + S.zero_ranges[z].first,
-                      U.begin() + S.zero_ranges[z].first + S.zero_ranges[z].second, 0.0);
-        for (auto [a, b] : P.extend_passes)
-            for (int64_t q = a; q < b; ++q) {
+        auto extend = [&](const std::pair<int64_t,int64_t>& pass) {
+            for (int64_t q = pass.first; q < pass.second; ++q) {
                 const ExtendTask& t = S.extend_tasks[q];
                 const int32_t* relc = S.rel.data() + t.rel_off;
                 const int kj = t.col_block, tj = relc[kj];
@@ -75,6 +74,8 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                         for (int r = (ki == kj ? c : 0); r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
                 }
             }
+        };
+        for (const auto& pass : P.extend_passes) extend(pass);
         for (size_t st = 0; st < P.potrf_steps.size(); ++st) {
             for (int64_t q = P.potrf_steps[st].first; q < P.potrf_steps[st].second; ++q) {
                 const PotrfTask& t = S.potrf_tasks[q];
@@ -108,6 +109,7 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
             for (int64_t q = P.update_steps[st].b; q < P.update_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
         }
         for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) gemm(S.gemm_tasks[q]);
+        for (const auto& pass : P.extend_u_passes) extend(pass);
     }
     return status == INT64_MAX ? -1 : status;
 }

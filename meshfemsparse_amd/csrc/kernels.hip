@@ -113,13 +113,6 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
-template <typename T>
-__global__ void k_zero_ranges(T* __restrict__ base, const int64_t* __restrict__ ranges) {
-    const int64_t off = ranges[2 * blockIdx.y], cnt = ranges[2 * blockIdx.y + 1];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt;
-         i += (int64_t)gridDim.x * blockDim.x)
-        base[off + i] = T(0);
-}
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
 template <typename T>
@@ -1319,11 +1312,6 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     ext_.upload(S->extend_tasks);
     potrf_.upload(S->potrf_tasks);
     gemm_.upload(S->gemm_tasks);
-    {
-        std::vector<int64_t> zr;
-        for (auto& p : S->zero_ranges) { zr.push_back(p.first); zr.push_back(p.second); }
-        zr_.upload(zr);
-    }
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
     invt_.upload(S->inv_tiles);
@@ -1389,24 +1377,18 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
         prof.end(s);
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
-    const int64_t* zr = zr_.as<int64_t>();
     const GemmTask* gt = gemm_.as<GemmTask>();
-    for (const LevelPlan& P : S.levels) {
-        const int64_t nz = P.zero_u.second - P.zero_u.first;
-        if (nz > 0) {
-            dim3 g(64, (unsigned)nz);
-            prof.begin(KernelProfile::ZERO_U, s);
-            k_zero_ranges<T><<<g, 256, 0, s>>>(U, zr + 2 * P.zero_u.first);
-            LAUNCH_CHECK();
-            prof.end(s);
-        }
-        for (auto [a, b] : P.extend_passes)
+    auto extend = [&](const std::vector<std::pair<int64_t,int64_t>>& passes) {
+        for (auto [a, b] : passes)
             if (b > a) {
                 prof.begin(KernelProfile::EXTEND, s);
                 k_extend<T><<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
+    };
+    for (const LevelPlan& P : S.levels) {
+        extend(P.extend_passes);
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];
             if (p1 > p0) {
@@ -1433,6 +1415,7 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
             launch_gemm<T>(P.syrk, gt, L, U, inv, s);
             prof.end(s);
         }
+        extend(P.extend_u_passes);
     }
     // explicit inverses of the solve column blocks' diagonal parts
     if (!S.inv_tiles.empty()) {

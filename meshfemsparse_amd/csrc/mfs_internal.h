@@ -132,12 +132,12 @@ struct SolveGatherTask {    // forward: gather children update vectors into x / 
 
 struct LevelPlan {
     // factorization
-    std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks
+    std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks (into panels)
+    std::vector<std::pair<int64_t,int64_t>> extend_u_passes; // after the Schur update (into U)
     std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
     std::vector<TaskRange> trsm_steps;                       // per step: gemm tasks
     std::vector<TaskRange> update_steps;                     // per step: gemm tasks
     TaskRange syrk;                                          // gemm tasks
-    std::pair<int64_t,int64_t> zero_u{0, 0};                 // U arena range to clear (doubles)
     // solves: gather tasks (forward: children -> x/u; backward: x -> u), and per column-block
     // step the diagonal tiles and the row tasks (backward steps in execution order)
     std::pair<int64_t,int64_t> gather{0, 0};
@@ -188,7 +188,6 @@ struct Symbolic {
     std::vector<ExtendTask> extend_tasks;
     std::vector<PotrfTask> potrf_tasks;
     std::vector<GemmTask> gemm_tasks;
-    std::vector<std::pair<int64_t,int64_t>> zero_ranges;   // U arena (offset, count) to clear
     // solves
     int32_t solve_block = 512;                 // column-block width (scalars, multiple of 64)
     std::vector<int32_t> tile_slot;            // per (supernode, 64-column tile): inverse slot

@@ -518,7 +518,6 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         r.e = (int64_t)all.size();
         return r;
     };
-    std::vector<std::pair<int64_t,int64_t>> zero_ranges;
     for (int32_t lv = 0; lv < S.num_levels; ++lv) {
         LevelPlan& P = S.levels[lv];
         const auto& sl = bylevel[lv];
@@ -527,9 +526,13 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             const int64_t D = 3 * (int64_t)S.sdeg[s];
             if (D > 0) S.u_off[s] = arena.alloc(D * D);
         }
-        // extend-add passes by child rank
+        // extend-add passes by child rank (no two tasks of a pass touch the same parent entry).
+        // Column blocks of a child's U that land in the parent's panel are added before the
+        // parent is factored; those that land in the parent's U are added after the parent's
+        // Schur update, which writes U with beta = 0, so U never needs clearing.
         int32_t maxc = 0;
         for (int32_t s : sl) maxc = std::max(maxc, S.child_ptr[s + 1] - S.child_ptr[s]);
+        auto extend_passes = [&](bool to_u, std::vector<std::pair<int64_t,int64_t>>& passes) {
         for (int32_t r = 0; r < maxc; ++r) {
             const int64_t b0 = (int64_t)S.extend_tasks.size();
             for (int32_t p : sl) {
@@ -537,6 +540,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 const int32_t c = S.children[S.child_ptr[p] + r];
                 if (S.sdeg[c] == 0) continue;
                 for (int32_t kj = 0; kj < S.sdeg[c]; ++kj) {
+                    if ((S.rel[S.rel_off[c] + kj] >= S.swidth[p]) != to_u) continue;
                     ExtendTask t;
                     t.u_child = S.u_off[c];
                     t.p_panel = S.panel_off[p];
@@ -551,9 +555,12 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     S.extend_tasks.push_back(t);
                 }
             }
-            P.extend_passes.push_back({b0, (int64_t)S.extend_tasks.size()});
+            if ((int64_t)S.extend_tasks.size() > b0) passes.push_back({b0, (int64_t)S.extend_tasks.size()});
         }
-        // release children's U (consumed by this level's extend-add)
+        };
+        extend_passes(false, P.extend_passes);
+        // release children's U (consumed by this level's extend-add; the U passes run at the end
+        // of the level and nothing else is allocated in between)
         for (int32_t p : sl)
             for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
                 const int32_t c = S.children[q];
@@ -562,16 +569,6 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     arena.release(S.u_off[c], D * D);
                 }
             }
-        // zero ranges for this level's U (cleared before the extend-add)
-        {
-            const int64_t z0 = (int64_t)zero_ranges.size();
-            for (int32_t s : sl)
-                if (S.u_off[s] >= 0) {
-                    const int64_t D = 3 * (int64_t)S.sdeg[s];
-                    zero_ranges.push_back({S.u_off[s], D * D});
-                }
-            P.zero_u = {z0, (int64_t)zero_ranges.size()};
-        }
         // dense steps: per supernode, outer blocks of `outer` columns; inside, `tile`-wide steps
         // (diagonal POTRF + tile inverse, TRSM of all rows below, then the left-looking update of
         // the next column block from the outer block's finished columns); after an outer block,
@@ -651,11 +648,12 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 if (D == 0) continue;
                 const int64_t off = S.panel_off[s];
                 // region rows/cols are indices into L21 (0..D); A = P[W + r, 0..W), C = U[r, c]
-                emit_lower(big, small, off + W, ld, off + W, ld, S.u_off[s], D, 0, D, D, W, GT_C_IN_U, true,
-                           /*c_is_u=*/true);
+                emit_lower(big, small, off + W, ld, off + W, ld, S.u_off[s], D, 0, D, D, W, GT_C_IN_U | GT_BETA0,
+                           true, /*c_is_u=*/true);
             }
             P.syrk = push_tasks(S.gemm_tasks, big, small);
         }
+        extend_passes(true, P.extend_u_passes);
         // solves.  Gather tasks (one per supernode of the level) serve both directions.
         {
             const int64_t g0 = (int64_t)S.gather_tasks.size();
@@ -748,7 +746,6 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     S.U_size = std::max<int64_t>(arena.peak, 32);
     S.inv_slots = std::max<int64_t>((int64_t)S.potrf_tasks.size(), 1);
     S.bpart_size = std::max<int64_t>(max_part, 1);
-    S.zero_ranges = std::move(zero_ranges);
 
     // ---- symmetric SpMV map over the full pattern (block-CSR, both triangles) ----
     {
