@@ -305,21 +305,25 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 }
 
 // ------------------------------------------------------------------------------------------
-// Diagonal tile POTRF (kb <= 64) + explicit inverse of the tile's Cholesky factor.
-// One wave: lane r owns row r of the tile (kept in LDS, a[c][r]); right-looking column steps.
-// The inverse is kept for the TRSM of this step and for both solve sweeps.
+// Diagonal tile POTRF (kb <= 64) + explicit inverse X = L^{-1} of the tile's Cholesky factor,
+// kept for the TRSM of this step and for both solve sweeps.  The tile sits in LDS (a[c][r]) and
+// is factored in 8 steps of 8 columns; x[c][r] holds X(r, c) once final and, before that, the
+// partial sum S(r, c) = sum_k L(r, k) X(k, c) of the block forward substitution
+// X_bj = -X_bb sum_{k=j}^{b-1} L_bk X_kj.  Step b (columns b0 .. b0+7):
+//   phase 1  wave 0: every lane factors the 8x8 diagonal block redundantly in registers
+//            (left-looking Cholesky and X_bb, no cross-lane traffic on the critical path), then
+//            lane i solves panel row b0+8+i, L(r, b) = A(r, b) X_bb^T;
+//            waves 1-3: the rank-8 update from block b-1 of block columns b+1 .. 7 (deferred from
+//            step b-1) and S += L(:, b-1) X(b-1, :) for the block rows b .. 7;
+//   phase 2  wave 0: lane c < b0 turns S(b0.., c) into X(b0.., c) = -X_bb S; lane 0 stores
+//            L_bb and X_bb; waves 1-3: the rank-8 update from block b of block column b+1.
+// A barrier ends each phase; the 8x8 blocks of the updates map one block per wave pass.  A
+// non-positive or NaN pivot records the first failing column and continues with 1 so no NaN
+// spreads; the host turns it into a pivot failure.
 // ------------------------------------------------------------------------------------------
 constexpr int TS = 64;   // diagonal tile size (scalars)
+constexpr int PB = 8;    // column step of the tile factorization
 
-__device__ __forceinline__ double readlane_t(double v, int lane) {
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float readlane_t(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
 __device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
 __device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 
@@ -327,157 +331,148 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
                                                T* __restrict__ L, T* __restrict__ inv,
                                                int* __restrict__ status) {
-    constexpr int NB = 16;
     const PotrfTask t = tasks[blockIdx.x];
     __shared__ T a[TS][TS + 1];         // a[c][r] = tile(r, c) -> L(r, c)
-    __shared__ T x[TS][TS + 1];         // x[c][r] = X(r, c), X = L^{-1}
-    const int tid = threadIdx.x, kb = t.kb;
+    __shared__ T x[TS][TS + 1];         // x[c][r] = S(r, c) -> X(r, c)
+    const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6, rr = tid & 7, cc = (tid >> 3) & 7;
     const int64_t ld = t.ld;
     T* D = L + t.diag_off;
-    for (int idx = tid; idx < TS * TS; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        T v;
-        if (r < kb && c < kb) v = (r >= c) ? D[c * ld + r] : T(0);
-        else v = (r == c) ? T(1) : T(0);
-        a[c][r] = v;
-        x[c][r] = T(0);
+    {
+        // every load of a thread is issued before the first use (one memory round trip); entries
+        // outside the kb x kb tile read a clamped in-tile address and are replaced by the identity
+        constexpr int PT = TS * TS / 256;
+        T v[PT];
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
+            v[q] = D[(int64_t)min(c, kb - 1) * ld + min(r, kb - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
+            a[c][r] = (r < kb && c < kb) ? ((r >= c) ? v[q] : T(0)) : ((r == c) ? T(1) : T(0));
+            x[c][r] = T(0);
+        }
     }
     __syncthreads();
-    for (int b = 0; b < TS / NB; ++b) {
-        const int b0 = NB * b;
-        if (tid < 64) {   // wave 0: diagonal block factor + inverse in registers
-            const int i = tid & (NB - 1);
-            T p[NB];
+    // A(r, c) -= sum_k L(r, k0 + k) L(c, k0 + k) for the entry (rr, cc) of block (i, j)
+    auto rank8 = [&](int k0, int i, int j) {
+        const int r = PB * i + rr, c = PB * j + cc;
+        T acc = T(0);
 #pragma unroll
-            for (int c = 0; c < NB; ++c) p[c] = (c <= i) ? a[b0 + c][b0 + i] : T(0);
+        for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * a[k0 + k][c];
+        a[c][r] -= acc;
+    };
+    constexpr int NBK = TS / PB;
+#pragma unroll 1
+    for (int bs = 0; bs < NBK; ++bs) {
+        const int b0 = PB * bs;
+        T lb[PB][PB], xb[PB][PB];       // L_bb, X_bb (lower triangles), wave 0
+        if (wave == 0) {
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                T d = readlane_t(p[j], j);
+            for (int r = 0; r < PB; ++r)
+#pragma unroll
+                for (int c = 0; c <= r; ++c) lb[r][c] = a[b0 + c][b0 + r];
+            T rd[PB];                    // 1 / L(j, j)
+#pragma unroll
+            for (int j = 0; j < PB; ++j) {
+                T d = lb[j][j];
+#pragma unroll
+                for (int k = 0; k < j; ++k) d -= lb[j][k] * lb[j][k];
                 if (!(d > T(0))) {
                     if (tid == 0) atomicMin(status, t.col0 + b0 + j);
                     d = T(1);
                 }
                 const T sd = sqrt_t(d);
-                const T l = (i > j) ? p[j] / sd : (i == j ? sd : T(0));
-                p[j] = l;
+                lb[j][j] = sd;
+                rd[j] = T(1) / sd;
 #pragma unroll
-                for (int c = j + 1; c < NB; ++c) {
-                    const T lc = readlane_t(l, c);
-                    if (i >= c) p[c] -= l * lc;
+                for (int i = j + 1; i < PB; ++i) {
+                    T v = lb[i][j];
+#pragma unroll
+                    for (int k = 0; k < j; ++k) v -= lb[i][k] * lb[j][k];
+                    lb[i][j] = v * rd[j];
                 }
             }
-            // X_bb column i: X(r, i) = (delta_ri - sum_{k<r} L(r, k) X(k, i)) / L(r, r)
-            T xc[NB];
 #pragma unroll
-            for (int r = 0; r < NB; ++r) {
-                T sacc = (r == i) ? T(1) : T(0);
+            for (int c = 0; c < PB; ++c) {
+                xb[c][c] = rd[c];
 #pragma unroll
-                for (int k = 0; k < r; ++k) sacc -= readlane_t(p[k], r) * xc[k];
-                xc[r] = sacc / readlane_t(p[r], r);
+                for (int i = c + 1; i < PB; ++i) {
+                    T v = T(0);
+#pragma unroll
+                    for (int k = c; k < i; ++k) v += lb[i][k] * xb[k][c];
+                    xb[i][c] = -v * rd[i];
+                }
             }
-            if (tid < NB) {
+            const int r = b0 + PB + tid;
+            if (r < TS) {
+                T ar[PB];
 #pragma unroll
-                for (int c = 0; c < NB; ++c) a[b0 + c][b0 + i] = (c <= i) ? p[c] : T(0);
+                for (int k = 0; k < PB; ++k) ar[k] = a[b0 + k][r];
 #pragma unroll
-                for (int r = 0; r < NB; ++r) x[b0 + i][b0 + r] = (r >= i) ? xc[r] : T(0);
+                for (int c = 0; c < PB; ++c) {
+                    T v = T(0);
+#pragma unroll
+                    for (int k = 0; k <= c; ++k) v += ar[k] * xb[c][k];
+                    a[b0 + c][r] = v;
+                }
             }
-        }
-        __syncthreads();
-        // panel TRSM: L(r, b0+c) = sum_k A(r, b0+k) X(b0+c, b0+k), rows r >= b0+16
-        const int nrow = TS - b0 - NB;
-        T tv[3];
+        } else if (bs > 0) {
+            const int k0 = b0 - PB;
+            const int m = NBK - 1 - bs;                      // block columns bs+1 .. NBK-1
+            const int na = m * (m + 1) / 2, ns = (NBK - bs) * bs;
+            for (int q = wave - 1; q < na + ns; q += 3) {
+                if (q < na) {
+                    int j = bs + 1, e = q;
+                    while (e >= NBK - j) { e -= NBK - j; ++j; }
+                    rank8(k0, j + e, j);
+                } else {
+                    const int e = q - na, i = bs + e / bs, j = e % bs;
+                    const int r = PB * i + rr, c = PB * j + cc;
+                    T acc = T(0);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int idx = tid + 256 * q;
-            tv[q] = T(0);
-            if (idx < nrow * NB) {
-                const int r = b0 + NB + (idx % nrow), c = idx / nrow;
-                T acc = T(0);
-#pragma unroll
-                for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * x[b0 + k][b0 + c];
-                tv[q] = acc;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int idx = tid + 256 * q;
-            if (idx < nrow * NB) {
-                const int r = b0 + NB + (idx % nrow), c = idx / nrow;
-                a[b0 + c][r] = tv[q];
-            }
-        }
-        __syncthreads();
-        // trailing SYRK: A(r, c) -= sum_k L(r, b0+k) L(c, b0+k) for b0+16 <= c <= r
-        for (int idx = tid; idx < nrow * nrow; idx += 256) {
-            const int r = b0 + NB + (idx % nrow), c = b0 + NB + (idx / nrow);
-            if (r < c) continue;
-            T acc = T(0);
-#pragma unroll
-            for (int k = 0; k < NB; ++k) acc += a[b0 + k][r] * a[b0 + k][c];
-            a[c][r] -= acc;
-        }
-        __syncthreads();
-    }
-    // off-diagonal blocks of X by block rows bi = 1..3: T = sum_{k=bj}^{bi-1} L_{bi,k} X_{k,bj},
-    // X_{bi,bj} = -X_{bi,bi} T  (all bj < bi at once)
-    for (int bi = 1; bi < TS / NB; ++bi) {
-        const int n = bi * NB * NB;           // bj < bi blocks of 16x16
-        T tv[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int idx = tid + 256 * q;
-            tv[q] = T(0);
-            if (idx < n) {
-                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
-                const int r = bi * NB + rr, c = bj * NB + cc;
-                T acc = T(0);
-                for (int k = bj * NB; k < bi * NB; ++k) acc += a[k][r] * x[c][k];
-                tv[q] = acc;
+                    for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * x[c][k0 + k];
+                    x[c][r] += acc;
+                }
             }
         }
         __syncthreads();
+        if (wave == 0) {
+            if (tid < b0) {
+                const int c = tid;
+                T sv[PB];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {       // stash T in the (still zero) X block
-            const int idx = tid + 256 * q;
-            if (idx < n) {
-                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
-                x[bj * NB + cc][bi * NB + rr] = tv[q];
+                for (int i = 0; i < PB; ++i) sv[i] = x[c][b0 + i];
+#pragma unroll
+                for (int i = 0; i < PB; ++i) {
+                    T v = T(0);
+#pragma unroll
+                    for (int k = 0; k <= i; ++k) v += xb[i][k] * sv[k];
+                    x[c][b0 + i] = -v;
+                }
             }
+            if (tid == 0) {
+#pragma unroll
+                for (int r = 0; r < PB; ++r)
+#pragma unroll
+                    for (int c = 0; c <= r; ++c) {
+                        a[b0 + c][b0 + r] = lb[r][c];
+                        x[b0 + c][b0 + r] = xb[r][c];
+                    }
+            }
+        } else if (bs + 1 < NBK) {
+            for (int i = bs + wave; i < NBK; i += 3) rank8(b0, i, bs + 1);
         }
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int idx = tid + 256 * q;
-            tv[q] = T(0);
-            if (idx < n) {
-                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
-                const int r = bi * NB + rr, c = bj * NB + cc;
-                T acc = T(0);
-#pragma unroll
-                for (int k = 0; k < NB; ++k) acc += x[bi * NB + k][r] * x[c][bi * NB + k];
-                tv[q] = -acc;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int idx = tid + 256 * q;
-            if (idx < n) {
-                const int rr = idx & (NB - 1), cc = (idx >> 4) & (NB - 1), bj = idx >> 8;
-                x[bj * NB + cc][bi * NB + rr] = tv[q];
-            }
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < TS * TS; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        if (r < kb && c < kb && r >= c) D[c * ld + r] = a[c][r];
     }
     T* out = inv + (int64_t)t.slot * TS * TS;
-    for (int idx = tid; idx < TS * TS; idx += 256) {
-        const int r = idx & 63, c = idx >> 6;
-        out[idx] = (r < kb && c < kb) ? x[c][r] : T(0);
+#pragma unroll
+    for (int q = 0; q < TS * TS / 256; ++q) {
+        const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
+        const T lv = a[c][r], xv = x[c][r];
+        if (r < kb && c < kb && r >= c) D[(int64_t)c * ld + r] = lv;
+        out[idx] = (r < kb && c < kb) ? xv : T(0);
     }
 }
 
