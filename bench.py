@@ -26,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 PEAK_HBM_GBS = 8000.0
 # kernel classes of mfs_kernel_profile
 KCLASSES = ["fill_L", "inject", "zero_U", "extend", "potrf", "trsm", "update", "syrk", "fgather",
@@ -92,7 +92,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16", "bf16x3"], default="fp64",
+                    help="bf16 / bf16x3: FP32 factor with bfloat16 Schur updates (x3: hi+lo operand split)")
     ap.add_argument("--stress", action="store_true",
                     help="per-element E log-uniform [1e3, 1e7], nu uniform [0.2, 0.45]")
     ap.add_argument("--forward-error", action="store_true",
@@ -141,8 +142,9 @@ def main():
         del He, g
     torch.cuda.synchronize()
     log(f"[bench] generated + uploaded {nseeds} seeds in {time.perf_counter() - t0:.1f}s")
-    single = args.precision == "fp32"
-    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    single = args.precision != "fp64"
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single,
+                                     args.precision if args.precision.startswith("bf16") else None)
     t0 = time.perf_counter()
     f.factorizeSymbolic(H, pins)
     t_sym = time.perf_counter() - t0
@@ -181,6 +183,7 @@ def main():
     kt = not args.no_kernel_timing
     ms = np.zeros(len(KCLASSES))
     cnt = np.zeros(len(KCLASSES), np.int64)
+    work = np.zeros(len(KCLASSES))
     if kt:
         _lib.call("mfs_kernel_profiling", f._h, 1, 1)
         for k in range(args.steps):
@@ -188,6 +191,7 @@ def main():
         torch.cuda.synchronize()
         _lib.call("mfs_kernel_profiling", f._h, 0, 0)
         _lib.call("mfs_kernel_profile", f._h, _lib.ptr(ms, _lib.c_double), _lib.ptr(cnt, _lib.c_i64))
+        _lib.call("mfs_kernel_profile_work", f._h, _lib.ptr(work, _lib.c_double))
         log("[bench] kernel-timing pass done")
     t_asm = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
     t_fac = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3
@@ -212,25 +216,34 @@ def main():
         log(f"[bench] forward errors vs FP64 factor: {fwd_err}")
     s_per_step = wall / args.steps
     factor_tflops = flops / t_fac / 1e12
-    # roofline of the dominant kernel (tile GEMM: TRSM + in-panel updates + Schur updates)
-    gemm_ms = ms[5] + ms[6] + ms[7]
-    gemm_launches = int(cnt[5] + cnt[6] + cnt[7])
+    # roofline of the dominant kernel: the tile GEMM (TRSM + in-panel updates + Schur updates);
+    # for the BF16 variants the bfloat16 Schur-update kernel (in-panel + update-matrix classes)
+    bf = args.precision.startswith("bf16")
+    gcls = [6, 7] if bf else [5, 6, 7]
+    gemm_ms = float(sum(ms[c] for c in gcls))
+    gemm_launches = int(sum(cnt[c] for c in gcls))
+    # algorithmic GEMM flops of a step (supernodal count minus the diagonal-tile POTRF share),
+    # apportioned to the classes by their executed tile flops
     potrf_flops = f.flopBreakdown()["potrf"]
-    gemm_flops_step = flops - potrf_flops
+    exec_all = float(work[5] + work[6] + work[7])
+    share = float(sum(work[c] for c in gcls)) / exec_all if exec_all > 0 else 1.0
+    gemm_work = (flops - potrf_flops) * share * args.steps
     if kt and gemm_launches:
         avg_launch_s = gemm_ms / 1e3 / gemm_launches
-        flops_per_launch = gemm_flops_step * args.steps / gemm_launches
+        flops_per_launch = gemm_work / gemm_launches
         achieved = flops_per_launch / avg_launch_s / 1e12
     else:
         avg_launch_s, achieved = None, factor_tflops
-    # HBM bytes per k_gemm launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes on this workload
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes on this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"k_gemm_traffic_64cube_{args.precision}.json")
     if N == 64 and os.path.exists(tpath):
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
-    peak = PEAK_TFLOPS[args.precision]
-    roofline = {"bound": "mfma",
-                "kernel": f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)",
+    peak = PEAK_TFLOPS["bf16" if bf else args.precision]
+    kname = (f"k_gemm_bf ({'hi+lo split ' if args.precision == 'bf16x3' else ''}bfloat16 MFMA Schur updates, "
+             f"FP32 accumulation; algorithmic flops)" if bf else
+             f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)")
+    roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic,
                 "avg_launch_ms": None if avg_launch_s is None else avg_launch_s * 1e3,
@@ -249,7 +262,11 @@ def main():
         "metric": "Newton linear-solve s/step @811k DOF to <=1e-10 rel. residual; factor TFLOP/s",
         "value": s_per_step, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": s_per_step * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if not single else "f32 factor/solve, f64 residual+refinement",
+        "dtype": "f64" if not single else ("f32 factor/solve, " + ("bf16 Schur updates (f32 accumulation), "
+                                                                  if args.precision == "bf16" else
+                                                                  "bf16x3 Schur updates (f32 accumulation), "
+                                                                  if args.precision == "bf16x3" else "")
+                                           + "f64 residual+refinement"),
         "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He on host)",
         "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean "
                                + ("E log-uniform [1e3,1e7] nu U[0.2,0.45]" if args.stress else "E=1e5 nu=0.3")

@@ -45,6 +45,8 @@ extern "C" {
 /* Factor precisions */
 #define MFS_PREC_FP64 0         /* plain FP64 path: the drop-in                                   */
 #define MFS_PREC_FP32 1         /* FP32 factor after symmetric diagonal scaling, FP64 refinement  */
+#define MFS_PREC_BF16 2         /* as FP32, Schur updates on bfloat16 MFMA (FP32 accumulation)    */
+#define MFS_PREC_BF16X3 3       /* as BF16 with each operand split hi + lo (three bf16 products)  */
 
 /* Element-Hessian layouts accepted by mfs_assemble_hessian */
 #define MFS_HE_FULL12   0       /* 144 doubles per element: 12x12 column-major (eval_He(ei))      */
@@ -202,6 +204,8 @@ int mfs_last_timings(mfs_factorizer f, double* ms4);
  * ms[20] and counts[20].                                                                      */
 int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
 int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
+/* Algorithmic flops of the profiled launches per class (the three GEMM classes; 0 elsewhere). */
+int mfs_kernel_profile_work(mfs_factorizer f, double* work);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Workload generator (the reference has no energy code; the caller supplies He).             */
@@ -229,6 +233,11 @@ int mfs_test_gemm_nt(int32_t M, int32_t N, int32_t K, double alpha, const double
 int mfs_test_gemm_nt_f32(int32_t M, int32_t N, int32_t K, float alpha, const float* A_dev,
                          int32_t lda, const float* B_dev, int32_t ldb, float beta, float* C_dev,
                          int32_t ldc, void* stream);
+/* The same through the bfloat16 tile kernel of the BF16 variants (float operands rounded to
+ * bfloat16, FP32 accumulation; split != 0: hi + lo operand split, three products).          */
+int mfs_test_gemm_nt_bf16(int32_t M, int32_t N, int32_t K, float alpha, const float* A_dev,
+                          int32_t lda, const float* B_dev, int32_t ldb, float beta, float* C_dev,
+                          int32_t ldc, int32_t split, void* stream);
 /* trs_kernels::MultiplyLowerBlock semantics (block-compressed row indices, BS=1..3):
  *   B(row_i, :) -= A(i, :) * B(supernode_start : supernode_start + size, :),
  *   row_i = indices[i / BS] + i % BS  for i < degree,

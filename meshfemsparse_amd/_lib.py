@@ -12,7 +12,7 @@ P_i32, P_i64, P_dbl, P_u8 = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctyp
 
 MFS_OK, MFS_ERR_ARG, MFS_ERR_STATE, MFS_ERR_NOT_SPD, MFS_ERR_DEVICE, MFS_ERR_ALLOC, MFS_ERR_UNSUPPORTED = \
     0, -1, -2, -3, -4, -5, -6
-MFS_PREC_FP64, MFS_PREC_FP32 = 0, 1
+MFS_PREC_FP64, MFS_PREC_FP32, MFS_PREC_BF16, MFS_PREC_BF16X3 = 0, 1, 2, 3
 MFS_HE_FULL12, MFS_HE_UPPER78 = 0, 1
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
@@ -56,6 +56,7 @@ _SIGS = {
     "mfs_last_timings": [c_vp, P_dbl],
     "mfs_kernel_profiling": [c_vp, c_i32, c_i32],
     "mfs_kernel_profile": [c_vp, P_dbl, P_i64],
+    "mfs_kernel_profile_work": [c_vp, P_dbl],
     "mfs_cube_mesh": [c_i32, P_i32, P_dbl],
     "mfs_perturb_nodes": [c_i32, ctypes.c_uint64, c_dbl, P_dbl],
     "mfs_neohookean_host": [P_dbl, P_dbl, P_i32, c_i64, P_dbl, P_dbl, c_dbl, c_dbl, P_dbl, P_dbl,
@@ -64,6 +65,8 @@ _SIGS = {
                          c_vp],
     "mfs_test_gemm_nt_f32": [c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i32, c_vp, c_i32, ctypes.c_float,
                              c_vp, c_i32, c_vp],
+    "mfs_test_gemm_nt_bf16": [c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i32, c_vp, c_i32, ctypes.c_float,
+                              c_vp, c_i32, c_i32, c_vp],
     "mfs_test_multiply_lower_block": [c_i32, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp,
                                       c_i64, c_vp],
 }

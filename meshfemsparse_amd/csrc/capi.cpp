@@ -109,7 +109,8 @@ int mfs_assembler_destroy(mfs_assembler a) {
 int mfs_factorizer_create(int precision, mfs_factorizer* out) {
     MFS_TRY
     CHECK_PTR(out);
-    if (precision != MFS_PREC_FP64 && precision != MFS_PREC_FP32)
+    if (precision != MFS_PREC_FP64 && precision != MFS_PREC_FP32 && precision != MFS_PREC_BF16 &&
+        precision != MFS_PREC_BF16X3)
         throw mfs::Error(MFS_ERR_ARG, "unknown precision");
     auto f = std::make_unique<mfs_factorizer_s>();
     f->dev = std::make_unique<mfs::FactorizerDev>(precision);
@@ -365,6 +366,14 @@ int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts) {
     MFS_CATCH
 }
 
+int mfs_kernel_profile_work(mfs_factorizer f, double* work) {
+    MFS_TRY
+    CHECK_PTR(f);
+    CHECK_PTR(work);
+    for (int c = 0; c < mfs::KernelProfile::NCLASS; ++c) work[c] = f->dev->prof.work[c];
+    MFS_CATCH
+}
+
 int mfs_cube_mesh(int32_t N, int32_t* tets, double* X) {
     MFS_TRY
     if (N <= 0 || N > 1000) throw mfs::Error(MFS_ERR_ARG, "N out of range");
@@ -405,6 +414,14 @@ int mfs_test_gemm_nt_f32(int32_t M, int32_t N, int32_t K, float alpha, const flo
                          const float* B, int32_t ldb, float beta, float* C, int32_t ldc, void* stream) {
     MFS_TRY
     mfs::test_gemm_nt(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+    MFS_CATCH
+}
+
+int mfs_test_gemm_nt_bf16(int32_t M, int32_t N, int32_t K, float alpha, const float* A, int32_t lda,
+                          const float* B, int32_t ldb, float beta, float* C, int32_t ldc, int32_t split,
+                          void* stream) {
+    MFS_TRY
+    mfs::test_gemm_nt_bf16(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split != 0, stream);
     MFS_CATCH
 }
 

@@ -41,6 +41,7 @@ struct KernelProfile {
     bool on = false;
     double ms[NCLASS] = {};
     int64_t count[NCLASS] = {};
+    double work[NCLASS] = {};  // algorithmic flops of the profiled launches (GEMM classes)
     void begin(int cls, void* stream, double work = 0.0, int64_t items = 0);
     void end(void* stream);
     void collect();            // after the stream has been synchronized
@@ -105,6 +106,8 @@ private:
 template <typename T>
 void test_gemm_nt(int M, int N, int K, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
                   int ldc, void* stream);
+void test_gemm_nt_bf16(int M, int N, int K, float alpha, const float* A, int lda, const float* B, int ldb,
+                       float beta, float* C, int ldc, bool split, void* stream);
 void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, int64_t size,
                                int64_t degree, const double* A, int64_t lda, int nrhs, double* B,
                                int64_t ldb, void* stream);

@@ -89,6 +89,7 @@ void KernelProfile::collect() {
         HIP_CHECK(hipEventElapsedTime(&m, (hipEvent_t)pool_[i], (hipEvent_t)pool_[i + 1]));
         ms[cls] += m;
         count[cls] += 1;
+        work[cls] += work_[q].first;
         if (fp) fprintf(fp, "%d %lld %.6e %.6f\n", cls, (long long)work_[q].second, work_[q].first, m);
     }
     if (fp) fclose(fp);
@@ -96,7 +97,7 @@ void KernelProfile::collect() {
     work_.clear();
 }
 void KernelProfile::reset() {
-    for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; }
+    for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; work[c] = 0; }
     marks_.clear();
     work_.clear();
 }
@@ -839,6 +840,142 @@ static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, c
     }
 }
 
+// Tile GEMM for the BF16 Schur-update variant of the FP32 factorization: same task format and
+// semantics as k_gemm on FP32 panels, but the operands are rounded to bfloat16 while they are
+// staged into LDS and multiplied by v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  SPLIT
+// keeps the rounding residual as a second bfloat16 term, v = hi + lo, and accumulates
+// hi*hi + hi*lo + lo*hi (three MFMAs), which carries about 16 significant bits of each operand
+// instead of 8.  LDS holds each operand tile k-contiguous per row (32 k plus 8 of padding, so
+// the 16-byte fragment reads of eight consecutive rows fall in distinct banks); lane l reads
+// row l % 16 and the eight k values 8 (l / 16) .. 8 (l / 16) + 7, the same k slots for A and B.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <int TM, int TN, bool SPLIT>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__ tasks,
+                                                    float* __restrict__ L, float* __restrict__ U,
+                                                    const float* __restrict__ inv, int remap) {
+    typedef f4 acc_t;
+    constexpr int KC = 32, LDK = 40, NP = SPLIT ? 2 : 1;
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int AKT = 256 / TM, BKT = 256 / TN;          // threads per operand row
+    constexpr int AKE = KC / AKT, BKE = KC / BKT;          // k values per thread and chunk
+    constexpr int STAGE = NP * (TM + TN) * LDK;            // bf16 per buffer
+    __shared__ __attribute__((aligned(16))) __bf16 smem[2 * STAGE];
+    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x, remap)];
+    const float* A = L + t.a_off;
+    const float* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
+    float* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
+    const int64_t lda = t.lda, ldb = t.ldb;
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    const int ar = tid % TM, br = tid % TN, ak0 = (tid / TM) * AKE, bk0 = (tid / TN) * BKE;
+    const bool arow = ar < t.M, brow = br < t.N;
+    const float* pa = A + (arow ? ar : 0) + (int64_t)ak0 * lda;
+    const float* pb = B + (brow ? br : 0) + (int64_t)bk0 * ldb;
+    float ra[AKE], rb[BKE];
+    auto gload = [&](int k0) {
+        if (k0 + KC <= t.K) {
+#pragma unroll
+            for (int u = 0; u < AKE; ++u) {
+                const float v = pa[u * lda];
+                ra[u] = arow ? v : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < BKE; ++u) {
+                const float v = pb[u * ldb];
+                rb[u] = brow ? v : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < AKE; ++u) ra[u] = (arow && k0 + ak0 + u < t.K) ? pa[u * lda] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < BKE; ++u) rb[u] = (brow && k0 + bk0 + u < t.K) ? pb[u * ldb] : 0.0f;
+        }
+        pa += KC * lda;
+        pb += KC * ldb;
+    };
+    auto put = [&](__bf16* base, int row, int k0_, const float* v, int n) {
+#pragma unroll
+        for (int g = 0; g < n / 8; ++g) {
+            bf8 hi;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) hi[u] = (__bf16)v[8 * g + u];
+            *(bf8*)&base[row * LDK + k0_ + 8 * g] = hi;
+            if constexpr (SPLIT) {
+                bf8 lo;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) lo[u] = (__bf16)(v[8 * g + u] - (float)hi[u]);
+                *(bf8*)&base[(TM + TN) * LDK + row * LDK + k0_ + 8 * g] = lo;
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        __bf16* st = smem + buf * STAGE;
+        put(st, ar, ak0, ra, AKE);
+        put(st + TM * LDK, br, bk0, rb, BKE);
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    for (int k0 = 0; k0 < t.K; k0 += KC) {
+        const bool more = k0 + KC < t.K;
+        if (more) gload(k0 + KC);
+        const __bf16* st = smem + buf * STAGE;
+        bf8 ah[FM], bh[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) ah[i] = *(const bf8*)&st[(wm + 16 * i + fr) * LDK + fk];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bh[j] = *(const bf8*)&st[(TM + wn + 16 * j + fr) * LDK + fk];
+        if constexpr (SPLIT) {
+            const __bf16* sl = st + (TM + TN) * LDK;
+            bf8 al[FM], bl[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) al[i] = *(const bf8*)&sl[(wm + 16 * i + fr) * LDK + fk];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bl[j] = *(const bf8*)&sl[(TM + wn + 16 * j + fr) * LDK + fk];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    const float alpha = (t.flags & GT_ALPHA_POS) ? 1.0f : -1.0f;
+    gemm_store_acc<float, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, (t.flags & GT_BETA0) != 0);
+}
+
+static void launch_gemm_bf(const TaskRange& r, const GemmTask* tasks, float* L, float* U, const float* inv,
+                           bool split, hipStream_t s) {
+    const int rm = gemm_remap();
+    if (r.m > r.b) {
+        if (split) k_gemm_bf<128, 128, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+        else k_gemm_bf<128, 128, false><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+        LAUNCH_CHECK();
+    }
+    if (r.e > r.m) {
+        if (split) k_gemm_bf<64, 64, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+        else k_gemm_bf<64, 64, false><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+        LAUNCH_CHECK();
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Solve kernels (x = factor-order reduced vector of the factor type; one RHS per blockIdx.y)
 // ------------------------------------------------------------------------------------------
@@ -1270,7 +1407,7 @@ __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start
 // FactorizerDev
 // ------------------------------------------------------------------------------------------
 FactorizerDev::FactorizerDev(int prec) : precision(prec) {
-    if (prec != 0 && prec != 1) throw Error(-1, "unknown precision");
+    if (prec < 0 || prec > 3) throw Error(-1, "unknown precision");
 }
 FactorizerDev::~FactorizerDev() = default;
 
@@ -1281,7 +1418,7 @@ void FactorizerDev::require(int t) const {
 
 void FactorizerDev::ensure_vec(DeviceArray& a, size_t bytes) { a.alloc(bytes); }
 
-size_t FactorizerDev::fsize() const { return precision == 1 ? sizeof(float) : sizeof(double); }
+size_t FactorizerDev::fsize() const { return precision != 0 ? sizeof(float) : sizeof(double); }
 
 void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
                              const int64_t* pins, int64_t npins) {
@@ -1336,7 +1473,7 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     U_.alloc(std::max<int64_t>(S->U_size, 1) * fs);
     inv_.alloc(S->inv_slots * TS * TS * fs);
     Xb_.alloc(S->xb_size * fs);
-    if (precision == 1) scale_.alloc(3 * S->n_red * sizeof(double));
+    if (precision != 0) scale_.alloc(3 * S->n_red * sizeof(double));
     status_.alloc(sizeof(int) * 4);
     sym = std::move(S);
     ftype = 1;
@@ -1358,7 +1495,7 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
     k_set_int<<<1, 1, 0, s>>>(status_.as<int>(), INT_MAX);
     LAUNCH_CHECK();
     const double* scale = nullptr;
-    if (precision == 1) {   // symmetric diagonal scaling of the factored matrix
+    if (precision != 0) {   // symmetric diagonal scaling of the factored matrix
         k_diag_scale<<<nblk(S.n_red, 256), 256, 0, s>>>(diagsrc_.as<int32_t>(), S.n_red, Ax, Bx, sigma,
                                                        scale_.as<double>(), status_.as<int>());
         LAUNCH_CHECK();
@@ -1373,6 +1510,17 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
     const GemmTask* gt = gemm_.as<GemmTask>();
+    // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
+    // tile kernel; TRSM and the diagonal tiles stay in the factor type
+    auto schur_gemm = [&](const TaskRange& r) {
+        if constexpr (sizeof(T) == 4) {
+            if (precision >= 2) {
+                launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3, s);
+                return;
+            }
+        }
+        launch_gemm<T>(r, gt, L, U, inv, s);
+    };
     auto extend = [&](const std::vector<std::pair<int64_t,int64_t>>& passes) {
         for (auto [a, b] : passes)
             if (b > a) {
@@ -1401,13 +1549,13 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
             const TaskRange& ur = P.update_steps[t];
             if (ur.e > ur.b) {
                 prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b);
-                launch_gemm<T>(ur, gt, L, U, inv, s);
+                schur_gemm(ur);
                 prof.end(s);
             }
         }
         if (P.syrk.e > P.syrk.b) {
             prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
-            launch_gemm<T>(P.syrk, gt, L, U, inv, s);
+            schur_gemm(P.syrk);
             prof.end(s);
         }
         extend(P.extend_u_passes);
@@ -1446,7 +1594,7 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
 void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, void* stream) {
     require(1);
     ftype = 1;
-    if (precision == 1) numeric_t<float>(Ax, sigma, Bx, stream);
+    if (precision != 0) numeric_t<float>(Ax, sigma, Bx, stream);
     else numeric_t<double>(Ax, sigma, Bx, stream);
 }
 
@@ -1682,7 +1830,7 @@ void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
     const Symbolic& S = *sym;
     std::vector<double> h(S.L_size);
     HIP_CHECK(hipDeviceSynchronize());
-    if (precision == 1) {
+    if (precision != 0) {
         std::vector<float> hf(S.L_size);
         HIP_CHECK(hipMemcpy(hf.data(), L_.p, S.L_size * sizeof(float), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < S.L_size; ++i) h[i] = hf[i];
@@ -1690,7 +1838,7 @@ void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
         HIP_CHECK(hipMemcpy(h.data(), L_.p, S.L_size * sizeof(double), hipMemcpyDeviceToHost));
     }
     panels_to_csc(S, h, Lp, Li, Lx);
-    if (precision == 1) {
+    if (precision != 0) {
         std::vector<double> sc(3 * S.n_red);
         HIP_CHECK(hipMemcpy(sc.data(), scale_.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
         const int64_t n = 3 * S.n_red;
@@ -1711,7 +1859,7 @@ void FactorizerDev::stash() {
     HIP_CHECK(hipMemcpy(invstash_.p, inv_.p, inv_.bytes, hipMemcpyDeviceToDevice));
     xbstash_.alloc(Xb_.bytes);
     HIP_CHECK(hipMemcpy(xbstash_.p, Xb_.p, Xb_.bytes, hipMemcpyDeviceToDevice));
-    if (precision == 1) {
+    if (precision != 0) {
         scalestash_.alloc(scale_.bytes);
         HIP_CHECK(hipMemcpy(scalestash_.p, scale_.p, scale_.bytes, hipMemcpyDeviceToDevice));
     }
@@ -1729,7 +1877,7 @@ void FactorizerDev::swap_stash() {
     std::swap(inv_.bytes, invstash_.bytes);
     std::swap(Xb_.p, xbstash_.p);
     std::swap(Xb_.bytes, xbstash_.bytes);
-    if (precision == 1) {
+    if (precision != 0) {
         std::swap(scale_.p, scalestash_.p);
         std::swap(scale_.bytes, scalestash_.bytes);
     }
@@ -1785,6 +1933,41 @@ void test_gemm_nt(int M, int N, int K, T alpha, const T* A, int lda, const T* B,
     launch_gemm<T>(r, d.as<GemmTask>(), const_cast<T*>(A), C, B, s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
+void test_gemm_nt_bf16(int M, int N, int K, float alpha, const float* A, int lda, const float* B, int ldb,
+                       float beta, float* C, int ldc, bool split, void* stream) {
+    if (!(alpha == 1.0f || alpha == -1.0f) || !(beta == 0.0f || beta == 1.0f))
+        throw Error(-1, "the tile kernel supports alpha in {+1,-1} and beta in {0,1}");
+    if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, N) || ldc < std::max(1, M))
+        throw Error(-1, "bad GEMM dimensions");
+    const int TT = (M >= 128 && N >= 128) ? 128 : 64;
+    std::vector<GemmTask> tasks;
+    for (int c0 = 0; c0 < N; c0 += TT)
+        for (int r0 = 0; r0 < M; r0 += TT) {
+            GemmTask g{};
+            g.a_off = r0;
+            g.lda = lda;
+            g.b_off = c0;
+            g.ldb = ldb;
+            g.c_off = (int64_t)c0 * ldc + r0;
+            g.ldc = ldc;
+            g.M = std::min(TT, M - r0);
+            g.N = std::min(TT, N - c0);
+            g.K = K;
+            g.flags = GT_C_IN_U | GT_B_IN_INV | (beta == 0.0f ? GT_BETA0 : 0) | (alpha > 0.0f ? GT_ALPHA_POS : 0);
+            tasks.push_back(g);
+        }
+    if (tasks.empty()) return;
+    DeviceArray d;
+    d.upload(tasks);
+    hipStream_t s = S_(stream);
+    TaskRange r;
+    r.b = 0;
+    r.m = TT == 128 ? (int64_t)tasks.size() : 0;
+    r.e = (int64_t)tasks.size();
+    launch_gemm_bf(r, d.as<GemmTask>(), const_cast<float*>(A), C, B, split, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
 template void test_gemm_nt<double>(int, int, int, double, const double*, int, const double*, int, double,
                                    double*, int, void*);
 template void test_gemm_nt<float>(int, int, int, float, const float*, int, const float*, int, float,

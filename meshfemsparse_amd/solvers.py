@@ -55,11 +55,22 @@ class GpuSupernodalFactorizer:
     """Supernodal LL^T on one MI355X: host ND ordering + symbolic analysis (once per pattern),
     device value injection, numeric factorization and solves (per Newton step)."""
 
-    def __init__(self, singlePrecision=False):
+    SCHUR_PRECISIONS = {None: _lib.MFS_PREC_FP32, "fp32": _lib.MFS_PREC_FP32, "bf16": _lib.MFS_PREC_BF16,
+                        "bf16x3": _lib.MFS_PREC_BF16X3}
+
+    def __init__(self, singlePrecision=False, schurUpdates=None):
+        """singlePrecision: FP32 factor (after symmetric diagonal scaling) refined in FP64.
+        schurUpdates (single precision only): "bf16" runs the Schur-complement updates on
+        bfloat16 MFMA with FP32 accumulation, "bf16x3" with a hi + lo operand split."""
         self.singlePrecision = bool(singlePrecision)
+        if schurUpdates not in self.SCHUR_PRECISIONS:
+            raise ValueError(f"unknown Schur-update precision {schurUpdates!r}")
+        if schurUpdates not in (None, "fp32") and not singlePrecision:
+            raise ValueError("bfloat16 Schur updates need singlePrecision=True")
+        self.schurUpdates = schurUpdates
         self._h = c_vp()
-        _lib.call("mfs_factorizer_create", _lib.MFS_PREC_FP32 if singlePrecision else _lib.MFS_PREC_FP64,
-                  self._h)
+        prec = self.SCHUR_PRECISIONS[schurUpdates] if singlePrecision else _lib.MFS_PREC_FP64
+        _lib.call("mfs_factorizer_create", prec, self._h)
         self._fixed = []
         self._n_full = 0
         self._n_red = 0
@@ -289,13 +300,14 @@ class GpuSupernodalFactorizer:
         return Lp, Li, Lx
 
 
-def make_cholesky_factorizer(provider=CholeskyProvider.GpuSupernodal, singlePrecision=False):
-    """make_cholesky_factorizer.cc:14-87 with the GPU provider; other providers are not built."""
+def make_cholesky_factorizer(provider=CholeskyProvider.GpuSupernodal, singlePrecision=False, schurUpdates=None):
+    """make_cholesky_factorizer.cc:14-87 with the GPU provider; other providers are not built.
+    schurUpdates selects the BF16 Schur-update variants of the single-precision factor."""
     if provider == CholeskyProvider.CatamariLegacy and singlePrecision:
         raise ValueError("Single precision is unavailable for legacy Catamari")
     if provider != CholeskyProvider.GpuSupernodal:
         raise RuntimeError(f"provider {provider.name} is not available in this build")
-    return GpuSupernodalFactorizer(singlePrecision)
+    return GpuSupernodalFactorizer(singlePrecision, schurUpdates)
 
 
 __all__ = ["CholeskyProvider", "CholeskySys", "FactorizationType", "GpuSupernodalFactorizer",

@@ -58,6 +58,51 @@ def test_gemm_nt_tile_kernel_fp32(dev, M, N, K, alpha, beta):
     assert np.array_equal(got[:, M:], C[:, M:])       # padding rows untouched
 
 
+def _bf16(x):
+    """float32 -> nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (64, 64, 64), (70, 33, 17), (130, 200, 300),
+                                   (256, 256, 1000)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0)])
+@pytest.mark.parametrize("split", [0, 1])
+def test_gemm_nt_tile_kernel_bf16(dev, M, N, K, alpha, beta, split):
+    """The bfloat16 tile kernel of the BF16 Schur-update variants.  Each operand is rounded to
+    bfloat16 (ties to even); with split the rounding residual is rounded again and the product is
+    hi*hi + hi*lo + lo*hi.  bfloat16 products are exact in FP32, so the kernel equals the FP64
+    product of the rounded operands up to FP32 accumulation, (terms + 2) 2^-24 sum |a||b|; the
+    split form is also within 2^-15 sum |a||b| of the exact product of the float inputs."""
+    rng = np.random.default_rng(M * 1000 + N + K + 11 + split)
+    lda, ldb, ldc = M + 3, N + 5, M + 1
+    A = rng.standard_normal((K, lda)).astype(np.float32)
+    B = rng.standard_normal((K, ldb)).astype(np.float32)
+    C = rng.standard_normal((N, ldc)).astype(np.float32)
+    Ad, Bd, Cd = (torch.from_numpy(a.ravel().copy()).to(dev) for a in (A, B, C))
+    _lib.call("mfs_test_gemm_nt_bf16", M, N, K, alpha, _lib.c_vp(Ad.data_ptr()), lda, _lib.c_vp(Bd.data_ptr()),
+              ldb, beta, _lib.c_vp(Cd.data_ptr()), ldc, split, _s())
+    Am, Bm = A[:, :M].T, B[:, :N].T
+    Cm = C[:, :M].T.astype(np.float64)
+    Ah, Bh = _bf16(Am), _bf16(Bm)
+    terms = [(Ah, Bh)]
+    if split:
+        Al, Bl = _bf16(Am - Ah), _bf16(Bm - Bh)
+        terms += [(Ah, Bl), (Al, Bh)]
+    prod = sum(a.astype(np.float64) @ b.astype(np.float64).T for a, b in terms)
+    mag = sum(np.abs(a.astype(np.float64)) @ np.abs(b.astype(np.float64)).T for a, b in terms)
+    exp = alpha * prod + beta * Cm
+    bound = (len(terms) * K + 2) * 2.0 ** -24 * (mag + np.abs(Cm))
+    got = Cd.cpu().numpy().reshape(N, ldc)
+    assert (np.abs(got[:, :M].T - exp) <= bound).all()
+    assert np.array_equal(got[:, M:], C[:, M:])       # padding rows untouched
+    if split:
+        exact = Am.astype(np.float64) @ Bm.astype(np.float64).T
+        absab = np.abs(Am.astype(np.float64)) @ np.abs(Bm.astype(np.float64)).T
+        assert (np.abs(prod - exact) <= 2.0 ** -15 * absab).all()
+
+
 def test_multiply_lower_block_reference_kat_on_device(dev):
     with open(os.path.join(GOLDEN, "multiply_lower_block_kat.json")) as f:
         kat = json.load(f)
