@@ -29,9 +29,9 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 PEAK_HBM_GBS = 8000.0
 # kernel classes of mfs_kernel_profile
-KCLASSES = ["fill_L", "inject", "zero_U", "extend", "potrf", "trsm", "update", "syrk", "fgather",
+KCLASSES = ["fill_L", "inject", "unused", "extend", "potrf", "trsm", "update", "syrk", "fgather",
             "fdiag", "fupdate", "bpartial", "bdiag", "spmv", "maps", "assemble", "blkinv", "bgather",
-            "convert", "refine"]
+            "convert", "refine", "fused"]
 
 
 def log(*a):

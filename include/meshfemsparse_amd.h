@@ -197,14 +197,14 @@ int mfs_last_timings(mfs_factorizer f, double* ms4);
 
 /* Per-kernel-class device time, accumulated from HIP event pairs placed around every launch while
  * enabled (numeric factorization and mfs_solve_refined; the events cost a few microseconds each).
- * Classes (20): fill L, inject, zero U, extend-add, POTRF tiles, TRSM GEMM tiles, in-panel update
- * GEMM tiles, Schur-update GEMM tiles, forward gather, forward diagonal, forward update GEMV,
- * backward partial GEMV^T, backward diagonal, SpMV, permutation maps, assembly, solve-block
- * inverses, backward gather, precision conversions, refinement vector kernels.
- * ms[20] and counts[20].                                                                      */
+ * Classes (21): fill L, inject, (unused), extend-add, POTRF tiles, TRSM GEMM tiles, in-panel
+ * update GEMM tiles, Schur-update GEMM tiles, forward gather, forward diagonal, forward update
+ * GEMV, backward partial GEMV^T, backward diagonal, SpMV, permutation maps, assembly, solve-block
+ * inverses, backward gather, precision conversions, refinement vector kernels, fused level
+ * kernel.  ms[21] and counts[21].                                                             */
 int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
 int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
-/* Algorithmic flops of the profiled launches per class (the three GEMM classes; 0 elsewhere). */
+/* Executed tile flops of the profiled launches per class (21; the three GEMM classes carry them).*/
 int mfs_kernel_profile_work(mfs_factorizer f, double* work);
 
 /* ------------------------------------------------------------------------------------------ */

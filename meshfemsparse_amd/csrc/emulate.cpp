@@ -52,64 +52,113 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                 c = (t.flags & GT_BETA0) ? v : c + v;
             }
     };
-    for (const LevelPlan& P : S.levels) {
-        auto extend = [&](const std::pair<int64_t,int64_t>& pass) {
-            for (int64_t q = pass.first; q < pass.second; ++q) {
-                const ExtendTask& t = S.extend_tasks[q];
-                const int32_t* relc = S.rel.data() + t.rel_off;
-                const int kj = t.col_block, tj = relc[kj];
-                const int64_t Dc = 3 * (int64_t)t.dc;
-                for (int ki = kj; ki < t.dc; ++ki) {
-                    const int ti = relc[ki];
-                    double* dst;
-                    int64_t ldd;
-                    if (tj < t.wp) { dst = L.data() + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti; ldd = t.ldp; }
-                    else {
-                        const int64_t Dp = 3 * (int64_t)t.dp;
-                        dst = U.data() + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
-                        ldd = Dp;
-                    }
-                    const double* src = U.data() + t.u_child + 3 * (int64_t)kj * Dc + 3 * (int64_t)ki;
-                    for (int c = 0; c < 3; ++c)
-                        for (int r = (ki == kj ? c : 0); r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+    auto extend_one = [&](const ExtendTask& t) {
+        const int32_t* relc = S.rel.data() + t.rel_off;
+        const int kj = t.col_block, tj = relc[kj];
+        const int64_t Dc = 3 * (int64_t)t.dc;
+        for (int ki = kj; ki < t.dc; ++ki) {
+            const int ti = relc[ki];
+            double* dst;
+            int64_t ldd;
+            if (tj < t.wp) { dst = L.data() + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti; ldd = t.ldp; }
+            else {
+                const int64_t Dp = 3 * (int64_t)t.dp;
+                dst = U.data() + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
+                ldd = Dp;
+            }
+            const double* src = U.data() + t.u_child + 3 * (int64_t)kj * Dc + 3 * (int64_t)ki;
+            for (int c = 0; c < 3; ++c)
+                for (int r = (ki == kj ? c : 0); r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+        }
+    };
+    auto potrf = [&](const PotrfTask& t) {
+        double* D = L.data() + t.diag_off;
+        const int kb = t.kb;
+        const int64_t ld = t.ld;
+        for (int j = 0; j < kb; ++j) {
+            double d = D[j * ld + j];
+            for (int k = 0; k < j; ++k) d -= D[k * ld + j] * D[k * ld + j];
+            if (!(d > 0.0)) { status = std::min<int64_t>(status, t.col0 + j); d = 1.0; }
+            d = std::sqrt(d);
+            D[j * ld + j] = d;
+            for (int r = j + 1; r < kb; ++r) {
+                double s = D[j * ld + r];
+                for (int k = 0; k < j; ++k) s -= D[k * ld + r] * D[k * ld + j];
+                D[j * ld + r] = s / d;
+            }
+        }
+        double* out = inv.data() + (int64_t)t.slot * TS * TS;
+        std::fill(out, out + TS * TS, 0.0);
+        for (int c = 0; c < kb; ++c) {
+            out[c * TS + c] = 1.0 / D[c * ld + c];
+            for (int i = c + 1; i < kb; ++i) {
+                double s = 0;
+                for (int k = c; k < i; ++k) s += D[k * ld + i] * out[c * TS + k];
+                out[c * TS + i] = -s / D[i * ld + i];
+            }
+        }
+    };
+    // a fused task: the children's extend-add split by destination (panel columns first, U
+    // columns after the Schur update), tile POTRF, TRSM with the tile inverse, U = -L21 L21^T
+    auto fused = [&](const FusedTask& f) {
+        auto children = [&](bool to_u) {
+            for (int32_t q = f.child_begin; q < f.child_end; ++q) {
+                const int32_t c = S.children[q];
+                for (int32_t kj = 0; kj < S.sdeg[c]; ++kj) {
+                    if ((S.rel[S.rel_off[c] + kj] >= f.wblk) != to_u) continue;
+                    ExtendTask t;
+                    t.u_child = S.u_off[c];
+                    t.p_panel = f.panel_off;
+                    t.u_parent = f.u_off;
+                    t.dc = S.sdeg[c];
+                    t.ldp = f.W + f.D;
+                    t.wp = f.wblk;
+                    t.dp = f.D / 3;
+                    t.rel_off = S.rel_off[c];
+                    t.col_block = kj;
+                    t.pad = 0;
+                    extend_one(t);
                 }
             }
         };
-        for (const auto& pass : P.extend_passes) extend(pass);
-        for (size_t st = 0; st < P.potrf_steps.size(); ++st) {
-            for (int64_t q = P.potrf_steps[st].first; q < P.potrf_steps[st].second; ++q) {
-                const PotrfTask& t = S.potrf_tasks[q];
-                double* D = L.data() + t.diag_off;
-                const int kb = t.kb;
-                const int64_t ld = t.ld;
-                for (int j = 0; j < kb; ++j) {
-                    double d = D[j * ld + j];
-                    for (int k = 0; k < j; ++k) d -= D[k * ld + j] * D[k * ld + j];
-                    if (!(d > 0.0)) { status = std::min<int64_t>(status, t.col0 + j); d = 1.0; }
-                    d = std::sqrt(d);
-                    D[j * ld + j] = d;
-                    for (int r = j + 1; r < kb; ++r) {
-                        double s = D[j * ld + r];
-                        for (int k = 0; k < j; ++k) s -= D[k * ld + r] * D[k * ld + j];
-                        D[j * ld + r] = s / d;
-                    }
-                }
-                double* out = inv.data() + (int64_t)t.slot * TS * TS;
-                std::fill(out, out + TS * TS, 0.0);
-                for (int c = 0; c < kb; ++c) {
-                    out[c * TS + c] = 1.0 / D[c * ld + c];
-                    for (int i = c + 1; i < kb; ++i) {
-                        double s = 0;
-                        for (int k = c; k < i; ++k) s += D[k * ld + i] * out[c * TS + k];
-                        out[c * TS + i] = -s / D[i * ld + i];
-                    }
-                }
+        children(false);
+        potrf(S.potrf_tasks[f.slot]);
+        const int64_t ld = f.W + f.D;
+        double* P = L.data() + f.panel_off;
+        const double* X = inv.data() + (int64_t)f.slot * TS * TS;
+        std::vector<double> row(f.W);
+        for (int32_t r = f.W; r < f.W + f.D; ++r) {
+            for (int k = 0; k < f.W; ++k) row[k] = P[(int64_t)k * ld + r];
+            for (int c = 0; c < f.W; ++c) {
+                double v = 0;
+                for (int k = 0; k <= c; ++k) v += row[k] * X[k * TS + c];
+                P[(int64_t)c * ld + r] = v;
             }
+        }
+        if (f.D > 0) {
+            double* Us = U.data() + f.u_off;
+            for (int32_t j = 0; j < f.D; ++j)
+                for (int32_t i = j; i < f.D; ++i) {
+                    double v = 0;
+                    for (int k = 0; k < f.W; ++k) v += P[(int64_t)k * ld + f.W + i] * P[(int64_t)k * ld + f.W + j];
+                    Us[(int64_t)j * f.D + i] = -v;
+                }
+        }
+        children(true);
+    };
+    for (const LevelPlan& P : S.levels) {
+        for (int64_t q = P.fused.first; q < P.fused.second; ++q) fused(S.fused_tasks[q]);
+        for (const auto& pass : P.extend_passes)
+            for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
+        for (size_t st = 0; st < P.potrf_steps.size(); ++st) {
+            for (int64_t q = P.potrf_steps[st].first; q < P.potrf_steps[st].second; ++q)
+                potrf(S.potrf_tasks[q]);
             for (int64_t q = P.trsm_steps[st].b; q < P.trsm_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
             for (int64_t q = P.update_steps[st].b; q < P.update_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
         }
         for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) gemm(S.gemm_tasks[q]);
-        for (const auto& pass : P.extend_u_passes) extend(pass);
+        for (const auto& pass : P.extend_u_passes)
+            for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
     }
     return status == INT64_MAX ? -1 : status;
 }

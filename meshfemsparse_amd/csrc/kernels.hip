@@ -328,13 +328,12 @@ constexpr int PB = 8;    // column step of the tile factorization
 __device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
 __device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 
+// One tile (256 threads, a and x in LDS).  On return the factor is stored to the panel, the
+// inverse to inv[t.slot], and x still holds X in LDS (all threads past the final barrier).
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
-                                               T* __restrict__ L, T* __restrict__ inv,
-                                               int* __restrict__ status) {
-    const PotrfTask t = tasks[blockIdx.x];
-    __shared__ T a[TS][TS + 1];         // a[c][r] = tile(r, c) -> L(r, c)
-    __shared__ T x[TS][TS + 1];         // x[c][r] = S(r, c) -> X(r, c)
+__device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L, T* __restrict__ inv,
+                                           int* __restrict__ status, T (*__restrict__ a)[TS + 1],
+                                           T (*__restrict__ x)[TS + 1]) {
     const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6, rr = tid & 7, cc = (tid >> 3) & 7;
     const int64_t ld = t.ld;
     T* D = L + t.diag_off;
@@ -477,6 +476,15 @@ __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tas
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
+                                               T* __restrict__ L, T* __restrict__ inv,
+                                               int* __restrict__ status) {
+    __shared__ T a[TS][TS + 1];         // a[c][r] = tile(r, c) -> L(r, c)
+    __shared__ T x[TS][TS + 1];         // x[c][r] = S(r, c) -> X(r, c)
+    potrf_tile<T>(tasks[blockIdx.x], L, inv, status, a, x);
+}
+
 // ------------------------------------------------------------------------------------------
 // Batched tile GEMM, C = alpha A B^T + beta C, TM x TN output tile per 256-thread workgroup
 // (2x2 waves), MFMA 16x16x4 (FP64 or FP32), K staged through double-buffered LDS in chunks of 16.
@@ -545,21 +553,18 @@ __device__ __forceinline__ void gemm_store_acc(const typename Mfma<T>::acc_t (&a
     }
 }
 
-// (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
+// One output tile of a task (256 threads; As/Bs: double-buffered operand chunks in LDS).
 template <typename T, int TM, int TN, int KC>
-__global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
-                                                 T* __restrict__ L, T* __restrict__ U,
-                                                 const T* __restrict__ inv, int remap) {
+__device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, T* __restrict__ U,
+                                          const T* __restrict__ inv, T (*__restrict__ As)[KC][TM],
+                                          T (*__restrict__ Bs)[KC][TN]) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int AST = 256 / TM, BST = 256 / TN;           // k stride of the loaders
     constexpr int AL = KC / AST, BL = KC / BST;              // loads per thread per chunk
-    const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x, remap)];
     const T* A = L + t.a_off;
     const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
     T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
-    __shared__ T As[2][KC][TM];
-    __shared__ T Bs[2][KC][TN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
     acc_t acc[FM][FN];
@@ -635,6 +640,132 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
     const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
     const bool beta0 = (t.flags & GT_BETA0) != 0;
     gemm_store_acc<T, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0);
+}
+
+// (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
+template <typename T, int TM, int TN, int KC>
+__global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
+                                                 T* __restrict__ L, T* __restrict__ U,
+                                                 const T* __restrict__ inv, int remap) {
+    __shared__ T As[2][KC][TM];
+    __shared__ T Bs[2][KC][TN];
+    gemm_tile<T, TM, TN, KC>(tasks[xcd_task(blockIdx.x, gridDim.x, remap)], L, U, inv, As, Bs);
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused level kernel: each workgroup factors whole small supernodes (one diagonal tile wide) of
+// one elimination-tree level, striding over the level's task list (persistent: the grid is sized
+// to the machine, not to the task count; tasks are sorted by cost so the stride balances).  Per
+// supernode: the children's update matrices are pulled into the panel (fixed child order), the
+// diagonal tile is factored (potrf_tile), the rows below are solved with the tile inverse, the
+// update matrix is formed with beta = 0, then the children's columns that land in the update
+// matrix are added.  A task writes only its own panel and update matrix, so workgroups never
+// touch the same entry; within a task, barriers order the stages.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void pull_children(const FusedTask& f, bool to_u, const int32_t* __restrict__ children,
+                                              const int32_t* __restrict__ sdeg, const int64_t* __restrict__ rel_off,
+                                              const int32_t* __restrict__ rel, const int64_t* __restrict__ uoff,
+                                              T* __restrict__ U, T* __restrict__ L) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t ldp = f.W + f.D;
+    for (int q = f.child_begin; q < f.child_end; ++q) {
+        const int c = children[q], dc = sdeg[c];
+        const int32_t* relc = rel + rel_off[c];
+        const int64_t Dc = 3 * (int64_t)dc;
+        const T* uc = U + uoff[c];
+        for (int kj = wave; kj < dc; kj += 4) {      // one column block per wave: distinct columns
+            const int tj = relc[kj];
+            if ((tj >= f.wblk) != to_u) continue;
+            for (int ki = kj + lane; ki < dc; ki += 64) {
+                const int ti = relc[ki];
+                T* dst;
+                int64_t ldd;
+                if (tj < f.wblk) {
+                    dst = L + f.panel_off + (int64_t)(3 * tj) * ldp + 3 * ti;
+                    ldd = ldp;
+                } else {
+                    dst = U + f.u_off + (int64_t)(3 * (tj - f.wblk)) * f.D + 3 * (ti - f.wblk);
+                    ldd = f.D;
+                }
+                const T* src = uc + (3 * (int64_t)kj) * Dc + 3 * (int64_t)ki;
+                if (ki == kj) {
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                        for (int r = cc; r < 3; ++r) dst[cc * ldd + r] += src[cc * Dc + r];
+                } else {
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) dst[cc * ldd + r] += src[cc * Dc + r];
+                }
+            }
+        }
+        __syncthreads();    // the next child may add to the same entries
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_fused(const FusedTask* __restrict__ tasks, int ntasks,
+                                                  const int32_t* __restrict__ children,
+                                                  const int32_t* __restrict__ sdeg,
+                                                  const int64_t* __restrict__ rel_off,
+                                                  const int32_t* __restrict__ rel,
+                                                  const int64_t* __restrict__ uoff, T* __restrict__ L,
+                                                  T* __restrict__ U, T* __restrict__ inv,
+                                                  int* __restrict__ status) {
+    constexpr int KC = 16;
+    __shared__ __attribute__((aligned(16))) T smem[2 * TS * (TS + 1)];
+    T (*a)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem);
+    T (*x)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem + TS * (TS + 1));
+    T (*As)[KC][TS] = reinterpret_cast<T (*)[KC][TS]>(smem);
+    T (*Bs)[KC][TS] = reinterpret_cast<T (*)[KC][TS]>(smem + 2 * KC * TS);
+    for (int item = blockIdx.x; item < ntasks; item += gridDim.x) {
+        const FusedTask f = tasks[item];
+        const int32_t ld = f.W + f.D;
+        pull_children<T>(f, false, children, sdeg, rel_off, rel, uoff, U, L);
+        PotrfTask pt;
+        pt.diag_off = f.panel_off;
+        pt.ld = ld;
+        pt.kb = f.W;
+        pt.slot = f.slot;
+        pt.col0 = f.col0;
+        potrf_tile<T>(pt, L, inv, status, a, x);
+        __syncthreads();    // tile factor and inverse stored; LDS free for the tile GEMMs
+        for (int r0 = f.W; r0 < ld; r0 += TS) {      // TRSM: L21 := A21 X^T, in place
+            GemmTask g{};
+            g.a_off = f.panel_off + r0;
+            g.lda = ld;
+            g.b_off = (int64_t)f.slot * TS * TS;
+            g.ldb = TS;
+            g.c_off = g.a_off;
+            g.ldc = ld;
+            g.M = min(TS, ld - r0);
+            g.N = f.W;
+            g.K = f.W;
+            g.flags = GT_BETA0 | GT_ALPHA_POS | GT_B_IN_INV;
+            gemm_tile<T, TS, TS, KC>(g, L, U, inv, As, Bs);
+        }
+        __syncthreads();    // L21 complete
+        for (int c0 = 0; c0 < f.D; c0 += TS)          // U = -L21 L21^T, lower tiles
+            for (int r0 = c0; r0 < f.D; r0 += TS) {
+                GemmTask g{};
+                g.a_off = f.panel_off + f.W + r0;
+                g.lda = ld;
+                g.b_off = f.panel_off + f.W + c0;
+                g.ldb = ld;
+                g.c_off = f.u_off + (int64_t)c0 * f.D + r0;
+                g.ldc = f.D;
+                g.M = min(TS, f.D - r0);
+                g.N = min(TS, f.D - c0);
+                g.K = f.W;
+                g.flags = GT_C_IN_U | GT_BETA0;
+                gemm_tile<T, TS, TS, KC>(g, L, U, inv, As, Bs);
+            }
+        __syncthreads();    // U written before the children's U columns are added
+        pull_children<T>(f, true, children, sdeg, rel_off, rel, uoff, U, L);
+    }
 }
 
 // executed flops of a task range (2 M N K per tile task)
@@ -1443,6 +1574,8 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     rel_.upload(S->rel);
     ext_.upload(S->extend_tasks);
     potrf_.upload(S->potrf_tasks);
+    fused_.upload(S->fused_tasks);
+    uoff_.upload(S->u_off);
     gemm_.upload(S->gemm_tasks);
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
@@ -1530,7 +1663,21 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
                 prof.end(s);
             }
     };
+    static const int fused_grid = [] {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        return 2 * std::max(ncu, 1);
+    }();
     for (const LevelPlan& P : S.levels) {
+        const int64_t nf = P.fused.second - P.fused.first;
+        if (nf > 0) {
+            prof.begin(KernelProfile::FUSED, s, 0.0, nf);
+            k_fused<T><<<(unsigned)std::min<int64_t>(nf, fused_grid), 256, 0, s>>>(
+                fused_.as<FusedTask>() + P.fused.first, (int)nf, children_.as<int32_t>(), sdeg_.as<int32_t>(),
+                reloff_.as<int64_t>(), rel_.as<int32_t>(), uoff_.as<int64_t>(), L, U, inv, status_.as<int>());
+            LAUNCH_CHECK();
+            prof.end(s);
+        }
         extend(P.extend_passes);
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];

@@ -115,6 +115,22 @@ struct SolveRowTask {       // a row range [r0, r0 + nr) of a supernode panel ag
 };
 static_assert(sizeof(SolveRowTask) == 40, "SolveRowTask layout");
 
+// A supernode factored entirely by one workgroup of the fused level kernel (width <= one
+// diagonal tile): pull extend-add of its children's update matrices into the panel, diagonal
+// tile POTRF + inverse, TRSM of the rows below, U = -L21 L21^T, then the children's columns that
+// land in U.
+struct FusedTask {
+    int64_t panel_off;      // L offset of the panel ((W + D) x W, ld = W + D)
+    int64_t u_off;          // U offset (D x D) or -1 when D == 0
+    int32_t W, D;           // scalars
+    int32_t child_begin, child_end;   // range in the child list
+    int32_t slot;           // inverse slot of the diagonal tile
+    int32_t col0;           // scalar factor column of the first column
+    int32_t wblk;           // width in blocks
+    int32_t pad;
+};
+static_assert(sizeof(FusedTask) == 48, "FusedTask layout");
+
 struct TaskRange {          // [b, m): 128x128-tile GEMM tasks, [m, e): 64x64-tile tasks
     int64_t b = 0, m = 0, e = 0;
 };
@@ -132,6 +148,7 @@ struct SolveGatherTask {    // forward: gather children update vectors into x / 
 
 struct LevelPlan {
     // factorization
+    std::pair<int64_t,int64_t> fused{0, 0};                  // range in fused tasks (one launch)
     std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks (into panels)
     std::vector<std::pair<int64_t,int64_t>> extend_u_passes; // after the Schur update (into U)
     std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
@@ -155,6 +172,7 @@ struct Symbolic {
     std::vector<int64_t> red_of_full;           // block: full -> reduced or -1
     // ordering: pos[reduced block] = factor position; iperm[pos] = reduced block
     std::vector<int32_t> pos, iperm;
+    std::vector<int32_t> nd_perm;              // the fill-reducing ordering before the postorder
     // supernodes (factor order)
     int32_t ns = 0;
     std::vector<int32_t> sfirst, swidth, sdeg, sparent, slevel;    // blocks
@@ -188,6 +206,7 @@ struct Symbolic {
     std::vector<ExtendTask> extend_tasks;
     std::vector<PotrfTask> potrf_tasks;
     std::vector<GemmTask> gemm_tasks;
+    std::vector<FusedTask> fused_tasks;
     // solves
     int32_t solve_block = 512;                 // column-block width (scalars, multiple of 64)
     std::vector<int32_t> tile_slot;            // per (supernode, 64-column tile): inverse slot
@@ -219,16 +238,22 @@ struct SymbolicOptions {
     int tile_group = 8;     // GEMM tiles are emitted in groups of this many tile columns
     int solve_block = 512;  // solve column-block width (multiple of 64)
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
+    int fuse_degree = 0;    // supernodes of width <= one tile and at most this many structure rows
+                            // (scalars) go to the fused level kernel; 0 disables it
+    const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
 };
 
 // Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
-// (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group); values are rounded
-// down to multiples of 64 where required.
+// (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
+// (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
+// 128x128 tiles); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
                  const int64_t* pins, int64_t npins, const SymbolicOptions& opt);
+// the nested-dissection ordering analyze() computes (perm[new] = old over the reduced blocks)
+std::vector<int32_t> nd_order(const Symbolic& S);
 
 // Host replay of the device schedule (returns the first failing pivot column or -1) and the
 // supernodal-panel -> CSC conversion.

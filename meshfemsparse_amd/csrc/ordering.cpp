@@ -10,8 +10,10 @@
 #include "mfs_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <queue>
+#include <thread>
 
 namespace mfs {
 
@@ -366,12 +368,55 @@ G induced(const G& g, const std::vector<int8_t>& where, int8_t part, std::vector
     return s;
 }
 
+// Connected components (label per vertex, components numbered by their smallest vertex).
+int32_t components(const G& g, std::vector<int32_t>& comp) {
+    comp.assign(g.n, -1);
+    int32_t nc = 0;
+    std::vector<int32_t> stack;
+    for (int32_t v0 = 0; v0 < g.n; ++v0) {
+        if (comp[v0] != -1) continue;
+        comp[v0] = nc;
+        stack.push_back(v0);
+        while (!stack.empty()) {
+            const int32_t v = stack.back();
+            stack.pop_back();
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e)
+                if (comp[g.adj[e]] == -1) { comp[g.adj[e]] = nc; stack.push_back(g.adj[e]); }
+        }
+        ++nc;
+    }
+    return nc;
+}
+
+void nd_recurse(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
+                const NDOptions& opt);
+
+// Components are ordered one after another (each by its own dissection): no separator is
+// needed between parts that share no edge.
+bool nd_components(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
+                   const NDOptions& opt) {
+    std::vector<int32_t> comp;
+    const int32_t nc = components(g, comp);
+    if (nc <= 1) return false;
+    std::vector<int32_t> local(g.n, -1), members;
+    std::vector<int8_t> where(g.n);
+    for (int32_t c = 0; c < nc; ++c) {
+        for (int32_t v = 0; v < g.n; ++v) where[v] = comp[v] == c ? 0 : 1;
+        G s = induced(g, where, 0, local, members);
+        std::vector<int32_t> sg(members.size());
+        for (size_t i = 0; i < members.size(); ++i) sg[i] = gid[members[i]];
+        nd_recurse(s, sg, out, rng, opt);
+    }
+    return true;
+}
+
 void nd_recurse(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
                 const NDOptions& opt) {
     if (g.n <= opt.leaf_size) {
         out.insert(out.end(), gid.begin(), gid.end());
         return;
     }
+    if (nd_components(g, gid, out, rng, opt)) return;
     std::vector<int8_t> where = bisect(g, rng, opt);
     int32_t cnt[3] = {0, 0, 0};
     for (int8_t w : where) cnt[w]++;
@@ -400,12 +445,46 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xa
     g.adj = adj;
     g.ew.assign(adj.size(), 1);
     g.vw.assign(n, 1);
-    std::vector<int32_t> gid(n);
-    std::iota(gid.begin(), gid.end(), 0);
     std::vector<int32_t> out;
     out.reserve(n);
-    Rng rng{opt.seed};
-    nd_recurse(g, gid, out, rng, opt);
+    // top level: the connected components (independent meshes of a batch) are dissected in
+    // parallel, each with its own random stream, and concatenated in component order
+    std::vector<int32_t> comp;
+    const int32_t nc = components(g, comp);
+    std::vector<std::vector<int32_t>> gids(nc);
+    std::vector<int32_t> local(n);
+    for (int32_t v = 0; v < n; ++v) {
+        local[v] = (int32_t)gids[comp[v]].size();
+        gids[comp[v]].push_back(v);
+    }
+    std::vector<std::vector<int32_t>> outs(nc);
+    auto work = [&](int32_t c) {
+        const std::vector<int32_t>& mem = gids[c];
+        G s;
+        s.n = (int32_t)mem.size();
+        s.xadj.assign(s.n + 1, 0);
+        s.vw.assign(s.n, 1);
+        for (int32_t i = 0; i < s.n; ++i) {
+            const int32_t v = mem[i];
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) { s.adj.push_back(local[g.adj[e]]); s.ew.push_back(1); }
+            s.xadj[i + 1] = (int64_t)s.adj.size();
+        }
+        Rng rng{nc == 1 ? opt.seed : opt.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(c + 1))};
+        nd_recurse(s, mem, outs[c], rng, opt);
+    };
+    const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+    if (nc > 1 && nt > 1) {
+        std::atomic<int32_t> next{0};
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&] {
+                for (int32_t c; (c = next.fetch_add(1)) < nc;) work(c);
+            });
+        for (auto& t : th) t.join();
+    } else {
+        for (int32_t c = 0; c < nc; ++c) work(c);
+    }
+    for (int32_t c = 0; c < nc; ++c) out.insert(out.end(), outs[c].begin(), outs[c].end());
     return out;
 }
 

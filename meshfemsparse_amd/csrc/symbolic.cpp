@@ -134,6 +134,8 @@ struct Arena {
 
 } // namespace
 
+std::vector<int32_t> nd_order(const Symbolic& S) { return S.nd_perm; }
+
 SymbolicOptions options_from_env() {
     SymbolicOptions opt;
     if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
@@ -141,6 +143,8 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
     if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
+    if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
+    if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
     return opt;
 }
 
@@ -209,7 +213,14 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     }
 
     // ---- ordering: ND, then etree postorder ----
-    std::vector<int32_t> perm = nested_dissection(n, xadj, adj, opt.nd);   // perm[new] = old
+    std::vector<int32_t> perm;   // perm[new] = old
+    if (opt.given_perm) {
+        if ((int32_t)opt.given_perm->size() != n) throw Error(-1, "given ordering has the wrong size");
+        perm = *opt.given_perm;
+    } else {
+        perm = nested_dissection(n, xadj, adj, opt.nd);
+    }
+    S.nd_perm = perm;
     auto build_patterns = [&](const std::vector<int32_t>& pos_, std::vector<int64_t>& Lp,
                               std::vector<int32_t>& Li, std::vector<int64_t>& Up,
                               std::vector<int32_t>& Ui) {
@@ -518,13 +529,53 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         r.e = (int64_t)all.size();
         return r;
     };
+    auto fused_sn = [&](int32_t s) {
+        return opt.fuse_degree > 0 && 3 * S.swidth[s] <= T && 3 * S.sdeg[s] <= opt.fuse_degree;
+    };
     for (int32_t lv = 0; lv < S.num_levels; ++lv) {
         LevelPlan& P = S.levels[lv];
-        const auto& sl = bylevel[lv];
+        const auto& sl_all = bylevel[lv];
+        std::vector<int32_t> sl;   // supernodes of the tiled multi-launch path
         // allocate U for this level
-        for (int32_t s : sl) {
+        for (int32_t s : sl_all) {
             const int64_t D = 3 * (int64_t)S.sdeg[s];
             if (D > 0) S.u_off[s] = arena.alloc(D * D);
+        }
+        // small supernodes: one fused task each (one launch for the level)
+        {
+            const int64_t f0 = (int64_t)S.fused_tasks.size();
+            for (int32_t s : sl_all) {
+                if (!fused_sn(s)) { sl.push_back(s); continue; }
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s];
+                PotrfTask pt;
+                pt.diag_off = S.panel_off[s];
+                pt.ld = W + D;
+                pt.kb = W;
+                pt.slot = (int32_t)S.potrf_tasks.size();
+                pt.col0 = 3 * S.sfirst[s];
+                S.potrf_tasks.push_back(pt);
+                S.tile_slot[tile_base[s]] = pt.slot;
+                S.flops_potrf += (double)W * W * W / 3.0;
+                FusedTask ft;
+                ft.panel_off = S.panel_off[s];
+                ft.u_off = S.u_off[s];
+                ft.W = W;
+                ft.D = D;
+                ft.child_begin = S.child_ptr[s];
+                ft.child_end = S.child_ptr[s + 1];
+                ft.slot = pt.slot;
+                ft.col0 = pt.col0;
+                ft.wblk = S.swidth[s];
+                ft.pad = 0;
+                S.fused_tasks.push_back(ft);
+            }
+            // most expensive first, so the kernel's static stride over the tasks balances
+            std::stable_sort(S.fused_tasks.begin() + f0, S.fused_tasks.end(), [](const FusedTask& x, const FusedTask& y) {
+                const double cx = (double)x.D * x.D * x.W + (double)x.W * x.W * (x.W + x.D);
+                const double cy = (double)y.D * y.D * y.W + (double)y.W * y.W * (y.W + y.D);
+                return cx > cy;
+            });
+            P.fused = {f0, (int64_t)S.fused_tasks.size()};
         }
         // extend-add passes by child rank (no two tasks of a pass touch the same parent entry).
         // Column blocks of a child's U that land in the parent's panel are added before the
@@ -561,7 +612,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         extend_passes(false, P.extend_passes);
         // release children's U (consumed by this level's extend-add; the U passes run at the end
         // of the level and nothing else is allocated in between)
-        for (int32_t p : sl)
+        for (int32_t p : sl_all)
             for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
                 const int32_t c = S.children[q];
                 if (S.u_off[c] >= 0) {
@@ -657,7 +708,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         // solves.  Gather tasks (one per supernode of the level) serve both directions.
         {
             const int64_t g0 = (int64_t)S.gather_tasks.size();
-            for (int32_t s : sl) {
+            for (int32_t s : sl_all) {
                 SolveGatherTask g{};
                 g.sn = s;
                 g.child_begin = S.child_ptr[s];
@@ -672,7 +723,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             P.gather = {g0, (int64_t)S.gather_tasks.size()};
         }
         int32_t maxblk = 0;
-        for (int32_t s : sl) maxblk = std::max(maxblk, sblock0[s + 1] - sblock0[s]);
+        for (int32_t s : sl_all) maxblk = std::max(maxblk, sblock0[s + 1] - sblock0[s]);
         // row ranges [lo, hi) split into chunks of at most `ch` rows that do not straddle W
         auto chunks = [](int32_t lo, int32_t W, int32_t hi, int32_t ch) {
             std::vector<std::pair<int32_t,int32_t>> out;
@@ -684,7 +735,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         // (one workgroup per 64-row tile), then every row below the block is updated
         for (int32_t k = 0; k < maxblk; ++k) {
             const int64_t d0 = (int64_t)S.fdiag_tiles.size(), u0 = (int64_t)S.fupd_tasks.size();
-            for (int32_t s : sl) {
+            for (int32_t s : sl_all) {
                 if (k >= sblock0[s + 1] - sblock0[s]) continue;
                 const int32_t bi = sblock0[s] + k;
                 const SolveBlock& b = S.sblocks[bi];
@@ -713,7 +764,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             for (int32_t st = 0; st < maxblk; ++st) {
                 const int64_t p0 = (int64_t)S.bpart_tasks.size(), d0 = (int64_t)S.bdiag_tiles.size();
                 int32_t slot = 0;
-                for (int32_t s : sl) {
+                for (int32_t s : sl_all) {
                     const int32_t nbs = sblock0[s + 1] - sblock0[s];
                     if (st >= nbs) continue;
                     const int32_t k = nbs - 1 - st, bi = sblock0[s] + k;

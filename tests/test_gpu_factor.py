@@ -71,10 +71,14 @@ def oracle_factor(H, f, sigma=0.0, Bx=None):
     return orc.chol(Cp, Ci, Cx), red, pos
 
 
-@pytest.mark.parametrize("N,outer", [(2, None), (5, None), (8, None), (8, 128), (9, 64)])
-def test_factor_matches_unique_cholesky(dev, N, outer, monkeypatch):
+@pytest.mark.parametrize("N,outer,fuse", [(2, None, 0), (5, None, 0), (8, None, 0), (8, 128, 0), (9, 64, 0),
+                                          (5, None, 100000), (8, None, 240), (9, 64, 100000)])
+def test_factor_matches_unique_cholesky(dev, N, outer, fuse, monkeypatch):
+    """fuse > 0 sends every supernode one diagonal tile wide with at most `fuse` structure rows
+    through the fused level kernel (one workgroup per supernode)."""
     if outer:
         monkeypatch.setenv("MFS_OUTER", str(outer))      # several outer blocks per supernode
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
     H, pins, _, _, _ = mesh_problem(dev, N)
     f = mfs.make_cholesky_factorizer()
     f.factorizeSymbolic(H, pins)
@@ -177,7 +181,9 @@ def test_check_solves_chain(dev, pin_mode, single):
 
 
 @pytest.mark.parametrize("single", [False, True])
-def test_check_solves_mesh(dev, single):
+@pytest.mark.parametrize("fuse", [0, 100000])
+def test_check_solves_mesh(dev, single, fuse, monkeypatch):
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
     H, pins, _, _, _ = mesh_problem(dev, 4, seed=3)
     check_solves(dev, H, pins, single)
 

@@ -148,14 +148,18 @@ def emulate(H, pins, Ax, sigma=0.0, x_perm=None):
     return iperm, (Lp, Li, Lx), fc[0]
 
 
-@pytest.mark.parametrize("N,sigma,outer", [(3, 0.0, None), (5, 0.0, None), (7, 2.5, None), (7, 0.0, 64),
-                                           (8, 0.0, 128)])
-def test_device_schedule_replay_matches_oracle(N, sigma, outer, monkeypatch):
+@pytest.mark.parametrize("N,sigma,outer,fuse", [(3, 0.0, None, 0), (5, 0.0, None, 0), (7, 2.5, None, 0),
+                                                (7, 0.0, 64, 0), (8, 0.0, 128, 0), (5, 0.0, None, 96),
+                                                (7, 0.0, None, 100000), (8, 1.5, 128, 300)])
+def test_device_schedule_replay_matches_oracle(N, sigma, outer, fuse, monkeypatch):
     """The device task lists (injection plan, extend-add, tile POTRF/TRSM, left-looking in-block
-    updates, outer trailing updates, Schur updates), replayed on the host, reproduce the oracle's
-    unique Cholesky factor.  Small outer blocks put several outer blocks in one supernode."""
+    updates, outer trailing updates, Schur updates, fused small-supernode tasks), replayed on the
+    host, reproduce the oracle's unique Cholesky factor.  Small outer blocks put several outer
+    blocks in one supernode; a fuse bound sends the narrow supernodes with at most that many
+    structure rows through the fused level kernel's task list."""
     if outer:
         monkeypatch.setenv("MFS_OUTER", str(outer))
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
     tets, Xr = wl.cube_mesh(N)
     X = wl.perturb(N, Xr, 0)
     He, _, _ = wl.neohookean(Xr, X, tets, gradient=False)
@@ -199,13 +203,15 @@ def chain_pattern(nb=20):
     return H, Ax
 
 
-@pytest.mark.parametrize("case,block,chunk", [("chain", None, None), ("mesh5", None, None),
-                                              ("mesh7", None, None), ("mesh7", 64, 64),
-                                              ("mesh7", 128, 64), ("mesh5", 192, 128)])
-def test_device_solve_schedule_replay(case, block, chunk, monkeypatch):
+@pytest.mark.parametrize("case,block,chunk,fuse", [("chain", None, None, 0), ("mesh5", None, None, 0),
+                                                   ("mesh7", None, None, 0), ("mesh7", 64, 64, 0),
+                                                   ("mesh7", 128, 64, 0), ("mesh5", 192, 128, 0),
+                                                   ("chain", None, None, 100000), ("mesh7", None, None, 200)])
+def test_device_solve_schedule_replay(case, block, chunk, fuse, monkeypatch):
     """Forward/backward solve task lists (column blocks with explicit diagonal-block inverses,
     row tasks, backward partial sums), replayed on the host, solve P A P^T y = P b to 1e-12.
     Small solve blocks / chunks force several blocks per supernode and several partial slots."""
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
     if block:
         monkeypatch.setenv("MFS_SOLVE_BLOCK", str(block))
         monkeypatch.setenv("MFS_SOLVE_CHUNK", str(chunk))
