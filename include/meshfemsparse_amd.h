@@ -147,6 +147,36 @@ int mfs_spmv(mfs_factorizer f, const double* Ax_dev, const double* x_dev, double
 int mfs_solve_refined(mfs_factorizer f, const double* Ax_dev, const double* b_dev, double* x_dev,
                       double tol, int32_t max_iters, int32_t* iters, double* relres, void* stream);
 
+/* Batched systems (many independent meshes assembled as one block-diagonal matrix; each mesh is
+ * a connected component and gets its own nested dissection): refinement until every segment
+ * [seg_ptr[s], seg_ptr[s+1]) of the full-length vectors has ||b_s - (A x)_s|| / ||b_s|| <= tol.
+ * seg_ptr: host, nseg + 1 entries, 0 .. 3 num_blocks.  relres_seg (host, nseg) may be NULL.
+ * No reference counterpart (the reference factors one system at a time).                   */
+int mfs_solve_refined_segments(mfs_factorizer f, const double* Ax_dev, const double* b_dev, double* x_dev,
+                               int64_t nseg, const int64_t* seg_ptr, double tol, int32_t max_iters,
+                               int32_t* iters, double* relres_max, double* relres_seg, void* stream);
+
+/* Capture the numeric factorization (all kernels from the zero fill to the block inverses) in a
+ * hipGraph on the library's own stream, joined to `stream` by events, and replay it on later
+ * calls with the same Ax_dev, Bx_dev and sigma (re-captured when they or the schedule change).
+ * No reference counterpart.                                                                  */
+int mfs_factorizer_set_graph(mfs_factorizer f, int32_t on);
+
+/* Autotuner (no reference counterpart): analyse once for the ordering, then for every candidate
+ * pair (fuse[i], tiles[j]) rebuild the schedule -- supernodes one diagonal tile wide with at most
+ * fuse[i] structure rows (scalars) go to the fused level kernel (0: none), GEMM regions at least
+ * tiles[j] in both dimensions use 128x128 tiles -- and time `reps` numeric factorizations of
+ * Ax_dev (after one warm-up).  ms[i * ntile + j] = average milliseconds; best = {fuse, tile}.
+ * The fastest schedule stays installed (symbolic state: factorize again before solving).     */
+int mfs_autotune(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t num_blocks, int32_t block_size,
+                 const int64_t* pinned_scalar_vars, int64_t num_pins, const double* Ax_dev, const int32_t* fuse,
+                 int32_t nfuse, const int32_t* tiles, int32_t ntile, int32_t reps, double* ms, int32_t* best,
+                 void* stream);
+
+/* Schedule statistics: out8 = {fused tasks, fused-kernel launches (levels), tiled supernodes,
+ * POTRF launches, GEMM launches, extend-add launches, levels, nodes of the captured graph}.   */
+int mfs_schedule_stats(mfs_factorizer f, int64_t* out8);
+
 /* State / statistics (CholeskyFactorizerBase.hh:55-59,193-260; CatamariFactorizer.cc:716-948). */
 int mfs_factorization_type(mfs_factorizer f, int32_t* type);   /* 0 none, 1 symbolic, 2 numeric */
 int mfs_num_successful_pivots(mfs_factorizer f, int64_t* n);

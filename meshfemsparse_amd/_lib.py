@@ -36,6 +36,12 @@ _SIGS = {
     "mfs_extract_full_solution": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],
     "mfs_spmv": [c_vp, c_vp, c_vp, c_vp, c_vp],
     "mfs_solve_refined": [c_vp, c_vp, c_vp, c_vp, c_dbl, c_i32, P_i32, P_dbl, c_vp],
+    "mfs_solve_refined_segments": [c_vp, c_vp, c_vp, c_vp, c_i64, P_i64, c_dbl, c_i32, P_i32, P_dbl, P_dbl,
+                                   c_vp],
+    "mfs_factorizer_set_graph": [c_vp, c_i32],
+    "mfs_autotune": [c_vp, P_i64, P_i64, c_i64, c_i32, P_i64, c_i64, c_vp, P_i32, c_i32, P_i32, c_i32, c_i32,
+                     P_dbl, P_i32, c_vp],
+    "mfs_schedule_stats": [c_vp, P_i64],
     "mfs_factorization_type": [c_vp, P_i32],
     "mfs_num_successful_pivots": [c_vp, P_i64],
     "mfs_sizes": [c_vp, P_i64, P_i64],

@@ -189,6 +189,62 @@ int mfs_solve_refined(mfs_factorizer f, const double* Ax, const double* b, doubl
     MFS_CATCH
 }
 
+int mfs_solve_refined_segments(mfs_factorizer f, const double* Ax, const double* b, double* x, int64_t nseg,
+                               const int64_t* seg_ptr, double tol, int32_t max_iters, int32_t* iters,
+                               double* relres_max, double* relres_seg, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ax); CHECK_PTR(b); CHECK_PTR(x); CHECK_PTR(seg_ptr);
+    int it = 0;
+    f->dev->solve_refined_segments(Ax, b, x, nseg, seg_ptr, tol, max_iters, &it, relres_max, relres_seg, stream);
+    if (iters) *iters = it;
+    MFS_CATCH
+}
+
+int mfs_factorizer_set_graph(mfs_factorizer f, int32_t on) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->set_graph(on != 0);
+    MFS_CATCH
+}
+
+int mfs_autotune(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t nb, int32_t bs, const int64_t* pins,
+                 int64_t npins, const double* Ax, const int32_t* fuse, int32_t nfuse, const int32_t* tiles,
+                 int32_t ntile, int32_t reps, double* ms, int32_t* best, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(Ax); CHECK_PTR(fuse); CHECK_PTR(tiles);
+    if (npins > 0) CHECK_PTR(pins);
+    std::vector<int> fz(fuse, fuse + std::max(nfuse, 0)), tz(tiles, tiles + std::max(ntile, 0));
+    int b2[2] = {0, 0};
+    f->dev->autotune(Ap, Ai, nb, bs, pins, npins, Ax, fz.data(), nfuse, tz.data(), ntile, reps, ms, b2, stream);
+    if (best) { best[0] = b2[0]; best[1] = b2[1]; }
+    MFS_CATCH
+}
+
+int mfs_schedule_stats(mfs_factorizer f, int64_t* out) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(out);
+    f->dev->require(1);
+    const auto& S = *f->dev->sym;
+    int64_t fused_launches = 0, potrf_launches = 0, gemm_launches = 0, extend_launches = 0;
+    for (const auto& P : S.levels) {
+        fused_launches += P.fused.second > P.fused.first;
+        for (auto [a, b] : P.potrf_steps) potrf_launches += b > a;
+        for (const auto& r : P.trsm_steps) gemm_launches += (r.m > r.b) + (r.e > r.m);
+        for (const auto& r : P.update_steps) gemm_launches += (r.m > r.b) + (r.e > r.m);
+        gemm_launches += (P.syrk.m > P.syrk.b) + (P.syrk.e > P.syrk.m);
+        extend_launches += (int64_t)P.extend_passes.size() + (int64_t)P.extend_u_passes.size();
+    }
+    out[0] = (int64_t)S.fused_tasks.size();
+    out[1] = fused_launches;
+    out[2] = S.ns - (int64_t)S.fused_tasks.size();
+    out[3] = potrf_launches;
+    out[4] = gemm_launches;
+    out[5] = extend_launches;
+    out[6] = S.num_levels;
+    out[7] = f->dev->graph_nodes();
+    MFS_CATCH
+}
+
 int mfs_factorization_type(mfs_factorizer f, int32_t* t) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(t);

@@ -68,6 +68,19 @@ public:
     void solve_refined(const double* Ax, const double* b, double* x, double tol, int max_iters,
                        int* iters, double* relres, void* stream);
     void export_csc(int64_t* Lp, int64_t* Li, double* Lx);
+    // refinement until every segment [seg[s], seg[s+1]) of the full-length vectors meets tol
+    void solve_refined_segments(const double* Ax, const double* b, double* x, int64_t nseg, const int64_t* seg,
+                                double tol, int max_iters, int* iters, double* relres_max, double* relres_seg,
+                                void* stream);
+    // capture the numeric factorization in a hipGraph (replayed while Ax, Bx, sigma stay the same)
+    void set_graph(bool on);
+    bool graph_on() const { return graph_on_; }
+    int64_t graph_nodes() const { return graph_nodes_; }
+    // time the numeric factorization under candidate schedules (fused-kernel bound x big-tile
+    // threshold) on the same ordering; keep the fastest.  ms[i * nt + j] for candidate (i, j).
+    void autotune(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs, const int64_t* pins, int64_t npins,
+                  const double* Ax, const int* fuse, int nfuse, const int* tiles, int ntile, int reps, double* ms,
+                  int* best, void* stream);
     void clear_factors();
     void stash();
     void swap_stash();
@@ -84,6 +97,10 @@ public:
     void require(int t) const;
 
 private:
+    void install(std::unique_ptr<Symbolic> S);
+    void drop_graph();
+    template <typename T> void enqueue_numeric(const double* Ax, double sigma, const double* Bx, void* stream,
+                                               void* ev_inject);
     void solve_permuted_inplace(double* xp, int nrhs, void* stream);   // factor-order vectors
     template <typename T> void numeric_t(const double* Ax, double sigma, const double* Bx, void* stream);
     template <typename T> void solve_t(T* xp, int nrhs, void* stream);
@@ -98,6 +115,17 @@ private:
     DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
     DeviceArray Lstash_, invstash_, scalestash_, xbstash_;
     bool stash_valid_ = false;
+    bool graph_on_ = false;
+    void* graph_ = nullptr;
+    void* graph_exec_ = nullptr;
+    void* cstream_ = nullptr;
+    void* join_[2] = {nullptr, nullptr};
+    const double* gkey_ax_ = nullptr;
+    const double* gkey_bx_ = nullptr;
+    double gkey_sigma_ = 0.0;
+    int64_t graph_nodes_ = 0;
+    int fused_grid_ = 512;
+    DeviceArray segp_, segsum_;
     int stash_type_ = 0;
     int64_t stash_succ_ = 0;
 };

@@ -1518,6 +1518,24 @@ __global__ void k_sum_final(const double* __restrict__ part, double* __restrict_
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// per-segment sum of squares over the free rows (mask >= 0): one workgroup per segment, strided
+// partial sums and a fixed-order tree, so the result does not depend on scheduling
+__global__ __launch_bounds__(256) void k_seg_sumsq(const double* __restrict__ a, const int64_t* __restrict__ mask,
+                                                   const int64_t* __restrict__ seg, double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int64_t b = seg[blockIdx.x], e = seg[blockIdx.x + 1];
+    double acc = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += 256)
+        if (mask[i] >= 0) acc += a[i] * a[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 // trs_kernels::MultiplyLowerBlock semantics on the device: the forward update's row dot
 // (lanes over rows, columns in order) with the rows scattered through `indices`.
 __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start, int64_t size,
@@ -1539,8 +1557,17 @@ __global__ void k_mlb(int BS, const int64_t* __restrict__ indices, int64_t start
 // ------------------------------------------------------------------------------------------
 FactorizerDev::FactorizerDev(int prec) : precision(prec) {
     if (prec < 0 || prec > 3) throw Error(-1, "unknown precision");
+    // persistent grid of the fused level kernel: two workgroups per compute unit
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    fused_grid_ = 2 * std::max(ncu, 1);
 }
-FactorizerDev::~FactorizerDev() = default;
+FactorizerDev::~FactorizerDev() {
+    drop_graph();
+    for (void* e : join_)
+        if (e) (void)hipEventDestroy((hipEvent_t)e);
+    if (cstream_) (void)hipStreamDestroy((hipStream_t)cstream_);
+}
 
 void FactorizerDev::require(int t) const {
     if (ftype < t)
@@ -1554,7 +1581,12 @@ size_t FactorizerDev::fsize() const { return precision != 0 ? sizeof(float) : si
 void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
                              const int64_t* pins, int64_t npins) {
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
-    auto S = std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, options_from_env()));
+    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, options_from_env())));
+}
+
+// Upload a symbolic analysis (task lists, maps) and size the factor storage for it.
+void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
+    drop_graph();
     ftype = 0;
     stash_valid_ = false;
     // injection plan entries with per-entry leading dimension and factor-order block row/column
@@ -1612,13 +1644,13 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     ftype = 1;
 }
 
+// Every device operation of one numeric factorization, enqueued on `s` (also the body of the
+// captured graph).  ev_inject, if given, is recorded after the value injection.
 template <typename T>
-void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, void* stream) {
+void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double* Bx, void* stream,
+                                    void* ev_inject) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
-    hipEvent_t ev[3];
-    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-    HIP_CHECK(hipEventRecord(ev[0], s));
     T* L = L_.as<T>();
     T* U = U_.as<T>();
     T* inv = inv_.as<T>();
@@ -1641,7 +1673,7 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
         LAUNCH_CHECK();
         prof.end(s);
     }
-    HIP_CHECK(hipEventRecord(ev[1], s));
+    if (ev_inject) HIP_CHECK(hipEventRecord((hipEvent_t)ev_inject, s));
     const GemmTask* gt = gemm_.as<GemmTask>();
     // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
     // tile kernel; TRSM and the diagonal tiles stay in the factor type
@@ -1663,16 +1695,11 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
                 prof.end(s);
             }
     };
-    static const int fused_grid = [] {
-        int dev = 0, ncu = 256;
-        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        return 2 * std::max(ncu, 1);
-    }();
     for (const LevelPlan& P : S.levels) {
         const int64_t nf = P.fused.second - P.fused.first;
         if (nf > 0) {
             prof.begin(KernelProfile::FUSED, s, 0.0, nf);
-            k_fused<T><<<(unsigned)std::min<int64_t>(nf, fused_grid), 256, 0, s>>>(
+            k_fused<T><<<(unsigned)std::min<int64_t>(nf, fused_grid_), 256, 0, s>>>(
                 fused_.as<FusedTask>() + P.fused.first, (int)nf, children_.as<int32_t>(), sdeg_.as<int32_t>(),
                 reloff_.as<int64_t>(), rel_.as<int32_t>(), uoff_.as<int64_t>(), L, U, inv, status_.as<int>());
             LAUNCH_CHECK();
@@ -1715,6 +1742,78 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
         LAUNCH_CHECK();
         prof.end(s);
     }
+}
+
+void FactorizerDev::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+    if (graph_) (void)hipGraphDestroy((hipGraph_t)graph_);
+    graph_exec_ = graph_ = nullptr;
+    graph_nodes_ = 0;
+}
+
+void FactorizerDev::set_graph(bool on) {
+    graph_on_ = on;
+    if (!on) drop_graph();
+}
+
+template <typename T>
+void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, void* stream) {
+    const Symbolic& S = *sym;
+    hipStream_t s = S_(stream);
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    if (graph_on_) {
+        // the whole factorization as one captured graph on the library's own stream (a graph
+        // cannot be captured on the legacy default stream), joined to the caller's stream by
+        // events; re-captured when the arguments or the schedule change
+        if (!cstream_) {
+            hipStream_t cs;
+            HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            cstream_ = cs;
+            for (auto& e : join_) {
+                hipEvent_t je;
+                HIP_CHECK(hipEventCreateWithFlags(&je, hipEventDisableTiming));
+                e = je;
+            }
+        }
+        hipStream_t cs = (hipStream_t)cstream_;
+        HIP_CHECK(hipEventRecord((hipEvent_t)join_[0], s));
+        HIP_CHECK(hipStreamWaitEvent(cs, (hipEvent_t)join_[0], 0));
+        if (!graph_exec_ || gkey_ax_ != Ax || gkey_bx_ != Bx || gkey_sigma_ != sigma) {
+            drop_graph();
+            const bool prof_on = prof.on;
+            prof.on = false;            // no event records inside the capture
+            HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                enqueue_numeric<T>(Ax, sigma, Bx, cs, nullptr);
+            } catch (...) {
+                (void)hipStreamEndCapture(cs, &g);
+                if (g) (void)hipGraphDestroy(g);
+                prof.on = prof_on;
+                throw;
+            }
+            prof.on = prof_on;
+            HIP_CHECK(hipStreamEndCapture(cs, &g));
+            graph_ = g;
+            hipGraphExec_t ge;
+            HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            graph_exec_ = ge;
+            size_t nn = 0;
+            HIP_CHECK(hipGraphGetNodes(g, nullptr, &nn));
+            graph_nodes_ = (int64_t)nn;
+            gkey_ax_ = Ax;
+            gkey_bx_ = Bx;
+            gkey_sigma_ = sigma;
+        }
+        HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec_, cs));
+        HIP_CHECK(hipEventRecord((hipEvent_t)join_[1], cs));
+        HIP_CHECK(hipStreamWaitEvent(s, (hipEvent_t)join_[1], 0));
+        HIP_CHECK(hipEventRecord(ev[1], s));
+    } else {
+        enqueue_numeric<T>(Ax, sigma, Bx, stream, ev[1]);
+    }
     HIP_CHECK(hipEventRecord(ev[2], s));
     int st = 0;
     HIP_CHECK(hipMemcpyAsync(&st, status_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1724,6 +1823,7 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
     HIP_CHECK(hipEventElapsedTime(&m01, ev[0], ev[1]));
     HIP_CHECK(hipEventElapsedTime(&m12, ev[1], ev[2]));
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (graph_on_) std::swap(m01, m12);   // one graph launch: no injection split
     timings[0] = m01;
     timings[1] = 0;
     timings[2] = m12;
@@ -1971,6 +2071,110 @@ void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, 
     }
 }
 
+void FactorizerDev::solve_refined_segments(const double* Ax, const double* b, double* x, int64_t nseg,
+                                           const int64_t* seg, double tol, int max_iters, int* iters,
+                                           double* relres_max, double* relres_seg, void* stream) {
+    require(2);
+    const int64_t nf = 3 * sym->n_full_blocks;
+    if (nseg < 1 || nseg > (int64_t)1 << 30) throw Error(-1, "bad segment count");
+    if (seg[0] != 0 || seg[nseg] != nf) throw Error(-1, "segments must cover the full vector");
+    for (int64_t q = 0; q < nseg; ++q)
+        if (seg[q + 1] < seg[q]) throw Error(-1, "segment offsets must be non-decreasing");
+    hipStream_t s = S_(stream);
+    std::vector<int64_t> sv(seg, seg + nseg + 1);
+    segp_.upload(sv);
+    segsum_.alloc((size_t)2 * nseg * sizeof(double));
+    double* bn = segsum_.as<double>();
+    double* rn = bn + nseg;
+    scratch_c_.alloc((size_t)3 * nf * sizeof(double));
+    double* y = scratch_c_.as<double>();
+    double* r = y + nf;
+    double* dx = r + nf;
+    const int64_t* mask = f2r_.as<int64_t>();
+    k_seg_sumsq<<<(unsigned)nseg, 256, 0, s>>>(b, mask, segp_.as<int64_t>(), bn);
+    LAUNCH_CHECK();
+    std::vector<double> hb(nseg), hr(nseg), rel(nseg);
+    HIP_CHECK(hipMemcpyAsync(hb.data(), bn, nseg * sizeof(double), hipMemcpyDeviceToHost, s));
+    solve(b, x, 1, stream);
+    int it = 0;
+    double worst = 0;
+    while (true) {
+        spmv(Ax, x, y, stream);
+        prof.begin(KernelProfile::REFINE, s);
+        k_residual<<<nblk(nf, 256), 256, 0, s>>>(b, y, mask, r, nf);
+        LAUNCH_CHECK();
+        k_seg_sumsq<<<(unsigned)nseg, 256, 0, s>>>(r, mask, segp_.as<int64_t>(), rn);
+        LAUNCH_CHECK();
+        prof.end(s);
+        HIP_CHECK(hipMemcpyAsync(hr.data(), rn, nseg * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        worst = 0;
+        for (int64_t q = 0; q < nseg; ++q) {
+            rel[q] = hb[q] > 0 ? std::sqrt(hr[q] / hb[q]) : std::sqrt(hr[q]);
+            worst = std::max(worst, rel[q]);
+        }
+        if (worst <= tol || it >= max_iters) break;
+        solve(r, dx, 1, stream);
+        prof.begin(KernelProfile::REFINE, s);
+        k_axpy1<<<nblk(nf, 256), 256, 0, s>>>(x, dx, nf);
+        LAUNCH_CHECK();
+        prof.end(s);
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (relres_max) *relres_max = worst;
+    if (relres_seg) std::copy(rel.begin(), rel.end(), relres_seg);
+    if (prof.on) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        prof.collect();
+    }
+}
+
+void FactorizerDev::autotune(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs, const int64_t* pins,
+                             int64_t npins, const double* Ax, const int* fuse, int nfuse, const int* tiles, int ntile,
+                             int reps, double* ms, int* best, void* stream) {
+    if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
+    if (nfuse < 1 || ntile < 1 || reps < 1) throw Error(-1, "autotune needs at least one candidate and one repetition");
+    SymbolicOptions base = options_from_env();
+    std::vector<int32_t> perm;
+    {
+        Symbolic S0 = analyze(Ap, Ai, nb, pins, npins, base);   // the ordering is shared by all candidates
+        perm = nd_order(S0);
+    }
+    hipStream_t s = S_(stream);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    double best_ms = 1e300;
+    int bi = 0, bj = 0;
+    for (int i = 0; i < nfuse; ++i)
+        for (int j = 0; j < ntile; ++j) {
+            SymbolicOptions o = base;
+            o.given_perm = &perm;
+            o.fuse_degree = std::max(0, fuse[i]);
+            o.big_tile = std::max(128, tiles[j]);
+            install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
+            numeric(Ax, 0.0, nullptr, stream);                   // warm-up (and pivot check)
+            HIP_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) numeric(Ax, 0.0, nullptr, stream);
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float m = 0;
+            HIP_CHECK(hipEventElapsedTime(&m, e0, e1));
+            const double t = (double)m / reps;
+            if (ms) ms[i * ntile + j] = t;
+            if (t < best_ms) { best_ms = t; bi = i; bj = j; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SymbolicOptions o = base;
+    o.given_perm = &perm;
+    o.fuse_degree = std::max(0, fuse[bi]);
+    o.big_tile = std::max(128, tiles[bj]);
+    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
+    if (best) { best[0] = o.fuse_degree; best[1] = o.big_tile; }
+}
+
 // L of P A P^T (for an FP32 factor: the factor of S P A P^T S, unscaled to L = S^{-1} L~)
 void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
     require(2);
@@ -2017,6 +2221,7 @@ void FactorizerDev::stash() {
 
 void FactorizerDev::swap_stash() {
     if (!stash_valid_) throw Error(-2, "No stashed factorization");
+    drop_graph();                 // the captured graph addresses the current panels
     // the panels, the tile and block inverses and (FP32) the scaling all belong to the factor
     std::swap(L_.p, Lstash_.p);
     std::swap(L_.bytes, Lstash_.bytes);

@@ -448,7 +448,7 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xa
     std::vector<int32_t> out;
     out.reserve(n);
     // top level: the connected components (independent meshes of a batch) are dissected in
-    // parallel, each with its own random stream, and concatenated in component order
+    // parallel, each from the same seed, and concatenated in component order
     std::vector<int32_t> comp;
     const int32_t nc = components(g, comp);
     std::vector<std::vector<int32_t>> gids(nc);
@@ -469,7 +469,7 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xa
             for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) { s.adj.push_back(local[g.adj[e]]); s.ew.push_back(1); }
             s.xadj[i + 1] = (int64_t)s.adj.size();
         }
-        Rng rng{nc == 1 ? opt.seed : opt.seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(c + 1))};
+        Rng rng{opt.seed};   // every component ordered exactly as it would be on its own
         nd_recurse(s, mem, outs[c], rng, opt);
     };
     const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));

@@ -498,7 +498,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         (void)c_is_u;
         if (c1 <= c0 || K <= 0) return;
         const bool use_big = allow_big && (c1 - c0) >= BT && (rhi - c0) >= BT;
-        const int TT = use_big ? BT : T;
+        const int TT = use_big ? 128 : T;     // the large tile of the GEMM kernel (BT is only the threshold)
         // groups of G tile columns, row by row inside a group: a run of G*G consecutive tasks
         // covers a G x G block of tiles that shares G row panels of A and G column panels of B
         // (the kernel hands each XCD a contiguous run of tasks, so a run shares one L2)
@@ -793,6 +793,20 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 P.bdiag_steps.push_back({d0, (int64_t)S.bdiag_tiles.size()});
             }
         }
+    }
+    // every task must fit the tile of the kernel that runs it: [b, m) 128x128, [m, e) 64x64
+    auto check_range = [&](const TaskRange& r) {
+        for (int64_t q = r.b; q < r.e; ++q) {
+            const GemmTask& g = S.gemm_tasks[q];
+            const int lim = q < r.m ? 128 : T;
+            if (g.M < 1 || g.N < 1 || g.M > lim || g.N > lim || g.K < 1)
+                throw Error(-2, "internal: GEMM task does not fit its tile kernel");
+        }
+    };
+    for (const LevelPlan& P : S.levels) {
+        for (const TaskRange& r : P.trsm_steps) check_range(r);
+        for (const TaskRange& r : P.update_steps) check_range(r);
+        check_range(P.syrk);
     }
     S.U_size = std::max<int64_t>(arena.peak, 32);
     S.inv_slots = std::max<int64_t>((int64_t)S.potrf_tasks.size(), 1);

@@ -289,6 +289,48 @@ class GpuSupernodalFactorizer:
                   int(maxIters), it, rr, _stream())
         return x, it.value, rr.value
 
+    def solveRefinedSegments(self, H, b, segments, tol=1e-10, maxIters=10):
+        """Refinement until every segment [segments[s], segments[s+1]) of the full-length vector
+        (one mesh of a batch) reaches ||b_s - (A x)_s|| / ||b_s|| <= tol.
+        Returns (x, iterations, max relative residual, per-segment relative residuals)."""
+        seg = np.ascontiguousarray(segments, np.int64)
+        x = torch.empty_like(b)
+        it, rr = _lib.c_i32(), _lib.c_double()
+        per = np.zeros(len(seg) - 1)
+        _lib.call("mfs_solve_refined_segments", self._h, _dptr(H.Ax), _dptr(b.contiguous()), _dptr(x), len(seg) - 1,
+                  ptr(seg, _lib.c_i64), float(tol), int(maxIters), it, rr, ptr(per, _lib.c_double), _stream())
+        return x, it.value, rr.value, per
+
+    def setGraphCapture(self, on=True):
+        """Capture the numeric factorization in a hipGraph and replay it while the value array,
+        shift and shift matrix stay the same."""
+        _lib.call("mfs_factorizer_set_graph", self._h, int(bool(on)))
+
+    def scheduleStats(self):
+        v = np.zeros(8, np.int64)
+        _lib.call("mfs_schedule_stats", self._h, ptr(v, _lib.c_i64))
+        keys = ["fused_tasks", "fused_launches", "tiled_supernodes", "potrf_launches", "gemm_launches",
+                "extend_launches", "levels", "graph_nodes"]
+        return {k: int(x) for k, x in zip(keys, v)}
+
+    def autotune(self, H, pinnedVars=(), fuse=(0, 192, 384, 768), tiles=(128, 256), reps=2):
+        """Time the numeric factorization of H under each (fused-kernel bound, big-tile threshold)
+        schedule on one ordering and keep the fastest; leaves a symbolic factorization."""
+        pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
+        fz = np.ascontiguousarray(fuse, np.int32)
+        tz = np.ascontiguousarray(tiles, np.int32)
+        ms = np.zeros(len(fz) * len(tz))
+        best = np.zeros(2, np.int32)
+        _lib.call("mfs_autotune", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64), H.n, 3,
+                  ptr(pins, _lib.c_i64), len(pins), _dptr(H.Ax), ptr(fz, _lib.c_i32), len(fz), ptr(tz, _lib.c_i32),
+                  len(tz), int(reps), ptr(ms, _lib.c_double), ptr(best, _lib.c_i32), _stream())
+        self._fixed = [int(p) for p in pins]
+        self._n_full = 3 * H.n
+        self._n_red = 3 * H.n - len(self._fixed)
+        self._Hpattern = (H.n, H.nz)
+        table = {(int(a), int(b)): float(ms[i * len(tz) + j]) for i, a in enumerate(fz) for j, b in enumerate(tz)}
+        return {"fuse_degree": int(best[0]), "big_tile": int(best[1]), "ms": table}
+
     def exportFactorCSC(self):
         n = self.n_reduced()
         nnz = self.getFactorNNZ()

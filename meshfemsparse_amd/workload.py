@@ -47,3 +47,31 @@ def z0_pins(N):
     M = N + 1
     nodes = np.arange(M * M, dtype=np.int64)
     return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+
+
+def batch_meshes(nmeshes=512, nmin=6, nmax=20, seed=0):
+    """Independent unit cubes with per-side hex counts n ~ U{nmin..nmax}, numbered one after
+    another as one disconnected mesh (its Hessian is block diagonal, one block per mesh).
+    Returns dict(n, tets, X_rest, node_off[m+1], pins, seg[m+1] (scalar offsets per mesh))."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, nmeshes)
+    tets, Xs, pins, node_off = [], [], [], [0]
+    for n in ns:
+        t, X = cube_mesh(int(n))
+        tets.append(t + node_off[-1])
+        Xs.append(X)
+        pins.append(z0_pins(int(n)) + 3 * node_off[-1])
+        node_off.append(node_off[-1] + len(X))
+    node_off = np.array(node_off, np.int64)
+    return dict(n=ns, tets=np.ascontiguousarray(np.concatenate(tets), np.int32), X_rest=np.concatenate(Xs),
+                node_off=node_off, pins=np.concatenate(pins), seg=3 * node_off)
+
+
+def batch_perturb(batch, seed, sigma_rel=1e-3):
+    """Each mesh perturbed with its own h = 1/n and a seed derived from (seed, mesh index)."""
+    Xr = batch["X_rest"]
+    X = np.empty_like(Xr)
+    no = batch["node_off"]
+    for m, n in enumerate(batch["n"]):
+        X[no[m]:no[m + 1]] = perturb(int(n), Xr[no[m]:no[m + 1]], seed * 1000003 + m, sigma_rel)
+    return X

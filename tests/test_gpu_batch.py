@@ -1,0 +1,123 @@
+"""GPU: batched factorization of many independent meshes (one block-diagonal system), the fused
+level kernel, hipGraph capture of the numeric factorization, and the autotuner.
+
+Each mesh of a batch is a connected component of the matrix graph and is ordered exactly as it
+would be on its own, so every mesh's block of the batch factor is that mesh's own factor; the
+batch solution is checked against per-mesh factorizations to the reference's 1e-11 bound
+(test_catamari_factorizer.cc:31), and per-mesh refinement against each mesh's own residual.
+"""
+import numpy as np
+import pytest
+import torch
+
+import meshfemsparse_amd as mfs
+from meshfemsparse_amd import workload as wl
+from tests import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_problem(dev, nmeshes=7, nmin=2, nmax=5, seed=3):
+    B = wl.batch_meshes(nmeshes, nmin, nmax, seed)
+    X = wl.batch_perturb(B, 0)
+    rng = np.random.default_rng(seed)
+    Ee = 10 ** rng.uniform(3, 6, len(B["tets"]))
+    He, g, _ = wl.neohookean(B["X_rest"], X, B["tets"], E_per_elem=Ee, nu=0.3)
+    nb = int(B["node_off"][-1])
+    asm = mfs.SystemAssembler(nb)
+    H = asm.blockSparsityPattern(B["tets"])
+    H.setZero()
+    asm.assembleHessian(H, B["tets"], torch.from_numpy(He.ravel()).to(dev))
+    return B, H, He, g, Ee
+
+
+def mesh_alone(dev, B, He, Ee, m):
+    """Mesh m of the batch as its own system."""
+    n = int(B["n"][m])
+    no = B["node_off"]
+    tets, _ = wl.cube_mesh(n)
+    ecount = [6 * int(k) ** 3 for k in B["n"]]
+    eb = int(np.sum(ecount[:m]))
+    nbm = int(no[m + 1] - no[m])
+    asm = mfs.SystemAssembler(nbm)
+    H = asm.blockSparsityPattern(tets)
+    H.setZero()
+    asm.assembleHessian(H, tets, torch.from_numpy(He[eb:eb + ecount[m]].ravel()).to(dev))
+    return H, wl.z0_pins(n)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("fuse", [0, 100000])
+def test_batch_matches_per_mesh_solves(dev, single, fuse, monkeypatch):
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
+    B, H, He, g, Ee = batch_problem(dev)
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    f.factorizeSymbolic(H, B["pins"])
+    f.factorizeNumeric(H)
+    b = torch.from_numpy(-g).to(dev)
+    x, it, rr, per = f.solveRefinedSegments(H, b, B["seg"], tol=1e-10, maxIters=30)
+    assert rr <= 1e-10 and (per <= 1e-10).all()
+    xh = x.cpu().numpy()
+    seg = B["seg"]
+    for m in range(len(B["n"])):
+        Hm, pm = mesh_alone(dev, B, He, Ee, m)
+        fm = mfs.make_cholesky_factorizer()
+        fm.factorizeSymbolic(Hm, pm)
+        fm.factorizeNumeric(Hm)
+        bm = torch.from_numpy(-g[seg[m]:seg[m + 1]].copy()).to(dev)
+        xm = fm.solve(bm).cpu().numpy()
+        assert np.linalg.norm(xh[seg[m]:seg[m + 1]] - xm) <= 1e-9 * np.linalg.norm(xm)
+
+
+def test_batch_factor_flops_are_per_mesh_sums(dev):
+    B, H, He, g, Ee = batch_problem(dev, nmeshes=5)
+    f = mfs.make_cholesky_factorizer()
+    f.factorizeSymbolic(H, B["pins"])
+    tot, nnz = 0.0, 0
+    for m in range(len(B["n"])):
+        Hm, pm = mesh_alone(dev, B, He, Ee, m)
+        fm = mfs.make_cholesky_factorizer()
+        fm.factorizeSymbolic(Hm, pm)
+        tot += fm.getFlopEstimate()
+        nnz += fm.getFactorNNZ()
+    assert f.getFactorNNZ() == nnz
+    assert abs(f.getFlopEstimate() - tot) <= 1e-9 * tot
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_graph_replays_equal_eager_factor(dev, single, monkeypatch):
+    """The captured numeric factorization, replayed several times with new values in the same
+    arrays and eager device work between replays, gives the eager factor bit for bit."""
+    monkeypatch.setenv("MFS_FUSE_DEGREE", "300")
+    B, H, He, g, Ee = batch_problem(dev, nmeshes=4, nmin=3, nmax=6)
+    fe = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    fg = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    fe.factorizeSymbolic(H, B["pins"])
+    fg.factorizeSymbolic(H, B["pins"])
+    fg.setGraphCapture(True)
+    A0 = H.Ax.clone()
+    for k, scale in enumerate([1.0, 2.5, 0.75, 1.0]):
+        H.Ax.copy_(A0 * scale)
+        junk = torch.randn(1 << 20, device=dev).sum()       # eager work between replays
+        fe.factorizeNumeric(H)
+        fg.factorizeNumeric(H)
+        torch.cuda.synchronize()
+        assert np.array_equal(orc.dense_L(*fg.exportFactorCSC()), orc.dense_L(*fe.exportFactorCSC())), k
+        del junk
+    assert fg.scheduleStats()["graph_nodes"] > 0
+    H.Ax.copy_(A0)
+
+
+def test_autotune_keeps_a_candidate_and_the_factor(dev):
+    B, H, He, g, Ee = batch_problem(dev, nmeshes=5, nmin=3, nmax=6)
+    f = mfs.make_cholesky_factorizer()
+    res = f.autotune(H, B["pins"], fuse=(0, 96, 100000), tiles=(128, 256), reps=1)
+    assert (res["fuse_degree"], res["big_tile"]) in res["ms"]
+    assert all(v > 0 for v in res["ms"].values())
+    f.factorizeNumeric(H)
+    ref = mfs.make_cholesky_factorizer()
+    ref.factorizeSymbolic(H, B["pins"])
+    ref.factorizeNumeric(H)
+    b = torch.from_numpy(-g).to(dev)
+    x1, x2 = f.solve(b), ref.solve(b)
+    assert torch.linalg.norm(x1 - x2) <= 1e-11 * torch.linalg.norm(x2)

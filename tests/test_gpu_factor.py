@@ -71,14 +71,18 @@ def oracle_factor(H, f, sigma=0.0, Bx=None):
     return orc.chol(Cp, Ci, Cx), red, pos
 
 
-@pytest.mark.parametrize("N,outer,fuse", [(2, None, 0), (5, None, 0), (8, None, 0), (8, 128, 0), (9, 64, 0),
-                                          (5, None, 100000), (8, None, 240), (9, 64, 100000)])
-def test_factor_matches_unique_cholesky(dev, N, outer, fuse, monkeypatch):
+@pytest.mark.parametrize("N,outer,fuse,big", [(2, None, 0, 128), (5, None, 0, 128), (8, None, 0, 128),
+                                              (8, 128, 0, 128), (9, 64, 0, 128), (5, None, 100000, 128),
+                                              (8, None, 240, 128), (9, 64, 100000, 128), (12, None, 0, 256),
+                                              (12, None, 600, 512)])
+def test_factor_matches_unique_cholesky(dev, N, outer, fuse, big, monkeypatch):
     """fuse > 0 sends every supernode one diagonal tile wide with at most `fuse` structure rows
-    through the fused level kernel (one workgroup per supernode)."""
+    through the fused level kernel (one workgroup per supernode); big is the region size from
+    which the GEMM tasks use 128x128 tiles."""
     if outer:
         monkeypatch.setenv("MFS_OUTER", str(outer))      # several outer blocks per supernode
     monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
+    monkeypatch.setenv("MFS_BIG_TILE", str(big))
     H, pins, _, _, _ = mesh_problem(dev, N)
     f = mfs.make_cholesky_factorizer()
     f.factorizeSymbolic(H, pins)

@@ -148,10 +148,12 @@ def emulate(H, pins, Ax, sigma=0.0, x_perm=None):
     return iperm, (Lp, Li, Lx), fc[0]
 
 
-@pytest.mark.parametrize("N,sigma,outer,fuse", [(3, 0.0, None, 0), (5, 0.0, None, 0), (7, 2.5, None, 0),
-                                                (7, 0.0, 64, 0), (8, 0.0, 128, 0), (5, 0.0, None, 96),
-                                                (7, 0.0, None, 100000), (8, 1.5, 128, 300)])
-def test_device_schedule_replay_matches_oracle(N, sigma, outer, fuse, monkeypatch):
+@pytest.mark.parametrize("N,sigma,outer,fuse,big", [(3, 0.0, None, 0, 128), (5, 0.0, None, 0, 128),
+                                                    (7, 2.5, None, 0, 128), (7, 0.0, 64, 0, 128),
+                                                    (8, 0.0, 128, 0, 128), (5, 0.0, None, 96, 128),
+                                                    (7, 0.0, None, 100000, 128), (8, 1.5, 128, 300, 128),
+                                                    (9, 0.0, None, 0, 256)])
+def test_device_schedule_replay_matches_oracle(N, sigma, outer, fuse, big, monkeypatch):
     """The device task lists (injection plan, extend-add, tile POTRF/TRSM, left-looking in-block
     updates, outer trailing updates, Schur updates, fused small-supernode tasks), replayed on the
     host, reproduce the oracle's unique Cholesky factor.  Small outer blocks put several outer
@@ -160,6 +162,7 @@ def test_device_schedule_replay_matches_oracle(N, sigma, outer, fuse, monkeypatc
     if outer:
         monkeypatch.setenv("MFS_OUTER", str(outer))
     monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
+    monkeypatch.setenv("MFS_BIG_TILE", str(big))
     tets, Xr = wl.cube_mesh(N)
     X = wl.perturb(N, Xr, 0)
     He, _, _ = wl.neohookean(Xr, X, tets, gradient=False)
@@ -239,3 +242,21 @@ def test_device_solve_schedule_replay(case, block, chunk, fuse, monkeypatch):
     _, _, fc, xs = emulate(H, pins, Ax, 0.0, A @ xt)
     assert fc == -1
     assert np.linalg.norm(xs - xt) <= 1e-12 * np.linalg.norm(xt) * 100
+
+
+def test_batch_analysis_is_per_mesh_analysis():
+    """A batch of independent meshes is one block-diagonal system; every mesh is a connected
+    component ordered exactly as on its own, so factor nnz and flops are the per-mesh sums."""
+    B = wl.batch_meshes(6, 2, 5, seed=1)
+    H = mfs.SystemAssembler(int(B["node_off"][-1])).blockSparsityPattern(B["tets"])
+    _, st, fl = analyze(H, B["pins"])
+    nnz, flops, nred = 0, 0.0, 0
+    for n in B["n"]:
+        tets, X = wl.cube_mesh(int(n))
+        Hm = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
+        _, stm, flm = analyze(Hm, wl.z0_pins(int(n)))
+        nnz += stm[3]
+        flops += flm[0]
+        nred += stm[0]
+    assert st[0] == nred and st[3] == nnz
+    assert abs(fl[0] - flops) <= 1e-12 * flops
