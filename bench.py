@@ -104,12 +104,16 @@ def bench_batch(args):
     log(f"[bench] batch of {args.batch}: {len(tets)} tets, {nb} nodes, {H.nz} blocks; {time.perf_counter() - t0:.1f}s")
     He_d, b_d = [], []
     t0 = time.perf_counter()
+    gasm = mfs.DeviceAssembler(H, tets)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    Xr_d, Ee_d = T(B["X_rest"]), T(Ee)
     for sd in range(nseeds):
-        X = wl.batch_perturb(B, sd)
-        He, g, _ = wl.neohookean(B["X_rest"], X, tets, E_per_elem=Ee, nu=0.3)
-        He_d.append(torch.from_numpy(He.ravel()).to(dev))
-        b_d.append(torch.from_numpy(-g).to(dev))
-        del He, g
+        He, g12 = gasm.neohookean(Xr_d, T(wl.batch_perturb(B, sd)), E_per_elem=Ee_d, nu=0.3)
+        g = torch.zeros(3 * nb, dtype=torch.float64, device=dev)
+        gasm.assembleGradient(g12, g)
+        He_d.append(He)
+        b_d.append(-g)
+        del g12
     torch.cuda.synchronize()
     log(f"[bench] generated + uploaded {nseeds} seeds in {time.perf_counter() - t0:.1f}s")
     single = args.precision != "fp64"
@@ -158,7 +162,7 @@ def bench_batch(args):
         "value": wall / args.steps, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if not single else "f32 factor/solve, f64 residual+refinement",
-        "data": "synthetic (seeded perturbations of each cube, neo-Hookean He on host)",
+        "data": "synthetic (seeded perturbations of each cube, neo-Hookean He and RHS from the device element kernel)",
         "config": {"workload": f"{args.batch} unit cubes, per-side n ~ U{{6..20}} hexes x 6 Kuhn tets, E log-uniform "
                                f"[1e3,1e6], nu 0.3, z=0 pinned, one block-diagonal system",
                    "free_dofs": 3 * nb - len(B["pins"]), "tets": len(tets), "factor_nnz": f.getFactorNNZ(),
@@ -234,12 +238,18 @@ def main():
         rng = np.random.default_rng(12345)
         Ee = 10 ** rng.uniform(3, 7, len(tets))
         nue = rng.uniform(0.2, 0.45, len(tets))
+    # element Hessians and gradients by the device element kernel (the same formulas as the host
+    # generator), gradients assembled on the device; RHS = -gradient
+    gasm = mfs.DeviceAssembler(H, tets)
+    T = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    Xr_d, Ee_d, nue_d = T(Xr), T(Ee), T(nue)
     for s in range(nseeds):
-        X = wl.perturb(N, Xr, seed=s)
-        He, g, _ = wl.neohookean(Xr, X, tets, E_per_elem=Ee, nu_per_elem=nue)
-        He_d.append(torch.from_numpy(He.ravel()).to(dev))
-        b_d.append(torch.from_numpy(-g).to(dev))
-        del He, g
+        He, g12 = gasm.neohookean(Xr_d, T(wl.perturb(N, Xr, seed=s)), E_per_elem=Ee_d, nu_per_elem=nue_d)
+        g = torch.zeros(3 * nb, dtype=torch.float64, device=dev)
+        gasm.assembleGradient(g12, g)
+        He_d.append(He)
+        b_d.append(-g)
+        del g12
     torch.cuda.synchronize()
     log(f"[bench] generated + uploaded {nseeds} seeds in {time.perf_counter() - t0:.1f}s")
     single = args.precision != "fp64"
@@ -367,7 +377,7 @@ def main():
                                                                   "bf16x3 Schur updates (f32 accumulation), "
                                                                   if args.precision == "bf16x3" else "")
                                            + "f64 residual+refinement"),
-        "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He on host)",
+        "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He and RHS from the device element kernel)",
         "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean "
                                + ("E log-uniform [1e3,1e7] nu U[0.2,0.45]" if args.stress else "E=1e5 nu=0.3")
                                + ", sigma=1e-3 h, z=0 pinned; assembly + "

@@ -92,6 +92,20 @@ int mfs_assemble_hessian(mfs_assembler a, const double* He_dev, int he_layout,
 int mfs_assembler_zero(mfs_assembler a, double* Ax_dev, void* stream);
 int mfs_assembler_destroy(mfs_assembler a);
 
+/* Element kernel (the reference's caller-supplied eval_He / eval_ge, SystemAssembler.hh:559-565,
+ * 700-735, computed on the device): compressible neo-Hookean
+ * psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2 per linear tet of the assembler's mesh, at
+ * rest positions Xr_dev and current positions X_dev (num_blocks x 3).  E_dev / nu_dev per element
+ * may be NULL (then E, nu).  He_dev (layout MFS_HE_FULL12 or MFS_HE_UPPER78) and g12_dev (12 per
+ * element: dE/dx of the 4 nodes) may each be NULL.  The math is the host generator's.       */
+int mfs_neohookean_elements(mfs_assembler a, const double* Xr_dev, const double* X_dev, const double* E_dev,
+                            const double* nu_dev, double E, double nu, double* He_dev, int32_t he_layout,
+                            double* g12_dev, void* stream);
+/* assembleGradient (SystemAssembler.hh:700-735, the gather variant :785-883): accumulates
+ * g[3 v + i] += sum over the elements of node v, in ascending element order, of g12[12 e + 3 a + i]
+ * (a = the node's local index), i.e. the result of a serial element-order loop.              */
+int mfs_assemble_gradient(mfs_assembler a, const double* g12_dev, double* g_dev, void* stream);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Factorizer                                                                                  */
 /* Replaces CholeskyFactorizerBase / CatamariFactorizer                                        */

@@ -26,6 +26,8 @@ _SIGS = {
     "mfs_assemble_hessian": [c_vp, c_vp, ctypes.c_int, c_vp, c_vp],
     "mfs_assembler_zero": [c_vp, c_vp, c_vp],
     "mfs_assembler_destroy": [c_vp],
+    "mfs_neohookean_elements": [c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_i32, c_vp, c_vp],
+    "mfs_assemble_gradient": [c_vp, c_vp, c_vp, c_vp],
     "mfs_factorizer_create": [ctypes.c_int, ctypes.POINTER(c_vp)],
     "mfs_factorizer_destroy": [c_vp],
     "mfs_factorize_symbolic": [c_vp, P_i64, P_i64, c_i64, c_i32, P_i64, c_i64],

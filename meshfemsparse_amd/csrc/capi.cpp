@@ -101,6 +101,22 @@ int mfs_assembler_zero(mfs_assembler a, double* Ax, void* stream) {
     MFS_CATCH
 }
 
+int mfs_neohookean_elements(mfs_assembler a, const double* Xr, const double* X, const double* Ee,
+                            const double* nue, double E, double nu, double* He, int32_t layout, double* g12,
+                            void* stream) {
+    MFS_TRY
+    CHECK_PTR(a); CHECK_PTR(Xr); CHECK_PTR(X);
+    a->dev->neohookean(Xr, X, Ee, nue, E, nu, He, layout, g12, stream);
+    MFS_CATCH
+}
+
+int mfs_assemble_gradient(mfs_assembler a, const double* g12, double* g, void* stream) {
+    MFS_TRY
+    CHECK_PTR(a); CHECK_PTR(g12); CHECK_PTR(g);
+    a->dev->assemble_gradient(g12, g, stream);
+    MFS_CATCH
+}
+
 int mfs_assembler_destroy(mfs_assembler a) {
     delete a;
     return MFS_OK;

@@ -28,10 +28,17 @@ public:
     AssemblerDev(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);
     void assemble(const double* He, int layout, double* Ax, void* stream);
     void zero(double* Ax, void* stream);
+    // neo-Hookean element Hessians (layout 0: 144, 1: 78 per element) and per-element gradients
+    // (12 per element); any output may be null
+    void neohookean(const double* Xr, const double* X, const double* Ee, const double* nue, double E, double nu,
+                    double* He, int layout, double* g12, void* stream);
+    // g += sum_e scatter(g12_e), fixed ascending element order per node
+    void assemble_gradient(const double* g12, double* g, void* stream);
     int64_t nslots() const { return nslots_; }
 private:
-    int64_t nslots_ = 0, ne_ = 0;
-    DeviceArray seg_, contrib_, diag_;
+    int64_t nslots_ = 0, ne_ = 0, nnodes_ = 0;
+    int npe_ = 0;
+    DeviceArray seg_, contrib_, diag_, tets_, nptr_, nent_;
 };
 
 // Optional per-kernel-class device timing (HIP event pairs around each launch).

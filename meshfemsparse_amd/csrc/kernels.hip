@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "device.h"
+#include "neohookean.h"
 
 namespace mfs {
 
@@ -187,13 +188,124 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Element kernel: neo-Hookean Hessian (12x12 or its local-upper 78 values) and gradient of every
+// linear tet, one thread per element (the element math is shared with the host generator).
+// Gradient assembly: one thread per node sums its elements' contributions in ascending element
+// order (the order of a serial element loop), accumulating into g.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_neohookean(const double* __restrict__ Xr, const double* __restrict__ X,
+                                                    const int32_t* __restrict__ tets, int64_t ne,
+                                                    const double* __restrict__ Ee, const double* __restrict__ nue,
+                                                    double E, double nu, double* __restrict__ He, int layout,
+                                                    double* __restrict__ g12) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double xr[4][3], x[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int64_t v = tets[4 * e + a];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            xr[a][r] = Xr[3 * v + r];
+            x[a][r] = X[3 * v + r];
+        }
+    }
+    NHState st;
+    nh_state(xr, x, Ee ? Ee[e] : E, nue ? nue[e] : nu, st);
+    if (g12) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g12[12 * e + 3 * a + i] = nh_grad(st, a, i);
+    }
+    if (!He) return;
+    if (layout == 0) {
+        double* h = He + 144 * e;
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) h[(3 * b + j) * 12 + 3 * a + i] = nh_hess(st, a, i, b, j);
+    } else {
+        // local-upper packing: off-diagonal (a, b), a < b, as column-major 3x3, then the upper
+        // triangles of the diagonal blocks
+        const int A[6] = {0, 0, 0, 1, 1, 2}, B[6] = {1, 2, 3, 2, 3, 3};
+        double* h = He + 78 * e;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) h[9 * q + 3 * c + r] = nh_hess(st, A[q], r, B[q], c);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r <= c; ++r) h[54 + 6 * a + (c * (c + 1)) / 2 + r] = nh_hess(st, a, r, a, c);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grad_gather(const int64_t* __restrict__ nptr, const int64_t* __restrict__ nent,
+                                                     const double* __restrict__ g12, double* __restrict__ g,
+                                                     int64_t nnodes) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    double s0 = g[3 * v], s1 = g[3 * v + 1], s2 = g[3 * v + 2];
+    for (int64_t q = nptr[v]; q < nptr[v + 1]; ++q) {
+        const int64_t w = nent[q];                   // (element << 2) | local node
+        const double* src = g12 + 12 * (w >> 2) + 3 * (w & 3);
+        s0 += src[0];
+        s1 += src[1];
+        s2 += src[2];
+    }
+    g[3 * v] = s0;
+    g[3 * v + 1] = s1;
+    g[3 * v + 2] = s2;
+}
+
 AssemblerDev::AssemblerDev(const Pattern& P, const int32_t* en, int64_t ne, int npe) {
     SlotMap M = build_slot_map(P, en, ne, npe);
     nslots_ = M.nslots;
     ne_ = ne;
+    npe_ = npe;
+    nnodes_ = P.n;
     seg_.upload(M.seg);
     contrib_.upload(M.contrib);
     diag_.upload(M.is_diag);
+    // connectivity and node -> (element, local node) lists in ascending element order
+    std::vector<int32_t> conn(en, en + ne * npe);
+    tets_.upload(conn);
+    std::vector<int64_t> ptr(P.n + 1, 0);
+    for (int64_t q = 0; q < ne * npe; ++q) ptr[en[q] + 1]++;
+    for (int64_t v = 0; v < P.n; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int64_t> ent(ptr[P.n]);
+    std::vector<int64_t> bk(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < npe; ++a) ent[bk[en[e * npe + a]]++] = (e << 2) | a;
+    nptr_.upload(ptr);
+    nent_.upload(ent);
+}
+
+void AssemblerDev::neohookean(const double* Xr, const double* X, const double* Ee, const double* nue, double E,
+                              double nu, double* He, int layout, double* g12, void* stream) {
+    if (npe_ != 4) throw Error(-6, "the element kernel handles linear tetrahedra (4 nodes)");
+    if (layout != 0 && layout != 1) throw Error(-1, "unknown element Hessian layout");
+    if (ne_ == 0) return;
+    k_neohookean<<<nblk(ne_, 256), 256, 0, S_(stream)>>>(Xr, X, tets_.as<int32_t>(), ne_, Ee, nue, E, nu, He, layout,
+                                                         g12);
+    LAUNCH_CHECK();
+}
+
+void AssemblerDev::assemble_gradient(const double* g12, double* g, void* stream) {
+    if (npe_ != 4) throw Error(-6, "gradient assembly handles 4-node elements");
+    if (nnodes_ == 0) return;
+    k_grad_gather<<<nblk(nnodes_, 256), 256, 0, S_(stream)>>>(nptr_.as<int64_t>(), nent_.as<int64_t>(), g12, g,
+                                                              nnodes_);
+    LAUNCH_CHECK();
 }
 
 void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stream) {

@@ -1,8 +1,8 @@
 // Synthetic Newton-step workload: Kuhn-split cube mesh, seeded Gaussian node perturbation and the
-// compressible neo-Hookean element gradient/Hessian (linear tets).  The reference ships no energy
-// code (its caller supplies eval_He, SystemAssembler.hh:559-565), so this file fixes the form:
-//   psi(F) = mu/2 (tr(F^T F) - 3) - mu ln J + lambda/2 (ln J)^2,   E_e = V_e psi(F_e).
+// compressible neo-Hookean element gradient/Hessian (linear tets, neohookean.h).  The reference
+// ships no energy code (its caller supplies eval_He, SystemAssembler.hh:559-565).
 #include "mfs_internal.h"
+#include "neohookean.h"
 
 #include <algorithm>
 #include <cmath>
@@ -25,89 +25,25 @@ inline double uniform_open0(uint64_t& s) {   // (0, 1]
     return (double)((splitmix64(s) >> 11) + 1) * (1.0 / 9007199254740992.0);
 }
 
-inline double det3(const double* A) {         // column-major 3x3
-    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
-           A[6] * (A[1] * A[5] - A[4] * A[2]);
-}
-inline void inv3(const double* A, double* R, double det) {
-    const double id = 1.0 / det;
-    R[0] = (A[4] * A[8] - A[5] * A[7]) * id;
-    R[1] = (A[2] * A[7] - A[1] * A[8]) * id;
-    R[2] = (A[1] * A[5] - A[2] * A[4]) * id;
-    R[3] = (A[5] * A[6] - A[3] * A[8]) * id;
-    R[4] = (A[0] * A[8] - A[2] * A[6]) * id;
-    R[5] = (A[2] * A[3] - A[0] * A[5]) * id;
-    R[6] = (A[3] * A[7] - A[4] * A[6]) * id;
-    R[7] = (A[1] * A[6] - A[0] * A[7]) * id;
-    R[8] = (A[0] * A[4] - A[1] * A[3]) * id;
-}
-
 void element(const double* Xr, const double* X, const int32_t* t, double E, double nu,
              double* He, double* g12, double* energy) {
-    double Dm[9], Ds[9], Dmi[9];
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) {
-            Dm[3 * c + r] = Xr[3 * t[c + 1] + r] - Xr[3 * t[0] + r];
-            Ds[3 * c + r] = X[3 * t[c + 1] + r] - X[3 * t[0] + r];
-        }
-    const double detDm = det3(Dm);
-    inv3(Dm, Dmi, detDm);
-    const double V = detDm / 6.0;
-    // shape-function gradients g_a (rest frame): g_{1..3} = rows of Dm^{-1}; g_0 = -sum
-    double g[4][3];
-    for (int a = 1; a < 4; ++a)
-        for (int k = 0; k < 3; ++k) g[a][k] = Dmi[3 * k + (a - 1)];
-    for (int k = 0; k < 3; ++k) g[0][k] = -(g[1][k] + g[2][k] + g[3][k]);
-    // F = Ds Dm^{-1}
-    double F[9];
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) {
-            double s = 0;
-            for (int k = 0; k < 3; ++k) s += Ds[3 * k + r] * Dmi[3 * c + k];
-            F[3 * c + r] = s;
-        }
-    const double J = det3(F);
-    double Fi[9];
-    inv3(F, Fi, J);
-    const double mu = E / (2.0 * (1.0 + nu));
-    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    const double lnJ = std::log(J);
-    // p_a = F^{-T} g_a, p_a[i] = sum_k Finv[k][i] g_a[k]  (Finv col-major: Fi[3*i + k] = Finv[k][i])
-    double p[4][3];
+    double xr[4][3], x[4][3];
     for (int a = 0; a < 4; ++a)
-        for (int i = 0; i < 3; ++i) {
-            double s = 0;
-            for (int k = 0; k < 3; ++k) s += Fi[3 * i + k] * g[a][k];
-            p[a][i] = s;
+        for (int r = 0; r < 3; ++r) {
+            xr[a][r] = Xr[3 * t[a] + r];
+            x[a][r] = X[3 * t[a] + r];
         }
-    if (energy) {
-        double I1 = 0;
-        for (int q = 0; q < 9; ++q) I1 += F[q] * F[q];
-        *energy = V * (0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.5 * lam * lnJ * lnJ);
-    }
-    if (g12) {
-        // dE/dx_a = V (mu F g_a + (lam lnJ - mu) p_a)
+    NHState st;
+    nh_state(xr, x, E, nu, st);
+    if (energy) *energy = nh_energy(st);
+    if (g12)
         for (int a = 0; a < 4; ++a)
-            for (int i = 0; i < 3; ++i) {
-                double Fg = 0;
-                for (int k = 0; k < 3; ++k) Fg += F[3 * k + i] * g[a][k];
-                g12[3 * a + i] = V * (mu * Fg + (lam * lnJ - mu) * p[a][i]);
-            }
-    }
-    if (He) {
-        // K_{ai,bj} = V [ mu d_ij (g_a . g_b) + (mu - lam lnJ) p_b[i] p_a[j] + lam p_a[i] p_b[j] ]
-        const double c2 = mu - lam * lnJ;
+            for (int i = 0; i < 3; ++i) g12[3 * a + i] = nh_grad(st, a, i);
+    if (He)
         for (int b = 0; b < 4; ++b)
             for (int j = 0; j < 3; ++j)
-                for (int a = 0; a < 4; ++a) {
-                    const double gab = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
-                    for (int i = 0; i < 3; ++i) {
-                        double v = c2 * p[b][i] * p[a][j] + lam * p[a][i] * p[b][j];
-                        if (i == j) v += mu * gab;
-                        He[(3 * b + j) * 12 + 3 * a + i] = V * v;
-                    }
-                }
-    }
+                for (int a = 0; a < 4; ++a)
+                    for (int i = 0; i < 3; ++i) He[(3 * b + j) * 12 + 3 * a + i] = nh_hess(st, a, i, b, j);
 }
 } // namespace
 

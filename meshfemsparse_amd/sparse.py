@@ -156,6 +156,24 @@ class DeviceAssembler:
     def zero(self, Ax):
         _lib.call("mfs_assembler_zero", self._h, _dptr(Ax), _stream())
 
+    def neohookean(self, X_rest, X, E=1e5, nu=0.3, E_per_elem=None, nu_per_elem=None, layout=MFS_HE_FULL12,
+                   hessians=True, gradients=True):
+        """Element Hessians and per-element gradients of the compressible neo-Hookean energy on the
+        device (float64 CUDA tensors; positions num_nodes x 3).  Returns (He, g12)."""
+        import torch
+        dev = X.device
+        per = 144 if layout == MFS_HE_FULL12 else 78
+        He = torch.empty(per * self.ne, dtype=torch.float64, device=dev) if hessians else None
+        g12 = torch.empty(12 * self.ne, dtype=torch.float64, device=dev) if gradients else None
+        opt = lambda t: None if t is None else _dptr(t.contiguous())
+        _lib.call("mfs_neohookean_elements", self._h, _dptr(X_rest.contiguous()), _dptr(X.contiguous()),
+                  opt(E_per_elem), opt(nu_per_elem), float(E), float(nu), opt(He), layout, opt(g12), _stream())
+        return He, g12
+
+    def assembleGradient(self, g12, g):
+        """g += sum_e scatter(g12_e) (SystemAssembler::assembleGradient accumulate semantics)."""
+        _lib.call("mfs_assemble_gradient", self._h, _dptr(g12), _dptr(g), _stream())
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.mfs_assembler_destroy(self._h)
