@@ -41,6 +41,7 @@ extern "C" {
 #define MFS_ERR_DEVICE      -4  /* HIP runtime error                                             */
 #define MFS_ERR_ALLOC       -5  /* host or device allocation failure                             */
 #define MFS_ERR_UNSUPPORTED -6  /* e.g. partially pinned blocks, non-uniform block sizes         */
+#define MFS_ERR_NOT_CONVERGED -7 /* iterative solve stopped at its iteration limit               */
 
 /* Factor precisions */
 #define MFS_PREC_FP64 0         /* plain FP64 path: the drop-in                                   */
@@ -55,6 +56,7 @@ extern "C" {
 typedef struct mfs_pattern_s*    mfs_pattern;
 typedef struct mfs_assembler_s*  mfs_assembler;
 typedef struct mfs_factorizer_s* mfs_factorizer;
+typedef struct mfs_learned_s*    mfs_learned;
 
 const char* mfs_last_error(void);
 int         mfs_version(void);
@@ -250,6 +252,56 @@ int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
 int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
 /* Executed tile flops of the profiled launches per class (21; the three GEMM classes carry them).*/
 int mfs_kernel_profile_work(mfs_factorizer f, double* work);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Learned block incomplete-Cholesky preconditioner (no reference counterpart: the reference     */
+/* preconditions only with exact factorizations).  A message-passing network (hidden 64, 4       */
+/* rounds, bfloat16-rounded weights, FP32 accumulation) over the free-node block graph emits a   */
+/* zero-fill lower block factor L on tril(A) in a greedy multicolor ordering; M = L L^T          */
+/* preconditions FP32 CG on the Jacobi-scaled system S A S, inside FP64 refinement.  Training on  */
+/* the device: loss mean_p |(L L^T - S A S) z_p|^2 / |S A S z_p|^2 over caller probes z_p, own   */
+/* forward/backward kernels, Adam.                                                              */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_learned_create(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks, const int64_t* pinned_scalar_vars,
+                       int64_t num_pins, uint64_t seed, mfs_learned* out);
+int mfs_learned_destroy(mfs_learned l);
+/* out8 = {free nodes, directed edges, lower blocks, colors, parameters, Adam steps, hidden, rounds} */
+int mfs_learned_info(mfs_learned l, int64_t* out8);
+/* offsets (floats) of the 22 parameter tensors W0 b0 {Wm bm Wu bu} x 4 Wd1 bd1 Wd2 bd2 Wdd bdd, and the end */
+int mfs_learned_param_offsets(mfs_learned l, int64_t* out23);
+/* graph arrays (any may be NULL): edge destination/source (free-node ids, grouped by destination,
+ * sources ascending), the edge, row and column of each lower block, node colors              */
+int mfs_learned_graph(mfs_learned l, int32_t* edge_dst, int32_t* edge_src, int32_t* lower_edge, int32_t* lower_dst,
+                      int32_t* lower_src, int32_t* color);
+/* the same graph from the pattern on the host (no device): sizes4 = {nodes, edges, lower blocks,
+ * colors}; call with NULL arrays first for the sizes                                          */
+int mfs_learned_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks,
+                             const int64_t* pinned_scalar_vars, int64_t num_pins, int64_t* sizes4,
+                             int32_t* edge_dst, int32_t* edge_src, int32_t* lower_edge, int32_t* lower_dst,
+                             int32_t* lower_src, int32_t* color);
+/* scaling, scaled blocks and node features from the assembled values (9 nz doubles, device)   */
+int mfs_learned_set_matrix(mfs_learned l, const double* Ax_dev, void* stream);
+/* host copies: node features (6 per node), scaled edge blocks A^(dst, src) (9 per edge,
+ * column-major), scaled diagonal blocks (9 per node), scaling (3 per node); any may be NULL   */
+int mfs_learned_features(mfs_learned l, float* node_feat, float* edge_feat, float* diag, double* scale);
+/* loss for host probes z (nprobe x 3 n floats, free-node order); backward != 0 also fills the
+ * parameter gradient                                                                          */
+int mfs_learned_loss(mfs_learned l, const float* z, int32_t nprobe, int32_t backward, double* loss, void* stream);
+int mfs_learned_adam_step(mfs_learned l, float lr, void* stream);
+/* master parameters (FP32): copy out to `get` and/or in from `set` (host arrays, nparams)     */
+int mfs_learned_params(mfs_learned l, float* get, const float* set);
+int mfs_learned_grads(mfs_learned l, float* out);
+/* network forward for the current matrix; host copies of the lower blocks (9 per block) and the
+ * diagonal blocks (9 per node), column-major; either may be NULL                              */
+int mfs_learned_factor(mfs_learned l, float* L_lower, float* L_diag, void* stream);
+/* A x = b (full-length device vectors, pins -> 0): FP32 CG preconditioned by block Jacobi
+ * (precond 0) or the network's factor (precond 1) inside FP64 refinement, until the unscaled
+ * relative residual <= tol; after max_pcg CG iterations in total the fallback factorizer (with
+ * a symbolic factorization of A) factors and solves exactly.  Without a fallback, stopping at
+ * the limit returns MFS_ERR_NOT_CONVERGED with x the last iterate.                           */
+int mfs_learned_solve(mfs_learned l, mfs_factorizer fallback, const double* Ax_dev, const double* b_dev,
+                      double* x_dev, int32_t precond, double tol, int32_t max_pcg, int32_t* pcg_iters,
+                      int32_t* outer_iters, double* relres, int32_t* used_fallback, void* stream);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Workload generator (the reference has no energy code; the caller supplies He).             */

@@ -12,6 +12,7 @@ P_i32, P_i64, P_dbl, P_u8 = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctyp
 
 MFS_OK, MFS_ERR_ARG, MFS_ERR_STATE, MFS_ERR_NOT_SPD, MFS_ERR_DEVICE, MFS_ERR_ALLOC, MFS_ERR_UNSUPPORTED = \
     0, -1, -2, -3, -4, -5, -6
+MFS_ERR_NOT_CONVERGED = -7
 MFS_PREC_FP64, MFS_PREC_FP32, MFS_PREC_BF16, MFS_PREC_BF16X3 = 0, 1, 2, 3
 MFS_HE_FULL12, MFS_HE_UPPER78 = 0, 1
 
@@ -44,6 +45,20 @@ _SIGS = {
     "mfs_autotune": [c_vp, P_i64, P_i64, c_i64, c_i32, P_i64, c_i64, c_vp, P_i32, c_i32, P_i32, c_i32, c_i32,
                      P_dbl, P_i32, c_vp],
     "mfs_schedule_stats": [c_vp, P_i64],
+    "mfs_learned_create": [P_i64, P_i64, c_i64, P_i64, c_i64, ctypes.c_uint64, ctypes.POINTER(c_vp)],
+    "mfs_learned_destroy": [c_vp],
+    "mfs_learned_info": [c_vp, P_i64],
+    "mfs_learned_param_offsets": [c_vp, P_i64],
+    "mfs_learned_graph": [c_vp, P_i32, P_i32, P_i32, P_i32, P_i32, P_i32],
+    "mfs_learned_analyze_host": [P_i64, P_i64, c_i64, P_i64, c_i64, P_i64, P_i32, P_i32, P_i32, P_i32, P_i32, P_i32],
+    "mfs_learned_set_matrix": [c_vp, c_vp, c_vp],
+    "mfs_learned_features": [c_vp, c_vp, c_vp, c_vp, c_vp],
+    "mfs_learned_loss": [c_vp, c_vp, c_i32, c_i32, P_dbl, c_vp],
+    "mfs_learned_adam_step": [c_vp, ctypes.c_float, c_vp],
+    "mfs_learned_params": [c_vp, c_vp, c_vp],
+    "mfs_learned_grads": [c_vp, c_vp],
+    "mfs_learned_factor": [c_vp, c_vp, c_vp, c_vp],
+    "mfs_learned_solve": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_i32, P_i32, P_i32, P_dbl, P_i32, c_vp],
     "mfs_factorization_type": [c_vp, P_i32],
     "mfs_num_successful_pivots": [c_vp, P_i64],
     "mfs_sizes": [c_vp, P_i64, P_i64],
@@ -101,6 +116,10 @@ class UnsupportedError(MFSError):
     pass
 
 
+class NotConvergedError(MFSError):
+    pass
+
+
 class InvalidArgumentError(MFSError, ValueError):
     pass
 
@@ -132,6 +151,8 @@ def check(status):
         raise UnsupportedError(status, msg)
     if status == MFS_ERR_ARG:
         raise InvalidArgumentError(status, msg)
+    if status == MFS_ERR_NOT_CONVERGED:
+        raise NotConvergedError(status, msg)
     raise MFSError(status, msg)
 
 

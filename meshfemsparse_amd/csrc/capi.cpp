@@ -5,6 +5,7 @@
 #include <memory>
 
 #include "device.h"
+#include "learned.h"
 
 namespace mfs {
 static thread_local std::string g_last_error;
@@ -14,6 +15,7 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 struct mfs_pattern_s { mfs::Pattern P; };
 struct mfs_assembler_s { std::unique_ptr<mfs::AssemblerDev> dev; };
 struct mfs_factorizer_s { std::unique_ptr<mfs::FactorizerDev> dev; };
+struct mfs_learned_s { std::unique_ptr<mfs::LearnedDev> dev; };
 
 #define MFS_TRY try {
 #define MFS_CATCH                                                                             \
@@ -505,4 +507,151 @@ int mfs_test_multiply_lower_block(int32_t BS, const int64_t* indices, int64_t st
     MFS_CATCH
 }
 
+
+int mfs_learned_create(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins, int64_t npins,
+                       uint64_t seed, mfs_learned* out) {
+    MFS_TRY
+    CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(out);
+    if (npins > 0) CHECK_PTR(pins);
+    auto l = std::make_unique<mfs_learned_s>();
+    l->dev = std::make_unique<mfs::LearnedDev>(Ap, Ai, nb, pins, npins, seed);
+    *out = l.release();
+    MFS_CATCH
+}
+
+int mfs_learned_destroy(mfs_learned l) {
+    delete l;
+    return MFS_OK;
+}
+
+int mfs_learned_info(mfs_learned l, int64_t* out8) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(out8);
+    const auto& g = l->dev->graph();
+    out8[0] = g.n;
+    out8[1] = (int64_t)g.esrc.size();
+    out8[2] = (int64_t)g.ledge.size();
+    out8[3] = g.ncolors;
+    out8[4] = l->dev->nparams();
+    out8[5] = l->dev->steps();
+    out8[6] = mfs::LearnedDev::H;
+    out8[7] = mfs::LearnedDev::NL;
+    MFS_CATCH
+}
+
+int mfs_learned_param_offsets(mfs_learned l, int64_t* out) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(out);
+    const auto& o = l->dev->param_offsets();
+    std::copy(o.begin(), o.end(), out);
+    MFS_CATCH
+}
+
+static void copy_lp_graph(const mfs::LPGraph& g, int32_t* edst, int32_t* esrc, int32_t* ledge, int32_t* ldst,
+                          int32_t* lsrc, int32_t* color) {
+    if (edst) std::copy(g.edst.begin(), g.edst.end(), edst);
+    if (esrc) std::copy(g.esrc.begin(), g.esrc.end(), esrc);
+    if (ledge) std::copy(g.ledge.begin(), g.ledge.end(), ledge);
+    if (ldst) std::copy(g.ldst.begin(), g.ldst.end(), ldst);
+    if (lsrc) std::copy(g.lsrc.begin(), g.lsrc.end(), lsrc);
+    if (color) std::copy(g.color.begin(), g.color.end(), color);
+}
+
+int mfs_learned_graph(mfs_learned l, int32_t* edst, int32_t* esrc, int32_t* ledge, int32_t* ldst, int32_t* lsrc,
+                      int32_t* color) {
+    MFS_TRY
+    CHECK_PTR(l);
+    copy_lp_graph(l->dev->graph(), edst, esrc, ledge, ldst, lsrc, color);
+    MFS_CATCH
+}
+
+int mfs_learned_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins, int64_t npins,
+                             int64_t* sizes4, int32_t* edst, int32_t* esrc, int32_t* ledge, int32_t* ldst,
+                             int32_t* lsrc, int32_t* color) {
+    MFS_TRY
+    CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(sizes4);
+    if (npins > 0) CHECK_PTR(pins);
+    const mfs::LPGraph g = mfs::build_lp_graph(Ap, Ai, nb, pins, npins);
+    sizes4[0] = g.n;
+    sizes4[1] = (int64_t)g.esrc.size();
+    sizes4[2] = (int64_t)g.ledge.size();
+    sizes4[3] = g.ncolors;
+    copy_lp_graph(g, edst, esrc, ledge, ldst, lsrc, color);
+    MFS_CATCH
+}
+
+int mfs_learned_set_matrix(mfs_learned l, const double* Ax, void* stream) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(Ax);
+    l->dev->set_matrix(Ax, stream);
+    MFS_CATCH
+}
+
+int mfs_learned_features(mfs_learned l, float* node_feat, float* edge_feat, float* diag, double* scale) {
+    MFS_TRY
+    CHECK_PTR(l);
+    l->dev->export_features(node_feat, edge_feat, diag, scale);
+    MFS_CATCH
+}
+
+int mfs_learned_loss(mfs_learned l, const float* z, int32_t nprobe, int32_t backward, double* loss, void* stream) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(z);
+    const double v = l->dev->loss_grad(z, nprobe, backward != 0, stream);
+    if (loss) *loss = v;
+    MFS_CATCH
+}
+
+int mfs_learned_adam_step(mfs_learned l, float lr, void* stream) {
+    MFS_TRY
+    CHECK_PTR(l);
+    l->dev->adam_step(lr, stream);
+    MFS_CATCH
+}
+
+int mfs_learned_params(mfs_learned l, float* get, const float* set) {
+    MFS_TRY
+    CHECK_PTR(l);
+    if (get) l->dev->get_params(get);
+    if (set) l->dev->set_params(set);
+    MFS_CATCH
+}
+
+int mfs_learned_grads(mfs_learned l, float* out) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(out);
+    l->dev->get_grads(out);
+    MFS_CATCH
+}
+
+int mfs_learned_factor(mfs_learned l, float* LE, float* LD, void* stream) {
+    MFS_TRY
+    CHECK_PTR(l);
+    l->dev->build_factor(stream);
+    l->dev->export_factor(LE, LD);
+    MFS_CATCH
+}
+
+int mfs_learned_solve(mfs_learned l, mfs_factorizer fallback, const double* Ax, const double* b, double* x,
+                      int32_t precond, double tol, int32_t max_pcg, int32_t* pcg_iters, int32_t* outer,
+                      double* relres, int32_t* used_fallback, void* stream) {
+    MFS_TRY
+    CHECK_PTR(l); CHECK_PTR(Ax); CHECK_PTR(b); CHECK_PTR(x);
+    int it = 0, ot = 0;
+    double rr = 0;
+    const bool ok = l->dev->solve(Ax, b, x, precond, tol, max_pcg, &it, &ot, &rr, stream);
+    int fb = 0;
+    if (!ok && fallback) {   // exact supernodal Cholesky (+ refinement) when CG ran out of iterations
+        fallback->dev->numeric(Ax, 0.0, nullptr, stream);
+        int fi = 0;
+        fallback->dev->solve_refined(Ax, b, x, tol, 10, &fi, &rr, stream);
+        fb = 1;
+    }
+    if (pcg_iters) *pcg_iters = it;
+    if (outer) *outer = ot;
+    if (relres) *relres = rr;
+    if (used_fallback) *used_fallback = fb;
+    if (!ok && !fallback) throw mfs::Error(MFS_ERR_NOT_CONVERGED, "preconditioned CG did not reach the tolerance");
+    MFS_CATCH
+}
 } // extern "C"

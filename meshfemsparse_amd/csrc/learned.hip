@@ -1,0 +1,1079 @@
+// Device side of the learned block incomplete-Cholesky preconditioner: feature extraction from the
+// assembled matrix, the message-passing network (forward and backward), the factor L and its
+// probe loss, Adam, and FP32 preconditioned CG with FP64 refinement.  All reductions run in a
+// fixed order (segmented sums over CSR lists, chunked partial sums reduced in chunk order), so
+// losses, gradients and solves are bitwise reproducible.
+//
+// Network (hidden H = 64, NL = 4 rounds; weights rounded to bfloat16 for use, FP32 accumulation):
+//   h_i = relu(x_i W0 + b0),                      x_i: 3 off-diagonal entries of the scaled diagonal
+//                                                  block and the 3 log ratios of its diagonal to
+//                                                  their geometric mean
+//   per round: m_e = relu([h_src(e), a_e] Wm + bm), a_e: the scaled 3x3 block A^(dst, src)
+//              M_i = sum_{e: dst(e) = i} m_e,  h_i += relu([h_i, M_i] Wu + bu)
+//   lower block q = (i, j): L_q = A^_ij + (relu([h_i, h_j] Wd1 + bd1) Wd2 + bd2)
+//   diagonal block i: L_ii = lower(A^_ii) + strict-lower corrections, diagonal exp(g_i),
+//                     g = h_i Wdd + bdd
+// With small output weights the initial factor is L = I + strict-lower(A^), the symmetric
+// Gauss-Seidel splitting of the scaled matrix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "device.h"
+#include "learned.h"
+
+namespace mfs {
+
+#define LP_CHECK(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            throw Error(-4, std::string("HIP error: ") + hipGetErrorString(e_) + " at " +    \
+                                __FILE__ + ":" + std::to_string(__LINE__));                 \
+    } while (0)
+#define LP_LAUNCH() LP_CHECK(hipGetLastError())
+
+namespace {
+
+inline hipStream_t SS(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+constexpr int H = LearnedDev::H;
+constexpr int WGROWS = 4096;     // rows per workgroup of the weight-gradient partial sums
+
+__global__ void k_fill_f(float* p, int64_t n, float v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_round_bf16(const float* __restrict__ s, float* __restrict__ d, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)(__bf16)s[i];
+}
+
+// Y(r, o) (+)= act(sum_k X(r, k) W'(k, o) + b(o)), o < O <= 64, K <= 128; W'(k, o) = W[k ldw + o]
+// or, transposed, W[o ldw + k].  64 rows per workgroup, operands in LDS; thread = (row, 16 cols).
+__global__ __launch_bounds__(256) void k_dense(const float* __restrict__ X, int ldx, int64_t N, int K,
+                                               const float* __restrict__ W, int ldw, int wt,
+                                               const float* __restrict__ b, float* __restrict__ Y, int ldy, int O,
+                                               int relu, int accumulate) {
+    __shared__ __attribute__((aligned(16))) float Ws[128][68];
+    __shared__ float Xs[64][129];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < K * 64; idx += 256) {
+        const int k = idx >> 6, o = idx & 63;
+        Ws[k][o] = o < O ? (wt ? W[(int64_t)o * ldw + k] : W[(int64_t)k * ldw + o]) : 0.0f;
+    }
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    for (int idx = tid; idx < 64 * K; idx += 256) {
+        const int rr = idx / K, k = idx - rr * K;
+        const int64_t r = r0 + rr;
+        Xs[rr][k] = r < N ? X[r * ldx + k] : 0.0f;
+    }
+    __syncthreads();
+    const int rr = tid >> 2, cg = (tid & 3) * 16;
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = (b && cg + c < O) ? b[cg + c] : 0.0f;
+    for (int k = 0; k < K; ++k) {
+        const float xv = Xs[rr][k];
+        const float4* wr = reinterpret_cast<const float4*>(&Ws[k][cg]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 w4 = wr[q];
+            acc[4 * q] += xv * w4.x;
+            acc[4 * q + 1] += xv * w4.y;
+            acc[4 * q + 2] += xv * w4.z;
+            acc[4 * q + 3] += xv * w4.w;
+        }
+    }
+    const int64_t r = r0 + rr;
+    if (r >= N) return;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        if (cg + c < O) {
+            float v = acc[c];
+            if (relu) v = fmaxf(v, 0.0f);
+            float* y = Y + r * ldy + cg + c;
+            *y = accumulate ? *y + v : v;
+        }
+    }
+}
+
+// partial weight gradients of one chunk of WGROWS rows: part[blk][(k, o)] = sum_r X(r, k) Z(r, o),
+// k = K is the bias row (X = 1); (K + 1) x 64 outputs per chunk
+__global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ X, int ldx, const float* __restrict__ Z,
+                                               int ldz, int64_t N, int K, int O, float* __restrict__ part) {
+    __shared__ float Xs[32][130];
+    __shared__ float Zs[32][65];
+    constexpr int MAXQ = (129 * 64 + 255) / 256;
+    const int tid = threadIdx.x, nout = (K + 1) * 64;
+    float acc[MAXQ];
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) acc[q] = 0.0f;
+    const int64_t rb = (int64_t)blockIdx.x * WGROWS, re = std::min<int64_t>(N, rb + WGROWS);
+    for (int64_t r0 = rb; r0 < re; r0 += 32) {
+        for (int idx = tid; idx < 32 * (K + 1); idx += 256) {
+            const int rr = idx / (K + 1), k = idx - rr * (K + 1);
+            const int64_t r = r0 + rr;
+            Xs[rr][k] = r < re ? (k < K ? X[r * ldx + k] : 1.0f) : 0.0f;
+        }
+        for (int idx = tid; idx < 32 * 64; idx += 256) {
+            const int rr = idx >> 6, o = idx & 63;
+            const int64_t r = r0 + rr;
+            Zs[rr][o] = (r < re && o < O) ? Z[r * ldz + o] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXQ; ++q) {
+            const int p = tid + 256 * q;
+            if (p < nout) {
+                const int k = p >> 6, o = p & 63;
+                float a = acc[q];
+#pragma unroll 8
+                for (int rr = 0; rr < 32; ++rr) a += Xs[rr][k] * Zs[rr][o];
+                acc[q] = a;
+            }
+        }
+        __syncthreads();
+    }
+    float* out = part + (int64_t)blockIdx.x * 129 * 64;
+#pragma unroll
+    for (int q = 0; q < MAXQ; ++q) {
+        const int p = tid + 256 * q;
+        if (p < nout) out[p] = acc[q];
+    }
+}
+
+// sums the chunk partials in chunk order into the weight (K x O, row stride O) and bias gradients
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nparts, int K, int O, float* __restrict__ gW,
+                               float* __restrict__ gb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (K + 1) * 64) return;
+    const int k = p >> 6, o = p & 63;
+    if (o >= O) return;
+    float s = 0.0f;
+    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * 129 * 64 + p];
+    if (k < K) gW[k * O + o] += s;
+    else gb[o] += s;
+}
+
+// out(r, off + c) = src(idx ? idx[r] : r, c) for c < k
+__global__ void k_gather_rows(float* __restrict__ out, int ldo, int off, const float* __restrict__ src, int lds,
+                              const int32_t* __restrict__ idx, int64_t N, int k) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= N * k) return;
+    const int64_t r = t / k;
+    const int c = (int)(t - r * k);
+    const int64_t s = idx ? idx[r] : r;
+    out[r * ldo + off + c] = src[s * lds + c];
+}
+
+// out(r, c) = src(idx[r], c) * (mask(r, c) > 0), k columns (backward of a gathered relu input)
+__global__ void k_gather_mask(float* __restrict__ out, const float* __restrict__ src, int lds,
+                              const int32_t* __restrict__ idx, const float* __restrict__ mask, int64_t N, int k) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= N * k) return;
+    const int64_t r = t / k;
+    const int c = (int)(t - r * k);
+    out[t] = mask[t] > 0.0f ? src[(int64_t)idx[r] * lds + c] : 0.0f;
+}
+
+// out(i, off + c) (+)= sum_{q = ptr[i]}^{ptr[i+1]-1} in(list ? list[q] : q, ioff + c), in list order
+__global__ void k_segsum(float* __restrict__ out, int ldo, int off, const float* __restrict__ in, int ldi, int ioff,
+                         int k, const int64_t* __restrict__ ptr, const int32_t* __restrict__ list, int64_t n,
+                         int accumulate) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int64_t i = t / k;
+    const int c = (int)(t - i * k);
+    float s = accumulate ? out[i * ldo + off + c] : 0.0f;
+    for (int64_t q = ptr[i]; q < ptr[i + 1]; ++q) s += in[(int64_t)(list ? list[q] : q) * ldi + ioff + c];
+    out[i * ldo + off + c] = s;
+}
+
+__global__ void k_add(float* __restrict__ y, const float* __restrict__ a, const float* __restrict__ b, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a[i] + b[i];
+}
+// y(r, c) += x(r, xoff + c), c < 64 (x row stride ldx)
+__global__ void k_add_cols(float* __restrict__ y, const float* __restrict__ x, int ldx, int xoff, int64_t N) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= N * H) return;
+    const int64_t r = t / H;
+    y[t] += x[r * ldx + xoff + (t - r * H)];
+}
+__global__ void k_mask(float* __restrict__ d, const float* __restrict__ g, const float* __restrict__ act, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = act[i] > 0.0f ? g[i] : 0.0f;
+}
+
+// ---- matrix-derived inputs ----------------------------------------------------------------
+// s = diag(A)^-1/2 per free scalar; A^ = S A S blocks of the edges (A^(dst, src), column-major)
+// and of the diagonal; node features; block-Jacobi inverses of the scaled diagonal blocks
+__global__ void k_lp_scale(const double* __restrict__ Ax, const int64_t* __restrict__ diagblk, int64_t n,
+                           double* __restrict__ s, int* __restrict__ bad) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const double* D = Ax + 9 * diagblk[v];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double d = D[4 * c];
+        if (!(d > 0.0)) atomicMin(bad, (int)(3 * v + c));
+        s[3 * v + c] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+    }
+}
+
+__global__ void k_lp_edges(const double* __restrict__ Ax, const int64_t* __restrict__ eblk,
+                           const int32_t* __restrict__ edst, const int32_t* __restrict__ esrc,
+                           const double* __restrict__ s, int64_t E, float* __restrict__ EF, double* __restrict__ EF64) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int64_t w = eblk[e];
+    const double* B = Ax + 9 * (w >> 1);
+    const bool tr = (w & 1) != 0;
+    const int64_t i = edst[e], j = esrc[e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double a = tr ? B[3 * r + c] : B[3 * c + r];     // A(dst r, src c)
+            const double v = s[3 * i + r] * a * s[3 * j + c];
+            EF[9 * e + 3 * c + r] = (float)v;
+            EF64[9 * e + 3 * c + r] = v;
+        }
+}
+
+__global__ void k_lp_nodes(const double* __restrict__ Ax, const int64_t* __restrict__ diagblk,
+                           const double* __restrict__ s, int64_t n, float* __restrict__ X0, float* __restrict__ AD,
+                           double* __restrict__ AD64, float* __restrict__ ADinv) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const double* D = Ax + 9 * diagblk[v];
+    double a[9];    // symmetric scaled block from the upper triangle
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double raw = r <= c ? D[3 * c + r] : D[3 * r + c];
+            a[3 * c + r] = s[3 * v + r] * raw * s[3 * v + c];
+        }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        AD[9 * v + q] = (float)a[q];
+        AD64[9 * v + q] = a[q];
+    }
+    const double l0 = log(D[0]), l1 = log(D[4]), l2 = log(D[8]), lm = (l0 + l1 + l2) / 3.0;
+    float* x = X0 + 6 * v;
+    x[0] = (float)a[3];    // (0, 1)
+    x[1] = (float)a[6];    // (0, 2)
+    x[2] = (float)a[7];    // (1, 2)
+    x[3] = (float)(l0 - lm);
+    x[4] = (float)(l1 - lm);
+    x[5] = (float)(l2 - lm);
+    // inverse of the symmetric 3x3 block (block Jacobi)
+    const double c00 = a[4] * a[8] - a[7] * a[5], c01 = a[6] * a[5] - a[3] * a[8], c02 = a[3] * a[7] - a[6] * a[4];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    const double id = 1.0 / det;
+    double m[9];
+    m[0] = c00 * id;
+    m[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    m[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    m[3] = c01 * id;
+    m[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    m[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    m[6] = c02 * id;
+    m[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    m[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ADinv[9 * v + q] = (float)m[q];
+}
+
+// ---- factor -------------------------------------------------------------------------------
+__global__ void k_lp_build_L(const float* __restrict__ EF, const float* __restrict__ AD,
+                             const int32_t* __restrict__ ledge, const float* __restrict__ Dl,
+                             const float* __restrict__ Gm, int64_t nl, int64_t n, float* __restrict__ LE,
+                             float* __restrict__ LD, float* __restrict__ LDinv) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < nl) {
+        const int64_t e = ledge[t];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) LE[9 * t + q] = EF[9 * e + q] + Dl[9 * t + q];
+    }
+    if (t < n) {
+        const float* a = AD + 9 * t;
+        const float* g = Gm + 6 * t;
+        float l[9] = {expf(g[0]), a[1] + g[3], a[2] + g[4], 0.0f, expf(g[1]), a[5] + g[5], 0.0f, 0.0f, expf(g[2])};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) LD[9 * t + q] = l[q];
+        // inverse of the lower-triangular block
+        const float i00 = 1.0f / l[0], i11 = 1.0f / l[4], i22 = 1.0f / l[8];
+        const float i10 = -l[1] * i00 * i11;
+        const float i21 = -l[5] * i11 * i22;
+        const float i20 = -(l[2] * i00 + l[5] * i10) * i22;
+        float* o = LDinv + 9 * t;
+        o[0] = i00; o[1] = i10; o[2] = i20; o[3] = 0.0f; o[4] = i11; o[5] = i21; o[6] = 0.0f; o[7] = 0.0f; o[8] = i22;
+    }
+}
+
+__device__ __forceinline__ void mv3(const float* __restrict__ B, const float* __restrict__ x, float* y) {   // y += B x
+#pragma unroll
+    for (int r = 0; r < 3; ++r) y[r] += B[r] * x[0] + B[3 + r] * x[1] + B[6 + r] * x[2];
+}
+__device__ __forceinline__ void mtv3(const float* __restrict__ B, const float* __restrict__ x, float* y) {  // y += B^T x
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] += B[3 * c] * x[0] + B[3 * c + 1] * x[1] + B[3 * c + 2] * x[2];
+}
+
+// y = L^T z per probe: y_j = L_jj^T z_j + sum over the column's blocks L_q^T z_row(q)
+__global__ void k_lp_lt_mv(const float* __restrict__ LE, const float* __restrict__ LD,
+                           const int64_t* __restrict__ lcptr, const int32_t* __restrict__ lcol,
+                           const int32_t* __restrict__ ldst, const float* __restrict__ z, float* __restrict__ y,
+                           int64_t n) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float* zp = z + blockIdx.y * 3 * n;
+    float acc[3] = {0, 0, 0};
+    mtv3(LD + 9 * j, zp + 3 * j, acc);
+    for (int64_t q = lcptr[j]; q < lcptr[j + 1]; ++q) {
+        const int32_t b = lcol[q];
+        mtv3(LE + 9 * (int64_t)b, zp + 3 * (int64_t)ldst[b], acc);
+    }
+    float* yp = y + blockIdx.y * 3 * n + 3 * j;
+    yp[0] = acc[0]; yp[1] = acc[1]; yp[2] = acc[2];
+}
+
+// w = L y per probe
+__global__ void k_lp_l_mv(const float* __restrict__ LE, const float* __restrict__ LD,
+                          const int64_t* __restrict__ lrptr, const int32_t* __restrict__ lsrc,
+                          const float* __restrict__ y, float* __restrict__ w, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* yp = y + blockIdx.y * 3 * n;
+    float acc[3] = {0, 0, 0};
+    mv3(LD + 9 * i, yp + 3 * i, acc);
+    for (int64_t q = lrptr[i]; q < lrptr[i + 1]; ++q) mv3(LE + 9 * q, yp + 3 * (int64_t)lsrc[q], acc);
+    float* wp = w + blockIdx.y * 3 * n + 3 * i;
+    wp[0] = acc[0]; wp[1] = acc[1]; wp[2] = acc[2];
+}
+
+// a = A^ z per probe (FP32), and the FP64 product of the refinement
+__global__ void k_lp_a_mv(const float* __restrict__ EF, const float* __restrict__ AD, const int64_t* __restrict__ dptr,
+                          const int32_t* __restrict__ esrc, const float* __restrict__ z, float* __restrict__ a,
+                          int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* zp = z + blockIdx.y * 3 * n;
+    float acc[3] = {0, 0, 0};
+    mv3(AD + 9 * i, zp + 3 * i, acc);
+    for (int64_t e = dptr[i]; e < dptr[i + 1]; ++e) mv3(EF + 9 * e, zp + 3 * (int64_t)esrc[e], acc);
+    float* ap = a + blockIdx.y * 3 * n + 3 * i;
+    ap[0] = acc[0]; ap[1] = acc[1]; ap[2] = acc[2];
+}
+__global__ void k_lp_a_mv64(const double* __restrict__ EF, const double* __restrict__ AD,
+                            const int64_t* __restrict__ dptr, const int32_t* __restrict__ esrc,
+                            const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[3] = {0, 0, 0};
+    auto mv = [&](const double* B, const double* v) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) acc[r] += B[r] * v[0] + B[3 + r] * v[1] + B[6 + r] * v[2];
+    };
+    mv(AD + 9 * i, x + 3 * i);
+    for (int64_t e = dptr[i]; e < dptr[i + 1]; ++e) mv(EF + 9 * e, x + 3 * (int64_t)esrc[e]);
+    y[3 * i] = acc[0]; y[3 * i + 1] = acc[1]; y[3 * i + 2] = acc[2];
+}
+
+// per-probe partial sums of |w - a|^2 and |a|^2 (NPART chunks per probe)
+constexpr int NPART = 256;
+__global__ __launch_bounds__(256) void k_lp_loss_parts(const float* __restrict__ w, const float* __restrict__ a,
+                                                       int64_t m, double* __restrict__ part) {
+    __shared__ double s0[256], s1[256];
+    const float* wp = w + blockIdx.y * m;
+    const float* ap = a + blockIdx.y * m;
+    double x0 = 0, x1 = 0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)NPART * 256) {
+        const double d = (double)wp[i] - (double)ap[i];
+        x0 += d * d;
+        x1 += (double)ap[i] * ap[i];
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { s0[threadIdx.x] += s0[threadIdx.x + k]; s1[threadIdx.x] += s1[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (blockIdx.y * NPART + blockIdx.x)] = s0[0];
+        part[2 * (blockIdx.y * NPART + blockIdx.x) + 1] = s1[0];
+    }
+}
+// loss = mean_p num_p / den_p; coef_p = 2 / (np den_p)
+__global__ void k_lp_loss_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double loss = 0;
+    for (int p = 0; p < np; ++p) {
+        double a = 0, b = 0;
+        for (int q = 0; q < NPART; ++q) { a += part[2 * (p * NPART + q)]; b += part[2 * (p * NPART + q) + 1]; }
+        loss += a / b;
+        out[1 + p] = 2.0 / (np * b);
+    }
+    out[0] = loss / np;
+}
+__global__ void k_lp_dw(const float* __restrict__ w, const float* __restrict__ a, const double* __restrict__ coef,
+                        float* __restrict__ dw, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t k = blockIdx.y * m + i;
+    dw[k] = (float)(coef[1 + blockIdx.y] * ((double)w[k] - (double)a[k]));
+}
+
+// dL_q = sum_p dw_i y_j^T + z_i dy_j^T (q = (i, j)); dL_ii the same on the lower triangle
+__global__ void k_lp_dL(const float* __restrict__ dw, const float* __restrict__ y, const float* __restrict__ z,
+                        const float* __restrict__ dy, const int32_t* __restrict__ ldst,
+                        const int32_t* __restrict__ lsrc, int64_t nl, int64_t n, int np, float* __restrict__ dLE,
+                        float* __restrict__ dLD) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < nl) {
+        const int64_t i = ldst[t], j = lsrc[t];
+        float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int p = 0; p < np; ++p) {
+            const int64_t o = (int64_t)p * 3 * n;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    g[3 * c + r] += dw[o + 3 * i + r] * y[o + 3 * j + c] + z[o + 3 * i + r] * dy[o + 3 * j + c];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dLE[9 * t + q] = g[q];
+    }
+    if (t < n) {
+        float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int p = 0; p < np; ++p) {
+            const int64_t o = (int64_t)p * 3 * n;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = c; r < 3; ++r)
+                    g[3 * c + r] += dw[o + 3 * t + r] * y[o + 3 * t + c] + z[o + 3 * t + r] * dy[o + 3 * t + c];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dLD[9 * t + q] = g[q];
+    }
+}
+// diagonal decoder gradient: d g_c = dL_cc exp(g_c), then the three strict-lower entries
+__global__ void k_lp_dg(const float* __restrict__ dLD, const float* __restrict__ Gm, int64_t n, float* __restrict__ dG) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float* d = dLD + 9 * t;
+    const float* g = Gm + 6 * t;
+    float* o = dG + 6 * t;
+    o[0] = d[0] * expf(g[0]);
+    o[1] = d[4] * expf(g[1]);
+    o[2] = d[8] * expf(g[2]);
+    o[3] = d[1];
+    o[4] = d[2];
+    o[5] = d[5];
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---- preconditioned CG ---------------------------------------------------------------------
+// forward sweep of one color: t_i = L_ii^-1 (r_i - sum_q L_q t_src(q))
+__global__ void k_lp_trsv_fwd(const int32_t* __restrict__ nodes, int64_t cnt, const float* __restrict__ LE,
+                              const float* __restrict__ LDinv, const int64_t* __restrict__ lrptr,
+                              const int32_t* __restrict__ lsrc, const float* __restrict__ r, float* __restrict__ t) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const int64_t i = nodes[k];
+    float v[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+    float s[3] = {0, 0, 0};
+    for (int64_t q = lrptr[i]; q < lrptr[i + 1]; ++q) mv3(LE + 9 * q, t + 3 * (int64_t)lsrc[q], s);
+    v[0] -= s[0]; v[1] -= s[1]; v[2] -= s[2];
+    float o[3] = {0, 0, 0};
+    mv3(LDinv + 9 * i, v, o);
+    t[3 * i] = o[0]; t[3 * i + 1] = o[1]; t[3 * i + 2] = o[2];
+}
+// backward sweep of one color: z_j = L_jj^-T (t_j - sum over the column's blocks L_q^T z_row(q))
+__global__ void k_lp_trsv_bwd(const int32_t* __restrict__ nodes, int64_t cnt, const float* __restrict__ LE,
+                              const float* __restrict__ LDinv, const int64_t* __restrict__ lcptr,
+                              const int32_t* __restrict__ lcol, const int32_t* __restrict__ ldst,
+                              const float* __restrict__ t, float* __restrict__ z) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const int64_t j = nodes[k];
+    float v[3] = {t[3 * j], t[3 * j + 1], t[3 * j + 2]};
+    float s[3] = {0, 0, 0};
+    for (int64_t q = lcptr[j]; q < lcptr[j + 1]; ++q) {
+        const int32_t b = lcol[q];
+        mtv3(LE + 9 * (int64_t)b, z + 3 * (int64_t)ldst[b], s);
+    }
+    v[0] -= s[0]; v[1] -= s[1]; v[2] -= s[2];
+    float o[3] = {0, 0, 0};
+    mtv3(LDinv + 9 * j, v, o);
+    z[3 * j] = o[0]; z[3 * j + 1] = o[1]; z[3 * j + 2] = o[2];
+}
+__global__ void k_lp_jacobi(const float* __restrict__ ADinv, const float* __restrict__ r, float* __restrict__ z,
+                            int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float o[3] = {0, 0, 0};
+    mv3(ADinv + 9 * i, r + 3 * i, o);
+    z[3 * i] = o[0]; z[3 * i + 1] = o[1]; z[3 * i + 2] = o[2];
+}
+
+// deterministic dot products: NPART partials, then one thread sums them in order
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_parts(const T* __restrict__ a, const T* __restrict__ b, int64_t m,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[256];
+    double x = 0;
+    for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)NPART * 256) x += (double)a[i] * b[i];
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_dot_final(const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int q = 0; q < NPART; ++q) s += part[q];
+    *out = s;
+}
+// CG scalars on the device: sc = {rz, pq, rr, alpha, beta, rz_new}
+__global__ void k_cg_alpha(double* sc) { if (threadIdx.x == 0) sc[3] = sc[1] != 0.0 ? sc[0] / sc[1] : 0.0; }
+__global__ void k_cg_beta(double* sc) {
+    if (threadIdx.x == 0) { sc[4] = sc[0] != 0.0 ? sc[5] / sc[0] : 0.0; sc[0] = sc[5]; }
+}
+__global__ void k_cg_xr(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+                        const float* __restrict__ q, const double* __restrict__ sc, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float a = (float)sc[3];
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+}
+__global__ void k_cg_p(float* __restrict__ p, const float* __restrict__ z, const double* __restrict__ sc, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    p[i] = z[i] + (float)sc[4] * p[i];
+}
+// full-length b -> scaled free vector; scaled free vector -> full-length x (pins = 0)
+__global__ void k_lp_in(const double* __restrict__ b, const int64_t* __restrict__ scal, const double* __restrict__ s,
+                        double* __restrict__ out, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) out[i] = s[i] * b[scal[i]];
+}
+__global__ void k_lp_out(const double* __restrict__ xh, const int64_t* __restrict__ scal, const double* __restrict__ s,
+                         double* __restrict__ x, int64_t m) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) x[scal[i]] = s[i] * xh[i];
+}
+__global__ void k_zero_d(double* p, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_axpy_fd(double* __restrict__ x, const float* __restrict__ d, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)d[i];
+}
+__global__ void k_sub_d(double* __restrict__ r, const double* __restrict__ b, const double* __restrict__ y, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - y[i];
+}
+// |S^-1 r|^2 contributions (the residual of the unscaled system)
+__global__ void k_unscale(const double* __restrict__ r, const double* __restrict__ s, double* __restrict__ o, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) o[i] = r[i] / s[i];
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------------------------
+struct LearnedDev::Impl {
+    // graph
+    DeviceArray dptr, esrc, edst, eblk, diagblk, sptr, sedge, ledge, ldst, lsrc, lrptr, lcptr, lcol, order, scal;
+    // matrix-derived
+    DeviceArray s, X0, EF, EF64, AD, AD64, ADinv, bad;
+    // parameters
+    DeviceArray P, Pq, G, Am, Av;
+    // activations
+    DeviceArray Hs[NL + 1], Min[NL], Ms[NL], Mag[NL], Uin[NL], Us[NL], Din, D1, Dl, Gm, LE, LD, LDinv;
+    // loss / backward
+    DeviceArray z, y, w, a, dw, dy, lpart, lout, dLE, dLD, dDl, dG, dD1, dDin, dH, dU, dUin, dMs, dMin, wpart;
+    // solve
+    DeviceArray bh, xh, rh, yh, tmp64, r32, x32, p32, q32, z32, t32, dpart, sc;
+};
+
+LearnedDev::LearnedDev(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins, int64_t npins,
+                       uint64_t seed)
+    : g_(build_lp_graph(Ap, Ai, nb, pins, npins)), d_(std::make_unique<Impl>()) {
+    Impl& D = *d_;
+    D.dptr.upload(g_.dptr);
+    D.esrc.upload(g_.esrc);
+    D.edst.upload(g_.edst);
+    D.eblk.upload(g_.eblk);
+    D.diagblk.upload(g_.diagblk);
+    D.sptr.upload(g_.sptr);
+    D.sedge.upload(g_.sedge);
+    D.ledge.upload(g_.ledge);
+    D.ldst.upload(g_.ldst);
+    D.lsrc.upload(g_.lsrc);
+    D.lrptr.upload(g_.lrptr);
+    D.lcptr.upload(g_.lcptr);
+    D.lcol.upload(g_.lcol);
+    D.order.upload(g_.order);
+    D.scal.upload(g_.scal);
+    // parameter layout (floats): W0, b0, per round Wm (73 x 64), bm, Wu (128 x 64), bu, then the
+    // decoders Wd1 (128 x 64), bd1, Wd2 (64 x 9), bd2, Wdd (64 x 6), bdd
+    auto push = [&](int64_t len) { poff_.push_back(np_); np_ += len; };
+    push(FN * H); push(H);
+    for (int l = 0; l < NL; ++l) { push((H + FE) * H); push(H); push(2 * H * H); push(H); }
+    push(2 * H * H); push(H); push(H * 9); push(9); push(H * 6); push(6);
+    poff_.push_back(np_);
+    std::vector<float> p(np_, 0.0f);
+    uint64_t st = seed ^ 0x1ea41edULL;
+    auto uni = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    auto glorot = [&](int idx, int fin, int fout, double gain) {
+        const double a = gain * std::sqrt(6.0 / (fin + fout));
+        for (int64_t q = poff_[idx]; q < poff_[idx + 1]; ++q) p[q] = (float)(a * uni());
+    };
+    glorot(0, FN, H, 1.0);
+    for (int l = 0; l < NL; ++l) {
+        glorot(2 + 4 * l, H + FE, H, 1.0);
+        glorot(4 + 4 * l, 2 * H, H, 0.5);        // residual branch starts small
+    }
+    const int dbase = 2 + 4 * NL;
+    glorot(dbase, 2 * H, H, 1.0);
+    glorot(dbase + 2, H, 9, 0.01);               // output layers near zero: L starts at I + tril(A^)
+    glorot(dbase + 4, H, 6, 0.01);
+    D.P.upload(p);
+    D.Pq.alloc(np_ * sizeof(float));
+    D.G.alloc(np_ * sizeof(float));
+    std::vector<float> zero(np_, 0.0f);
+    D.Am.upload(zero);
+    D.Av.upload(zero);
+    D.bad.alloc(sizeof(int));
+    const int64_t n = g_.n, E = (int64_t)g_.esrc.size(), nl = (int64_t)g_.ledge.size();
+    D.s.alloc(3 * n * sizeof(double));
+    D.X0.alloc(n * FN * sizeof(float));
+    D.EF.alloc(std::max<int64_t>(E, 1) * 9 * sizeof(float));
+    D.EF64.alloc(std::max<int64_t>(E, 1) * 9 * sizeof(double));
+    D.AD.alloc(n * 9 * sizeof(float));
+    D.AD64.alloc(n * 9 * sizeof(double));
+    D.ADinv.alloc(n * 9 * sizeof(float));
+    for (int l = 0; l <= NL; ++l) D.Hs[l].alloc(n * H * sizeof(float));
+    for (int l = 0; l < NL; ++l) {
+        D.Min[l].alloc(std::max<int64_t>(E, 1) * (H + FE) * sizeof(float));
+        D.Ms[l].alloc(std::max<int64_t>(E, 1) * H * sizeof(float));
+        D.Mag[l].alloc(n * H * sizeof(float));
+        D.Uin[l].alloc(n * 2 * H * sizeof(float));
+        D.Us[l].alloc(n * H * sizeof(float));
+    }
+    const int64_t nl1 = std::max<int64_t>(nl, 1);
+    D.Din.alloc(nl1 * 2 * H * sizeof(float));
+    D.D1.alloc(nl1 * H * sizeof(float));
+    D.Dl.alloc(nl1 * 9 * sizeof(float));
+    D.Gm.alloc(n * 6 * sizeof(float));
+    D.LE.alloc(nl1 * 9 * sizeof(float));
+    D.LD.alloc(n * 9 * sizeof(float));
+    D.LDinv.alloc(n * 9 * sizeof(float));
+    D.dLE.alloc(nl1 * 9 * sizeof(float));
+    D.dLD.alloc(n * 9 * sizeof(float));
+    D.dG.alloc(n * 6 * sizeof(float));
+    D.dD1.alloc(nl1 * H * sizeof(float));
+    D.dDin.alloc(nl1 * 2 * H * sizeof(float));
+    D.dH.alloc(n * H * sizeof(float));
+    D.dU.alloc(n * H * sizeof(float));
+    D.dUin.alloc(n * 2 * H * sizeof(float));
+    D.dMs.alloc(std::max<int64_t>(E, 1) * H * sizeof(float));
+    D.dMin.alloc(std::max<int64_t>(E, 1) * H * sizeof(float));
+    const int64_t rows = std::max<int64_t>(std::max<int64_t>(E, nl), n);
+    D.wpart.alloc(((rows + WGROWS - 1) / WGROWS) * 129 * 64 * sizeof(float));
+    D.lout.alloc(sizeof(double) * 65);
+    D.dpart.alloc(sizeof(double) * NPART);
+    D.sc.alloc(sizeof(double) * 8);
+}
+
+LearnedDev::~LearnedDev() = default;
+
+void LearnedDev::set_matrix(const double* Ax, void* stream) {
+    Impl& D = *d_;
+    hipStream_t s = SS(stream);
+    const int64_t n = g_.n, E = (int64_t)g_.esrc.size();
+    int big = INT_MAX;
+    LP_CHECK(hipMemcpyAsync(D.bad.p, &big, sizeof(int), hipMemcpyHostToDevice, s));
+    k_lp_scale<<<nb256(n), 256, 0, s>>>(Ax, D.diagblk.as<int64_t>(), n, D.s.as<double>(), D.bad.as<int>());
+    LP_LAUNCH();
+    if (E) {
+        k_lp_edges<<<nb256(E), 256, 0, s>>>(Ax, D.eblk.as<int64_t>(), D.edst.as<int32_t>(), D.esrc.as<int32_t>(),
+                                            D.s.as<double>(), E, D.EF.as<float>(), D.EF64.as<double>());
+        LP_LAUNCH();
+    }
+    k_lp_nodes<<<nb256(n), 256, 0, s>>>(Ax, D.diagblk.as<int64_t>(), D.s.as<double>(), n, D.X0.as<float>(),
+                                        D.AD.as<float>(), D.AD64.as<double>(), D.ADinv.as<float>());
+    LP_LAUNCH();
+    int st = 0;
+    LP_CHECK(hipMemcpyAsync(&st, D.bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    LP_CHECK(hipStreamSynchronize(s));
+    if (st != INT_MAX) throw Error(-3, "non-positive diagonal entry at free scalar " + std::to_string(st));
+}
+
+void LearnedDev::dense(const float* X, int ldx, int64_t N, int K, const float* W, int ldw, bool wt, const float* b,
+                       float* Y, int ldy, int O, bool relu, void* stream) {
+    if (N <= 0) return;
+    if (K > 128 || O > 64) throw Error(-2, "internal: dense layer shape");
+    k_dense<<<(unsigned)((N + 63) / 64), 256, 0, SS(stream)>>>(X, ldx, N, K, W, ldw, wt ? 1 : 0, b, Y, ldy, O,
+                                                              relu ? 1 : 0, 0);
+    LP_LAUNCH();
+}
+
+void LearnedDev::wgrad(const float* X, int ldx, const float* Z, int ldz, int64_t N, int K, int O, float* gW,
+                       float* gb, void* stream) {
+    if (N <= 0) return;
+    const int nparts = (int)((N + WGROWS - 1) / WGROWS);
+    k_wgrad<<<nparts, 256, 0, SS(stream)>>>(X, ldx, Z, ldz, N, K, O, d_->wpart.as<float>());
+    LP_LAUNCH();
+    k_wgrad_reduce<<<nb256((K + 1) * 64), 256, 0, SS(stream)>>>(d_->wpart.as<float>(), nparts, K, O, gW, gb);
+    LP_LAUNCH();
+}
+
+void LearnedDev::forward(void* stream) {
+    Impl& D = *d_;
+    hipStream_t s = SS(stream);
+    const int64_t n = g_.n, E = (int64_t)g_.esrc.size(), nl = (int64_t)g_.ledge.size();
+    k_round_bf16<<<nb256(np_), 256, 0, s>>>(D.P.as<float>(), D.Pq.as<float>(), np_);
+    LP_LAUNCH();
+    const float* W = D.Pq.as<float>();
+    auto P = [&](int idx) { return W + poff_[idx]; };
+    dense(D.X0.as<float>(), FN, n, FN, P(0), H, false, P(1), D.Hs[0].as<float>(), H, H, true, stream);
+    for (int l = 0; l < NL; ++l) {
+        const int b = 2 + 4 * l;
+        float* Min = D.Min[l].as<float>();
+        if (E) {
+            k_gather_rows<<<nb256(E * H), 256, 0, s>>>(Min, H + FE, 0, D.Hs[l].as<float>(), H, D.esrc.as<int32_t>(), E, H);
+            LP_LAUNCH();
+            k_gather_rows<<<nb256(E * FE), 256, 0, s>>>(Min, H + FE, H, D.EF.as<float>(), FE, nullptr, E, FE);
+            LP_LAUNCH();
+        }
+        dense(Min, H + FE, E, H + FE, P(b), H, false, P(b + 1), D.Ms[l].as<float>(), H, H, true, stream);
+        k_segsum<<<nb256(n * H), 256, 0, s>>>(D.Mag[l].as<float>(), H, 0, D.Ms[l].as<float>(), H, 0, H,
+                                              D.dptr.as<int64_t>(), nullptr, n, 0);
+        LP_LAUNCH();
+        float* Uin = D.Uin[l].as<float>();
+        k_gather_rows<<<nb256(n * H), 256, 0, s>>>(Uin, 2 * H, 0, D.Hs[l].as<float>(), H, nullptr, n, H);
+        LP_LAUNCH();
+        k_gather_rows<<<nb256(n * H), 256, 0, s>>>(Uin, 2 * H, H, D.Mag[l].as<float>(), H, nullptr, n, H);
+        LP_LAUNCH();
+        dense(Uin, 2 * H, n, 2 * H, P(b + 2), H, false, P(b + 3), D.Us[l].as<float>(), H, H, true, stream);
+        k_add<<<nb256(n * H), 256, 0, s>>>(D.Hs[l + 1].as<float>(), D.Hs[l].as<float>(), D.Us[l].as<float>(), n * H);
+        LP_LAUNCH();
+    }
+    const int db = 2 + 4 * NL;
+    const float* h = D.Hs[NL].as<float>();
+    if (nl) {
+        k_gather_rows<<<nb256(nl * H), 256, 0, s>>>(D.Din.as<float>(), 2 * H, 0, h, H, D.ldst.as<int32_t>(), nl, H);
+        LP_LAUNCH();
+        k_gather_rows<<<nb256(nl * H), 256, 0, s>>>(D.Din.as<float>(), 2 * H, H, h, H, D.lsrc.as<int32_t>(), nl, H);
+        LP_LAUNCH();
+    }
+    dense(D.Din.as<float>(), 2 * H, nl, 2 * H, P(db), H, false, P(db + 1), D.D1.as<float>(), H, H, true, stream);
+    dense(D.D1.as<float>(), H, nl, H, P(db + 2), 9, false, P(db + 3), D.Dl.as<float>(), 9, 9, false, stream);
+    dense(h, H, n, H, P(db + 4), 6, false, P(db + 5), D.Gm.as<float>(), 6, 6, false, stream);
+    k_lp_build_L<<<nb256(std::max(nl, n)), 256, 0, s>>>(D.EF.as<float>(), D.AD.as<float>(), D.ledge.as<int32_t>(),
+                                                       D.Dl.as<float>(), D.Gm.as<float>(), nl, n, D.LE.as<float>(),
+                                                       D.LD.as<float>(), D.LDinv.as<float>());
+    LP_LAUNCH();
+}
+
+void LearnedDev::build_factor(void* stream) { forward(stream); }
+
+double LearnedDev::loss_grad(const float* zh, int nprobe, bool bwd, void* stream) {
+    Impl& D = *d_;
+    hipStream_t s = SS(stream);
+    if (nprobe < 1 || nprobe > 64) throw Error(-1, "1 to 64 probes");
+    const int64_t n = g_.n, m = 3 * n, nl = (int64_t)g_.ledge.size();
+    const size_t vb = (size_t)nprobe * m * sizeof(float);
+    for (DeviceArray* a : {&D.z, &D.y, &D.w, &D.a, &D.dw, &D.dy}) a->alloc(vb);
+    D.lpart.alloc((size_t)2 * NPART * nprobe * sizeof(double));
+    LP_CHECK(hipMemcpyAsync(D.z.p, zh, vb, hipMemcpyHostToDevice, s));
+    nprobe_ = nprobe;
+    forward(stream);
+    dim3 gv(nb256(n), (unsigned)nprobe), gm(nb256(m), (unsigned)nprobe);
+    k_lp_lt_mv<<<gv, 256, 0, s>>>(D.LE.as<float>(), D.LD.as<float>(), D.lcptr.as<int64_t>(), D.lcol.as<int32_t>(),
+                                  D.ldst.as<int32_t>(), D.z.as<float>(), D.y.as<float>(), n);
+    LP_LAUNCH();
+    k_lp_l_mv<<<gv, 256, 0, s>>>(D.LE.as<float>(), D.LD.as<float>(), D.lrptr.as<int64_t>(), D.lsrc.as<int32_t>(),
+                                 D.y.as<float>(), D.w.as<float>(), n);
+    LP_LAUNCH();
+    k_lp_a_mv<<<gv, 256, 0, s>>>(D.EF.as<float>(), D.AD.as<float>(), D.dptr.as<int64_t>(), D.esrc.as<int32_t>(),
+                                 D.z.as<float>(), D.a.as<float>(), n);
+    LP_LAUNCH();
+    k_lp_loss_parts<<<dim3(NPART, (unsigned)nprobe), 256, 0, s>>>(D.w.as<float>(), D.a.as<float>(), m,
+                                                                    D.lpart.as<double>());
+    LP_LAUNCH();
+    k_lp_loss_final<<<1, 64, 0, s>>>(D.lpart.as<double>(), nprobe, D.lout.as<double>());
+    LP_LAUNCH();
+    double loss = 0;
+    if (bwd) {
+        k_lp_dw<<<gm, 256, 0, s>>>(D.w.as<float>(), D.a.as<float>(), D.lout.as<double>(), D.dw.as<float>(), m);
+        LP_LAUNCH();
+        k_lp_lt_mv<<<gv, 256, 0, s>>>(D.LE.as<float>(), D.LD.as<float>(), D.lcptr.as<int64_t>(), D.lcol.as<int32_t>(),
+                                      D.ldst.as<int32_t>(), D.dw.as<float>(), D.dy.as<float>(), n);
+        LP_LAUNCH();
+        k_lp_dL<<<nb256(std::max(nl, n)), 256, 0, s>>>(D.dw.as<float>(), D.y.as<float>(), D.z.as<float>(),
+                                                      D.dy.as<float>(), D.ldst.as<int32_t>(), D.lsrc.as<int32_t>(),
+                                                      nl, n, nprobe, D.dLE.as<float>(), D.dLD.as<float>());
+        LP_LAUNCH();
+        backward(stream);
+    }
+    LP_CHECK(hipMemcpyAsync(&loss, D.lout.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    LP_CHECK(hipStreamSynchronize(s));
+    return loss;
+}
+
+void LearnedDev::backward(void* stream) {
+    Impl& D = *d_;
+    hipStream_t s = SS(stream);
+    const int64_t n = g_.n, E = (int64_t)g_.esrc.size(), nl = (int64_t)g_.ledge.size();
+    float* G = D.G.as<float>();
+    const float* W = D.Pq.as<float>();
+    auto P = [&](int idx) { return W + poff_[idx]; };
+    auto Gp = [&](int idx) { return G + poff_[idx]; };
+    k_fill_f<<<std::min<unsigned>(nb256(np_), 4096), 256, 0, s>>>(G, np_, 0.0f);
+    LP_LAUNCH();
+    const int db = 2 + 4 * NL;
+    // decoders
+    const float* h = D.Hs[NL].as<float>();
+    k_lp_dg<<<nb256(n), 256, 0, s>>>(D.dLD.as<float>(), D.Gm.as<float>(), n, D.dG.as<float>());
+    LP_LAUNCH();
+    wgrad(D.D1.as<float>(), H, D.dLE.as<float>(), 9, nl, H, 9, Gp(db + 2), Gp(db + 3), stream);
+    dense(D.dLE.as<float>(), 9, nl, 9, P(db + 2), 9, true, nullptr, D.dD1.as<float>(), H, H, false, stream);
+    k_mask<<<nb256(nl * H), 256, 0, s>>>(D.dD1.as<float>(), D.dD1.as<float>(), D.D1.as<float>(), nl * H);
+    LP_LAUNCH();
+    wgrad(D.Din.as<float>(), 2 * H, D.dD1.as<float>(), H, nl, 2 * H, H, Gp(db), Gp(db + 1), stream);
+    for (int c0 = 0; c0 < 2 * H; c0 += H)
+        dense(D.dD1.as<float>(), H, nl, H, P(db) + (int64_t)c0 * H, H, true, nullptr, D.dDin.as<float>() + c0, 2 * H,
+              H, false, stream);
+    wgrad(h, H, D.dG.as<float>(), 6, n, H, 6, Gp(db + 4), Gp(db + 5), stream);
+    dense(D.dG.as<float>(), 6, n, 6, P(db + 4), 6, true, nullptr, D.dH.as<float>(), H, H, false, stream);
+    k_segsum<<<nb256(n * H), 256, 0, s>>>(D.dH.as<float>(), H, 0, D.dDin.as<float>(), 2 * H, 0, H,
+                                          D.lrptr.as<int64_t>(), nullptr, n, 1);
+    LP_LAUNCH();
+    k_segsum<<<nb256(n * H), 256, 0, s>>>(D.dH.as<float>(), H, 0, D.dDin.as<float>(), 2 * H, H, H,
+                                          D.lcptr.as<int64_t>(), D.lcol.as<int32_t>(), n, 1);
+    LP_LAUNCH();
+    // rounds, last first; dH holds the gradient of h after the round
+    for (int l = NL - 1; l >= 0; --l) {
+        const int b = 2 + 4 * l;
+        k_mask<<<nb256(n * H), 256, 0, s>>>(D.dU.as<float>(), D.dH.as<float>(), D.Us[l].as<float>(), n * H);
+        LP_LAUNCH();
+        wgrad(D.Uin[l].as<float>(), 2 * H, D.dU.as<float>(), H, n, 2 * H, H, Gp(b + 2), Gp(b + 3), stream);
+        for (int c0 = 0; c0 < 2 * H; c0 += H)
+            dense(D.dU.as<float>(), H, n, H, P(b + 2) + (int64_t)c0 * H, H, true, nullptr, D.dUin.as<float>() + c0,
+                  2 * H, H, false, stream);
+        k_add_cols<<<nb256(n * H), 256, 0, s>>>(D.dH.as<float>(), D.dUin.as<float>(), 2 * H, 0, n);
+        LP_LAUNCH();
+        if (E) {
+            k_gather_mask<<<nb256(E * H), 256, 0, s>>>(D.dMs.as<float>(), D.dUin.as<float>() + H, 2 * H,
+                                                       D.edst.as<int32_t>(), D.Ms[l].as<float>(), E, H);
+            LP_LAUNCH();
+        }
+        wgrad(D.Min[l].as<float>(), H + FE, D.dMs.as<float>(), H, E, H + FE, H, Gp(b), Gp(b + 1), stream);
+        dense(D.dMs.as<float>(), H, E, H, P(b), H, true, nullptr, D.dMin.as<float>(), H, H, false, stream);
+        k_segsum<<<nb256(n * H), 256, 0, s>>>(D.dH.as<float>(), H, 0, D.dMin.as<float>(), H, 0, H,
+                                              D.sptr.as<int64_t>(), D.sedge.as<int32_t>(), n, 1);
+        LP_LAUNCH();
+    }
+    k_mask<<<nb256(n * H), 256, 0, s>>>(D.dU.as<float>(), D.dH.as<float>(), D.Hs[0].as<float>(), n * H);
+    LP_LAUNCH();
+    wgrad(D.X0.as<float>(), FN, D.dU.as<float>(), H, n, FN, H, Gp(0), Gp(1), stream);
+}
+
+void LearnedDev::adam_step(float lr, void* stream) {
+    Impl& D = *d_;
+    ++t_;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)t_), c2 = 1.0f - std::pow(b2, (float)t_);
+    k_adam<<<nb256(np_), 256, 0, SS(stream)>>>(D.P.as<float>(), D.G.as<float>(), D.Am.as<float>(), D.Av.as<float>(),
+                                               np_, lr, b1, b2, 1e-8f, c1, c2);
+    LP_LAUNCH();
+}
+
+bool LearnedDev::solve(const double* Ax, const double* b, double* x, int precond, double tol, int max_pcg,
+                       int* pcg_iters, int* outer_iters, double* relres, void* stream) {
+    Impl& D = *d_;
+    hipStream_t s = SS(stream);
+    if (precond != 0 && precond != 1) throw Error(-1, "preconditioner 0 (block Jacobi) or 1 (learned factor)");
+    const int64_t n = g_.n, m = 3 * n;
+    set_matrix(Ax, stream);
+    if (precond == 1) forward(stream);
+    for (DeviceArray* a : {&D.bh, &D.xh, &D.rh, &D.yh, &D.tmp64}) a->alloc(m * sizeof(double));
+    for (DeviceArray* a : {&D.r32, &D.x32, &D.p32, &D.q32, &D.z32, &D.t32}) a->alloc(m * sizeof(float));
+    double* sc = D.sc.as<double>();
+    auto dot = [&](const auto* u, const auto* v, double* out) {
+        using T = std::remove_cv_t<std::remove_pointer_t<decltype(u)>>;
+        k_dot_parts<T><<<NPART, 256, 0, s>>>(u, v, m, D.dpart.as<double>());
+        LP_LAUNCH();
+        k_dot_final<<<1, 64, 0, s>>>(D.dpart.as<double>(), out);
+        LP_LAUNCH();
+    };
+    auto fetch = [&](const double* p) {
+        double h = 0;
+        LP_CHECK(hipMemcpyAsync(&h, p, sizeof(double), hipMemcpyDeviceToHost, s));
+        LP_CHECK(hipStreamSynchronize(s));
+        return h;
+    };
+    auto precondition = [&](const float* r, float* z) {
+        if (precond == 0) {
+            k_lp_jacobi<<<nb256(n), 256, 0, s>>>(D.ADinv.as<float>(), r, z, n);
+            LP_LAUNCH();
+            return;
+        }
+        const int32_t* ord = D.order.as<int32_t>();
+        for (int c = 0; c < g_.ncolors; ++c) {
+            const int64_t cnt = g_.cptr[c + 1] - g_.cptr[c];
+            k_lp_trsv_fwd<<<nb256(cnt), 256, 0, s>>>(ord + g_.cptr[c], cnt, D.LE.as<float>(), D.LDinv.as<float>(),
+                                                     D.lrptr.as<int64_t>(), D.lsrc.as<int32_t>(), r, D.t32.as<float>());
+            LP_LAUNCH();
+        }
+        for (int c = g_.ncolors - 1; c >= 0; --c) {
+            const int64_t cnt = g_.cptr[c + 1] - g_.cptr[c];
+            k_lp_trsv_bwd<<<nb256(cnt), 256, 0, s>>>(ord + g_.cptr[c], cnt, D.LE.as<float>(), D.LDinv.as<float>(),
+                                                     D.lcptr.as<int64_t>(), D.lcol.as<int32_t>(), D.ldst.as<int32_t>(),
+                                                     D.t32.as<float>(), z);
+            LP_LAUNCH();
+        }
+    };
+    // b^ = S b (free rows), |b| of the unscaled system for the stopping test
+    k_lp_in<<<nb256(m), 256, 0, s>>>(b, D.scal.as<int64_t>(), D.s.as<double>(), D.bh.as<double>(), m);
+    LP_LAUNCH();
+    k_unscale<<<nb256(m), 256, 0, s>>>(D.bh.as<double>(), D.s.as<double>(), D.tmp64.as<double>(), m);
+    LP_LAUNCH();
+    dot(D.tmp64.as<double>(), D.tmp64.as<double>(), sc + 6);
+    const double bnorm = std::sqrt(fetch(sc + 6));
+    k_zero_d<<<nb256(m), 256, 0, s>>>(D.xh.as<double>(), m);
+    LP_LAUNCH();
+    LP_CHECK(hipMemcpyAsync(D.rh.p, D.bh.p, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int total = 0, outer = 0;
+    double rel = 1.0;
+    bool ok = false;
+    const double inner_tol = 1e-6;
+    while (true) {
+        k_unscale<<<nb256(m), 256, 0, s>>>(D.rh.as<double>(), D.s.as<double>(), D.tmp64.as<double>(), m);
+        LP_LAUNCH();
+        dot(D.tmp64.as<double>(), D.tmp64.as<double>(), sc + 6);
+        rel = bnorm > 0 ? std::sqrt(fetch(sc + 6)) / bnorm : 0.0;
+        if (rel <= tol) { ok = true; break; }
+        if (total >= max_pcg) break;
+        // FP32 PCG on A^ d = r^ to inner_tol (relative to |r^|)
+        float* r = D.r32.as<float>();
+        float* xx = D.x32.as<float>();
+        float* p = D.p32.as<float>();
+        float* q = D.q32.as<float>();
+        float* z = D.z32.as<float>();
+        k_d2f<<<nb256(m), 256, 0, s>>>(D.rh.as<double>(), r, m);
+        LP_LAUNCH();
+        k_fill_f<<<std::min<unsigned>(nb256(m), 4096), 256, 0, s>>>(xx, m, 0.0f);
+        LP_LAUNCH();
+        dot(r, r, sc + 2);
+        const double r0 = std::sqrt(fetch(sc + 2));
+        precondition(r, z);
+        LP_CHECK(hipMemcpyAsync(p, z, m * sizeof(float), hipMemcpyDeviceToDevice, s));
+        dot(r, z, sc + 0);
+        while (total < max_pcg) {
+            k_lp_a_mv<<<nb256(n), 256, 0, s>>>(D.EF.as<float>(), D.AD.as<float>(), D.dptr.as<int64_t>(),
+                                               D.esrc.as<int32_t>(), p, q, n);
+            LP_LAUNCH();
+            dot(p, q, sc + 1);
+            k_cg_alpha<<<1, 64, 0, s>>>(sc);
+            LP_LAUNCH();
+            k_cg_xr<<<nb256(m), 256, 0, s>>>(xx, r, p, q, sc, m);
+            LP_LAUNCH();
+            ++total;
+            dot(r, r, sc + 2);
+            if (std::sqrt(fetch(sc + 2)) <= inner_tol * r0) break;
+            precondition(r, z);
+            dot(r, z, sc + 5);
+            k_cg_beta<<<1, 64, 0, s>>>(sc);
+            LP_LAUNCH();
+            k_cg_p<<<nb256(m), 256, 0, s>>>(p, z, sc, m);
+            LP_LAUNCH();
+        }
+        // x^ += d; r^ = b^ - A^ x^ in FP64
+        k_axpy_fd<<<nb256(m), 256, 0, s>>>(D.xh.as<double>(), xx, m);
+        LP_LAUNCH();
+        k_lp_a_mv64<<<nb256(n), 256, 0, s>>>(D.EF64.as<double>(), D.AD64.as<double>(), D.dptr.as<int64_t>(),
+                                             D.esrc.as<int32_t>(), D.xh.as<double>(), D.yh.as<double>(), n);
+        LP_LAUNCH();
+        k_sub_d<<<nb256(m), 256, 0, s>>>(D.rh.as<double>(), D.bh.as<double>(), D.yh.as<double>(), m);
+        LP_LAUNCH();
+        ++outer;
+    }
+    // x = S x^ on the free rows, 0 on the pinned ones
+    k_zero_d<<<nb256(3 * g_.nb_full), 256, 0, s>>>(x, 3 * g_.nb_full);
+    LP_LAUNCH();
+    k_lp_out<<<nb256(m), 256, 0, s>>>(D.xh.as<double>(), D.scal.as<int64_t>(), D.s.as<double>(), x, m);
+    LP_LAUNCH();
+    LP_CHECK(hipStreamSynchronize(s));
+    if (pcg_iters) *pcg_iters = total;
+    if (outer_iters) *outer_iters = outer;
+    if (relres) *relres = rel;
+    return ok;
+}
+
+void LearnedDev::get_params(float* h) const {
+    LP_CHECK(hipMemcpy(h, d_->P.p, np_ * sizeof(float), hipMemcpyDeviceToHost));
+}
+void LearnedDev::set_params(const float* h) {
+    LP_CHECK(hipMemcpy(d_->P.p, h, np_ * sizeof(float), hipMemcpyHostToDevice));
+}
+void LearnedDev::get_grads(float* h) const {
+    LP_CHECK(hipDeviceSynchronize());
+    LP_CHECK(hipMemcpy(h, d_->G.p, np_ * sizeof(float), hipMemcpyDeviceToHost));
+}
+void LearnedDev::export_features(float* nf, float* ef, float* diag, double* scale) const {
+    LP_CHECK(hipDeviceSynchronize());
+    const int64_t n = g_.n, E = (int64_t)g_.esrc.size();
+    if (nf) LP_CHECK(hipMemcpy(nf, d_->X0.p, n * FN * sizeof(float), hipMemcpyDeviceToHost));
+    if (ef && E) LP_CHECK(hipMemcpy(ef, d_->EF.p, E * 9 * sizeof(float), hipMemcpyDeviceToHost));
+    if (diag) LP_CHECK(hipMemcpy(diag, d_->AD.p, n * 9 * sizeof(float), hipMemcpyDeviceToHost));
+    if (scale) LP_CHECK(hipMemcpy(scale, d_->s.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+}
+void LearnedDev::export_factor(float* LE, float* LD) const {
+    LP_CHECK(hipDeviceSynchronize());
+    const int64_t n = g_.n, nl = (int64_t)g_.ledge.size();
+    if (LE && nl) LP_CHECK(hipMemcpy(LE, d_->LE.p, nl * 9 * sizeof(float), hipMemcpyDeviceToHost));
+    if (LD) LP_CHECK(hipMemcpy(LD, d_->LD.p, n * 9 * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+} // namespace mfs

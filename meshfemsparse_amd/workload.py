@@ -75,3 +75,15 @@ def batch_perturb(batch, seed, sigma_rel=1e-3):
     for m, n in enumerate(batch["n"]):
         X[no[m]:no[m + 1]] = perturb(int(n), Xr[no[m]:no[m + 1]], seed * 1000003 + m, sigma_rel)
     return X
+
+
+def grf_log_E(X_rest, tets, seed, mean=np.log(1e5), std=1.5, corr_len=0.1, features=256):
+    """Per-element Young's modulus E = exp(g) with g a Gaussian random field at the element
+    centroids: mean `mean`, standard deviation `std`, squared-exponential correlation of length
+    `corr_len`, sampled by random Fourier features (omega ~ N(0, I / corr_len^2))."""
+    rng = np.random.default_rng(seed)
+    c = X_rest[tets].mean(axis=1)                         # centroids (ne, 3)
+    om = rng.standard_normal((features, 3)) / corr_len
+    ph = rng.uniform(0.0, 2.0 * np.pi, features)
+    g = np.sqrt(2.0 / features) * np.cos(c @ om.T + ph).sum(axis=1)
+    return np.exp(mean + std * g)
