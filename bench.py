@@ -177,6 +177,105 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Learned block-IC preconditioner: train the message-passing network on the device on
+    `--train` generated Hessians of the 48^3 cube (log E a Gaussian random field, mean ln 1e5,
+    std 1.5, correlation length 0.1; nu 0.3; sigma = 1e-3 h), then solve held-out 64^3 steps by
+    FP32 PCG + FP64 refinement to 1e-10 with the trained factor, falling back to the supernodal
+    Cholesky after 200 CG iterations; block-Jacobi PCG and the untrained network are the
+    baselines.  One step = one held-out 64^3 solve."""
+    import torch
+    import meshfemsparse_amd as mfs
+    from meshfemsparse_amd import workload as wl, learned
+    dev = torch.device("cuda:0")
+
+    def setup(N):
+        tets, Xr = wl.cube_mesh(N)
+        asm = mfs.SystemAssembler(len(Xr))
+        H = asm.blockSparsityPattern(tets)
+        H.setZero()
+        da = mfs.DeviceAssembler(H, tets)
+        return dict(N=N, tets=tets, Xr=Xr, H=H, da=da, pins=wl.z0_pins(N),
+                    Xr_d=torch.from_numpy(Xr).to(dev), tets_d=torch.from_numpy(tets).to(dev))
+
+    def sample(m, seed):
+        """assemble the Hessian of mesh m for (perturbation, stiffness field) seed; returns -grad"""
+        X = torch.from_numpy(wl.perturb(m["N"], m["Xr"], seed)).to(dev)
+        Ee = wl.grf_log_E(m["Xr_d"], m["tets_d"], seed).contiguous()
+        He, g12 = m["da"].neohookean(m["Xr_d"], X, E_per_elem=Ee, nu=0.3)
+        m["H"].Ax.zero_()
+        m["da"].assemble(He, m["H"].Ax)
+        g = torch.zeros(3 * len(m["Xr"]), dtype=torch.float64, device=dev)
+        m["da"].assembleGradient(g12, g)
+        return -g
+
+    tr = setup(args.learn_n)
+    lp = learned.LearnedPreconditioner(tr["H"], tr["pins"], seed=1)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    losses = []
+    for k in range(args.train):
+        sample(tr, 10000 + k)
+        losses.append(lp.trainStep(tr["H"], nprobe=4, lr=args.lr if k < args.train // 2 else args.lr / 4, rng=rng))
+        if k % 100 == 0:
+            log(f"[bench] train step {k}: loss {losses[-1]:.4f} ({time.perf_counter() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    params = lp.params()
+    del lp, tr
+    ev = setup(args.n)
+    lpe = learned.LearnedPreconditioner(ev["H"], ev["pins"], seed=1)
+    f = mfs.make_cholesky_factorizer()
+    f.factorizeSymbolic(ev["H"], ev["pins"])
+    untrained = lpe.params()
+    res = {"trained": [], "untrained": [], "jacobi": []}
+    times = []
+    for k in range(args.warmup + args.steps):
+        b = sample(ev, 20000 + k)
+        for name, prm, pc in (("jacobi", None, "jacobi"), ("untrained", untrained, "learned"),
+                              ("trained", params, "learned")):
+            if prm is not None:
+                lpe.setParams(prm)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            x, it, outer, rr, fb = lpe.solve(ev["H"], b, precond=pc, tol=1e-10, maxPCG=args.max_pcg, fallback=f)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t1
+            if k >= args.warmup:
+                res[name].append(dict(cg_iters=it, refinements=outer, relres=rr, fallback=fb, s=dt))
+                if name == "trained":
+                    times.append(dt)
+        log(f"[bench] held-out step {k}: " + ", ".join(f"{n}: {r[-1]['cg_iters'] if r else '-'} it"
+                                                      for n, r in res.items()))
+    # CG iterations without the fallback limit, to report what each preconditioner needs
+    b = sample(ev, 20000)
+    full = {}
+    for name, prm, pc in (("jacobi", None, "jacobi"), ("trained", params, "learned")):
+        if prm is not None:
+            lpe.setParams(prm)
+        try:
+            _, it, outer, rr, fb = lpe.solve(ev["H"], b, precond=pc, tol=1e-10, maxPCG=args.max_pcg_free)
+            full[name] = dict(cg_iters=it, refinements=outer, relres=rr)
+        except mfs.MFSError:
+            full[name] = dict(cg_iters=None, note=f"not converged in {args.max_pcg_free} CG iterations")
+    mean = lambda name, key: float(np.mean([r[key] for r in res[name]]))
+    out = {
+        "metric": "learned-preconditioner Newton linear-solve s/step (64^3 held-out, <=1e-10 rel. residual)",
+        "value": float(np.mean(times)), "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(np.mean(times)) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 PCG (bf16-rounded network weights), f64 refinement",
+        "data": "synthetic (GRF stiffness fields, seeded perturbations; device element kernel)",
+        "config": {"workload": f"train {args.train} Hessians of the {args.learn_n}^3 cube, evaluate held-out "
+                               f"{args.n}^3 steps; FP32 PCG + FP64 refinement to 1e-10, Cholesky fallback after "
+                               f"{args.max_pcg} CG iterations", "network": lpe.info()},
+        "train_s": t_train, "train_loss_first": float(np.mean(losses[:20])), "train_loss_last": float(np.mean(losses[-20:])),
+        "held_out": {n: dict(cg_iters_mean=mean(n, "cg_iters"), fallback_rate=mean(n, "fallback"), s_mean=mean(n, "s"))
+                     for n in res},
+        "cg_iters_without_limit": full,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,8 +293,14 @@ def main():
                     help="per-element E log-uniform [1e3, 1e7], nu uniform [0.2, 0.45]")
     ap.add_argument("--forward-error", action="store_true",
                     help="also factor in FP64 and report the forward error of every step (fp32)")
-    ap.add_argument("--workload", choices=["cube", "batch"], default="cube",
-                    help="cube: the 64^3 Newton step (headline); batch: 512 uneven meshes in one system")
+    ap.add_argument("--workload", choices=["cube", "batch", "learned"], default="cube",
+                    help="cube: the 64^3 Newton step (headline); batch: 512 uneven meshes in one system; "
+                         "learned: train the learned preconditioner, evaluate held-out solves")
+    ap.add_argument("--train", type=int, default=2000, help="learned: training Hessians")
+    ap.add_argument("--learn-n", type=int, default=48, help="learned: cube side of the training meshes")
+    ap.add_argument("--lr", type=float, default=2e-3, help="learned: Adam learning rate")
+    ap.add_argument("--max-pcg", type=int, default=200, help="learned: CG iterations before the Cholesky fallback")
+    ap.add_argument("--max-pcg-free", type=int, default=5000, help="learned: CG limit of the unlimited comparison")
     ap.add_argument("--batch", type=int, default=512, help="number of meshes of --workload batch")
     ap.add_argument("--no-graph", action="store_true", help="batch: launch the factorization without a hipGraph")
     args = ap.parse_args()
@@ -206,6 +311,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
 
     import torch

@@ -82,8 +82,17 @@ def grf_log_E(X_rest, tets, seed, mean=np.log(1e5), std=1.5, corr_len=0.1, featu
     centroids: mean `mean`, standard deviation `std`, squared-exponential correlation of length
     `corr_len`, sampled by random Fourier features (omega ~ N(0, I / corr_len^2))."""
     rng = np.random.default_rng(seed)
-    c = X_rest[tets].mean(axis=1)                         # centroids (ne, 3)
     om = rng.standard_normal((features, 3)) / corr_len
     ph = rng.uniform(0.0, 2.0 * np.pi, features)
+    if not isinstance(X_rest, np.ndarray):
+        # torch tensors (e.g. on the GPU, for bulk training-data synthesis): same field
+        import torch
+        Xt = X_rest.to(torch.float64)
+        c = Xt[tets.long()].mean(dim=1)
+        omt = torch.as_tensor(om, dtype=torch.float64, device=Xt.device)
+        pht = torch.as_tensor(ph, dtype=torch.float64, device=Xt.device)
+        g = np.sqrt(2.0 / features) * torch.cos(c @ omt.T + pht).sum(dim=1)
+        return torch.exp(mean + std * g)
+    c = X_rest[tets].mean(axis=1)                         # centroids (ne, 3)
     g = np.sqrt(2.0 / features) * np.cos(c @ om.T + ph).sum(axis=1)
     return np.exp(mean + std * g)
