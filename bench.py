@@ -276,6 +276,62 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_replay(args):
+    """benchmark_linear_systems' replay (tests/benchmark_linear_systems.cc:216-342) of a recorded
+    Newton sequence: the n^3 cube's pattern and `--steps` Hessians (new perturbation each) are
+    written in the reference's MatrixRecorder layout, then read back and replayed through the
+    GPU factorizer (symbolic per pattern; numeric factorization timed per matrix; backward error of
+    a random-solution solve).  --replay-dir replays an existing recording instead."""
+    import tempfile
+    import torch
+    import meshfemsparse_amd as mfs
+    from meshfemsparse_amd import workload as wl
+    from meshfemsparse_amd.recorder import MatrixRecorder, replay
+    dev = torch.device("cuda:0")
+    tmp = None
+    d = args.replay_dir
+    if not d:
+        tmp = tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp"))
+        d = tmp.name
+        N = args.n
+        tets, Xr = wl.cube_mesh(N)
+        H = mfs.SystemAssembler(len(Xr)).blockSparsityPattern(tets)
+        H.setZero()
+        da = mfs.DeviceAssembler(H, tets)
+        rec = MatrixRecorder(d)
+        rec.countSparsityUpdateCall()
+        rec.recordSymbolic(H, wl.z0_pins(N))
+        Xr_d = torch.from_numpy(Xr).to(dev)
+        for k in range(args.steps):
+            He, _ = da.neohookean(Xr_d, torch.from_numpy(wl.perturb(N, Xr, k)).to(dev), gradients=False)
+            H.Ax.zero_()
+            da.assemble(He, H.Ax)
+            rec.recordNumeric(H)
+            del He
+        log(f"[bench] recorded {args.steps} matrices of the {N}^3 cube in {d}")
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, args.precision != "fp64",
+                                     args.precision if args.precision.startswith("bf16") else None)
+    t0 = time.perf_counter()
+    out = replay(d, f, repeats=1 + args.warmup, device=dev)
+    wall = time.perf_counter() - t0
+    nf = out["numeric_s"]
+    res = {
+        "metric": "recorded-sequence replay: numeric factorization s per matrix (benchmark_linear_systems)",
+        "value": float(np.mean(nf)) if nf else None, "unit": "s/factorization", "n_gpus": 1,
+        "steps": len(nf), "warmup": 0, "ms_per_step": float(np.mean(nf)) * 1e3 if nf else None,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "recorded synthetic Newton sequence (reference MatrixRecorder layout)",
+        "config": {"workload": f"replay of {args.replay_dir or f'{args.n}^3 cube, {args.steps} recorded Hessians'}"},
+        "posdef": out["posdef"], "indef": out["indef"], "symbolic": out["symbolic"],
+        "backward_error_max": max(out["backward_error"]) if out["backward_error"] else None,
+        "replay_wall_s": wall,
+    }
+    print(json.dumps(res), flush=True)
+    if tmp:
+        tmp.cleanup()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,9 +349,11 @@ def main():
                     help="per-element E log-uniform [1e3, 1e7], nu uniform [0.2, 0.45]")
     ap.add_argument("--forward-error", action="store_true",
                     help="also factor in FP64 and report the forward error of every step (fp32)")
-    ap.add_argument("--workload", choices=["cube", "batch", "learned"], default="cube",
+    ap.add_argument("--workload", choices=["cube", "batch", "learned", "replay"], default="cube",
                     help="cube: the 64^3 Newton step (headline); batch: 512 uneven meshes in one system; "
-                         "learned: train the learned preconditioner, evaluate held-out solves")
+                         "learned: train the learned preconditioner, evaluate held-out solves; replay: "
+                         "record and replay a Newton sequence like benchmark_linear_systems")
+    ap.add_argument("--replay-dir", default="", help="replay: an existing recording directory")
     ap.add_argument("--train", type=int, default=2000, help="learned: training Hessians")
     ap.add_argument("--learn-n", type=int, default=48, help="learned: cube side of the training meshes")
     ap.add_argument("--lr", type=float, default=2e-3, help="learned: Adam learning rate")
@@ -314,6 +372,9 @@ def main():
         return
     if args.workload == "learned":
         bench_learned(args)
+        return
+    if args.workload == "replay":
+        bench_replay(args)
         return
 
     import torch

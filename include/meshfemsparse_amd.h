@@ -304,6 +304,18 @@ int mfs_learned_solve(mfs_learned l, mfs_factorizer fallback, const double* Ax_d
                       int32_t* outer_iters, double* relres, int32_t* used_fallback, void* stream);
 
 /* ------------------------------------------------------------------------------------------ */
+/* BlockCSCHessian binary files (BlockCSCHessianBase::dumpBinaryToStream / constructFromBinaryStream, */
+/* BlockCSCHessian.cc:11-103; MatrixRecorder.hh:36-72 names the sequences): the reference's format   */
+/* (size_t block size 3, size_t blocks, int64 m n nz snz, uint32 symmetry 1, Ap, Ai, Ax in the      */
+/* non-contiguous layout = scalar CSC of the upper triangle).  In memory Ax is contiguous (9 per    */
+/* block, strict lower of diagonal blocks 0 on read).  Ax NULL writes / reads the pattern only;    */
+/* read with NULL arrays returns the sizes (snz = 0: sparsity only).                              */
+/* ------------------------------------------------------------------------------------------ */
+int mfs_blockcsc_write(const char* path, const int64_t* Ap, const int64_t* Ai, int64_t num_blocks, const double* Ax);
+int mfs_blockcsc_read(const char* path, int64_t* num_blocks, int64_t* nz, int64_t* snz, int64_t* Ap, int64_t* Ai,
+                      double* Ax);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Workload generator (the reference has no energy code; the caller supplies He).             */
 /* Unit cube, N^3 hexes, each split into 6 Kuhn tets; node id = x + (N+1)(y + (N+1) z).       */
 /* ------------------------------------------------------------------------------------------ */

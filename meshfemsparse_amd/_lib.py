@@ -80,6 +80,8 @@ _SIGS = {
     "mfs_kernel_profiling": [c_vp, c_i32, c_i32],
     "mfs_kernel_profile": [c_vp, P_dbl, P_i64],
     "mfs_kernel_profile_work": [c_vp, P_dbl],
+    "mfs_blockcsc_write": [ctypes.c_char_p, P_i64, P_i64, c_i64, P_dbl],
+    "mfs_blockcsc_read": [ctypes.c_char_p, P_i64, P_i64, P_i64, P_i64, P_i64, P_dbl],
     "mfs_cube_mesh": [c_i32, P_i32, P_dbl],
     "mfs_perturb_nodes": [c_i32, ctypes.c_uint64, c_dbl, P_dbl],
     "mfs_neohookean_host": [P_dbl, P_dbl, P_i32, c_i64, P_dbl, P_dbl, c_dbl, c_dbl, P_dbl, P_dbl,

@@ -654,4 +654,19 @@ int mfs_learned_solve(mfs_learned l, mfs_factorizer fallback, const double* Ax, 
     if (!ok && !fallback) throw mfs::Error(MFS_ERR_NOT_CONVERGED, "preconditioned CG did not reach the tolerance");
     MFS_CATCH
 }
+
+int mfs_blockcsc_write(const char* path, const int64_t* Ap, const int64_t* Ai, int64_t nb, const double* Ax) {
+    MFS_TRY
+    CHECK_PTR(path); CHECK_PTR(Ap); CHECK_PTR(Ai);
+    if (nb <= 0) throw mfs::Error(MFS_ERR_ARG, "bad block count");
+    mfs::write_blockcsc(path, Ap, Ai, nb, Ax);
+    MFS_CATCH
+}
+
+int mfs_blockcsc_read(const char* path, int64_t* nb, int64_t* nz, int64_t* snz, int64_t* Ap, int64_t* Ai, double* Ax) {
+    MFS_TRY
+    CHECK_PTR(path);
+    mfs::read_blockcsc(path, nb, nz, snz, Ap, Ai, Ax);
+    MFS_CATCH
+}
 } // extern "C"

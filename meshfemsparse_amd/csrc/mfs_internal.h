@@ -265,6 +265,11 @@ void emulate_solve(const Symbolic& S, const std::vector<double>& L, const std::v
                    std::vector<double>& x);
 void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx);
 
+// BlockCSCHessian binary files (BlockCSCHessian.cc:11-103): contiguous 3x3 values in memory,
+// the reference's non-contiguous layout on disk
+void write_blockcsc(const char* path, const int64_t* Ap, const int64_t* Ai, int64_t nb, const double* Ax);
+void read_blockcsc(const char* path, int64_t* nb, int64_t* nz, int64_t* snz, int64_t* Ap, int64_t* Ai, double* Ax);
+
 // Workload (host)
 void cube_mesh(int N, int32_t* tets, double* X);
 void perturb_nodes(int N, uint64_t seed, double sigma_rel, double* X);

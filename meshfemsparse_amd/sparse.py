@@ -73,6 +73,39 @@ class BlockCSCHessian:
     def diagBlockLoc(self, j):
         return 9 * (int(self.Ap[j + 1]) - 1)
 
+    def trace(self):
+        """Sum of the diagonal scalar entries (BlockCSCHessianBase::trace)."""
+        Ax = self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor) else np.asarray(self.Ax)
+        d = 9 * (self.Ap[1:] - 1)
+        return float(Ax[d].sum() + Ax[d + 4].sum() + Ax[d + 8].sum())
+
+    def dumpBinaryToFile(self, path):
+        """BlockCSCHessianBase::dumpBinaryToStream (BlockCSCHessian.cc:11-40): the reference's
+        binary format, non-contiguous layout on disk; a sparsity-only matrix writes no values."""
+        Ax = None
+        if self.Ax is not None:
+            Ax = np.ascontiguousarray(self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor)
+                                      else self.Ax, np.float64)
+        _lib.call("mfs_blockcsc_write", str(path).encode(), ptr(self.Ap, _lib.c_i64), ptr(self.Ai, _lib.c_i64),
+                  self.n, None if Ax is None else ptr(Ax, _lib.c_double))
+
+    @classmethod
+    def constructFromBinaryFile(cls, path, device="cuda"):
+        """BlockCSCHessianBase::constructFromBinaryStream (BlockCSCHessian.cc:88-103), returning the
+        contiguous-block layout with values on `device` (None for a sparsity-only file)."""
+        nb, nz, snz = _lib.c_i64(), _lib.c_i64(), _lib.c_i64()
+        p = str(path).encode()
+        _lib.call("mfs_blockcsc_read", p, nb, nz, snz, None, None, None)
+        Ap = np.zeros(nb.value + 1, np.int64)
+        Ai = np.zeros(nz.value, np.int64)
+        Ax = np.zeros(9 * nz.value) if snz.value else None
+        _lib.call("mfs_blockcsc_read", p, nb, nz, snz, ptr(Ap, _lib.c_i64), ptr(Ai, _lib.c_i64),
+                  None if Ax is None else ptr(Ax, _lib.c_double))
+        H = cls(Ap, Ai, device)
+        if Ax is not None:
+            H.Ax = torch.from_numpy(Ax).to(device)
+        return H
+
     def toDense(self):
         """Dense symmetric scalar matrix (host, float64); for small tests."""
         Ax = self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor) else np.asarray(self.Ax)
