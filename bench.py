@@ -31,7 +31,7 @@ PEAK_HBM_GBS = 8000.0
 # kernel classes of mfs_kernel_profile
 KCLASSES = ["fill_L", "inject", "unused", "extend", "potrf", "trsm", "update", "syrk", "fgather",
             "fdiag", "fupdate", "bpartial", "bdiag", "spmv", "maps", "assemble", "blkinv", "bgather",
-            "convert", "refine", "fused"]
+            "convert", "refine", "fused", "panel"]
 
 
 def log(*a):
@@ -504,7 +504,7 @@ def main():
     # algorithmic GEMM flops of a step (supernodal count minus the diagonal-tile POTRF share),
     # apportioned to the classes by their executed tile flops
     potrf_flops = f.flopBreakdown()["potrf"]
-    exec_all = float(work[5] + work[6] + work[7])
+    exec_all = float(work[5] + work[6] + work[7] + work[KCLASSES.index("panel")])
     share = float(sum(work[c] for c in gcls)) / exec_all if exec_all > 0 else 1.0
     gemm_work = (flops - potrf_flops) * share * args.steps
     if kt and gemm_launches:

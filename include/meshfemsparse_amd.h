@@ -189,9 +189,10 @@ int mfs_autotune(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t
                  int32_t nfuse, const int32_t* tiles, int32_t ntile, int32_t reps, double* ms, int32_t* best,
                  void* stream);
 
-/* Schedule statistics: out8 = {fused tasks, fused-kernel launches (levels), tiled supernodes,
- * POTRF launches, GEMM launches, extend-add launches, levels, nodes of the captured graph}.   */
-int mfs_schedule_stats(mfs_factorizer f, int64_t* out8);
+/* Schedule statistics: out9 = {fused tasks, fused-kernel launches (levels), tiled supernodes,
+ * separate POTRF launches, GEMM launches, extend-add launches, levels, nodes of the captured
+ * graph, fused diagonal-tile + TRSM launches}.                                                */
+int mfs_schedule_stats(mfs_factorizer f, int64_t* out9);
 
 /* State / statistics (CholeskyFactorizerBase.hh:55-59,193-260; CatamariFactorizer.cc:716-948). */
 int mfs_factorization_type(mfs_factorizer f, int32_t* type);   /* 0 none, 1 symbolic, 2 numeric */
@@ -247,10 +248,11 @@ int mfs_last_timings(mfs_factorizer f, double* ms4);
  * update GEMM tiles, Schur-update GEMM tiles, forward gather, forward diagonal, forward update
  * GEMV, backward partial GEMV^T, backward diagonal, SpMV, permutation maps, assembly, solve-block
  * inverses, backward gather, precision conversions, refinement vector kernels, fused level
- * kernel.  ms[21] and counts[21].                                                             */
+ * kernel, fused diagonal-tile + TRSM steps.  ms[22] and counts[22].                          */
 int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
 int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
-/* Executed tile flops of the profiled launches per class (21; the three GEMM classes carry them).*/
+/* Executed tile flops of the profiled launches per class (22; the GEMM classes and the fused   */
+/* diagonal-tile + TRSM steps carry them).                                                     */
 int mfs_kernel_profile_work(mfs_factorizer f, double* work);
 
 /* ------------------------------------------------------------------------------------------ */

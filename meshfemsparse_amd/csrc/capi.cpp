@@ -243,11 +243,18 @@ int mfs_schedule_stats(mfs_factorizer f, int64_t* out) {
     CHECK_PTR(f); CHECK_PTR(out);
     f->dev->require(1);
     const auto& S = *f->dev->sym;
-    int64_t fused_launches = 0, potrf_launches = 0, gemm_launches = 0, extend_launches = 0;
+    int64_t fused_launches = 0, potrf_launches = 0, gemm_launches = 0, extend_launches = 0, panel_launches = 0;
     for (const auto& P : S.levels) {
         fused_launches += P.fused.second > P.fused.first;
-        for (auto [a, b] : P.potrf_steps) potrf_launches += b > a;
-        for (const auto& r : P.trsm_steps) gemm_launches += (r.m > r.b) + (r.e > r.m);
+        for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
+            if (P.panel_steps[t].second > P.panel_steps[t].first) {
+                ++panel_launches;
+                continue;
+            }
+            potrf_launches += P.potrf_steps[t].second > P.potrf_steps[t].first;
+            const auto& r = P.trsm_steps[t];
+            gemm_launches += (r.m > r.b) + (r.e > r.m);
+        }
         for (const auto& r : P.update_steps) gemm_launches += (r.m > r.b) + (r.e > r.m);
         gemm_launches += (P.syrk.m > P.syrk.b) + (P.syrk.e > P.syrk.m);
         extend_launches += (int64_t)P.extend_passes.size() + (int64_t)P.extend_u_passes.size();
@@ -260,6 +267,7 @@ int mfs_schedule_stats(mfs_factorizer f, int64_t* out) {
     out[5] = extend_launches;
     out[6] = S.num_levels;
     out[7] = f->dev->graph_nodes();
+    out[8] = panel_launches;
     MFS_CATCH
 }
 

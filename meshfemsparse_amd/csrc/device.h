@@ -44,7 +44,7 @@ private:
 // Optional per-kernel-class device timing (HIP event pairs around each launch).
 struct KernelProfile {
     enum { FILL_L, INJECT, UNUSED, EXTEND, POTRF, TRSM, UPDATE, SYRK, FGATHER, FDIAG, FGEMV, BGEMV,
-           BDIAG, SPMV, MAPS, ASSEMBLE, BLKINV, BGATHER, CONVERT, REFINE, FUSED, NCLASS };
+           BDIAG, SPMV, MAPS, ASSEMBLE, BLKINV, BGATHER, CONVERT, REFINE, FUSED, PANEL, NCLASS };
     bool on = false;
     double ms[NCLASS] = {};
     int64_t count[NCLASS] = {};
@@ -115,7 +115,7 @@ private:
     size_t fsize() const;
 
     DeviceArray L_, U_, inv_, status_;
-    DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, fused_, uoff_;
+    DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, fused_, uoff_, panel_, pcount_;
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;

@@ -442,10 +442,19 @@ __device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 
 // One tile (256 threads, a and x in LDS).  On return the factor is stored to the panel, the
 // inverse to inv[t.slot], and x still holds X in LDS (all threads past the final barrier).
+// Shared tiles: with `counter`, `count` workgroups factor the same tile; each counts itself in
+// after its tile load has completed, and only the last to arrive stores the factor, the inverse
+// and a pivot failure (so no workgroup can see a stored factor in place of its input), then
+// resets the counter for the next factorization.  No workgroup waits for another.
+// With `rt`, the rows x cols block at `rsrc` (leading dimension rld) is staged into rt[col][row]
+// (zero elsewhere) by the same round trip that loads the tile.
 template <typename T>
 __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L, T* __restrict__ inv,
                                            int* __restrict__ status, T (*__restrict__ a)[TS + 1],
-                                           T (*__restrict__ x)[TS + 1]) {
+                                           T (*__restrict__ x)[TS + 1], int* __restrict__ counter = nullptr,
+                                           int count = 1, int* __restrict__ s_flag = nullptr,
+                                           T (*__restrict__ rt)[TS + 1] = nullptr, const T* __restrict__ rsrc = nullptr,
+                                           int64_t rld = 0, int rrows = 0, int rcols = 0) {
     const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6, rr = tid & 7, cc = (tid >> 3) & 7;
     const int64_t ld = t.ld;
     T* D = L + t.diag_off;
@@ -453,11 +462,18 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
         // every load of a thread is issued before the first use (one memory round trip); entries
         // outside the kb x kb tile read a clamped in-tile address and are replaced by the identity
         constexpr int PT = TS * TS / 256;
-        T v[PT];
+        T v[PT], w[PT];
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
             const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
             v[q] = D[(int64_t)min(c, kb - 1) * ld + min(r, kb - 1)];
+        }
+        if (rt) {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
+                w[q] = (r < rrows && c < rcols) ? rsrc[(int64_t)c * rld + r] : T(0);
+            }
         }
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
@@ -465,8 +481,20 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
             a[c][r] = (r < kb && c < kb) ? ((r >= c) ? v[q] : T(0)) : ((r == c) ? T(1) : T(0));
             x[c][r] = T(0);
         }
+        if (rt) {
+#pragma unroll
+            for (int q = 0; q < PT; ++q) {
+                const int idx = tid + 256 * q;
+                rt[idx >> 6][idx & 63] = w[q];
+            }
+        }
     }
-    __syncthreads();
+    __syncthreads();    // every load of the tile has returned
+    // count in now; the returned value is only needed at the end, so the atomic's round trip
+    // overlaps the factorization
+    int arrived = 0;
+    if (counter && tid == 0) arrived = atomicAdd(counter, 1);
+    int fail = INT_MAX;    // first failing column (wave 0, lane 0)
     // A(r, c) -= sum_k L(r, k0 + k) L(c, k0 + k) for the entry (rr, cc) of block (i, j)
     auto rank8 = [&](int k0, int i, int j) {
         const int r = PB * i + rr, c = PB * j + cc;
@@ -492,7 +520,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
 #pragma unroll
                 for (int k = 0; k < j; ++k) d -= lb[j][k] * lb[j][k];
                 if (!(d > T(0))) {
-                    if (tid == 0) atomicMin(status, t.col0 + b0 + j);
+                    fail = min(fail, t.col0 + b0 + j);
                     d = T(1);
                 }
                 const T sd = sqrt_t(d);
@@ -578,6 +606,16 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
         }
         __syncthreads();
     }
+    if (counter) {
+        if (tid == 0) {
+            const bool last = arrived == count - 1;
+            if (last) atomicExch(counter, 0);
+            *s_flag = last;
+        }
+        __syncthreads();
+        if (*s_flag == 0) return;
+    }
+    if (tid == 0 && fail != INT_MAX) atomicMin(status, fail);
     T* out = inv + (int64_t)t.slot * TS * TS;
 #pragma unroll
     for (int q = 0; q < TS * TS / 256; ++q) {
@@ -764,6 +802,53 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
     gemm_tile<T, TM, TN, KC>(tasks[xcd_task(blockIdx.x, gridDim.x, remap)], L, U, inv, As, Bs);
 }
 
+// Fused diagonal tile + TRSM of one step: workgroup = one 64-row tile below the diagonal tile
+// (or a diagonal tile with no rows below).  The tile is factored in LDS by every workgroup of its
+// group (potrf_tile with the shared-tile counter), then the row tile is solved in place,
+// L21 := A21 X^T, with X = L11^-1 straight from LDS: the same MFMA products in the same k order as
+// the separate TRSM launch of the FP64 path.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ items,
+                                               const PotrfTask* __restrict__ ptasks,
+                                               const GemmTask* __restrict__ gtasks, T* __restrict__ L,
+                                               T* __restrict__ inv, int* __restrict__ status,
+                                               int* __restrict__ counters) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T a[TS][TS + 1];
+    __shared__ T x[TS][TS + 1];
+    __shared__ T rt[TS][TS + 1];     // the row tile, rt[k][r] = A21(r, k)
+    __shared__ int flag;
+    const PanelItem it = items[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool rows = it.trsm >= 0;
+    GemmTask g{};
+    if (rows) g = gtasks[it.trsm];
+    potrf_tile<T>(ptasks[it.potrf], L, inv, status, a, x, it.count > 1 ? counters + it.potrf : nullptr,
+                  it.count, &flag, rows ? rt : nullptr, L + g.a_off, g.lda, g.M, g.K);
+    if (!rows) return;
+    T* C = L + g.a_off;         // in place (c_off == a_off)
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32, cc = lane & 15;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+    for (int ks = 0; ks < TS; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T af[2], bf[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) af[i] = rt[kr][wm + 16 * i + cc];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bf[j] = x[kr][wn + 16 * j + cc];     // B(n, k) = X(n, k)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+    }
+    gemm_store_acc<T, 2, 2>(acc, C, g.ldc, g.M, g.N, wm, wn, lane, T(1), true);
+}
+
 // ------------------------------------------------------------------------------------------
 // Fused level kernel: each workgroup factors whole small supernodes (one diagonal tile wide) of
 // one elimination-tree level, striding over the level's task list (persistent: the grid is sized
@@ -888,17 +973,21 @@ __global__ __launch_bounds__(256, 2) void k_fused(const FusedTask* __restrict__ 
 // fragment for the whole chunk with one 16-byte LDS read; A and B use the same mapping, so the
 // product is unchanged.  In FP32 the C tile leaves through LDS in two halves for coalesced row
 // access; in FP64 the accumulators are stored directly.
-template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> { static constexpr int KC = 16, LDK = 20; };
-template <> struct GemmCfg<double> { static constexpr int KC = 16, LDK = 18; };
+// A chunk of KC k values (a multiple of 16) is one global round trip; the MFMA loop walks it in
+// blocks of 16 with the fragment mapping above.  Row stride LDK = KC + 16 bytes of padding: an odd
+// number of 16-byte units, so 16 consecutive rows start in distinct 16-byte bank groups.
+template <typename T, int KC> struct GemmCfg {
+    static_assert(KC % 16 == 0, "K chunk");
+    static constexpr int LDK = KC + 16 / (int)sizeof(T);
+};
 
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, int KC>
 __global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ tasks,
                                                   T* __restrict__ L, T* __restrict__ U,
                                                   const T* __restrict__ inv, int remap) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef T v4 __attribute__((ext_vector_type(4)));
-    constexpr int KC = GemmCfg<T>::KC, LDK = GemmCfg<T>::LDK;
+    constexpr int LDK = GemmCfg<T, KC>::LDK;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int AG = TM * (KC / 4) / 256, BG = TN * (KC / 4) / 256;   // (row, 4-k group)s per thread
     constexpr int OPS = 2 * (TM + TN) * LDK, EPI = WN * (TM + 1);
@@ -976,17 +1065,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ t
     for (int k0 = 0; k0 < t.K; k0 += KC) {
         const bool more = k0 + KC < t.K;
         if (more) gload(k0 + KC);
-        v4 af[FM], bf[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = *(const v4*)&As[(buf * TM + wm + 16 * i + fr) * LDK + fg];
+        for (int kb = 0; kb < KC; kb += 16) {
+            v4 af[FM], bf[FN];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = *(const v4*)&Bs[(buf * TN + wn + 16 * j + fr) * LDK + fg];
+            for (int i = 0; i < FM; ++i) af[i] = *(const v4*)&As[(buf * TM + wm + 16 * i + fr) * LDK + kb + fg];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+            for (int j = 0; j < FN; ++j) bf[j] = *(const v4*)&Bs[(buf * TN + wn + 16 * j + fr) * LDK + kb + fg];
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+            for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i][s], bf[j][s], acc[i][j]);
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i][s], bf[j][s], acc[i][j]);
+        }
         if (more) sstore(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -1039,7 +1131,9 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     return w;
 }
 
-// tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 2 for FP32, 1 for FP64),
+// tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 2 for FP32, 1 for FP64) for the
+// 128x128 tiles and MFS_GEMM_V64 for the 64x64 tiles (default: as MFS_GEMM_V), MFS_GEMM_KC2 and
+// MFS_GEMM_KC64 (k_gemm2's K chunk for the 128x128 and the 64x64 tiles: 16, 32 or 64; default 16),
 // MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (k_gemm's FP32 K chunk,
 // 16 or 32, default 16; FP64 uses 16 so that two workgroups share a CU's LDS)
 static int gemm_remap() {
@@ -1050,6 +1144,46 @@ static int gemm_version() {
     static const int v = getenv("MFS_GEMM_V") ? atoi(getenv("MFS_GEMM_V")) : 0;
     return v;
 }
+static int env_int(const char* name, int dflt) {
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+static int gemm_version64() {
+    static const int v = env_int("MFS_GEMM_V64", 0);
+    return v;
+}
+static int gemm_kc2() {
+    static const int v = env_int("MFS_GEMM_KC2", 16);
+    return v;
+}
+static int gemm_kc64() {
+    static const int v = env_int("MFS_GEMM_KC64", 16);
+    return v;
+}
+template <typename T, int TM>
+static void launch_gemm2(int kc, unsigned grid, const GemmTask* tasks, T* L, T* U, const T* inv, int rm,
+                         hipStream_t s) {
+    // larger chunks only where two workgroups still share a CU's LDS (measured: no gain on the
+    // 64^3 schedule; kept as a tuning knob)
+    constexpr bool k32_fits = 2 * 2 * (2 * TM) * (32 + 16 / (int)sizeof(T)) * (int)sizeof(T) <= 160 * 1024;
+    constexpr bool k64_fits = 2 * 2 * (2 * TM) * (64 + 16 / (int)sizeof(T)) * (int)sizeof(T) <= 160 * 1024;
+    if constexpr (k64_fits) {
+        if (kc >= 64) {
+            k_gemm2<T, TM, TM, 64><<<grid, 256, 0, s>>>(tasks, L, U, inv, rm);
+            LAUNCH_CHECK();
+            return;
+        }
+    }
+    if constexpr (k32_fits) {
+        if (kc >= 32) {
+            k_gemm2<T, TM, TM, 32><<<grid, 256, 0, s>>>(tasks, L, U, inv, rm);
+            LAUNCH_CHECK();
+            return;
+        }
+    }
+    k_gemm2<T, TM, TM, 16><<<grid, 256, 0, s>>>(tasks, L, U, inv, rm);
+    LAUNCH_CHECK();
+}
 static int gemm_kc32() {
     static const int v = getenv("MFS_GEMM_KC") ? (atoi(getenv("MFS_GEMM_KC")) >= 32) : 0;
     return v;
@@ -1059,22 +1193,23 @@ static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, c
     const int rm = gemm_remap();
     // default: the k-contiguous kernel for FP32, the k-major kernel for FP64 (measured best)
     const int ver = gemm_version() ? gemm_version() : (sizeof(T) == 4 ? 2 : 1);
-    if (ver == 2) {
-        if (r.m > r.b) {
-            k_gemm2<T, 128, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-            LAUNCH_CHECK();
-        }
-        if (r.e > r.m) {
-            k_gemm2<T, 64, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
-            LAUNCH_CHECK();
-        }
-        return;
-    }
+    const int ver64 = gemm_version64() ? gemm_version64() : ver;
     const bool k32 = sizeof(T) == 4 && gemm_kc32();
     if (r.m > r.b) {
-        if (k32) k_gemm<T, 128, 128, 32><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        else k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        LAUNCH_CHECK();
+        if (ver == 2) launch_gemm2<T, 128>(gemm_kc2(), (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, s);
+        else {
+            if constexpr (sizeof(T) == 4) {
+                if (k32) k_gemm<T, 128, 128, 32><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+                else k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+            } else {
+                k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+            }
+            LAUNCH_CHECK();
+        }
+    }
+    if (r.e > r.m && ver64 == 2) {
+        launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
+        return;
     }
     if (r.e > r.m) {
         if (k32) k_gemm<T, 64, 64, 32><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
@@ -1718,6 +1853,10 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     rel_.upload(S->rel);
     ext_.upload(S->extend_tasks);
     potrf_.upload(S->potrf_tasks);
+    panel_.upload(S->panel_items);
+    // shared-tile counters start at zero and every fused launch leaves them at zero
+    pcount_.alloc(std::max<size_t>(S->potrf_tasks.size(), 1) * sizeof(int));
+    HIP_CHECK(hipMemset(pcount_.p, 0, pcount_.bytes));
     fused_.upload(S->fused_tasks);
     uoff_.upload(S->u_off);
     gemm_.upload(S->gemm_tasks);
@@ -1820,14 +1959,21 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
         extend(P.extend_passes);
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];
-            if (p1 > p0) {
+            const TaskRange& tr = P.trsm_steps[t];
+            auto [i0, i1] = P.panel_steps[t];
+            if (i1 > i0) {
+                prof.begin(KernelProfile::PANEL, s, prof.on ? gemm_work(S, tr) : 0.0, i1 - i0);
+                k_panel<T><<<(unsigned)(i1 - i0), 256, 0, s>>>(panel_.as<PanelItem>() + i0, potrf_.as<PotrfTask>(),
+                                                             gt, L, inv, status_.as<int>(), pcount_.as<int>());
+                LAUNCH_CHECK();
+                prof.end(s);
+            } else if (p1 > p0) {
                 prof.begin(KernelProfile::POTRF, s, 0.0, p1 - p0);
                 k_potrf<T><<<(unsigned)(p1 - p0), 256, 0, s>>>(potrf_.as<PotrfTask>() + p0, L, inv, status_.as<int>());
                 LAUNCH_CHECK();
                 prof.end(s);
             }
-            const TaskRange& tr = P.trsm_steps[t];
-            if (tr.e > tr.b) {
+            if (tr.e > tr.b && i1 == i0) {
                 prof.begin(KernelProfile::TRSM, s, prof.on ? gemm_work(S, tr) : 0.0, tr.e - tr.b);
                 launch_gemm<T>(tr, gt, L, U, inv, s);
                 prof.end(s);

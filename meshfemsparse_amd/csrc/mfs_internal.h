@@ -74,6 +74,13 @@ struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], 
     int32_t col0;           // scalar column (factor order) of the tile's first column
 };
 
+struct PanelItem {          // one workgroup of a fused diagonal-tile + TRSM step: factor potrf
+                            // task `potrf` (every item of that task redundantly, in LDS), then
+                            // solve the rows of TRSM task `trsm` (-1: none); `count` items share
+                            // the potrf task and the last of them to load the tile stores it
+    int32_t potrf, trsm, count, pad;
+};
+
 struct ExtendTask {         // extend-add of child's update matrix into the parent front
     int64_t u_child;        // offset of child's U (D_c x D_c, ld = D_c) in the update arena
     int64_t p_panel;        // parent panel offset in L
@@ -153,6 +160,8 @@ struct LevelPlan {
     std::vector<std::pair<int64_t,int64_t>> extend_u_passes; // after the Schur update (into U)
     std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
     std::vector<TaskRange> trsm_steps;                       // per step: gemm tasks
+    std::vector<std::pair<int64_t,int64_t>> panel_steps;     // per step: range in panel items (empty:
+                                                             // separate POTRF and TRSM launches)
     std::vector<TaskRange> update_steps;                     // per step: gemm tasks
     TaskRange syrk;                                          // gemm tasks
     // solves: gather tasks (forward: children -> x/u; backward: x -> u), and per column-block
@@ -205,6 +214,7 @@ struct Symbolic {
     std::vector<LevelPlan> levels;
     std::vector<ExtendTask> extend_tasks;
     std::vector<PotrfTask> potrf_tasks;
+    std::vector<PanelItem> panel_items;
     std::vector<GemmTask> gemm_tasks;
     std::vector<FusedTask> fused_tasks;
     // solves
@@ -240,6 +250,8 @@ struct SymbolicOptions {
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
     int fuse_degree = 0;    // supernodes of width <= one tile and at most this many structure rows
                             // (scalars) go to the fused level kernel; 0 disables it
+    int panel_max = 512;    // a step whose TRSM needs at most this many row tiles (and POTRF
+                            // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
 };
 
@@ -247,7 +259,8 @@ struct SymbolicOptions {
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
 // (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
-// 128x128 tiles); values are rounded down to multiples of 64 where required.
+// 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound); values are rounded down to
+// multiples of 64 where required.
 SymbolicOptions options_from_env();
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,

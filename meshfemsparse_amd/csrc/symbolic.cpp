@@ -145,6 +145,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
     if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
+    if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
     return opt;
 }
 
@@ -667,7 +668,26 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                         small.push_back(g);
                     }
                 }
-                P.trsm_steps.push_back(push_tasks(S.gemm_tasks, big, small));
+                const TaskRange tr = push_tasks(S.gemm_tasks, big, small);
+                P.trsm_steps.push_back(tr);
+                // fused diagonal tile + TRSM: one workgroup per row tile (one per POTRF task
+                // without rows below), tasks in the order they were emitted above
+                const int64_t nrow = tr.e - tr.b, np = (int64_t)active.size();
+                const int64_t i0 = (int64_t)S.panel_items.size();
+                if (np > 0 && std::max(nrow, np) <= opt.panel_max) {
+                    int64_t g = tr.m;
+                    for (size_t a = 0; a < active.size(); ++a) {
+                        const int32_t s = active[a].first;
+                        const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s];
+                        const int32_t k1 = t * T + std::min(T, W - t * T);
+                        const int32_t nt = (W + D - k1 + T - 1) / T;
+                        const int32_t pi = (int32_t)(p0 + (int64_t)a);
+                        if (nt == 0) S.panel_items.push_back({pi, -1, 1, 0});
+                        for (int32_t q = 0; q < nt; ++q) S.panel_items.push_back({pi, (int32_t)g++, nt, 0});
+                    }
+                    if (g != tr.e) throw Error(-1, "internal: panel items do not cover the TRSM tasks");
+                }
+                P.panel_steps.push_back({i0, (int64_t)S.panel_items.size()});
             }
             {   // after TRSM(t): the next step's column block gets the left-looking update from the
                 // outer block's finished columns (K = k1 - J0), or, when step t closes its outer
@@ -807,6 +827,27 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         for (const TaskRange& r : P.trsm_steps) check_range(r);
         for (const TaskRange& r : P.update_steps) check_range(r);
         check_range(P.syrk);
+    }
+    // fused panel items: each TRSM task solves with its own POTRF task's tile, fits one 64-row
+    // tile, and the items of a POTRF task are `count` consecutive entries
+    for (size_t q = 0; q < S.panel_items.size();) {
+        const PanelItem& it = S.panel_items[q];
+        const PotrfTask& pt = S.potrf_tasks.at(it.potrf);
+        if (it.count < 1 || q + it.count > S.panel_items.size())
+            throw Error(-2, "internal: panel item count");
+        for (int32_t e = 0; e < it.count; ++e) {
+            const PanelItem& ie = S.panel_items[q + e];
+            if (ie.potrf != it.potrf || ie.count != it.count) throw Error(-2, "internal: panel item group");
+            if (ie.trsm >= 0) {
+                const GemmTask& g = S.gemm_tasks.at(ie.trsm);
+                if (g.b_off != (int64_t)pt.slot * T * T || g.N != pt.kb || g.K != pt.kb || g.M < 1 || g.M > T ||
+                    g.flags != (GT_BETA0 | GT_ALPHA_POS | GT_B_IN_INV))
+                    throw Error(-2, "internal: panel item TRSM task does not match its tile");
+            } else if (it.count != 1) {
+                throw Error(-2, "internal: panel item without rows must be alone");
+            }
+        }
+        q += it.count;
     }
     S.U_size = std::max<int64_t>(arena.peak, 32);
     S.inv_slots = std::max<int64_t>((int64_t)S.potrf_tasks.size(), 1);

@@ -307,10 +307,10 @@ class GpuSupernodalFactorizer:
         _lib.call("mfs_factorizer_set_graph", self._h, int(bool(on)))
 
     def scheduleStats(self):
-        v = np.zeros(8, np.int64)
+        v = np.zeros(9, np.int64)
         _lib.call("mfs_schedule_stats", self._h, ptr(v, _lib.c_i64))
         keys = ["fused_tasks", "fused_launches", "tiled_supernodes", "potrf_launches", "gemm_launches",
-                "extend_launches", "levels", "graph_nodes"]
+                "extend_launches", "levels", "graph_nodes", "panel_launches"]
         return {k: int(x) for k, x in zip(keys, v)}
 
     def autotune(self, H, pinnedVars=(), fuse=(0, 192, 384, 768), tiles=(128, 256), reps=2):

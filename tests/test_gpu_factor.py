@@ -96,6 +96,57 @@ def test_factor_matches_unique_cholesky(dev, N, outer, fuse, big, monkeypatch):
     assert np.array_equal(orc.dense_L(*f.exportFactorCSC()), Lg)
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("N,panel", [(8, 4), (9, 512), (12, 40)])
+def test_fused_panel_steps_match_separate_launches(dev, N, panel, single, monkeypatch):
+    """Steps whose TRSM needs at most `panel` row tiles run as one launch (every workgroup
+    factors the diagonal tile in LDS, the last one to load it stores it).  In FP64 the fused TRSM
+    forms the same MFMA products in the same order as the separate launch, so the factor is
+    bitwise the one of the unfused schedule; in FP32 it agrees to FP32 accuracy.  Refactoring
+    reproduces it bit for bit (the shared-tile counters are left at zero)."""
+    H, pins, _, _, _ = mesh_problem(dev, N)
+
+    def factor(pm):
+        monkeypatch.setenv("MFS_PANEL_MAX", str(pm))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+        f.factorizeSymbolic(H, pins)
+        f.factorizeNumeric(H)
+        return f, orc.dense_L(*f.exportFactorCSC())
+
+    f0, L0 = factor(0)
+    f1, L1 = factor(panel)
+    st0, st1 = f0.scheduleStats(), f1.scheduleStats()
+    assert st0["panel_launches"] == 0 and st1["panel_launches"] > 0
+    assert st1["panel_launches"] + st1["potrf_launches"] == st0["potrf_launches"]
+    if single:
+        (Lp, Li, Lx), _, _ = oracle_factor(H, f1)
+        Lo = orc.dense_L(Lp, Li, Lx)
+        cn = np.abs(Lo).max(axis=0)
+        assert (np.abs(L1 - Lo).max(axis=0) <= 2e-5 * cn).all()
+    else:
+        assert np.array_equal(L1, L0)
+    for _ in range(2):
+        f1.factorizeNumeric(H)
+        assert np.array_equal(orc.dense_L(*f1.exportFactorCSC()), L1)
+
+
+def test_fused_panel_step_reports_failing_pivot(dev, monkeypatch):
+    """-A through the fused steps: the pivot failure is recorded once by the storing workgroup
+    and raised like in the separate launches."""
+    monkeypatch.setenv("MFS_PANEL_MAX", "512")
+    H, pins, _, _, _ = mesh_problem(dev, 6)
+    f = mfs.make_cholesky_factorizer()
+    f.factorizeSymbolic(H, pins)
+    assert f.scheduleStats()["panel_launches"] > 0
+    H.Ax.neg_()
+    with pytest.raises(mfs.NotPositiveDefiniteError):
+        f.factorizeNumeric(H)
+    assert not f.checkPosDef()
+    H.Ax.neg_()
+    f.factorizeNumeric(H)
+    assert f.checkPosDef()
+
+
 def dense_of(H):
     return H.toDense()
 
