@@ -41,6 +41,17 @@ def lib():
         L.orc_ltsolve.argtypes = [i64, P(i64), P(i64), P(f64), P(f64)]
         L.orc_multiply_lower_block.argtypes = [ctypes.c_int, P(i64), i64, i64, i64, P(f64), i64,
                                                ctypes.c_int, P(f64), i64]
+        L.orc_blas_load.argtypes = [ctypes.c_char_p]
+        L.orc_blas_load.restype = ctypes.c_int
+        L.orc_sn_analyze.argtypes = [i64, P(i64), P(i64), ctypes.c_int]
+        L.orc_sn_analyze.restype = ctypes.c_void_p
+        L.orc_sn_info.argtypes = [ctypes.c_void_p, P(f64)]
+        L.orc_sn_factor.argtypes = [ctypes.c_void_p, P(i64), P(i64), P(f64), ctypes.c_int]
+        L.orc_sn_factor.restype = i64
+        L.orc_sn_solve.argtypes = [ctypes.c_void_p, P(f64)]
+        L.orc_sn_dense_L.argtypes = [ctypes.c_void_p, P(f64)]
+        L.orc_sn_free.argtypes = [ctypes.c_void_p]
+        L.orc_sn_phase_times.argtypes = [P(f64)]
         _lib = L
     return _lib
 
@@ -192,3 +203,73 @@ def dense_L(Lp, Li, Lx):
         for p in range(Lp[j], Lp[j + 1]):
             L[Li[p], j] = Lx[p]
     return L
+
+
+def blas_path():
+    """The OpenBLAS that ships with scipy (LP64, symbols prefixed scipy_)."""
+    import glob
+    import scipy
+    libs = glob.glob(os.path.join(os.path.dirname(scipy.__file__), "..", "scipy.libs", "libscipy_openblas*.so"))
+    if not libs:
+        raise ImportError("scipy's bundled OpenBLAS not found")
+    return os.path.realpath(libs[0])
+
+
+_blas_loaded = False
+
+
+class Supernodal:
+    """Multifrontal supernodal Cholesky on the host (oracle/supernodal.c) of a scalar lower CSC."""
+
+    def __init__(self, Cp, Ci, relax=4):
+        global _blas_loaded
+        L = lib()
+        if not _blas_loaded:
+            # idle BLAS threads spin for 2^4 cycles only, so they do not compete with the OpenMP
+            # threads of the tree levels (read when the library initialises)
+            os.environ.setdefault("OPENBLAS_THREAD_TIMEOUT", "4")
+            if L.orc_blas_load(blas_path().encode()) != 0:
+                raise ImportError("cannot load the BLAS for the supernodal CPU factorization")
+            _blas_loaded = True
+        self.Cp = np.ascontiguousarray(Cp, np.int64)
+        self.Ci = np.ascontiguousarray(Ci, np.int64)
+        self.n = len(self.Cp) - 1
+        self._h = L.orc_sn_analyze(self.n, _p(self.Cp, i64), _p(self.Ci, i64), relax)
+        if not self._h:
+            raise ValueError("scalar dimension must be a multiple of 3")
+        info = np.zeros(6)
+        L.orc_sn_info(self._h, _p(info, f64))
+        self.info = dict(supernodes=int(info[0]), levels=int(info[1]), panel_scalars=int(info[2]),
+                         flops=float(info[3]), max_width=int(info[4]), arena_scalars=int(info[5]))
+
+    def factor(self, Cx, threads=None):
+        Cx = np.ascontiguousarray(Cx, np.float64)
+        t = threads or os.cpu_count() or 1
+        r = lib().orc_sn_factor(self._h, _p(self.Cp, i64), _p(self.Ci, i64), _p(Cx, f64), int(t))
+        if r == -2:
+            raise ImportError("BLAS not loaded")
+        if r >= 0:
+            raise NotSPD(r)
+
+    @staticmethod
+    def phase_times():
+        """Seconds of the last factorization: per phase summed over threads (assembly, potrf,
+        trsm, syrk, rest), then wall time in tree-parallel levels and in BLAS-parallel levels."""
+        t = np.zeros(7)
+        lib().orc_sn_phase_times(_p(t, f64))
+        return dict(zip(["assembly", "potrf", "trsm", "syrk", "rest", "wall_tree", "wall_blas"], t.tolist()))
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, copy=True)
+        lib().orc_sn_solve(self._h, _p(x, f64))
+        return x
+
+    def dense_L(self):
+        Ld = np.zeros((self.n, self.n))
+        lib().orc_sn_dense_L(self._h, _p(Ld, f64))
+        return Ld.T.copy()       # written column-major
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_sn_free(self._h)
+            self._h = None
