@@ -250,7 +250,7 @@ struct SymbolicOptions {
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
     int fuse_degree = 0;    // supernodes of width <= one tile and at most this many structure rows
                             // (scalars) go to the fused level kernel; 0 disables it
-    int panel_max = 512;    // a step whose TRSM needs at most this many row tiles (and POTRF
+    int panel_max = 256;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
 };
