@@ -72,6 +72,13 @@ int mfs_pattern_build(const int32_t* elem_nodes, int64_t num_elems, int32_t node
 int mfs_pattern_get(mfs_pattern p, int64_t* num_blocks, int64_t* nz,
                     const int64_t** Ap, const int64_t** Ai);
 int mfs_pattern_destroy(mfs_pattern p);
+/* The same construction on the device (SystemAssembler.hh:251-465 buckets, built by GPU kernels):
+ * elem_nodes_dev is device int32 [num_elems * nodes_per_elem]; the result equals
+ * mfs_pattern_build's.  mfs_pattern_get returns host copies; mfs_pattern_get_device the device
+ * arrays (owned by the handle).  Synchronizes the stream.                                     */
+int mfs_pattern_build_device(const int32_t* elem_nodes_dev, int64_t num_elems, int32_t nodes_per_elem,
+                             int64_t num_blocks, void* stream, mfs_pattern* out);
+int mfs_pattern_get_device(mfs_pattern p, const int64_t** Ap_dev, const int64_t** Ai_dev);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Hessian assembly                                                                            */

@@ -22,6 +22,8 @@ _SIGS = {
     "mfs_version": [],
     "mfs_pattern_build": [P_i32, c_i64, c_i32, c_i64, ctypes.POINTER(c_vp)],
     "mfs_pattern_get": [c_vp, P_i64, P_i64, ctypes.POINTER(P_i64), ctypes.POINTER(P_i64)],
+    "mfs_pattern_build_device": [c_vp, c_i64, c_i32, c_i64, c_vp, ctypes.POINTER(c_vp)],
+    "mfs_pattern_get_device": [c_vp, ctypes.POINTER(P_i64), ctypes.POINTER(P_i64)],
     "mfs_pattern_destroy": [c_vp],
     "mfs_assembler_create": [P_i64, P_i64, c_i64, P_i32, c_i64, c_i32, ctypes.POINTER(c_vp)],
     "mfs_assemble_hessian": [c_vp, c_vp, ctypes.c_int, c_vp, c_vp],

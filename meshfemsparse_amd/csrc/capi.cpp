@@ -12,7 +12,10 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
 } // namespace mfs
 
-struct mfs_pattern_s { mfs::Pattern P; };
+struct mfs_pattern_s {
+    mfs::Pattern P;
+    mfs::DeviceArray Ap_d, Ai_d;     // filled by mfs_pattern_build_device
+};
 struct mfs_assembler_s { std::unique_ptr<mfs::AssemblerDev> dev; };
 struct mfs_factorizer_s { std::unique_ptr<mfs::FactorizerDev> dev; };
 struct mfs_learned_s { std::unique_ptr<mfs::LearnedDev> dev; };
@@ -50,6 +53,25 @@ int mfs_pattern_build(const int32_t* en, int64_t ne, int32_t npe, int64_t nb, mf
     auto p = std::make_unique<mfs_pattern_s>();
     p->P = mfs::build_block_pattern(en, ne, npe, nb);
     *out = p.release();
+    MFS_CATCH
+}
+
+int mfs_pattern_build_device(const int32_t* en_dev, int64_t ne, int32_t npe, int64_t nb, void* stream,
+                             mfs_pattern* out) {
+    MFS_TRY
+    CHECK_PTR(out);
+    auto h = std::make_unique<mfs_pattern_s>();
+    h->P = mfs::build_block_pattern_device(en_dev, ne, npe, nb, stream, h->Ap_d, h->Ai_d);
+    *out = h.release();
+    MFS_CATCH
+}
+
+int mfs_pattern_get_device(mfs_pattern p, const int64_t** Ap_dev, const int64_t** Ai_dev) {
+    MFS_TRY
+    CHECK_PTR(p); CHECK_PTR(Ap_dev); CHECK_PTR(Ai_dev);
+    if (!p->Ap_d.p) throw mfs::Error(-2, "pattern was built on the host (mfs_pattern_build)");
+    *Ap_dev = p->Ap_d.as<const int64_t>();
+    *Ai_dev = p->Ai_d.as<const int64_t>();
     MFS_CATCH
 }
 

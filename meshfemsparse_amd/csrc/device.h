@@ -23,6 +23,11 @@ struct DeviceArray {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Block sparsity pattern built on the device from device element connectivity (same Ap/Ai as
+// build_block_pattern); the arrays are left in Ap_d / Ai_d and returned on the host.
+Pattern build_block_pattern_device(const int32_t* elem_nodes_dev, int64_t ne, int npe, int64_t nblocks,
+                                   void* stream, DeviceArray& Ap_d, DeviceArray& Ai_d);
+
 class AssemblerDev {
 public:
     AssemblerDev(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);

@@ -137,17 +137,27 @@ class SystemAssembler:
         return 3 * self.nb
 
     def blockSparsityPattern(self, elem_nodes):
-        en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
-        ne, npe = en.shape
+        """SystemAssembler<3>::blockSparsityPattern (SystemAssembler.hh:251-465).  A CUDA int32
+        tensor of element nodes is processed on the device (mfs_pattern_build_device), anything
+        else on the host; both give the same pattern."""
         lib = _lib.load()
         h = c_vp()
-        _lib.check(lib.mfs_pattern_build(ptr(en, _lib.c_i32), ne, npe, self.nb, h))
+        if isinstance(elem_nodes, torch.Tensor) and elem_nodes.is_cuda:
+            en_d = elem_nodes.to(torch.int32).contiguous()
+            ne, npe = en_d.shape
+            s = c_vp(torch.cuda.current_stream(en_d.device).cuda_stream)
+            _lib.check(lib.mfs_pattern_build_device(c_vp(en_d.data_ptr()), ne, npe, self.nb, s, h))
+        else:
+            en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+            ne, npe = en.shape
+            _lib.check(lib.mfs_pattern_build(ptr(en, _lib.c_i32), ne, npe, self.nb, h))
         try:
             nb, nz = c_i64(), c_i64()
             Ap, Ai = P_i64(), P_i64()
             _lib.check(lib.mfs_pattern_get(h, nb, nz, Ap, Ai))
             Apn = np.ctypeslib.as_array(Ap, shape=(self.nb + 1,)).copy()
-            Ain = np.ctypeslib.as_array(Ai, shape=(max(nz.value, 1),))[:nz.value].copy()
+            Ain = (np.ctypeslib.as_array(Ai, shape=(nz.value,)).copy() if nz.value > 0
+                   else np.zeros(0, np.int64))
         finally:
             lib.mfs_pattern_destroy(h)
         return BlockCSCHessian(Apn, Ain)
