@@ -7,7 +7,9 @@ from ._lib import MFSError, NotPositiveDefiniteError, UnsupportedError, load
 from .sparse import BlockCSCHessian, SystemAssembler, DeviceAssembler
 from .solvers import (CholeskyProvider, CholeskySys, FactorizationType, GpuSupernodalFactorizer,
                       make_cholesky_factorizer)
+from .bordered import BorderedSparseHessian, BorderedSparseFactorization
 
 __all__ = ["BlockCSCHessian", "SystemAssembler", "DeviceAssembler", "CholeskyProvider", "CholeskySys",
-           "FactorizationType", "GpuSupernodalFactorizer", "make_cholesky_factorizer", "MFSError",
+           "FactorizationType", "GpuSupernodalFactorizer", "make_cholesky_factorizer", "BorderedSparseHessian",
+           "BorderedSparseFactorization", "MFSError",
            "NotPositiveDefiniteError", "UnsupportedError", "load"]
