@@ -52,24 +52,28 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                 c = (t.flags & GT_BETA0) ? v : c + v;
             }
     };
-    auto extend_one = [&](const ExtendTask& t) {
+    // rows [row0, row1) of the child's column block kj (row1 = 3 dc for a whole column)
+    auto extend_rows = [&](const ExtendTask& t, int64_t row0, int64_t row1) {
         const int32_t* relc = S.rel.data() + t.rel_off;
         const int kj = t.col_block, tj = relc[kj];
         const int64_t Dc = 3 * (int64_t)t.dc;
-        for (int ki = kj; ki < t.dc; ++ki) {
-            const int ti = relc[ki];
+        for (int64_t i = row0; i < row1; ++i) {
+            const int ki = (int)(i / 3), r = (int)(i % 3), ti = relc[ki];
             double* dst;
             int64_t ldd;
-            if (tj < t.wp) { dst = L.data() + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti; ldd = t.ldp; }
+            if (tj < t.wp) { dst = L.data() + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti + r; ldd = t.ldp; }
             else {
                 const int64_t Dp = 3 * (int64_t)t.dp;
-                dst = U.data() + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
+                dst = U.data() + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp) + r;
                 ldd = Dp;
             }
-            const double* src = U.data() + t.u_child + 3 * (int64_t)kj * Dc + 3 * (int64_t)ki;
+            const double* src = U.data() + t.u_child + 3 * (int64_t)kj * Dc + i;
             for (int c = 0; c < 3; ++c)
-                for (int r = (ki == kj ? c : 0); r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
+                if (ki != kj || r >= c) dst[c * ldd] += src[c * Dc];
         }
+    };
+    auto extend_one = [&](const ExtendTask& t) {
+        extend_rows(t, t.row0, std::min<int64_t>(3 * (int64_t)t.dc, (int64_t)t.row0 + kExtendRows));
     };
     auto potrf = [&](const PotrfTask& t) {
         double* D = L.data() + t.diag_off;
@@ -116,8 +120,8 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                     t.dp = f.D / 3;
                     t.rel_off = S.rel_off[c];
                     t.col_block = kj;
-                    t.pad = 0;
-                    extend_one(t);
+                    t.row0 = 3 * kj;
+                    extend_rows(t, 3 * kj, 3 * (int64_t)t.dc);
                 }
             }
         };

@@ -384,36 +384,59 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ tasks,
                                                const int32_t* __restrict__ rel,
                                                T* __restrict__ U, T* __restrict__ L) {
+    // lane = one scalar row of the column block, so every load and store of a wave covers
+    // consecutive addresses (child rows are contiguous; parent rows follow the increasing
+    // relative indices); each parent entry still receives exactly one add per launch
     const ExtendTask t = tasks[blockIdx.x];
     const int32_t* relc = rel + t.rel_off;
     const int kj = t.col_block;
     const int64_t Dc = 3 * (int64_t)t.dc;
     const int tj = relc[kj];
-    const T* uc = U + t.u_child;
-    for (int ki = kj + (int)threadIdx.x; ki < t.dc; ki += 64) {
-        const int ti = relc[ki];
-        T* dst;
-        int64_t ldd;
-        if (tj < t.wp) {
-            dst = L + t.p_panel + (int64_t)(3 * tj) * t.ldp + 3 * ti;
-            ldd = t.ldp;
-        } else {
-            const int64_t Dp = 3 * (int64_t)t.dp;
-            dst = U + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp + 3 * (ti - t.wp);
-            ldd = Dp;
+    const T* src = U + t.u_child + (3 * (int64_t)kj) * Dc;
+    T* dst;
+    int64_t ldd, shift;
+    if (tj < t.wp) {
+        dst = L + t.p_panel + (int64_t)(3 * tj) * t.ldp;
+        ldd = t.ldp;
+        shift = 0;
+    } else {
+        const int64_t Dp = 3 * (int64_t)t.dp;
+        dst = U + t.u_parent + (int64_t)(3 * (tj - t.wp)) * Dp;
+        ldd = Dp;
+        shift = 3 * (int64_t)t.wp;
+    }
+    // UN row chunks per pass: every index and value load of the pass is issued before its
+    // stores (the child and parent matrices share one arena, so the compiler may not move loads
+    // of the next rows above these stores by itself)
+    constexpr int UN = 4;
+    const int64_t row1 = min(Dc, (int64_t)t.row0 + kExtendRows);
+    for (int64_t i0 = t.row0 + (int64_t)threadIdx.x; i0 < row1; i0 += 64 * UN) {
+        int64_t d[UN];
+        int r[UN];
+        bool ok[UN], diag[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const int64_t i = i0 + 64 * u;
+            ok[u] = i < row1;
+            const int ki = ok[u] ? (int)(i / 3) : kj;
+            r[u] = (int)(i - 3 * (int64_t)ki);
+            diag[u] = ki == kj;     // diagonal block: its upper part (r < c) is not stored
+            d[u] = 3 * (int64_t)relc[ki] + r[u] - shift;
         }
-        const T* src = uc + (3 * (int64_t)kj) * Dc + 3 * (int64_t)ki;
-        if (ki == kj) {
+        T sv[UN][3], dv[UN][3];
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const bool on = ok[u] && (!diag[u] || r[u] >= c);
+                sv[u][c] = on ? src[c * Dc + i0 + 64 * u] : T(0);
+                dv[u][c] = on ? dst[c * ldd + d[u]] : T(0);
+            }
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = c; r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) dst[c * ldd + r] += src[c * Dc + r];
-        }
+                if (ok[u] && (!diag[u] || r[u] >= c)) dst[c * ldd + d[u]] = dv[u][c] + sv[u][c];
     }
 }
 

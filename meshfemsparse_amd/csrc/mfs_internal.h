@@ -91,8 +91,10 @@ struct ExtendTask {         // extend-add of child's update matrix into the pare
     int32_t dp;             // parent degree (blocks)
     int64_t rel_off;        // offset of child's relative indices (dc entries) in rel array
     int32_t col_block;      // column block kj of the child's U handled by this task
-    int32_t pad;
+    int32_t row0;           // first scalar row of the column block handled by this task: the
+                            // task covers rows [row0, min(3 dc, row0 + kExtendRows))
 };
+constexpr int32_t kExtendRows = 768;   // rows per extend-add task (a multiple of 3 and of 64)
 
 // Supernodal solves work on column blocks of at most `solve_block` columns of a supernode.  Each
 // block's diagonal part is inverted explicitly after the numeric factorization (X = L_bb^{-1},

@@ -603,8 +603,11 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     t.dp = S.sdeg[p];
                     t.rel_off = S.rel_off[c];
                     t.col_block = kj;
-                    t.pad = 0;
-                    S.extend_tasks.push_back(t);
+                    // long columns are split into row chunks (disjoint parent entries)
+                    for (int32_t r0 = 3 * kj; r0 < 3 * S.sdeg[c]; r0 += kExtendRows) {
+                        t.row0 = r0;
+                        S.extend_tasks.push_back(t);
+                    }
                 }
             }
             if ((int64_t)S.extend_tasks.size() > b0) passes.push_back({b0, (int64_t)S.extend_tasks.size()});
