@@ -132,6 +132,8 @@ private:
     void* graph_exec_ = nullptr;
     void* cstream_ = nullptr;
     void* join_[2] = {nullptr, nullptr};
+    void* side_ = nullptr;                 // lookahead stream (default priority)
+    std::vector<void*> la_ev_;             // its fork/join events
     const double* gkey_ax_ = nullptr;
     const double* gkey_bx_ = nullptr;
     double gkey_sigma_ = 0.0;

@@ -159,6 +159,7 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                 potrf(S.potrf_tasks[q]);
             for (int64_t q = P.trsm_steps[st].b; q < P.trsm_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
             for (int64_t q = P.update_steps[st].b; q < P.update_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
+            for (int64_t q = P.rest_steps[st].b; q < P.rest_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
         }
         for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) gemm(S.gemm_tasks[q]);
         for (const auto& pass : P.extend_u_passes)

@@ -1836,7 +1836,9 @@ FactorizerDev::~FactorizerDev() {
     drop_graph();
     for (void* e : join_)
         if (e) (void)hipEventDestroy((hipEvent_t)e);
+    for (void* e : la_ev_) (void)hipEventDestroy((hipEvent_t)e);
     if (cstream_) (void)hipStreamDestroy((hipStream_t)cstream_);
+    if (side_) (void)hipStreamDestroy((hipStream_t)side_);
 }
 
 void FactorizerDev::require(int t) const {
@@ -1951,14 +1953,32 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     const GemmTask* gt = gemm_.as<GemmTask>();
     // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
     // tile kernel; TRSM and the diagonal tiles stay in the factor type
-    auto schur_gemm = [&](const TaskRange& r) {
+    auto schur_gemm = [&](const TaskRange& r, hipStream_t st) {
         if constexpr (sizeof(T) == 4) {
             if (precision >= 2) {
-                launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3, s);
+                launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3, st);
                 return;
             }
         }
-        launch_gemm<T>(r, gt, L, U, inv, s);
+        launch_gemm<T>(r, gt, L, U, inv, st);
+    };
+    // lookahead: the rest of an outer trailing update runs on the side stream after the step's
+    // TRSM; the stream waits for it before the next outer trailing update or the Schur update,
+    // which write the same columns (so every entry still receives its updates in block order)
+    hipStream_t side = nullptr;
+    size_t nev = 0;
+    hipEvent_t pending = nullptr;
+    auto event = [&]() {
+        if (nev == la_ev_.size()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            la_ev_.push_back(e);
+        }
+        return (hipEvent_t)la_ev_[nev++];
+    };
+    auto join = [&]() {
+        if (pending) HIP_CHECK(hipStreamWaitEvent(s, pending, 0));
+        pending = nullptr;
     };
     auto extend = [&](const std::vector<std::pair<int64_t,int64_t>>& passes) {
         for (auto [a, b] : passes)
@@ -2002,15 +2022,34 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
                 prof.end(s);
             }
             const TaskRange& ur = P.update_steps[t];
+            const TaskRange& rr = P.rest_steps[t];
+            if (P.trailing_step[t]) join();
+            if (rr.e > rr.b) {
+                if (!side_) {
+                    hipStream_t ss;
+                    HIP_CHECK(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
+                    side_ = ss;
+                }
+                side = (hipStream_t)side_;
+                hipEvent_t e = event();
+                HIP_CHECK(hipEventRecord(e, s));
+                HIP_CHECK(hipStreamWaitEvent(side, e, 0));
+                prof.begin(KernelProfile::UPDATE, side, prof.on ? gemm_work(S, rr) : 0.0, rr.e - rr.b);
+                schur_gemm(rr, side);
+                prof.end(side);
+                pending = event();
+                HIP_CHECK(hipEventRecord(pending, side));
+            }
             if (ur.e > ur.b) {
                 prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b);
-                schur_gemm(ur);
+                schur_gemm(ur, s);
                 prof.end(s);
             }
         }
+        join();
         if (P.syrk.e > P.syrk.b) {
             prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
-            schur_gemm(P.syrk);
+            schur_gemm(P.syrk, s);
             prof.end(s);
         }
         extend(P.extend_u_passes);

@@ -165,6 +165,9 @@ struct LevelPlan {
     std::vector<std::pair<int64_t,int64_t>> panel_steps;     // per step: range in panel items (empty:
                                                              // separate POTRF and TRSM launches)
     std::vector<TaskRange> update_steps;                     // per step: gemm tasks
+    std::vector<TaskRange> rest_steps;                       // per step: trailing-update tasks that
+                                                             // may overlap the next outer block
+    std::vector<char> trailing_step;                         // per step: 1 if it closes an outer block
     TaskRange syrk;                                          // gemm tasks
     // solves: gather tasks (forward: children -> x/u; backward: x -> u), and per column-block
     // step the diagonal tiles and the row tasks (backward steps in execution order)
@@ -252,6 +255,8 @@ struct SymbolicOptions {
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
     int fuse_degree = 0;    // supernodes of width <= one tile and at most this many structure rows
                             // (scalars) go to the fused level kernel; 0 disables it
+    bool lookahead = true;  // split outer trailing updates: the next outer block's columns
+                            // first, the rest beside the next outer block's steps
     int panel_max = 256;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
@@ -261,8 +266,8 @@ struct SymbolicOptions {
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
 // (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
-// 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound); values are rounded down to
-// multiples of 64 where required.
+// 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
+// of the outer trailing updates); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,

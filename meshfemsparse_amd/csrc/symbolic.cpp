@@ -146,6 +146,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
     if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
+    if (const char* e = getenv("MFS_LOOKAHEAD")) opt.lookahead = atoi(e) != 0;
     return opt;
 }
 
@@ -694,8 +695,12 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             }
             {   // after TRSM(t): the next step's column block gets the left-looking update from the
                 // outer block's finished columns (K = k1 - J0), or, when step t closes its outer
-                // block, all later columns get the outer trailing update (K = outer)
-                std::vector<GemmTask> big, small;
+                // block, all later columns get the outer trailing update (K = outer).  With
+                // lookahead the trailing update is split: the next outer block's columns stay in
+                // the step (they are next on the critical path), the columns after them form the
+                // step's rest range, which may run beside the next outer block's steps.
+                std::vector<GemmTask> big, small, rbig, rsmall;
+                bool trailing = false;
                 for (auto [s, slot] : active) {
                     (void)slot;
                     const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
@@ -707,11 +712,18 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                         emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
                                    k1, k1 + kb1, W + D, k1 - J0, 0, true);
                     } else if (Jend < W) {
+                        trailing = true;
+                        const int32_t split = opt.lookahead ? std::min(W, Jend + OB) : W;
                         emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
-                                   Jend, W, W + D, Jend - J0, 0, true);
+                                   Jend, split, W + D, Jend - J0, 0, true);
+                        if (split < W)
+                            emit_lower(rbig, rsmall, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off,
+                                       ld, split, W, W + D, Jend - J0, 0, true);
                     }
                 }
                 P.update_steps.push_back(push_tasks(S.gemm_tasks, big, small));
+                P.rest_steps.push_back(push_tasks(S.gemm_tasks, rbig, rsmall));
+                P.trailing_step.push_back(trailing ? 1 : 0);
             }
         }
         // Schur update U -= L21 L21^T (lower tiles), K = W
@@ -829,6 +841,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     for (const LevelPlan& P : S.levels) {
         for (const TaskRange& r : P.trsm_steps) check_range(r);
         for (const TaskRange& r : P.update_steps) check_range(r);
+        for (const TaskRange& r : P.rest_steps) check_range(r);
         check_range(P.syrk);
     }
     // fused panel items: each TRSM task solves with its own POTRF task's tile, fits one 64-row

@@ -130,6 +130,33 @@ def test_fused_panel_steps_match_separate_launches(dev, N, panel, single, monkey
         assert np.array_equal(orc.dense_L(*f1.exportFactorCSC()), L1)
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("outer", [128, 192])
+def test_lookahead_matches_in_order_schedule(dev, outer, single, monkeypatch):
+    """With lookahead the rest of every outer trailing update runs on a second stream beside the
+    next outer block's steps; each entry still receives its updates in block order, so the factor
+    is bitwise the in-order schedule's, eagerly and replayed as a captured graph."""
+    H, pins, _, _, _ = mesh_problem(dev, 10)
+    monkeypatch.setenv("MFS_OUTER", str(outer))
+
+    def factor(la, graph=False):
+        monkeypatch.setenv("MFS_LOOKAHEAD", str(la))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+        f.factorizeSymbolic(H, pins)
+        if graph:
+            f.setGraphCapture(True)
+        f.factorizeNumeric(H)
+        L1 = orc.dense_L(*f.exportFactorCSC())
+        if graph:
+            f.factorizeNumeric(H)          # a replay of the captured graph
+            assert np.array_equal(orc.dense_L(*f.exportFactorCSC()), L1)
+        return L1
+
+    L0 = factor(0)
+    assert np.array_equal(factor(1), L0)
+    assert np.array_equal(factor(1, graph=True), L0)
+
+
 def test_fused_panel_step_reports_failing_pivot(dev, monkeypatch):
     """-A through the fused steps: the pivot failure is recorded once by the storing workgroup
     and raised like in the separate launches."""
