@@ -748,46 +748,39 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
     const int ar = tid % TM, ak = tid / TM, br = tid % TN, bk = tid / TN;
     const int64_t lda = t.lda, ldb = t.ldb;
     const bool arow = ar < t.M, brow = br < t.N;
-    // running operand pointers (rows beyond M / N read row 0 and are zeroed by a select); only
-    // a final partial chunk checks k
-    const T* pa = A + (arow ? ar : 0) + (int64_t)ak * lda;
-    const T* pb = B + (brow ? br : 0) + (int64_t)bk * ldb;
+    // Loads only fill registers: rows beyond M / N read row 0 and k beyond K reads column K - 1
+    // (always in bounds), and the masks are applied when the chunk is stored to LDS after the
+    // current chunk's MFMAs, so the loads stay in flight under them.
+    const T* Ac = A + (arow ? ar : 0);
+    const T* Bc = B + (brow ? br : 0);
+    const T* pa = Ac + (int64_t)ak * lda;
+    const T* pb = Bc + (int64_t)bk * ldb;
     T ra[AL], rb[BL];
     auto gload = [&](int k0) {
         if (k0 + KC <= t.K) {
 #pragma unroll
-            for (int q = 0; q < AL; ++q) {
-                const T v = pa[(int64_t)(AST * q) * lda];
-                ra[q] = arow ? v : T(0);
-            }
+            for (int q = 0; q < AL; ++q) ra[q] = pa[(int64_t)(AST * q) * lda];
 #pragma unroll
-            for (int q = 0; q < BL; ++q) {
-                const T v = pb[(int64_t)(BST * q) * ldb];
-                rb[q] = brow ? v : T(0);
-            }
+            for (int q = 0; q < BL; ++q) rb[q] = pb[(int64_t)(BST * q) * ldb];
         } else {
 #pragma unroll
-            for (int q = 0; q < AL; ++q) {
-                const int kk = k0 + ak + AST * q;
-                ra[q] = (arow && kk < t.K) ? pa[(int64_t)(AST * q) * lda] : T(0);
-            }
+            for (int q = 0; q < AL; ++q) ra[q] = Ac[(int64_t)min(k0 + ak + AST * q, t.K - 1) * lda];
 #pragma unroll
-            for (int q = 0; q < BL; ++q) {
-                const int kk = k0 + bk + BST * q;
-                rb[q] = (brow && kk < t.K) ? pb[(int64_t)(BST * q) * ldb] : T(0);
-            }
+            for (int q = 0; q < BL; ++q) rb[q] = Bc[(int64_t)min(k0 + bk + BST * q, t.K - 1) * ldb];
         }
         pa += KC * lda;
         pb += KC * ldb;
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int k0) {
 #pragma unroll
-        for (int q = 0; q < AL; ++q) As[buf][ak + AST * q][ar] = ra[q];
+        for (int q = 0; q < AL; ++q)
+            As[buf][ak + AST * q][ar] = (arow && k0 + ak + AST * q < t.K) ? ra[q] : T(0);
 #pragma unroll
-        for (int q = 0; q < BL; ++q) Bs[buf][bk + BST * q][br] = rb[q];
+        for (int q = 0; q < BL; ++q)
+            Bs[buf][bk + BST * q][br] = (brow && k0 + bk + BST * q < t.K) ? rb[q] : T(0);
     };
     gload(0);
-    sstore(0);
+    sstore(0, 0);
     __syncthreads();
     int buf = 0;
     for (int k0 = 0; k0 < t.K; k0 += KC) {
@@ -806,7 +799,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
         }
-        if (more) sstore(buf ^ 1);
+        if (more) sstore(buf ^ 1, k0 + KC);
         __syncthreads();
         buf ^= 1;
     }
@@ -1032,11 +1025,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ t
     // loader: thread -> row tid % TM, k groups tid / TM + (256 / TM) q.  Rows beyond M (N) read
     // row 0 and are zeroed by a select; running pointers advance by KC columns per chunk; only a
     // final partial chunk checks k.
+    // loads only fill registers (row 0 for rows beyond M / N, column K - 1 for k beyond K); the
+    // masks are applied when the chunk is stored to LDS, after the current chunk's MFMAs
     const int ar = tid % TM, br = tid % TN;
     const int akg = tid / TM, bkg = tid / TN;
     const bool arow = ar < t.M, brow = br < t.N;
-    const T* pa = A + (arow ? ar : 0) + (int64_t)(4 * akg) * lda;
-    const T* pb = B + (brow ? br : 0) + (int64_t)(4 * bkg) * ldb;
+    const T* Ac = A + (arow ? ar : 0);
+    const T* Bc = B + (brow ? br : 0);
+    const T* pa = Ac + (int64_t)(4 * akg) * lda;
+    const T* pb = Bc + (int64_t)(4 * bkg) * ldb;
     const int64_t aq = (int64_t)(4 * (256 / TM)) * lda, bq = (int64_t)(4 * (256 / TN)) * ldb;
     v4 ra[AG], rb[BG];
     auto gload = [&](int k0) {
@@ -1044,44 +1041,48 @@ __global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ t
 #pragma unroll
             for (int q = 0; q < AG; ++q)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const T v = pa[q * aq + u * lda];
-                    ra[q][u] = arow ? v : T(0);
-                }
+                for (int u = 0; u < 4; ++u) ra[q][u] = pa[q * aq + u * lda];
 #pragma unroll
             for (int q = 0; q < BG; ++q)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const T v = pb[q * bq + u * ldb];
-                    rb[q][u] = brow ? v : T(0);
-                }
+                for (int u = 0; u < 4; ++u) rb[q][u] = pb[q * bq + u * ldb];
         } else {
 #pragma unroll
             for (int q = 0; q < AG; ++q) {
                 const int kk = k0 + 4 * (akg + (256 / TM) * q);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) ra[q][u] = (arow && kk + u < t.K) ? pa[q * aq + u * lda] : T(0);
+                for (int u = 0; u < 4; ++u) ra[q][u] = Ac[(int64_t)min(kk + u, t.K - 1) * lda];
             }
 #pragma unroll
             for (int q = 0; q < BG; ++q) {
                 const int kk = k0 + 4 * (bkg + (256 / TN) * q);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) rb[q][u] = (brow && kk + u < t.K) ? pb[q * bq + u * ldb] : T(0);
+                for (int u = 0; u < 4; ++u) rb[q][u] = Bc[(int64_t)min(kk + u, t.K - 1) * ldb];
             }
         }
         pa += KC * lda;
         pb += KC * ldb;
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int k0) {
 #pragma unroll
-        for (int q = 0; q < AG; ++q)
-            *(v4*)&As[(buf * TM + ar) * LDK + 4 * (akg + (256 / TM) * q)] = ra[q];
+        for (int q = 0; q < AG; ++q) {
+            const int kk = k0 + 4 * (akg + (256 / TM) * q);
+            v4 v;
 #pragma unroll
-        for (int q = 0; q < BG; ++q)
-            *(v4*)&Bs[(buf * TN + br) * LDK + 4 * (bkg + (256 / TN) * q)] = rb[q];
+            for (int u = 0; u < 4; ++u) v[u] = (arow && kk + u < t.K) ? ra[q][u] : T(0);
+            *(v4*)&As[(buf * TM + ar) * LDK + 4 * (akg + (256 / TM) * q)] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < BG; ++q) {
+            const int kk = k0 + 4 * (bkg + (256 / TN) * q);
+            v4 v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = (brow && kk + u < t.K) ? rb[q][u] : T(0);
+            *(v4*)&Bs[(buf * TN + br) * LDK + 4 * (bkg + (256 / TN) * q)] = v;
+        }
     };
     gload(0);
-    sstore(0);
+    sstore(0, 0);
     __syncthreads();
     int buf = 0;
     const int fr = lane & 15, fg = 4 * (lane >> 4);
@@ -1102,7 +1103,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm2(const GemmTask* __restrict__ t
 #pragma unroll
                     for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i][s], bf[j][s], acc[i][j]);
         }
-        if (more) sstore(buf ^ 1);
+        if (more) sstore(buf ^ 1, k0 + KC);
         __syncthreads();
         buf ^= 1;
     }
@@ -1276,26 +1277,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     const int ar = tid % TM, br = tid % TN, ak0 = (tid / TM) * AKE, bk0 = (tid / TN) * BKE;
     const bool arow = ar < t.M, brow = br < t.N;
-    const float* pa = A + (arow ? ar : 0) + (int64_t)ak0 * lda;
-    const float* pb = B + (brow ? br : 0) + (int64_t)bk0 * ldb;
+    // loads only fill registers (row 0 beyond M / N, column K - 1 beyond K); masks at the store
+    const float* Ac = A + (arow ? ar : 0);
+    const float* Bc = B + (brow ? br : 0);
+    const float* pa = Ac + (int64_t)ak0 * lda;
+    const float* pb = Bc + (int64_t)bk0 * ldb;
     float ra[AKE], rb[BKE];
     auto gload = [&](int k0) {
         if (k0 + KC <= t.K) {
 #pragma unroll
-            for (int u = 0; u < AKE; ++u) {
-                const float v = pa[u * lda];
-                ra[u] = arow ? v : 0.0f;
-            }
+            for (int u = 0; u < AKE; ++u) ra[u] = pa[u * lda];
 #pragma unroll
-            for (int u = 0; u < BKE; ++u) {
-                const float v = pb[u * ldb];
-                rb[u] = brow ? v : 0.0f;
-            }
+            for (int u = 0; u < BKE; ++u) rb[u] = pb[u * ldb];
         } else {
 #pragma unroll
-            for (int u = 0; u < AKE; ++u) ra[u] = (arow && k0 + ak0 + u < t.K) ? pa[u * lda] : 0.0f;
+            for (int u = 0; u < AKE; ++u) ra[u] = Ac[(int64_t)min(k0 + ak0 + u, t.K - 1) * lda];
 #pragma unroll
-            for (int u = 0; u < BKE; ++u) rb[u] = (brow && k0 + bk0 + u < t.K) ? pb[u * ldb] : 0.0f;
+            for (int u = 0; u < BKE; ++u) rb[u] = Bc[(int64_t)min(k0 + bk0 + u, t.K - 1) * ldb];
         }
         pa += KC * lda;
         pb += KC * ldb;
@@ -1315,13 +1313,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
             }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int k0) {
         __bf16* st = smem + buf * STAGE;
-        put(st, ar, ak0, ra, AKE);
-        put(st + TM * LDK, br, bk0, rb, BKE);
+        float ma[AKE], mb[BKE];
+#pragma unroll
+        for (int u = 0; u < AKE; ++u) ma[u] = (arow && k0 + ak0 + u < t.K) ? ra[u] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < BKE; ++u) mb[u] = (brow && k0 + bk0 + u < t.K) ? rb[u] : 0.0f;
+        put(st, ar, ak0, ma, AKE);
+        put(st + TM * LDK, br, bk0, mb, BKE);
     };
     gload(0);
-    sstore(0);
+    sstore(0, 0);
     __syncthreads();
     int buf = 0;
     const int fr = lane & 15, fk = 8 * (lane >> 4);
@@ -1354,7 +1357,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
 #pragma unroll
             for (int j = 0; j < FN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        if (more) sstore(buf ^ 1);
+        if (more) sstore(buf ^ 1, k0 + KC);
         __syncthreads();
         buf ^= 1;
     }
