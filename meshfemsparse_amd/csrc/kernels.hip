@@ -1155,7 +1155,7 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     return w;
 }
 
-// tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 2 for FP32, 1 for FP64) for the
+// tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 1) for the
 // 128x128 tiles and MFS_GEMM_V64 for the 64x64 tiles (default: as MFS_GEMM_V), MFS_GEMM_KC2 and
 // MFS_GEMM_KC64 (k_gemm2's K chunk for the 128x128 and the 64x64 tiles: 16, 32 or 64; default 16),
 // MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (k_gemm's FP32 K chunk,
@@ -1215,8 +1215,9 @@ static int gemm_kc32() {
 template <typename T>
 static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
     const int rm = gemm_remap();
-    // default: the k-contiguous kernel for FP32, the k-major kernel for FP64 (measured best)
-    const int ver = gemm_version() ? gemm_version() : (sizeof(T) == 4 ? 2 : 1);
+    // default: the k-major kernel for both precisions (measured best once its prefetch loads
+    // stay in flight under the MFMAs; the k-contiguous kernel remains selectable)
+    const int ver = gemm_version() ? gemm_version() : 1;
     const int ver64 = gemm_version64() ? gemm_version64() : ver;
     const bool k32 = sizeof(T) == 4 && gemm_kc32();
     if (r.m > r.b) {
