@@ -2028,7 +2028,13 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
             const TaskRange& ur = P.update_steps[t];
             const TaskRange& rr = P.rest_steps[t];
             if (P.trailing_step[t]) join();
-            if (rr.e > rr.b) {
+            if (rr.e > rr.b && prof.on) {
+                // per-kernel timing: the rest runs in order on the stream, so the measured launch
+                // durations are not stretched by kernels sharing the device
+                prof.begin(KernelProfile::UPDATE, s, gemm_work(S, rr), rr.e - rr.b);
+                schur_gemm(rr, s);
+                prof.end(s);
+            } else if (rr.e > rr.b) {
                 if (!side_) {
                     hipStream_t ss;
                     HIP_CHECK(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
