@@ -1558,8 +1558,16 @@ __global__ __launch_bounds__(256) void k_sfdiag(const SolveTile* __restrict__ ti
     const T* Xr = Xb + b.xb_off + r0 + lane;
     T acc = T(0);
     if (lane < nr) {
-        // batches of 8 independent loads (columns wave, wave + 4, ...) keep the memory pipe full
+        // batches of 32 independent loads (columns wave, wave + 4, ...) per memory round trip;
+        // the products are summed in column order as in a plain loop
         int c = wave;
+        for (; c + 124 < nc; c += 128) {
+            T v[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) v[q] = Xr[(int64_t)(c + 4 * q) * b.nb];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) acc += v[q] * z[c + 4 * q];
+        }
         for (; c + 28 < nc; c += 32) {
             T v[8];
 #pragma unroll
@@ -1591,7 +1599,15 @@ __global__ __launch_bounds__(256) void k_sfupd(const SolveRowTask* __restrict__ 
     const T* P = L + t.l_off + lane;
     T acc = T(0);
     if (lane < t.nr) {
+        // 32 independent loads per memory round trip, products summed in column order
         int c = wave;
+        for (; c + 124 < t.nc; c += 128) {
+            T v[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) v[q] = P[(int64_t)(c + 4 * q) * t.ld];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) acc += v[q] * yv[c + 4 * q];
+        }
         for (; c + 28 < t.nc; c += 32) {
             T v[8];
 #pragma unroll
@@ -1629,16 +1645,29 @@ __global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__
         xv[m] = r < t.nr ? src[r] : T(0);
     }
     T* part = partall + blockIdx.y * ldp + t.d_off;
-    for (int c = wave; c < t.nc; c += 4) {
-        const T* P = L + t.l_off + (int64_t)c * t.ld + lane;
-        T v[SOLVE_RC / 64];
+    if (t.nc < 1) return;
+    // a wave's columns (at most 4 of the task's <= 16) are loaded in one round trip; rows beyond
+    // nr read row 0 of the column (in bounds) and are zeroed by the product with xv = 0
+    constexpr int MR = SOLVE_RC / 64;
+    T v[4][MR];
 #pragma unroll
-        for (int m = 0; m < SOLVE_RC / 64; ++m) v[m] = (lane + 64 * m < t.nr) ? P[64 * m] : T(0);
+    for (int j = 0; j < 4; ++j) {
+        const int c = wave + 4 * j;
+        const T* P = L + t.l_off + (int64_t)min(c, t.nc - 1) * t.ld;
+#pragma unroll
+        for (int m = 0; m < MR; ++m) {
+            const int r = lane + 64 * m;
+            v[j][m] = P[r < t.nr ? r : 0];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = wave + 4 * j;
         T acc = T(0);
 #pragma unroll
-        for (int m = 0; m < SOLVE_RC / 64; ++m) acc += v[m] * xv[m];
+        for (int m = 0; m < MR; ++m) acc += v[j][m] * xv[m];
         acc = wave_sum(acc);
-        if (lane == 0) part[c] = acc;
+        if (lane == 0 && c < t.nc) part[c] = acc;
     }
 }
 
@@ -1665,16 +1694,23 @@ __global__ __launch_bounds__(256) void k_sbdiag(const SolveTile* __restrict__ ti
         z[r] = v;
     }
     __syncthreads();
-    for (int c = wave; c < ncol; c += 4) {
-        const T* Xc = Xb + b.xb_off + (int64_t)(c0 + c) * b.nb + c0;
-        T v[16];
+    // two columns per memory round trip; rows beyond nz read row 0 (in bounds), weighted by 0
+    for (int c = wave; c < ncol; c += 8) {
+        T v[2][16];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = (lane + 64 * m < nz) ? Xc[lane + 64 * m] : T(0);
-        T acc = T(0);
+        for (int j = 0; j < 2; ++j) {
+            const T* Xc = Xb + b.xb_off + (int64_t)(c0 + min(c + 4 * j, ncol - 1)) * b.nb + c0;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) acc += v[m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
-        acc = wave_sum(acc);
-        if (lane == 0) x[c0 + c] = acc;
+            for (int m = 0; m < 16; ++m) v[j][m] = Xc[lane + 64 * m < nz ? lane + 64 * m : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            T acc = T(0);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc += v[j][m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
+            acc = wave_sum(acc);
+            if (lane == 0 && c + 4 * j < ncol) x[c0 + c + 4 * j] = acc;
+        }
     }
 }
 
