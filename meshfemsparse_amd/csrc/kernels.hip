@@ -1489,16 +1489,36 @@ __global__ __launch_bounds__(256) void k_blkinv(const SolveTile* __restrict__ ti
             for (int fj = 0; fj < 2; ++fj)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[fi][fj][q] = T(0);
+        // A = L_im, B = X_mj (X_mj of m < i were stored by this workgroup before).  The tiles
+        // of step m + 1 are loaded into registers (indices clamped in bounds) while step m's
+        // MFMAs run, and masked when stored to LDS.
+        // thread: row r = lane of the tile, columns k = wave + 4 q
+        constexpr int PQ = TS * TS / 256;
+        T ra[PQ], rb[PQ];
+        const T* arow_p = Lb + TS * i + min(lane, ri - 1);
+        auto gl = [&](int m) {
+            const int cm = min(TS, nb - TS * m);
+            const T* a = arow_p + (int64_t)(TS * m) * ld;
+            const T* bcol = X + (int64_t)(TS * j) * nb + TS * m + min(lane, cm - 1);
+#pragma unroll
+            for (int q = 0; q < PQ; ++q) {
+                const int k = wave + 4 * q;
+                ra[q] = a[(int64_t)min(k, cm - 1) * ld];
+                rb[q] = bcol[(int64_t)min(k, cj - 1) * nb];
+            }
+        };
+        gl(j);
         for (int m = j; m < i; ++m) {
             const int cm = min(TS, nb - TS * m);
             __syncthreads();
-            for (int idx = tid; idx < TS * TS; idx += 256) {
-                const int r = idx & 63, k = idx >> 6;      // A = L_im
-                As[k][r] = (r < ri && k < cm) ? Lb[(int64_t)(TS * m + k) * ld + TS * i + r] : T(0);
-                const int kk = idx & 63, c = idx >> 6;     // B = X_mj
-                Bs[kk][c] = (kk < cm && c < cj) ? X[(int64_t)(TS * j + c) * nb + TS * m + kk] : T(0);
+#pragma unroll
+            for (int q = 0; q < PQ; ++q) {
+                const int k = wave + 4 * q;
+                As[k][lane] = (lane < ri && k < cm) ? ra[q] : T(0);      // A(r, k) = L_im
+                Bs[lane][k] = (lane < cm && k < cj) ? rb[q] : T(0);      // B(kk = r, c = k) = X_mj
             }
             __syncthreads();
+            if (m + 1 < i) gl(m + 1);
             mma_tile(acc);
         }
         __syncthreads();
