@@ -726,15 +726,33 @@ __device__ __forceinline__ void gemm_store_acc(const typename Mfma<T>::acc_t (&a
     }
 }
 
-// One output tile of a task (256 threads; As/Bs: double-buffered operand chunks in LDS).
-template <typename T, int TM, int TN, int KC>
+// Raw buffer loads (resource word 3 for gfx9: 32-bit data format, no swizzle, stride 0).
+constexpr int kBufWord3 = 0x00020000;
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// One output tile of a task (256 threads; As/Bs: double-buffered operand chunks in LDS, rows padded
+// by PAD elements).  The operand loads are raw buffer loads: per K chunk one buffer resource based at
+// column k0 of the operand (uniform, built by scalar instructions), a per-thread 32-bit row offset
+// and a uniform per-load column offset, so the loop carries no per-load vector address arithmetic.
+// Rows beyond M (N) read row 0 and only feed output rows (columns) that are never stored, so full
+// chunks are staged unmasked; the one partial chunk clamps k to K - 1 and zeroes k >= K at the
+// LDS store.  FP32 pads the LDS rows by 16 words (the 4 k rows one MFMA step reads then fall in
+// different banks).  The loads of the next chunk stay in flight under the current chunk's MFMAs.
+template <typename T> constexpr int gemm_pad() { return sizeof(T) == 4 ? 16 : 0; }
+template <typename T, int TM, int TN, int KC, int PAD = gemm_pad<T>()>
 __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, T* __restrict__ U,
-                                          const T* __restrict__ inv, T (*__restrict__ As)[KC][TM],
-                                          T (*__restrict__ Bs)[KC][TN]) {
+                                            const T* __restrict__ inv, T (*__restrict__ As)[KC][TM + PAD],
+                                            T (*__restrict__ Bs)[KC][TN + PAD]) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int AST = 256 / TM, BST = 256 / TN;           // k stride of the loaders
-    constexpr int AL = KC / AST, BL = KC / BST;              // loads per thread per chunk
+    constexpr int AST = 256 / TM, BST = 256 / TN;
+    constexpr int AL = KC / AST, BL = KC / BST;
     const T* A = L + t.a_off;
     const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
     T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
@@ -746,45 +764,50 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     const int ar = tid % TM, ak = tid / TM, br = tid % TN, bk = tid / TN;
-    const int64_t lda = t.lda, ldb = t.ldb;
-    const bool arow = ar < t.M, brow = br < t.N;
-    // Loads only fill registers: rows beyond M / N read row 0 and k beyond K reads column K - 1
-    // (always in bounds), and the masks are applied when the chunk is stored to LDS after the
-    // current chunk's MFMAs, so the loads stay in flight under them.
-    const T* Ac = A + (arow ? ar : 0);
-    const T* Bc = B + (brow ? br : 0);
-    const T* pa = Ac + (int64_t)ak * lda;
-    const T* pb = Bc + (int64_t)bk * ldb;
+    const int lda = t.lda, ldb = t.ldb, K = t.K;
+    const int arow = ar < t.M ? ar : 0, brow = br < t.N ? br : 0;
+    const int va = (arow + ak * lda) * (int)sizeof(T), vb = (brow + bk * ldb) * (int)sizeof(T);
+    const int ea = KC * lda * (int)sizeof(T), eb = KC * ldb * (int)sizeof(T);   // chunk extents
     T ra[AL], rb[BL];
     auto gload = [&](int k0) {
-        if (k0 + KC <= t.K) {
+        const __amdgpu_buffer_rsrc_t rA =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(A + (int64_t)k0 * lda), (short)0, ea, kBufWord3);
+        const __amdgpu_buffer_rsrc_t rB =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(B + (int64_t)k0 * ldb), (short)0, eb, kBufWord3);
+        if (k0 + KC <= K) {
 #pragma unroll
-            for (int q = 0; q < AL; ++q) ra[q] = pa[(int64_t)(AST * q) * lda];
+            for (int q = 0; q < AL; ++q) ra[q] = buf_load<T>(rA, va, AST * q * lda * (int)sizeof(T));
 #pragma unroll
-            for (int q = 0; q < BL; ++q) rb[q] = pb[(int64_t)(BST * q) * ldb];
+            for (int q = 0; q < BL; ++q) rb[q] = buf_load<T>(rB, vb, BST * q * ldb * (int)sizeof(T));
         } else {
+            const int kl = K - 1 - k0;
 #pragma unroll
-            for (int q = 0; q < AL; ++q) ra[q] = Ac[(int64_t)min(k0 + ak + AST * q, t.K - 1) * lda];
+            for (int q = 0; q < AL; ++q)
+                ra[q] = buf_load<T>(rA, (arow + min(ak + AST * q, kl) * lda) * (int)sizeof(T), 0);
 #pragma unroll
-            for (int q = 0; q < BL; ++q) rb[q] = Bc[(int64_t)min(k0 + bk + BST * q, t.K - 1) * ldb];
+            for (int q = 0; q < BL; ++q)
+                rb[q] = buf_load<T>(rB, (brow + min(bk + BST * q, kl) * ldb) * (int)sizeof(T), 0);
         }
-        pa += KC * lda;
-        pb += KC * ldb;
     };
     auto sstore = [&](int buf, int k0) {
+        if (k0 + KC <= K) {
 #pragma unroll
-        for (int q = 0; q < AL; ++q)
-            As[buf][ak + AST * q][ar] = (arow && k0 + ak + AST * q < t.K) ? ra[q] : T(0);
+            for (int q = 0; q < AL; ++q) As[buf][ak + AST * q][ar] = ra[q];
 #pragma unroll
-        for (int q = 0; q < BL; ++q)
-            Bs[buf][bk + BST * q][br] = (brow && k0 + bk + BST * q < t.K) ? rb[q] : T(0);
+            for (int q = 0; q < BL; ++q) Bs[buf][bk + BST * q][br] = rb[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < AL; ++q) As[buf][ak + AST * q][ar] = (k0 + ak + AST * q < K) ? ra[q] : T(0);
+#pragma unroll
+            for (int q = 0; q < BL; ++q) Bs[buf][bk + BST * q][br] = (k0 + bk + BST * q < K) ? rb[q] : T(0);
+        }
     };
     gload(0);
     sstore(0, 0);
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < t.K; k0 += KC) {
-        const bool more = k0 + KC < t.K;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const bool more = k0 + KC < K;
         if (more) gload(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
@@ -813,8 +836,9 @@ template <typename T, int TM, int TN, int KC>
 __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
                                                  T* __restrict__ L, T* __restrict__ U,
                                                  const T* __restrict__ inv, int remap) {
-    __shared__ T As[2][KC][TM];
-    __shared__ T Bs[2][KC][TN];
+    constexpr int PAD = gemm_pad<T>();
+    __shared__ T As[2][KC][TM + PAD];
+    __shared__ T Bs[2][KC][TN + PAD];
     gemm_tile<T, TM, TN, KC>(tasks[xcd_task(blockIdx.x, gridDim.x, remap)], L, U, inv, As, Bs);
 }
 
@@ -932,8 +956,10 @@ __global__ __launch_bounds__(256, 2) void k_fused(const FusedTask* __restrict__ 
     __shared__ __attribute__((aligned(16))) T smem[2 * TS * (TS + 1)];
     T (*a)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem);
     T (*x)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem + TS * (TS + 1));
-    T (*As)[KC][TS] = reinterpret_cast<T (*)[KC][TS]>(smem);
-    T (*Bs)[KC][TS] = reinterpret_cast<T (*)[KC][TS]>(smem + 2 * KC * TS);
+    constexpr int PAD = gemm_pad<T>();
+    static_assert(4 * KC * (TS + PAD) <= 2 * TS * (TS + 1), "tile GEMM chunks fit the fused kernel's LDS");
+    T (*As)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem);
+    T (*Bs)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem + 2 * KC * (TS + PAD));
     for (int item = blockIdx.x; item < ntasks; item += gridDim.x) {
         const FusedTask f = tasks[item];
         const int32_t ld = f.W + f.D;
@@ -1158,8 +1184,7 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
 // tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 1) for the
 // 128x128 tiles and MFS_GEMM_V64 for the 64x64 tiles (default: as MFS_GEMM_V), MFS_GEMM_KC2 and
 // MFS_GEMM_KC64 (k_gemm2's K chunk for the 128x128 and the 64x64 tiles: 16, 32 or 64; default 16),
-// MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0), MFS_GEMM_KC (k_gemm's FP32 K chunk,
-// 16 or 32, default 16; FP64 uses 16 so that two workgroups share a CU's LDS)
+// MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0)
 static int gemm_remap() {
     static const int v = getenv("MFS_GEMM_REMAP") ? atoi(getenv("MFS_GEMM_REMAP")) : 0;
     return v;
@@ -1208,38 +1233,26 @@ static void launch_gemm2(int kc, unsigned grid, const GemmTask* tasks, T* L, T* 
     k_gemm2<T, TM, TM, 16><<<grid, 256, 0, s>>>(tasks, L, U, inv, rm);
     LAUNCH_CHECK();
 }
-static int gemm_kc32() {
-    static const int v = getenv("MFS_GEMM_KC") ? (atoi(getenv("MFS_GEMM_KC")) >= 32) : 0;
-    return v;
-}
 template <typename T>
 static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
     const int rm = gemm_remap();
-    // default: the k-major kernel for both precisions (measured best once its prefetch loads
-    // stay in flight under the MFMAs; the k-contiguous kernel remains selectable)
+    // default: the k-major buffer-load kernel for both precisions (the k-contiguous kernel remains
+    // selectable)
     const int ver = gemm_version() ? gemm_version() : 1;
     const int ver64 = gemm_version64() ? gemm_version64() : ver;
-    const bool k32 = sizeof(T) == 4 && gemm_kc32();
     if (r.m > r.b) {
         if (ver == 2) launch_gemm2<T, 128>(gemm_kc2(), (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, s);
         else {
-            if constexpr (sizeof(T) == 4) {
-                if (k32) k_gemm<T, 128, 128, 32><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-                else k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-            } else {
-                k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-            }
+            k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
             LAUNCH_CHECK();
         }
     }
-    if (r.e > r.m && ver64 == 2) {
-        launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
-        return;
-    }
     if (r.e > r.m) {
-        if (k32) k_gemm<T, 64, 64, 32><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
-        else k_gemm<T, 64, 64, 16><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
-        LAUNCH_CHECK();
+        if (ver64 == 2) launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
+        else {
+            k_gemm<T, 64, 64, 16><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+            LAUNCH_CHECK();
+        }
     }
 }
 
