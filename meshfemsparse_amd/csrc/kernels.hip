@@ -1283,34 +1283,37 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
     float* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
-    const int64_t lda = t.lda, ldb = t.ldb;
+    const int lda = t.lda, ldb = t.ldb, K = t.K;
     acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     const int ar = tid % TM, br = tid % TN, ak0 = (tid / TM) * AKE, bk0 = (tid / TN) * BKE;
-    const bool arow = ar < t.M, brow = br < t.N;
-    // loads only fill registers (row 0 beyond M / N, column K - 1 beyond K); masks at the store
-    const float* Ac = A + (arow ? ar : 0);
-    const float* Bc = B + (brow ? br : 0);
-    const float* pa = Ac + (int64_t)ak0 * lda;
-    const float* pb = Bc + (int64_t)bk0 * ldb;
+    // Raw buffer loads as in gemm_tile: per chunk a resource based at column k0, a per-thread row
+    // offset and uniform per-load column offsets.  Rows beyond M / N read row 0 (they only feed
+    // outputs that are never stored); the partial chunk clamps k to K - 1 and zeroes k >= K.
+    const int arow = ar < t.M ? ar : 0, brow = br < t.N ? br : 0;
+    const int va = (arow + ak0 * lda) * 4, vb = (brow + bk0 * ldb) * 4;
+    const int ea = KC * lda * 4, eb = KC * ldb * 4;
     float ra[AKE], rb[BKE];
     auto gload = [&](int k0) {
-        if (k0 + KC <= t.K) {
+        const __amdgpu_buffer_rsrc_t rA =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(A + (int64_t)k0 * lda), (short)0, ea, kBufWord3);
+        const __amdgpu_buffer_rsrc_t rB =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(B + (int64_t)k0 * ldb), (short)0, eb, kBufWord3);
+        if (k0 + KC <= K) {
 #pragma unroll
-            for (int u = 0; u < AKE; ++u) ra[u] = pa[u * lda];
+            for (int u = 0; u < AKE; ++u) ra[u] = buf_load<float>(rA, va, u * lda * 4);
 #pragma unroll
-            for (int u = 0; u < BKE; ++u) rb[u] = pb[u * ldb];
+            for (int u = 0; u < BKE; ++u) rb[u] = buf_load<float>(rB, vb, u * ldb * 4);
         } else {
+            const int kl = K - 1 - k0;
 #pragma unroll
-            for (int u = 0; u < AKE; ++u) ra[u] = Ac[(int64_t)min(k0 + ak0 + u, t.K - 1) * lda];
+            for (int u = 0; u < AKE; ++u) ra[u] = buf_load<float>(rA, (arow + min(ak0 + u, kl) * lda) * 4, 0);
 #pragma unroll
-            for (int u = 0; u < BKE; ++u) rb[u] = Bc[(int64_t)min(k0 + bk0 + u, t.K - 1) * ldb];
+            for (int u = 0; u < BKE; ++u) rb[u] = buf_load<float>(rB, (brow + min(bk0 + u, kl) * ldb) * 4, 0);
         }
-        pa += KC * lda;
-        pb += KC * ldb;
     };
     auto put = [&](__bf16* base, int row, int k0_, const float* v, int n) {
 #pragma unroll
@@ -1329,21 +1332,26 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
     };
     auto sstore = [&](int buf, int k0) {
         __bf16* st = smem + buf * STAGE;
-        float ma[AKE], mb[BKE];
+        if (k0 + KC <= K) {
+            put(st, ar, ak0, ra, AKE);
+            put(st + TM * LDK, br, bk0, rb, BKE);
+        } else {
+            float ma[AKE], mb[BKE];
 #pragma unroll
-        for (int u = 0; u < AKE; ++u) ma[u] = (arow && k0 + ak0 + u < t.K) ? ra[u] : 0.0f;
+            for (int u = 0; u < AKE; ++u) ma[u] = (k0 + ak0 + u < K) ? ra[u] : 0.0f;
 #pragma unroll
-        for (int u = 0; u < BKE; ++u) mb[u] = (brow && k0 + bk0 + u < t.K) ? rb[u] : 0.0f;
-        put(st, ar, ak0, ma, AKE);
-        put(st + TM * LDK, br, bk0, mb, BKE);
+            for (int u = 0; u < BKE; ++u) mb[u] = (k0 + bk0 + u < K) ? rb[u] : 0.0f;
+            put(st, ar, ak0, ma, AKE);
+            put(st + TM * LDK, br, bk0, mb, BKE);
+        }
     };
     gload(0);
     sstore(0, 0);
     __syncthreads();
     int buf = 0;
     const int fr = lane & 15, fk = 8 * (lane >> 4);
-    for (int k0 = 0; k0 < t.K; k0 += KC) {
-        const bool more = k0 + KC < t.K;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const bool more = k0 + KC < K;
         if (more) gload(k0 + KC);
         const __bf16* st = smem + buf * STAGE;
         bf8 ah[FM], bh[FN];
