@@ -57,6 +57,7 @@ typedef struct mfs_pattern_s*    mfs_pattern;
 typedef struct mfs_assembler_s*  mfs_assembler;
 typedef struct mfs_factorizer_s* mfs_factorizer;
 typedef struct mfs_learned_s*    mfs_learned;
+typedef struct mfs_nd_s*         mfs_nd;
 
 const char* mfs_last_error(void);
 int         mfs_version(void);
@@ -221,6 +222,39 @@ int mfs_clear_stashed_factorization(mfs_factorizer f);
  * levels, max supernode width and max row count below a supernode (scalars).               */
 int mfs_supernode_stats(mfs_factorizer f, int64_t* num_supernodes, int64_t* num_levels,
                         int64_t* max_width, int64_t* max_degree);
+
+/* Separator tree of the most recent nested-dissection analysis (CholeskyFactorizerBase.hh:51-59,
+ * NDOrdering {CParent, CMember, blockSize}): components are postordered (CParent[c] > c, or -1
+ * for a root); CMember[v] is the component of reduced ordering variable v (unpermuted reduced
+ * block: pins removed), block_size = 3.  Query the sizes with CParent = CMember = NULL.
+ * MFS_ERR_STATE when no symbolic analysis exists (or its ordering was not a dissection).      */
+int mfs_nd_ordering(mfs_factorizer f, int64_t* num_components, int64_t* num_vars, int64_t* block_size,
+                    int64_t* CParent, int64_t* CMember);
+
+/* Temporal reuse of the separator tree between symbolic analyses (CatamariFactorizer.hh:84-90;
+ * ParallelNestedDissection/README.md:86-127): with period P > 0 the next P analyses repair only
+ * the subtrees whose separators the new pattern violates, then the tree is rebuilt.  P = 0
+ * (default) disables reuse; a new period or different pins restart the history.
+ * stats4 = {full_rebuild (0/1), graph_vertices, recomputed_subtrees, repartitioned_vertices}
+ * of the last analysis, nd_seconds its ordering time (cholmod_nesdis_parallel.hh:33-39).      */
+int mfs_set_temporal_reuse_period(mfs_factorizer f, int64_t period);
+int mfs_get_temporal_reuse_period(mfs_factorizer f, int64_t* period);
+int mfs_reset_temporal_reuse(mfs_factorizer f);
+int mfs_temporal_reuse_statistics(mfs_factorizer f, int64_t* stats4, double* nd_seconds);
+
+/* Host nested dissection of a symmetric graph with its own reuse history, as the factorizer
+ * uses it (nested_dissection_from_graph + TemporalReuseState, cholmod_nesdis_parallel.hh:41-75).
+ * Graph: CSC of both triangles (p[n+1], i[p[n]]; diagonal entries ignored); vertex ids must keep
+ * their meaning between calls.  mfs_nd_order writes perm[new] = old and the number of
+ * components; mfs_nd_forest then gives CParent[num_components] and CMember[n] of that call.   */
+int mfs_nd_create(mfs_nd* out);
+int mfs_nd_destroy(mfs_nd h);
+int mfs_nd_set_temporal_reuse_period(mfs_nd h, int64_t period);
+int mfs_nd_reset(mfs_nd h);
+int mfs_nd_order(mfs_nd h, int64_t n, const int64_t* p, const int64_t* i, int64_t* perm,
+                 int64_t* num_components);
+int mfs_nd_forest(mfs_nd h, int64_t* CParent, int64_t* CMember);
+int mfs_nd_statistics(mfs_nd h, int64_t* stats4, double* nd_seconds);
 
 /* Host-only symbolic analysis (no device work): the ordering and supernode statistics the
  * factorizer would use.  iperm_blocks[n_reduced_blocks] = reduced block at factor position k;

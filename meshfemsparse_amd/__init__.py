@@ -8,8 +8,9 @@ from .sparse import BlockCSCHessian, SystemAssembler, DeviceAssembler
 from .solvers import (CholeskyProvider, CholeskySys, FactorizationType, GpuSupernodalFactorizer,
                       make_cholesky_factorizer)
 from .bordered import BorderedSparseHessian, BorderedSparseFactorization
+from .ordering import NDOrdering, NestedDissection, TemporalReuseStatistics
 
 __all__ = ["BlockCSCHessian", "SystemAssembler", "DeviceAssembler", "CholeskyProvider", "CholeskySys",
            "FactorizationType", "GpuSupernodalFactorizer", "make_cholesky_factorizer", "BorderedSparseHessian",
-           "BorderedSparseFactorization", "MFSError",
+           "BorderedSparseFactorization", "NDOrdering", "NestedDissection", "TemporalReuseStatistics", "MFSError",
            "NotPositiveDefiniteError", "UnsupportedError", "load"]

@@ -76,6 +76,18 @@ _SIGS = {
     "mfs_supernode_stats": [c_vp, P_i64, P_i64, P_i64, P_i64],
     "mfs_factor_export_csc": [c_vp, P_i64, P_i64, P_dbl],
     "mfs_analyze_host": [P_i64, P_i64, c_i64, P_i64, c_i64, P_i64, P_i64, P_dbl],
+    "mfs_nd_ordering": [c_vp, P_i64, P_i64, P_i64, P_i64, P_i64],
+    "mfs_set_temporal_reuse_period": [c_vp, c_i64],
+    "mfs_get_temporal_reuse_period": [c_vp, P_i64],
+    "mfs_reset_temporal_reuse": [c_vp],
+    "mfs_temporal_reuse_statistics": [c_vp, P_i64, P_dbl],
+    "mfs_nd_create": [ctypes.POINTER(c_vp)],
+    "mfs_nd_destroy": [c_vp],
+    "mfs_nd_set_temporal_reuse_period": [c_vp, c_i64],
+    "mfs_nd_reset": [c_vp],
+    "mfs_nd_order": [c_vp, c_i64, P_i64, P_i64, P_i64, P_i64],
+    "mfs_nd_forest": [c_vp, P_i64, P_i64],
+    "mfs_nd_statistics": [c_vp, P_i64, P_dbl],
     "mfs_debug_emulate_factor": [P_i64, P_i64, c_i64, P_i64, c_i64, P_dbl, c_dbl, P_i64, P_i64, P_dbl, P_i64,
                                  P_dbl],
     "mfs_last_timings": [c_vp, P_dbl],

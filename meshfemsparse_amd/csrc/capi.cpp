@@ -400,6 +400,138 @@ int mfs_supernode_stats(mfs_factorizer f, int64_t* ns, int64_t* nl, int64_t* mw,
     MFS_CATCH
 }
 
+int mfs_nd_ordering(mfs_factorizer f, int64_t* ncomp, int64_t* nvars, int64_t* block_size, int64_t* CParent,
+                    int64_t* CMember) {
+    MFS_TRY
+    CHECK_PTR(f);
+    const mfs::Symbolic* S = f->dev->symbolic_state();
+    if (!S || !S->has_nd_forest) throw mfs::Error(MFS_ERR_STATE, "no nested-dissection ordering available");
+    const mfs::SepForest& F = S->nd_forest;
+    if (ncomp) *ncomp = (int64_t)F.parent.size();
+    if (nvars) *nvars = (int64_t)F.member.size();
+    if (block_size) *block_size = 3;
+    if (CParent)
+        for (size_t c = 0; c < F.parent.size(); ++c) CParent[c] = F.parent[c];
+    if (CMember)
+        for (size_t v = 0; v < F.member.size(); ++v) CMember[v] = F.member[v];
+    MFS_CATCH
+}
+
+static void reuse_stats_out(const mfs::NDReuseStats& st, int64_t* s4, double* secs) {
+    if (s4) {
+        s4[0] = st.full_rebuild ? 1 : 0;
+        s4[1] = st.graph_vertices;
+        s4[2] = st.recomputed_subtrees;
+        s4[3] = st.repartitioned_vertices;
+    }
+    if (secs) *secs = st.nd_seconds;
+}
+
+int mfs_set_temporal_reuse_period(mfs_factorizer f, int64_t period) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->set_reuse_period(period);
+    MFS_CATCH
+}
+
+int mfs_get_temporal_reuse_period(mfs_factorizer f, int64_t* period) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(period);
+    *period = f->dev->reuse_period();
+    MFS_CATCH
+}
+
+int mfs_reset_temporal_reuse(mfs_factorizer f) {
+    MFS_TRY
+    CHECK_PTR(f);
+    f->dev->reset_reuse();
+    MFS_CATCH
+}
+
+int mfs_temporal_reuse_statistics(mfs_factorizer f, int64_t* stats4, double* nd_seconds) {
+    MFS_TRY
+    CHECK_PTR(f);
+    reuse_stats_out(f->dev->reuse_stats(), stats4, nd_seconds);
+    MFS_CATCH
+}
+
+struct mfs_nd_s {
+    mfs::NDState state;
+};
+
+int mfs_nd_create(mfs_nd* out) {
+    MFS_TRY
+    CHECK_PTR(out);
+    *out = new mfs_nd_s();
+    MFS_CATCH
+}
+
+int mfs_nd_destroy(mfs_nd h) {
+    delete h;
+    return MFS_OK;
+}
+
+int mfs_nd_set_temporal_reuse_period(mfs_nd h, int64_t period) {
+    MFS_TRY
+    CHECK_PTR(h);
+    if (period < 0) throw mfs::Error(MFS_ERR_ARG, "the temporal reuse period must be >= 0");
+    if (period != h->state.period) h->state.reset();
+    h->state.period = period;
+    MFS_CATCH
+}
+
+int mfs_nd_reset(mfs_nd h) {
+    MFS_TRY
+    CHECK_PTR(h);
+    h->state.reset();
+    MFS_CATCH
+}
+
+int mfs_nd_order(mfs_nd h, int64_t n, const int64_t* p, const int64_t* i, int64_t* perm, int64_t* ncomp) {
+    MFS_TRY
+    CHECK_PTR(h);
+    if (n < 0 || n > INT32_MAX) throw mfs::Error(MFS_ERR_ARG, "bad graph size");
+    if (n > 0) { CHECK_PTR(p); CHECK_PTR(i); CHECK_PTR(perm); }
+    std::vector<int64_t> xadj(n + 1, 0);
+    std::vector<int32_t> adj;
+    if (n > 0) {
+        if (p[0] != 0) throw mfs::Error(MFS_ERR_ARG, "p[0] must be 0");
+        adj.reserve(p[n]);
+        for (int64_t v = 0; v < n; ++v) {
+            if (p[v + 1] < p[v]) throw mfs::Error(MFS_ERR_ARG, "column pointers must not decrease");
+            for (int64_t k = p[v]; k < p[v + 1]; ++k) {
+                const int64_t u = i[k];
+                if (u < 0 || u >= n) throw mfs::Error(MFS_ERR_ARG, "row index out of range");
+                if (u != v) adj.push_back((int32_t)u);
+            }
+            xadj[v + 1] = (int64_t)adj.size();
+        }
+    }
+    mfs::NDOptions opt;
+    const std::vector<int32_t> pr = mfs::nested_dissection((int32_t)n, xadj, adj, opt, &h->state);
+    for (int64_t k = 0; k < n; ++k) perm[k] = pr[k];
+    if (ncomp) *ncomp = (int64_t)h->state.forest.parent.size();
+    MFS_CATCH
+}
+
+int mfs_nd_forest(mfs_nd h, int64_t* CParent, int64_t* CMember) {
+    MFS_TRY
+    CHECK_PTR(h);
+    const mfs::SepForest& F = h->state.forest;
+    if (CParent)
+        for (size_t c = 0; c < F.parent.size(); ++c) CParent[c] = F.parent[c];
+    if (CMember)
+        for (size_t v = 0; v < F.member.size(); ++v) CMember[v] = F.member[v];
+    MFS_CATCH
+}
+
+int mfs_nd_statistics(mfs_nd h, int64_t* stats4, double* nd_seconds) {
+    MFS_TRY
+    CHECK_PTR(h);
+    reuse_stats_out(h->state.stats, stats4, nd_seconds);
+    MFS_CATCH
+}
+
 int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
                      int64_t npins, int64_t* iperm, int64_t* stats, double* flops) {
     MFS_TRY

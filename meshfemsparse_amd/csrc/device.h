@@ -94,6 +94,12 @@ public:
                   const double* Ax, const int* fuse, int nfuse, const int* tiles, int ntile, int reps, double* ms,
                   int* best, void* stream);
     void clear_factors();
+    // temporal reuse of the nested-dissection separator forest between symbolic analyses
+    void set_reuse_period(int64_t period);
+    int64_t reuse_period() const { return nd_state_.period; }
+    void reset_reuse() { nd_state_.reset(); }
+    const NDReuseStats& reuse_stats() const { return nd_state_.stats; }
+    const Symbolic* symbolic_state() const { return ftype >= 1 ? sym.get() : nullptr; }
     void stash();
     void swap_stash();
     void clear_stash();
@@ -127,6 +133,9 @@ private:
     DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
     DeviceArray Lstash_, invstash_, scalestash_, xbstash_;
     bool stash_valid_ = false;
+    NDState nd_state_;              // ordering history (temporal reuse)
+    std::vector<int64_t> nd_pins_;  // sorted pins of the history
+    NDState* ordering_state(const int64_t* pins, int64_t npins);
     bool graph_on_ = false;
     void* graph_ = nullptr;
     void* graph_exec_ = nullptr;

@@ -1934,7 +1934,25 @@ size_t FactorizerDev::fsize() const { return precision != 0 ? sizeof(float) : si
 void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs,
                              const int64_t* pins, int64_t npins) {
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
-    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, options_from_env())));
+    SymbolicOptions o = options_from_env();
+    o.nd_state = ordering_state(pins, npins);
+    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
+}
+
+// The ordering history is kept across analyses with the same pinned variables
+// (CatamariFactorizer.hh:236-244: a change of the pins restarts it).
+NDState* FactorizerDev::ordering_state(const int64_t* pins, int64_t npins) {
+    std::vector<int64_t> sp(pins, pins + npins);
+    std::sort(sp.begin(), sp.end());
+    if (sp != nd_pins_) nd_state_.reset();
+    nd_pins_ = std::move(sp);
+    return &nd_state_;
+}
+
+void FactorizerDev::set_reuse_period(int64_t period) {
+    if (period < 0) throw Error(-1, "the temporal reuse period must be >= 0");
+    if (period != nd_state_.period) nd_state_.reset();
+    nd_state_.period = period;
 }
 
 // Upload a symbolic analysis (task lists, maps) and size the factor storage for it.
@@ -2544,10 +2562,15 @@ void FactorizerDev::autotune(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     if (nfuse < 1 || ntile < 1 || reps < 1) throw Error(-1, "autotune needs at least one candidate and one repetition");
     SymbolicOptions base = options_from_env();
     std::vector<int32_t> perm;
+    SepForest forest;
     {
-        Symbolic S0 = analyze(Ap, Ai, nb, pins, npins, base);   // the ordering is shared by all candidates
+        SymbolicOptions o0 = base;
+        o0.nd_state = ordering_state(pins, npins);
+        Symbolic S0 = analyze(Ap, Ai, nb, pins, npins, o0);     // the ordering is shared by all candidates
         perm = nd_order(S0);
+        forest = std::move(S0.nd_forest);
     }
+    base.given_forest = &forest;
     hipStream_t s = S_(stream);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));

@@ -50,8 +50,33 @@ struct NDOptions {
     double imbalance = 0.06;
     uint64_t seed = 0x5eed;
 };
+// Separator forest of a dissection, postordered (children before parents, parent[c] > c or -1):
+// component c owns verts[c] (a separator or a leaf, in ordering order); member[v] = component
+// of vertex v; the ordering is the postorder traversal (children in order, then own vertices).
+struct SepForest {
+    std::vector<int32_t> parent, roots, member;
+    std::vector<std::vector<int32_t>> children, verts;
+};
+struct NDReuseStats {
+    bool full_rebuild = false;
+    int64_t graph_vertices = 0, recomputed_subtrees = 0, repartitioned_vertices = 0;
+    double nd_seconds = 0;
+};
+// History of one ordering stream (vertex ids must keep their meaning between calls).
+struct NDState {
+    int64_t period = 0;            // 0: every call dissects from scratch
+    int64_t since_rebuild = 0;     // incremental analyses since the last full dissection
+    SepForest forest;              // forest of the last call
+    NDReuseStats stats;            // statistics of the last call
+    void reset() { since_rebuild = 0; forest = SepForest(); stats = NDReuseStats(); }
+};
 std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
-                                       const std::vector<int32_t>& adj, const NDOptions& opt);
+                                       const std::vector<int32_t>& adj, const NDOptions& opt,
+                                       NDState* state);
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
+                                       const std::vector<int32_t>& adj, const NDOptions& opt,
+                                       SepForest* forest = nullptr);
+std::vector<int32_t> forest_order(const SepForest& f);
 
 // ------------------------------------------------------------------------------------------
 // Symbolic analysis result (all indices in *block* units unless noted "scalar").
@@ -187,6 +212,8 @@ struct Symbolic {
     // ordering: pos[reduced block] = factor position; iperm[pos] = reduced block
     std::vector<int32_t> pos, iperm;
     std::vector<int32_t> nd_perm;              // the fill-reducing ordering before the postorder
+    SepForest nd_forest;                       // its separator forest (over reduced blocks)
+    bool has_nd_forest = false;
     // supernodes (factor order)
     int32_t ns = 0;
     std::vector<int32_t> sfirst, swidth, sdeg, sparent, slevel;    // blocks
@@ -260,6 +287,8 @@ struct SymbolicOptions {
     int panel_max = 256;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
+    const SepForest* given_forest = nullptr;            // ... and its separator forest, if any
+    NDState* nd_state = nullptr;                        // ordering history (temporal reuse)
 };
 
 // Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <numeric>
 #include <queue>
 #include <thread>
@@ -388,13 +389,29 @@ int32_t components(const G& g, std::vector<int32_t>& comp) {
     return nc;
 }
 
-void nd_recurse(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
-                const NDOptions& opt);
+// Separator forest under construction: components get ids as they are closed, so children
+// always precede their parent (postorder).  A component owns either the vertices of a leaf or
+// the vertices of a separator; disconnected parts add no component of their own.
+struct ForestBuilder {
+    std::vector<int32_t> parent;
+    std::vector<std::vector<int32_t>> children, verts;
+    int32_t add(std::vector<int32_t> own, std::vector<int32_t> kids) {
+        const int32_t id = (int32_t)parent.size();
+        for (int32_t c : kids) parent[c] = id;
+        parent.push_back(-1);
+        children.push_back(std::move(kids));
+        verts.push_back(std::move(own));
+        return id;
+    }
+};
+
+std::vector<int32_t> nd_recurse(const G& g, const std::vector<int32_t>& gid, Rng& rng, const NDOptions& opt,
+                                ForestBuilder& fb);
 
 // Components are ordered one after another (each by its own dissection): no separator is
-// needed between parts that share no edge.
-bool nd_components(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
-                   const NDOptions& opt) {
+// needed between parts that share no edge.  Returns false for a connected graph.
+bool nd_components(const G& g, const std::vector<int32_t>& gid, Rng& rng, const NDOptions& opt,
+                   ForestBuilder& fb, std::vector<int32_t>& roots) {
     std::vector<int32_t> comp;
     const int32_t nc = components(g, comp);
     if (nc <= 1) return false;
@@ -405,73 +422,137 @@ bool nd_components(const G& g, const std::vector<int32_t>& gid, std::vector<int3
         G s = induced(g, where, 0, local, members);
         std::vector<int32_t> sg(members.size());
         for (size_t i = 0; i < members.size(); ++i) sg[i] = gid[members[i]];
-        nd_recurse(s, sg, out, rng, opt);
+        const std::vector<int32_t> r = nd_recurse(s, sg, rng, opt, fb);
+        roots.insert(roots.end(), r.begin(), r.end());
     }
     return true;
 }
 
-void nd_recurse(const G& g, const std::vector<int32_t>& gid, std::vector<int32_t>& out, Rng& rng,
-                const NDOptions& opt) {
-    if (g.n <= opt.leaf_size) {
-        out.insert(out.end(), gid.begin(), gid.end());
-        return;
-    }
-    if (nd_components(g, gid, out, rng, opt)) return;
+// Dissects g (vertex v is gid[v] globally); returns the roots of its separator forest in order.
+std::vector<int32_t> nd_recurse(const G& g, const std::vector<int32_t>& gid, Rng& rng, const NDOptions& opt,
+                                ForestBuilder& fb) {
+    if (g.n == 0) return {};
+    if (g.n <= opt.leaf_size) return {fb.add(gid, {})};
+    std::vector<int32_t> roots;
+    if (nd_components(g, gid, rng, opt, fb, roots)) return roots;
     std::vector<int8_t> where = bisect(g, rng, opt);
     int32_t cnt[3] = {0, 0, 0};
     for (int8_t w : where) cnt[w]++;
-    if (cnt[0] == 0 || cnt[1] == 0) {   // could not split (e.g. clique-like): leaf
-        out.insert(out.end(), gid.begin(), gid.end());
-        return;
-    }
-    std::vector<int32_t> local(g.n, -1), members;
+    if (cnt[0] == 0 || cnt[1] == 0) return {fb.add(gid, {})};    // could not split: leaf
+    std::vector<int32_t> local(g.n, -1), members, kids;
     for (int8_t part = 0; part < 2; ++part) {
         G s = induced(g, where, part, local, members);
         std::vector<int32_t> sg(members.size());
         for (size_t i = 0; i < members.size(); ++i) sg[i] = gid[members[i]];
-        nd_recurse(s, sg, out, rng, opt);
+        const std::vector<int32_t> r = nd_recurse(s, sg, rng, opt, fb);
+        kids.insert(kids.end(), r.begin(), r.end());
     }
+    std::vector<int32_t> sep;
     for (int32_t v = 0; v < g.n; ++v)
-        if (where[v] == 2) out.push_back(gid[v]);
+        if (where[v] == 2) sep.push_back(gid[v]);
+    if (sep.empty()) return kids;
+    return {fb.add(std::move(sep), std::move(kids))};
 }
 
-} // namespace
+// Dissection of the subgraph induced by `verts` (global ids, in this order) as a forest whose
+// vertex lists hold global ids.  Every call starts from the same seed.
+ForestBuilder dissect_region(int32_t n, const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
+                             const std::vector<int32_t>& verts, const NDOptions& opt,
+                             std::vector<int32_t>& roots) {
+    std::vector<int32_t> local(n, -1);
+    for (size_t i = 0; i < verts.size(); ++i) local[verts[i]] = (int32_t)i;
+    G s;
+    s.n = (int32_t)verts.size();
+    s.xadj.assign(s.n + 1, 0);
+    s.vw.assign(s.n, 1);
+    for (int32_t i = 0; i < s.n; ++i) {
+        const int32_t v = verts[i];
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e)
+            if (local[adj[e]] >= 0 && adj[e] != v) { s.adj.push_back(local[adj[e]]); s.ew.push_back(1); }
+        s.xadj[i + 1] = (int64_t)s.adj.size();
+    }
+    ForestBuilder fb;
+    Rng rng{opt.seed};
+    roots = nd_recurse(s, verts, rng, opt, fb);
+    return fb;
+}
 
-std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
-                                       const std::vector<int32_t>& adj, const NDOptions& opt) {
+// Appends the ordering of the subtree of c (children first, in order, then c's own vertices).
+void flatten(const SepForest& f, int32_t c, std::vector<int32_t>& out) {
+    std::vector<std::pair<int32_t, size_t>> st{{c, 0}};
+    while (!st.empty()) {
+        auto& [node, k] = st.back();
+        if (k < f.children[node].size()) {
+            const int32_t ch = f.children[node][k++];
+            st.push_back({ch, 0});
+        } else {
+            out.insert(out.end(), f.verts[node].begin(), f.verts[node].end());
+            st.pop_back();
+        }
+    }
+}
+
+// Renumbers the live components reachable from the roots in postorder and rebuilds member.
+SepForest compact(const SepForest& f, int32_t n) {
+    SepForest o;
+    std::vector<int32_t> id(f.parent.size(), -1);
+    std::vector<std::pair<int32_t, size_t>> st;
+    for (int32_t r : f.roots) {
+        st.push_back({r, 0});
+        while (!st.empty()) {
+            auto& [node, k] = st.back();
+            if (k < f.children[node].size()) {
+                const int32_t ch = f.children[node][k++];
+                st.push_back({ch, 0});
+            } else {
+                id[node] = (int32_t)o.parent.size();
+                o.parent.push_back(-1);
+                o.verts.push_back(f.verts[node]);
+                std::vector<int32_t> kids;
+                for (int32_t ch : f.children[node]) kids.push_back(id[ch]);
+                for (int32_t ch : kids) o.parent[ch] = id[node];
+                o.children.push_back(std::move(kids));
+                st.pop_back();
+            }
+        }
+        o.roots.push_back(id[r]);
+    }
+    o.member.assign(n, -1);
+    for (size_t c = 0; c < o.verts.size(); ++c)
+        for (int32_t v : o.verts[c]) o.member[v] = (int32_t)c;
+    for (int32_t v = 0; v < n; ++v)
+        if (o.member[v] < 0) throw Error(-1, "separator forest does not cover every vertex");
+    return o;
+}
+
+// Appends the builder's components to f (ids offset by the current size); returns the offset.
+int32_t append(SepForest& f, const ForestBuilder& fb) {
+    const int32_t off = (int32_t)f.parent.size();
+    for (size_t c = 0; c < fb.parent.size(); ++c) {
+        f.parent.push_back(fb.parent[c] < 0 ? -1 : fb.parent[c] + off);
+        std::vector<int32_t> kids = fb.children[c];
+        for (int32_t& k : kids) k += off;
+        f.children.push_back(std::move(kids));
+        f.verts.push_back(fb.verts[c]);
+    }
+    return off;
+}
+
+// Full dissection: connected components in parallel (each from the same seed, as it would be
+// ordered on its own), concatenated in component order.
+SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
+                          const NDOptions& opt) {
     G g;
     g.n = n;
     g.xadj = xadj;
     g.adj = adj;
-    g.ew.assign(adj.size(), 1);
-    g.vw.assign(n, 1);
-    std::vector<int32_t> out;
-    out.reserve(n);
-    // top level: the connected components (independent meshes of a batch) are dissected in
-    // parallel, each from the same seed, and concatenated in component order
     std::vector<int32_t> comp;
     const int32_t nc = components(g, comp);
     std::vector<std::vector<int32_t>> gids(nc);
-    std::vector<int32_t> local(n);
-    for (int32_t v = 0; v < n; ++v) {
-        local[v] = (int32_t)gids[comp[v]].size();
-        gids[comp[v]].push_back(v);
-    }
-    std::vector<std::vector<int32_t>> outs(nc);
-    auto work = [&](int32_t c) {
-        const std::vector<int32_t>& mem = gids[c];
-        G s;
-        s.n = (int32_t)mem.size();
-        s.xadj.assign(s.n + 1, 0);
-        s.vw.assign(s.n, 1);
-        for (int32_t i = 0; i < s.n; ++i) {
-            const int32_t v = mem[i];
-            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) { s.adj.push_back(local[g.adj[e]]); s.ew.push_back(1); }
-            s.xadj[i + 1] = (int64_t)s.adj.size();
-        }
-        Rng rng{opt.seed};   // every component ordered exactly as it would be on its own
-        nd_recurse(s, mem, outs[c], rng, opt);
-    };
+    for (int32_t v = 0; v < n; ++v) gids[comp[v]].push_back(v);
+    std::vector<ForestBuilder> fbs(nc);
+    std::vector<std::vector<int32_t>> roots(nc);
+    auto work = [&](int32_t c) { fbs[c] = dissect_region(n, xadj, adj, gids[c], opt, roots[c]); };
     const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
     if (nc > 1 && nt > 1) {
         std::atomic<int32_t> next{0};
@@ -484,8 +565,111 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xa
     } else {
         for (int32_t c = 0; c < nc; ++c) work(c);
     }
-    for (int32_t c = 0; c < nc; ++c) out.insert(out.end(), outs[c].begin(), outs[c].end());
+    SepForest f;
+    for (int32_t c = 0; c < nc; ++c) {
+        const int32_t off = append(f, fbs[c]);
+        for (int32_t r : roots[c]) f.roots.push_back(r + off);
+    }
+    return compact(f, n);
+}
+
+} // namespace
+
+std::vector<int32_t> forest_order(const SepForest& f) {
+    std::vector<int32_t> out;
+    for (int32_t r : f.roots) flatten(f, r, out);
     return out;
+}
+
+// Temporal reuse (ParallelNestedDissection/persistent_separator_tree.hh:81-113 and
+// cholmod_nesdis_parallel.cc:1391-1506): an edge whose end components are not on one root path
+// violates the separator of their lowest common ancestor; the outermost violated components are
+// re-dissected on the current graph and spliced in place, everything else keeps its ordering.
+// An edge between two trees of the forest, a violation at the sole root, no (or a mismatching)
+// previous forest, or `period` incremental analyses since the last full one: full dissection.
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
+                                       const std::vector<int32_t>& adj, const NDOptions& opt,
+                                       NDState* state) {
+    const auto t0 = std::chrono::steady_clock::now();
+    NDReuseStats st;
+    st.graph_vertices = n;
+    SepForest next;
+    bool full = !state || state->period <= 0 || state->forest.member.size() != (size_t)n ||
+                state->since_rebuild >= state->period || n == 0;
+    std::vector<int32_t> dirty_roots;
+    if (!full) {
+        const SepForest& f = state->forest;
+        const int32_t nc = (int32_t)f.parent.size();
+        std::vector<int32_t> depth(nc, 0);
+        for (int32_t c = nc - 1; c >= 0; --c)          // parents have larger ids
+            depth[c] = f.parent[c] < 0 ? 0 : depth[f.parent[c]] + 1;
+        auto lca = [&](int32_t a, int32_t b) {
+            while (depth[a] > depth[b]) a = f.parent[a];
+            while (depth[b] > depth[a]) b = f.parent[b];
+            while (a != b && a >= 0 && b >= 0) { a = f.parent[a]; b = f.parent[b]; }
+            return a == b ? a : -1;
+        };
+        std::vector<char> dirty(nc, 0);
+        for (int32_t v = 0; v < n && !full; ++v)
+            for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+                const int32_t u = adj[e];
+                if (u >= v) continue;
+                const int32_t a = f.member[v], b = f.member[u], r = lca(a, b);
+                if (r == a || r == b) continue;
+                if (r < 0 || (f.parent[r] < 0 && f.roots.size() == 1)) { full = true; break; }
+                dirty[r] = 1;
+            }
+        if (!full)
+            for (int32_t c = 0; c < nc; ++c)
+                if (dirty[c]) {
+                    int32_t p = f.parent[c];
+                    while (p >= 0 && !dirty[p]) p = f.parent[p];
+                    if (p < 0) dirty_roots.push_back(c);
+                }
+    }
+    if (full) {
+        next = full_dissection(n, xadj, adj, opt);
+        st.full_rebuild = true;
+        st.repartitioned_vertices = n;
+    } else if (dirty_roots.empty()) {
+        next = state->forest;
+    } else {
+        SepForest f = state->forest;
+        for (int32_t d : dirty_roots) {
+            std::vector<int32_t> region;
+            flatten(f, d, region);                 // the region's previous ordering
+            st.repartitioned_vertices += (int64_t)region.size();
+            std::vector<int32_t> roots;
+            const ForestBuilder fb = dissect_region(n, xadj, adj, region, opt, roots);
+            const int32_t off = append(f, fb);
+            for (int32_t& r : roots) r += off;
+            std::vector<int32_t>& slot = f.parent[d] < 0 ? f.roots : f.children[f.parent[d]];
+            const auto it = std::find(slot.begin(), slot.end(), d);
+            const size_t pos = (size_t)(it - slot.begin());
+            slot.erase(it);
+            slot.insert(slot.begin() + pos, roots.begin(), roots.end());
+            for (int32_t r : roots) f.parent[r] = f.parent[d];
+        }
+        next = compact(f, n);                       // drops the replaced subtrees
+        st.recomputed_subtrees = (int64_t)dirty_roots.size();
+    }
+    std::vector<int32_t> perm = forest_order(next);
+    st.nd_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (state) {
+        state->since_rebuild = st.full_rebuild ? 0 : state->since_rebuild + 1;
+        state->forest = std::move(next);
+        state->stats = st;
+    }
+    return perm;
+}
+
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
+                                       const std::vector<int32_t>& adj, const NDOptions& opt,
+                                       SepForest* forest) {
+    NDState s;
+    std::vector<int32_t> perm = nested_dissection(n, xadj, adj, opt, &s);
+    if (forest) *forest = std::move(s.forest);
+    return perm;
 }
 
 } // namespace mfs

@@ -219,8 +219,16 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     if (opt.given_perm) {
         if ((int32_t)opt.given_perm->size() != n) throw Error(-1, "given ordering has the wrong size");
         perm = *opt.given_perm;
+        if (opt.given_forest && opt.given_forest->member.size() == (size_t)n) {
+            S.nd_forest = *opt.given_forest;
+            S.has_nd_forest = true;
+        }
     } else {
-        perm = nested_dissection(n, xadj, adj, opt.nd);
+        NDState local;
+        NDState* st = opt.nd_state ? opt.nd_state : &local;
+        perm = nested_dissection(n, xadj, adj, opt.nd, st);
+        S.nd_forest = st->forest;
+        S.has_nd_forest = true;
     }
     S.nd_perm = perm;
     auto build_patterns = [&](const std::vector<int32_t>& pos_, std::vector<int64_t>& Lp,

@@ -15,6 +15,7 @@ import torch
 
 from . import _lib
 from ._lib import MFSError, NotPositiveDefiniteError, c_vp, ptr
+from .ordering import NDOrdering, TemporalReuseStatistics, _stats
 from .sparse import _dptr, _stream
 
 
@@ -35,6 +36,7 @@ class CholeskyProvider(enum.Enum):
     CatamariNative = 12
     CHOLMODNative = 13
     GpuSupernodal = 100
+    GpuSupernodalReuse = 101      # GpuSupernodal with temporal ND reuse, period 32 (as CatamariNesdisReuse)
 
 
 class CholeskySys(enum.Enum):
@@ -105,7 +107,8 @@ class GpuSupernodalFactorizer:
                                " factorization does not exist")
 
     def provider(self):
-        return CholeskyProvider.GpuSupernodal
+        # CatamariFactorizer.hh:138: the reuse provider whenever a reuse period is set
+        return CholeskyProvider.GpuSupernodalReuse if self.temporalReusePeriod() else CholeskyProvider.GpuSupernodal
 
     def preferInPlaceSolve(self):
         return True
@@ -167,6 +170,40 @@ class GpuSupernodalFactorizer:
         a = np.zeros(4)
         _lib.call("mfs_last_timings", self._h, ptr(a, _lib.c_double))
         return dict(inject_ms=a[0], levels_ms=a[2], total_ms=a[3])
+
+    # -- nested-dissection ordering and its temporal reuse ----------------------------------------
+    def ndOrdering(self):
+        """Separator tree of the most recent analysis (CholeskyFactorizerBase.hh:51-59): CParent of
+        the postordered components, CMember of every unpermuted reduced block, blockSize 3; None
+        when there is no analysis (or its ordering was not a dissection)."""
+        nc, nv, bs = _lib.c_i64(), _lib.c_i64(), _lib.c_i64()
+        if _lib.load().mfs_nd_ordering(self._h, nc, nv, bs, None, None) != _lib.MFS_OK:
+            return None
+        parent = np.empty(nc.value, np.int64)
+        member = np.empty(nv.value, np.int64)
+        _lib.call("mfs_nd_ordering", self._h, nc, nv, bs, ptr(parent, _lib.c_i64), ptr(member, _lib.c_i64))
+        return NDOrdering(parent, member, bs.value)
+
+    def setTemporalReusePeriod(self, period):
+        """CatamariFactorizer.hh:84: P > 0 lets the next P symbolic analyses repair only the
+        separator subtrees the new pattern violates; 0 disables reuse."""
+        _lib.call("mfs_set_temporal_reuse_period", self._h, int(period))
+
+    def temporalReusePeriod(self):
+        v = _lib.c_i64()
+        _lib.call("mfs_get_temporal_reuse_period", self._h, v)
+        return v.value
+
+    def resetTemporalReuse(self):
+        _lib.call("mfs_reset_temporal_reuse", self._h)
+
+    def resetSymbolicFactorizationReuse(self):
+        """CholeskyFactorizerBase.hh:246 (CatamariFactorizer.hh:87): e.g. after SparsityLRU
+        reports EXPIRED, the next analysis dissects from scratch."""
+        self.resetTemporalReuse()
+
+    def temporalReuseStatistics(self):
+        return _stats("mfs_temporal_reuse_statistics", self._h)
 
     # -- factorization --------------------------------------------------------------------------
     def factorizeSymbolic(self, H, pinnedVars=()):
@@ -347,10 +384,14 @@ def make_cholesky_factorizer(provider=CholeskyProvider.GpuSupernodal, singlePrec
     schurUpdates selects the BF16 Schur-update variants of the single-precision factor."""
     if provider == CholeskyProvider.CatamariLegacy and singlePrecision:
         raise ValueError("Single precision is unavailable for legacy Catamari")
-    if provider != CholeskyProvider.GpuSupernodal:
+    if provider not in (CholeskyProvider.GpuSupernodal, CholeskyProvider.GpuSupernodalReuse):
         raise RuntimeError(f"provider {provider.name} is not available in this build")
-    return GpuSupernodalFactorizer(singlePrecision, schurUpdates)
+    f = GpuSupernodalFactorizer(singlePrecision, schurUpdates)
+    if provider == CholeskyProvider.GpuSupernodalReuse:
+        f.setTemporalReusePeriod(32)       # make_cholesky_factorizer.cc:68-70
+    return f
 
 
 __all__ = ["CholeskyProvider", "CholeskySys", "FactorizationType", "GpuSupernodalFactorizer",
-           "make_cholesky_factorizer", "MFSError", "NotPositiveDefiniteError"]
+           "make_cholesky_factorizer", "MFSError", "NotPositiveDefiniteError", "NDOrdering",
+           "TemporalReuseStatistics"]
