@@ -381,6 +381,7 @@ def main():
     ap.add_argument("--max-pcg-free", type=int, default=5000, help="learned: CG limit of the unlimited comparison")
     ap.add_argument("--batch", type=int, default=512, help="number of meshes of --workload batch")
     ap.add_argument("--no-graph", action="store_true", help="batch: launch the factorization without a hipGraph")
+    ap.add_argument("--graph", action="store_true", help="cube: replay the numeric factorization as a captured hipGraph")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding of the factorization "
@@ -452,6 +453,7 @@ def main():
     nred = f.n_reduced()
     log(f"[bench] symbolic {t_sym:.2f}s: n_red={nred} nnz(L)={f.getFactorNNZ():.3e} flops={flops:.3e} {st}")
     dasm = asm._assembler(H, np.ascontiguousarray(tets, np.int32))
+    f.setGraphCapture(args.graph)
 
     def step(s):
         e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -484,6 +486,7 @@ def main():
     cnt = np.zeros(len(KCLASSES), np.int64)
     work = np.zeros(len(KCLASSES))
     if kt:
+        f.setGraphCapture(False)            # per-launch events need the launches themselves
         _lib.call("mfs_kernel_profiling", f._h, 1, 1)
         for k in range(args.steps):
             step(args.warmup + k)

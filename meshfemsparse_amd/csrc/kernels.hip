@@ -1425,11 +1425,28 @@ __global__ __launch_bounds__(256) void k_fgather(const SolveGatherTask* __restri
         const int Dc = 3 * sdeg[c];
         const T* uc = u + usoff[c];
         const int32_t* relc = rel + rel_off[c];
-        for (int k = threadIdx.x; k < Dc; k += blockDim.x) {
-            const int tgt = 3 * relc[k / 3] + k % 3;
-            const T v = uc[k];
-            if (tgt < t.w) x[t.x_off + tgt] -= v;
-            else us[tgt - t.w] += v;
+        // 8 entries per thread and pass: all index and value loads, then all target reads, then
+        // the stores (a child's targets are distinct, so the batch has no conflicts; children
+        // stay in order, one per barrier)
+        constexpr int UB = 8;
+        for (int k0 = threadIdx.x; k0 < Dc; k0 += UB * blockDim.x) {
+            int tg[UB];
+            T v[UB], cur[UB];
+#pragma unroll
+            for (int e = 0; e < UB; ++e) {
+                const int k = min(k0 + e * (int)blockDim.x, Dc - 1);
+                tg[e] = 3 * relc[k / 3] + k % 3;
+                v[e] = uc[k];
+            }
+#pragma unroll
+            for (int e = 0; e < UB; ++e)
+                if (k0 + e * (int)blockDim.x < Dc) cur[e] = tg[e] < t.w ? x[t.x_off + tg[e]] : us[tg[e] - t.w];
+#pragma unroll
+            for (int e = 0; e < UB; ++e)
+                if (k0 + e * (int)blockDim.x < Dc) {
+                    if (tg[e] < t.w) x[t.x_off + tg[e]] = cur[e] - v[e];
+                    else us[tg[e] - t.w] = cur[e] + v[e];
+                }
         }
         __syncthreads();
     }
@@ -1445,7 +1462,22 @@ __global__ __launch_bounds__(256) void k_bgather(const SolveGatherTask* __restri
     const T* x = xall + blockIdx.y * ldx;
     T* u = uall + blockIdx.y * ldu + t.u_off;
     const int32_t* rw = rows + t.rows_off;
-    for (int i = threadIdx.x; i < t.d; i += blockDim.x) u[i] = x[3 * (int64_t)rw[i / 3] + i % 3];
+    // 8 entries per thread and pass (index loads, then value loads, then stores)
+    constexpr int UB = 8;
+    for (int i0 = threadIdx.x; i0 < t.d; i0 += UB * blockDim.x) {
+        int64_t src[UB];
+        T v[UB];
+#pragma unroll
+        for (int e = 0; e < UB; ++e) {
+            const int i = min(i0 + e * (int)blockDim.x, t.d - 1);
+            src[e] = 3 * (int64_t)rw[i / 3] + i % 3;
+        }
+#pragma unroll
+        for (int e = 0; e < UB; ++e) v[e] = x[src[e]];
+#pragma unroll
+        for (int e = 0; e < UB; ++e)
+            if (i0 + e * (int)blockDim.x < t.d) u[i0 + e * blockDim.x] = v[e];
+    }
 }
 
 // deterministic wave-wide sum (fixed butterfly); every lane ends with the same value
@@ -1730,22 +1762,31 @@ __global__ __launch_bounds__(256) void k_sbdiag(const SolveTile* __restrict__ ti
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = TS * st.t, ncol = min(TS, b.nb - c0), nz = b.nb - c0;
     for (int r = tid; r < nz; r += 256) {
+        // the partial-sum slots are subtracted in slot order; up to 8 of them per memory round trip
         T v = y[c0 + r];
-        for (int p = b.part_begin; p < b.part_end; ++p) v -= part[(int64_t)p * sb + c0 + r];
+        int p = b.part_begin;
+        for (; p + 8 <= b.part_end; p += 8) {
+            T w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = part[(int64_t)(p + u) * sb + c0 + r];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v -= w[u];
+        }
+        for (; p < b.part_end; ++p) v -= part[(int64_t)p * sb + c0 + r];
         z[r] = v;
     }
     __syncthreads();
-    // two columns per memory round trip; rows beyond nz read row 0 (in bounds), weighted by 0
-    for (int c = wave; c < ncol; c += 8) {
-        T v[2][16];
+    // four columns per memory round trip; rows beyond nz read row 0 (in bounds), weighted by 0
+    for (int c = wave; c < ncol; c += 16) {
+        T v[4][16];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             const T* Xc = Xb + b.xb_off + (int64_t)(c0 + min(c + 4 * j, ncol - 1)) * b.nb + c0;
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[j][m] = Xc[lane + 64 * m < nz ? lane + 64 * m : 0];
         }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < 4; ++j) {
             T acc = T(0);
 #pragma unroll
             for (int m = 0; m < 16; ++m) acc += v[j][m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
