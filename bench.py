@@ -177,6 +177,22 @@ def bench_batch(args):
     t_asm = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
     t_fac = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3
     t_sol = np.mean([r[0][2].elapsed_time(r[0][3]) for r in recs]) / 1e3
+    # per-class device time from a pass with an event pair around every launch (eager launches,
+    # outside the timed region)
+    kms = {}
+    if not args.no_kernel_timing:
+        from meshfemsparse_amd import _lib
+        f.setGraphCapture(False)
+        ms = np.zeros(len(KCLASSES))
+        cnt = np.zeros(len(KCLASSES), np.int64)
+        _lib.call("mfs_kernel_profiling", f._h, 1, 1)
+        for k in range(args.steps):
+            step(args.warmup + k)
+        torch.cuda.synchronize()
+        _lib.call("mfs_kernel_profiling", f._h, 0, 0)
+        _lib.call("mfs_kernel_profile", f._h, _lib.ptr(ms, _lib.c_double), _lib.ptr(cnt, _lib.c_i64))
+        kms = {c: round(v / args.steps, 3) for c, v in zip(KCLASSES, ms) if v > 0}
+        log("[bench] kernel-timing pass done")
     out = {
         "metric": "batched Newton linear-solve s/step (512 meshes, every mesh to <=1e-10 rel. residual); factor TFLOP/s",
         "value": wall / args.steps, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -192,7 +208,7 @@ def bench_batch(args):
                    "graph": not args.no_graph},
         "factor_tflops": flops / t_fac / 1e12, "t_assembly_s": t_asm, "t_factor_s": t_fac, "t_solve_s": t_sol,
         "refine_iters": [r[1] for r in recs], "relres_max": max(r[2] for r in recs), "symbolic_s": t_sym,
-        "autotune_s": t_tune,
+        "autotune_s": t_tune, "kernel_ms_per_step": kms,
     }
     print(json.dumps(out), flush=True)
 
