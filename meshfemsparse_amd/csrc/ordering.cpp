@@ -455,11 +455,13 @@ std::vector<int32_t> nd_recurse(const G& g, const std::vector<int32_t>& gid, Rng
 }
 
 // Dissection of the subgraph induced by `verts` (global ids, in this order) as a forest whose
-// vertex lists hold global ids.  Every call starts from the same seed.
-ForestBuilder dissect_region(int32_t n, const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
+// vertex lists hold global ids.  Every call starts from the same seed.  `local` is scratch of n
+// entries, -1 for every vertex of the region and its neighbours; the call sets the region's
+// entries and restores them to -1 (calls on disjoint regions whose vertices have no neighbour in
+// another running call's region may share it).
+ForestBuilder dissect_region(const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
                              const std::vector<int32_t>& verts, const NDOptions& opt,
-                             std::vector<int32_t>& roots) {
-    std::vector<int32_t> local(n, -1);
+                             std::vector<int32_t>& roots, std::vector<int32_t>& local) {
     for (size_t i = 0; i < verts.size(); ++i) local[verts[i]] = (int32_t)i;
     G s;
     s.n = (int32_t)verts.size();
@@ -471,6 +473,7 @@ ForestBuilder dissect_region(int32_t n, const std::vector<int64_t>& xadj, const 
             if (local[adj[e]] >= 0 && adj[e] != v) { s.adj.push_back(local[adj[e]]); s.ew.push_back(1); }
         s.xadj[i + 1] = (int64_t)s.adj.size();
     }
+    for (int32_t v : verts) local[v] = -1;
     ForestBuilder fb;
     Rng rng{opt.seed};
     roots = nd_recurse(s, verts, rng, opt, fb);
@@ -552,7 +555,8 @@ SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std
     for (int32_t v = 0; v < n; ++v) gids[comp[v]].push_back(v);
     std::vector<ForestBuilder> fbs(nc);
     std::vector<std::vector<int32_t>> roots(nc);
-    auto work = [&](int32_t c) { fbs[c] = dissect_region(n, xadj, adj, gids[c], opt, roots[c]); };
+    std::vector<int32_t> local(n, -1);     // shared: the components are disjoint and closed
+    auto work = [&](int32_t c) { fbs[c] = dissect_region(xadj, adj, gids[c], opt, roots[c], local); };
     const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
     if (nc > 1 && nt > 1) {
         std::atomic<int32_t> next{0};
@@ -635,12 +639,13 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xa
         next = state->forest;
     } else {
         SepForest f = state->forest;
+        std::vector<int32_t> local(n, -1);
         for (int32_t d : dirty_roots) {
             std::vector<int32_t> region;
             flatten(f, d, region);                 // the region's previous ordering
             st.repartitioned_vertices += (int64_t)region.size();
             std::vector<int32_t> roots;
-            const ForestBuilder fb = dissect_region(n, xadj, adj, region, opt, roots);
+            const ForestBuilder fb = dissect_region(xadj, adj, region, opt, roots, local);
             const int32_t off = append(f, fb);
             for (int32_t& r : roots) r += off;
             std::vector<int32_t>& slot = f.parent[d] < 0 ? f.roots : f.children[f.parent[d]];
