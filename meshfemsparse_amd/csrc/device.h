@@ -136,6 +136,7 @@ private:
     NDState nd_state_;              // ordering history (temporal reuse)
     std::vector<int64_t> nd_pins_;  // sorted pins of the history
     NDState* ordering_state(const int64_t* pins, int64_t npins);
+    int schur_tile() const;
     bool graph_on_ = false;
     void* graph_ = nullptr;
     void* graph_exec_ = nullptr;

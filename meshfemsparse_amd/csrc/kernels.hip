@@ -1266,14 +1266,17 @@ static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, c
 // row l % 16 and the eight k values 8 (l / 16) .. 8 (l / 16) + 7, the same k slots for A and B.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-template <int TM, int TN, bool SPLIT>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__ tasks,
-                                                    float* __restrict__ L, float* __restrict__ U,
-                                                    const float* __restrict__ inv, int remap) {
+// NT threads (2 x NT/128 waves): 256 for 128x128 and 64x64 tiles, 512 for 256x256 tiles (half the
+// operand loads and conversions per flop; its two LDS buffers fill a CU's 160 KB).
+template <int TM, int TN, bool SPLIT, int NT = 256>
+__global__ __launch_bounds__(NT, 512 / NT) void k_gemm_bf(const GemmTask* __restrict__ tasks,
+                                                          float* __restrict__ L, float* __restrict__ U,
+                                                          const float* __restrict__ inv, int remap) {
     typedef f4 acc_t;
     constexpr int KC = 32, LDK = 40, NP = SPLIT ? 2 : 1;
-    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int AKT = 256 / TM, BKT = 256 / TN;          // threads per operand row
+    constexpr int WGY = NT / 128;                          // wave columns
+    constexpr int WM = TM / 2, WN = TN / WGY, FM = WM / 16, FN = WN / 16;
+    constexpr int AKT = NT / TM, BKT = NT / TN;            // threads per operand row
     constexpr int AKE = KC / AKT, BKE = KC / BKT;          // k values per thread and chunk
     constexpr int STAGE = NP * (TM + TN) * LDK;            // bf16 per buffer
     __shared__ __attribute__((aligned(16))) __bf16 smem[2 * STAGE];
@@ -1354,31 +1357,31 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
         const bool more = k0 + KC < K;
         if (more) gload(k0 + KC);
         const __bf16* st = smem + buf * STAGE;
-        bf8 ah[FM], bh[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) ah[i] = *(const bf8*)&st[(wm + 16 * i + fr) * LDK + fk];
+        const __bf16* sl = st + (TM + TN) * LDK;
+        // B fragments once per chunk, A fragments one row block at a time (registers); every
+        // accumulator still receives lo*hi, hi*lo, then hi*hi in each chunk
+        bf8 bh[FN], bl[SPLIT ? FN : 1];
 #pragma unroll
         for (int j = 0; j < FN; ++j) bh[j] = *(const bf8*)&st[(TM + wn + 16 * j + fr) * LDK + fk];
         if constexpr (SPLIT) {
-            const __bf16* sl = st + (TM + TN) * LDK;
-            bf8 al[FM], bl[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i) al[i] = *(const bf8*)&sl[(wm + 16 * i + fr) * LDK + fk];
 #pragma unroll
             for (int j = 0; j < FN; ++j) bl[j] = *(const bf8*)&sl[(TM + wn + 16 * j + fr) * LDK + fk];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                }
         }
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i) {
+            const bf8 ah = *(const bf8*)&st[(wm + 16 * i + fr) * LDK + fk];
+            if constexpr (SPLIT) {
+                const bf8 al = *(const bf8*)&sl[(wm + 16 * i + fr) * LDK + fk];
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+        }
         if (more) sstore(buf ^ 1, k0 + KC);
         __syncthreads();
         buf ^= 1;
@@ -1387,10 +1390,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf(const GemmTask* __restrict__
     gemm_store_acc<float, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, (t.flags & GT_BETA0) != 0);
 }
 
+// big256: the range's large tiles are 256x256 (Schur ranges of an analysis with schur_tile 256)
 static void launch_gemm_bf(const TaskRange& r, const GemmTask* tasks, float* L, float* U, const float* inv,
-                           bool split, hipStream_t s) {
+                           bool split, bool big256, hipStream_t s) {
     const int rm = gemm_remap();
-    if (r.m > r.b) {
+    if (r.m > r.b && big256) {
+        if (split) k_gemm_bf<256, 256, true, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
+        else k_gemm_bf<256, 256, false, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
+        LAUNCH_CHECK();
+    } else if (r.m > r.b) {
         if (split) k_gemm_bf<128, 128, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
         else k_gemm_bf<128, 128, false><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
         LAUNCH_CHECK();
@@ -1977,7 +1985,15 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
     SymbolicOptions o = options_from_env();
     o.nd_state = ordering_state(pins, npins);
+    o.schur_tile = schur_tile();
     install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
+}
+
+// 256x256 Schur tiles for the bfloat16 Schur-update kernel (MFS_BF16_TILE=128 keeps 128x128)
+int FactorizerDev::schur_tile() const {
+    if (precision < 2) return 128;
+    const char* e = getenv("MFS_BF16_TILE");
+    return (e && atoi(e) == 128) ? 128 : 256;
 }
 
 // The ordering history is kept across analyses with the same pinned variables
@@ -2093,10 +2109,12 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     const GemmTask* gt = gemm_.as<GemmTask>();
     // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
     // tile kernel; TRSM and the diagonal tiles stay in the factor type
-    auto schur_gemm = [&](const TaskRange& r, hipStream_t st) {
+    // (`syrk`: the range is a Schur update U = -L21 L21^T, whose large tiles are S.schur_tile)
+    auto schur_gemm = [&](const TaskRange& r, hipStream_t st, bool syrk = false) {
         if constexpr (sizeof(T) == 4) {
             if (precision >= 2) {
-                launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3, st);
+                launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3,
+                               syrk && S.schur_tile == 256, st);
                 return;
             }
         }
@@ -2195,7 +2213,7 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
         join();
         if (P.syrk.e > P.syrk.b) {
             prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
-            schur_gemm(P.syrk, s);
+            schur_gemm(P.syrk, s, true);
             prof.end(s);
         }
         extend(P.extend_u_passes);
@@ -2602,6 +2620,7 @@ void FactorizerDev::autotune(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     if (bs != 3) throw Error(-6, "only uniform 3x3 blocks are supported");
     if (nfuse < 1 || ntile < 1 || reps < 1) throw Error(-1, "autotune needs at least one candidate and one repetition");
     SymbolicOptions base = options_from_env();
+    base.schur_tile = schur_tile();
     std::vector<int32_t> perm;
     SepForest forest;
     {
@@ -2762,7 +2781,7 @@ void test_gemm_nt_bf16(int M, int N, int K, float alpha, const float* A, int lda
         throw Error(-1, "the tile kernel supports alpha in {+1,-1} and beta in {0,1}");
     if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, N) || ldc < std::max(1, M))
         throw Error(-1, "bad GEMM dimensions");
-    const int TT = (M >= 128 && N >= 128) ? 128 : 64;
+    const int TT = (M >= 256 && N >= 256) ? 256 : (M >= 128 && N >= 128) ? 128 : 64;
     std::vector<GemmTask> tasks;
     for (int c0 = 0; c0 < N; c0 += TT)
         for (int r0 = 0; r0 < M; r0 += TT) {
@@ -2785,9 +2804,9 @@ void test_gemm_nt_bf16(int M, int N, int K, float alpha, const float* A, int lda
     hipStream_t s = S_(stream);
     TaskRange r;
     r.b = 0;
-    r.m = TT == 128 ? (int64_t)tasks.size() : 0;
+    r.m = TT >= 128 ? (int64_t)tasks.size() : 0;
     r.e = (int64_t)tasks.size();
-    launch_gemm_bf(r, d.as<GemmTask>(), const_cast<float*>(A), C, B, split, s);
+    launch_gemm_bf(r, d.as<GemmTask>(), const_cast<float*>(A), C, B, split, TT == 256, s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
 

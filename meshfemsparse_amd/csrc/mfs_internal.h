@@ -232,6 +232,7 @@ struct Symbolic {
     int64_t u_solve_size = 0;                  // doubles per RHS for solve update vectors
     std::vector<int64_t> u_solve_off;          // per supernode
     int64_t factor_nnz = 0;                    // scalar lower nnz incl. diag of the panels
+    int32_t schur_tile = 128;                  // large tile of the Schur-update GEMM tasks
     double flops = 0.0;                        // POTRF+TRSM+SYRK over supernodes (scalar)
     double flops_unrelaxed = 0.0;              // sum_j c_j^2 of the unrelaxed column counts
     double flops_potrf = 0.0;                  // diagonal-tile POTRF share of `flops` (sum kb^3/3)
@@ -277,6 +278,8 @@ struct SymbolicOptions {
     int tile = 64;          // diagonal tile / inner step width (scalars)
     int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
+    int schur_tile = 128;   // large tile of the Schur update U = -L21 L21^T: 128, or 256 for the
+                            // bfloat16 Schur kernel (regions at least 256 in both dims)
     int tile_group = 8;     // GEMM tiles are emitted in groups of this many tile columns
     int solve_block = 512;  // solve column-block width (multiple of 64)
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)

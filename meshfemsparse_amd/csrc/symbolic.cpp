@@ -498,6 +498,8 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         S.xb_size = std::max<int64_t>(xb, 1);
     }
     const int BT = opt.big_tile;
+    if (opt.schur_tile != 128 && opt.schur_tile != 256) throw Error(-1, "the Schur tile must be 128 or 256");
+    S.schur_tile = opt.schur_tile;
     // Tiles of the lower part (rows >= cols, in region coordinates) of the GEMM region
     // C(r, c) (+)= -A(r, :) B(c, :)^T, r in [max(c0,rlo), rhi), c in [c0, c1), with A(r,k) at
     // a_base + r + k*lda, B(c,k) at b_base + c + k*ldb and C(r,c) at c_base + r + c*ldc.
@@ -507,8 +509,12 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                           int flags, bool allow_big, bool c_is_u = false) {
         (void)c_is_u;
         if (c1 <= c0 || K <= 0) return;
-        const bool use_big = allow_big && (c1 - c0) >= BT && (rhi - c0) >= BT;
-        const int TT = use_big ? 128 : T;     // the large tile of the GEMM kernel (BT is only the threshold)
+        // the Schur update U = -L21 L21^T (C in the update arena) may use the larger Schur tile;
+        // TRSM, in-panel and trailing updates keep 128x128 tiles
+        const int BIG = (flags & GT_C_IN_U) ? opt.schur_tile : 128;
+        const int BTH = std::max(BT, BIG);
+        const bool use_big = allow_big && (c1 - c0) >= BTH && (rhi - c0) >= BTH;
+        const int TT = use_big ? BIG : T;     // the large tile of the GEMM kernel (BT is only the threshold)
         // groups of G tile columns, row by row inside a group: a run of G*G consecutive tasks
         // covers a G x G block of tiles that shares G row panels of A and G column panels of B
         // (the kernel hands each XCD a contiguous run of tasks, so a run shares one L2)
@@ -841,7 +847,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     auto check_range = [&](const TaskRange& r) {
         for (int64_t q = r.b; q < r.e; ++q) {
             const GemmTask& g = S.gemm_tasks[q];
-            const int lim = q < r.m ? 128 : T;
+            const int lim = q < r.m ? ((g.flags & GT_C_IN_U) ? S.schur_tile : 128) : T;
             if (g.M < 1 || g.N < 1 || g.M > lim || g.N > lim || g.K < 1)
                 throw Error(-2, "internal: GEMM task does not fit its tile kernel");
         }

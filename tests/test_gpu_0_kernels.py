@@ -65,8 +65,9 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
+# shapes of at least 256 in both dimensions run on the 256x256 tile kernel (512 threads)
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (64, 64, 64), (70, 33, 17), (130, 200, 300),
-                                   (256, 256, 1000)])
+                                   (256, 256, 1000), (300, 260, 100), (513, 300, 77)])
 @pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0)])
 @pytest.mark.parametrize("split", [0, 1])
 def test_gemm_nt_tile_kernel_bf16(dev, M, N, K, alpha, beta, split):
