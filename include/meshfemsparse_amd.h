@@ -257,7 +257,9 @@ int mfs_nd_forest(mfs_nd h, int64_t* CParent, int64_t* CMember);
 int mfs_nd_statistics(mfs_nd h, int64_t* stats4, double* nd_seconds);
 
 /* Host-only symbolic analysis (no device work): the ordering and supernode statistics the
- * factorizer would use.  iperm_blocks[n_reduced_blocks] = reduced block at factor position k;
+ * factorizer would use (the MFS_* tuning variables of the environment apply to both; without
+ * MFS_LEAF the dissection runs with leaf sizes 96 and 128 and keeps the ordering with fewer
+ * unrelaxed factor flops).  iperm_blocks[n_reduced_blocks] = reduced block at factor position k;
  * stats[8] = {n_reduced_blocks, num_supernodes, num_levels, factor_nnz (scalar), max_width,
  * max_degree, L doubles, U arena doubles}; flops[2] = {supernodal POTRF+TRSM+SYRK flops,
  * sum over scalar columns of c_j^2 for the unrelaxed column counts c_j}.  Any output may be NULL. */

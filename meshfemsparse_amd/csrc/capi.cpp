@@ -537,8 +537,7 @@ int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int
     MFS_TRY
     CHECK_PTR(Ap); CHECK_PTR(Ai);
     if (npins > 0) CHECK_PTR(pins);
-    mfs::SymbolicOptions opt;
-    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, opt);
+    mfs::Symbolic S = mfs::analyze(Ap, Ai, nb, pins, npins < 0 ? 0 : npins, mfs::options_from_env());
     if (iperm)
         for (int64_t k = 0; k < S.n_red; ++k) iperm[k] = S.iperm[k];
     if (stats) {

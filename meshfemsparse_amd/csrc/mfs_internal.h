@@ -68,7 +68,8 @@ struct NDState {
     int64_t since_rebuild = 0;     // incremental analyses since the last full dissection
     SepForest forest;              // forest of the last call
     NDReuseStats stats;            // statistics of the last call
-    void reset() { since_rebuild = 0; forest = SepForest(); stats = NDReuseStats(); }
+    int leaf_size = 0;             // leaf size the forest was dissected with (0: none yet)
+    void reset() { since_rebuild = 0; forest = SepForest(); stats = NDReuseStats(); leaf_size = 0; }
 };
 std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
                                        const std::vector<int32_t>& adj, const NDOptions& opt,
@@ -272,6 +273,8 @@ struct Symbolic {
 
 struct SymbolicOptions {
     NDOptions nd;
+    int nd_leaf_alt = 128;  // a full dissection is also run with this leaf size and the ordering
+                            // with fewer predicted factor flops is kept; 0 disables the second run
     // relaxed amalgamation thresholds (scalar columns)
     int nrelax0 = 12, nrelax1 = 48, nrelax2 = 192;
     double zrelax0 = 0.8, zrelax1 = 0.2, zrelax2 = 0.05;
@@ -296,7 +299,7 @@ struct SymbolicOptions {
 
 // Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
-// (nested-dissection leaf size), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
+// (nested-dissection leaf size; setting it disables the second leaf-size candidate), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
 // of the outer trailing updates); values are rounded down to multiples of 64 where required.

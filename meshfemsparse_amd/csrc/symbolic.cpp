@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <map>
 #include <numeric>
+#include <exception>
+#include <thread>
 
 namespace mfs {
 
@@ -141,7 +143,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
     if (const char* e = getenv("MFS_SOLVE_BLOCK")) opt.solve_block = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
-    if (const char* e = getenv("MFS_LEAF")) opt.nd.leaf_size = std::max(8, atoi(e));
+    if (const char* e = getenv("MFS_LEAF")) { opt.nd.leaf_size = std::max(8, atoi(e)); opt.nd_leaf_alt = 0; }
     if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
     if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
@@ -214,23 +216,6 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             if (e.i != e.j) { adj[bk[e.i]++] = e.j; adj[bk[e.j]++] = e.i; }
     }
 
-    // ---- ordering: ND, then etree postorder ----
-    std::vector<int32_t> perm;   // perm[new] = old
-    if (opt.given_perm) {
-        if ((int32_t)opt.given_perm->size() != n) throw Error(-1, "given ordering has the wrong size");
-        perm = *opt.given_perm;
-        if (opt.given_forest && opt.given_forest->member.size() == (size_t)n) {
-            S.nd_forest = *opt.given_forest;
-            S.has_nd_forest = true;
-        }
-    } else {
-        NDState local;
-        NDState* st = opt.nd_state ? opt.nd_state : &local;
-        perm = nested_dissection(n, xadj, adj, opt.nd, st);
-        S.nd_forest = st->forest;
-        S.has_nd_forest = true;
-    }
-    S.nd_perm = perm;
     auto build_patterns = [&](const std::vector<int32_t>& pos_, std::vector<int64_t>& Lp,
                               std::vector<int32_t>& Li, std::vector<int64_t>& Up,
                               std::vector<int32_t>& Ui) {
@@ -257,6 +242,83 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             std::sort(Ui.begin() + Up[v], Ui.begin() + Up[v + 1]);
         }
     };
+    // Factor flops of the ordering p (perm[new] = old) over unamalgamated columns, after the
+    // etree postorder: the measure the leaf-size search compares orderings by.
+    auto predicted_flops = [&](const std::vector<int32_t>& p) {
+        std::vector<int32_t> pv(n), pv2(n), par;
+        std::vector<int64_t> lp, up;
+        std::vector<int32_t> li, ui;
+        for (int32_t k = 0; k < n; ++k) pv[p[k]] = k;
+        build_patterns(pv, lp, li, up, ui);
+        etree(n, up, ui, par);
+        const std::vector<int32_t> post = postorder(n, par);
+        for (int32_t k = 0; k < n; ++k) pv2[p[post[k]]] = k;
+        build_patterns(pv2, lp, li, up, ui);
+        etree(n, up, ui, par);
+        const std::vector<int64_t> cnt = column_counts(n, par, lp, li);
+        double f = 0;
+        for (int32_t j = 0; j < n; ++j) {
+            const double c = 3.0 * cnt[j];
+            f += c * c + (c - 1) * (c - 1) + (c - 2) * (c - 2);
+        }
+        return f;
+    };
+
+    // ---- ordering: ND, then etree postorder ----
+    std::vector<int32_t> perm;   // perm[new] = old
+    if (opt.given_perm) {
+        if ((int32_t)opt.given_perm->size() != n) throw Error(-1, "given ordering has the wrong size");
+        perm = *opt.given_perm;
+        if (opt.given_forest && opt.given_forest->member.size() == (size_t)n) {
+            S.nd_forest = *opt.given_forest;
+            S.has_nd_forest = true;
+        }
+    } else {
+        NDState local;
+        NDState* st = opt.nd_state ? opt.nd_state : &local;
+        // Leaf-size search: a full dissection runs with the configured leaf size and with
+        // nd_leaf_alt (on two threads), and the ordering with fewer predicted factor flops is
+        // kept; the state remembers the winner's leaf size for its incremental re-dissections.
+        NDOptions oa = opt.nd, ob = opt.nd;
+        if (st->leaf_size > 0) oa.leaf_size = st->leaf_size;
+        ob.leaf_size = oa.leaf_size == opt.nd.leaf_size ? opt.nd_leaf_alt : opt.nd.leaf_size;
+        const bool search = opt.nd_leaf_alt > 0 && ob.leaf_size != oa.leaf_size &&
+                            n > std::min(oa.leaf_size, ob.leaf_size);
+        // nested_dissection's own test before it looks at the graph (the rest of its decision
+        // does not depend on the leaf size)
+        const bool surely_full = st->period <= 0 || st->forest.member.size() != (size_t)n ||
+                                 st->since_rebuild >= st->period;
+        NDState sb;
+        if (search) sb = *st;
+        std::vector<int32_t> pb;
+        double fb = 0;
+        std::exception_ptr eb;
+        auto second = [&] {
+            try {
+                pb = nested_dissection(n, xadj, adj, ob, &sb);
+                fb = predicted_flops(pb);
+            } catch (...) { eb = std::current_exception(); }
+        };
+        std::thread th;
+        struct Joiner { std::thread& t; ~Joiner() { if (t.joinable()) t.join(); } } joiner{th};
+        if (search && surely_full) th = std::thread(second);
+        perm = nested_dissection(n, xadj, adj, oa, st);
+        st->leaf_size = oa.leaf_size;
+        if (search && st->stats.full_rebuild) {
+            const double fa = predicted_flops(perm);
+            if (th.joinable()) th.join();
+            else second();
+            if (eb) std::rethrow_exception(eb);
+            if (fb < fa) {
+                perm = std::move(pb);
+                *st = std::move(sb);
+                st->leaf_size = ob.leaf_size;
+            }
+        }
+        S.nd_forest = st->forest;
+        S.has_nd_forest = true;
+    }
+    S.nd_perm = perm;
     std::vector<int32_t> posv(n);
     for (int32_t k = 0; k < n; ++k) posv[perm[k]] = k;
     std::vector<int64_t> Lp, Up;

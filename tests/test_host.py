@@ -123,6 +123,27 @@ def test_symbolic_counts_match_oracle_factor(N):
     assert st[3] >= Lp[-1]
 
 
+@pytest.mark.parametrize("N", [10, 14])
+def test_leaf_size_search_keeps_the_cheaper_ordering(N, monkeypatch):
+    """Without MFS_LEAF the analysis dissects with leaf sizes 96 and 128 and keeps the ordering
+    with fewer unrelaxed factor flops; MFS_LEAF pins one leaf size.  The two sizes are chosen so
+    that the candidates differ (at N=10 leaf 96 wins, at N=14 leaf 128 wins)."""
+    tets, X = wl.cube_mesh(N)
+    H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
+    pins = wl.z0_pins(N)
+    cand = {}
+    for leaf in ("96", "128"):
+        monkeypatch.setenv("MFS_LEAF", leaf)
+        cand[leaf] = analyze(H, pins)
+    monkeypatch.delenv("MFS_LEAF")
+    ip, st, fl = analyze(H, pins)
+    assert cand["96"][2][1] != cand["128"][2][1]
+    win = min(cand, key=lambda k: cand[k][2][1])
+    assert win == ("96" if N == 10 else "128")
+    assert np.array_equal(ip, cand[win][0]) and np.array_equal(st, cand[win][1])
+    assert fl[1] == min(c[2][1] for c in cand.values())
+
+
 def test_partial_pins_are_rejected():
     tets, X = wl.cube_mesh(1)
     H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
