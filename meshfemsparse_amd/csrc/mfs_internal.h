@@ -49,7 +49,18 @@ struct NDOptions {
     int coarse_size = 120;
     double imbalance = 0.06;
     uint64_t seed = 0x5eed;
+    // A full dissection orders every connected component `trials` times and keeps the ordering
+    // with the fewest predicted factor flops.  Trial t uses leaf_alt for odd t, imbalance_alt for
+    // t & 2, and seed + t / 4; trial 0 is the plain options above.
+    int trials = 8;
+    int leaf_alt = 128;
+    double imbalance_alt = 0.15;
 };
+// Predicted factor flops (3x3 blocks, unamalgamated columns: sum of c^2 + (c-1)^2 + (c-2)^2 over
+// block column counts c, scalar) of ordering the vertex set `order` (ids into the CSR graph,
+// closed under adjacency) in that order.  pos: scratch of at least n entries, all -1 (restored).
+double order_flops(const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
+                   const std::vector<int32_t>& order, std::vector<int32_t>& pos);
 // Separator forest of a dissection, postordered (children before parents, parent[c] > c or -1):
 // component c owns verts[c] (a separator or a leaf, in ordering order); member[v] = component
 // of vertex v; the ordering is the postorder traversal (children in order, then own vertices).
@@ -68,8 +79,7 @@ struct NDState {
     int64_t since_rebuild = 0;     // incremental analyses since the last full dissection
     SepForest forest;              // forest of the last call
     NDReuseStats stats;            // statistics of the last call
-    int leaf_size = 0;             // leaf size the forest was dissected with (0: none yet)
-    void reset() { since_rebuild = 0; forest = SepForest(); stats = NDReuseStats(); leaf_size = 0; }
+    void reset() { since_rebuild = 0; forest = SepForest(); stats = NDReuseStats(); }
 };
 std::vector<int32_t> nested_dissection(int32_t n, const std::vector<int64_t>& xadj,
                                        const std::vector<int32_t>& adj, const NDOptions& opt,
@@ -273,8 +283,6 @@ struct Symbolic {
 
 struct SymbolicOptions {
     NDOptions nd;
-    int nd_leaf_alt = 128;  // a full dissection is also run with this leaf size and the ordering
-                            // with fewer predicted factor flops is kept; 0 disables the second run
     // relaxed amalgamation thresholds (scalar columns)
     int nrelax0 = 12, nrelax1 = 48, nrelax2 = 192;
     double zrelax0 = 0.8, zrelax1 = 0.2, zrelax2 = 0.05;
@@ -299,7 +307,9 @@ struct SymbolicOptions {
 
 // Options with the tuning knobs of the environment applied: MFS_OUTER (outer panel block),
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
-// (nested-dissection leaf size; setting it disables the second leaf-size candidate), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
+// (nested-dissection leaf size; setting it makes one trial per component unless MFS_ND_TRIALS
+// is also set), MFS_ND_TRIALS / MFS_ND_IMBALANCE / MFS_ND_COARSE / MFS_ND_SEED (dissection
+// trials per component, bisection imbalance, coarsest graph size, random seed), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
 // of the outer trailing updates); values are rounded down to multiples of 64 where required.

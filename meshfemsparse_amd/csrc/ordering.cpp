@@ -481,7 +481,8 @@ ForestBuilder dissect_region(const std::vector<int64_t>& xadj, const std::vector
 }
 
 // Appends the ordering of the subtree of c (children first, in order, then c's own vertices).
-void flatten(const SepForest& f, int32_t c, std::vector<int32_t>& out) {
+template <class Forest>
+void flatten(const Forest& f, int32_t c, std::vector<int32_t>& out) {
     std::vector<std::pair<int32_t, size_t>> st{{c, 0}};
     while (!st.empty()) {
         auto& [node, k] = st.back();
@@ -541,8 +542,18 @@ int32_t append(SepForest& f, const ForestBuilder& fb) {
     return off;
 }
 
-// Full dissection: connected components in parallel (each from the same seed, as it would be
-// ordered on its own), concatenated in component order.
+// Options of dissection trial t (trial 0: the options themselves).
+NDOptions trial_options(const NDOptions& opt, int t) {
+    NDOptions o = opt;
+    if (t & 1) o.leaf_size = opt.leaf_alt;
+    if (t & 2) o.imbalance = opt.imbalance_alt;
+    o.seed = opt.seed + (uint64_t)(t / 4);
+    return o;
+}
+
+// Full dissection: every connected component dissected opt.trials times (trial options above),
+// keeping the trial with the fewest predicted factor flops (the first on ties); the (component,
+// trial) pairs run in parallel and the kept components are concatenated in component order.
 SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
                           const NDOptions& opt) {
     G g;
@@ -553,26 +564,54 @@ SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std
     const int32_t nc = components(g, comp);
     std::vector<std::vector<int32_t>> gids(nc);
     for (int32_t v = 0; v < n; ++v) gids[comp[v]].push_back(v);
-    std::vector<ForestBuilder> fbs(nc);
-    std::vector<std::vector<int32_t>> roots(nc);
-    std::vector<int32_t> local(n, -1);     // shared: the components are disjoint and closed
-    auto work = [&](int32_t c) { fbs[c] = dissect_region(xadj, adj, gids[c], opt, roots[c], local); };
-    const int nt = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
-    if (nc > 1 && nt > 1) {
-        std::atomic<int32_t> next{0};
+    const int K = std::max(1, opt.trials);
+    const int32_t small = std::min(opt.leaf_size, K > 1 ? opt.leaf_alt : opt.leaf_size);
+    std::vector<std::pair<int32_t, int>> items;     // (component, trial)
+    std::vector<int> ntrial(nc);
+    for (int32_t c = 0; c < nc; ++c) {
+        ntrial[c] = (int32_t)gids[c].size() > small ? K : 1;   // a single leaf needs no trials
+        for (int t = 0; t < ntrial[c]; ++t) items.push_back({c, t});
+    }
+    std::vector<std::vector<ForestBuilder>> fbs(nc);
+    std::vector<std::vector<std::vector<int32_t>>> roots(nc);
+    std::vector<std::vector<double>> flops(nc);
+    for (int32_t c = 0; c < nc; ++c) {
+        fbs[c].resize(ntrial[c]);
+        roots[c].resize(ntrial[c]);
+        flops[c].assign(ntrial[c], 0.0);
+    }
+    const int nt = std::max(1, std::min<int>({(int)std::thread::hardware_concurrency(), 16, (int)items.size()}));
+    std::vector<std::vector<int32_t>> scratch(nt);  // per-thread vertex maps (trials of one
+                                                    // component overlap in their vertices)
+    auto work = [&](int tid, size_t q) {
+        std::vector<int32_t>& loc = scratch[tid];
+        if (loc.empty()) loc.assign(n, -1);
+        const auto [c, t] = items[q];
+        fbs[c][t] = dissect_region(xadj, adj, gids[c], trial_options(opt, t), roots[c][t], loc);
+        if (ntrial[c] > 1) {
+            std::vector<int32_t> ord;
+            for (int32_t r : roots[c][t]) flatten(fbs[c][t], r, ord);
+            flops[c][t] = order_flops(xadj, adj, ord, loc);
+        }
+    };
+    if (nt > 1) {
+        std::atomic<size_t> next{0};
         std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&] {
-                for (int32_t c; (c = next.fetch_add(1)) < nc;) work(c);
+        for (int tid = 0; tid < nt; ++tid)
+            th.emplace_back([&, tid] {
+                for (size_t q; (q = next.fetch_add(1)) < items.size();) work(tid, q);
             });
         for (auto& t : th) t.join();
     } else {
-        for (int32_t c = 0; c < nc; ++c) work(c);
+        for (size_t q = 0; q < items.size(); ++q) work(0, q);
     }
     SepForest f;
     for (int32_t c = 0; c < nc; ++c) {
-        const int32_t off = append(f, fbs[c]);
-        for (int32_t r : roots[c]) f.roots.push_back(r + off);
+        int best = 0;
+        for (int t = 1; t < ntrial[c]; ++t)
+            if (flops[c][t] < flops[c][best]) best = t;
+        const int32_t off = append(f, fbs[c][best]);
+        for (int32_t r : roots[c][best]) f.roots.push_back(r + off);
     }
     return compact(f, n);
 }

@@ -19,8 +19,6 @@
 #include <algorithm>
 #include <map>
 #include <numeric>
-#include <exception>
-#include <thread>
 
 namespace mfs {
 
@@ -136,6 +134,54 @@ struct Arena {
 
 } // namespace
 
+double order_flops(const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
+                   const std::vector<int32_t>& order, std::vector<int32_t>& pos) {
+    const int32_t m = (int32_t)order.size();
+    for (int32_t k = 0; k < m; ++k) pos[order[k]] = k;
+    // upper pattern (column k: rows i < k), then the etree postorder
+    std::vector<int64_t> Up(m + 1, 0);
+    std::vector<int32_t> Ui, parent;
+    for (int32_t k = 0; k < m; ++k) {
+        const int32_t v = order[k];
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e)
+            if (pos[adj[e]] < k) Ui.push_back(pos[adj[e]]);
+        Up[k + 1] = (int64_t)Ui.size();
+    }
+    etree(m, Up, Ui, parent);
+    const std::vector<int32_t> post = postorder(m, parent);
+    for (int32_t k = 0; k < m; ++k) pos[order[post[k]]] = k;
+    std::vector<int64_t> Lp(m + 1, 0);
+    std::vector<int32_t> Li;
+    Up.assign(m + 1, 0);
+    Ui.clear();
+    for (int32_t k = 0; k < m; ++k) {             // column k of the postordered matrix
+        const int32_t v = order[post[k]];
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+            const int32_t i = pos[adj[e]];
+            if (i < k) Ui.push_back(i);
+        }
+        Up[k + 1] = (int64_t)Ui.size();
+    }
+    for (int32_t k = 0; k < m; ++k)
+        for (int64_t p = Up[k]; p < Up[k + 1]; ++p) Lp[Ui[p] + 1]++;
+    for (int32_t k = 0; k < m; ++k) Lp[k + 1] += Lp[k];
+    Li.resize(Lp[m]);
+    {
+        std::vector<int64_t> b(Lp.begin(), Lp.end() - 1);
+        for (int32_t k = 0; k < m; ++k)
+            for (int64_t p = Up[k]; p < Up[k + 1]; ++p) Li[b[Ui[p]]++] = k;
+    }
+    etree(m, Up, Ui, parent);
+    const std::vector<int64_t> cnt = column_counts(m, parent, Lp, Li);
+    for (int32_t v : order) pos[v] = -1;
+    double f = 0;
+    for (int32_t j = 0; j < m; ++j) {
+        const double c = 3.0 * cnt[j];
+        f += c * c + (c - 1) * (c - 1) + (c - 2) * (c - 2);
+    }
+    return f;
+}
+
 std::vector<int32_t> nd_order(const Symbolic& S) { return S.nd_perm; }
 
 SymbolicOptions options_from_env() {
@@ -143,7 +189,11 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_OUTER")) opt.outer = std::max(64, atoi(e) / 64 * 64);
     if (const char* e = getenv("MFS_SOLVE_BLOCK")) opt.solve_block = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
-    if (const char* e = getenv("MFS_LEAF")) { opt.nd.leaf_size = std::max(8, atoi(e)); opt.nd_leaf_alt = 0; }
+    if (const char* e = getenv("MFS_LEAF")) { opt.nd.leaf_size = std::max(8, atoi(e)); opt.nd.trials = 1; }
+    if (const char* e = getenv("MFS_ND_TRIALS")) opt.nd.trials = std::min(64, std::max(1, atoi(e)));
+    if (const char* e = getenv("MFS_ND_IMBALANCE")) opt.nd.imbalance = std::min(0.5, std::max(0.0, atof(e)));
+    if (const char* e = getenv("MFS_ND_COARSE")) opt.nd.coarse_size = std::max(16, atoi(e));
+    if (const char* e = getenv("MFS_ND_SEED")) opt.nd.seed = strtoull(e, nullptr, 0);
     if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
     if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
@@ -242,28 +292,6 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             std::sort(Ui.begin() + Up[v], Ui.begin() + Up[v + 1]);
         }
     };
-    // Factor flops of the ordering p (perm[new] = old) over unamalgamated columns, after the
-    // etree postorder: the measure the leaf-size search compares orderings by.
-    auto predicted_flops = [&](const std::vector<int32_t>& p) {
-        std::vector<int32_t> pv(n), pv2(n), par;
-        std::vector<int64_t> lp, up;
-        std::vector<int32_t> li, ui;
-        for (int32_t k = 0; k < n; ++k) pv[p[k]] = k;
-        build_patterns(pv, lp, li, up, ui);
-        etree(n, up, ui, par);
-        const std::vector<int32_t> post = postorder(n, par);
-        for (int32_t k = 0; k < n; ++k) pv2[p[post[k]]] = k;
-        build_patterns(pv2, lp, li, up, ui);
-        etree(n, up, ui, par);
-        const std::vector<int64_t> cnt = column_counts(n, par, lp, li);
-        double f = 0;
-        for (int32_t j = 0; j < n; ++j) {
-            const double c = 3.0 * cnt[j];
-            f += c * c + (c - 1) * (c - 1) + (c - 2) * (c - 2);
-        }
-        return f;
-    };
-
     // ---- ordering: ND, then etree postorder ----
     std::vector<int32_t> perm;   // perm[new] = old
     if (opt.given_perm) {
@@ -276,45 +304,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     } else {
         NDState local;
         NDState* st = opt.nd_state ? opt.nd_state : &local;
-        // Leaf-size search: a full dissection runs with the configured leaf size and with
-        // nd_leaf_alt (on two threads), and the ordering with fewer predicted factor flops is
-        // kept; the state remembers the winner's leaf size for its incremental re-dissections.
-        NDOptions oa = opt.nd, ob = opt.nd;
-        if (st->leaf_size > 0) oa.leaf_size = st->leaf_size;
-        ob.leaf_size = oa.leaf_size == opt.nd.leaf_size ? opt.nd_leaf_alt : opt.nd.leaf_size;
-        const bool search = opt.nd_leaf_alt > 0 && ob.leaf_size != oa.leaf_size &&
-                            n > std::min(oa.leaf_size, ob.leaf_size);
-        // nested_dissection's own test before it looks at the graph (the rest of its decision
-        // does not depend on the leaf size)
-        const bool surely_full = st->period <= 0 || st->forest.member.size() != (size_t)n ||
-                                 st->since_rebuild >= st->period;
-        NDState sb;
-        if (search) sb = *st;
-        std::vector<int32_t> pb;
-        double fb = 0;
-        std::exception_ptr eb;
-        auto second = [&] {
-            try {
-                pb = nested_dissection(n, xadj, adj, ob, &sb);
-                fb = predicted_flops(pb);
-            } catch (...) { eb = std::current_exception(); }
-        };
-        std::thread th;
-        struct Joiner { std::thread& t; ~Joiner() { if (t.joinable()) t.join(); } } joiner{th};
-        if (search && surely_full) th = std::thread(second);
-        perm = nested_dissection(n, xadj, adj, oa, st);
-        st->leaf_size = oa.leaf_size;
-        if (search && st->stats.full_rebuild) {
-            const double fa = predicted_flops(perm);
-            if (th.joinable()) th.join();
-            else second();
-            if (eb) std::rethrow_exception(eb);
-            if (fb < fa) {
-                perm = std::move(pb);
-                *st = std::move(sb);
-                st->leaf_size = ob.leaf_size;
-            }
-        }
+        perm = nested_dissection(n, xadj, adj, opt.nd, st);
         S.nd_forest = st->forest;
         S.has_nd_forest = true;
     }
