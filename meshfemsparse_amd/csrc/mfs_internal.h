@@ -53,6 +53,7 @@ struct NDOptions {
     // with the fewest predicted factor flops.  Trial t uses leaf_alt for odd t, imbalance_alt for
     // t & 2, and seed + t / 4; trial 0 is the plain options above.
     int trials = 8;
+    int nseps = 1;          // bisections tried at every level (the smallest separator is kept)
     int leaf_alt = 128;
     double imbalance_alt = 0.15;
 };
@@ -309,7 +310,9 @@ struct SymbolicOptions {
 // MFS_SOLVE_BLOCK (solve column block), MFS_SOLVE_CHUNK (backward rows per task), MFS_LEAF
 // (nested-dissection leaf size; setting it makes one trial per component unless MFS_ND_TRIALS
 // is also set), MFS_ND_TRIALS / MFS_ND_IMBALANCE / MFS_ND_COARSE / MFS_ND_SEED (dissection
-// trials per component, bisection imbalance, coarsest graph size, random seed), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
+// trials per component, bisection imbalance, coarsest graph size, random seed), MFS_ND_NSEPS
+// (bisections tried per level), MFS_RELAX ("n0,n1,n2,z0,z1,z2": relaxed amalgamation
+// thresholds), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
 // of the outer trailing updates); values are rounded down to multiples of 64 where required.

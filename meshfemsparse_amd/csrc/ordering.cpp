@@ -10,6 +10,7 @@
 #include "mfs_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <chrono>
 #include <numeric>
@@ -435,10 +436,20 @@ std::vector<int32_t> nd_recurse(const G& g, const std::vector<int32_t>& gid, Rng
     if (g.n <= opt.leaf_size) return {fb.add(gid, {})};
     std::vector<int32_t> roots;
     if (nd_components(g, gid, rng, opt, fb, roots)) return roots;
+    // opt.nseps bisections; the smallest vertex separator that splits the graph is kept
+    auto sep_size = [](const std::vector<int8_t>& w) {
+        int64_t c[3] = {0, 0, 0};
+        for (int8_t x : w) c[x]++;
+        return (c[0] == 0 || c[1] == 0) ? INT64_MAX : c[2];
+    };
     std::vector<int8_t> where = bisect(g, rng, opt);
-    int32_t cnt[3] = {0, 0, 0};
-    for (int8_t w : where) cnt[w]++;
-    if (cnt[0] == 0 || cnt[1] == 0) return {fb.add(gid, {})};    // could not split: leaf
+    int64_t best = sep_size(where);
+    for (int t = 1; t < opt.nseps; ++t) {
+        std::vector<int8_t> w = bisect(g, rng, opt);
+        const int64_t sz = sep_size(w);
+        if (sz < best) { best = sz; where.swap(w); }
+    }
+    if (best == INT64_MAX) return {fb.add(gid, {})};    // could not split: leaf
     std::vector<int32_t> local(g.n, -1), members, kids;
     for (int8_t part = 0; part < 2; ++part) {
         G s = induced(g, where, part, local, members);

@@ -13,6 +13,7 @@
 // that receives its children's update matrices (extend-add) and then -L21 L21^T.  The level
 // schedule (supernode heights in the assembly tree) and every dense tile task are built here,
 // so the device side only replays fixed task lists.
+#include <cstdio>
 #include <cstdlib>
 #include "mfs_internal.h"
 
@@ -191,6 +192,15 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_SOLVE_CHUNK")) opt.solve_chunk = std::min(1024, std::max(64, atoi(e) / 64 * 64));
     if (const char* e = getenv("MFS_LEAF")) { opt.nd.leaf_size = std::max(8, atoi(e)); opt.nd.trials = 1; }
     if (const char* e = getenv("MFS_ND_TRIALS")) opt.nd.trials = std::min(64, std::max(1, atoi(e)));
+    if (const char* e = getenv("MFS_ND_NSEPS")) opt.nd.nseps = std::min(16, std::max(1, atoi(e)));
+    if (const char* e = getenv("MFS_RELAX")) {
+        int n0, n1, n2;
+        double z0, z1, z2;
+        if (sscanf(e, "%d,%d,%d,%lf,%lf,%lf", &n0, &n1, &n2, &z0, &z1, &z2) == 6) {
+            opt.nrelax0 = n0; opt.nrelax1 = n1; opt.nrelax2 = n2;
+            opt.zrelax0 = z0; opt.zrelax1 = z1; opt.zrelax2 = z2;
+        }
+    }
     if (const char* e = getenv("MFS_ND_IMBALANCE")) opt.nd.imbalance = std::min(0.5, std::max(0.0, atof(e)));
     if (const char* e = getenv("MFS_ND_COARSE")) opt.nd.coarse_size = std::max(16, atoi(e));
     if (const char* e = getenv("MFS_ND_SEED")) opt.nd.seed = strtoull(e, nullptr, 0);
