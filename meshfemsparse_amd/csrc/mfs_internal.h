@@ -58,8 +58,9 @@ struct NDOptions {
     double imbalance_alt = 0.15;
 };
 // Predicted factor flops (3x3 blocks, unamalgamated columns: sum of c^2 + (c-1)^2 + (c-2)^2 over
-// block column counts c, scalar) of ordering the vertex set `order` (ids into the CSR graph,
-// closed under adjacency) in that order.  pos: scratch of at least n entries, all -1 (restored).
+// block column counts c, scalar) of ordering the vertex set `order` (ids into the CSR graph; edges
+// to vertices outside the set are ignored) in that order.  pos: scratch of at least n entries,
+// all -1 (restored).
 double order_flops(const std::vector<int64_t>& xadj, const std::vector<int32_t>& adj,
                    const std::vector<int32_t>& order, std::vector<int32_t>& pos);
 // Separator forest of a dissection, postordered (children before parents, parent[c] > c or -1):

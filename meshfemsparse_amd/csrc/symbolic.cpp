@@ -144,8 +144,10 @@ double order_flops(const std::vector<int64_t>& xadj, const std::vector<int32_t>&
     std::vector<int32_t> Ui, parent;
     for (int32_t k = 0; k < m; ++k) {
         const int32_t v = order[k];
-        for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e)
-            if (pos[adj[e]] < k) Ui.push_back(pos[adj[e]]);
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+            const int32_t i = pos[adj[e]];
+            if (i >= 0 && i < k) Ui.push_back(i);
+        }
         Up[k + 1] = (int64_t)Ui.size();
     }
     etree(m, Up, Ui, parent);
@@ -159,7 +161,7 @@ double order_flops(const std::vector<int64_t>& xadj, const std::vector<int32_t>&
         const int32_t v = order[post[k]];
         for (int64_t e = xadj[v]; e < xadj[v + 1]; ++e) {
             const int32_t i = pos[adj[e]];
-            if (i < k) Ui.push_back(i);
+            if (i >= 0 && i < k) Ui.push_back(i);
         }
         Up[k + 1] = (int64_t)Ui.size();
     }
