@@ -13,6 +13,7 @@
 #include <climits>
 #include <atomic>
 #include <chrono>
+#include <exception>
 #include <numeric>
 #include <queue>
 #include <thread>
@@ -607,12 +608,20 @@ SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std
     };
     if (nt > 1) {
         std::atomic<size_t> next{0};
+        std::vector<std::exception_ptr> err(nt);      // a worker's first exception, rethrown here
         std::vector<std::thread> th;
         for (int tid = 0; tid < nt; ++tid)
             th.emplace_back([&, tid] {
-                for (size_t q; (q = next.fetch_add(1)) < items.size();) work(tid, q);
+                try {
+                    for (size_t q; (q = next.fetch_add(1)) < items.size();) work(tid, q);
+                } catch (...) {
+                    err[tid] = std::current_exception();
+                    next.store(items.size());           // the other workers stop early
+                }
             });
         for (auto& t : th) t.join();
+        for (const auto& e : err)
+            if (e) std::rethrow_exception(e);
     } else {
         for (size_t q = 0; q < items.size(); ++q) work(0, q);
     }
