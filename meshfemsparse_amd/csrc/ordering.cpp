@@ -584,6 +584,11 @@ SepForest full_dissection(int32_t n, const std::vector<int64_t>& xadj, const std
         ntrial[c] = (int32_t)gids[c].size() > small ? K : 1;   // a single leaf needs no trials
         for (int t = 0; t < ntrial[c]; ++t) items.push_back({c, t});
     }
+    // largest components first, so that no large dissection starts last (results do not depend
+    // on the order: every item is independent)
+    std::stable_sort(items.begin(), items.end(), [&](const auto& a, const auto& b) {
+        return gids[a.first].size() > gids[b.first].size();
+    });
     std::vector<std::vector<ForestBuilder>> fbs(nc);
     std::vector<std::vector<std::vector<int32_t>>> roots(nc);
     std::vector<std::vector<double>> flops(nc);
