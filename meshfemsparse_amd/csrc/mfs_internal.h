@@ -54,6 +54,7 @@ struct NDOptions {
     // t & 2, and seed + t / 4; trial 0 is the plain options above.
     int trials = 8;
     int nseps = 1;          // bisections tried at every level (the smallest separator is kept)
+    int node_passes = 0;    // node-based FM passes on every vertex separator (0: none)
     int leaf_alt = 128;
     double imbalance_alt = 0.15;
 };
@@ -312,7 +313,7 @@ struct SymbolicOptions {
 // (nested-dissection leaf size; setting it makes one trial per component unless MFS_ND_TRIALS
 // is also set), MFS_ND_TRIALS / MFS_ND_IMBALANCE / MFS_ND_COARSE / MFS_ND_SEED (dissection
 // trials per component, bisection imbalance, coarsest graph size, random seed), MFS_ND_NSEPS
-// (bisections tried per level), MFS_RELAX ("n0,n1,n2,z0,z1,z2": relaxed amalgamation
+// (bisections tried per level), MFS_ND_NODEFM (node-based FM passes per separator), MFS_RELAX ("n0,n1,n2,z0,z1,z2": relaxed amalgamation
 // thresholds), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split

@@ -17,6 +17,7 @@
 #include <numeric>
 #include <queue>
 #include <thread>
+#include <tuple>
 
 namespace mfs {
 
@@ -328,6 +329,91 @@ void edge_to_vertex_separator(const G& g, std::vector<int8_t>& where) {
     for (size_t i = 0; i < R.size(); ++i) if (zR[i]) where[R[i]] = 2;
 }
 
+// Node-based FM refinement of a vertex separator (where: 0/1 sides, 2 separator).  A move takes a
+// separator vertex into side s and pulls its neighbours on the other side into the separator
+// (gain = own weight - pulled weight), so every state is a valid separator.  Moves go best gain
+// first, each vertex at most once per pass, hill-climbing through non-improving moves; the pass
+// then rolls back to the smallest separator it saw (ties: better balance).  Sides stay within
+// the bisection's balance bound.
+void node_fm_refine(const G& g, std::vector<int8_t>& where, double ub, int npasses) {
+    const int32_t n = g.n;
+    const int64_t tw = g.total_vw();
+    int64_t maxvw = 0;
+    for (int32_t v = 0; v < n; ++v) maxvw = std::max<int64_t>(maxvw, g.vw[v]);
+    const int64_t maxw = std::max<int64_t>((int64_t)((1.0 + ub) * 0.5 * tw), (tw + 1) / 2 + maxvw);
+    const int limit = (int)std::min<int64_t>(300, std::max<int64_t>(25, n / 50));
+    auto gain = [&](int32_t v, int s) {
+        int64_t pulled = 0;
+        for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e)
+            if (where[g.adj[e]] == 1 - s) pulled += g.vw[g.adj[e]];
+        return (int64_t)g.vw[v] - pulled;
+    };
+    std::vector<char> locked(n);
+    std::vector<std::pair<int32_t, int8_t>> log;   // (vertex, previous side)
+    for (int pass = 0; pass < npasses; ++pass) {
+        int64_t pw[3] = {0, 0, 0};
+        for (int32_t v = 0; v < n; ++v) pw[where[v]] += g.vw[v];
+        const int64_t sep0 = pw[2];
+        int64_t best_sep = pw[2], best_bal = std::abs(pw[0] - pw[1]);
+        size_t best_len = 0;
+        std::fill(locked.begin(), locked.end(), 0);
+        log.clear();
+        using Ent = std::tuple<int64_t, int32_t, int8_t>;   // gain, vertex, side
+        std::priority_queue<Ent> pq;
+        auto push = [&](int32_t v) {
+            pq.push({gain(v, 0), v, 0});
+            pq.push({gain(v, 1), v, 1});
+        };
+        for (int32_t v = 0; v < n; ++v)
+            if (where[v] == 2) push(v);
+        int nobetter = 0;
+        while (!pq.empty() && nobetter < limit) {
+            const auto [gn, v, sd] = pq.top();
+            pq.pop();
+            if (where[v] != 2 || locked[v]) continue;
+            const int s = sd;
+            const int64_t gv = gain(v, s);
+            if (gv != gn) { pq.push({gv, v, sd}); continue; }     // stale entry
+            if (pw[s] + g.vw[v] > maxw) continue;
+            log.push_back({v, 2});
+            where[v] = (int8_t)s;
+            locked[v] = 1;
+            pw[s] += g.vw[v];
+            pw[2] -= g.vw[v];
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+                const int32_t u = g.adj[e];
+                if (where[u] == 1 - s) {
+                    log.push_back({u, (int8_t)(1 - s)});
+                    where[u] = 2;
+                    pw[1 - s] -= g.vw[u];
+                    pw[2] += g.vw[u];
+                }
+            }
+            // gains of separator vertices next to v or to the vertices it pulled have changed
+            for (int64_t e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
+                const int32_t u = g.adj[e];
+                if (where[u] != 2 || locked[u]) continue;
+                push(u);
+                for (int64_t f = g.xadj[u]; f < g.xadj[u + 1]; ++f) {
+                    const int32_t w = g.adj[f];
+                    if (where[w] == 2 && !locked[w]) push(w);
+                }
+            }
+            const int64_t bal = std::abs(pw[0] - pw[1]);
+            if (pw[2] < best_sep || (pw[2] == best_sep && bal < best_bal)) {
+                best_sep = pw[2];
+                best_bal = bal;
+                best_len = log.size();
+                nobetter = 0;
+            } else {
+                ++nobetter;
+            }
+        }
+        for (size_t i = log.size(); i > best_len; --i) where[log[i - 1].first] = log[i - 1].second;
+        if (best_sep >= sep0) break;
+    }
+}
+
 std::vector<int8_t> bisect(const G& g0, Rng& rng, const NDOptions& opt) {
     std::vector<G> graphs;
     std::vector<std::vector<int32_t>> cmaps;
@@ -350,6 +436,7 @@ std::vector<int8_t> bisect(const G& g0, Rng& rng, const NDOptions& opt) {
         fm_refine(graphs[lvl], where, opt.imbalance, 6);
     }
     edge_to_vertex_separator(g0, where);
+    if (opt.node_passes > 0) node_fm_refine(g0, where, opt.imbalance, opt.node_passes);
     return where;
 }
 

@@ -195,6 +195,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_LEAF")) { opt.nd.leaf_size = std::max(8, atoi(e)); opt.nd.trials = 1; }
     if (const char* e = getenv("MFS_ND_TRIALS")) opt.nd.trials = std::min(64, std::max(1, atoi(e)));
     if (const char* e = getenv("MFS_ND_NSEPS")) opt.nd.nseps = std::min(16, std::max(1, atoi(e)));
+    if (const char* e = getenv("MFS_ND_NODEFM")) opt.nd.node_passes = std::min(16, std::max(0, atoi(e)));
     if (const char* e = getenv("MFS_RELAX")) {
         int n0, n1, n2;
         double z0, z1, z2;

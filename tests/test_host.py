@@ -151,6 +151,14 @@ def test_dissection_trials_keep_the_cheapest_ordering(N, monkeypatch):
     assert sorted(ip8.tolist()) == list(range(len(ip8)))
 
 
+def test_node_refined_separators_give_a_consistent_analysis(monkeypatch):
+    """Node-based FM refinement of the separators (MFS_ND_NODEFM) still yields a permutation whose
+    exact column counts match the oracle's factor under that ordering."""
+    monkeypatch.setenv("MFS_ND_TRIALS", "1")
+    monkeypatch.setenv("MFS_ND_NODEFM", "4")
+    test_symbolic_counts_match_oracle_factor(7)
+
+
 def test_partial_pins_are_rejected():
     tets, X = wl.cube_mesh(1)
     H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
