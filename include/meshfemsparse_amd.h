@@ -36,11 +36,13 @@ extern "C" {
 /* ------------------------------------------------------------------------------------------ */
 #define MFS_OK               0
 #define MFS_ERR_ARG         -1  /* invalid argument / size mismatch      (std::invalid_argument) */
-#define MFS_ERR_STATE       -2  /* missing factorization / wrong phase   (std::runtime_error)    */
+#define MFS_ERR_STATE       -2  /* missing factorization / wrong phase / internal failure       */
+                                /*                                        (std::runtime_error)    */
 #define MFS_ERR_NOT_SPD     -3  /* pivot failure; see mfs_num_successful_pivots                  */
 #define MFS_ERR_DEVICE      -4  /* HIP runtime error                                             */
 #define MFS_ERR_ALLOC       -5  /* host or device allocation failure                             */
-#define MFS_ERR_UNSUPPORTED -6  /* e.g. partially pinned blocks, non-uniform block sizes         */
+#define MFS_ERR_UNSUPPORTED -6  /* e.g. non-uniform block sizes, partial pins in the learned     */
+                                /* preconditioner                                                */
 #define MFS_ERR_NOT_CONVERGED -7 /* iterative solve stopped at its iteration limit               */
 
 /* Factor precisions */
@@ -128,8 +130,11 @@ int mfs_factorizer_destroy(mfs_factorizer f);
 /* factorizeSymbolic(const BlockCSCHessianBase&, pinnedScalarVars) (CatamariFactorizer.cc:136-616):
  * whole-block pins -> block row/col removal (SparseMatrices.hh:2020-2076) -> nested-dissection
  * ordering -> elimination tree, relaxed supernodes, row structures, extend-add maps, injection
- * plan, level schedule.  Host arrays.  block_size must be 3.  Partially pinned blocks return
- * MFS_ERR_UNSUPPORTED (the reference falls back to a scalar factorization there).            */
+ * plan, level schedule.  Host arrays.  block_size must be 3.  A partially pinned block stays in
+ * the block pattern with its pinned components replaced by the identity at injection; the
+ * reference falls back to a scalar factorization there (CatamariFactorizer.cc:207-216); both
+ * factor the same reduced matrix, so solves, sizes (n_reduced = free scalars), the reduced and
+ * permuted vectors and the exported factor are those of the free scalars.                    */
 int mfs_factorize_symbolic(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai,
                            int64_t num_blocks, int32_t block_size,
                            const int64_t* pinned_scalar_vars, int64_t num_pins);
@@ -140,6 +145,20 @@ int mfs_factorize_symbolic(mfs_factorizer f, const int64_t* Ap, const int64_t* A
  * On a pivot failure returns MFS_ERR_NOT_SPD and drops back to the symbolic state.           */
 int mfs_factorize_numeric(mfs_factorizer f, const double* Ax_dev, double sigma,
                           const double* Bx_dev, void* stream);
+
+/* The SuiteSparseMatrix overloads of CholeskyFactorizerBase (CholeskyFactorizerBase.hh:95-113:
+ * factorizeSymbolic(const SuiteSparseMatrix&, pins), factorizeNumeric(const SuiteSparseMatrix&),
+ * factorizeNumericWithShift(A, sigma[, B])): a scalar symmetric matrix as its upper triangle in
+ * CSC (SuiteSparse UPPER_TRIANGLE; Ap[n+1], Ai[nnz] int64, rows sorted, row <= column).  Scalars
+ * are grouped three to a block (the last block padded; padding is masked like a pinned scalar)
+ * and factored by the block path.  After mfs_factorize_symbolic_scalar the factorizer takes the
+ * matrix's nnz scalar values (device) in mfs_factorize_numeric_scalar, mfs_spmv and
+ * mfs_solve_refined; full-length vectors have n entries.  Bx_dev (may be NULL) holds the values
+ * of a matrix with A's pattern.                                                               */
+int mfs_factorize_symbolic_scalar(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t n,
+                                  const int64_t* pinned_scalar_vars, int64_t num_pins);
+int mfs_factorize_numeric_scalar(mfs_factorizer f, const double* Ax_dev, double sigma,
+                                 const double* Bx_dev, void* stream);
 
 /* CholeskyFactorizerBase::solve (CholeskyFactorizerBase.hh:262-287) with the fused pin removal
  * and permutation (removeFixedEntries/extractFullSolution, :298-363): b, x are full-length
@@ -160,7 +179,8 @@ int mfs_extract_full_solution(mfs_factorizer f, const double* x_reduced_dev, dou
 
 /* Symmetric block SpMV from the upper blocks: y = A x over the full (unreduced) pattern the
  * factorizer was analysed with (BlockCSCHessian::applyRaw, BlockCSCHessian.hh:1059-1120).
- * Each output row sums its blocks in a fixed order (deterministic).                          */
+ * Each output row sums its blocks in a fixed order (deterministic).  After a scalar analysis
+ * Ax_dev holds the scalar values and x, y have n entries.                                     */
 int mfs_spmv(mfs_factorizer f, const double* Ax_dev, const double* x_dev, double* y_dev,
              void* stream);
 
@@ -244,7 +264,8 @@ int mfs_temporal_reuse_statistics(mfs_factorizer f, int64_t* stats4, double* nd_
 
 /* Host nested dissection of a symmetric graph with its own reuse history, as the factorizer
  * uses it (nested_dissection_from_graph + TemporalReuseState, cholmod_nesdis_parallel.hh:41-75).
- * Graph: CSC of both triangles (p[n+1], i[p[n]]; diagonal entries ignored); vertex ids must keep
+ * Graph: CSC of both triangles (p[n+1], i[p[n]]; diagonal entries ignored; a graph that does not
+ * list every edge in both directions is rejected with MFS_ERR_ARG); vertex ids must keep
  * their meaning between calls.  mfs_nd_order writes perm[new] = old and the number of
  * components; mfs_nd_forest then gives CParent[num_components] and CMember[n] of that call.   */
 int mfs_nd_create(mfs_nd* out);
@@ -305,7 +326,7 @@ int mfs_kernel_profile_work(mfs_factorizer f, double* work);
 /* zero-fill lower block factor L on tril(A) in a greedy multicolor ordering; M = L L^T          */
 /* preconditions FP32 CG on the Jacobi-scaled system S A S, inside FP64 refinement.  Training on  */
 /* the device: loss mean_p |(L L^T - S A S) z_p|^2 / |S A S z_p|^2 over caller probes z_p, own   */
-/* forward/backward kernels, Adam.                                                              */
+/* forward/backward kernels, Adam.  Pins must be whole blocks (MFS_ERR_UNSUPPORTED otherwise).  */
 /* ------------------------------------------------------------------------------------------ */
 int mfs_learned_create(const int64_t* Ap, const int64_t* Ai, int64_t num_blocks, const int64_t* pinned_scalar_vars,
                        int64_t num_pins, uint64_t seed, mfs_learned* out);

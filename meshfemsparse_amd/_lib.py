@@ -35,6 +35,8 @@ _SIGS = {
     "mfs_factorizer_destroy": [c_vp],
     "mfs_factorize_symbolic": [c_vp, P_i64, P_i64, c_i64, c_i32, P_i64, c_i64],
     "mfs_factorize_numeric": [c_vp, c_vp, c_dbl, c_vp, c_vp],
+    "mfs_factorize_symbolic_scalar": [c_vp, P_i64, P_i64, c_i64, P_i64, c_i64],
+    "mfs_factorize_numeric_scalar": [c_vp, c_vp, c_dbl, c_vp, c_vp],
     "mfs_solve": [c_vp, c_vp, c_vp, c_i32, c_vp],
     "mfs_solve_raw_reduced": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],
     "mfs_remove_fixed_entries": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],

@@ -1,7 +1,9 @@
 // extern "C" boundary: status codes + thread-local error message around the C++ objects.
 #include "../../include/meshfemsparse_amd.h"
 
+#include <algorithm>
 #include <cstring>
+#include <stdexcept>
 #include <memory>
 
 #include "device.h"
@@ -31,9 +33,13 @@ struct mfs_learned_s { std::unique_ptr<mfs::LearnedDev> dev; };
         mfs::set_last_error("host allocation failed");                                        \
         return MFS_ERR_ALLOC;                                                                 \
     }                                                                                         \
-    catch (const std::exception& e) {                                                         \
+    catch (const std::invalid_argument& e) {                                                  \
         mfs::set_last_error(e.what());                                                        \
         return MFS_ERR_ARG;                                                                   \
+    }                                                                                         \
+    catch (const std::exception& e) {                                                         \
+        mfs::set_last_error(std::string("internal failure: ") + e.what());                    \
+        return MFS_ERR_STATE;                                                                 \
     }                                                                                         \
     return MFS_OK;
 
@@ -175,7 +181,25 @@ int mfs_factorize_symbolic(mfs_factorizer f, const int64_t* Ap, const int64_t* A
 int mfs_factorize_numeric(mfs_factorizer f, const double* Ax, double sigma, const double* Bx, void* stream) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(Ax);
+    if (f->dev->ftype >= 1 && f->dev->scalar_mode())
+        throw mfs::Error(MFS_ERR_ARG, "the symbolic factorization is of a scalar matrix: use mfs_factorize_numeric_scalar");
     f->dev->numeric(Ax, sigma, Bx, stream);
+    MFS_CATCH
+}
+
+int mfs_factorize_symbolic_scalar(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t n,
+                                  const int64_t* pins, int64_t npins) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ap); CHECK_PTR(Ai);
+    if (npins > 0) CHECK_PTR(pins);
+    f->dev->symbolic_scalar(Ap, Ai, n, pins, npins < 0 ? 0 : npins);
+    MFS_CATCH
+}
+
+int mfs_factorize_numeric_scalar(mfs_factorizer f, const double* Ax, double sigma, const double* Bx, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(Ax);
+    f->dev->numeric_scalar(Ax, sigma, Bx, stream);
     MFS_CATCH
 }
 
@@ -311,8 +335,8 @@ int mfs_sizes(mfs_factorizer f, int64_t* nf, int64_t* nr) {
     MFS_TRY
     CHECK_PTR(f);
     f->dev->require(1);
-    if (nf) *nf = 3 * f->dev->sym->n_full_blocks;
-    if (nr) *nr = 3 * f->dev->sym->n_red;
+    if (nf) *nf = f->dev->sym->n_full;
+    if (nr) *nr = f->dev->sym->n_free;
     MFS_CATCH
 }
 
@@ -320,7 +344,7 @@ int mfs_factor_nnz(mfs_factorizer f, int64_t* nnz) {
     MFS_TRY
     CHECK_PTR(f); CHECK_PTR(nnz);
     f->dev->require(1);
-    *nnz = f->dev->sym->factor_nnz;
+    *nnz = f->dev->sym->factor_nnz_free;
     MFS_CATCH
 }
 
@@ -348,8 +372,9 @@ int mfs_get_inverse_permutation(mfs_factorizer f, int64_t* ip) {
     CHECK_PTR(f); CHECK_PTR(ip);
     f->dev->require(1);
     const auto& S = *f->dev->sym;
-    for (int64_t k = 0; k < S.n_red; ++k)
-        for (int c = 0; c < 3; ++c) ip[3 * k + c] = 3 * (int64_t)S.iperm[k] + c;
+    // factor-order reduced index -> reduced index of the same scalar
+    for (int64_t v = 0; v < S.n_full; ++v)
+        if (S.full_to_uperm[v] >= 0) ip[S.full_to_uperm[v]] = S.full_to_red[v];
     MFS_CATCH
 }
 
@@ -507,6 +532,23 @@ int mfs_nd_order(mfs_nd h, int64_t n, const int64_t* p, const int64_t* i, int64_
             xadj[v + 1] = (int64_t)adj.size();
         }
     }
+    {
+        // the graph must list every edge in both directions (a one-triangle pattern would leave
+        // vertices without their neighbours): compare the edge set with its transpose
+        std::vector<std::pair<int32_t, int32_t>> e, et;
+        e.reserve(adj.size());
+        et.reserve(adj.size());
+        for (int64_t v = 0; v < n; ++v)
+            for (int64_t k = xadj[v]; k < xadj[v + 1]; ++k) {
+                e.emplace_back((int32_t)v, adj[k]);
+                et.emplace_back(adj[k], (int32_t)v);
+            }
+        std::sort(e.begin(), e.end());
+        e.erase(std::unique(e.begin(), e.end()), e.end());
+        std::sort(et.begin(), et.end());
+        et.erase(std::unique(et.begin(), et.end()), et.end());
+        if (e != et) throw mfs::Error(MFS_ERR_ARG, "the graph must be symmetric (both triangles of the pattern)");
+    }
     mfs::NDOptions opt;
     const std::vector<int32_t> pr = mfs::nested_dissection((int32_t)n, xadj, adj, opt, &h->state);
     for (int64_t k = 0; k < n; ++k) perm[k] = pr[k];
@@ -541,7 +583,7 @@ int mfs_analyze_host(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int
     if (iperm)
         for (int64_t k = 0; k < S.n_red; ++k) iperm[k] = S.iperm[k];
     if (stats) {
-        stats[0] = S.n_red; stats[1] = S.ns; stats[2] = S.num_levels; stats[3] = S.factor_nnz;
+        stats[0] = S.n_red; stats[1] = S.ns; stats[2] = S.num_levels; stats[3] = S.factor_nnz_free;
         stats[4] = S.max_width; stats[5] = S.max_degree; stats[6] = S.L_size; stats[7] = S.U_size;
     }
     if (flops) { flops[0] = S.flops; flops[1] = S.flops_unrelaxed; }
@@ -560,11 +602,15 @@ int mfs_debug_emulate_factor(const int64_t* Ap, const int64_t* Ai, int64_t nb, c
     if (failed_col) *failed_col = fc;
     if (Lp && Li && Lx) mfs::panels_to_csc(S, L, Lp, Li, Lx);
     if (x_perm && fc < 0) {
-        std::vector<double> x(x_perm, x_perm + 3 * S.n_red), Xb;
+        // x_perm holds the free scalars in factor order; masked positions are zero internally
+        std::vector<double> x(3 * S.n_red, 0.0), Xb;
+        for (int64_t u = 0; u < S.n_free; ++u) x[S.uperm_to_int[u]] = x_perm[u];
         mfs::emulate_block_inverses(S, L, inv, Xb);
         mfs::emulate_solve(S, L, Xb, x);
-        std::copy(x.begin(), x.end(), x_perm);
+        for (int64_t u = 0; u < S.n_free; ++u) x_perm[u] = x[S.uperm_to_int[u]];
     }
+    if (failed_col && fc >= 0 && S.masked())
+        *failed_col = std::lower_bound(S.uperm_to_int.begin(), S.uperm_to_int.end(), fc) - S.uperm_to_int.begin();
     MFS_CATCH
 }
 

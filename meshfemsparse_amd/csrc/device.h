@@ -72,6 +72,11 @@ public:
     void symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, int bs, const int64_t* pins,
                   int64_t npins);
     void numeric(const double* Ax, double sigma, const double* Bx, void* stream);
+    // a scalar upper-triangular CSC matrix (SuiteSparseMatrix overloads): grouped into 3x3 blocks
+    // with padding, then the block path; numeric / spmv / refinement then take the scalar values
+    void symbolic_scalar(const int64_t* Ap, const int64_t* Ai, int64_t n, const int64_t* pins, int64_t npins);
+    void numeric_scalar(const double* Ax, double sigma, const double* Bx, void* stream);
+    bool scalar_mode() const { return scalar_mode_; }
     void solve(const double* b, double* x, int nrhs, void* stream);
     void solve_raw_reduced(const double* b, double* x, int nrhs, bool already_permuted, void* stream);
     void remove_fixed(const double* xf, double* xr, int nrhs, bool permute, void* stream);
@@ -120,6 +125,8 @@ private:
     template <typename T> void enqueue_numeric(const double* Ax, double sigma, const double* Bx, void* stream,
                                                void* ev_inject);
     void solve_permuted_inplace(double* xp, int nrhs, void* stream);   // factor-order vectors
+    void solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream);
+    void scalar_values(const double* Ax, double* Axb, void* stream);   // scalar -> block values
     template <typename T> void numeric_t(const double* Ax, double sigma, const double* Bx, void* stream);
     template <typename T> void solve_t(T* xp, int nrhs, void* stream);
     void ensure_vec(DeviceArray& a, size_t bytes);
@@ -129,7 +136,12 @@ private:
     DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, fused_, uoff_, panel_, pcount_;
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
-    DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_;
+    DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_, f2up_, up2i_, cmask_;
+    const uint8_t* cmask() const { return cmask_.as<const uint8_t>(); }
+    // scalar matrices: value positions in the block values, block copies of A and B, padded vectors
+    bool scalar_mode_ = false;
+    int64_t scalar_nnz_ = 0;
+    DeviceArray s2b_, Axb_, Bxb_, xpad_, ypad_;
     DeviceArray usolve_, ysolve_, bpart_, scratch_b_, scratch_c_;
     DeviceArray Lstash_, invstash_, scalestash_, xbstash_;
     bool stash_valid_ = false;

@@ -26,11 +26,14 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                                     S.panel_off.begin()) - 1;
         const int64_t ld = 3 * ((int64_t)S.swidth[s] + S.sdeg[s]);
         const double* a = Ax + 9 * (int64_t)S.plan_src[q];
+        const int mr = S.masked() ? S.comp_mask[S.plan_row[q]] : 0;
+        const int mc = S.masked() ? S.comp_mask[S.plan_col[q]] : 0;
         for (int c = 0; c < 3; ++c)
             for (int r = 0; r < 3; ++r) {
                 if (mode == 2 && r < c) continue;
                 double v = mode == 0 ? a[3 * c + r] : a[3 * r + c];
                 if (mode == 2 && r == c) v += sigma;
+                if (((mr >> r) | (mc >> c)) & 1) v = (mode == 2 && r == c) ? 1.0 : 0.0;   // masked: identity
                 L[dst + c * ld + r] = v;
             }
     }
@@ -173,11 +176,20 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
 namespace mfs {
 
 void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp, int64_t* Li, double* Lx) {
-    const int64_t n = 3 * S.n_red;
+    // columns and rows in factor order over the free scalars (masked ones dropped: they hold the
+    // identity), n_free columns
+    const int64_t ni = 3 * S.n_red, n = S.n_free;
+    std::vector<int64_t> uidx(ni, -1);
+    for (int64_t u = 0; u < n; ++u) uidx[S.uperm_to_int[u]] = u;
     std::vector<int64_t> cnt(n + 1, 0);
     for (int32_t sn = 0; sn < S.ns; ++sn) {
         const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn];
-        for (int64_t c = 0; c < W; ++c) cnt[3 * (int64_t)S.sfirst[sn] + c + 1] = W + D - c;
+        const int64_t f = 3 * (int64_t)S.sfirst[sn];
+        int64_t struct_free = 0;
+        for (int64_t r = 0; r < D; ++r) struct_free += uidx[3 * (int64_t)S.rows[S.srows_off[sn] + r / 3] + r % 3] >= 0;
+        int64_t below = 0;
+        for (int64_t c = W - 1; c >= 0; --c)
+            if (uidx[f + c] >= 0) cnt[uidx[f + c] + 1] = ++below + struct_free;
     }
     for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
     for (int64_t j = 0; j <= n; ++j) Lp[j] = cnt[j];
@@ -185,9 +197,12 @@ void panels_to_csc(const Symbolic& S, const std::vector<double>& h, int64_t* Lp,
         const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn], ld = W + D;
         const int64_t f = 3 * (int64_t)S.sfirst[sn];
         for (int64_t c = 0; c < W; ++c) {
-            int64_t q = Lp[f + c];
+            if (uidx[f + c] < 0) continue;
+            int64_t q = Lp[uidx[f + c]];
             for (int64_t r = c; r < W + D; ++r) {
-                Li[q] = r < W ? f + r : 3 * (int64_t)S.rows[S.srows_off[sn] + (r - W) / 3] + (r - W) % 3;
+                const int64_t row = r < W ? f + r : 3 * (int64_t)S.rows[S.srows_off[sn] + (r - W) / 3] + (r - W) % 3;
+                if (uidx[row] < 0) continue;
+                Li[q] = uidx[row];
                 Lx[q] = h[S.panel_off[sn] + c * ld + r];
                 ++q;
             }

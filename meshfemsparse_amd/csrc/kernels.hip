@@ -329,11 +329,15 @@ struct PlanEnt {
     int32_t rb, cb;     // factor-order block row / column of the lower-triangle block
 };
 
+// cmask (NULL: none) flags masked components per factor-order block (bit c): pinned scalars of a
+// partially pinned block and the padding of a scalar matrix; their rows and columns become the
+// identity, so the free scalars factor as the reduced matrix.
 template <typename T>
 __global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan, int64_t n,
                                                 const double* __restrict__ Ax,
                                                 const double* __restrict__ Bx, double sigma,
                                                 const double* __restrict__ scale,
+                                                const uint8_t* __restrict__ cmask,
                                                 T* __restrict__ L) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -343,6 +347,7 @@ __global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan
     const double* a = Ax + 9 * (int64_t)pe.src;
     const double* bb = Bx ? Bx + 9 * (int64_t)pe.src : nullptr;
     const int64_t ld = pe.ld;
+    const int mr = cmask ? cmask[pe.rb] : 0, mc = cmask ? cmask[pe.cb] : 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -353,22 +358,27 @@ __global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan
             if (bb) v += sigma * bb[srcidx];
             else if (mode == 2 && r == c) v += sigma;
             if (scale) v *= scale[3 * (int64_t)pe.rb + r] * scale[3 * (int64_t)pe.cb + c];
+            if (((mr >> r) | (mc >> c)) & 1) v = (mode == 2 && r == c) ? 1.0 : 0.0;
             d[c * ld + r] = (T)v;
         }
 }
 
 // s_k = 1 / sqrt((A + shift)_kk) in factor order; a non-positive diagonal is a pivot failure
+// (masked components: s = 1)
 __global__ void k_diag_scale(const int32_t* __restrict__ diag_src, int64_t nblocks,
                              const double* __restrict__ Ax, const double* __restrict__ Bx,
-                             double sigma, double* __restrict__ scale, int* __restrict__ status) {
+                             double sigma, const uint8_t* __restrict__ cmask, double* __restrict__ scale,
+                             int* __restrict__ status) {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= nblocks) return;
     const int64_t src = 9 * (int64_t)diag_src[k];
+    const int m = cmask ? cmask[k] : 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double d = 0.0;
         if (src >= 0) d = Ax[src + 4 * c] + (Bx ? sigma * Bx[src + 4 * c] : sigma);
         else if (!Bx) d = sigma;
+        if ((m >> c) & 1) d = 1.0;
         if (!(d > 0.0)) {
             atomicMin(status, (int)(3 * k + c));
             d = 1.0;
@@ -1989,6 +1999,49 @@ void FactorizerDev::symbolic(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
     install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
 }
 
+void FactorizerDev::symbolic_scalar(const int64_t* Ap, const int64_t* Ai, int64_t n, const int64_t* pins,
+                                    int64_t npins) {
+    ScalarBlocking B = block_scalar_pattern(Ap, Ai, n);
+    SymbolicOptions o = options_from_env();
+    o.nd_state = ordering_state(pins, npins);
+    o.schur_tile = schur_tile();
+    o.n_scalar = n;
+    install(std::make_unique<Symbolic>(analyze(B.P.Ap.data(), B.P.Ai.data(), B.P.n, pins, npins, o)));
+    scalar_mode_ = true;
+    scalar_nnz_ = (int64_t)B.value_pos.size();
+    s2b_.upload(B.value_pos);
+    Axb_.alloc(std::max<int64_t>(9 * B.P.nz(), 1) * sizeof(double));
+    Bxb_.release();
+}
+
+__global__ void k_scalar_to_block(const int64_t* __restrict__ pos, int64_t n, const double* __restrict__ a,
+                                  double* __restrict__ ab) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < n) ab[pos[k]] = a[k];
+}
+
+void FactorizerDev::scalar_values(const double* Ax, double* Axb, void* stream) {
+    hipStream_t s = S_(stream);
+    fill<double>(Axb, 9 * sym->nz_full, 0.0, s);
+    if (scalar_nnz_) {
+        k_scalar_to_block<<<nblk(scalar_nnz_, 256), 256, 0, s>>>(s2b_.as<int64_t>(), scalar_nnz_, Ax, Axb);
+        LAUNCH_CHECK();
+    }
+}
+
+void FactorizerDev::numeric_scalar(const double* Ax, double sigma, const double* Bx, void* stream) {
+    require(1);
+    if (!scalar_mode_) throw Error(-2, "the symbolic factorization is of a block matrix");
+    scalar_values(Ax, Axb_.as<double>(), stream);
+    const double* bxb = nullptr;
+    if (Bx) {
+        Bxb_.alloc(Axb_.bytes);
+        scalar_values(Bx, Bxb_.as<double>(), stream);
+        bxb = Bxb_.as<double>();
+    }
+    numeric(Axb_.as<double>(), sigma, bxb, stream);
+}
+
 // 256x256 Schur tiles for the bfloat16 Schur-update kernel (MFS_BF16_TILE=128 keeps 128x128)
 int FactorizerDev::schur_tile() const {
     if (precision < 2) return 128;
@@ -2016,6 +2069,7 @@ void FactorizerDev::set_reuse_period(int64_t period) {
 void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     drop_graph();
     ftype = 0;
+    scalar_mode_ = false;
     stash_valid_ = false;
     // injection plan entries with per-entry leading dimension and factor-order block row/column
     std::vector<PlanEnt> pe(S->plan_src.size());
@@ -2059,12 +2113,11 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     spmv_col_.upload(S->spmv_col);
     f2p_.upload(S->full_to_perm);
     f2r_.upload(S->full_to_red);
-    {
-        std::vector<int64_t> r2p(3 * S->n_red);
-        for (int64_t r = 0; r < S->n_red; ++r)
-            for (int c = 0; c < 3; ++c) r2p[3 * r + c] = 3 * (int64_t)S->pos[r] + c;
-        r2p_.upload(r2p);
-    }
+    f2up_.upload(S->full_to_uperm);
+    r2p_.upload(S->red_to_int);
+    up2i_.upload(S->uperm_to_int);
+    if (S->masked()) cmask_.upload(S->comp_mask);
+    else cmask_.release();
     const size_t fs = fsize();
     L_.alloc(std::max<int64_t>(S->L_size, 1) * fs);
     U_.alloc(std::max<int64_t>(S->U_size, 1) * fs);
@@ -2094,14 +2147,14 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     const double* scale = nullptr;
     if (precision != 0) {   // symmetric diagonal scaling of the factored matrix
         k_diag_scale<<<nblk(S.n_red, 256), 256, 0, s>>>(diagsrc_.as<int32_t>(), S.n_red, Ax, Bx, sigma,
-                                                       scale_.as<double>(), status_.as<int>());
+                                                       cmask(), scale_.as<double>(), status_.as<int>());
         LAUNCH_CHECK();
         scale = scale_.as<double>();
     }
     const int64_t np = (int64_t)S.plan_src.size();
     if (np) {
         prof.begin(KernelProfile::INJECT, s);
-        k_inject<T><<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, scale, L);
+        k_inject<T><<<nblk(np, 256), 256, 0, s>>>(planent_.as<PlanEnt>(), np, Ax, Bx, sigma, scale, cmask(), L);
         LAUNCH_CHECK();
         prof.end(s);
     }
@@ -2313,12 +2366,15 @@ void FactorizerDev::numeric_t(const double* Ax, double sigma, const double* Bx, 
     timings[2] = m12;
     timings[3] = m01 + m12;
     if (st != INT_MAX) {
-        num_successful = st;
+        // pivots before the failing one, counted over the free scalars (masked ones are the identity)
+        num_successful = S.masked() ? (int64_t)(std::lower_bound(S.uperm_to_int.begin(), S.uperm_to_int.end(),
+                                                                 (int64_t)st) - S.uperm_to_int.begin())
+                                    : st;
         ftype = 1;
-        throw Error(-3, std::to_string(st) + "/" + std::to_string(3 * S.n_red) +
+        throw Error(-3, std::to_string(num_successful) + "/" + std::to_string(S.n_free) +
                             "  pivots successful in numeric factorization (non-positive definite?)");
     }
-    num_successful = 3 * S.n_red;
+    num_successful = S.n_free;
     ftype = 2;
 }
 
@@ -2439,15 +2495,22 @@ void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
     require(2);
     if (nrhs < 1) throw Error(-1, "Must specify at least one rhs.");
     const Symbolic& S = *sym;
+    const int64_t ni = 3 * S.n_red;
+    scratch_b_.alloc((size_t)ni * nrhs * sizeof(double));
+    solve_full_with(b, x, nrhs, scratch_b_.as<double>(), stream);
+}
+
+// the full-length solve through a caller-provided factor-order vector xp (3 n_red x nrhs)
+void FactorizerDev::solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream) {
+    const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
-    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
-    scratch_b_.alloc((size_t)nr * nrhs * sizeof(double));
-    double* xp = scratch_b_.as<double>();
+    const int64_t nf = S.n_full, ni = 3 * S.n_red;
+    if (S.masked()) fill<double>(xp, ni * nrhs, 0.0, s);     // masked positions take no value
     dim3 g(nblk(nf, 256), (unsigned)nrhs);
-    k_gather_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, b, nf, xp, nr);
+    k_gather_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, b, nf, xp, ni);
     LAUNCH_CHECK();
     solve_permuted_inplace(xp, nrhs, stream);
-    k_scatter_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, xp, nr, x, nf);
+    k_scatter_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, xp, ni, x, nf);
     LAUNCH_CHECK();
 }
 
@@ -2457,23 +2520,24 @@ void FactorizerDev::solve_raw_reduced(const double* b, double* x, int nrhs, bool
     if (nrhs < 1) throw Error(-1, "Must specify at least one rhs.");
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
-    const int64_t nr = 3 * S.n_red;
-    scratch_b_.alloc((size_t)nr * nrhs * sizeof(double));
+    const int64_t nr = S.n_free, ni = 3 * S.n_red;
+    scratch_b_.alloc((size_t)ni * nrhs * sizeof(double));
     double* xp = scratch_b_.as<double>();
     dim3 g(nblk(nr, 256), (unsigned)nrhs);
-    if (already_permuted) {
+    if (already_permuted && !S.masked()) {
         k_copy<double><<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(b, xp, nr * nrhs);
         LAUNCH_CHECK();
     } else {
-        k_gather_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, b, nr, xp, nr);
+        if (S.masked()) fill<double>(xp, ni * nrhs, 0.0, s);
+        k_gather_map<<<g, 256, 0, s>>>((already_permuted ? up2i_ : r2p_).as<int64_t>(), nr, b, nr, xp, ni);
         LAUNCH_CHECK();
     }
     solve_permuted_inplace(xp, nrhs, stream);
-    if (already_permuted) {
+    if (already_permuted && !S.masked()) {
         k_copy<double><<<std::min<unsigned>(nblk(nr * nrhs, 256), 8192), 256, 0, s>>>(xp, x, nr * nrhs);
         LAUNCH_CHECK();
     } else {
-        k_scatter_map<<<g, 256, 0, s>>>(r2p_.as<int64_t>(), nr, xp, nr, x, nr);
+        k_scatter_map<<<g, 256, 0, s>>>((already_permuted ? up2i_ : r2p_).as<int64_t>(), nr, xp, ni, x, nr);
         LAUNCH_CHECK();
     }
 }
@@ -2481,27 +2545,45 @@ void FactorizerDev::solve_raw_reduced(const double* b, double* x, int nrhs, bool
 void FactorizerDev::remove_fixed(const double* xf, double* xr, int nrhs, bool permute, void* stream) {
     require(1);
     const Symbolic& S = *sym;
-    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
+    const int64_t nf = S.n_full, nr = S.n_free;
     dim3 g(nblk(nf, 256), (unsigned)nrhs);
-    k_gather_map<<<g, 256, 0, S_(stream)>>>((permute ? f2p_ : f2r_).as<int64_t>(), nf, xf, nf, xr, nr);
+    k_gather_map<<<g, 256, 0, S_(stream)>>>((permute ? f2up_ : f2r_).as<int64_t>(), nf, xf, nf, xr, nr);
     LAUNCH_CHECK();
 }
 
 void FactorizerDev::extract_full(const double* xr, double* xf, int nrhs, bool permute, void* stream) {
     require(1);
     const Symbolic& S = *sym;
-    const int64_t nf = 3 * S.n_full_blocks, nr = 3 * S.n_red;
+    const int64_t nf = S.n_full, nr = S.n_free;
     dim3 g(nblk(nf, 256), (unsigned)nrhs);
-    k_scatter_map<<<g, 256, 0, S_(stream)>>>((permute ? f2p_ : f2r_).as<int64_t>(), nf, xr, nr, xf, nf);
+    k_scatter_map<<<g, 256, 0, S_(stream)>>>((permute ? f2up_ : f2r_).as<int64_t>(), nf, xr, nr, xf, nf);
     LAUNCH_CHECK();
 }
 
 void FactorizerDev::spmv(const double* Ax, const double* x, double* y, void* stream) {
     require(1);
     const int64_t nb = sym->n_full_blocks;
+    hipStream_t s = S_(stream);
+    if (scalar_mode_) {
+        // scalar values and length-n vectors: through the block values and padded vectors
+        const int64_t n = sym->n_full;
+        scalar_values(Ax, Axb_.as<double>(), stream);
+        xpad_.alloc(3 * nb * sizeof(double));
+        ypad_.alloc(3 * nb * sizeof(double));
+        fill<double>(xpad_.as<double>(), 3 * nb, 0.0, s);
+        k_copy<double><<<std::min<unsigned>(nblk(n, 256), 8192), 256, 0, s>>>(x, xpad_.as<double>(), n);
+        LAUNCH_CHECK();
+        k_spmv<<<nblk(nb, 256), 256, 0, s>>>(spmv_ptr_.as<int64_t>(), spmv_ent_.as<int64_t>(),
+                                            spmv_col_.as<int32_t>(), Axb_.as<double>(), xpad_.as<double>(),
+                                            ypad_.as<double>(), nb);
+        LAUNCH_CHECK();
+        k_copy<double><<<std::min<unsigned>(nblk(n, 256), 8192), 256, 0, s>>>(ypad_.as<double>(), y, n);
+        LAUNCH_CHECK();
+        return;
+    }
     prof.begin(KernelProfile::SPMV, stream);
-    k_spmv<<<nblk(nb, 256), 256, 0, S_(stream)>>>(spmv_ptr_.as<int64_t>(), spmv_ent_.as<int64_t>(),
-                                                 spmv_col_.as<int32_t>(), Ax, x, y, nb);
+    k_spmv<<<nblk(nb, 256), 256, 0, s>>>(spmv_ptr_.as<int64_t>(), spmv_ent_.as<int64_t>(),
+                                        spmv_col_.as<int32_t>(), Ax, x, y, nb);
     LAUNCH_CHECK();
     prof.end(stream);
 }
@@ -2510,7 +2592,7 @@ void FactorizerDev::solve_refined(const double* Ax, const double* b, double* x, 
                                   int max_iters, int* iters, double* relres, void* stream) {
     require(2);
     hipStream_t s = S_(stream);
-    const int64_t nf = 3 * sym->n_full_blocks;
+    const int64_t nf = sym->n_full;
     scratch_c_.alloc((size_t)(3 * nf + 2 * NRED + 8) * sizeof(double));
     double* y = scratch_c_.as<double>();
     double* r = y + nf;
@@ -2559,7 +2641,7 @@ void FactorizerDev::solve_refined_segments(const double* Ax, const double* b, do
                                            const int64_t* seg, double tol, int max_iters, int* iters,
                                            double* relres_max, double* relres_seg, void* stream) {
     require(2);
-    const int64_t nf = 3 * sym->n_full_blocks;
+    const int64_t nf = sym->n_full;
     if (nseg < 1 || nseg > (int64_t)1 << 30) throw Error(-1, "bad segment count");
     if (seg[0] != 0 || seg[nseg] != nf) throw Error(-1, "segments must cover the full vector");
     for (int64_t q = 0; q < nseg; ++q)
@@ -2682,9 +2764,9 @@ void FactorizerDev::export_csc(int64_t* Lp, int64_t* Li, double* Lx) {
     if (precision != 0) {
         std::vector<double> sc(3 * S.n_red);
         HIP_CHECK(hipMemcpy(sc.data(), scale_.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
-        const int64_t n = 3 * S.n_red;
+        const int64_t n = S.n_free;
         for (int64_t j = 0; j < n; ++j)
-            for (int64_t q = Lp[j]; q < Lp[j + 1]; ++q) Lx[q] /= sc[Li[q]];
+            for (int64_t q = Lp[j]; q < Lp[j + 1]; ++q) Lx[q] /= sc[S.uperm_to_int[Li[q]]];
     }
 }
 

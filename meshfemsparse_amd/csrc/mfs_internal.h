@@ -279,9 +279,22 @@ struct Symbolic {
     std::vector<int64_t> spmv_ptr;
     std::vector<int64_t> spmv_ent;             // (blockidx << 2) | mode, mode 0 as-is, 1 transpose, 2 diag
     std::vector<int32_t> spmv_col;
-    // pin maps (scalar): full index -> permuted reduced scalar index or -1
-    std::vector<int64_t> full_to_perm;
-    std::vector<int64_t> full_to_red;
+    // Scalar variables of a kept block that are pinned (partially pinned blocks) or padding (a
+    // scalar matrix grouped into 3x3 blocks) are "masked": their rows and columns are replaced by
+    // the identity at injection, so the factor restricted to the free scalars is the Cholesky
+    // factor of the reduced matrix.  Internal vectors have 3 n_red entries in factor order; the
+    // caller's vectors have n_full (full) or n_free (reduced, pins removed) entries.
+    int64_t n_full = 0;                        // scalar length of full vectors (3 nb, or n of a scalar matrix)
+    int64_t n_free = 0;                        // free scalars = reduced length
+    std::vector<uint8_t> comp_mask;            // per factor position: bit c = component c masked (empty: none)
+    int64_t factor_nnz_free = 0;               // nnz of the factor restricted to the free scalars
+    // maps over the caller's index spaces (scalar)
+    std::vector<int64_t> full_to_perm;         // full -> internal factor-order index, -1 if pinned/padding
+    std::vector<int64_t> full_to_red;          // full -> reduced index (free scalars in order), -1 if pinned
+    std::vector<int64_t> full_to_uperm;        // full -> factor-order reduced index (masked positions skipped)
+    std::vector<int64_t> red_to_int;           // reduced -> internal factor-order index
+    std::vector<int64_t> uperm_to_int;         // factor-order reduced -> internal index
+    bool masked() const { return !comp_mask.empty(); }
 };
 
 struct SymbolicOptions {
@@ -303,6 +316,8 @@ struct SymbolicOptions {
                             // first, the rest beside the next outer block's steps
     int panel_max = 256;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
+    int64_t n_scalar = -1;  // scalar length of the caller's matrix when it is not 3 nb (a scalar
+                            // matrix grouped into blocks: scalars n_scalar .. 3 nb - 1 are padding)
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
     const SepForest* given_forest = nullptr;            // ... and its separator forest, if any
     NDState* nd_state = nullptr;                        // ordering history (temporal reuse)
@@ -319,6 +334,17 @@ struct SymbolicOptions {
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
 // of the outer trailing updates); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
+
+// A scalar symmetric matrix stored as its upper triangle in CSC (the reference's SuiteSparseMatrix
+// with UPPER_TRIANGLE symmetry, CholeskyFactorizerBase.hh:95-113) grouped into 3x3 blocks: scalar
+// v belongs to block v / 3 (the last block is padded when n is not a multiple of 3); block (I, J)
+// is stored when any scalar entry falls in it, and every diagonal block is stored.
+// value_pos[k] = position of scalar entry k in the contiguous block values (9 per block).
+struct ScalarBlocking {
+    Pattern P;
+    std::vector<int64_t> value_pos;
+};
+ScalarBlocking block_scalar_pattern(const int64_t* Ap, const int64_t* Ai, int64_t n);
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
                  const int64_t* pins, int64_t npins, const SymbolicOptions& opt);

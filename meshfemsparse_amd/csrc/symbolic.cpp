@@ -215,6 +215,46 @@ SymbolicOptions options_from_env() {
     return opt;
 }
 
+ScalarBlocking block_scalar_pattern(const int64_t* Ap, const int64_t* Ai, int64_t n) {
+    if (n <= 0) throw Error(-1, "empty matrix");
+    if (n >= (int64_t(1) << 31)) throw Error(-6, "matrix too large");
+    if (Ap[0] != 0) throw Error(-1, "column pointers must start at 0");
+    for (int64_t j = 0; j < n; ++j) {
+        if (Ap[j + 1] < Ap[j]) throw Error(-1, "column pointers must not decrease");
+        for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+            if (Ai[k] < 0 || Ai[k] > j)
+                throw Error(-1, "the scalar matrix must hold its upper triangle (row <= column)");
+            if (k > Ap[j] && Ai[k] <= Ai[k - 1]) throw Error(-1, "row indices must be sorted and unique");
+        }
+    }
+    ScalarBlocking B;
+    const int64_t nb = (n + 2) / 3;
+    B.P.n = nb;
+    B.P.Ap.assign(nb + 1, 0);
+    std::vector<int64_t> rows;
+    for (int64_t J = 0; J < nb; ++J) {
+        rows.clear();
+        for (int64_t j = 3 * J; j < std::min(n, 3 * J + 3); ++j)
+            for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) rows.push_back(Ai[k] / 3);
+        rows.push_back(J);
+        std::sort(rows.begin(), rows.end());
+        rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+        B.P.Ai.insert(B.P.Ai.end(), rows.begin(), rows.end());
+        B.P.Ap[J + 1] = (int64_t)B.P.Ai.size();
+    }
+    B.value_pos.resize(Ap[n]);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t J = j / 3;
+        const int64_t* b0 = B.P.Ai.data() + B.P.Ap[J];
+        const int64_t* b1 = B.P.Ai.data() + B.P.Ap[J + 1];
+        for (int64_t k = Ap[j]; k < Ap[j + 1]; ++k) {
+            const int64_t blk = (std::lower_bound(b0, b1, Ai[k] / 3) - B.P.Ai.data());
+            B.value_pos[k] = 9 * blk + Ai[k] % 3 + 3 * (j % 3);
+        }
+    }
+    return B;
+}
+
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t* pins,
                  int64_t npins, const SymbolicOptions& opt) {
     Symbolic S;
@@ -231,27 +271,36 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         if (Ai[Ap[j + 1] - 1] != j) throw Error(-1, "diagonal block must be last in its column");
     }
 
-    // ---- whole-block pins (CatamariFactorizer.cc:187-246) ----
-    std::vector<int8_t> pinned_comp(nb, 0);
+    // ---- pins (CatamariFactorizer.cc:187-246): whole-block pins remove the block's row and
+    // column; a partially pinned block stays and its pinned components are masked (the
+    // reference falls back to a scalar factorization there, :207-216, with the same solution).
+    // Padding scalars of a scalar matrix grouped into blocks are masked the same way. ----
+    const int64_t nsf = opt.n_scalar < 0 ? 3 * nb : opt.n_scalar;
+    if (nsf > 3 * nb || nsf <= 3 * (nb - 1)) throw Error(-1, "scalar size does not match the block count");
+    S.n_full = nsf;
+    std::vector<uint8_t> bmask(nb, 0);       // per full block: bit c = component c pinned or padding
+    for (int64_t v = nsf; v < 3 * nb; ++v) bmask[v / 3] |= (uint8_t)(1u << (v % 3));
     {
-        std::vector<char> seen((size_t)nb * 3, 0);
+        std::vector<char> seen((size_t)nsf, 0);
         for (int64_t q = 0; q < npins; ++q) {
             const int64_t v = pins[q];
-            if (v < 0 || v >= 3 * nb) throw Error(-1, "Fixed variable index out of range");
+            if (v < 0 || v >= nsf) throw Error(-1, "Fixed variable index out of range");
             if (seen[v]) continue;
             seen[v] = 1;
             S.fixed_scalar.push_back(v);
-            pinned_comp[v / 3]++;
+            bmask[v / 3] |= (uint8_t)(1u << (v % 3));
         }
         std::sort(S.fixed_scalar.begin(), S.fixed_scalar.end());
-        for (int64_t b = 0; b < nb; ++b)
-            if (pinned_comp[b] != 0 && pinned_comp[b] != 3)
-                throw Error(-6, "Partially-pinned block variables are not supported by the block factorization");
     }
+    S.n_free = nsf - (int64_t)S.fixed_scalar.size();
     S.red_of_full.assign(nb, -1);
     int64_t nr = 0;
+    bool any_mask = false;
     for (int64_t b = 0; b < nb; ++b)
-        if (!pinned_comp[b]) S.red_of_full[b] = nr++;
+        if (bmask[b] != 7) {
+            S.red_of_full[b] = nr++;
+            any_mask |= bmask[b] != 0;
+        }
     S.n_red = nr;
     if (nr == 0) throw Error(-1, "every variable is pinned");
     const int32_t n = (int32_t)nr;
@@ -979,14 +1028,52 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     }
 
     // ---- pin / permutation maps (scalar) ----
-    S.full_to_perm.assign(3 * nb, -1);
-    S.full_to_red.assign(3 * nb, -1);
-    for (int64_t b = 0; b < nb; ++b) {
-        const int64_t r = S.red_of_full[b];
-        if (r < 0) continue;
-        for (int c = 0; c < 3; ++c) {
-            S.full_to_red[3 * b + c] = 3 * r + c;
-            S.full_to_perm[3 * b + c] = 3 * (int64_t)S.pos[r] + c;
+    if (any_mask) {
+        S.comp_mask.assign(n, 0);
+        for (int64_t b = 0; b < nb; ++b)
+            if (S.red_of_full[b] >= 0) S.comp_mask[S.pos[S.red_of_full[b]]] = bmask[b];
+    }
+    const int64_t ni = 3 * (int64_t)n;
+    auto is_masked = [&](int64_t p) { return any_mask && ((S.comp_mask[p / 3] >> (p % 3)) & 1); };
+    std::vector<int64_t> int_to_uperm(ni, -1);
+    S.uperm_to_int.resize(S.n_free);
+    {
+        int64_t u = 0;
+        for (int64_t p = 0; p < ni; ++p)
+            if (!is_masked(p)) {
+                if (u >= S.n_free) throw Error(-2, "internal: free scalar count");
+                int_to_uperm[p] = u;
+                S.uperm_to_int[u++] = p;
+            }
+        if (u != S.n_free) throw Error(-2, "internal: free scalar count");
+    }
+    S.full_to_perm.assign(nsf, -1);
+    S.full_to_red.assign(nsf, -1);
+    S.full_to_uperm.assign(nsf, -1);
+    S.red_to_int.resize(S.n_free);
+    int64_t ur = 0;
+    for (int64_t v = 0; v < nsf; ++v) {
+        const int64_t r = S.red_of_full[v / 3];
+        if (r < 0 || ((bmask[v / 3] >> (v % 3)) & 1)) continue;
+        const int64_t p = 3 * (int64_t)S.pos[r] + v % 3;
+        S.full_to_perm[v] = p;
+        S.full_to_red[v] = ur;
+        S.full_to_uperm[v] = int_to_uperm[p];
+        S.red_to_int[ur++] = p;
+    }
+    // nnz of the factor restricted to the free scalars (masked rows and columns hold the identity)
+    S.factor_nnz_free = S.factor_nnz;
+    if (any_mask) {
+        S.factor_nnz_free = 0;
+        for (int32_t sn = 0; sn < S.ns; ++sn) {
+            const int64_t W = 3 * (int64_t)S.swidth[sn], D = 3 * (int64_t)S.sdeg[sn];
+            int64_t struct_free = 0;
+            for (int64_t r = 0; r < D; ++r)
+                struct_free += !is_masked(3 * (int64_t)S.rows[S.srows_off[sn] + r / 3] + r % 3);
+            const int64_t f0 = 3 * (int64_t)S.sfirst[sn];
+            int64_t below = 0;      // free own columns at or after c
+            for (int64_t c = W - 1; c >= 0; --c)
+                if (!is_masked(f0 + c)) S.factor_nnz_free += ++below + struct_free;
         }
     }
     return S;

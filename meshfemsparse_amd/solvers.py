@@ -16,7 +16,7 @@ import torch
 from . import _lib
 from ._lib import MFSError, NotPositiveDefiniteError, c_vp, ptr
 from .ordering import NDOrdering, TemporalReuseStatistics, _stats
-from .sparse import _dptr, _stream
+from .sparse import SuiteSparseMatrix, _dptr, _stream
 
 
 class CholeskyProvider(enum.Enum):
@@ -207,34 +207,46 @@ class GpuSupernodalFactorizer:
 
     # -- factorization --------------------------------------------------------------------------
     def factorizeSymbolic(self, H, pinnedVars=()):
+        """BlockCSCHessian (3x3 blocks) or SuiteSparseMatrix (scalar, upper triangle) overloads.
+        Pins may split blocks: a partially pinned block keeps its free components."""
         pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
-        _lib.call("mfs_factorize_symbolic", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
-                  H.n, 3, ptr(pins, _lib.c_i64), len(pins))
+        scalar = isinstance(H, SuiteSparseMatrix)
+        if scalar:
+            _lib.call("mfs_factorize_symbolic_scalar", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
+                      H.n, ptr(pins, _lib.c_i64), len(pins))
+        else:
+            _lib.call("mfs_factorize_symbolic", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
+                      H.n, 3, ptr(pins, _lib.c_i64), len(pins))
         self._fixed = [int(p) for p in pins]
-        self._n_full = 3 * H.n
-        self._n_red = 3 * H.n - len(self._fixed)
-        self._Hpattern = (H.n, H.nz)
+        self._n_full = H.n if scalar else 3 * H.n
+        self._n_red = self._n_full - len(self._fixed)
+        self._Hpattern = (scalar, H.n, H.nz)
 
     def _check_pattern(self, H):
-        if self._Hpattern is not None and self._Hpattern != (H.n, H.nz):
+        if self._Hpattern is not None and self._Hpattern != (isinstance(H, SuiteSparseMatrix), H.n, H.nz):
             raise ValueError("matrix pattern differs from the symbolic factorization")
         if H.isSparsityOnly():
             raise ValueError("cannot factorize a sparsity-only matrix")
 
+    def _numeric(self, A, sigma, Bp):
+        name = "mfs_factorize_numeric_scalar" if isinstance(A, SuiteSparseMatrix) else "mfs_factorize_numeric"
+        _lib.call(name, self._h, _dptr(A.Ax), float(sigma), Bp, _stream())
+
     def factorizeNumeric(self, H, isInTryCatch=False):
         self.assertFactorization(FactorizationType.Symbolic)
         self._check_pattern(H)
-        _lib.call("mfs_factorize_numeric", self._h, _dptr(H.Ax), 0.0, None, _stream())
+        self._numeric(H, 0.0, None)
 
     def factorizeNumericWithShift(self, A, sigma, B=None, isInTryCatch=False):
         self.assertFactorization(FactorizationType.Symbolic)
         self._check_pattern(A)
         Bp = None
         if B is not None:
-            if B.n != A.n or B.nz != A.nz or not np.array_equal(B.Ap, A.Ap) or not np.array_equal(B.Ai, A.Ai):
+            if (type(B) is not type(A) or B.n != A.n or B.nz != A.nz or not np.array_equal(B.Ap, A.Ap)
+                    or not np.array_equal(B.Ai, A.Ai)):
                 raise ValueError("Shift matrix must have the same sparsity and value layout as A")
             Bp = _dptr(B.Ax)
-        _lib.call("mfs_factorize_numeric", self._h, _dptr(A.Ax), float(sigma), Bp, _stream())
+        self._numeric(A, sigma, Bp)
 
     def factorize(self, H, fixedVars=()):
         self.factorizeSymbolic(H, fixedVars)
@@ -364,7 +376,7 @@ class GpuSupernodalFactorizer:
         self._fixed = [int(p) for p in pins]
         self._n_full = 3 * H.n
         self._n_red = 3 * H.n - len(self._fixed)
-        self._Hpattern = (H.n, H.nz)
+        self._Hpattern = (False, H.n, H.nz)
         table = {(int(a), int(b)): float(ms[i * len(tz) + j]) for i, a in enumerate(fz) for j, b in enumerate(tz)}
         return {"fuse_degree": int(best[0]), "big_tile": int(best[1]), "ms": table}
 

@@ -123,6 +123,54 @@ class BlockCSCHessian:
         return A
 
 
+class SuiteSparseMatrix:
+    """Scalar symmetric matrix stored as its upper triangle in CSC: the reference's
+    SuiteSparseMatrix with UPPER_TRIANGLE symmetry, the argument type of the scalar
+    CholeskyFactorizerBase overloads (CholeskyFactorizerBase.hh:95-113).  Ap/Ai are int64 host
+    arrays (rows sorted, row <= column); Ax holds the nz values on the device."""
+
+    blockSize = 1
+
+    def __init__(self, n, Ap, Ai, Ax=None, device="cuda"):
+        self.n = self.m = int(n)
+        self.Ap = np.ascontiguousarray(Ap, dtype=np.int64)
+        self.Ai = np.ascontiguousarray(Ai, dtype=np.int64)
+        if len(self.Ap) != self.n + 1:
+            raise ValueError("Ap must have n + 1 entries")
+        self.nz = int(self.Ap[-1])
+        self.Ax = Ax
+        self.device = device
+
+    @classmethod
+    def fromTriplets(cls, n, rows, cols, vals, device="cuda"):
+        """TripletMatrix -> SuiteSparseMatrix(UPPER_TRIANGLE): entries with row > col are
+        mirrored into the upper triangle, duplicates summed."""
+        import torch
+        r = np.asarray(rows, np.int64)
+        c = np.asarray(cols, np.int64)
+        i, j = np.minimum(r, c), np.maximum(r, c)
+        key = j * n + i
+        uk, inv = np.unique(key, return_inverse=True)
+        v = np.zeros(len(uk))
+        np.add.at(v, inv, np.asarray(vals, np.float64))
+        jj, ii = uk // n, uk % n
+        Ap = np.zeros(n + 1, np.int64)
+        np.add.at(Ap, jj + 1, 1)
+        Ap = np.cumsum(Ap)
+        return cls(n, Ap, ii, torch.from_numpy(v).to(device), device)
+
+    def isSparsityOnly(self):
+        return self.Ax is None
+
+    def toDense(self):
+        A = np.zeros((self.n, self.n))
+        Ax = self.Ax.cpu().numpy()
+        for j in range(self.n):
+            for k in range(self.Ap[j], self.Ap[j + 1]):
+                A[self.Ai[k], j] = A[j, self.Ai[k]] = Ax[k]
+        return A
+
+
 class SystemAssembler:
     """SystemAssembler<3> over `numBlockVars` 3-vector variables."""
 

@@ -121,3 +121,34 @@ def test_autotune_keeps_a_candidate_and_the_factor(dev):
     b = torch.from_numpy(-g).to(dev)
     x1, x2 = f.solve(b), ref.solve(b)
     assert torch.linalg.norm(x1 - x2) <= 1e-11 * torch.linalg.norm(x2)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("fuse", [0, 100000])
+def test_batch_factor_matches_oracle_cholesky(dev, single, fuse, monkeypatch):
+    """The batched factor against the oracle's up-looking Cholesky factor of the whole
+    block-diagonal system under the batch ordering (CatamariFactorizer.cc:634-714: every mesh's
+    block must be that mesh's unique factor): 1e-12 in FP64, 2e-5 column-relative in FP32; no
+    entry couples two meshes."""
+    from tests.test_gpu_factor import oracle_factor
+    monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
+    B, H, He, g, Ee = batch_problem(dev, nmeshes=6, nmin=2, nmax=5, seed=5)
+    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+    f.factorizeSymbolic(H, B["pins"])
+    f.factorizeNumeric(H)
+    if fuse:
+        assert f.scheduleStats()["fused_tasks"] > 0
+    (Lp, Li, Lx), red, pos = oracle_factor(H, f)
+    Lo = orc.dense_L(Lp, Li, Lx)
+    Lg = orc.dense_L(*f.exportFactorCSC())
+    if single:
+        cn = np.abs(Lo).max(axis=0)
+        assert (np.abs(Lg - Lo).max(axis=0) <= 2e-5 * cn).all()
+    else:
+        assert np.abs(Lg - Lo).max() <= 1e-12 * np.abs(Lo).max()
+    # mesh of every factor-order scalar column: the factor is block diagonal over the meshes
+    ip = f.getInversePermutation()                       # factor position -> reduced scalar
+    free = np.setdiff1d(np.arange(3 * H.n), B["pins"])
+    mesh_of = np.searchsorted(B["seg"], free[ip], side="right") - 1
+    rows, cols = np.nonzero(Lg)
+    assert np.array_equal(mesh_of[rows], mesh_of[cols])

@@ -13,6 +13,7 @@ import torch
 import meshfemsparse_amd as mfs
 from meshfemsparse_amd import workload as wl
 from meshfemsparse_amd import _lib
+from tests import oracle_lib as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -92,3 +93,44 @@ def test_device_elements_drive_the_newton_step(dev):
     f.factorizeNumeric(H2)
     x2, _, _ = f.solveRefined(H2, T(-g_h), tol=1e-12)
     assert torch.linalg.norm(x - x2) <= 1e-11 * torch.linalg.norm(x2)
+
+
+def test_element_kernel_matches_energy_finite_differences(dev):
+    """The device element Hessians and gradients against central finite differences of the
+    oracle's energy (orc_neohookean_energy: psi = mu/2 (I1 - 3) - mu ln J + lambda/2 (ln J)^2
+    times the rest volume), element by element, on a strongly deformed mesh with per-element
+    E and nu.  Steps 1e-5 h (gradient) and 1e-4 h (Hessian): truncation ~1e-9 and ~3e-8
+    relative, rounding below 1e-10, so the gradient must agree to 1e-8 and the Hessian to 1e-6
+    (normwise per element); a dropped term or a wrong sign in either formula is off by O(1)."""
+    N = 3
+    tets, Xr, X, H, da = setup(dev, N, seed=5)
+    rng = np.random.default_rng(11)
+    Ee = 10 ** rng.uniform(3, 6, len(tets))
+    nue = rng.uniform(0.2, 0.45, len(tets))
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    He_d, g12 = da.neohookean(T(Xr), T(X), E_per_elem=T(Ee), nu_per_elem=T(nue))
+    He_d = He_d.cpu().numpy().reshape(-1, 12, 12)       # column-major: He_d[e, col, row]
+    g12 = g12.cpu().numpy().reshape(-1, 12)
+    d = 1e-4 / N
+    for e in rng.choice(len(tets), 12, replace=False):
+        t = tets[e:e + 1]
+        loc = np.zeros((4, 3))
+        loc[:] = X[t[0]]
+        Xl = np.array(Xr[t[0]])
+        tl = np.arange(4, dtype=np.int32)[None, :]
+
+        def psi(y):
+            return orc.energy(Xl, y.reshape(4, 3), tl, Ee=Ee[e:e + 1], nue=nue[e:e + 1])
+
+        x0 = loc.ravel()
+        eye = np.eye(12) * d
+        dg = d / 10
+        gfd = np.array([(psi(x0 + eye[i] / 10) - psi(x0 - eye[i] / 10)) / (2 * dg) for i in range(12)])
+        Hfd = np.empty((12, 12))
+        for i in range(12):
+            for j in range(12):
+                Hfd[i, j] = (psi(x0 + eye[i] + eye[j]) - psi(x0 + eye[i] - eye[j]) - psi(x0 - eye[i] + eye[j])
+                             + psi(x0 - eye[i] - eye[j])) / (4 * d * d)
+        assert np.linalg.norm(g12[e] - gfd) <= 1e-8 * np.linalg.norm(gfd), e
+        Hd = He_d[e].T                                 # [row, col]
+        assert np.linalg.norm(Hd - Hfd) <= 1e-6 * np.linalg.norm(Hfd), e

@@ -358,11 +358,96 @@ def test_single_refined_newton_step_16cube(dev):
     assert e1 < 1e-6 and e1 < e0
 
 
-def test_partially_pinned_block_is_rejected(dev):
+@pytest.mark.parametrize("single", [False, True])
+def test_check_solves_chain_partial_pin(dev, single):
+    """The reference's pin mode 2 (test_catamari_factorizer.cc:121): one component of block 0
+    pinned.  The block stays with that component masked (the reference's scalar fallback,
+    CatamariFactorizer.cc:207-216, solves the same reduced system)."""
     H = chain_problem(dev)
+    check_solves(dev, H, [1], single)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_check_solves_mesh_partial_pins(dev, single):
+    """A mesh with whole-block (z = 0) pins and single components pinned on other nodes."""
+    H, pins, _, _, _ = mesh_problem(dev, 4, seed=5)
+    extra = [3 * 40 + 0, 3 * 41 + 2, 3 * 57 + 1, 3 * 57 + 2, 3 * 90 + 1]
+    check_solves(dev, H, sorted(set(list(pins) + extra)), single)
+
+
+def scalar_chain(dev, BS, nb=20):
+    """checkBlocks<BS> of the reference (test_catamari_factorizer.cc:100-122) as the scalar
+    matrix it expands to: chain stencils {i, i+1} of BS x BS blocks, 10.1 on the diagonal and
+    -0.1 (1 + (i + j) % 5) elsewhere."""
+    rows, cols, vals = [], [], []
+    for b in range(nb):
+        for bj in (b, b + 1):
+            if bj >= nb:
+                continue
+            for r in range(BS):
+                for c in range(BS):
+                    i, j = BS * b + r, BS * bj + c
+                    if i <= j:
+                        rows.append(i)
+                        cols.append(j)
+                        vals.append(10.1 if i == j else -0.1 * (1 + (i + j) % 5))
+    return mfs.SuiteSparseMatrix.fromTriplets(BS * nb, rows, cols, vals, device=dev)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("BS", [1, 2])
+@pytest.mark.parametrize("pin_mode", [0, 1, 2])
+def test_check_solves_scalar_blocks(dev, BS, pin_mode, single):
+    """The reference's "scalar blocks" / "blocks of size two" sections through the scalar
+    SuiteSparseMatrix overloads (CholeskyFactorizerBase.hh:95-113)."""
+    A = scalar_chain(dev, BS)
+    pins = list(range(BS)) if pin_mode == 1 else [1] if pin_mode == 2 else []
+    check_solves(dev, A, pins, single)
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("pinned", [False, True])
+def test_check_solves_plain_scalar_matrix(dev, pinned, single):
+    """The reference's "plain scalar matrix" section (test_catamari_factorizer.cc:152-161): a
+    20 x 20 tridiagonal matrix, 4.1 on the diagonal and -0.7 beside it, pins {} or {1}."""
+    n = 20
+    rows = list(range(n)) + list(range(n - 1))
+    cols = list(range(n)) + list(range(1, n))
+    vals = [4.1] * n + [-0.7] * (n - 1)
+    A = mfs.SuiteSparseMatrix.fromTriplets(n, rows, cols, vals, device=dev)
+    check_solves(dev, A, [1] if pinned else [], single)
+
+
+def test_scalar_matrix_factor_matches_lapack(dev):
+    """A random sparse SPD scalar matrix (n = 61, not a multiple of 3) with pins: the exported
+    factor is the unique Cholesky factor of the reduced matrix in the analysis order, which
+    getInversePermutation reports (LAPACK via numpy), and the solve matches a dense solve."""
+    rng = np.random.default_rng(3)
+    n = 61
+    M = (rng.random((n, n)) < 0.08) * rng.standard_normal((n, n))
+    A = M @ M.T + n * np.eye(n)
+    iu, ju = np.nonzero(np.triu(A))
+    S = mfs.SuiteSparseMatrix.fromTriplets(n, iu, ju, A[iu, ju], device=dev)
+    pins = [0, 5, 6, 60]
     f = mfs.make_cholesky_factorizer()
-    with pytest.raises(mfs.UnsupportedError):
-        f.factorizeSymbolic(H, [1])
+    f.factorize(S, pins)
+    free = np.setdiff1d(np.arange(n), pins)
+    assert f.n_reduced() == len(free)
+    order = f.getInversePermutation()            # factor position -> reduced index
+    assert np.array_equal(np.sort(order), np.arange(len(free)))
+    Ared = A[np.ix_(free, free)][np.ix_(order, order)]
+    Lref = np.linalg.cholesky(Ared)
+    Lg = orc.dense_L(*f.exportFactorCSC())
+    assert np.abs(Lg - Lref).max() <= 1e-12 * np.abs(Lref).max()
+    b = rng.standard_normal(n)
+    x = f.solve(torch.from_numpy(b).to(dev)).cpu().numpy()
+    xo = np.zeros(n)
+    xo[free] = np.linalg.solve(A[np.ix_(free, free)], b[free])
+    assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo)
+    y = f.apply(S, torch.from_numpy(x).to(dev)).cpu().numpy()
+    assert np.linalg.norm(y - A @ x) <= 1e-13 * np.linalg.norm(A @ x)
+    xr, it, rr = f.solveRefined(S, torch.from_numpy(b).to(dev), tol=1e-12)
+    assert rr <= 1e-12
 
 
 def test_solve_without_factor_raises(dev):

@@ -159,17 +159,65 @@ def test_node_refined_separators_give_a_consistent_analysis(monkeypatch):
     test_symbolic_counts_match_oracle_factor(7)
 
 
-def test_partial_pins_are_rejected():
-    tets, X = wl.cube_mesh(1)
-    H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
-    with pytest.raises(_lib.UnsupportedError):
-        analyze(H, np.array([1]))
+def dense_upper_blocks(H, Ax):
+    """Dense symmetric matrix of a block-CSC upper pattern with values Ax (numpy)."""
+    A = np.zeros((3 * H.n, 3 * H.n))
+    for j in range(H.n):
+        for k in range(H.Ap[j], H.Ap[j + 1]):
+            i = H.Ai[k]
+            B = Ax[9 * k:9 * k + 9].reshape(3, 3).T
+            if i == j:
+                B = np.triu(B) + np.triu(B, 1).T
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = B
+            else:
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = B
+                A[3 * j:3 * j + 3, 3 * i:3 * i + 3] = B.T
+    return A
+
+
+def free_factor_order(H, pins, iperm_blocks):
+    """Free scalars in factor order: blocks in the analysis order, free components ascending."""
+    pinned = np.zeros(3 * H.n, bool)
+    pinned[pins] = True
+    kept = [b for b in range(H.n) if not pinned[3 * b:3 * b + 3].all()]
+    return np.array([3 * kept[r] + c for r in iperm_blocks for c in range(3) if not pinned[3 * kept[r] + c]])
+
+
+@pytest.mark.parametrize("pins", [[1], [4, 5], [0, 1, 2, 7], [2, 9, 10, 11, 13, 23]])
+def test_partially_pinned_blocks_factor_the_reduced_matrix(pins):
+    """Partially pinned blocks stay in the block pattern with their pinned components masked to
+    the identity (CatamariFactorizer.cc:207-216 falls back to a scalar factorization instead; the
+    reduced system and its solution are the same).  The host replay of the device schedule gives
+    the unique Cholesky factor of the reduced matrix in the analysis order (LAPACK), and its
+    forward/backward sweeps solve the reduced system."""
+    N = 3
+    tets, Xr = wl.cube_mesh(N)
+    X = wl.perturb(N, Xr, 3, sigma_rel=0.02)
+    He, g, _ = wl.neohookean(Xr, X, tets)
+    H = mfs.SystemAssembler(len(Xr)).blockSparsityPattern(tets)
+    Ax = orc.assemble(H.Ap, H.Ai, H.n, tets, He)
+    pins = np.array(sorted(set(pins + list(range(3 * (N + 1) ** 2)))), np.int64)   # z = 0 plane too
+    iperm, st, fl = analyze(H, pins)
+    order = free_factor_order(H, pins, iperm)
+    assert len(order) == 3 * H.n - len(pins)
+    A = dense_upper_blocks(H, Ax)[np.ix_(order, order)]
+    b = np.random.default_rng(1).standard_normal(len(order))
+    _, (Lp, Li, Lx), fc, xs = emulate(H, pins, Ax, x_perm=b.copy())
+    assert fc == -1
+    L = orc.dense_L(Lp, Li, Lx)
+    Lref = np.linalg.cholesky(A)
+    assert np.abs(L - Lref).max() <= 1e-12 * np.abs(Lref).max()
+    assert np.linalg.norm(xs - np.linalg.solve(A, b)) <= 1e-11 * np.linalg.norm(xs)
+    # a failing pivot is reported as the number of free scalars factored before it
+    _, _, fc = emulate(H, pins, -Ax)
+    assert fc == 0
 
 
 def emulate(H, pins, Ax, sigma=0.0, x_perm=None):
     iperm, st, fl = analyze(H, pins)
     nred, nnz = st[0], st[3]
-    Lp = np.zeros(3 * nred + 1, np.int64)
+    nfree = 3 * H.n - len(np.unique(pins))
+    Lp = np.zeros(nfree + 1, np.int64)
     Li = np.zeros(nnz, np.int64)
     Lx = np.zeros(nnz)
     fc = np.zeros(1, np.int64)
@@ -296,3 +344,27 @@ def test_batch_analysis_is_per_mesh_analysis():
         nred += stm[0]
     assert st[0] == nred and st[3] == nnz
     assert abs(fl[0] - flops) <= 1e-12 * flops
+
+
+def test_nd_order_rejects_a_one_triangle_graph():
+    """mfs_nd_order takes the graph with both triangles; a pattern holding only one triangle
+    (every edge listed once) is rejected, not ordered."""
+    tets, X = wl.cube_mesh(3)
+    H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
+    n = H.n
+    up = [(int(H.Ai[k]), j) for j in range(n) for k in range(H.Ap[j], H.Ap[j + 1]) if H.Ai[k] != j]
+
+    def csc(pairs):
+        cols = [[] for _ in range(n)]
+        for i, j in pairs:
+            cols[j].append(i)
+        p = np.zeros(n + 1, np.int64)
+        p[1:] = np.cumsum([len(c) for c in cols])
+        return p, np.array([i for c in cols for i in sorted(c)], np.int64)
+
+    nd = mfs.NestedDissection()
+    perm = nd.order(*csc(up + [(j, i) for i, j in up]))
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    for one in (up, [(j, i) for i, j in up]):
+        with pytest.raises(_lib.InvalidArgumentError):
+            mfs.NestedDissection().order(*csc(one))
