@@ -15,6 +15,7 @@ MFS_OK, MFS_ERR_ARG, MFS_ERR_STATE, MFS_ERR_NOT_SPD, MFS_ERR_DEVICE, MFS_ERR_ALL
 MFS_ERR_NOT_CONVERGED = -7
 MFS_PREC_FP64, MFS_PREC_FP32, MFS_PREC_BF16, MFS_PREC_BF16X3 = 0, 1, 2, 3
 MFS_HE_FULL12, MFS_HE_UPPER78 = 0, 1
+MFS_SOLVE_NOALLOC_MAX_BYTES = 64 << 20
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 _SIGS = {
@@ -38,6 +39,7 @@ _SIGS = {
     "mfs_factorize_symbolic_scalar": [c_vp, P_i64, P_i64, c_i64, P_i64, c_i64],
     "mfs_factorize_numeric_scalar": [c_vp, c_vp, c_dbl, c_vp, c_vp],
     "mfs_solve": [c_vp, c_vp, c_vp, c_i32, c_vp],
+    "mfs_solve_noalloc": [c_vp, c_vp, c_vp, c_i32, c_vp],
     "mfs_solve_raw_reduced": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],
     "mfs_remove_fixed_entries": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],
     "mfs_extract_full_solution": [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp],

@@ -210,6 +210,14 @@ int mfs_solve(mfs_factorizer f, const double* b, double* x, int32_t nrhs, void* 
     MFS_CATCH
 }
 
+int mfs_solve_noalloc(mfs_factorizer f, const double* b, double* x, int32_t nrhs, void* stream) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(b); CHECK_PTR(x);
+    if (f->dev->prof.on) throw mfs::Error(MFS_ERR_STATE, "the allocation-free solve needs kernel profiling off");
+    f->dev->solve_in(b, x, nrhs, mfs::noalloc_scratch(), MFS_SOLVE_NOALLOC_MAX_BYTES, stream);
+    MFS_CATCH
+}
+
 int mfs_solve_raw_reduced(mfs_factorizer f, const double* b, double* x, int32_t nrhs,
                           int32_t already_permuted, void* stream) {
     MFS_TRY

@@ -78,6 +78,10 @@ public:
     void numeric_scalar(const double* Ax, double sigma, const double* Bx, void* stream);
     bool scalar_mode() const { return scalar_mode_; }
     void solve(const double* b, double* x, int nrhs, void* stream);
+    // the same solve with every scratch vector carved out of `base` (solve_scratch_bytes(nrhs)
+    // bytes of device memory): allocates nothing
+    void solve_in(const double* b, double* x, int nrhs, void* base, size_t bytes, void* stream);
+    size_t solve_scratch_bytes(int nrhs) const;
     void solve_raw_reduced(const double* b, double* x, int nrhs, bool already_permuted, void* stream);
     void remove_fixed(const double* xf, double* xr, int nrhs, bool permute, void* stream);
     void extract_full(const double* xr, double* xf, int nrhs, bool permute, void* stream);
@@ -124,11 +128,20 @@ private:
     void drop_graph();
     template <typename T> void enqueue_numeric(const double* Ax, double sigma, const double* Bx, void* stream,
                                                void* ev_inject);
-    void solve_permuted_inplace(double* xp, int nrhs, void* stream);   // factor-order vectors
-    void solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream);
+    // scratch of one solve (nrhs columns); null members: the factorizer's own buffers
+    struct SolveScratch {
+        double* xp = nullptr;   // factor-order right-hand sides (3 n_red x nrhs doubles)
+        void* Y = nullptr;      // forward results y_b (3 n_red x nrhs, factor type)
+        void* u = nullptr;      // update vectors (u_solve_size x nrhs)
+        void* part = nullptr;   // backward partial sums (bpart_size x solve_block x nrhs)
+        float* xf = nullptr;    // FP32 copy of xp (single-precision factors)
+    };
+    void solve_permuted_inplace(double* xp, int nrhs, void* stream, const SolveScratch* ws = nullptr);
+    void solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream,
+                         const SolveScratch* ws = nullptr);
     void scalar_values(const double* Ax, double* Axb, void* stream);   // scalar -> block values
     template <typename T> void numeric_t(const double* Ax, double sigma, const double* Bx, void* stream);
-    template <typename T> void solve_t(T* xp, int nrhs, void* stream);
+    template <typename T> void solve_t(T* xp, int nrhs, void* stream, const SolveScratch* ws);
     void ensure_vec(DeviceArray& a, size_t bytes);
     size_t fsize() const;
 
@@ -165,6 +178,10 @@ private:
     int stash_type_ = 0;
     int64_t stash_succ_ = 0;
 };
+
+// device address of the allocation-free solve's scratch (MFS_SOLVE_NOALLOC_MAX_BYTES) on the
+// current device, looked up on its first use there
+void* noalloc_scratch();
 
 // test hooks
 template <typename T>

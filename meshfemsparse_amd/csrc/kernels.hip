@@ -20,6 +20,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <mutex>
+
+#include "../../include/meshfemsparse_amd.h"
 #include "device.h"
 #include "neohookean.h"
 
@@ -2391,19 +2394,26 @@ void FactorizerDev::numeric(const double* Ax, double sigma, const double* Bx, vo
 // the final x of each supernode's structure rows, then per step (last block first) partial sums
 // of L(rows below, b)^T x and x_b = X_b^T (y_b - partials) (into xp).
 template <typename T>
-void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
+void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* ws) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
     const int64_t ldx = 3 * S.n_red;
     const int64_t ldu = std::max<int64_t>(S.u_solve_size, 1);
     const int64_t ldp = S.bpart_size * S.solve_block;
     const size_t fs = sizeof(T);
-    usolve_.alloc((size_t)ldu * nrhs * fs);
-    bpart_.alloc((size_t)ldp * nrhs * fs);
-    ysolve_.alloc((size_t)ldx * nrhs * fs);
-    T* u = usolve_.as<T>();
-    T* part = bpart_.as<T>();
-    T* Y = ysolve_.as<T>();
+    T *u, *part, *Y;
+    if (ws) {
+        u = static_cast<T*>(ws->u);
+        part = static_cast<T*>(ws->part);
+        Y = static_cast<T*>(ws->Y);
+    } else {
+        usolve_.alloc((size_t)ldu * nrhs * fs);
+        bpart_.alloc((size_t)ldp * nrhs * fs);
+        ysolve_.alloc((size_t)ldx * nrhs * fs);
+        u = usolve_.as<T>();
+        part = bpart_.as<T>();
+        Y = ysolve_.as<T>();
+    }
     const T* L = L_.as<T>();
     const T* Xb = Xb_.as<T>();
     const SolveBlock* blk = sblk_.as<SolveBlock>();
@@ -2469,22 +2479,27 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream) {
 }
 
 // factor-order double vectors in/out; FP32 factors solve the scaled system in single precision
-void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream) {
+void FactorizerDev::solve_permuted_inplace(double* xp, int nrhs, void* stream, const SolveScratch* ws) {
     const Symbolic& S = *sym;
     if (precision == 0) {
-        solve_t<double>(xp, nrhs, stream);
+        solve_t<double>(xp, nrhs, stream, ws);
         return;
     }
     hipStream_t s = S_(stream);
     const int64_t n = 3 * S.n_red;
-    xf_.alloc((size_t)n * nrhs * sizeof(float));
-    float* xf = xf_.as<float>();
+    float* xf;
+    if (ws) {
+        xf = ws->xf;
+    } else {
+        xf_.alloc((size_t)n * nrhs * sizeof(float));
+        xf = xf_.as<float>();
+    }
     dim3 g(nblk(n, 256), (unsigned)nrhs);
     prof.begin(KernelProfile::CONVERT, s);
     k_to_factor<float><<<g, 256, 0, s>>>(xp, scale_.as<double>(), xf, n, n);
     LAUNCH_CHECK();
     prof.end(s);
-    solve_t<float>(xf, nrhs, stream);
+    solve_t<float>(xf, nrhs, stream, ws);
     prof.begin(KernelProfile::CONVERT, s);
     k_from_factor<float><<<g, 256, 0, s>>>(xf, scale_.as<double>(), xp, n, n);
     LAUNCH_CHECK();
@@ -2500,8 +2515,48 @@ void FactorizerDev::solve(const double* b, double* x, int nrhs, void* stream) {
     solve_full_with(b, x, nrhs, scratch_b_.as<double>(), stream);
 }
 
+// Bytes of one solve's scratch for nrhs columns, each piece 256-byte aligned: xp, Y, u, part
+// (and xf for single-precision factors).
+size_t FactorizerDev::solve_scratch_bytes(int nrhs) const {
+    const Symbolic& S = *sym;
+    const size_t fs = fsize();
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t ni = 3 * (size_t)S.n_red, r = (size_t)nrhs;
+    size_t b = up(ni * r * sizeof(double)) + up(ni * r * fs) +
+               up((size_t)std::max<int64_t>(S.u_solve_size, 1) * r * fs) +
+               up((size_t)std::max<int64_t>(S.bpart_size * S.solve_block, 1) * r * fs);
+    if (precision != 0) b += up(ni * r * sizeof(float));
+    return b;
+}
+
+void FactorizerDev::solve_in(const double* b, double* x, int nrhs, void* base, size_t bytes, void* stream) {
+    require(2);
+    if (nrhs < 1) throw Error(-1, "Must specify at least one rhs.");
+    const size_t need = solve_scratch_bytes(nrhs);
+    if (need > bytes)
+        throw Error(-1, "the solve needs " + std::to_string(need) + " bytes of scratch for " + std::to_string(nrhs) +
+                            " right-hand side(s); the allocation-free solve has " + std::to_string(bytes));
+    const Symbolic& S = *sym;
+    const size_t fs = fsize();
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t ni = 3 * (size_t)S.n_red, r = (size_t)nrhs;
+    char* p = static_cast<char*>(base);
+    SolveScratch ws;
+    ws.xp = reinterpret_cast<double*>(p);
+    p += up(ni * r * sizeof(double));
+    ws.Y = p;
+    p += up(ni * r * fs);
+    ws.u = p;
+    p += up((size_t)std::max<int64_t>(S.u_solve_size, 1) * r * fs);
+    ws.part = p;
+    p += up((size_t)std::max<int64_t>(S.bpart_size * S.solve_block, 1) * r * fs);
+    if (precision != 0) ws.xf = reinterpret_cast<float*>(p);
+    solve_full_with(b, x, nrhs, ws.xp, stream, &ws);
+}
+
 // the full-length solve through a caller-provided factor-order vector xp (3 n_red x nrhs)
-void FactorizerDev::solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream) {
+void FactorizerDev::solve_full_with(const double* b, double* x, int nrhs, double* xp, void* stream,
+                                    const SolveScratch* ws) {
     const Symbolic& S = *sym;
     hipStream_t s = S_(stream);
     const int64_t nf = S.n_full, ni = 3 * S.n_red;
@@ -2509,7 +2564,7 @@ void FactorizerDev::solve_full_with(const double* b, double* x, int nrhs, double
     dim3 g(nblk(nf, 256), (unsigned)nrhs);
     k_gather_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, b, nf, xp, ni);
     LAUNCH_CHECK();
-    solve_permuted_inplace(xp, nrhs, stream);
+    solve_permuted_inplace(xp, nrhs, stream, ws);
     k_scatter_map<<<g, 256, 0, s>>>(f2p_.as<int64_t>(), nf, xp, ni, x, nf);
     LAUNCH_CHECK();
 }
@@ -2908,6 +2963,36 @@ void test_multiply_lower_block(int BS, const int64_t* indices, int64_t start, in
     k_mlb<<<g, 256, (size_t)size * sizeof(double), s>>>(BS, indices, start, size, degree, A, lda, B, ldb);
     LAUNCH_CHECK();
     HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free solve: its scratch is this file-scope device array (one copy per GPU, held
+// until the process ends).  Host code never passes the array's name to a kernel: that is a
+// host-side stand-in, not a device address.  The device address is looked up once per device
+// with hipGetSymbolAddress and only that pointer is used.  Every allocation-free solve on one
+// GPU shares the array, so calls that overlap on one GPU race.
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[MFS_SOLVE_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}
+
+void* noalloc_scratch() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(-4, "device index out of range for the solve scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(-4, std::string("hipGetSymbolAddress of the solve scratch failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 } // namespace mfs
