@@ -278,7 +278,8 @@ int mfs_temporal_reuse_statistics(mfs_factorizer f, int64_t* stats4, double* nd_
  * uses it (nested_dissection_from_graph + TemporalReuseState, cholmod_nesdis_parallel.hh:41-75).
  * Graph: CSC of both triangles (p[n+1], i[p[n]]; diagonal entries ignored; a graph that does not
  * list every edge in both directions is rejected with MFS_ERR_ARG); vertex ids must keep
- * their meaning between calls.  mfs_nd_order writes perm[new] = old and the number of
+ * their meaning between calls.  The dissection knobs of the environment (MFS_ND_*, MFS_LEAF)
+ * apply as in the factorizer.  mfs_nd_order writes perm[new] = old and the number of
  * components; mfs_nd_forest then gives CParent[num_components] and CMember[n] of that call.   */
 int mfs_nd_create(mfs_nd* out);
 int mfs_nd_destroy(mfs_nd h);

@@ -557,7 +557,7 @@ int mfs_nd_order(mfs_nd h, int64_t n, const int64_t* p, const int64_t* i, int64_
         et.erase(std::unique(et.begin(), et.end()), et.end());
         if (e != et) throw mfs::Error(MFS_ERR_ARG, "the graph must be symmetric (both triangles of the pattern)");
     }
-    mfs::NDOptions opt;
+    const mfs::NDOptions opt = mfs::options_from_env().nd;     // the factorizer's dissection knobs
     const std::vector<int32_t> pr = mfs::nested_dissection((int32_t)n, xadj, adj, opt, &h->state);
     for (int64_t k = 0; k < n; ++k) perm[k] = pr[k];
     if (ncomp) *ncomp = (int64_t)h->state.forest.parent.size();

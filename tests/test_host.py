@@ -159,6 +159,38 @@ def test_node_refined_separators_give_a_consistent_analysis(monkeypatch):
     test_symbolic_counts_match_oracle_factor(7)
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_node_refined_separators_are_valid_and_not_heavier(seed, monkeypatch):
+    """With node-based FM refinement the dissection is still a separator tree of the graph (every
+    edge joins components on one root path, nd_oracle.is_separator_tree), and for the same seed
+    its separators hold no more vertices than without the refinement."""
+    from tests import nd_oracle
+    tets, X = wl.cube_mesh(10)
+    H = mfs.SystemAssembler(len(X)).blockSparsityPattern(tets)
+    n = H.n
+    up = [(int(H.Ai[k]), j) for j in range(n) for k in range(H.Ap[j], H.Ap[j + 1]) if H.Ai[k] != j]
+    cols = [[] for _ in range(n)]
+    for i, j in up + [(j, i) for i, j in up]:
+        cols[j].append(i)
+    p = np.zeros(n + 1, np.int64)
+    p[1:] = np.cumsum([len(c) for c in cols])
+    ii = np.array([i for c in cols for i in sorted(c)], np.int64)
+    monkeypatch.setenv("MFS_ND_TRIALS", "1")
+    monkeypatch.setenv("MFS_ND_SEED", str(seed))
+    weight = {}
+    for passes in (0, 4):
+        monkeypatch.setenv("MFS_ND_NODEFM", str(passes))
+        nd = mfs.NestedDissection()
+        perm = nd.order(p, ii)
+        assert np.array_equal(np.sort(perm), np.arange(n))
+        o = nd.ndOrdering()
+        assert nd_oracle.is_separator_tree(o.CParent, o.CMember, nd_oracle.edges_of(p, ii))
+        is_sep = np.zeros(len(o.CParent), bool)
+        is_sep[o.CParent[o.CParent >= 0]] = True
+        weight[passes] = int(is_sep[o.CMember].sum())
+    assert weight[4] <= weight[0], weight
+
+
 def dense_upper_blocks(H, Ax):
     """Dense symmetric matrix of a block-CSC upper pattern with values Ax (numpy)."""
     A = np.zeros((3 * H.n, 3 * H.n))
