@@ -1,19 +1,28 @@
 """Per-Newton-step linear solve on one MI355X: assembly -> numeric factorization -> solve (+FP64
 refinement to <= 1e-10 relative residual) on the 64^3 Kuhn-tet cube (811,200 free DOFs).
 
-One step = one Newton-like re-perturbation (new seed, sigma = 1e-3 h): element Hessians and the
-RHS (= -gradient) for every seed are generated beforehand and resident in HBM; the timed region
-covers Ax := 0, assembly, value injection + numeric factorization, solve + refinement.  Symbolic
-analysis (ordering, supernodes, schedule) runs once before the timed region and is reported
-separately.  Prints one JSON line.
+One step = one Newton-like re-perturbation (new seed, sigma = 1e-3 h): element Hessians (the 78
+local-upper doubles per element the assembly reads, or full 12x12 with --he-layout full12) and the
+RHS (= -gradient) for every seed are generated beforehand by the device element kernel and are
+resident in HBM; the timed region covers Ax := 0, assembly, value injection + numeric
+factorization, solve + refinement.  Symbolic analysis (ordering, supernodes, schedule) runs once
+before the timed region and is reported separately.  Prints one JSON line.
 
-With --precision fp32 the factorization runs in FP32 on MFMA after symmetric diagonal scaling and
-FP64 iterative refinement brings the residual to <= 1e-10; --stress draws per-element E
-log-uniform in [1e3, 1e7] and nu uniform in [0.2, 0.45] (the conditioning stress case), and the
-forward error against the FP64 factor's solution is reported per step.
+The default FP64 run also
+  * repeats the steps with an event pair around every launch (per-class device time; the tile
+    GEMM's average launch gives `roofline`, assembly and solves give `roofline_assembly` and
+    `roofline_solve` against the HBM peak);
+  * times the precision configuration (per-element E log-uniform in [1e3, 1e7], nu uniform in
+    [0.2, 0.45]) in FP64 and in the --stress-legs forms (FP32 and BF16x3 factor + FP64
+    refinement), with refinement steps, forward error against the FP64 solution and speed-up;
+  * measures HBM traffic with two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over a child
+    run, each scaled by the same counter on a 2 GiB device copy in that pass;
+  * times the CPU baseline (the oracle's supernodal Cholesky on the host) for one 64^3 step and,
+    as the median of five, the 16^3 configuration.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--precision fp64|fp32] [--stress]
-                  [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--precision fp64|fp32|bf16|bf16x3]
+                  [--stress] [--stress-legs fp32,bf16x3] [--no-cpu] [--no-traffic]
+                  [--workload cube|batch|learned|replay]
 """
 import argparse
 import json
@@ -38,13 +47,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_sample, flops_target, threads):
+def cpu_baseline(n_sample, flops_target, threads, reps=1):
     """The oracle's multifrontal supernodal Cholesky (oracle/supernodal.c: relaxed supernodes,
     per-supernode POTRF/TRSM/SYRK from the OpenBLAS that ships with scipy, elimination tree level
-    by level over `threads` OpenMP threads, the BLAS threaded at the top levels) timed on this
-    host for one Newton step of the n_sample^3 cube with the GPU path's ordering: serial
-    reference-semantics assembly, value injection into P A P^T, numeric factorization, forward
-    and backward solves.  Scaled to the full workload by the ratio of factorization flops."""
+    by level over `threads` OpenMP threads with a single-threaded BLAS, the BLAS threaded only at
+    the top levels that have fewer supernodes than threads) timed on this host for one Newton
+    step of the n_sample^3 cube with the GPU path's ordering: serial reference-semantics assembly,
+    value injection into P A P^T, numeric factorization, forward and backward solves.  The median
+    of `reps` timed steps is reported; it is scaled to the full workload by the ratio of
+    factorization flops (1 when n_sample is the full size)."""
     from tests import oracle_lib as orc
     from meshfemsparse_amd import workload as wl
     from meshfemsparse_amd import _lib
@@ -73,22 +84,34 @@ def cpu_baseline(n_sample, flops_target, threads):
     Cp, Ci, Cx = orc.permuted_lower(Ap, Ai, nb, orc.assemble(Ap, Ai, nb, tets, He), red, pos)
     S = orc.Supernodal(Cp, Ci, relax=32)
     S.factor(Cx, threads)
-    t0 = time.perf_counter()
-    Ax = orc.assemble(Ap, Ai, nb, tets, He)
-    t1 = time.perf_counter()
-    Cp, Ci, Cx = orc.permuted_lower(Ap, Ai, nb, Ax, red, pos)
-    S.factor(Cx, threads)
-    t2 = time.perf_counter()
-    x = S.solve(bp)
-    t3 = time.perf_counter()
-    A = None
+    runs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        Ax = orc.assemble(Ap, Ai, nb, tets, He)
+        t1 = time.perf_counter()
+        Cp, Ci, Cx = orc.permuted_lower(Ap, Ai, nb, Ax, red, pos)
+        S.factor(Cx, threads)
+        t2 = time.perf_counter()
+        x = S.solve(bp)
+        t3 = time.perf_counter()
+        runs.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
     if nred * 3 <= 3000:      # residual check on small samples
         A = orc.dense_from_lower(Cp, Ci, Cx)
         assert np.linalg.norm(A @ x - bp) <= 1e-10 * np.linalg.norm(bp)
+    runs.sort()
+    tot, ta, tf, ts = runs[len(runs) // 2]
     scale = flops_target / fl[0]
-    return dict(t_assembly=t1 - t0, t_factor=t2 - t1, t_solve=t3 - t2, sample_flops=fl[0], cpu_flops=S.info["flops"],
-                factor_gflops=S.info["flops"] / (t2 - t1) / 1e9, scale=scale, s_sample=t3 - t0,
-                s_per_step=(t3 - t0) * scale, supernodes=S.info["supernodes"])
+    return dict(t_assembly=ta, t_factor=tf, t_solve=ts, sample_flops=fl[0], cpu_flops=S.info["flops"],
+                factor_gflops=S.info["flops"] / tf / 1e9, scale=scale, s_sample=tot, s_per_step=tot * scale,
+                supernodes=S.info["supernodes"], spread=(runs[-1][0] - runs[0][0]) / tot if reps > 1 else 0.0)
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when it is set (the share of the host that belongs to this GPU)."""
+    aff = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(aff, env) if env > 0 else aff), aff
 
 
 def cpu_model():
@@ -368,6 +391,390 @@ def bench_replay(args):
         tmp.cleanup()
 
 
+def cube_inputs(dev, N, nseeds, stress, layout):
+    """Pattern, assembler, pins and per-seed element Hessians (layout) and RHS = -gradient of the
+    N^3 cube, generated by the device element kernel and resident in HBM."""
+    import torch
+    import meshfemsparse_amd as mfs
+    from meshfemsparse_amd import workload as wl
+    tets, Xr = wl.cube_mesh(N)
+    nb = len(Xr)
+    asm = mfs.SystemAssembler(nb)
+    H = asm.blockSparsityPattern(tets)
+    H.setZero()
+    Ee = nue = None
+    if stress:
+        rng = np.random.default_rng(12345)
+        Ee = 10 ** rng.uniform(3, 7, len(tets))
+        nue = rng.uniform(0.2, 0.45, len(tets))
+    gasm = mfs.DeviceAssembler(H, tets)
+    T = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+    Xr_d, Ee_d, nue_d = T(Xr), T(Ee), T(nue)
+    He_d, b_d = [], []
+    for sd in range(nseeds):
+        He, g12 = gasm.neohookean(Xr_d, T(wl.perturb(N, Xr, seed=sd)), E_per_elem=Ee_d, nu_per_elem=nue_d,
+                                  layout=layout)
+        g = torch.zeros(3 * nb, dtype=torch.float64, device=dev)
+        gasm.assembleGradient(g12, g)
+        He_d.append(He)
+        b_d.append(-g)
+        del g12
+    torch.cuda.synchronize()
+    return dict(tets=tets, H=H, pins=wl.z0_pins(N), asm=gasm, He=He_d, b=b_d, nb=nb)
+
+
+def make_factorizer(precision):
+    import meshfemsparse_amd as mfs
+    return mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, precision != "fp64",
+                                        precision if precision.startswith("bf16") else None)
+
+
+MAX_REFINE = {"fp64": 10, "fp32": 30, "bf16x3": 30, "bf16": 80}
+
+
+def run_steps(inp, f, layout, seeds, max_iters):
+    """One Newton step per seed (Ax := 0, assembly, numeric factorization, solve + refinement to
+    1e-10), timed by the wall clock around all of them; per-stage device times from events on
+    the current stream (the library launches on it)."""
+    import torch
+    H, asm = inp["H"], inp["asm"]
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sd in seeds:
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        e[0].record()
+        asm.zero(H.Ax)
+        e[1].record()
+        asm.assemble(inp["He"][sd], H.Ax, layout)
+        e[2].record()
+        f.factorizeNumeric(H)
+        e[3].record()
+        x, it, rr = f.solveRefined(H, inp["b"][sd], tol=1e-10, maxIters=max_iters)
+        e[4].record()
+        recs.append((e, it, rr, x))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = lambda a, b: float(np.mean([r[0][a].elapsed_time(r[0][b]) for r in recs]))
+    return dict(wall=wall, s_per_step=wall / len(seeds), t_zero_ms=ms(0, 1), t_assembly_ms=ms(1, 2),
+                t_factor_s=ms(2, 3) / 1e3, t_solve_s=ms(3, 4) / 1e3, iters=[r[1] for r in recs],
+                relres_max=max(r[2] for r in recs), x=[r[3] for r in recs])
+
+
+def stress_legs(args, dev, layout):
+    """The precision configuration: the 64^3 cube with per-element E log-uniform in [1e3, 1e7]
+    and nu uniform in [0.2, 0.45], timed in FP64 and in each reduced-precision form (factor +
+    FP64 refinement to 1e-10) on the same inputs; forward error of every refined solution against
+    the FP64 factor's solution of the same step."""
+    import torch
+    forms = [p for p in args.stress_legs.split(",") if p]
+    nseeds = args.stress_steps + 1
+    inp = cube_inputs(dev, args.n, nseeds, True, layout)
+    seeds = list(range(1, nseeds))
+    out = {}
+    x64 = None
+    for form in ["fp64"] + forms:
+        f = make_factorizer(form)
+        f.factorizeSymbolic(inp["H"], inp["pins"])
+        run_steps(inp, f, layout, [0], MAX_REFINE[form])                   # warmup
+        r = run_steps(inp, f, layout, seeds, MAX_REFINE[form])
+        leg = dict(s_per_step=r["s_per_step"], t_factor_s=r["t_factor_s"], t_solve_s=r["t_solve_s"],
+                   refine_iters=r["iters"], relres_max=r["relres_max"],
+                   factor_tflops=f.getFlopEstimate() / r["t_factor_s"] / 1e12)
+        if form == "fp64":
+            x64 = [x.clone() for x in r["x"]]
+        else:
+            leg["forward_error_vs_fp64_max"] = max(float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+                                                   for x, y in zip(r["x"], x64))
+            leg["speedup_vs_fp64"] = out["fp64"]["s_per_step"] / r["s_per_step"]
+        out[form] = leg
+        log(f"[bench] stress {form}: {r['s_per_step']:.4f} s/step, refinement {r['iters']}")
+        del f, r
+    del inp
+    torch.cuda.synchronize()
+    return out
+
+
+def _counter_sums(csv_dir):
+    """Per kernel name: (dispatches, counter sum in KiB, summed duration ns) from a rocprofv3
+    counter-collection CSV (one counter per pass)."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no counter-collection CSV")
+    sums = {}
+    for fn in files:
+        for r in csv.DictReader(open(fn)):
+            name = r["Kernel_Name"]
+            v = float(r["Counter_Value"])
+            d = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+            c, tot, dur, mx = sums.get(name, (0, 0.0, 0.0, 0.0))
+            sums[name] = (c + 1, tot + v, dur + d, max(mx, v))
+    return sums
+
+
+def traffic_passes(args, precision):
+    """HBM traffic of the kernels from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
+    a child run of this script (--traffic-probe: a 2 GiB device copy, then one Newton step), each
+    counter scaled by what it reads for the copy (2 GiB read, 2 GiB written) in the same pass.
+    Returns per kernel family bytes per dispatch, or None with the reason."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"mfs_{counter.lower()}_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = [prof, "--pmc", counter, "--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--n", str(args.n), "--precision",
+               precision, "--he-layout", args.he_layout]
+        log(f"[bench] traffic pass {counter} ...")
+        p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, start_new_session=True,
+                             cwd=os.environ.get("TMPDIR", "/tmp"))
+        try:
+            _, err = p.communicate(timeout=args.traffic_timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter} pass exceeded {args.traffic_timeout}s"
+        try:
+            res[counter] = _counter_sums(d)
+        except Exception as e:   # the profiler may end with a signal after writing complete results
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter} pass (rc {p.returncode}): {e}: {err.decode(errors='replace')[-300:]}"
+        shutil.rmtree(d, ignore_errors=True)
+    calib = {}
+    for counter in res:
+        cands = [(v[3], name) for name, v in res[counter].items() if "copyBuffer" in name]
+        if not cands:
+            return None, "calibration copy not found"
+        calib[counter] = (2 << 30) / (max(cands)[0] * 1024.0)
+    fams = {"k_gemm": lambda n: n.startswith("void mfs::k_gemm<"),
+            "k_gemm_bf": lambda n: n.startswith("void mfs::k_gemm_bf<"),
+            "k_assemble": lambda n: "mfs::k_assemble" in n,
+            "solve": lambda n: any(k in n for k in ("mfs::k_sf", "mfs::k_sb", "mfs::k_fgather", "mfs::k_bgather")),
+            "k_fill": lambda n: "mfs::k_fill<" in n, "k_extend": lambda n: "mfs::k_extend<" in n}
+    out = {"scale_fetch": calib["FETCH_SIZE"], "scale_write": calib["WRITE_SIZE"]}
+    for fam, pred in fams.items():
+        rb = wb = dur = 0.0
+        cnt = 0
+        for name, (c, tot, du, _) in res["FETCH_SIZE"].items():
+            if pred(name):
+                rb += tot * 1024.0 * calib["FETCH_SIZE"]
+                dur += du
+                cnt += c
+        for name, (c, tot, _, _) in res["WRITE_SIZE"].items():
+            if pred(name):
+                wb += tot * 1024.0 * calib["WRITE_SIZE"]
+        if cnt:
+            out[fam] = dict(dispatches=cnt, read_bytes=rb, write_bytes=wb, bytes_per_dispatch=(rb + wb) / cnt,
+                            implied_GBps=(rb + wb) / dur if dur > 0 else None)
+    return out, None
+
+
+def traffic_probe(args):
+    """Child of traffic_passes: a 2 GiB device copy (the counters' calibration), then one Newton
+    step of the cube (assembly, factorization, solve) with symbolic analysis beforehand."""
+    import torch
+    dev = torch.device("cuda:0")
+    a = torch.ones(1 << 28, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    layout = 1 if args.he_layout == "upper78" else 0
+    inp = cube_inputs(dev, args.n, 1, False, layout)
+    f = make_factorizer(args.precision)
+    f.factorizeSymbolic(inp["H"], inp["pins"])
+    run_steps(inp, f, layout, [0], MAX_REFINE[args.precision])
+    torch.cuda.synchronize()
+
+
+def bench_cube(args):
+    import torch
+    from meshfemsparse_amd import _lib
+    dev = torch.device("cuda:0")
+    layout = _lib.MFS_HE_UPPER78 if args.he_layout == "upper78" else _lib.MFS_HE_FULL12
+    N = args.n
+    nseeds = args.steps + args.warmup
+    t0 = time.perf_counter()
+    inp = cube_inputs(dev, N, nseeds, args.stress, layout)
+    H, pins, nb = inp["H"], inp["pins"], inp["nb"]
+    log(f"[bench] mesh N={N}: {len(inp['tets'])} tets, {nb} nodes, {H.nz} blocks; inputs for {nseeds} seeds "
+        f"in {time.perf_counter() - t0:.1f}s")
+    single = args.precision != "fp64"
+    f = make_factorizer(args.precision)
+    t0 = time.perf_counter()
+    f.factorizeSymbolic(H, pins)
+    t_sym = time.perf_counter() - t0
+    st = f.supernodeStats()
+    flops = f.getFlopEstimate()
+    nred = f.n_reduced()
+    nnz = f.getFactorNNZ()
+    log(f"[bench] symbolic {t_sym:.2f}s: n_red={nred} nnz(L)={nnz:.3e} flops={flops:.3e} {st}")
+    f.setGraphCapture(args.graph)
+    maxit = MAX_REFINE[args.precision]
+    run_steps(inp, f, layout, list(range(args.warmup)), maxit)
+    log("[bench] warmup done")
+    r = run_steps(inp, f, layout, list(range(args.warmup, nseeds)), maxit)
+    log(f"[bench] {args.steps} timed steps: {r['s_per_step']:.4f} s/step")
+    wall = r["wall"]
+    timed = {k: r[k] for k in ("t_zero_ms", "t_assembly_ms", "t_factor_s", "t_solve_s", "iters", "relres_max")}
+    ntets = len(inp["tets"])
+    # the same steps again with a HIP event pair around every kernel launch (per-class device
+    # time and the dominant kernel's average launch duration); kept out of the timed region
+    # because the events add a few microseconds between dependent launches
+    kt = not args.no_kernel_timing
+    ms = np.zeros(len(KCLASSES))
+    cnt = np.zeros(len(KCLASSES), np.int64)
+    work = np.zeros(len(KCLASSES))
+    gbytes = np.zeros(len(KCLASSES))
+    asm_ms = None
+    nsolves = 0
+    if kt:
+        f.setGraphCapture(False)            # per-launch events need the launches themselves
+        _lib.call("mfs_kernel_profiling", f._h, 1, 1)
+        rk = run_steps(inp, f, layout, list(range(args.warmup, nseeds)), maxit)
+        torch.cuda.synchronize()
+        _lib.call("mfs_kernel_profiling", f._h, 0, 0)
+        _lib.call("mfs_kernel_profile", f._h, _lib.ptr(ms, _lib.c_double), _lib.ptr(cnt, _lib.c_i64))
+        _lib.call("mfs_kernel_profile_work", f._h, _lib.ptr(work, _lib.c_double))
+        _lib.call("mfs_kernel_profile_bytes", f._h, _lib.ptr(gbytes, _lib.c_double))
+        nsolves = sum(1 + i for i in rk["iters"])
+        asm_ms = rk["t_assembly_ms"]
+        log("[bench] kernel-timing pass done")
+    s_per_step = wall / args.steps
+    factor_tflops = flops / timed["t_factor_s"] / 1e12
+    fs = 4 if single else 8
+    # roofline of the dominant kernel: the tile GEMM (TRSM + in-panel updates + Schur updates);
+    # for the BF16 variants the bfloat16 Schur-update kernel (in-panel + update-matrix classes)
+    bf = args.precision.startswith("bf16")
+    gcls = [6, 7] if bf else [5, 6, 7]
+    gemm_ms = float(sum(ms[c] for c in gcls))
+    gemm_launches = int(sum(cnt[c] for c in gcls))
+    # algorithmic GEMM flops of a step (supernodal count minus the diagonal-tile POTRF share),
+    # apportioned to the classes by their executed tile flops
+    potrf_flops = f.flopBreakdown()["potrf"]
+    exec_all = float(work[5] + work[6] + work[7] + work[KCLASSES.index("panel")])
+    share = float(sum(work[c] for c in gcls)) / exec_all if exec_all > 0 else 1.0
+    gemm_work = (flops - potrf_flops) * share * args.steps
+    if kt and gemm_launches:
+        avg_launch_s = gemm_ms / 1e3 / gemm_launches
+        flops_per_launch = gemm_work / gemm_launches
+        achieved = flops_per_launch / avg_launch_s / 1e12
+        noreuse_per_launch = float(sum(gbytes[c] for c in gcls)) / gemm_launches
+    else:
+        avg_launch_s, achieved, noreuse_per_launch = None, factor_tflops, None
+    peak = PEAK_TFLOPS["bf16" if bf else args.precision]
+    kname = (f"k_gemm_bf ({'hi+lo split ' if args.precision == 'bf16x3' else ''}bfloat16 MFMA Schur updates, "
+             f"FP32 accumulation; algorithmic flops)" if bf else
+             f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)")
+    roofline = {"bound": "mfma", "kernel": kname,
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None,
+                "avg_launch_ms": None if avg_launch_s is None else avg_launch_s * 1e3,
+                "launches_per_step": gemm_launches / max(args.steps, 1),
+                "noreuse_bytes_per_launch": noreuse_per_launch}
+    # HBM rooflines of assembly and solves (algorithmic bytes / measured time / 8 TB/s)
+    nz = H.nz
+    per_elem = 78 if layout == _lib.MFS_HE_UPPER78 else 144
+    asm_bytes = ntets * (per_elem * 8 + 10 * 4) + 4 * (nz + 1) + nz + 2 * 9 * nz * 8
+    roof_asm = None
+    if kt and asm_ms:
+        a_gbs = asm_bytes / (asm_ms / 1e3) / 1e9
+        roof_asm = {"bound": "hbm", "kernel": "k_assemble (slot-map gather, fixed element order per 3x3 slot)",
+                    "achieved": a_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": a_gbs / PEAK_HBM_GBS,
+                    "algorithmic_bytes": asm_bytes, "avg_launch_ms": asm_ms,
+                    "bytes_basis": f"He {per_elem} doubles + 10 slot-map entries per element, segment pointers "
+                                   f"and diagonal flags per slot, Ax read + write"}
+    solve_cls = [KCLASSES.index(c) for c in ("fgather", "fdiag", "fupdate", "bpartial", "bdiag", "bgather")]
+    roof_solve = None
+    if kt and nsolves:
+        t_solve_k = float(sum(ms[c] for c in solve_cls)) / nsolves / 1e3
+        sbytes = 2.0 * nnz * fs
+        s_gbs = sbytes / t_solve_k / 1e9
+        roof_solve = {"bound": "hbm", "kernel": "supernodal forward + backward sweeps (one right-hand side)",
+                      "achieved": s_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": s_gbs / PEAK_HBM_GBS,
+                      "algorithmic_bytes": sbytes, "kernel_ms_per_solve": t_solve_k * 1e3,
+                      "bytes_basis": "2 nnz(L) x element size (the factor read once per sweep)"}
+    kernel_ms = {k: round(float(m) / args.steps, 3) for k, m in zip(KCLASSES, ms) if cnt[KCLASSES.index(k)]}
+    del r
+    # the precision configuration (FP64 vs FP32 / BF16x3 + refinement on the stressed cube)
+    legs = None
+    if args.stress_legs and not args.stress and args.precision == "fp64":
+        f.clearFactors()
+        legs = stress_legs(args, dev, layout)
+    del inp
+    del f
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    traffic, why = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        traffic, why = traffic_passes(args, args.precision)
+    if traffic:
+        fam = "k_gemm_bf" if bf else "k_gemm"
+        if fam in traffic:
+            roofline["traffic"] = traffic[fam]["bytes_per_dispatch"]
+            if noreuse_per_launch:
+                roofline["traffic_vs_noreuse"] = traffic[fam]["bytes_per_dispatch"] / noreuse_per_launch
+        if roof_asm and "k_assemble" in traffic:
+            roof_asm["traffic"] = traffic["k_assemble"]["bytes_per_dispatch"]
+        if roof_solve and "solve" in traffic:
+            roof_solve["traffic"] = traffic["solve"]["read_bytes"] + traffic["solve"]["write_bytes"]
+    cpu = None
+    if not args.no_cpu:
+        threads, aff = cpu_threads()
+        log(f"[bench] CPU baseline: one {args.cpu_n}^3 step, {threads} threads ...")
+        c = cpu_baseline(args.cpu_n, flops, threads)
+        c16 = cpu_baseline(16, 1.0, threads, reps=5)
+        cpu = {"value": c["s_per_step"], "unit": "s/step", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(), "affinity_cpus": aff,
+               "sample": f"oracle multifrontal supernodal Cholesky (relaxed supernodes, per-supernode POTRF/TRSM/"
+                         f"SYRK from scipy's OpenBLAS single-threaded inside the OpenMP tree levels and threaded at "
+                         f"the top levels, {threads} threads), same ordering as the GPU path, one Newton step of the "
+                         f"{args.cpu_n}^3 cube: assembly {c['t_assembly']:.2f}s (serial) + injection and factor "
+                         f"{c['t_factor']:.2f}s ({c['factor_gflops']:.0f} GFLOP/s) + solve {c['t_solve']:.2f}s = "
+                         f"{c['s_sample']:.2f}s" + (f", scaled x{c['scale']:.3f} by the factorization flop ratio"
+                                                    if abs(c["scale"] - 1.0) > 1e-9 else " (the full workload)"),
+               "s_per_step_16cube": c16["s_sample"], "s_16cube_spread_of_5": c16["spread"]}
+    out = {
+        "metric": "Newton linear-solve s/step @811k DOF to <=1e-10 rel. residual; factor TFLOP/s",
+        "value": s_per_step, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": s_per_step * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if not single else ("f32 factor/solve, " + ("bf16 Schur updates (f32 accumulation), "
+                                                                  if args.precision == "bf16" else
+                                                                  "bf16x3 Schur updates (f32 accumulation), "
+                                                                  if args.precision == "bf16x3" else "")
+                                           + "f64 residual+refinement"),
+        "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He and RHS from the device element kernel)",
+        "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean "
+                               + ("E log-uniform [1e3,1e7] nu U[0.2,0.45]" if args.stress else "E=1e5 nu=0.3")
+                               + ", sigma=1e-3 h, z=0 pinned; assembly + "
+                               + ("FP32 scaled supernodal Cholesky + FP64 refinement to 1e-10" if single
+                                  else "FP64 supernodal Cholesky + solve/refine to 1e-10"),
+                   "free_dofs": nred, "tets": ntets, "factor_nnz": nnz,
+                   "factor_flops": flops, "supernodes": st["num_supernodes"], "levels": st["num_levels"],
+                   "he_layout": args.he_layout},
+        "factor_tflops": factor_tflops, "t_assembly_s": (timed["t_zero_ms"] + timed["t_assembly_ms"]) / 1e3,
+        "t_factor_s": timed["t_factor_s"], "t_solve_s": timed["t_solve_s"],
+        "refine_iters": timed["iters"], "relres_max": timed["relres_max"],
+        "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms,
+        "roofline": roofline, "roofline_assembly": roof_asm, "roofline_solve": roof_solve,
+        "traffic_note": why if traffic is None else
+        f"rocprofv3 FETCH_SIZE x{traffic['scale_fetch']:.3f} and WRITE_SIZE x{traffic['scale_write']:.3f} "
+        f"(the same counters on a 2 GiB device copy in each pass); L2-to-memory bytes",
+        "traffic_kernels": traffic,
+        "precision_config": legs,
+        "cpu_baseline": cpu,
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,16 +782,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64, help="hexes per cube side (64 = 811,200 free DOFs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-n", type=int, default=48, help="cube side of the CPU baseline sample")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="first run a device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--cpu-n", type=int, default=64, help="cube side of the CPU baseline step (64: the full workload)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32", "bf16", "bf16x3"], default="fp64",
                     help="bf16 / bf16x3: FP32 factor with bfloat16 Schur updates (x3: hi+lo operand split)")
     ap.add_argument("--stress", action="store_true",
                     help="per-element E log-uniform [1e3, 1e7], nu uniform [0.2, 0.45]")
-    ap.add_argument("--forward-error", action="store_true",
-                    help="also factor in FP64 and report the forward error of every step (fp32)")
+    ap.add_argument("--stress-legs", default="fp32,bf16x3",
+                    help="fp64 headline runs: also time the stressed cube in FP64 and these forms (empty: none)")
+    ap.add_argument("--stress-steps", type=int, default=3)
+    ap.add_argument("--he-layout", choices=["upper78", "full12"], default="upper78",
+                    help="element Hessians as the 78 local-upper doubles the assembly reads, or full 12x12")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--traffic-timeout", type=int, default=240)
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--workload", choices=["cube", "batch", "learned", "replay"], default="cube",
                     help="cube: the 64^3 Newton step (headline); batch: 512 uneven meshes in one system; "
                          "learned: train the learned preconditioner, evaluate held-out solves; replay: "
@@ -403,7 +814,9 @@ def main():
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding of the factorization "
               "is a design, not an implementation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.workload == "batch":
         bench_batch(args)
         return
@@ -413,197 +826,7 @@ def main():
     if args.workload == "replay":
         bench_replay(args)
         return
-
-    import torch
-    import meshfemsparse_amd as mfs
-    from meshfemsparse_amd import _lib, workload as wl
-
-    dev = torch.device("cuda:0")
-    if args.calibrate_copy_mib:
-        src = torch.ones(args.calibrate_copy_mib * 131072, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB done")
-        del src, dst
-    N = args.n
-    nseeds = args.steps + args.warmup
-    t0 = time.perf_counter()
-    tets, Xr = wl.cube_mesh(N)
-    nb = len(Xr)
-    asm = mfs.SystemAssembler(nb)
-    H = asm.blockSparsityPattern(tets)
-    H.setZero()
-    pins = wl.z0_pins(N)
-    log(f"[bench] mesh N={N}: {len(tets)} tets, {nb} nodes, {H.nz} blocks; pattern {time.perf_counter() - t0:.2f}s")
-    # inputs for every step, resident in HBM
-    He_d, b_d = [], []
-    t0 = time.perf_counter()
-    Ee = nue = None
-    if args.stress:
-        rng = np.random.default_rng(12345)
-        Ee = 10 ** rng.uniform(3, 7, len(tets))
-        nue = rng.uniform(0.2, 0.45, len(tets))
-    # element Hessians and gradients by the device element kernel (the same formulas as the host
-    # generator), gradients assembled on the device; RHS = -gradient
-    gasm = mfs.DeviceAssembler(H, tets)
-    T = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
-    Xr_d, Ee_d, nue_d = T(Xr), T(Ee), T(nue)
-    for s in range(nseeds):
-        He, g12 = gasm.neohookean(Xr_d, T(wl.perturb(N, Xr, seed=s)), E_per_elem=Ee_d, nu_per_elem=nue_d)
-        g = torch.zeros(3 * nb, dtype=torch.float64, device=dev)
-        gasm.assembleGradient(g12, g)
-        He_d.append(He)
-        b_d.append(-g)
-        del g12
-    torch.cuda.synchronize()
-    log(f"[bench] generated + uploaded {nseeds} seeds in {time.perf_counter() - t0:.1f}s")
-    single = args.precision != "fp64"
-    f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single,
-                                     args.precision if args.precision.startswith("bf16") else None)
-    t0 = time.perf_counter()
-    f.factorizeSymbolic(H, pins)
-    t_sym = time.perf_counter() - t0
-    st = f.supernodeStats()
-    flops = f.getFlopEstimate()
-    nred = f.n_reduced()
-    log(f"[bench] symbolic {t_sym:.2f}s: n_red={nred} nnz(L)={f.getFactorNNZ():.3e} flops={flops:.3e} {st}")
-    dasm = asm._assembler(H, np.ascontiguousarray(tets, np.int32))
-    f.setGraphCapture(args.graph)
-
-    def step(s):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        e[0].record()
-        dasm.zero(H.Ax)
-        dasm.assemble(He_d[s], H.Ax)
-        e[1].record()
-        f.factorizeNumeric(H)
-        e[2].record()
-        x, iters, rr = f.solveRefined(H, b_d[s], tol=1e-10, maxIters=30 if single else 10)
-        e[3].record()
-        return e, iters, rr, x
-
-    for s in range(args.warmup):
-        step(s)
-        log(f"[bench] warmup step {s} done")
-    torch.cuda.synchronize()
-    recs = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        recs.append(step(args.warmup + k))
-        log(f"[bench] step {k} done")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # the same steps again with a HIP event pair around every kernel launch (per-class device
-    # time and the dominant kernel's average launch duration); kept out of the timed region
-    # because the events add a few microseconds between dependent launches
-    kt = not args.no_kernel_timing
-    ms = np.zeros(len(KCLASSES))
-    cnt = np.zeros(len(KCLASSES), np.int64)
-    work = np.zeros(len(KCLASSES))
-    if kt:
-        f.setGraphCapture(False)            # per-launch events need the launches themselves
-        _lib.call("mfs_kernel_profiling", f._h, 1, 1)
-        for k in range(args.steps):
-            step(args.warmup + k)
-        torch.cuda.synchronize()
-        _lib.call("mfs_kernel_profiling", f._h, 0, 0)
-        _lib.call("mfs_kernel_profile", f._h, _lib.ptr(ms, _lib.c_double), _lib.ptr(cnt, _lib.c_i64))
-        _lib.call("mfs_kernel_profile_work", f._h, _lib.ptr(work, _lib.c_double))
-        log("[bench] kernel-timing pass done")
-    t_asm = np.mean([r[0][0].elapsed_time(r[0][1]) for r in recs]) / 1e3
-    t_fac = np.mean([r[0][1].elapsed_time(r[0][2]) for r in recs]) / 1e3
-    t_sol = np.mean([r[0][2].elapsed_time(r[0][3]) for r in recs]) / 1e3
-    iters = [r[1] for r in recs]
-    rrs = [r[2] for r in recs]
-    fwd_err = None
-    if single and args.forward_error:
-        # forward error of each step's refined FP32-factor solution against the FP64 factor's solve
-        xs = [r[3].clone() for r in recs]
-        del recs
-        f64 = mfs.make_cholesky_factorizer()
-        f64.factorizeSymbolic(H, pins)
-        fwd_err = []
-        for k in range(args.steps):
-            dasm.zero(H.Ax)
-            dasm.assemble(He_d[args.warmup + k], H.Ax)
-            f64.factorizeNumeric(H)
-            x64, _, _ = f64.solveRefined(H, b_d[args.warmup + k], tol=1e-12, maxIters=5)
-            fwd_err.append(float(torch.linalg.norm(xs[k] - x64) / torch.linalg.norm(x64)))
-        del f64
-        log(f"[bench] forward errors vs FP64 factor: {fwd_err}")
-    s_per_step = wall / args.steps
-    factor_tflops = flops / t_fac / 1e12
-    # roofline of the dominant kernel: the tile GEMM (TRSM + in-panel updates + Schur updates);
-    # for the BF16 variants the bfloat16 Schur-update kernel (in-panel + update-matrix classes)
-    bf = args.precision.startswith("bf16")
-    gcls = [6, 7] if bf else [5, 6, 7]
-    gemm_ms = float(sum(ms[c] for c in gcls))
-    gemm_launches = int(sum(cnt[c] for c in gcls))
-    # algorithmic GEMM flops of a step (supernodal count minus the diagonal-tile POTRF share),
-    # apportioned to the classes by their executed tile flops
-    potrf_flops = f.flopBreakdown()["potrf"]
-    exec_all = float(work[5] + work[6] + work[7] + work[KCLASSES.index("panel")])
-    share = float(sum(work[c] for c in gcls)) / exec_all if exec_all > 0 else 1.0
-    gemm_work = (flops - potrf_flops) * share * args.steps
-    if kt and gemm_launches:
-        avg_launch_s = gemm_ms / 1e3 / gemm_launches
-        flops_per_launch = gemm_work / gemm_launches
-        achieved = flops_per_launch / avg_launch_s / 1e12
-    else:
-        avg_launch_s, achieved = None, factor_tflops
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes on this workload
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"k_gemm_traffic_64cube_{args.precision}.json")
-    if N == 64 and os.path.exists(tpath):
-        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
-    peak = PEAK_TFLOPS["bf16" if bf else args.precision]
-    kname = (f"k_gemm_bf ({'hi+lo split ' if args.precision == 'bf16x3' else ''}bfloat16 MFMA Schur updates, "
-             f"FP32 accumulation; algorithmic flops)" if bf else
-             f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)")
-    roofline = {"bound": "mfma", "kernel": kname,
-                "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": traffic,
-                "avg_launch_ms": None if avg_launch_s is None else avg_launch_s * 1e3,
-                "launches_per_step": gemm_launches / max(args.steps, 1)}
-    kernel_ms = {k: round(float(m) / args.steps, 3) for k, m in zip(KCLASSES, ms) if cnt[KCLASSES.index(k)]}
-    cpu = None
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-        log(f"[bench] CPU baseline on a {args.cpu_n}^3 sample, {threads} threads ...")
-        c = cpu_baseline(args.cpu_n, flops, threads)
-        c16 = cpu_baseline(16, 1.0, threads)
-        cpu = {"value": c["s_per_step"], "unit": "s/step", "cores": threads, "kind": "port",
-               "cpu_model": cpu_model(),
-               "sample": f"oracle multifrontal supernodal Cholesky (relaxed supernodes, per-supernode POTRF/TRSM/"
-                         f"SYRK from scipy's OpenBLAS, tree-parallel levels over {threads} OpenMP threads, threaded "
-                         f"BLAS at the top levels), same ordering as the GPU path, on the {args.cpu_n}^3 cube: "
-                         f"assembly {c['t_assembly']:.2f}s (serial) + injection and factor {c['t_factor']:.2f}s "
-                         f"({c['factor_gflops']:.0f} GFLOP/s) + solve {c['t_solve']:.2f}s = {c['s_sample']:.2f}s, "
-                         f"scaled x{c['scale']:.2f} by the factorization flop ratio to the full workload",
-               "s_per_step_16cube": c16["s_sample"]}
-    out = {
-        "metric": "Newton linear-solve s/step @811k DOF to <=1e-10 rel. residual; factor TFLOP/s",
-        "value": s_per_step, "unit": "s/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": s_per_step * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if not single else ("f32 factor/solve, " + ("bf16 Schur updates (f32 accumulation), "
-                                                                  if args.precision == "bf16" else
-                                                                  "bf16x3 Schur updates (f32 accumulation), "
-                                                                  if args.precision == "bf16x3" else "")
-                                           + "f64 residual+refinement"),
-        "data": "synthetic (seeded perturbations of the unit cube, neo-Hookean He and RHS from the device element kernel)",
-        "config": {"workload": f"{N}^3 hex cube x 6 Kuhn tets, neo-Hookean "
-                               + ("E log-uniform [1e3,1e7] nu U[0.2,0.45]" if args.stress else "E=1e5 nu=0.3")
-                               + ", sigma=1e-3 h, z=0 pinned; assembly + "
-                               + ("FP32 scaled supernodal Cholesky + FP64 refinement to 1e-10" if single
-                                  else "FP64 supernodal Cholesky + solve/refine to 1e-10"),
-                   "free_dofs": nred, "tets": len(tets), "factor_nnz": f.getFactorNNZ(), "factor_flops": flops,
-                   "supernodes": st["num_supernodes"], "levels": st["num_levels"]},
-        "factor_tflops": factor_tflops, "t_assembly_s": t_asm, "t_factor_s": t_fac, "t_solve_s": t_sol,
-        "refine_iters": iters, "relres_max": max(rrs), "forward_error_vs_fp64": fwd_err,
-        "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms,
-        "roofline": roofline, "cpu_baseline": cpu,
-    }
+    out = bench_cube(args)
     print(json.dumps(out), flush=True)
 
 

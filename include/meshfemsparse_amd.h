@@ -331,6 +331,9 @@ int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
 /* Executed tile flops of the profiled launches per class (22; the GEMM classes and the fused   */
 /* diagonal-tile + TRSM steps carry them).                                                     */
 int mfs_kernel_profile_work(mfs_factorizer f, double* work);
+/* Bytes the profiled GEMM launches of each class (22) would move with no cache reuse between their
+ * tiles: every tile reads its A and B panels and writes its C (reading it first unless beta = 0). */
+int mfs_kernel_profile_bytes(mfs_factorizer f, double* bytes);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Learned block incomplete-Cholesky preconditioner (no reference counterpart: the reference     */

@@ -98,6 +98,7 @@ _SIGS = {
     "mfs_kernel_profiling": [c_vp, c_i32, c_i32],
     "mfs_kernel_profile": [c_vp, P_dbl, P_i64],
     "mfs_kernel_profile_work": [c_vp, P_dbl],
+    "mfs_kernel_profile_bytes": [c_vp, P_dbl],
     "mfs_blockcsc_write": [ctypes.c_char_p, P_i64, P_i64, c_i64, P_dbl],
     "mfs_blockcsc_read": [ctypes.c_char_p, P_i64, P_i64, P_i64, P_i64, P_i64, P_dbl],
     "mfs_cube_mesh": [c_i32, P_i32, P_dbl],

@@ -663,6 +663,13 @@ int mfs_kernel_profile_work(mfs_factorizer f, double* work) {
     MFS_CATCH
 }
 
+int mfs_kernel_profile_bytes(mfs_factorizer f, double* bytes) {
+    MFS_TRY
+    CHECK_PTR(f); CHECK_PTR(bytes);
+    for (int c = 0; c < mfs::KernelProfile::NCLASS; ++c) bytes[c] = f->dev->prof.bytes[c];
+    MFS_CATCH
+}
+
 int mfs_cube_mesh(int32_t N, int32_t* tets, double* X) {
     MFS_TRY
     if (N <= 0 || N > 1000) throw mfs::Error(MFS_ERR_ARG, "N out of range");

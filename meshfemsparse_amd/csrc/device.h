@@ -54,7 +54,9 @@ struct KernelProfile {
     double ms[NCLASS] = {};
     int64_t count[NCLASS] = {};
     double work[NCLASS] = {};  // algorithmic flops of the profiled launches (GEMM classes)
-    void begin(int cls, void* stream, double work = 0.0, int64_t items = 0);
+    double bytes[NCLASS] = {}; // GEMM classes: operand bytes with no cache reuse across tiles
+                               // (every tile reads its A and B panels and reads/writes its C)
+    void begin(int cls, void* stream, double work = 0.0, int64_t items = 0, double bytes = 0.0);
     void end(void* stream);
     void collect();            // after the stream has been synchronized
     void reset();
@@ -63,6 +65,7 @@ private:
     std::vector<void*> pool_;
     std::vector<std::pair<int, size_t>> marks_;
     std::vector<std::pair<double, int64_t>> work_;   // per mark: flops (or bytes), tasks
+    std::vector<double> bytes_;                      // per mark: no-reuse operand bytes
 };
 
 class FactorizerDev {

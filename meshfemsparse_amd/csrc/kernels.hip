@@ -66,9 +66,10 @@ void DeviceArray::upload_raw(const void* src, size_t b) {
     if (b) HIP_CHECK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
 }
 
-void KernelProfile::begin(int cls, void* stream, double work, int64_t items) {
+void KernelProfile::begin(int cls, void* stream, double work, int64_t items, double nbytes) {
     if (!on) return;
     work_.push_back({work, items});
+    bytes_.push_back(nbytes);
     const size_t i = marks_.size() * 2;
     while (pool_.size() < i + 2) {
         hipEvent_t e;
@@ -94,16 +95,19 @@ void KernelProfile::collect() {
         ms[cls] += m;
         count[cls] += 1;
         work[cls] += work_[q].first;
+        bytes[cls] += bytes_[q];
         if (fp) fprintf(fp, "%d %lld %.6e %.6f\n", cls, (long long)work_[q].second, work_[q].first, m);
     }
     if (fp) fclose(fp);
     marks_.clear();
     work_.clear();
+    bytes_.clear();
 }
 void KernelProfile::reset() {
-    for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; work[c] = 0; }
+    for (int c = 0; c < NCLASS; ++c) { ms[c] = 0; count[c] = 0; work[c] = 0; bytes[c] = 0; }
     marks_.clear();
     work_.clear();
+    bytes_.clear();
 }
 KernelProfile::~KernelProfile() {
     for (void* e : pool_) (void)hipEventDestroy((hipEvent_t)e);
@@ -1193,6 +1197,16 @@ static double gemm_work(const Symbolic& S, const TaskRange& r) {
     }
     return w;
 }
+// bytes the tiles of r would move with no reuse between tiles: A (M x K) and B (N x K) panels
+// read once per tile, C written (and read first unless beta = 0); elements of `fs` bytes
+static double gemm_noreuse_bytes(const Symbolic& S, const TaskRange& r, size_t fs) {
+    double b = 0.0;
+    for (int64_t q = r.b; q < r.e; ++q) {
+        const GemmTask& g = S.gemm_tasks[q];
+        b += ((double)(g.M + g.N) * g.K + (double)g.M * g.N * ((g.flags & GT_BETA0) ? 1 : 2)) * (double)fs;
+    }
+    return b;
+}
 
 // tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 1) for the
 // 128x128 tiles and MFS_GEMM_V64 for the 64x64 tiles (default: as MFS_GEMM_V), MFS_GEMM_KC2 and
@@ -2231,7 +2245,8 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
                 prof.end(s);
             }
             if (tr.e > tr.b && i1 == i0) {
-                prof.begin(KernelProfile::TRSM, s, prof.on ? gemm_work(S, tr) : 0.0, tr.e - tr.b);
+                prof.begin(KernelProfile::TRSM, s, prof.on ? gemm_work(S, tr) : 0.0, tr.e - tr.b,
+                           prof.on ? gemm_noreuse_bytes(S, tr, sizeof(T)) : 0.0);
                 launch_gemm<T>(tr, gt, L, U, inv, s);
                 prof.end(s);
             }
@@ -2241,7 +2256,8 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
             if (rr.e > rr.b && prof.on) {
                 // per-kernel timing: the rest runs in order on the stream, so the measured launch
                 // durations are not stretched by kernels sharing the device
-                prof.begin(KernelProfile::UPDATE, s, gemm_work(S, rr), rr.e - rr.b);
+                prof.begin(KernelProfile::UPDATE, s, gemm_work(S, rr), rr.e - rr.b,
+                           gemm_noreuse_bytes(S, rr, sizeof(T)));
                 schur_gemm(rr, s);
                 prof.end(s);
             } else if (rr.e > rr.b) {
@@ -2254,21 +2270,24 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
                 hipEvent_t e = event();
                 HIP_CHECK(hipEventRecord(e, s));
                 HIP_CHECK(hipStreamWaitEvent(side, e, 0));
-                prof.begin(KernelProfile::UPDATE, side, prof.on ? gemm_work(S, rr) : 0.0, rr.e - rr.b);
+                prof.begin(KernelProfile::UPDATE, side, prof.on ? gemm_work(S, rr) : 0.0, rr.e - rr.b,
+                           prof.on ? gemm_noreuse_bytes(S, rr, sizeof(T)) : 0.0);
                 schur_gemm(rr, side);
                 prof.end(side);
                 pending = event();
                 HIP_CHECK(hipEventRecord(pending, side));
             }
             if (ur.e > ur.b) {
-                prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b);
+                prof.begin(KernelProfile::UPDATE, s, prof.on ? gemm_work(S, ur) : 0.0, ur.e - ur.b,
+                           prof.on ? gemm_noreuse_bytes(S, ur, sizeof(T)) : 0.0);
                 schur_gemm(ur, s);
                 prof.end(s);
             }
         }
         join();
         if (P.syrk.e > P.syrk.b) {
-            prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b);
+            prof.begin(KernelProfile::SYRK, s, prof.on ? gemm_work(S, P.syrk) : 0.0, P.syrk.e - P.syrk.b,
+                       prof.on ? gemm_noreuse_bytes(S, P.syrk, sizeof(T)) : 0.0);
             schur_gemm(P.syrk, s, true);
             prof.end(s);
         }
