@@ -43,7 +43,7 @@ public:
 private:
     int64_t nslots_ = 0, ne_ = 0, nnodes_ = 0;
     int npe_ = 0;
-    DeviceArray seg_, contrib_, diag_, tets_, nptr_, nent_;
+    DeviceArray seg_, contrib_, diag_, pslot_, tets_, nptr_, nent_;
 };
 
 // Optional per-kernel-class device timing (HIP event pairs around each launch).

@@ -133,66 +133,87 @@ static void fill(T* p, int64_t n, T v, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Assembly: one thread per destination 3x3 slot; contributions in ascending element order.
+// Assembly.  A wave is cut into 7 segments of 9 lanes; a segment owns one destination 3x3 slot
+// and lane q of it the slot's entry q (column-major, row q % 3, column q / 3).  The segment walks
+// the slot's contributions in ascending element order and every lane adds its entry of each
+// contribution, so each entry receives the serial element-order sum of the reference, bit for
+// bit, while the 9 lanes of a segment read one contribution's 9 values together (one 72-byte
+// run of the element's local-upper Hessian) and write the slot's 9 values together.  A
+// workgroup (4 waves, 28 consecutive slot positions) stages its contribution lists in LDS.
+// Positions are grouped so that the slots of a workgroup have similar contribution counts
+// (off-diagonal and diagonal slots apart), and consecutive workgroups run on one XCD, so the
+// element Hessians the neighbouring slots share are fetched into one L2.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ double he_entry(const double* __restrict__ He, int layout, int64_t e,
-                                           int pair, int r, int c) {
+template <int LAYOUT>
+__device__ __forceinline__ double he_entry(const double* __restrict__ He, int64_t e, int pair, int r, int c) {
     // local-upper block `pair` of element e, entry (r, c) of the 3x3 block
-    if (layout == 0) {
-        const int A[10] = {0, 0, 0, 1, 1, 2, 0, 1, 2, 3};
-        const int B[10] = {1, 2, 3, 2, 3, 3, 0, 1, 2, 3};
-        return He[e * 144 + (3 * B[pair] + c) * 12 + 3 * A[pair] + r];
+    if constexpr (LAYOUT == 0) {
+        const int a = pair < 6 ? (pair < 3 ? 0 : (pair < 5 ? 1 : 2)) : pair - 6;
+        const int b = pair < 6 ? (pair < 3 ? pair + 1 : (pair < 5 ? pair - 1 : 3)) : pair - 6;
+        return He[e * 144 + (3 * b + c) * 12 + 3 * a + r];
+    } else {
+        if (pair < 6) return He[e * 78 + 9 * pair + 3 * c + r];
+        return He[e * 78 + 54 + 6 * (pair - 6) + (c * (c + 1)) / 2 + r];   // r <= c
     }
-    if (pair < 6) return He[e * 78 + 9 * pair + 3 * c + r];
-    return He[e * 78 + 54 + 6 * (pair - 6) + (c * (c + 1)) / 2 + r];   // r <= c
 }
 
+constexpr int kAsmSeg = 9;                     // lanes per slot
+constexpr int kAsmSlotsPerWave = 7;            // 63 of 64 lanes
+constexpr int kAsmSlotsPerWG = 4 * kAsmSlotsPerWave;
+constexpr int kAsmLds = 2048;                  // staged contribution entries per workgroup
+
+// workgroup b of a launch of n workgroups runs on XCD b % 8 (round-robin dispatch); it takes
+// chunk (b % 8) * ceil(n / 8) + b / 8, so every XCD works through one contiguous range
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t n) {
+    const int64_t per = (n + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+template <int LAYOUT>
 __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ seg,
                                                   const uint32_t* __restrict__ contrib,
+                                                  const int32_t* __restrict__ slot_of_pos,
                                                   const uint8_t* __restrict__ is_diag,
-                                                  const double* __restrict__ He, int layout,
-                                                  double* __restrict__ Ax, int64_t nslots) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    double v[9];
-    double* dst = Ax + 9 * s;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) v[q] = dst[q];
-    const int32_t b = seg[s], e_ = seg[s + 1];
-    if (is_diag[s]) {
-        for (int32_t p = b; p < e_; ++p) {
-            const uint32_t w = contrib[p];
-            const int64_t el = w >> 5;
-            const int pair = (w >> 1) & 15;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = 0; r <= c; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, r, c);
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int r = 0; r <= c; ++r) dst[3 * c + r] = v[3 * c + r];
-    } else {
-        for (int32_t p = b; p < e_; ++p) {
-            const uint32_t w = contrib[p];
-            const int64_t el = w >> 5;
-            const int pair = (w >> 1) & 15;
-            if (w & 1u) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, c, r);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) v[3 * c + r] += he_entry(He, layout, el, pair, r, c);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 9; ++q) dst[q] = v[q];
+                                                  const double* __restrict__ He,
+                                                  double* __restrict__ Ax, int64_t npos) {
+    __shared__ uint32_t lc[kAsmLds];
+    const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int64_t p0 = chunk * kAsmSlotsPerWG;
+    if (p0 >= npos) return;                    // the whole workgroup leaves together
+    const int64_t pend = min(p0 + (int64_t)kAsmSlotsPerWG, npos);
+    const int32_t cb = seg[p0], ce = seg[pend];
+    const bool staged = ce - cb <= kAsmLds;
+    if (staged)
+        for (int32_t k = threadIdx.x; k < ce - cb; k += 256) lc[k] = contrib[cb + k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int sg = lane / kAsmSeg, q = lane % kAsmSeg;
+    const int r = q % 3, c = q / 3;
+    const int64_t p = p0 + (threadIdx.x >> 6) * kAsmSlotsPerWave + sg;
+    if (sg >= kAsmSlotsPerWave || p >= pend) return;
+    const bool diag = is_diag[p] != 0;
+    if (diag && r > c) return;                 // strict lower part of a diagonal block: untouched
+    const int64_t s = slot_of_pos[p];
+    double v = Ax[9 * s + q];
+    const int32_t b = seg[p], e_ = seg[p + 1];
+    // a transposed contribution adds entry (c, r) of the element's block
+    auto value = [&](int32_t k) {
+        const uint32_t w = staged ? lc[k - cb] : contrib[k];
+        const int64_t el = w >> 5;
+        const int pair = (w >> 1) & 15;
+        return (w & 1u) ? he_entry<LAYOUT>(He, el, pair, c, r) : he_entry<LAYOUT>(He, el, pair, r, c);
+    };
+    int32_t k = b;
+    // four contributions' loads in flight at a time, added in ascending element order
+    for (; k + 4 <= e_; k += 4) {
+        const double x0 = value(k), x1 = value(k + 1), x2 = value(k + 2), x3 = value(k + 3);
+        v += x0;
+        v += x1;
+        v += x2;
+        v += x3;
     }
+    for (; k < e_; ++k) v += value(k);
+    Ax[9 * s + q] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -283,6 +304,7 @@ AssemblerDev::AssemblerDev(const Pattern& P, const int32_t* en, int64_t ne, int 
     seg_.upload(M.seg);
     contrib_.upload(M.contrib);
     diag_.upload(M.is_diag);
+    pslot_.upload(M.slot_of_pos);
     // connectivity and node -> (element, local node) lists in ascending element order
     std::vector<int32_t> conn(en, en + ne * npe);
     tets_.upload(conn);
@@ -318,8 +340,14 @@ void AssemblerDev::assemble_gradient(const double* g12, double* g, void* stream)
 void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stream) {
     if (layout != 0 && layout != 1) throw Error(-1, "unknown element Hessian layout");
     if (nslots_ == 0) return;
-    k_assemble<<<nblk(nslots_, 256), 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(),
-                                                          diag_.as<uint8_t>(), He, layout, Ax, nslots_);
+    const int64_t nwg = (nslots_ + kAsmSlotsPerWG - 1) / kAsmSlotsPerWG;
+    const unsigned grid = (unsigned)(8 * ((nwg + 7) / 8));       // whole rounds of the 8 XCDs
+    if (layout == 0)
+        k_assemble<0><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
+                                                    diag_.as<uint8_t>(), He, Ax, nslots_);
+    else
+        k_assemble<1><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
+                                                    diag_.as<uint8_t>(), He, Ax, nslots_);
     LAUNCH_CHECK();
 }
 

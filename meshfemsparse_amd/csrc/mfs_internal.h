@@ -31,12 +31,17 @@ Pattern build_block_pattern(const int32_t* elem_nodes, int64_t ne, int npe, int6
 // transpose) contributions sorted by element.  Entry = (e << 5) | (pair << 1) | transpose.
 // Pairs 0..5 are the off-diagonal local-upper blocks (0,1),(0,2),(0,3),(1,2),(1,3),(2,3);
 // pairs 6..9 are the diagonal blocks 0..3 (upper triangle only).
+// The slots are listed in processing order: groups of 256 consecutive columns, and inside each
+// group the off-diagonal slots before the diagonal ones (ascending slot within each part), so
+// that the slots a workgroup takes together have similar contribution counts.  seg, contrib and
+// the per-position arrays follow that order.
 // ------------------------------------------------------------------------------------------
 struct SlotMap {
     int64_t nslots = 0;
-    std::vector<int32_t> seg;        // nslots + 1
-    std::vector<uint32_t> contrib;   // seg[nslots]
-    std::vector<uint8_t> is_diag;    // per slot
+    std::vector<int32_t> seg;          // nslots + 1 (by position)
+    std::vector<uint32_t> contrib;     // seg[nslots]
+    std::vector<int32_t> slot_of_pos;  // destination slot of each position
+    std::vector<uint8_t> is_diag;      // per position
 };
 SlotMap build_slot_map(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);
 

@@ -76,6 +76,27 @@ SlotMap build_slot_map(const Pattern& P, const int32_t* en, int64_t ne, int npe)
     SlotMap S;
     S.nslots = P.nz();
     if (S.nslots >= (int64_t(1) << 31)) throw Error(-6, "pattern too large for 32-bit slot map");
+    // processing order of the slots
+    std::vector<uint8_t> diag_of_slot(S.nslots, 0);
+    for (int64_t j = 0; j < P.n; ++j)
+        for (int64_t k = P.Ap[j]; k < P.Ap[j + 1]; ++k)
+            if (P.Ai[k] == j) diag_of_slot[k] = 1;
+    S.slot_of_pos.resize(S.nslots);
+    std::vector<int32_t> pos_of_slot(S.nslots);
+    {
+        int64_t q = 0;
+        for (int64_t g = 0; g < P.n; g += 256) {
+            const int64_t g1 = std::min<int64_t>(P.n, g + 256);
+            for (int d = 0; d < 2; ++d)
+                for (int64_t j = g; j < g1; ++j) {
+                    for (int64_t k = P.Ap[j]; k < P.Ap[j + 1]; ++k)
+                        if (diag_of_slot[k] == d) {
+                            pos_of_slot[k] = (int32_t)q;
+                            S.slot_of_pos[q++] = (int32_t)k;
+                        }
+                }
+        }
+    }
     std::vector<int64_t> slot_of((size_t)ne * 10);
     std::vector<int32_t> cnt(S.nslots + 1, 0);
     for (int64_t e = 0; e < ne; ++e) {
@@ -87,12 +108,12 @@ SlotMap build_slot_map(const Pattern& P, const int32_t* en, int64_t ne, int npe)
             const int64_t na = v[kPairA[p]], nbb = v[kPairB[p]];
             const int64_t s = find_slot(P, std::min(na, nbb), std::max(na, nbb));
             slot_of[10 * e + p] = s;
-            cnt[s + 1]++;
+            cnt[pos_of_slot[s] + 1]++;
         }
         for (int a = 0; a < 4; ++a) {
             const int64_t s = find_slot(P, v[a], v[a]);
             slot_of[10 * e + 6 + a] = s;
-            cnt[s + 1]++;
+            cnt[pos_of_slot[s] + 1]++;
         }
     }
     for (int64_t s = 0; s < S.nslots; ++s) cnt[s + 1] += cnt[s];
@@ -105,13 +126,11 @@ SlotMap build_slot_map(const Pattern& P, const int32_t* en, int64_t ne, int npe)
             const int64_t s = slot_of[10 * e + p];
             uint32_t tr = 0;
             if (p < 6) tr = v[kPairA[p]] > v[kPairB[p]] ? 1u : 0u;
-            S.contrib[back[s]++] = ((uint32_t)e << 5) | ((uint32_t)p << 1) | tr;
+            S.contrib[back[pos_of_slot[s]]++] = ((uint32_t)e << 5) | ((uint32_t)p << 1) | tr;
         }
     }
-    S.is_diag.assign(S.nslots, 0);
-    for (int64_t j = 0; j < P.n; ++j)
-        for (int64_t k = P.Ap[j]; k < P.Ap[j + 1]; ++k)
-            if (P.Ai[k] == j) S.is_diag[k] = 1;
+    S.is_diag.resize(S.nslots);
+    for (int64_t q = 0; q < S.nslots; ++q) S.is_diag[q] = diag_of_slot[S.slot_of_pos[q]];
     return S;
 }
 

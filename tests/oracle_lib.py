@@ -20,6 +20,9 @@ def lib():
     if _lib is None:
         if not os.path.exists(_PATH):
             raise ImportError(f"{_PATH} missing; run __graft_entry__.build()")
+        # OpenMP workers of the tree levels sleep instead of spinning while the BLAS threads run
+        # the top levels (read by libgomp when the oracle library loads it)
+        os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
         L = ctypes.CDLL(_PATH)
         P = ctypes.POINTER
         L.orc_cube_mesh.argtypes = [ctypes.c_int, P(i32), P(f64)]

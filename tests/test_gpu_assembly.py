@@ -23,7 +23,7 @@ def pack78(He):
     return np.concatenate(out, 1)
 
 
-@pytest.mark.parametrize("N,scramble", [(1, False), (3, True), (6, False)])
+@pytest.mark.parametrize("N,scramble", [(1, False), (3, True), (6, False), (13, True)])
 def test_assembly_bit_exact(dev, N, scramble):
     tets, Xr = wl.cube_mesh(N)
     if scramble:
@@ -48,3 +48,28 @@ def test_assembly_bit_exact(dev, N, scramble):
     H.setZero()
     asm.assembleHessian(H, tets, torch.from_numpy(He.ravel()).to(dev))
     assert np.array_equal(H.Ax.cpu().numpy(), ref)
+
+
+def test_assembly_untouched_lower_and_crowded_slots(dev):
+    """Strict-lower entries of diagonal blocks keep whatever the caller left there (the reference
+    reads and writes only the upper triangle), and a slot with thousands of contributions (a node
+    shared by 2100 elements: more contribution entries than a workgroup stages in LDS) still sums
+    them in ascending element order."""
+    rng = np.random.default_rng(9)
+    nb = 700
+    others = np.array([rng.choice(np.arange(1, nb), 3, replace=False) for _ in range(2100)], np.int32)
+    tets = np.concatenate([np.zeros((2100, 1), np.int32), others], 1)
+    tets = np.array([rng.permutation(t) for t in tets], np.int32)
+    He = rng.standard_normal((len(tets), 144))
+    asm = mfs.SystemAssembler(nb)
+    H = asm.blockSparsityPattern(tets)
+    garbage = rng.standard_normal(9 * H.nz)
+    H.Ax = torch.from_numpy(garbage.copy()).to(dev)
+    asm.assembleHessian(H, tets, torch.from_numpy(He.ravel()).to(dev))
+    ref = orc.assemble(H.Ap, H.Ai, nb, tets, He, Ax=garbage)
+    got = H.Ax.cpu().numpy()
+    assert np.array_equal(got, ref)
+    diag = np.array([H.Ap[j + 1] - 1 for j in range(nb) if H.Ap[j + 1] > H.Ap[j] and H.Ai[H.Ap[j + 1] - 1] == j])
+    assert len(diag) > 600
+    for q in (1, 2, 5):                          # strict-lower entries (1,0), (2,0), (2,1)
+        assert np.array_equal(got[9 * diag + q], garbage[9 * diag + q])
