@@ -170,12 +170,12 @@ int mfs_solve(mfs_factorizer f, const double* b_dev, double* x_dev, int32_t nrhs
  * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The same kernels as mfs_solve, with every scratch vector (factor-order
  * right-hand sides, forward results, update vectors, backward partial sums) carved out of a
- * library-owned static device array of MFS_SOLVE_NOALLOC_MAX_BYTES per GPU (64 MiB: one
- * right-hand side of the 64^3 cube, 811,200 unknowns, needs 58.7 MB in FP64).  A solve whose
+ * library-owned static device array of MFS_SOLVE_NOALLOC_MAX_BYTES per GPU (96 MiB: one
+ * right-hand side of the 64^3 cube, 811,200 unknowns, needs 68.4 MB in FP64).  A solve whose
  * scratch exceeds it returns MFS_ERR_ARG with the size it needs.  The array keeps the last call's
  * contents and is shared by every call on that GPU: calls that overlap on one GPU race.  Kernel
  * profiling (mfs_kernel_profiling) must be off.                                               */
-#define MFS_SOLVE_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+#define MFS_SOLVE_NOALLOC_MAX_BYTES ((size_t)96 << 20)
 int mfs_solve_noalloc(mfs_factorizer f, const double* b_dev, double* x_dev, int32_t nrhs, void* stream);
 
 /* solveRawReduced / solveRawReducedInPlace (CatamariFactorizer.cc:788-860): reduced-length

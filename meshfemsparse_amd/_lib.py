@@ -15,7 +15,7 @@ MFS_OK, MFS_ERR_ARG, MFS_ERR_STATE, MFS_ERR_NOT_SPD, MFS_ERR_DEVICE, MFS_ERR_ALL
 MFS_ERR_NOT_CONVERGED = -7
 MFS_PREC_FP64, MFS_PREC_FP32, MFS_PREC_BF16, MFS_PREC_BF16X3 = 0, 1, 2, 3
 MFS_HE_FULL12, MFS_HE_UPPER78 = 0, 1
-MFS_SOLVE_NOALLOC_MAX_BYTES = 64 << 20
+MFS_SOLVE_NOALLOC_MAX_BYTES = 96 << 20
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 _SIGS = {

@@ -270,7 +270,7 @@ struct Symbolic {
     std::vector<GemmTask> gemm_tasks;
     std::vector<FusedTask> fused_tasks;
     // solves
-    int32_t solve_block = 512;                 // column-block width (scalars, multiple of 64)
+    int32_t solve_block = 1024;                // column-block width (scalars, multiple of 64)
     std::vector<int32_t> tile_slot;            // per (supernode, 64-column tile): inverse slot
     std::vector<SolveBlock> sblocks;           // all column blocks (per supernode, in order)
     std::vector<SolveTile> inv_tiles;          // block-inverse work: (block, column tile)
@@ -313,7 +313,7 @@ struct SymbolicOptions {
     int schur_tile = 128;   // large tile of the Schur update U = -L21 L21^T: 128, or 256 for the
                             // bfloat16 Schur kernel (regions at least 256 in both dims)
     int tile_group = 8;     // GEMM tiles are emitted in groups of this many tile columns
-    int solve_block = 512;  // solve column-block width (multiple of 64)
+    int solve_block = 1024; // solve column-block width (multiple of 64, at most 1024)
     int solve_chunk = 1024; // backward partial-sum rows per task (multiple of 64, at most 1024)
     int fuse_degree = 0;    // supernodes of width <= one tile and at most this many structure rows
                             // (scalars) go to the fused level kernel; 0 disables it

@@ -58,7 +58,7 @@ def test_noalloc_solve_rejects_oversized_scratch(dev):
     H, pins, g, _, _ = mesh_problem(dev, 16, seed=1)
     f = mfs.make_cholesky_factorizer()
     f.factorize(H, pins)
-    nrhs = 100                                   # about 0.8 MB of scratch per right-hand side
+    nrhs = 150                                   # about 1 MB of scratch per right-hand side
     b = torch.zeros(nrhs * 3 * H.n, dtype=torch.float64, device=dev)
     with pytest.raises(_lib.InvalidArgumentError, match="scratch"):
         c_solve("mfs_solve_noalloc", f, b, nrhs)
