@@ -169,7 +169,7 @@ def bench_batch(args):
     f.factorizeSymbolic(H, B["pins"])
     t_sym = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tune = f.autotune(H, B["pins"], fuse=(0, 192, 384, 768), tiles=(128, 256), reps=1)
+    tune = f.autotune(H, B["pins"], fuse=(0, 192, 384, 768), tiles=(128, 64), reps=3)
     t_tune = time.perf_counter() - t0
     f.setGraphCapture(not args.no_graph)
     log(f"[bench] symbolic {t_sym:.2f}s, autotune {t_tune:.1f}s: {tune}")
@@ -226,8 +226,10 @@ def bench_batch(args):
                                f"[1e3,1e6], nu 0.3, z=0 pinned, one block-diagonal system",
                    "free_dofs": 3 * nb - len(B["pins"]), "tets": len(tets), "factor_nnz": f.getFactorNNZ(),
                    "factor_flops": flops, "schedule": f.scheduleStats(), "autotune": {
-                       "fuse_degree": tune["fuse_degree"], "big_tile": tune["big_tile"],
-                       "ms": {f"{a},{b}": round(v, 3) for (a, b), v in tune["ms"].items()}},
+                       "fuse_degree": tune["fuse_degree"], "bucket_tiles": tune["bucket_tiles"],
+                       "ms": {"fuse": {str(a): round(v, 3) for a, v in tune["ms"]["fuse"].items()},
+                              "bucket_tiles": [{str(t): round(v, 3) for t, v in d.items()}
+                                               for d in tune["ms"]["bucket_tiles"]]}},
                    "graph": not args.no_graph},
         "factor_tflops": flops / t_fac / 1e12, "t_assembly_s": t_asm, "t_factor_s": t_fac, "t_solve_s": t_sol,
         "refine_iters": [r[1] for r in recs], "relres_max": max(r[2] for r in recs), "symbolic_s": t_sym,

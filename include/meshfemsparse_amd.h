@@ -218,12 +218,17 @@ int mfs_solve_refined_segments(mfs_factorizer f, const double* Ax_dev, const dou
  * No reference counterpart.                                                                  */
 int mfs_factorizer_set_graph(mfs_factorizer f, int32_t on);
 
-/* Autotuner (no reference counterpart): analyse once for the ordering, then for every candidate
- * pair (fuse[i], tiles[j]) rebuild the schedule -- supernodes one diagonal tile wide with at most
- * fuse[i] structure rows (scalars) go to the fused level kernel (0: none), GEMM regions at least
- * tiles[j] in both dimensions use 128x128 tiles -- and time `reps` numeric factorizations of
- * Ax_dev (after one warm-up).  ms[i * ntile + j] = average milliseconds; best = {fuse, tile}.
- * The fastest schedule stays installed (symbolic state: factorize again before solving).     */
+/* Autotuner (no reference counterpart): analyse once for the ordering, then choose the schedule
+ * by timing `reps` numeric factorizations of Ax_dev (the fastest of them, after one warm-up) per
+ * candidate, by coordinate descent: first the fused level kernel's bound fuse[0..nfuse) (supernodes
+ * one diagonal tile wide with at most that many structure rows, scalars, are factored by the
+ * persistent level kernel; 0: none), then for each supernode size bucket (front W + D < 512,
+ * < 2048, larger) the largest GEMM tile among tiles[0..ntile) (64: 64x64 tiles only; 128: 128x128
+ * tiles where the region is large enough).  The first candidates are the starting point, and a
+ * candidate replaces the incumbent only when it is at least 2 % faster.  ms[nfuse + 3 ntile] =
+ * milliseconds of every candidate (fuse bounds, then bucket 0, 1, 2 tiles); best[4] = {fuse bound,
+ * tile of bucket 0, 1, 2}.  The chosen schedule stays installed (symbolic state: factorize again
+ * before solving).                                                                            */
 int mfs_autotune(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t num_blocks, int32_t block_size,
                  const int64_t* pinned_scalar_vars, int64_t num_pins, const double* Ax_dev, const int32_t* fuse,
                  int32_t nfuse, const int32_t* tiles, int32_t ntile, int32_t reps, double* ms, int32_t* best,

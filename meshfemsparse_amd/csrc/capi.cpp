@@ -286,9 +286,10 @@ int mfs_autotune(mfs_factorizer f, const int64_t* Ap, const int64_t* Ai, int64_t
     CHECK_PTR(f); CHECK_PTR(Ap); CHECK_PTR(Ai); CHECK_PTR(Ax); CHECK_PTR(fuse); CHECK_PTR(tiles);
     if (npins > 0) CHECK_PTR(pins);
     std::vector<int> fz(fuse, fuse + std::max(nfuse, 0)), tz(tiles, tiles + std::max(ntile, 0));
-    int b2[2] = {0, 0};
-    f->dev->autotune(Ap, Ai, nb, bs, pins, npins, Ax, fz.data(), nfuse, tz.data(), ntile, reps, ms, b2, stream);
-    if (best) { best[0] = b2[0]; best[1] = b2[1]; }
+    int b4[4] = {0, 0, 0, 0};
+    f->dev->autotune(Ap, Ai, nb, bs, pins, npins, Ax, fz.data(), nfuse, tz.data(), ntile, reps, ms, b4, stream);
+    if (best)
+        for (int i = 0; i < 4; ++i) best[i] = b4[i];
     MFS_CATCH
 }
 

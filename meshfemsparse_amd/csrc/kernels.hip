@@ -2815,38 +2815,61 @@ void FactorizerDev::autotune(const int64_t* Ap, const int64_t* Ai, int64_t nb, i
         forest = std::move(S0.nd_forest);
     }
     base.given_forest = &forest;
+    base.given_perm = &perm;
     hipStream_t s = S_(stream);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    double best_ms = 1e300;
-    int bi = 0, bj = 0;
-    for (int i = 0; i < nfuse; ++i)
-        for (int j = 0; j < ntile; ++j) {
-            SymbolicOptions o = base;
-            o.given_perm = &perm;
-            o.fuse_degree = std::max(0, fuse[i]);
-            o.big_tile = std::max(128, tiles[j]);
-            install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
-            numeric(Ax, 0.0, nullptr, stream);                   // warm-up (and pivot check)
+    // fastest of `reps` numeric factorizations after one warm-up, under the schedule of `o`
+    auto time_cfg = [&](const SymbolicOptions& o) {
+        install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
+        numeric(Ax, 0.0, nullptr, stream);                       // warm-up (and pivot check)
+        double t = 1e300;
+        for (int r = 0; r < reps; ++r) {
             HIP_CHECK(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r) numeric(Ax, 0.0, nullptr, stream);
+            numeric(Ax, 0.0, nullptr, stream);
             HIP_CHECK(hipEventRecord(e1, s));
             HIP_CHECK(hipEventSynchronize(e1));
             float m = 0;
             HIP_CHECK(hipEventElapsedTime(&m, e0, e1));
-            const double t = (double)m / reps;
-            if (ms) ms[i * ntile + j] = t;
-            if (t < best_ms) { best_ms = t; bi = i; bj = j; }
+            t = std::min(t, (double)m);
+        }
+        return t;
+    };
+    // coordinate descent from the first candidates: the fused-kernel bound first, then the
+    // largest GEMM tile of each size bucket; a candidate replaces the incumbent only when it is
+    // at least 2 % faster, so timing noise between near-equal schedules does not flip the choice
+    constexpr double kGain = 0.98;
+    SymbolicOptions cur = base;
+    cur.fuse_degree = std::max(0, fuse[0]);
+    for (int b = 0; b < 3; ++b) cur.bucket_tile[b] = tiles[0] > 64 ? 128 : 64;
+    double t_cur = time_cfg(cur);
+    if (ms) ms[0] = t_cur;
+    for (int i = 1; i < nfuse; ++i) {
+        SymbolicOptions o = cur;
+        o.fuse_degree = std::max(0, fuse[i]);
+        const double t = time_cfg(o);
+        if (ms) ms[i] = t;
+        if (t < kGain * t_cur) { cur = o; t_cur = t; }
+    }
+    for (int b = 0; b < 3; ++b)
+        for (int j = 0; j < ntile; ++j) {
+            SymbolicOptions o = cur;
+            o.bucket_tile[b] = tiles[j] > 64 ? 128 : 64;
+            double t = t_cur;
+            if (o.bucket_tile[b] != cur.bucket_tile[b]) {
+                t = time_cfg(o);
+                if (t < kGain * t_cur) { cur = o; t_cur = t; }
+            }
+            if (ms) ms[nfuse + b * ntile + j] = t;
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    SymbolicOptions o = base;
-    o.given_perm = &perm;
-    o.fuse_degree = std::max(0, fuse[bi]);
-    o.big_tile = std::max(128, tiles[bj]);
-    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, o)));
-    if (best) { best[0] = o.fuse_degree; best[1] = o.big_tile; }
+    install(std::make_unique<Symbolic>(analyze(Ap, Ai, nb, pins, npins, cur)));
+    if (best) {
+        best[0] = cur.fuse_degree;
+        for (int b = 0; b < 3; ++b) best[1 + b] = cur.bucket_tile[b];
+    }
 }
 
 // L of P A P^T (for an FP32 factor: the factor of S P A P^T S, unscaled to L = S^{-1} L~)

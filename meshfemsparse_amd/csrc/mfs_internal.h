@@ -310,6 +310,11 @@ struct SymbolicOptions {
     int tile = 64;          // diagonal tile / inner step width (scalars)
     int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
+    // Supernode size buckets by front size W + D (scalars): [0, 512), [512, 2048), [2048, inf).
+    // bucket_tile[b] is the largest GEMM tile of the supernodes of bucket b: 128 (128x128 tiles
+    // where the region is large enough; 256 for the bfloat16 Schur kernel) or 64 (64x64 tiles
+    // only).  The autotuner picks them per bucket.
+    int bucket_tile[3] = {128, 128, 128};
     int schur_tile = 128;   // large tile of the Schur update U = -L21 L21^T: 128, or 256 for the
                             // bfloat16 Schur kernel (regions at least 256 in both dims)
     int tile_group = 8;     // GEMM tiles are emitted in groups of this many tile columns
@@ -337,7 +342,8 @@ struct SymbolicOptions {
 // thresholds), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
-// of the outer trailing updates); values are rounded down to multiples of 64 where required.
+// of the outer trailing updates), MFS_BUCKET_TILES ("t0,t1,t2": largest GEMM tile, 64 or 128, per
+// supernode size bucket); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 
 // A scalar symmetric matrix stored as its upper triangle in CSC (the reference's SuiteSparseMatrix
@@ -350,6 +356,9 @@ struct ScalarBlocking {
     std::vector<int64_t> value_pos;
 };
 ScalarBlocking block_scalar_pattern(const int64_t* Ap, const int64_t* Ai, int64_t n);
+
+// size bucket of a supernode front of W + D scalars (SymbolicOptions::bucket_tile)
+inline int front_bucket(int64_t front) { return front < 512 ? 0 : (front < 2048 ? 1 : 2); }
 
 Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nblocks,
                  const int64_t* pins, int64_t npins, const SymbolicOptions& opt);

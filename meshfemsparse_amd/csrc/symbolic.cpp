@@ -212,6 +212,14 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
     if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_LOOKAHEAD")) opt.lookahead = atoi(e) != 0;
+    if (const char* e = getenv("MFS_BUCKET_TILES")) {
+        int t0, t1, t2;
+        if (sscanf(e, "%d,%d,%d", &t0, &t1, &t2) == 3) {
+            opt.bucket_tile[0] = t0 > 64 ? 128 : 64;
+            opt.bucket_tile[1] = t1 > 64 ? 128 : 64;
+            opt.bucket_tile[2] = t2 > 64 ? 128 : 64;
+        }
+    }
     return opt;
 }
 
@@ -582,11 +590,14 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     // solve column blocks: per supernode, blocks of SB columns with explicit diagonal inverses
     const int SB = std::min(1024 / T, std::max(opt.solve_block / T, 1)) * T;   // kernels hold <= 1024
     S.solve_block = SB;
+    // blocks wider than 512 columns only for wide supernodes (they halve the sequential solve steps
+    // at the top of the tree; narrower supernodes keep the cheaper 512-column block inverses)
+    auto sbw = [&](int32_t s) { return (SB > 512 && 3 * S.swidth[s] < 4096) ? 512 : SB; };
     std::vector<int32_t> tile_base(ns + 1, 0), sblock0(ns + 1, 0);
     for (int32_t s = 0; s < ns; ++s) {
         const int32_t W = 3 * S.swidth[s];
         tile_base[s + 1] = tile_base[s] + (W + T - 1) / T;
-        sblock0[s + 1] = sblock0[s] + (W + SB - 1) / SB;
+        sblock0[s + 1] = sblock0[s] + (W + sbw(s) - 1) / sbw(s);
     }
     S.tile_slot.assign(tile_base[ns], -1);
     S.sblocks.resize(sblock0[ns]);
@@ -595,7 +606,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         for (int32_t s = 0; s < ns; ++s) {
             const int32_t W = 3 * S.swidth[s], ld = W + 3 * S.sdeg[s];
             for (int32_t k = sblock0[s]; k < sblock0[s + 1]; ++k) {
-                const int32_t c0 = (k - sblock0[s]) * SB, nbk = std::min(SB, W - c0);
+                const int32_t c0 = (k - sblock0[s]) * sbw(s), nbk = std::min(sbw(s), W - c0);
                 SolveBlock& b = S.sblocks[k];
                 b.l_off = S.panel_off[s] + (int64_t)c0 * ld + c0;
                 b.xb_off = xb;
@@ -658,6 +669,10 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         all.insert(all.end(), small.begin(), small.end());
         r.e = (int64_t)all.size();
         return r;
+    };
+    // the supernode's size bucket allows tiles larger than 64x64
+    auto big_ok = [&](int32_t s) {
+        return opt.bucket_tile[front_bucket(3 * ((int64_t)S.swidth[s] + S.sdeg[s]))] > 64;
     };
     auto fused_sn = [&](int32_t s) {
         return opt.fuse_degree > 0 && 3 * S.swidth[s] <= T && 3 * S.sdeg[s] <= opt.fuse_degree;
@@ -838,15 +853,15 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     if (k1 < Jend) {
                         const int32_t kb1 = std::min(T, W - k1);
                         emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
-                                   k1, k1 + kb1, W + D, k1 - J0, 0, true);
+                                   k1, k1 + kb1, W + D, k1 - J0, 0, big_ok(s));
                     } else if (Jend < W) {
                         trailing = true;
                         const int32_t split = opt.lookahead ? std::min(W, Jend + OB) : W;
                         emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
-                                   Jend, split, W + D, Jend - J0, 0, true);
+                                   Jend, split, W + D, Jend - J0, 0, big_ok(s));
                         if (split < W)
                             emit_lower(rbig, rsmall, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off,
-                                       ld, split, W, W + D, Jend - J0, 0, true);
+                                       ld, split, W, W + D, Jend - J0, 0, big_ok(s));
                     }
                 }
                 P.update_steps.push_back(push_tasks(S.gemm_tasks, big, small));
@@ -863,7 +878,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 const int64_t off = S.panel_off[s];
                 // region rows/cols are indices into L21 (0..D); A = P[W + r, 0..W), C = U[r, c]
                 emit_lower(big, small, off + W, ld, off + W, ld, S.u_off[s], D, 0, D, D, W, GT_C_IN_U | GT_BETA0,
-                           true, /*c_is_u=*/true);
+                           big_ok(s), /*c_is_u=*/true);
             }
             P.syrk = push_tasks(S.gemm_tasks, big, small);
         }
@@ -902,7 +917,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 if (k >= sblock0[s + 1] - sblock0[s]) continue;
                 const int32_t bi = sblock0[s] + k;
                 const SolveBlock& b = S.sblocks[bi];
-                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * SB;
+                const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * sbw(s);
                 for (int32_t t = 0; t < (b.nb + T - 1) / T; ++t) S.fdiag_tiles.push_back({bi, t});
                 for (auto [r0, r1] : chunks(c0 + b.nb, W, W + D, T)) {
                     SolveRowTask rt{};
@@ -932,7 +947,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     if (st >= nbs) continue;
                     const int32_t k = nbs - 1 - st, bi = sblock0[s] + k;
                     SolveBlock& b = S.sblocks[bi];
-                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * SB;
+                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], c0 = k * sbw(s);
                     b.part_begin = slot;
                     for (auto [r0, r1] : chunks(c0 + b.nb, W, W + D, RC)) {
                         for (int32_t g = 0; g < b.nb; g += CG) {

@@ -362,14 +362,18 @@ class GpuSupernodalFactorizer:
                 "extend_launches", "levels", "graph_nodes", "panel_launches"]
         return {k: int(x) for k, x in zip(keys, v)}
 
-    def autotune(self, H, pinnedVars=(), fuse=(0, 192, 384, 768), tiles=(128, 256), reps=2):
-        """Time the numeric factorization of H under each (fused-kernel bound, big-tile threshold)
-        schedule on one ordering and keep the fastest; leaves a symbolic factorization."""
+    def autotune(self, H, pinnedVars=(), fuse=(192, 0, 384), tiles=(128, 64), reps=3):
+        """Choose the schedule of H's numeric factorization by timing on one ordering: first the
+        fused level kernel's bound among `fuse` (persistent kernel for supernodes one tile wide
+        with at most that many structure rows; 0: none), then per supernode size bucket (front
+        W + D < 512, < 2048, larger) the largest GEMM tile among `tiles` (128 or 64).  The first
+        candidates are the start and a candidate must be 2 % faster to replace the incumbent.
+        Leaves a symbolic factorization with the chosen schedule."""
         pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
         fz = np.ascontiguousarray(fuse, np.int32)
         tz = np.ascontiguousarray(tiles, np.int32)
-        ms = np.zeros(len(fz) * len(tz))
-        best = np.zeros(2, np.int32)
+        ms = np.zeros(len(fz) + 3 * len(tz))
+        best = np.zeros(4, np.int32)
         _lib.call("mfs_autotune", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64), H.n, 3,
                   ptr(pins, _lib.c_i64), len(pins), _dptr(H.Ax), ptr(fz, _lib.c_i32), len(fz), ptr(tz, _lib.c_i32),
                   len(tz), int(reps), ptr(ms, _lib.c_double), ptr(best, _lib.c_i32), _stream())
@@ -377,8 +381,10 @@ class GpuSupernodalFactorizer:
         self._n_full = 3 * H.n
         self._n_red = 3 * H.n - len(self._fixed)
         self._Hpattern = (False, H.n, H.nz)
-        table = {(int(a), int(b)): float(ms[i * len(tz) + j]) for i, a in enumerate(fz) for j, b in enumerate(tz)}
-        return {"fuse_degree": int(best[0]), "big_tile": int(best[1]), "ms": table}
+        nf = len(fz)
+        table = {"fuse": {int(a): float(ms[i]) for i, a in enumerate(fz)},
+                 "bucket_tiles": [{int(t): float(ms[nf + b * len(tz) + j]) for j, t in enumerate(tz)} for b in range(3)]}
+        return {"fuse_degree": int(best[0]), "bucket_tiles": [int(x) for x in best[1:]], "ms": table}
 
     def exportFactorCSC(self):
         n = self.n_reduced()

@@ -111,9 +111,11 @@ def test_graph_replays_equal_eager_factor(dev, single, monkeypatch):
 def test_autotune_keeps_a_candidate_and_the_factor(dev):
     B, H, He, g, Ee = batch_problem(dev, nmeshes=5, nmin=3, nmax=6)
     f = mfs.make_cholesky_factorizer()
-    res = f.autotune(H, B["pins"], fuse=(0, 96, 100000), tiles=(128, 256), reps=1)
-    assert (res["fuse_degree"], res["big_tile"]) in res["ms"]
-    assert all(v > 0 for v in res["ms"].values())
+    res = f.autotune(H, B["pins"], fuse=(0, 96, 100000), tiles=(128, 64), reps=1)
+    assert res["fuse_degree"] in res["ms"]["fuse"]
+    assert all(t in (64, 128) for t in res["bucket_tiles"])
+    assert all(v > 0 for v in res["ms"]["fuse"].values())
+    assert all(v > 0 for d in res["ms"]["bucket_tiles"] for v in d.values())
     f.factorizeNumeric(H)
     ref = mfs.make_cholesky_factorizer()
     ref.factorizeSymbolic(H, B["pins"])
@@ -152,3 +154,25 @@ def test_batch_factor_matches_oracle_cholesky(dev, single, fuse, monkeypatch):
     mesh_of = np.searchsorted(B["seg"], free[ip], side="right") - 1
     rows, cols = np.nonzero(Lg)
     assert np.array_equal(mesh_of[rows], mesh_of[cols])
+
+
+@pytest.mark.parametrize("tiles", ["64,64,64", "64,128,64", "128,64,128"])
+def test_bucket_tiles_give_the_same_factor(dev, tiles, monkeypatch):
+    """Per-bucket tile choices change only the GEMM tiling of each supernode: the batched FP64
+    factor of every choice equals the oracle's Cholesky factor (and the default schedule's) to
+    1e-12."""
+    from tests.test_gpu_factor import oracle_factor
+    B, H, He, g, Ee = batch_problem(dev, nmeshes=5, nmin=6, nmax=9, seed=11)
+    f0 = mfs.make_cholesky_factorizer()
+    f0.factorizeSymbolic(H, B["pins"])
+    f0.factorizeNumeric(H)
+    L0 = orc.dense_L(*f0.exportFactorCSC())
+    monkeypatch.setenv("MFS_BUCKET_TILES", tiles)
+    f1 = mfs.make_cholesky_factorizer()
+    f1.factorizeSymbolic(H, B["pins"])
+    f1.factorizeNumeric(H)
+    L1 = orc.dense_L(*f1.exportFactorCSC())
+    (Lp, Li, Lx), _, _ = oracle_factor(H, f1)
+    Lo = orc.dense_L(Lp, Li, Lx)
+    assert np.abs(L1 - Lo).max() <= 1e-12 * np.abs(Lo).max()
+    assert np.abs(L1 - L0).max() <= 1e-12 * np.abs(Lo).max()
