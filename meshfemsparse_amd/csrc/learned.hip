@@ -54,98 +54,136 @@ __global__ void k_round_bf16(const float* __restrict__ s, float* __restrict__ d,
     if (i < n) d[i] = (float)(__bf16)s[i];
 }
 
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4_t mfma_bf16(bf8_t a, bf8_t b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 // Y(r, o) (+)= act(sum_k X(r, k) W'(k, o) + b(o)), o < O <= 64, K <= 128; W'(k, o) = W[k ldw + o]
-// or, transposed, W[o ldw + k].  64 rows per workgroup, operands in LDS; thread = (row, 16 cols).
+// or, transposed, W[o ldw + k].  On v_mfma_f32_16x16x32_bf16: the weights are bfloat16 values
+// already (exact in bfloat16), every FP32 activation is split into hi = bf16(x) and
+// lo = bf16(x - hi) and both parts are multiplied (about 16 significant bits), FP32
+// accumulation over k in a fixed order.  64 rows x 64 outputs per workgroup, operands in LDS
+// (rows k-contiguous); wave w owns rows 16 w .. 16 w + 15, fragment (m = lane % 16,
+// k = 8 (lane / 16) .. + 7), accumulator (row 4 (lane / 16) + i, column lane % 16).
 __global__ __launch_bounds__(256) void k_dense(const float* __restrict__ X, int ldx, int64_t N, int K,
                                                const float* __restrict__ W, int ldw, int wt,
                                                const float* __restrict__ b, float* __restrict__ Y, int ldy, int O,
                                                int relu, int accumulate) {
-    __shared__ __attribute__((aligned(16))) float Ws[128][68];
-    __shared__ float Xs[64][129];
+    constexpr int KP = 128 + 8;               // row stride (bfloat16): 16-byte fragment reads of 16 rows hit distinct banks
+    __shared__ __attribute__((aligned(16))) __bf16 Xh[64 * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Xl[64 * KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ws[64 * KP];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < K * 64; idx += 256) {
-        const int k = idx >> 6, o = idx & 63;
-        Ws[k][o] = o < O ? (wt ? W[(int64_t)o * ldw + k] : W[(int64_t)k * ldw + o]) : 0.0f;
+    const int Kp = (K + 31) & ~31;
+    for (int idx = tid; idx < 64 * Kp; idx += 256) {
+        const int o = idx / Kp, k = idx - o * Kp;
+        float w = 0.0f;
+        if (o < O && k < K) w = wt ? W[(int64_t)o * ldw + k] : W[(int64_t)k * ldw + o];
+        Ws[o * KP + k] = (__bf16)w;
     }
     const int64_t r0 = (int64_t)blockIdx.x * 64;
-    for (int idx = tid; idx < 64 * K; idx += 256) {
-        const int rr = idx / K, k = idx - rr * K;
+    for (int idx = tid; idx < 64 * Kp; idx += 256) {
+        const int rr = idx / Kp, k = idx - rr * Kp;
         const int64_t r = r0 + rr;
-        Xs[rr][k] = r < N ? X[r * ldx + k] : 0.0f;
+        const float x = (r < N && k < K) ? X[r * ldx + k] : 0.0f;
+        const __bf16 h = (__bf16)x;
+        Xh[rr * KP + k] = h;
+        Xl[rr * KP + k] = (__bf16)(x - (float)h);
     }
     __syncthreads();
-    const int rr = tid >> 2, cg = (tid & 3) * 16;
-    float acc[16];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    f4_t acc[4];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = (b && cg + c < O) ? b[cg + c] : 0.0f;
-    for (int k = 0; k < K; ++k) {
-        const float xv = Xs[rr][k];
-        const float4* wr = reinterpret_cast<const float4*>(&Ws[k][cg]);
+    for (int j = 0; j < 4; ++j) acc[j] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < Kp; k0 += 32) {
+        const bf8_t ah = *(const bf8_t*)&Xh[(16 * wave + fr) * KP + k0 + fk];
+        const bf8_t al = *(const bf8_t*)&Xl[(16 * wave + fr) * KP + k0 + fk];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 w4 = wr[q];
-            acc[4 * q] += xv * w4.x;
-            acc[4 * q + 1] += xv * w4.y;
-            acc[4 * q + 2] += xv * w4.z;
-            acc[4 * q + 3] += xv * w4.w;
+        for (int j = 0; j < 4; ++j) {
+            const bf8_t bw = *(const bf8_t*)&Ws[(16 * j + fr) * KP + k0 + fk];
+            acc[j] = mfma_bf16(al, bw, acc[j]);
+            acc[j] = mfma_bf16(ah, bw, acc[j]);
         }
     }
-    const int64_t r = r0 + rr;
-    if (r >= N) return;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        if (cg + c < O) {
-            float v = acc[c];
+    for (int j = 0; j < 4; ++j) {
+        const int col = 16 * j + fr;
+        if (col >= O) continue;
+        const float bias = b ? b[col] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = r0 + 16 * wave + 4 * (lane >> 4) + i;
+            if (r >= N) continue;
+            float v = acc[j][i] + bias;
             if (relu) v = fmaxf(v, 0.0f);
-            float* y = Y + r * ldy + cg + c;
+            float* y = Y + r * ldy + col;
             *y = accumulate ? *y + v : v;
         }
     }
 }
 
 // partial weight gradients of one chunk of WGROWS rows: part[blk][(k, o)] = sum_r X(r, k) Z(r, o),
-// k = K is the bias row (X = 1); (K + 1) x 64 outputs per chunk
+// k = K is the bias row (X = 1); (K + 1) x 64 outputs per chunk.  On v_mfma_f32_16x16x32_bf16 with
+// both FP32 operands split hi + lo (hi*hi + hi*lo + lo*hi, about 16 significant bits each), rows
+// in steps of 32 in ascending order: wave w owns output columns 16 w .. 16 w + 15 and all 9 row
+// tiles (k < 144).  The slabs are staged transposed (k-major, rows contiguous) in LDS.
 __global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ X, int ldx, const float* __restrict__ Z,
                                                int ldz, int64_t N, int K, int O, float* __restrict__ part) {
-    __shared__ float Xs[32][130];
-    __shared__ float Zs[32][65];
-    constexpr int MAXQ = (129 * 64 + 255) / 256;
-    const int tid = threadIdx.x, nout = (K + 1) * 64;
-    float acc[MAXQ];
+    constexpr int RP = 32 + 8;                // row stride of the transposed slabs (bfloat16)
+    constexpr int MT = 9;                     // 16-row tiles of k: K + 1 <= 144
+    __shared__ __attribute__((aligned(16))) __bf16 Xh[16 * MT * RP];
+    __shared__ __attribute__((aligned(16))) __bf16 Xl[16 * MT * RP];
+    __shared__ __attribute__((aligned(16))) __bf16 Zh[64 * RP];
+    __shared__ __attribute__((aligned(16))) __bf16 Zl[64 * RP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    f4_t acc[MT];
 #pragma unroll
-    for (int q = 0; q < MAXQ; ++q) acc[q] = 0.0f;
+    for (int i = 0; i < MT; ++i) acc[i] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
     const int64_t rb = (int64_t)blockIdx.x * WGROWS, re = std::min<int64_t>(N, rb + WGROWS);
     for (int64_t r0 = rb; r0 < re; r0 += 32) {
-        for (int idx = tid; idx < 32 * (K + 1); idx += 256) {
-            const int rr = idx / (K + 1), k = idx - rr * (K + 1);
+        for (int idx = tid; idx < 32 * 16 * MT; idx += 256) {
+            const int rr = idx / (16 * MT), k = idx - rr * (16 * MT);
             const int64_t r = r0 + rr;
-            Xs[rr][k] = r < re ? (k < K ? X[r * ldx + k] : 1.0f) : 0.0f;
+            const float x = (r < re && k <= K) ? (k < K ? X[r * ldx + k] : 1.0f) : 0.0f;
+            const __bf16 h = (__bf16)x;
+            Xh[k * RP + rr] = h;
+            Xl[k * RP + rr] = (__bf16)(x - (float)h);
         }
         for (int idx = tid; idx < 32 * 64; idx += 256) {
             const int rr = idx >> 6, o = idx & 63;
             const int64_t r = r0 + rr;
-            Zs[rr][o] = (r < re && o < O) ? Z[r * ldz + o] : 0.0f;
+            const float z = (r < re && o < O) ? Z[r * ldz + o] : 0.0f;
+            const __bf16 h = (__bf16)z;
+            Zh[o * RP + rr] = h;
+            Zl[o * RP + rr] = (__bf16)(z - (float)h);
         }
         __syncthreads();
+        const bf8_t bh = *(const bf8_t*)&Zh[(16 * wave + fr) * RP + fk];
+        const bf8_t bl = *(const bf8_t*)&Zl[(16 * wave + fr) * RP + fk];
 #pragma unroll
-        for (int q = 0; q < MAXQ; ++q) {
-            const int p = tid + 256 * q;
-            if (p < nout) {
-                const int k = p >> 6, o = p & 63;
-                float a = acc[q];
-#pragma unroll 8
-                for (int rr = 0; rr < 32; ++rr) a += Xs[rr][k] * Zs[rr][o];
-                acc[q] = a;
-            }
+        for (int i = 0; i < MT; ++i) {
+            const bf8_t ah = *(const bf8_t*)&Xh[(16 * i + fr) * RP + fk];
+            const bf8_t al = *(const bf8_t*)&Xl[(16 * i + fr) * RP + fk];
+            acc[i] = mfma_bf16(al, bh, acc[i]);
+            acc[i] = mfma_bf16(ah, bl, acc[i]);
+            acc[i] = mfma_bf16(ah, bh, acc[i]);
         }
         __syncthreads();
     }
     float* out = part + (int64_t)blockIdx.x * 129 * 64;
+    const int o = 16 * wave + fr;
 #pragma unroll
-    for (int q = 0; q < MAXQ; ++q) {
-        const int p = tid + 256 * q;
-        if (p < nout) out[p] = acc[q];
-    }
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = 16 * i + 4 * (lane >> 4) + q;
+            if (k <= K) out[k * 64 + o] = acc[i][q];
+        }
 }
 
 // sums the chunk partials in chunk order into the weight (K x O, row stride O) and bias gradients
