@@ -505,11 +505,11 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 constexpr int TS = 64;   // diagonal tile size (scalars)
 constexpr int PB = 8;    // column step of the tile factorization
 
-__device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
-__device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
-
 // One tile (256 threads, a and x in LDS).  On return the factor is stored to the panel, the
 // inverse to inv[t.slot], and x still holds X in LDS (all threads past the final barrier).
+// The tile is factored and inverted in double whatever the storage type T: in the FP32 forms the
+// diagonal-block pivots, the tile's own updates and its inverse are FP64 computations on the FP32
+// input, rounded to FP32 only when stored (a and x are double arrays for every T).
 // Shared tiles: with `counter`, `count` workgroups factor the same tile; each counts itself in
 // after its tile load has completed, and only the last to arrive stores the factor, the inverse
 // and a pivot failure (so no workgroup can see a stored factor in place of its input), then
@@ -518,8 +518,8 @@ __device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
 // (zero elsewhere) by the same round trip that loads the tile.
 template <typename T>
 __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L, T* __restrict__ inv,
-                                           int* __restrict__ status, T (*__restrict__ a)[TS + 1],
-                                           T (*__restrict__ x)[TS + 1], int* __restrict__ counter = nullptr,
+                                           int* __restrict__ status, double (*__restrict__ a)[TS + 1],
+                                           double (*__restrict__ x)[TS + 1], int* __restrict__ counter = nullptr,
                                            int count = 1, int* __restrict__ s_flag = nullptr,
                                            T (*__restrict__ rt)[TS + 1] = nullptr, const T* __restrict__ rsrc = nullptr,
                                            int64_t rld = 0, int rrows = 0, int rcols = 0) {
@@ -530,7 +530,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
         // every load of a thread is issued before the first use (one memory round trip); entries
         // outside the kb x kb tile read a clamped in-tile address and are replaced by the identity
         constexpr int PT = TS * TS / 256;
-        T v[PT], w[PT];
+        T v[PT], w[PT];   // (w: the row tile, kept in T)
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
             const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
@@ -546,8 +546,8 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
             const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
-            a[c][r] = (r < kb && c < kb) ? ((r >= c) ? v[q] : T(0)) : ((r == c) ? T(1) : T(0));
-            x[c][r] = T(0);
+            a[c][r] = (r < kb && c < kb) ? ((r >= c) ? (double)v[q] : 0.0) : ((r == c) ? 1.0 : 0.0);
+            x[c][r] = 0.0;
         }
         if (rt) {
 #pragma unroll
@@ -566,7 +566,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
     // A(r, c) -= sum_k L(r, k0 + k) L(c, k0 + k) for the entry (rr, cc) of block (i, j)
     auto rank8 = [&](int k0, int i, int j) {
         const int r = PB * i + rr, c = PB * j + cc;
-        T acc = T(0);
+        double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * a[k0 + k][c];
         a[c][r] -= acc;
@@ -575,28 +575,28 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
 #pragma unroll 1
     for (int bs = 0; bs < NBK; ++bs) {
         const int b0 = PB * bs;
-        T lb[PB][PB], xb[PB][PB];       // L_bb, X_bb (lower triangles), wave 0
+        double lb[PB][PB], xb[PB][PB];  // L_bb, X_bb (lower triangles), wave 0
         if (wave == 0) {
 #pragma unroll
             for (int r = 0; r < PB; ++r)
 #pragma unroll
                 for (int c = 0; c <= r; ++c) lb[r][c] = a[b0 + c][b0 + r];
-            T rd[PB];                    // 1 / L(j, j)
+            double rd[PB];               // 1 / L(j, j)
 #pragma unroll
             for (int j = 0; j < PB; ++j) {
-                T d = lb[j][j];
+                double d = lb[j][j];
 #pragma unroll
                 for (int k = 0; k < j; ++k) d -= lb[j][k] * lb[j][k];
-                if (!(d > T(0))) {
+                if (!(d > 0.0)) {
                     fail = min(fail, t.col0 + b0 + j);
-                    d = T(1);
+                    d = 1.0;
                 }
-                const T sd = sqrt_t(d);
+                const double sd = sqrt(d);
                 lb[j][j] = sd;
-                rd[j] = T(1) / sd;
+                rd[j] = 1.0 / sd;
 #pragma unroll
                 for (int i = j + 1; i < PB; ++i) {
-                    T v = lb[i][j];
+                    double v = lb[i][j];
 #pragma unroll
                     for (int k = 0; k < j; ++k) v -= lb[i][k] * lb[j][k];
                     lb[i][j] = v * rd[j];
@@ -607,7 +607,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
                 xb[c][c] = rd[c];
 #pragma unroll
                 for (int i = c + 1; i < PB; ++i) {
-                    T v = T(0);
+                    double v = 0.0;
 #pragma unroll
                     for (int k = c; k < i; ++k) v += lb[i][k] * xb[k][c];
                     xb[i][c] = -v * rd[i];
@@ -615,12 +615,12 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
             }
             const int r = b0 + PB + tid;
             if (r < TS) {
-                T ar[PB];
+                double ar[PB];
 #pragma unroll
                 for (int k = 0; k < PB; ++k) ar[k] = a[b0 + k][r];
 #pragma unroll
                 for (int c = 0; c < PB; ++c) {
-                    T v = T(0);
+                    double v = 0.0;
 #pragma unroll
                     for (int k = 0; k <= c; ++k) v += ar[k] * xb[c][k];
                     a[b0 + c][r] = v;
@@ -638,7 +638,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
                 } else {
                     const int e = q - na, i = bs + e / bs, j = e % bs;
                     const int r = PB * i + rr, c = PB * j + cc;
-                    T acc = T(0);
+                    double acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * x[c][k0 + k];
                     x[c][r] += acc;
@@ -649,12 +649,12 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
         if (wave == 0) {
             if (tid < b0) {
                 const int c = tid;
-                T sv[PB];
+                double sv[PB];
 #pragma unroll
                 for (int i = 0; i < PB; ++i) sv[i] = x[c][b0 + i];
 #pragma unroll
                 for (int i = 0; i < PB; ++i) {
-                    T v = T(0);
+                    double v = 0.0;
 #pragma unroll
                     for (int k = 0; k <= i; ++k) v += xb[i][k] * sv[k];
                     x[c][b0 + i] = -v;
@@ -688,9 +688,9 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
 #pragma unroll
     for (int q = 0; q < TS * TS / 256; ++q) {
         const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
-        const T lv = a[c][r], xv = x[c][r];
-        if (r < kb && c < kb && r >= c) D[(int64_t)c * ld + r] = lv;
-        out[idx] = (r < kb && c < kb) ? xv : T(0);
+        const double lv = a[c][r], xv = x[c][r];
+        if (r < kb && c < kb && r >= c) D[(int64_t)c * ld + r] = (T)lv;
+        out[idx] = (r < kb && c < kb) ? (T)xv : T(0);
     }
 }
 
@@ -698,8 +698,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_potrf(const PotrfTask* __restrict__ tasks,
                                                T* __restrict__ L, T* __restrict__ inv,
                                                int* __restrict__ status) {
-    __shared__ T a[TS][TS + 1];         // a[c][r] = tile(r, c) -> L(r, c)
-    __shared__ T x[TS][TS + 1];         // x[c][r] = S(r, c) -> X(r, c)
+    __shared__ double a[TS][TS + 1];    // a[c][r] = tile(r, c) -> L(r, c)
+    __shared__ double x[TS][TS + 1];    // x[c][r] = S(r, c) -> X(r, c)
     potrf_tile<T>(tasks[blockIdx.x], L, inv, status, a, x);
 }
 
@@ -899,8 +899,8 @@ __global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ ite
                                                T* __restrict__ inv, int* __restrict__ status,
                                                int* __restrict__ counters) {
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T a[TS][TS + 1];
-    __shared__ T x[TS][TS + 1];
+    __shared__ double a[TS][TS + 1];
+    __shared__ double x[TS][TS + 1];
     __shared__ T rt[TS][TS + 1];     // the row tile, rt[k][r] = A21(r, k)
     __shared__ int flag;
     const PanelItem it = items[blockIdx.x];
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ ite
 #pragma unroll
         for (int i = 0; i < 2; ++i) af[i] = rt[kr][wm + 16 * i + cc];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bf[j] = x[kr][wn + 16 * j + cc];     // B(n, k) = X(n, k)
+        for (int j = 0; j < 2; ++j) bf[j] = (T)x[kr][wn + 16 * j + cc];  // B(n, k) = X(n, k), rounded to T
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -998,13 +998,14 @@ __global__ __launch_bounds__(256, 2) void k_fused(const FusedTask* __restrict__ 
                                                   T* __restrict__ U, T* __restrict__ inv,
                                                   int* __restrict__ status) {
     constexpr int KC = 16;
-    __shared__ __attribute__((aligned(16))) T smem[2 * TS * (TS + 1)];
-    T (*a)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem);
-    T (*x)[TS + 1] = reinterpret_cast<T (*)[TS + 1]>(smem + TS * (TS + 1));
+    __shared__ __attribute__((aligned(16))) double smem[2 * TS * (TS + 1)];
+    double (*a)[TS + 1] = reinterpret_cast<double (*)[TS + 1]>(smem);
+    double (*x)[TS + 1] = reinterpret_cast<double (*)[TS + 1]>(smem + TS * (TS + 1));
     constexpr int PAD = gemm_pad<T>();
-    static_assert(4 * KC * (TS + PAD) <= 2 * TS * (TS + 1), "tile GEMM chunks fit the fused kernel's LDS");
-    T (*As)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem);
-    T (*Bs)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem + 2 * KC * (TS + PAD));
+    static_assert(4 * KC * (TS + PAD) * sizeof(T) <= sizeof(smem), "tile GEMM chunks fit the fused kernel's LDS");
+    T* smem_t = reinterpret_cast<T*>(smem);
+    T (*As)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem_t);
+    T (*Bs)[KC][TS + PAD] = reinterpret_cast<T (*)[KC][TS + PAD]>(smem_t + 2 * KC * (TS + PAD));
     for (int item = blockIdx.x; item < ntasks; item += gridDim.x) {
         const FusedTask f = tasks[item];
         const int32_t ld = f.W + f.D;

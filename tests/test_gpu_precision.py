@@ -16,7 +16,9 @@ Checks:
     log-uniform in [1e3, 1e7], nu uniform in [0.2, 0.45]), and the refined solution agrees with
     the oracle's FP64 factor solve;
   * loss of definiteness in the bfloat16 forms is reported like in FP32
-    (test_catamari_factorizer.cc:197-212).
+    (test_catamari_factorizer.cc:197-212);
+  * the diagonal blocks of every single-precision form are factored in FP64 (a dense one-tile
+    matrix gives the FP64 factor of its FP32 entries, rounded once).
 The 64^3 stress configuration runs in tests/test_gpu_large.py.
 """
 import numpy as np
@@ -138,3 +140,48 @@ def test_bf16_negated_matrix_fails_and_recovers(dev, form):
     b = torch.from_numpy(-g).to(dev)
     _, _, rr = f.solveRefined(H, b, tol=1e-10, maxIters=40)
     assert rr <= 1e-10
+
+
+def chol_fp32(A):
+    """Column Cholesky with every operation in float32 (numpy.linalg computes float32 input in
+    double), to show what FP32 pivots would give."""
+    A = A.astype(np.float32)
+    n = len(A)
+    L = np.zeros_like(A)
+    for j in range(n):
+        d = np.float32(A[j, j] - np.dot(L[j, :j], L[j, :j]))
+        L[j, j] = np.sqrt(d)
+        L[j + 1:, j] = (A[j + 1:, j] - L[j + 1:, :j] @ L[j, :j]) / L[j, j]
+    return L.astype(np.float64)
+
+
+@pytest.mark.parametrize("form", [None, "bf16", "bf16x3"])
+def test_diagonal_block_pivots_in_fp64(dev, form):
+    """In the single-precision forms the diagonal blocks are factored in FP64 from their FP32
+    entries: a dense 60 x 60 SPD matrix with unit diagonal (one supernode, one diagonal tile; the
+    diagonal scaling is exactly 1) and condition number ~1e5 gives the FP64 Cholesky factor of its
+    FP32-rounded entries, rounded once to FP32 (column-relative error <= 1e-7).  Pivots in FP32
+    would miss that bound by three orders of magnitude (shown on the host)."""
+    rng = np.random.default_rng(7)
+    n = 60
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    M = (Q * np.logspace(0, -5, n)) @ Q.T
+    d = np.sqrt(np.diag(M))
+    A = M / d[:, None] / d[None, :]
+    np.fill_diagonal(A, 1.0)
+    iu, ju = np.nonzero(np.triu(np.ones((n, n))))
+    S = mfs.SuiteSparseMatrix.fromTriplets(n, iu, ju, A[iu, ju], device=dev)
+    f = factorizer(form)
+    f.factorize(S)
+    assert f.supernodeStats()["num_supernodes"] == 1 and f.supernodeStats()["max_width"] == n
+    order = f.getInversePermutation()
+    A32 = A.astype(np.float32).astype(np.float64)[np.ix_(order, order)]
+    Lref = np.linalg.cholesky(A32)
+    cm = np.abs(Lref).max(axis=0)
+    Lg = orc.dense_L(*f.exportFactorCSC())
+    err = (np.abs(Lg - Lref).max(axis=0) / cm).max()
+    err32 = (np.abs(chol_fp32(A32) - Lref).max(axis=0) / cm).max()
+    print(f"form {form}: FP64-pivot tile error {err:.2e}; FP32 pivots would give {err32:.2e}")
+    assert err <= 1e-7
+    assert err32 > 1e3 * 1e-7
+    assert np.array_equal(Lg, Lg.astype(np.float32).astype(np.float64))     # stored in FP32
