@@ -100,6 +100,14 @@ int mfs_assembler_create(const int64_t* Ap, const int64_t* Ai, int64_t num_block
                          mfs_assembler* out);
 int mfs_assemble_hessian(mfs_assembler a, const double* He_dev, int he_layout,
                          double* Ax_dev, void* stream);
+/* The same into the non-contiguous block layout, BlockCSCHessian<..., ContiguousBlocks=false> */
+/* (BlockCSCHessian.hh:52-97; StoreFullDiagonalBlocks = ContiguousBlocks, :570): Ax is the      */
+/* scalar upper-triangle CSC of the 3n x 3n matrix (9*nz - 3*n values; block column j at        */
+/* 9*Ap[j] - 3*j, scalar column 3j+c holding 3 rows per off-diagonal block, then rows 3j..3j+c  */
+/* of the diagonal block).  Every column's last block must be its diagonal block               */
+/* (MFS_ERR_ARG otherwise).  Same element order and sums as mfs_assemble_hessian.               */
+int mfs_assemble_hessian_noncontiguous(mfs_assembler a, const double* He_dev, int he_layout,
+                                       double* Ax_dev, void* stream);
 /* Ax = 0 over the 9*nz values (BlockCSCHessianBase::setZero, BlockCSCHessian.hh:638-649).    */
 int mfs_assembler_zero(mfs_assembler a, double* Ax_dev, void* stream);
 int mfs_assembler_destroy(mfs_assembler a);

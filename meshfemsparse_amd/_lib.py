@@ -28,6 +28,7 @@ _SIGS = {
     "mfs_pattern_destroy": [c_vp],
     "mfs_assembler_create": [P_i64, P_i64, c_i64, P_i32, c_i64, c_i32, ctypes.POINTER(c_vp)],
     "mfs_assemble_hessian": [c_vp, c_vp, ctypes.c_int, c_vp, c_vp],
+    "mfs_assemble_hessian_noncontiguous": [c_vp, c_vp, ctypes.c_int, c_vp, c_vp],
     "mfs_assembler_zero": [c_vp, c_vp, c_vp],
     "mfs_assembler_destroy": [c_vp],
     "mfs_neohookean_elements": [c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_i32, c_vp, c_vp],

@@ -98,9 +98,7 @@ class BorderedSparseHessian:
     def addDiag(self, d):
         if self.H_ss is not None:
             ax = self.H_ss.Ax
-            k = torch.as_tensor(9 * (np.asarray(self.H_ss.Ap[1:]) - 1), device=ax.device)
-            for c in (0, 4, 8):
-                ax[k + c] += d
+            ax[torch.as_tensor(self.H_ss.diagEntryPositions(), device=ax.device)] += d
         self.H_dd.diagonal().add_(d)
 
     def addLowRank(self, V, w=1.0):

@@ -124,6 +124,14 @@ int mfs_assemble_hessian(mfs_assembler a, const double* He, int layout, double* 
     MFS_CATCH
 }
 
+int mfs_assemble_hessian_noncontiguous(mfs_assembler a, const double* He, int layout, double* Ax, void* stream) {
+    MFS_TRY
+    CHECK_PTR(a); CHECK_PTR(Ax);
+    if (a->dev->nslots() > 0) CHECK_PTR(He);
+    a->dev->assemble(He, layout, Ax, stream, false);
+    MFS_CATCH
+}
+
 int mfs_assembler_zero(mfs_assembler a, double* Ax, void* stream) {
     MFS_TRY
     CHECK_PTR(a); CHECK_PTR(Ax);

@@ -31,7 +31,9 @@ Pattern build_block_pattern_device(const int32_t* elem_nodes_dev, int64_t ne, in
 class AssemblerDev {
 public:
     AssemblerDev(const Pattern& P, const int32_t* elem_nodes, int64_t ne, int npe);
-    void assemble(const double* He, int layout, double* Ax, void* stream);
+    // contiguous: Ax holds 9 values per block; otherwise the non-contiguous (scalar upper CSC)
+    // layout, which needs every diagonal block in the pattern
+    void assemble(const double* He, int layout, double* Ax, void* stream, bool contiguous = true);
     void zero(double* Ax, void* stream);
     // neo-Hookean element Hessians (layout 0: 144, 1: 78 per element) and per-element gradients
     // (12 per element); any output may be null
@@ -41,9 +43,13 @@ public:
     void assemble_gradient(const double* g12, double* g, void* stream);
     int64_t nslots() const { return nslots_; }
 private:
+    void build_noncontiguous_map();
     int64_t nslots_ = 0, ne_ = 0, nnodes_ = 0;
     int npe_ = 0;
+    bool diag_last_ = false;            // every column ends with its diagonal block
+    std::vector<int64_t> ap_;
     DeviceArray seg_, contrib_, diag_, pslot_, tets_, nptr_, nent_;
+    DeviceArray ncbase_, ncstride_;     // per slot, built on the first non-contiguous assembly
 };
 
 // Optional per-kernel-class device timing (HIP event pairs around each launch).

@@ -175,6 +175,8 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
                                                   const int32_t* __restrict__ slot_of_pos,
                                                   const uint8_t* __restrict__ is_diag,
                                                   const double* __restrict__ He,
+                                                  const int64_t* __restrict__ ncbase,
+                                                  const int32_t* __restrict__ ncstride,
                                                   double* __restrict__ Ax, int64_t npos) {
     __shared__ uint32_t lc[kAsmLds];
     const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
@@ -194,7 +196,10 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
     const bool diag = is_diag[p] != 0;
     if (diag && r > c) return;                 // strict lower part of a diagonal block: untouched
     const int64_t s = slot_of_pos[p];
-    double v = Ax[9 * s + q];
+    // contiguous blocks: 9 values per slot; non-contiguous (scalar upper CSC): column c of the
+    // slot's block starts c * stride + c (c + 1) / 2 after the slot's base
+    const int64_t dst = ncbase ? ncbase[s] + (int64_t)c * ncstride[s] + (c * (c + 1)) / 2 + r : 9 * s + q;
+    double v = Ax[dst];
     const int32_t b = seg[p], e_ = seg[p + 1];
     // a transposed contribution adds entry (c, r) of the element's block
     auto value = [&](int32_t k) {
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
         v += x3;
     }
     for (; k < e_; ++k) v += value(k);
-    Ax[9 * s + q] = v;
+    Ax[dst] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -297,6 +302,10 @@ __global__ __launch_bounds__(256) void k_grad_gather(const int64_t* __restrict__
 
 AssemblerDev::AssemblerDev(const Pattern& P, const int32_t* en, int64_t ne, int npe) {
     SlotMap M = build_slot_map(P, en, ne, npe);
+    ap_ = P.Ap;
+    diag_last_ = true;
+    for (int64_t j = 0; j < P.n; ++j)
+        diag_last_ = diag_last_ && P.Ap[j + 1] > P.Ap[j] && P.Ai[P.Ap[j + 1] - 1] == j;
     nslots_ = M.nslots;
     ne_ = ne;
     npe_ = npe;
@@ -337,17 +346,41 @@ void AssemblerDev::assemble_gradient(const double* g12, double* g, void* stream)
     LAUNCH_CHECK();
 }
 
-void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stream) {
+// Non-contiguous layout (scalar upper CSC): block column j starts at 9 Ap[j] - 3 j; its scalar
+// column c holds 3 (ne - 1) + c + 1 entries (3 rows per off-diagonal block, then the diagonal
+// block's rows 0..c), so block idx of the column starts at base = start + 3 idx in column 0 and
+// column c of it c * 3 (ne - 1) + c (c + 1) / 2 further on.
+void AssemblerDev::build_noncontiguous_map() {
+    if (!diag_last_) throw Error(-1, "the non-contiguous layout needs every diagonal block in the pattern");
+    const int64_t n = (int64_t)ap_.size() - 1;
+    std::vector<int64_t> base(nslots_);
+    std::vector<int32_t> stride(nslots_);
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t ncol = ap_[j + 1] - ap_[j], s0 = 9 * ap_[j] - 3 * j;
+        if (3 * (ncol - 1) > INT32_MAX) throw Error(-1, "column too long for the non-contiguous map");
+        for (int64_t k = ap_[j]; k < ap_[j + 1]; ++k) {
+            base[k] = s0 + 3 * (k - ap_[j]);
+            stride[k] = (int32_t)(3 * (ncol - 1));
+        }
+    }
+    ncbase_.upload(base);
+    ncstride_.upload(stride);
+}
+
+void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stream, bool contiguous) {
     if (layout != 0 && layout != 1) throw Error(-1, "unknown element Hessian layout");
+    if (!contiguous && ncbase_.bytes == 0 && nslots_ > 0) build_noncontiguous_map();
     if (nslots_ == 0) return;
+    const int64_t* nb = contiguous ? nullptr : ncbase_.as<int64_t>();
+    const int32_t* ns = contiguous ? nullptr : ncstride_.as<int32_t>();
     const int64_t nwg = (nslots_ + kAsmSlotsPerWG - 1) / kAsmSlotsPerWG;
     const unsigned grid = (unsigned)(8 * ((nwg + 7) / 8));       // whole rounds of the 8 XCDs
     if (layout == 0)
         k_assemble<0><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
-                                                    diag_.as<uint8_t>(), He, Ax, nslots_);
+                                                    diag_.as<uint8_t>(), He, nb, ns, Ax, nslots_);
     else
         k_assemble<1><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
-                                                    diag_.as<uint8_t>(), He, Ax, nslots_);
+                                                    diag_.as<uint8_t>(), He, nb, ns, Ax, nslots_);
     LAUNCH_CHECK();
 }
 

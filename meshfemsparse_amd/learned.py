@@ -36,6 +36,11 @@ def graph_host(H, pins):
     return out
 
 
+
+def _contiguous_only(H):
+    if not H.hasContiguousBlocks():
+        raise ValueError("the learned preconditioner reads contiguous 3x3 blocks; use cloneWithLayout(True)")
+
 class LearnedPreconditioner:
     def __init__(self, H, pinnedVars=(), seed=0):
         pins = np.unique(np.asarray(pinnedVars, np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
@@ -71,6 +76,7 @@ class LearnedPreconditioner:
         return out
 
     def setMatrix(self, H):
+        _contiguous_only(H)
         _lib.call("mfs_learned_set_matrix", self._h, _dptr(H.Ax), _stream())
 
     def features(self):
@@ -125,6 +131,7 @@ class LearnedPreconditioner:
     def solve(self, H, b, precond="learned", tol=1e-10, maxPCG=200, fallback=None):
         """A x = b to tol (FP32 PCG + FP64 refinement; the fallback factorizer takes over after
         maxPCG CG iterations).  Returns (x, pcg iterations, refinement steps, relres, used fallback)."""
+        _contiguous_only(H)
         x = torch.empty_like(b)
         it, ot, fb = _lib.c_i32(), _lib.c_i32(), _lib.c_i32()
         rr = _lib.c_double()

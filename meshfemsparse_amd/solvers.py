@@ -16,7 +16,20 @@ import torch
 from . import _lib
 from ._lib import MFSError, NotPositiveDefiniteError, c_vp, ptr
 from .ordering import NDOrdering, TemporalReuseStatistics, _stats
-from .sparse import SuiteSparseMatrix, _dptr, _stream
+from .sparse import BlockCSCHessian, SuiteSparseMatrix, _dptr, _stream
+
+
+def _as_scalar(H):
+    """The scalar upper-CSC view a non-contiguous block matrix is factored through (its value
+    array is already that layout, BlockCSCHessian.hh:52-97); other matrices unchanged."""
+    if isinstance(H, BlockCSCHessian) and not H.hasContiguousBlocks():
+        return H.toScalar()
+    return H
+
+
+def _pattern_key(H):
+    nc = isinstance(H, BlockCSCHessian) and not H.hasContiguousBlocks()
+    return ("noncontiguous" if nc else "scalar" if isinstance(H, SuiteSparseMatrix) else "block", H.n, H.nz)
 
 
 class CholeskyProvider(enum.Enum):
@@ -207,29 +220,34 @@ class GpuSupernodalFactorizer:
 
     # -- factorization --------------------------------------------------------------------------
     def factorizeSymbolic(self, H, pinnedVars=()):
-        """BlockCSCHessian (3x3 blocks) or SuiteSparseMatrix (scalar, upper triangle) overloads.
-        Pins may split blocks: a partially pinned block keeps its free components."""
+        """BlockCSCHessian (3x3 blocks, contiguous or not) or SuiteSparseMatrix (scalar, upper
+        triangle) overloads.  Pins may split blocks: a partially pinned block keeps its free
+        components.  A non-contiguous block matrix is analysed as the scalar matrix its values
+        already form; the factor and its ordering are those of the same matrix in blocks."""
         pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
-        scalar = isinstance(H, SuiteSparseMatrix)
+        key = _pattern_key(H)
+        S = _as_scalar(H)
+        scalar = isinstance(S, SuiteSparseMatrix)
         if scalar:
-            _lib.call("mfs_factorize_symbolic_scalar", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
-                      H.n, ptr(pins, _lib.c_i64), len(pins))
+            _lib.call("mfs_factorize_symbolic_scalar", self._h, ptr(S.Ap, _lib.c_i64), ptr(S.Ai, _lib.c_i64),
+                      S.n, ptr(pins, _lib.c_i64), len(pins))
         else:
             _lib.call("mfs_factorize_symbolic", self._h, ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64),
                       H.n, 3, ptr(pins, _lib.c_i64), len(pins))
         self._fixed = [int(p) for p in pins]
-        self._n_full = H.n if scalar else 3 * H.n
+        self._n_full = S.n if scalar else 3 * H.n
         self._n_red = self._n_full - len(self._fixed)
-        self._Hpattern = (scalar, H.n, H.nz)
+        self._Hpattern = key
 
     def _check_pattern(self, H):
-        if self._Hpattern is not None and self._Hpattern != (isinstance(H, SuiteSparseMatrix), H.n, H.nz):
-            raise ValueError("matrix pattern differs from the symbolic factorization")
+        if self._Hpattern is not None and self._Hpattern != _pattern_key(H):
+            raise ValueError("matrix pattern or value layout differs from the symbolic factorization")
         if H.isSparsityOnly():
             raise ValueError("cannot factorize a sparsity-only matrix")
 
     def _numeric(self, A, sigma, Bp):
-        name = "mfs_factorize_numeric_scalar" if isinstance(A, SuiteSparseMatrix) else "mfs_factorize_numeric"
+        scalar = isinstance(_as_scalar(A), SuiteSparseMatrix)
+        name = "mfs_factorize_numeric_scalar" if scalar else "mfs_factorize_numeric"
         _lib.call(name, self._h, _dptr(A.Ax), float(sigma), Bp, _stream())
 
     def factorizeNumeric(self, H, isInTryCatch=False):
@@ -242,7 +260,7 @@ class GpuSupernodalFactorizer:
         self._check_pattern(A)
         Bp = None
         if B is not None:
-            if (type(B) is not type(A) or B.n != A.n or B.nz != A.nz or not np.array_equal(B.Ap, A.Ap)
+            if (_pattern_key(B) != _pattern_key(A) or not np.array_equal(B.Ap, A.Ap)
                     or not np.array_equal(B.Ai, A.Ai)):
                 raise ValueError("Shift matrix must have the same sparsity and value layout as A")
             Bp = _dptr(B.Ax)
@@ -327,11 +345,13 @@ class GpuSupernodalFactorizer:
 
     # -- extras: SpMV, refinement, factor export ------------------------------------------------
     def apply(self, H, x):
+        self._check_pattern(H)
         y = torch.empty_like(x)
         _lib.call("mfs_spmv", self._h, _dptr(H.Ax), _dptr(x.contiguous()), _dptr(y), _stream())
         return y
 
     def solveRefined(self, H, b, tol=1e-10, maxIters=10):
+        self._check_pattern(H)
         x = torch.empty_like(b)
         it, rr = _lib.c_i32(), _lib.c_double()
         _lib.call("mfs_solve_refined", self._h, _dptr(H.Ax), _dptr(b.contiguous()), _dptr(x), float(tol),
@@ -342,6 +362,7 @@ class GpuSupernodalFactorizer:
         """Refinement until every segment [segments[s], segments[s+1]) of the full-length vector
         (one mesh of a batch) reaches ||b_s - (A x)_s|| / ||b_s|| <= tol.
         Returns (x, iterations, max relative residual, per-segment relative residuals)."""
+        self._check_pattern(H)
         seg = np.ascontiguousarray(segments, np.int64)
         x = torch.empty_like(b)
         it, rr = _lib.c_i32(), _lib.c_double()
@@ -368,7 +389,9 @@ class GpuSupernodalFactorizer:
         with at most that many structure rows; 0: none), then per supernode size bucket (front
         W + D < 512, < 2048, larger) the largest GEMM tile among `tiles` (128 or 64).  The first
         candidates are the start and a candidate must be 2 % faster to replace the incumbent.
-        Leaves a symbolic factorization with the chosen schedule."""
+        Leaves a symbolic factorization with the chosen schedule.  H has contiguous blocks."""
+        if not isinstance(H, BlockCSCHessian) or not H.hasContiguousBlocks():
+            raise ValueError("autotune takes a block matrix with contiguous blocks")
         pins = np.unique(np.asarray(pinnedVars, dtype=np.int64)) if len(pinnedVars) else np.zeros(0, np.int64)
         fz = np.ascontiguousarray(fuse, np.int32)
         tz = np.ascontiguousarray(tiles, np.int32)
@@ -380,7 +403,7 @@ class GpuSupernodalFactorizer:
         self._fixed = [int(p) for p in pins]
         self._n_full = 3 * H.n
         self._n_red = 3 * H.n - len(self._fixed)
-        self._Hpattern = (False, H.n, H.nz)
+        self._Hpattern = _pattern_key(H)
         nf = len(fz)
         table = {"fuse": {int(a): float(ms[i]) for i, a in enumerate(fz)},
                  "bucket_tiles": [{int(t): float(ms[nf + b * len(tz) + j]) for j, t in enumerate(tz)} for b in range(3)]}

@@ -1,10 +1,13 @@
 """Host mirror of the reference's block-CSC Hessian and assembler interfaces.
 
-BlockCSCHessian  <- BlockCSCHessian<OptimizationVarStructure<3>, ContiguousBlocks=true>
-                    (src/lib/MeshFEMSparse/BlockCSCHessian.hh:560-571, 798-875): upper block
-                    pattern Ap/Ai (int64, host) and the value array Ax (9 doubles per block,
-                    column-major 3x3, diagonal block last per column, upper triangle meaningful),
-                    held in device memory.
+BlockCSCHessian  <- BlockCSCHessian<OptimizationVarStructure<3>, ContiguousBlocks>
+                    (src/lib/MeshFEMSparse/BlockCSCHessian.hh:52-97, 560-571, 798-875): upper
+                    block pattern Ap/Ai (int64, host) and the value array Ax in device memory.
+                    Contiguous blocks (the default): 9 doubles per block, column-major 3x3,
+                    diagonal block last per column, upper triangle meaningful.  Non-contiguous:
+                    Ax is the scalar upper-triangle CSC of the 3n x 3n matrix (diagonal blocks
+                    store their upper triangle only), as StoreFullDiagonalBlocks = ContiguousBlocks
+                    makes it in the reference.
 SystemAssembler  <- SystemAssembler<3> (src/lib/MeshFEMSparse/SystemAssembler.hh):
                     blockSparsityPattern (:251-465) and assembleHessian (:559-565), which
                     accumulates into H.Ax and allocates it when H is sparsity-only.
@@ -31,11 +34,11 @@ def _dptr(t):
 
 
 class BlockCSCHessian:
-    """Uniform 3x3-block upper-triangle CSC matrix with contiguous blocks."""
+    """Uniform 3x3-block upper-triangle CSC matrix, contiguous or non-contiguous blocks."""
 
     blockSize = 3
 
-    def __init__(self, Ap, Ai, device="cuda"):
+    def __init__(self, Ap, Ai, device="cuda", contiguous=True):
         self.Ap = np.ascontiguousarray(Ap, dtype=np.int64)
         self.Ai = np.ascontiguousarray(Ai, dtype=np.int64)
         self.n = len(self.Ap) - 1
@@ -43,13 +46,22 @@ class BlockCSCHessian:
         self.nz = int(self.Ap[-1])
         self.Ax = None
         self.device = device
+        self.contiguousBlocks = bool(contiguous)
+        self._layout = None
+        if not self.contiguousBlocks:
+            self._check_diagonal_blocks()
 
     # -- BlockCSCHessianBase interface --------------------------------------------------------
     def numScalarCols(self):
         return 3 * self.n
 
     def scalarNNZ(self):
-        return 9 * self.nz
+        """9 values per block; the non-contiguous layout drops the strict lower triangle of the
+        diagonal blocks (BlockCSCHessian.hh:74-80)."""
+        return 9 * self.nz if self.contiguousBlocks else 9 * self.nz - 3 * self.n
+
+    def hasContiguousBlocks(self):
+        return self.contiguousBlocks
 
     def isSparsityOnly(self):
         return self.Ax is None
@@ -59,6 +71,106 @@ class BlockCSCHessian:
 
     def uniformBlockSize(self):
         return True
+
+    def _check_diagonal_blocks(self):
+        last = self.Ap[1:] - 1
+        if self.n and ((np.diff(self.Ap) <= 0).any() or (self.Ai[last] != np.arange(self.n)).any()):
+            raise ValueError("the non-contiguous layout needs every diagonal block (BlockCSCHessian.hh:77)")
+
+    def col_nnz(self, bj):
+        return int(self.Ap[bj + 1] - self.Ap[bj])
+
+    def scalarColStride(self, bj):
+        """Entries of the first scalar column of block column bj (BlockCSCHessian.hh:64-71);
+        each later scalar column of bj holds one more (the diagonal block's upper triangle)."""
+        if self.contiguousBlocks:
+            return 3
+        return 3 * (self.col_nnz(bj) - 1) + 1
+
+    def scalarOffsetForColumn(self, bj):
+        """BlockCSCHessian.hh:73-80."""
+        return 9 * int(self.Ap[bj]) - (0 if self.contiguousBlocks else 3 * bj)
+
+    def findEntry(self, bi, bj):
+        lo, hi = int(self.Ap[bj]), int(self.Ap[bj + 1])
+        k = lo + int(np.searchsorted(self.Ai[lo:hi], bi))
+        if k >= hi or self.Ai[k] != bi:
+            raise KeyError(f"block ({bi}, {bj}) is not in the pattern")
+        return k
+
+    def scalarOffsetWithinColumn(self, bi, bj):
+        """BlockCSCHessian.hh:88-92."""
+        idx = self.findEntry(bi, bj) - int(self.Ap[bj])
+        return (9 if self.contiguousBlocks else 3) * idx
+
+    def locForBlock(self, bi, bj):
+        """Position in Ax of entry (0, 0) of block (bi, bj) (BlockCSCHessian.hh:83-86)."""
+        return self.scalarOffsetForColumn(bj) + self.scalarOffsetWithinColumn(bi, bj)
+
+    def _layout_maps(self):
+        """(pos, Sp, Si): pos[9 k + 3 c + r] = position of block k's entry (r, c) in the
+        non-contiguous value array (-1 for the strict lower triangle of a diagonal block), and
+        the scalar upper CSC pattern (Sp, Si) that array follows."""
+        if self._layout is None:
+            self._check_diagonal_blocks()
+            n, Ap, Ai = self.n, self.Ap, self.Ai
+            ne = np.diff(Ap)
+            col = np.repeat(np.arange(n), ne)                   # block column of every block
+            idx = np.arange(self.nz) - Ap[col]
+            base = 9 * Ap[col] - 3 * col + 3 * idx
+            stride = 3 * (ne[col] - 1)
+            diag = Ai == col
+            pos = np.empty((self.nz, 3, 3), np.int64)           # [k][c][r]
+            for c in range(3):
+                for r in range(3):
+                    pos[:, c, r] = np.where(diag & (r > c), -1, base + c * stride + c * (c + 1) // 2 + r)
+            pos = pos.reshape(-1)
+            Sp = np.zeros(3 * n + 1, np.int64)
+            lens = 3 * (ne[:, None] - 1) + np.arange(1, 4)[None, :]
+            Sp[1:] = np.cumsum(lens.reshape(-1))
+            Si = np.empty(int(Sp[-1]), np.int64)
+            valid = pos >= 0
+            rows = (3 * Ai[:, None, None] + np.arange(3)[None, None, :]).repeat(3, axis=1).reshape(-1)
+            Si[pos[valid]] = rows[valid]
+            self._layout = (pos, Sp, Si)
+        return self._layout
+
+    def cloneWithLayout(self, contiguous):
+        """BlockCSCHessian::cloneWithLayout: the same matrix in the other value layout (the
+        strict lower triangle of diagonal blocks becomes zero going to contiguous blocks)."""
+        H = BlockCSCHessian(self.Ap, self.Ai, self.device, contiguous)
+        if contiguous == self.contiguousBlocks or self.Ax is None:
+            H.Ax = None if self.Ax is None else self.Ax.clone()
+            return H
+        pos = self._layout_maps()[0]
+        H._layout = self._layout
+        valid = np.nonzero(pos >= 0)[0]
+        dev = self.Ax.device
+        src_c = torch.from_numpy(valid).to(dev)
+        dst_nc = torch.from_numpy(pos[valid]).to(dev)
+        if contiguous:
+            H.Ax = torch.zeros(9 * self.nz, dtype=self.Ax.dtype, device=dev)
+            H.Ax[src_c] = self.Ax[dst_nc]
+        else:
+            H.Ax = torch.empty(9 * self.nz - 3 * self.n, dtype=self.Ax.dtype, device=dev)
+            H.Ax[dst_nc] = self.Ax[src_c]
+        return H
+
+    def cloneWithContiguousBlocks(self):
+        return self.cloneWithLayout(True)
+
+    def cloneWithNoncontiguousBlocks(self):
+        return self.cloneWithLayout(False)
+
+    def toScalar(self):
+        """The scalar upper-triangle CSC matrix (BlockCSCHessianBase::toScalar).  For
+        non-contiguous blocks the value array is shared, not copied."""
+        _, Sp, Si = self._layout_maps()
+        if self.contiguousBlocks:
+            Ax = None if self.Ax is None else self.cloneWithLayout(False).Ax
+        else:
+            Ax = self.Ax
+        return SuiteSparseMatrix(3 * self.n, Sp, Si, Ax, self.device)
 
     def setZero(self):
         """BlockCSCHessian.hh:638-649: allocate Ax on first use, otherwise zero it."""
@@ -71,28 +183,39 @@ class BlockCSCHessian:
         return self.Ax
 
     def diagBlockLoc(self, j):
-        return 9 * (int(self.Ap[j + 1]) - 1)
+        return self.locForBlock(j, j)
+
+    def diagEntryPositions(self):
+        """Positions in Ax of the 3n diagonal scalar entries, in scalar order."""
+        k = self.Ap[1:] - 1                                     # diagonal block of each column
+        cont = 9 * k[:, None] + np.array([0, 4, 8])[None, :]
+        if self.contiguousBlocks:
+            return cont.reshape(-1)
+        return self._layout_maps()[0][cont.reshape(-1)]
+
+    def _host_contiguous(self):
+        H = self if self.contiguousBlocks else self.cloneWithLayout(True)
+        return H.Ax.detach().cpu().numpy() if isinstance(H.Ax, torch.Tensor) else np.asarray(H.Ax)
 
     def trace(self):
         """Sum of the diagonal scalar entries (BlockCSCHessianBase::trace)."""
         Ax = self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor) else np.asarray(self.Ax)
-        d = 9 * (self.Ap[1:] - 1)
-        return float(Ax[d].sum() + Ax[d + 4].sum() + Ax[d + 8].sum())
+        return float(Ax[self.diagEntryPositions()].sum())
 
     def dumpBinaryToFile(self, path):
         """BlockCSCHessianBase::dumpBinaryToStream (BlockCSCHessian.cc:11-40): the reference's
         binary format, non-contiguous layout on disk; a sparsity-only matrix writes no values."""
         Ax = None
         if self.Ax is not None:
-            Ax = np.ascontiguousarray(self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor)
-                                      else self.Ax, np.float64)
+            Ax = np.ascontiguousarray(self._host_contiguous(), np.float64)
         _lib.call("mfs_blockcsc_write", str(path).encode(), ptr(self.Ap, _lib.c_i64), ptr(self.Ai, _lib.c_i64),
                   self.n, None if Ax is None else ptr(Ax, _lib.c_double))
 
     @classmethod
-    def constructFromBinaryFile(cls, path, device="cuda"):
-        """BlockCSCHessianBase::constructFromBinaryStream (BlockCSCHessian.cc:88-103), returning the
-        contiguous-block layout with values on `device` (None for a sparsity-only file)."""
+    def constructFromBinaryFile(cls, path, device="cuda", contiguous=True):
+        """BlockCSCHessianBase::constructFromBinaryStream (BlockCSCHessian.cc:88-103), returning
+        the contiguous-block layout (ContiguousBlocksDefault) unless `contiguous` is False, with
+        values on `device` (None for a sparsity-only file)."""
         nb, nz, snz = _lib.c_i64(), _lib.c_i64(), _lib.c_i64()
         p = str(path).encode()
         _lib.call("mfs_blockcsc_read", p, nb, nz, snz, None, None, None)
@@ -104,11 +227,11 @@ class BlockCSCHessian:
         H = cls(Ap, Ai, device)
         if Ax is not None:
             H.Ax = torch.from_numpy(Ax).to(device)
-        return H
+        return H if contiguous else H.cloneWithLayout(False)
 
     def toDense(self):
         """Dense symmetric scalar matrix (host, float64); for small tests."""
-        Ax = self.Ax.detach().cpu().numpy() if isinstance(self.Ax, torch.Tensor) else np.asarray(self.Ax)
+        Ax = self._host_contiguous()
         A = np.zeros((3 * self.n, 3 * self.n))
         for j in range(self.n):
             for k in range(self.Ap[j], self.Ap[j + 1]):
@@ -219,13 +342,14 @@ class SystemAssembler:
         return a
 
     def assembleHessian(self, H, elem_nodes, He, layout=MFS_HE_FULL12):
-        """Accumulate element Hessians He (device, ne x 144 or ne x 78 float64) into H.Ax."""
+        """Accumulate element Hessians He (device, ne x 144 or ne x 78 float64) into H.Ax, in
+        H's value layout."""
         if H is None:
             raise RuntimeError("Attempted to assemble into a BorderedSparseHessian with no block sparsity pattern.")
         if H.isSparsityOnly():
             H.setZero()
         en = np.ascontiguousarray(elem_nodes, dtype=np.int32)
-        self._assembler(H, en).assemble(He, H.Ax, layout)
+        self._assembler(H, en).assemble(He, H.Ax, layout, H.hasContiguousBlocks())
 
 
 class DeviceAssembler:
@@ -238,11 +362,14 @@ class DeviceAssembler:
         _lib.call("mfs_assembler_create", ptr(H.Ap, _lib.c_i64), ptr(H.Ai, _lib.c_i64), H.n,
                   ptr(en, _lib.c_i32), self.ne, npe, self._h)
 
-    def assemble(self, He, Ax, layout=MFS_HE_FULL12):
+    def assemble(self, He, Ax, layout=MFS_HE_FULL12, contiguous=True):
+        """Ax += the assembled element Hessians; `contiguous` selects the value layout of Ax
+        (9 per block, or the non-contiguous scalar upper CSC)."""
         per = 144 if layout == MFS_HE_FULL12 else 78
         if He.numel() != per * self.ne:
             raise ValueError(f"He must hold {per} doubles per element")
-        _lib.call("mfs_assemble_hessian", self._h, _dptr(He), layout, _dptr(Ax), _stream())
+        name = "mfs_assemble_hessian" if contiguous else "mfs_assemble_hessian_noncontiguous"
+        _lib.call(name, self._h, _dptr(He), layout, _dptr(Ax), _stream())
 
     def zero(self, Ax):
         _lib.call("mfs_assembler_zero", self._h, _dptr(Ax), _stream())

@@ -149,6 +149,8 @@ def expand_values(H, Ap, Ai):
     tests/benchmark_sparsity_lru.cc)."""
     import torch
     from .sparse import BlockCSCHessian
+    if not H.hasContiguousBlocks():
+        return expand_values(H.cloneWithLayout(True), Ap, Ai).cloneWithLayout(False)
     n = H.n
     Ap = np.asarray(Ap, np.int64)
     Ai = np.asarray(Ai, np.int64)
