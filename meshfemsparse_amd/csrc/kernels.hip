@@ -144,18 +144,39 @@ static void fill(T* p, int64_t n, T v, hipStream_t s) {
 // (off-diagonal and diagonal slots apart), and consecutive workgroups run on one XCD, so the
 // element Hessians the neighbouring slots share are fetched into one L2.
 // ------------------------------------------------------------------------------------------
+// Offset in He (in doubles) of entry (r, c) of contribution w: element w >> 5, local-upper block
+// pair = (w >> 1) & 15 (6 off-diagonal blocks (a, b), a < b, then the 4 diagonal blocks),
+// transposed if w & 1 (then entry (c, r) of the block is read).  `lo` holds the lane's in-block
+// offsets for (r, c) and (c, r): off-diagonal blocks at lo.x / lo.y, diagonal blocks at lo.z / lo.w
+// (UPPER78 only; r <= c there).  Selects only, so the loads of several contributions issue
+// together.
+struct AsmLane { int x, y, z, w; };
+
 template <int LAYOUT>
-__device__ __forceinline__ double he_entry(const double* __restrict__ He, int64_t e, int pair, int r, int c) {
-    // local-upper block `pair` of element e, entry (r, c) of the 3x3 block
+__device__ __forceinline__ AsmLane asm_lane(int r, int c) {
+    if constexpr (LAYOUT == 0) return AsmLane{12 * c + r, 12 * r + c, 0, 0};
+    else return AsmLane{3 * c + r, 3 * r + c, (c * (c + 1)) / 2 + r, (r * (r + 1)) / 2 + c};
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ uint32_t he_local(uint32_t w, const AsmLane& lo) {
+    const uint32_t pair = (w >> 1) & 15u;
+    const bool tr = (w & 1u) != 0;
     if constexpr (LAYOUT == 0) {
-        const int a = pair < 6 ? (pair < 3 ? 0 : (pair < 5 ? 1 : 2)) : pair - 6;
-        const int b = pair < 6 ? (pair < 3 ? pair + 1 : (pair < 5 ? pair - 1 : 3)) : pair - 6;
-        return He[e * 144 + (3 * b + c) * 12 + 3 * a + r];
+        // 36 b + 3 a of block (a, b), 8 bits per pair: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3) (0,0)
+        // (1,1) | (2,2) (3,3)
+        constexpr uint64_t K0 = 36ull | 72ull << 8 | 108ull << 16 | 75ull << 24 | 111ull << 32 | 114ull << 40 |
+                                0ull << 48 | 39ull << 56;
+        constexpr uint64_t K1 = 78ull | 117ull << 8;
+        const uint32_t ab = (uint32_t)((pair < 8 ? K0 >> (8 * pair) : K1 >> (8 * (pair - 8))) & 0xffu);
+        return ab + (uint32_t)(tr ? lo.y : lo.x);
     } else {
-        if (pair < 6) return He[e * 78 + 9 * pair + 3 * c + r];
-        return He[e * 78 + 54 + 6 * (pair - 6) + (c * (c + 1)) / 2 + r];   // r <= c
+        return pair < 6 ? 9 * pair + (uint32_t)(tr ? lo.y : lo.x) : 18 + 6 * pair + (uint32_t)(tr ? lo.w : lo.z);
     }
 }
+
+// He element stride in doubles
+template <int LAYOUT> constexpr uint32_t he_stride() { return LAYOUT == 0 ? 144u : 78u; }
 
 constexpr int kAsmSeg = 9;                     // lanes per slot
 constexpr int kAsmSlotsPerWave = 7;            // 63 of 64 lanes
@@ -169,7 +190,8 @@ __device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t n) {
     return (b % 8) * per + b / 8;
 }
 
-template <int LAYOUT>
+// WIDE: He spans 4 GiB or more (64-bit element offsets); otherwise 32-bit byte offsets from He.
+template <int LAYOUT, bool WIDE>
 __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ seg,
                                                   const uint32_t* __restrict__ contrib,
                                                   const int32_t* __restrict__ slot_of_pos,
@@ -201,23 +223,42 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
     const int64_t dst = ncbase ? ncbase[s] + (int64_t)c * ncstride[s] + (c * (c + 1)) / 2 + r : 9 * s + q;
     double v = Ax[dst];
     const int32_t b = seg[p], e_ = seg[p + 1];
-    // a transposed contribution adds entry (c, r) of the element's block
-    auto value = [&](int32_t k) {
-        const uint32_t w = staged ? lc[k - cb] : contrib[k];
-        const int64_t el = w >> 5;
-        const int pair = (w >> 1) & 15;
-        return (w & 1u) ? he_entry<LAYOUT>(He, el, pair, c, r) : he_entry<LAYOUT>(He, el, pair, r, c);
+    const AsmLane lo = asm_lane<LAYOUT>(r, c);
+    auto load = [&](uint32_t w) -> double {
+        if constexpr (WIDE)
+            return He[(int64_t)(w >> 5) * he_stride<LAYOUT>() + he_local<LAYOUT>(w, lo)];
+        else
+            return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(He) +
+                                                    ((w >> 5) * he_stride<LAYOUT>() + he_local<LAYOUT>(w, lo)) * 8u);
     };
-    int32_t k = b;
-    // four contributions' loads in flight at a time, added in ascending element order
-    for (; k + 4 <= e_; k += 4) {
-        const double x0 = value(k), x1 = value(k + 1), x2 = value(k + 2), x3 = value(k + 3);
-        v += x0;
-        v += x1;
-        v += x2;
-        v += x3;
+    // batches of 8 contributions: their loads issue together (past the end a batch re-reads the
+    // last contribution and skips it), the values are added in ascending element order
+    constexpr int B = 8;
+    if (staged) {
+        for (int32_t k = b; k < e_; k += B) {
+            uint32_t w[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) w[i] = lc[min(k + i, e_ - 1) - cb];
+            double x[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) x[i] = load(w[i]);
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+                if (k + i < e_) v += x[i];
+        }
+    } else {
+        for (int32_t k = b; k < e_; k += B) {
+            uint32_t w[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) w[i] = contrib[min(k + i, e_ - 1)];
+            double x[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) x[i] = load(w[i]);
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+                if (k + i < e_) v += x[i];
+        }
     }
-    for (; k < e_; ++k) v += value(k);
     Ax[dst] = v;
 }
 
@@ -375,12 +416,13 @@ void AssemblerDev::assemble(const double* He, int layout, double* Ax, void* stre
     const int32_t* ns = contiguous ? nullptr : ncstride_.as<int32_t>();
     const int64_t nwg = (nslots_ + kAsmSlotsPerWG - 1) / kAsmSlotsPerWG;
     const unsigned grid = (unsigned)(8 * ((nwg + 7) / 8));       // whole rounds of the 8 XCDs
-    if (layout == 0)
-        k_assemble<0><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
-                                                    diag_.as<uint8_t>(), He, nb, ns, Ax, nslots_);
-    else
-        k_assemble<1><<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
-                                                    diag_.as<uint8_t>(), He, nb, ns, Ax, nslots_);
+    const bool wide = (uint64_t)ne_ * (layout == 0 ? 144u : 78u) * 8u >= (1ull << 32);
+    auto go = [&](auto kern) {
+        kern<<<grid, 256, 0, S_(stream)>>>(seg_.as<int32_t>(), contrib_.as<uint32_t>(), pslot_.as<int32_t>(),
+                                           diag_.as<uint8_t>(), He, nb, ns, Ax, nslots_);
+    };
+    if (layout == 0) wide ? go(k_assemble<0, true>) : go(k_assemble<0, false>);
+    else wide ? go(k_assemble<1, true>) : go(k_assemble<1, false>);
     LAUNCH_CHECK();
 }
 
