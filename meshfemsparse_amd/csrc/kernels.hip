@@ -1749,90 +1749,91 @@ __global__ __launch_bounds__(256) void k_blkinv(const SolveTile* __restrict__ ti
     }
 }
 
-// Forward diagonal: y(rows of tile t) = X(rows, 0 .. 64(t+1)) x_b; lanes over rows, the four waves
-// over interleaved columns, partial sums combined in a fixed order.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sfdiag(const SolveTile* __restrict__ tiles,
-                                                const SolveBlock* __restrict__ blocks,
-                                                const T* __restrict__ Xb,
-                                                const T* __restrict__ xall, T* __restrict__ yall,
-                                                int64_t ldx) {
+// Columns c, c + NW, ..., below nc of a lane's row (ld apart), weighted by yv, added to acc in
+// column order: one memory round trip for up to Q columns (past nc the loads repeat column nc - 1
+// and are skipped).
+template <typename T, int Q, int NW>
+__device__ __forceinline__ T gemv_tail(const T* __restrict__ P, int64_t ld, const T* __restrict__ yv, int c,
+                                       int nc, T acc) {
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = P[(int64_t)min(c + NW * q, nc - 1) * ld];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (c + NW * q < nc) acc += v[q] * yv[c + NW * q];
+    return acc;
+}
+
+// A lane's row of a column-major panel P (ld) times yv over the columns wave, wave + NW, ... < nc,
+// summed in column order: batches of 32 independent loads per memory round trip.
+template <typename T, int NW>
+__device__ __forceinline__ T gemv_row(const T* __restrict__ P, int64_t ld, const T* __restrict__ yv, int nc,
+                                      int wave) {
+    T acc = T(0);
+    int c = wave;
+    for (; c + 31 * NW < nc; c += 32 * NW) {
+        T v[32];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) v[q] = P[(int64_t)(c + NW * q) * ld];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) acc += v[q] * yv[c + NW * q];
+    }
+    if (c + 8 * NW < nc) acc = gemv_tail<T, 32, NW>(P, ld, yv, c, nc, acc);
+    else if (c < nc) acc = gemv_tail<T, 8, NW>(P, ld, yv, c, nc, acc);
+    return acc;
+}
+
+// Sum of the NW per-wave partials of row r in wave order (fixed).
+template <typename T, int NW>
+__device__ __forceinline__ T wave_partials(const T (*red)[TS], int r) {
+    T v = red[0][r];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) v += red[w][r];
+    return v;
+}
+
+// Forward diagonal: y(rows of tile t) = X(rows, 0 .. 64(t+1)) x_b; lanes over rows, the NW waves
+// over interleaved columns, partial sums combined in a fixed order.  NW = 16 (1024 threads) for
+// launches of few tiles, where one workgroup's latency is the step's.
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sfdiag(const SolveTile* __restrict__ tiles,
+                                                    const SolveBlock* __restrict__ blocks,
+                                                    const T* __restrict__ Xb,
+                                                    const T* __restrict__ xall, T* __restrict__ yall,
+                                                    int64_t ldx) {
     const SolveTile st = tiles[blockIdx.x];
     const SolveBlock b = blocks[st.blk];
     const T* x = xall + blockIdx.y * ldx + b.x_off;
     T* y = yall + blockIdx.y * ldx + b.x_off;
     __shared__ T z[1024];
-    __shared__ T red[4][TS];
+    __shared__ T red[NW][TS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = TS * st.t, nr = min(TS, b.nb - r0), nc = min(b.nb, r0 + TS);
-    for (int c = tid; c < nc; c += 256) z[c] = x[c];
+    for (int c = tid; c < nc; c += 64 * NW) z[c] = x[c];
     __syncthreads();
-    const T* Xr = Xb + b.xb_off + r0 + lane;
-    T acc = T(0);
-    if (lane < nr) {
-        // batches of 32 independent loads (columns wave, wave + 4, ...) per memory round trip;
-        // the products are summed in column order as in a plain loop
-        int c = wave;
-        for (; c + 124 < nc; c += 128) {
-            T v[32];
-#pragma unroll
-            for (int q = 0; q < 32; ++q) v[q] = Xr[(int64_t)(c + 4 * q) * b.nb];
-#pragma unroll
-            for (int q = 0; q < 32; ++q) acc += v[q] * z[c + 4 * q];
-        }
-        for (; c + 28 < nc; c += 32) {
-            T v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = Xr[(int64_t)(c + 4 * q) * b.nb];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc += v[q] * z[c + 4 * q];
-        }
-        for (; c < nc; c += 4) acc += Xr[(int64_t)c * b.nb] * z[c];
-    }
-    red[wave][lane] = acc;
+    red[wave][lane] = lane < nr ? gemv_row<T, NW>(Xb + b.xb_off + r0 + lane, b.nb, z, nc, wave) : T(0);
     __syncthreads();
-    if (tid < nr) y[r0 + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    if (tid < nr) y[r0 + tid] = wave_partials<T, NW>(red, tid);
 }
 
 // Forward update: rows [r0, r0 + nr) below block b: x -= L(rows, b) y_b (rows of the supernode)
 // or u += L(rows, b) y_b (structure rows, the update vector passed to the parent).
-template <typename T>
-__global__ __launch_bounds__(256) void k_sfupd(const SolveRowTask* __restrict__ tasks,
-                                               const T* __restrict__ L,
-                                               const T* __restrict__ yall, T* __restrict__ xall,
-                                               int64_t ldx, T* __restrict__ uall, int64_t ldu) {
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sfupd(const SolveRowTask* __restrict__ tasks,
+                                                   const T* __restrict__ L,
+                                                   const T* __restrict__ yall, T* __restrict__ xall,
+                                                   int64_t ldx, T* __restrict__ uall, int64_t ldu) {
     const SolveRowTask t = tasks[blockIdx.x];
     __shared__ T yv[1024];
-    __shared__ T red[4][TS];
+    __shared__ T red[NW][TS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const T* y = yall + blockIdx.y * ldx + t.v_off;
-    for (int c = tid; c < t.nc; c += 256) yv[c] = y[c];
+    for (int c = tid; c < t.nc; c += 64 * NW) yv[c] = y[c];
     __syncthreads();
-    const T* P = L + t.l_off + lane;
-    T acc = T(0);
-    if (lane < t.nr) {
-        // 32 independent loads per memory round trip, products summed in column order
-        int c = wave;
-        for (; c + 124 < t.nc; c += 128) {
-            T v[32];
-#pragma unroll
-            for (int q = 0; q < 32; ++q) v[q] = P[(int64_t)(c + 4 * q) * t.ld];
-#pragma unroll
-            for (int q = 0; q < 32; ++q) acc += v[q] * yv[c + 4 * q];
-        }
-        for (; c + 28 < t.nc; c += 32) {
-            T v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = P[(int64_t)(c + 4 * q) * t.ld];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc += v[q] * yv[c + 4 * q];
-        }
-        for (; c < t.nc; c += 4) acc += P[(int64_t)c * t.ld] * yv[c];
-    }
-    red[wave][lane] = acc;
+    red[wave][lane] = lane < t.nr ? gemv_row<T, NW>(L + t.l_off + lane, t.ld, yv, t.nc, wave) : T(0);
     __syncthreads();
     if (tid < t.nr) {
-        const T v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        const T v = wave_partials<T, NW>(red, tid);
         if (t.mode == 0) xall[blockIdx.y * ldx + t.d_off + tid] -= v;
         else uall[blockIdx.y * ldu + t.d_off + tid] += v;
     }
@@ -1841,26 +1842,18 @@ __global__ __launch_bounds__(256) void k_sfupd(const SolveRowTask* __restrict__ 
 // Backward partial sums: part[d_off + c] = sum over rows [r0, r0 + nr) of L(row, c) x(row) for the
 // task's nc (<= 16) columns; one wave per column (lanes over rows), x held in registers.
 constexpr int SOLVE_RC = 1024;   // maximum rows per backward partial task
-template <typename T>
-__global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__ tasks,
-                                                const T* __restrict__ L,
-                                                const T* __restrict__ xall, int64_t ldx,
-                                                const T* __restrict__ uall, int64_t ldu,
-                                                T* __restrict__ partall, int64_t ldp) {
-    const SolveRowTask t = tasks[blockIdx.x];
+template <typename T, int MR>
+__device__ __forceinline__ void sbpart_body(const SolveRowTask& t, const T* __restrict__ L,
+                                            const T* __restrict__ src, T* __restrict__ part) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const T* src = (t.mode == 0 ? xall + blockIdx.y * ldx : uall + blockIdx.y * ldu) + t.v_off;
-    T xv[SOLVE_RC / 64];
+    T xv[MR];
 #pragma unroll
-    for (int m = 0; m < SOLVE_RC / 64; ++m) {
+    for (int m = 0; m < MR; ++m) {
         const int r = lane + 64 * m;
         xv[m] = r < t.nr ? src[r] : T(0);
     }
-    T* part = partall + blockIdx.y * ldp + t.d_off;
-    if (t.nc < 1) return;
     // a wave's columns (at most 4 of the task's <= 16) are loaded in one round trip; rows beyond
     // nr read row 0 of the column (in bounds) and are zeroed by the product with xv = 0
-    constexpr int MR = SOLVE_RC / 64;
     T v[4][MR];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -1883,57 +1876,98 @@ __global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__
     }
 }
 
-// Backward diagonal: x(columns of tile t) = X(rows >= 64t, columns)^T z, z = y_b - sum of the
-// block's partial sums (fixed order); one wave per column, lanes over rows.
+// Backward partial sums: part[d_off + c] = sum over rows [r0, r0 + nr) of L(row, c) x(row) for the
+// task's nc (<= 16) columns; one wave per column (lanes over rows), x held in registers.  The
+// register tiles cover the task's rows in 2, 4, 8 or 16 chunks of 64 (uniform per workgroup).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sbdiag(const SolveTile* __restrict__ tiles,
-                                                const SolveBlock* __restrict__ blocks,
-                                                const T* __restrict__ Xb,
-                                                const T* __restrict__ yall, T* __restrict__ xall,
-                                                int64_t ldx, const T* __restrict__ partall, int64_t ldp,
-                                                int sb) {
+__global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__ tasks,
+                                                const T* __restrict__ L,
+                                                const T* __restrict__ xall, int64_t ldx,
+                                                const T* __restrict__ uall, int64_t ldu,
+                                                T* __restrict__ partall, int64_t ldp) {
+    const SolveRowTask t = tasks[blockIdx.x];
+    if (t.nc < 1) return;
+    const T* src = (t.mode == 0 ? xall + blockIdx.y * ldx : uall + blockIdx.y * ldu) + t.v_off;
+    T* part = partall + blockIdx.y * ldp + t.d_off;
+    const int mr = (t.nr + 63) / 64;
+    if (mr <= 2) sbpart_body<T, 2>(t, L, src, part);
+    else if (mr <= 4) sbpart_body<T, 4>(t, L, src, part);
+    else if (mr <= 8) sbpart_body<T, 8>(t, L, src, part);
+    else sbpart_body<T, SOLVE_RC / 64>(t, L, src, part);
+}
+
+// Column passes of the backward diagonal: a wave's CP columns c + NW j per memory round trip (CP
+// = 4, or 2 for 16 row chunks in 1024-thread workgroups, whose waves hold at most 128 registers),
+// the rows in MZ chunks of 64; rows beyond nz read row 0 (in bounds), weighted by 0.
+template <typename T, int MZ, int NW>
+__device__ __forceinline__ void sbdiag_cols(const T* __restrict__ X, int nb, int c0, int ncol, int nz,
+                                            const T* __restrict__ z, T* __restrict__ x) {
+    constexpr int CP = (NW == 16 && MZ == 16) ? 2 : 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = wave; c < ncol; c += CP * NW) {
+        T v[CP][MZ];
+#pragma unroll
+        for (int j = 0; j < CP; ++j) {
+            const T* Xc = X + (int64_t)(c0 + min(c + NW * j, ncol - 1)) * nb + c0;
+#pragma unroll
+            for (int m = 0; m < MZ; ++m) v[j][m] = Xc[lane + 64 * m < nz ? lane + 64 * m : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < CP; ++j) {
+            T acc = T(0);
+#pragma unroll
+            for (int m = 0; m < MZ; ++m) acc += v[j][m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
+            acc = wave_sum(acc);
+            if (lane == 0 && c + NW * j < ncol) x[c0 + c + NW * j] = acc;
+        }
+    }
+}
+
+// Backward diagonal: x(columns of tile t) = X(rows >= 64t, columns)^T z, z = y_b - sum of the
+// block's partial sums (fixed order); one wave per column, lanes over rows.  NW = 16 (1024
+// threads: one column pass) for launches of few tiles.
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void k_sbdiag(const SolveTile* __restrict__ tiles,
+                                                    const SolveBlock* __restrict__ blocks,
+                                                    const T* __restrict__ Xb,
+                                                    const T* __restrict__ yall, T* __restrict__ xall,
+                                                    int64_t ldx, const T* __restrict__ partall, int64_t ldp,
+                                                    int sb) {
     const SolveTile st = tiles[blockIdx.x];
     const SolveBlock b = blocks[st.blk];
     const T* y = yall + blockIdx.y * ldx + b.x_off;
     T* x = xall + blockIdx.y * ldx + b.x_off;
     const T* part = partall + blockIdx.y * ldp;
     __shared__ T z[1024];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int c0 = TS * st.t, ncol = min(TS, b.nb - c0), nz = b.nb - c0;
-    for (int r = tid; r < nz; r += 256) {
-        // the partial-sum slots are subtracted in slot order; up to 8 of them per memory round trip
+    for (int r = tid; r < nz; r += 64 * NW) {
+        // the partial-sum slots are subtracted in slot order, 8 per memory round trip (past the
+        // last slot the loads repeat it and are skipped)
         T v = y[c0 + r];
-        int p = b.part_begin;
-        for (; p + 8 <= b.part_end; p += 8) {
+        for (int p = b.part_begin; p < b.part_end; p += 8) {
             T w[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) w[u] = part[(int64_t)(p + u) * sb + c0 + r];
+            for (int u = 0; u < 8; ++u) w[u] = part[(int64_t)min(p + u, b.part_end - 1) * sb + c0 + r];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v -= w[u];
+            for (int u = 0; u < 8; ++u)
+                if (p + u < b.part_end) v -= w[u];
         }
-        for (; p < b.part_end; ++p) v -= part[(int64_t)p * sb + c0 + r];
         z[r] = v;
     }
     __syncthreads();
-    // four columns per memory round trip; rows beyond nz read row 0 (in bounds), weighted by 0
-    for (int c = wave; c < ncol; c += 16) {
-        T v[4][16];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const T* Xc = Xb + b.xb_off + (int64_t)(c0 + min(c + 4 * j, ncol - 1)) * b.nb + c0;
-#pragma unroll
-            for (int m = 0; m < 16; ++m) v[j][m] = Xc[lane + 64 * m < nz ? lane + 64 * m : 0];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            T acc = T(0);
-#pragma unroll
-            for (int m = 0; m < 16; ++m) acc += v[j][m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
-            acc = wave_sum(acc);
-            if (lane == 0 && c + 4 * j < ncol) x[c0 + c + 4 * j] = acc;
-        }
-    }
+    const int mz = (nz + 63) / 64;
+    if (mz <= 2) sbdiag_cols<T, 2, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
+    else if (mz <= 4) sbdiag_cols<T, 4, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
+    else if (mz <= 8) sbdiag_cols<T, 8, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
+    else sbdiag_cols<T, 16, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
 }
+
+// Solve launches of fewer workgroups than this (per right-hand side) run the 16-wave forms of
+// the diagonal and forward-update kernels: with at most a few workgroups per CU the step's time is
+// one workgroup's latency, which more waves per workgroup shorten.  The choice depends on the
+// plan only, so results do not depend on the number of right-hand sides.
+constexpr int64_t kSolveWideBelow = 512;
 
 // factor-order vector conversions around a scaled (or lower-precision) solve: y = (T)(x * s)
 template <typename T>
@@ -2556,7 +2590,10 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
                 prof.begin(KernelProfile::FDIAG, s);
-                k_sfdiag<T><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
+                if (d1 - d0 < kSolveWideBelow)
+                    k_sfdiag<T, 16><<<g, 1024, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
+                else
+                    k_sfdiag<T, 4><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -2564,7 +2601,10 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
             if (u1 > u0) {
                 dim3 g((unsigned)(u1 - u0), (unsigned)nrhs);
                 prof.begin(KernelProfile::FGEMV, s);
-                k_sfupd<T><<<g, 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
+                if (u1 - u0 < kSolveWideBelow)
+                    k_sfupd<T, 16><<<g, 1024, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
+                else
+                    k_sfupd<T, 4><<<g, 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -2592,8 +2632,12 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
                 prof.begin(KernelProfile::BDIAG, s);
-                k_sbdiag<T><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
-                                              S.solve_block);
+                if (d1 - d0 < kSolveWideBelow)
+                    k_sbdiag<T, 16><<<g, 1024, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
+                                                       S.solve_block);
+                else
+                    k_sbdiag<T, 4><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
+                                                     S.solve_block);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
