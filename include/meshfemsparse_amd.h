@@ -338,7 +338,8 @@ int mfs_last_timings(mfs_factorizer f, double* ms4);
  * update GEMM tiles, Schur-update GEMM tiles, forward gather, forward diagonal, forward update
  * GEMV, backward partial GEMV^T, backward diagonal, SpMV, permutation maps, assembly, solve-block
  * inverses, backward gather, precision conversions, refinement vector kernels, fused level
- * kernel, fused diagonal-tile + TRSM steps.  ms[22] and counts[22].                          */
+ * kernel, fused diagonal-tile + TRSM steps.  ms[22] and counts[22].  mfs_kernel_profile waits  *
+ * for profiled launches still running and folds them in.                                      */
 int mfs_kernel_profiling(mfs_factorizer f, int32_t enable, int32_t reset);
 int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts);
 /* Executed tile flops of the profiled launches per class (22; the GEMM classes and the fused   */

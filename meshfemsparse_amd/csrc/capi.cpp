@@ -657,6 +657,7 @@ int mfs_kernel_profile(mfs_factorizer f, double* ms, int64_t* counts) {
     MFS_TRY
     CHECK_PTR(f);
     auto& P = f->dev->prof;
+    P.collect();                       // launches not yet folded in (plain solves), after they end
     for (int c = 0; c < mfs::KernelProfile::NCLASS; ++c) {
         if (ms) ms[c] = P.ms[c];
         if (counts) counts[c] = P.count[c];

@@ -1819,11 +1819,13 @@ __global__ __launch_bounds__(64 * NW) void k_sfdiag(const SolveTile* __restrict_
 // Forward update: rows [r0, r0 + nr) below block b: x -= L(rows, b) y_b (rows of the supernode)
 // or u += L(rows, b) y_b (structure rows, the update vector passed to the parent).
 template <typename T, int NW>
-__global__ __launch_bounds__(64 * NW) void k_sfupd(const SolveRowTask* __restrict__ tasks,
+__global__ __launch_bounds__(64 * NW) void k_sfupd(const SolveRowTask* __restrict__ tasks, int64_t ntasks,
                                                    const T* __restrict__ L,
                                                    const T* __restrict__ yall, T* __restrict__ xall,
                                                    int64_t ldx, T* __restrict__ uall, int64_t ldu) {
-    const SolveRowTask t = tasks[blockIdx.x];
+    const int64_t ti = xcd_chunk(blockIdx.x, gridDim.x);
+    if (ti >= ntasks) return;                  // the whole workgroup leaves together
+    const SolveRowTask t = tasks[ti];
     __shared__ T yv[1024];
     __shared__ T red[NW][TS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1839,40 +1841,132 @@ __global__ __launch_bounds__(64 * NW) void k_sfupd(const SolveRowTask* __restric
     }
 }
 
-// Backward partial sums: part[d_off + c] = sum over rows [r0, r0 + nr) of L(row, c) x(row) for the
-// task's nc (<= 16) columns; one wave per column (lanes over rows), x held in registers.
-constexpr int SOLVE_RC = 1024;   // maximum rows per backward partial task
-template <typename T, int MR>
-__device__ __forceinline__ void sbpart_body(const SolveRowTask& t, const T* __restrict__ L,
-                                            const T* __restrict__ src, T* __restrict__ part) {
+// Quarter-tile forms for launches of few tiles: the 64-row tile (or, backward, 64-column tile)
+// of a task is split over 4 workgroups of 16 rows (columns), so a step's bytes are read by 4 times
+// as many CUs.  Forward: a lane works on row lane % 16 of its quarter and column stream
+// 4 wave + lane / 16 (columns stream, stream + 64, ...); the four streams of a wave are combined
+// by shuffles and the waves through LDS, both in a fixed order.
+template <typename T>
+__device__ __forceinline__ T gemv_row_q(const T* __restrict__ P, int64_t ld, const T* __restrict__ yv, int nc,
+                                        int sid) {
+    // nc <= 1024: at most 16 columns per stream
+    if (sid + 8 * 64 < nc) return gemv_tail<T, 16, 64>(P, ld, yv, sid, nc, T(0));
+    if (sid < nc) return gemv_tail<T, 8, 64>(P, ld, yv, sid, nc, T(0));
+    return T(0);
+}
+
+// Row sums of a 16-row quarter: lanes r, r + 16, r + 32, r + 48 of each wave, then the 16 waves.
+template <typename T>
+__device__ __forceinline__ bool quarter_sum(T acc, T (*red)[16], T& out) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T a0 = __shfl(acc, lane & 15, 64), a1 = __shfl(acc, (lane & 15) + 16, 64);
+    const T a2 = __shfl(acc, (lane & 15) + 32, 64), a3 = __shfl(acc, (lane & 15) + 48, 64);
+    if (lane < 16) red[wave][lane] = ((a0 + a1) + a2) + a3;
+    __syncthreads();
+    if (tid >= 16) return false;
+    T v = red[0][tid];
+#pragma unroll
+    for (int w = 1; w < 16; ++w) v += red[w][tid];
+    out = v;
+    return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sfdiag_q(const SolveTile* __restrict__ tiles,
+                                                   const SolveBlock* __restrict__ blocks,
+                                                   const T* __restrict__ Xb,
+                                                   const T* __restrict__ xall, T* __restrict__ yall,
+                                                   int64_t ldx) {
+    const SolveTile st = tiles[blockIdx.x >> 2];
+    const SolveBlock b = blocks[st.blk];
+    const T* x = xall + blockIdx.y * ldx + b.x_off;
+    T* y = yall + blockIdx.y * ldx + b.x_off;
+    __shared__ T z[1024];
+    __shared__ T red[16][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = TS * st.t + 16 * (blockIdx.x & 3), nr = min(16, b.nb - r0), nc = min(b.nb, TS * (st.t + 1));
+    if (nr <= 0) return;                       // the whole workgroup leaves together
+    for (int c = tid; c < nc; c += 1024) z[c] = x[c];
+    __syncthreads();
+    const int r = lane & 15;
+    const T acc = r < nr ? gemv_row_q<T>(Xb + b.xb_off + r0 + r, b.nb, z, nc, 4 * wave + (lane >> 4)) : T(0);
+    T v;
+    if (quarter_sum<T>(acc, red, v) && tid < nr) y[r0 + tid] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sfupd_q(const SolveRowTask* __restrict__ tasks, int64_t ntasks,
+                                                  const T* __restrict__ L,
+                                                  const T* __restrict__ yall, T* __restrict__ xall,
+                                                  int64_t ldx, T* __restrict__ uall, int64_t ldu) {
+    const int64_t qi = xcd_chunk(blockIdx.x, gridDim.x);
+    if (qi >= 4 * ntasks) return;              // the whole workgroup leaves together
+    const SolveRowTask t = tasks[qi >> 2];
+    const int q0 = 16 * (int)(qi & 3), nr = min(16, t.nr - q0);
+    if (nr <= 0) return;                       // the whole workgroup leaves together
+    __shared__ T yv[1024];
+    __shared__ T red[16][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T* y = yall + blockIdx.y * ldx + t.v_off;
+    for (int c = tid; c < t.nc; c += 1024) yv[c] = y[c];
+    __syncthreads();
+    const int r = lane & 15;
+    const T acc = r < nr ? gemv_row_q<T>(L + t.l_off + q0 + r, t.ld, yv, t.nc, 4 * wave + (lane >> 4)) : T(0);
+    T v;
+    if (quarter_sum<T>(acc, red, v) && tid < nr) {
+        if (t.mode == 0) xall[blockIdx.y * ldx + t.d_off + q0 + tid] -= v;
+        else uall[blockIdx.y * ldu + t.d_off + q0 + tid] += v;
+    }
+}
+
+constexpr int SOLVE_RC = 1024;   // maximum rows per backward partial task
+// Rows [64 M0, 64 (M0 + MR)) of a task: a wave's columns (at most 4 of the task's <= 16) are
+// loaded in one round trip and their products added to acc[j] in row order; rows beyond nr read
+// row 0 of the column (in bounds) and are zeroed by the product with x = 0.
+template <typename T, int MR>
+__device__ __forceinline__ void sbpart_rows(const SolveRowTask& t, const T* __restrict__ L,
+                                            const T* __restrict__ src, T (&acc)[4], int M0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     T xv[MR];
 #pragma unroll
     for (int m = 0; m < MR; ++m) {
-        const int r = lane + 64 * m;
+        const int r = lane + 64 * (M0 + m);
         xv[m] = r < t.nr ? src[r] : T(0);
     }
-    // a wave's columns (at most 4 of the task's <= 16) are loaded in one round trip; rows beyond
-    // nr read row 0 of the column (in bounds) and are zeroed by the product with xv = 0
     T v[4][MR];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int c = wave + 4 * j;
-        const T* P = L + t.l_off + (int64_t)min(c, t.nc - 1) * t.ld;
+        const T* P = L + t.l_off + (int64_t)min(wave + 4 * j, t.nc - 1) * t.ld;
 #pragma unroll
         for (int m = 0; m < MR; ++m) {
-            const int r = lane + 64 * m;
+            const int r = lane + 64 * (M0 + m);
             v[j][m] = P[r < t.nr ? r : 0];
         }
     }
 #pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int m = 0; m < MR; ++m) acc[j] += v[j][m] * xv[m];
+}
+
+// MR chunks of 64 rows (16 as two halves of 8, which keeps the kernel's registers, and so its
+// occupancy, at the 8-chunk size)
+template <typename T, int MR>
+__device__ __forceinline__ void sbpart_body(const SolveRowTask& t, const T* __restrict__ L,
+                                            const T* __restrict__ src, T* __restrict__ part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    if constexpr (MR == 16) {
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) sbpart_rows<T, 8>(t, L, src, acc, 8 * h);
+    } else {
+        sbpart_rows<T, MR>(t, L, src, acc, 0);
+    }
+#pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int c = wave + 4 * j;
-        T acc = T(0);
-#pragma unroll
-        for (int m = 0; m < MR; ++m) acc += v[j][m] * xv[m];
-        acc = wave_sum(acc);
-        if (lane == 0 && c < t.nc) part[c] = acc;
+        const T a = wave_sum(acc[j]);
+        if (lane == 0 && c < t.nc) part[c] = a;
     }
 }
 
@@ -1880,12 +1974,14 @@ __device__ __forceinline__ void sbpart_body(const SolveRowTask& t, const T* __re
 // task's nc (<= 16) columns; one wave per column (lanes over rows), x held in registers.  The
 // register tiles cover the task's rows in 2, 4, 8 or 16 chunks of 64 (uniform per workgroup).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__ tasks,
+__global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__ tasks, int64_t ntasks,
                                                 const T* __restrict__ L,
                                                 const T* __restrict__ xall, int64_t ldx,
                                                 const T* __restrict__ uall, int64_t ldu,
                                                 T* __restrict__ partall, int64_t ldp) {
-    const SolveRowTask t = tasks[blockIdx.x];
+    const int64_t ti = xcd_chunk(blockIdx.x, gridDim.x);
+    if (ti >= ntasks) return;                  // the whole workgroup leaves together
+    const SolveRowTask t = tasks[ti];
     if (t.nc < 1) return;
     const T* src = (t.mode == 0 ? xall + blockIdx.y * ldx : uall + blockIdx.y * ldu) + t.v_off;
     T* part = partall + blockIdx.y * ldp + t.d_off;
@@ -1897,12 +1993,12 @@ __global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__
 }
 
 // Column passes of the backward diagonal: a wave's CP columns c + NW j per memory round trip (CP
-// = 4, or 2 for 16 row chunks in 1024-thread workgroups, whose waves hold at most 128 registers),
+// = 4, or 2 for 16 row chunks, which keeps the kernel's registers at the 8-chunk size),
 // the rows in MZ chunks of 64; rows beyond nz read row 0 (in bounds), weighted by 0.
 template <typename T, int MZ, int NW>
 __device__ __forceinline__ void sbdiag_cols(const T* __restrict__ X, int nb, int c0, int ncol, int nz,
                                             const T* __restrict__ z, T* __restrict__ x) {
-    constexpr int CP = (NW == 16 && MZ == 16) ? 2 : 4;
+    constexpr int CP = MZ == 16 ? 2 : 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int c = wave; c < ncol; c += CP * NW) {
         T v[CP][MZ];
@@ -1961,6 +2057,84 @@ __global__ __launch_bounds__(64 * NW) void k_sbdiag(const SolveTile* __restrict_
     else if (mz <= 4) sbdiag_cols<T, 4, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
     else if (mz <= 8) sbdiag_cols<T, 8, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
     else sbdiag_cols<T, 16, NW>(Xb + b.xb_off, b.nb, c0, ncol, nz, z, x);
+}
+
+// Backward diagonal, quarter tiles: 16 of the tile's 64 columns per workgroup, one column per
+// wave (all its rows in one round trip); z as in k_sbdiag.
+template <typename T, int MZ>
+__device__ __forceinline__ void sbdiag_col(const T* __restrict__ X, int nb, int c0, int c, int nz,
+                                           const T* __restrict__ z, T* __restrict__ x) {
+    const int lane = threadIdx.x & 63;
+    const T* Xc = X + (int64_t)(c0 + c) * nb + c0;
+    T v[MZ];
+#pragma unroll
+    for (int m = 0; m < MZ; ++m) v[m] = Xc[lane + 64 * m < nz ? lane + 64 * m : 0];
+    T acc = T(0);
+#pragma unroll
+    for (int m = 0; m < MZ; ++m) acc += v[m] * ((lane + 64 * m < nz) ? z[lane + 64 * m] : T(0));
+    acc = wave_sum(acc);
+    if (lane == 0) x[c0 + c] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sbdiag_q(const SolveTile* __restrict__ tiles,
+                                                   const SolveBlock* __restrict__ blocks,
+                                                   const T* __restrict__ Xb,
+                                                   const T* __restrict__ yall, T* __restrict__ xall,
+                                                   int64_t ldx, const T* __restrict__ partall, int64_t ldp,
+                                                   int sb) {
+    const SolveTile st = tiles[blockIdx.x >> 2];
+    const SolveBlock b = blocks[st.blk];
+    const int c0 = TS * st.t, nz = b.nb - c0, cq = 16 * (blockIdx.x & 3);
+    const int ncol = min(16, min(TS, b.nb - c0) - cq);
+    if (ncol <= 0) return;                     // the whole workgroup leaves together
+    const T* y = yall + blockIdx.y * ldx + b.x_off;
+    T* x = xall + blockIdx.y * ldx + b.x_off;
+    const T* part = partall + blockIdx.y * ldp;
+    __shared__ T z[1024];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int r = tid; r < nz; r += 1024) {
+        T v = y[c0 + r];
+        for (int p = b.part_begin; p < b.part_end; p += 8) {
+            T w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = part[(int64_t)min(p + u, b.part_end - 1) * sb + c0 + r];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (p + u < b.part_end) v -= w[u];
+        }
+        z[r] = v;
+    }
+    __syncthreads();
+    if (wave >= ncol) return;
+    const int mz = (nz + 63) / 64, c = cq + wave;
+    if (mz <= 2) sbdiag_col<T, 2>(Xb + b.xb_off, b.nb, c0, c, nz, z, x);
+    else if (mz <= 4) sbdiag_col<T, 4>(Xb + b.xb_off, b.nb, c0, c, nz, z, x);
+    else if (mz <= 8) sbdiag_col<T, 8>(Xb + b.xb_off, b.nb, c0, c, nz, z, x);
+    else sbdiag_col<T, 16>(Xb + b.xb_off, b.nb, c0, c, nz, z, x);
+}
+
+// Grid for an XCD-contiguous task map (xcd_chunk): n workgroups rounded up to whole rounds of the
+// 8 XCDs; the kernels skip chunks past their task count.
+static unsigned xcd_grid(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
+
+// Block-inverse entries a range of diagonal-solve tiles reads: forward, rows of tile t times the
+// columns up to its end; backward, columns of tile t times the rows from its start.
+static double diag_tile_entries(const std::vector<SolveTile>& v, const std::vector<SolveBlock>& blocks, int64_t b,
+                                int64_t e, bool forward) {
+    double n = 0.0;
+    for (int64_t q = b; q < e; ++q) {
+        const int nb = blocks[v[q].blk].nb, t0 = 64 * v[q].t, w = std::min(64, nb - t0);
+        n += (double)w * (forward ? std::min(nb, t0 + 64) : nb - t0);
+    }
+    return n;
+}
+
+// Factor entries a range of solve row tasks reads (nr x nc each).
+static double row_task_bytes(const std::vector<SolveRowTask>& v, int64_t b, int64_t e) {
+    double n = 0.0;
+    for (int64_t q = b; q < e; ++q) n += (double)v[q].nr * v[q].nc;
+    return n;
 }
 
 // Solve launches of fewer workgroups than this (per right-hand side) run the 16-wave forms of
@@ -2589,9 +2763,10 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
             auto [d0, d1] = P.fdiag_steps[t];
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
-                prof.begin(KernelProfile::FDIAG, s);
+                prof.begin(KernelProfile::FDIAG, s, 0.0, 0,
+                           prof.on ? diag_tile_entries(S.fdiag_tiles, S.sblocks, d0, d1, true) * sizeof(T) * nrhs : 0.0);
                 if (d1 - d0 < kSolveWideBelow)
-                    k_sfdiag<T, 16><<<g, 1024, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
+                    k_sfdiag_q<T><<<dim3(4 * g.x, g.y), 1024, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
                 else
                     k_sfdiag<T, 4><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
                 LAUNCH_CHECK();
@@ -2599,12 +2774,14 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
             }
             auto [u0, u1] = P.fupd_steps[t];
             if (u1 > u0) {
-                dim3 g((unsigned)(u1 - u0), (unsigned)nrhs);
-                prof.begin(KernelProfile::FGEMV, s);
-                if (u1 - u0 < kSolveWideBelow)
-                    k_sfupd<T, 16><<<g, 1024, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
+                const int64_t nt = u1 - u0;
+                prof.begin(KernelProfile::FGEMV, s, 0.0, 0, prof.on ? row_task_bytes(S.fupd_tasks, u0, u1) * sizeof(T) * nrhs : 0.0);
+                if (nt < kSolveWideBelow)
+                    k_sfupd_q<T><<<dim3(xcd_grid(4 * nt), (unsigned)nrhs), 1024, 0, s>>>(
+                        fupd_.as<SolveRowTask>() + u0, nt, L, Y, xp, ldx, u, ldu);
                 else
-                    k_sfupd<T, 4><<<g, 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, L, Y, xp, ldx, u, ldu);
+                    k_sfupd<T, 4><<<dim3(xcd_grid(nt), (unsigned)nrhs), 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, nt,
+                                                                                     L, Y, xp, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -2622,19 +2799,20 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
         for (size_t t = 0; t < P.bdiag_steps.size(); ++t) {
             auto [p0, p1] = P.bpart_steps[t];
             if (p1 > p0) {
-                dim3 g((unsigned)(p1 - p0), (unsigned)nrhs);
-                prof.begin(KernelProfile::BGEMV, s);
-                k_sbpart<T><<<g, 256, 0, s>>>(bprt_.as<SolveRowTask>() + p0, L, xp, ldx, u, ldu, part, ldp);
+                prof.begin(KernelProfile::BGEMV, s, 0.0, 0, prof.on ? row_task_bytes(S.bpart_tasks, p0, p1) * sizeof(T) * nrhs : 0.0);
+                k_sbpart<T><<<dim3(xcd_grid(p1 - p0), (unsigned)nrhs), 256, 0, s>>>(bprt_.as<SolveRowTask>() + p0, p1 - p0,
+                                                                                    L, xp, ldx, u, ldu, part, ldp);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
             auto [d0, d1] = P.bdiag_steps[t];
             if (d1 > d0) {
                 dim3 g((unsigned)(d1 - d0), (unsigned)nrhs);
-                prof.begin(KernelProfile::BDIAG, s);
+                prof.begin(KernelProfile::BDIAG, s, 0.0, 0,
+                           prof.on ? diag_tile_entries(S.bdiag_tiles, S.sblocks, d0, d1, false) * sizeof(T) * nrhs : 0.0);
                 if (d1 - d0 < kSolveWideBelow)
-                    k_sbdiag<T, 16><<<g, 1024, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
-                                                       S.solve_block);
+                    k_sbdiag_q<T><<<dim3(4 * g.x, g.y), 1024, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx,
+                                                                     part, ldp, S.solve_block);
                 else
                     k_sbdiag<T, 4><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
                                                      S.solve_block);
