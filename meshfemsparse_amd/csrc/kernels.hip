@@ -573,12 +573,46 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
 //            step b-1) and S += L(:, b-1) X(b-1, :) for the block rows b .. 7;
 //   phase 2  wave 0: lane c < b0 turns S(b0.., c) into X(b0.., c) = -X_bb S; lane 0 stores
 //            L_bb and X_bb; waves 1-3: the rank-8 update from block b of block column b+1.
-// A barrier ends each phase; the 8x8 blocks of the updates map one block per wave pass.  A
+// A barrier ends each phase; the updates of waves 1-3 run as 16x16 FP64 MFMA tiles (K = 8) masked
+// to their regions, round robin over the three waves.  A
 // non-positive or NaN pivot records the first failing column and continues with 1 so no NaN
 // spreads; the host turns it into a pivot failure.
 // ------------------------------------------------------------------------------------------
 constexpr int TS = 64;   // diagonal tile size (scalars)
 constexpr int PB = 8;    // column step of the tile factorization
+
+// One 16x16 tile of a rank-8 update inside the diagonal tile, on FP64 MFMA (16x16x4, two steps):
+//   INV = false: A(r, c) -= sum_k L(r, k0 + k) L(c, k0 + k)   (C = a, B(k, c) = a[k0 + k][c])
+//   INV = true:  S(r, c) += sum_k L(r, k0 + k) X(k0 + k, c)   (C = x, B(k, c) = x[c][k0 + k])
+// for the rows R0 .. R0+15 and columns C0 .. C0+15 with r >= rlo, clo <= c < chi and, if `lower`,
+// r >= c; other entries of the tile are neither read nor written.  Operand layouts of
+// v_mfma_f64_16x16x4f64: A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in lane
+// j + 16 (i % 4), register i / 4.
+template <bool INV>
+__device__ __forceinline__ void tile_rank8(double (*__restrict__ a)[TS + 1], double (*__restrict__ x)[TS + 1],
+                                           int k0, int R0, int C0, int rlo, int clo, int chi, bool lower) {
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+    double (*cm)[TS + 1] = INV ? x : a;
+    const int c = C0 + i;
+    d4 acc;
+    bool ok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = R0 + 4 * q + kq;
+        ok[q] = r >= rlo && c >= clo && c < chi && (!lower || r >= c);
+        acc[q] = ok[q] ? cm[c][r] : 0.0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = k0 + 4 * h + kq;
+        const double av = a[k][R0 + i];
+        const double bv = INV ? x[C0 + i][k] : a[k][C0 + i];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(INV ? av : -av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (ok[q]) cm[c][R0 + 4 * q + kq] = acc[q];
+}
 
 // One tile (256 threads, a and x in LDS).  On return the factor is stored to the panel, the
 // inverse to inv[t.slot], and x still holds X in LDS (all threads past the final barrier).
@@ -598,7 +632,7 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
                                            int count = 1, int* __restrict__ s_flag = nullptr,
                                            T (*__restrict__ rt)[TS + 1] = nullptr, const T* __restrict__ rsrc = nullptr,
                                            int64_t rld = 0, int rrows = 0, int rcols = 0) {
-    const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6, rr = tid & 7, cc = (tid >> 3) & 7;
+    const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6;
     const int64_t ld = t.ld;
     T* D = L + t.diag_off;
     {
@@ -638,14 +672,6 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
     int arrived = 0;
     if (counter && tid == 0) arrived = atomicAdd(counter, 1);
     int fail = INT_MAX;    // first failing column (wave 0, lane 0)
-    // A(r, c) -= sum_k L(r, k0 + k) L(c, k0 + k) for the entry (rr, cc) of block (i, j)
-    auto rank8 = [&](int k0, int i, int j) {
-        const int r = PB * i + rr, c = PB * j + cc;
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * a[k0 + k][c];
-        a[c][r] -= acc;
-    };
     constexpr int NBK = TS / PB;
 #pragma unroll 1
     for (int bs = 0; bs < NBK; ++bs) {
@@ -702,21 +728,21 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
                 }
             }
         } else if (bs > 0) {
+            // deferred from step bs-1 (panel k0): the trailing update of columns >= b0 + 8 and
+            // S += L(:, k0..) X(k0.., :) for rows >= b0, columns < b0; 16x16 MFMA tiles (waves
+            // 1-3 round robin), anchored at multiples of 16 and masked to those regions
             const int k0 = b0 - PB;
-            const int m = NBK - 1 - bs;                      // block columns bs+1 .. NBK-1
-            const int na = m * (m + 1) / 2, ns = (NBK - bs) * bs;
+            const int t0 = (b0 + PB) / 16;                   // first tile row/column of the trailing part
+            const int m = 4 - t0, na = m * (m + 1) / 2;
+            const int s0 = b0 / 16, ns = (4 - s0) * ((b0 + 15) / 16);
             for (int q = wave - 1; q < na + ns; q += 3) {
                 if (q < na) {
-                    int j = bs + 1, e = q;
-                    while (e >= NBK - j) { e -= NBK - j; ++j; }
-                    rank8(k0, j + e, j);
+                    int tj = t0, e = q;
+                    while (e >= 4 - tj) { e -= 4 - tj; ++tj; }
+                    tile_rank8<false>(a, x, k0, 16 * (tj + e), 16 * tj, b0 + PB, b0 + PB, TS, true);
                 } else {
-                    const int e = q - na, i = bs + e / bs, j = e % bs;
-                    const int r = PB * i + rr, c = PB * j + cc;
-                    double acc = 0.0;
-#pragma unroll
-                    for (int k = 0; k < PB; ++k) acc += a[k0 + k][r] * x[c][k0 + k];
-                    x[c][r] += acc;
+                    const int e = q - na, nsc = (b0 + 15) / 16;
+                    tile_rank8<true>(a, x, k0, 16 * (s0 + e / nsc), 16 * (e % nsc), b0, 0, b0, false);
                 }
             }
         }
@@ -745,7 +771,10 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
                     }
             }
         } else if (bs + 1 < NBK) {
-            for (int i = bs + wave; i < NBK; i += 3) rank8(b0, i, bs + 1);
+            // lookahead: block column bs+1 (columns b0 + 8 .. b0 + 15) from this step's panel
+            const int tc = (b0 + PB) / 16;
+            for (int tr = tc + wave - 1; tr < 4; tr += 3)
+                tile_rank8<false>(a, x, b0, 16 * tr, 16 * tc, b0 + PB, b0 + PB, b0 + 2 * PB, true);
         }
         __syncthreads();
     }
