@@ -597,6 +597,26 @@ def traffic_probe(args):
     torch.cuda.synchronize()
 
 
+def hbm_copy_gbs(dev, gib=2, reps=5):
+    """Bytes moved (read + write) per second by a plain 2 GiB device-to-device copy: what this
+    GPU's HBM delivers to a streaming kernel, beside the published 8 TB/s."""
+    import torch
+    a = torch.empty(gib << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * (gib << 30) * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def bench_cube(args):
     import torch
     from meshfemsparse_amd import _lib
@@ -715,6 +735,11 @@ def bench_cube(args):
     del f
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
+    copy_gbs = hbm_copy_gbs(dev)
+    for rf in (roof_asm, roof_solve):
+        if rf:
+            rf["copy_gbs"] = copy_gbs
+            rf["frac_of_copy"] = rf["achieved"] / copy_gbs
     traffic, why = (None, "skipped (--no-traffic)")
     if not args.no_traffic:
         traffic, why = traffic_passes(args, args.precision)
@@ -769,7 +794,7 @@ def bench_cube(args):
         "factor_tflops": factor_tflops, "t_assembly_s": (timed["t_zero_ms"] + timed["t_assembly_ms"]) / 1e3,
         "t_factor_s": timed["t_factor_s"], "t_solve_s": timed["t_solve_s"],
         "refine_iters": timed["iters"], "relres_max": timed["relres_max"],
-        "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms,
+        "symbolic_s": t_sym, "kernel_ms_per_step": kernel_ms, "hbm_copy_gbs": copy_gbs,
         "roofline": roofline, "roofline_assembly": roof_asm, "roofline_solve": roof_solve,
         "traffic_note": why if traffic is None else
         f"rocprofv3 FETCH_SIZE x{traffic['scale_fetch']:.3f} and WRITE_SIZE x{traffic['scale_write']:.3f} "
