@@ -623,15 +623,11 @@ __device__ __forceinline__ void tile_rank8(double (*__restrict__ a)[TS + 1], dou
 // after its tile load has completed, and only the last to arrive stores the factor, the inverse
 // and a pivot failure (so no workgroup can see a stored factor in place of its input), then
 // resets the counter for the next factorization.  No workgroup waits for another.
-// With `rt`, the rows x cols block at `rsrc` (leading dimension rld) is staged into rt[col][row]
-// (zero elsewhere) by the same round trip that loads the tile.
 template <typename T>
 __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L, T* __restrict__ inv,
                                            int* __restrict__ status, double (*__restrict__ a)[TS + 1],
                                            double (*__restrict__ x)[TS + 1], int* __restrict__ counter = nullptr,
-                                           int count = 1, int* __restrict__ s_flag = nullptr,
-                                           T (*__restrict__ rt)[TS + 1] = nullptr, const T* __restrict__ rsrc = nullptr,
-                                           int64_t rld = 0, int rrows = 0, int rcols = 0) {
+                                           int count = 1, int* __restrict__ s_flag = nullptr) {
     const int tid = threadIdx.x, kb = t.kb, wave = tid >> 6;
     const int64_t ld = t.ld;
     T* D = L + t.diag_off;
@@ -639,31 +635,17 @@ __device__ __forceinline__ void potrf_tile(const PotrfTask& t, T* __restrict__ L
         // every load of a thread is issued before the first use (one memory round trip); entries
         // outside the kb x kb tile read a clamped in-tile address and are replaced by the identity
         constexpr int PT = TS * TS / 256;
-        T v[PT], w[PT];   // (w: the row tile, kept in T)
+        T v[PT];
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
             const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
             v[q] = D[(int64_t)min(c, kb - 1) * ld + min(r, kb - 1)];
-        }
-        if (rt) {
-#pragma unroll
-            for (int q = 0; q < PT; ++q) {
-                const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
-                w[q] = (r < rrows && c < rcols) ? rsrc[(int64_t)c * rld + r] : T(0);
-            }
         }
 #pragma unroll
         for (int q = 0; q < PT; ++q) {
             const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
             a[c][r] = (r < kb && c < kb) ? ((r >= c) ? (double)v[q] : 0.0) : ((r == c) ? 1.0 : 0.0);
             x[c][r] = 0.0;
-        }
-        if (rt) {
-#pragma unroll
-            for (int q = 0; q < PT; ++q) {
-                const int idx = tid + 256 * q;
-                rt[idx >> 6][idx & 63] = w[q];
-            }
         }
     }
     __syncthreads();    // every load of the tile has returned
@@ -995,9 +977,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ ta
 // (or a diagonal tile with no rows below).  The tile is factored in LDS by every workgroup of its
 // group (potrf_tile with the shared-tile counter), then the row tile is solved in place,
 // L21 := A21 X^T, with X = L11^-1 straight from LDS: the same MFMA products in the same k order as
-// the separate TRSM launch of the FP64 path.
+// the separate TRSM launch of the FP64 path.  The row tile's MFMA operands are loaded from global
+// memory after the factorization (it is this workgroup's alone), so the kernel holds only the tile
+// and its inverse in LDS (66.6 KB), and its registers stay within half a SIMD's file: a CU running
+// one 128x128 FP64 GEMM workgroup (64 KB, at most 256 registers per wave) of a concurrent launch
+// still has room for it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ items,
+__global__ __launch_bounds__(256, 2) void k_panel(const PanelItem* __restrict__ items,
                                                const PotrfTask* __restrict__ ptasks,
                                                const GemmTask* __restrict__ gtasks, T* __restrict__ L,
                                                T* __restrict__ inv, int* __restrict__ status,
@@ -1005,7 +991,6 @@ __global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ ite
     typedef typename Mfma<T>::acc_t acc_t;
     __shared__ double a[TS][TS + 1];
     __shared__ double x[TS][TS + 1];
-    __shared__ T rt[TS][TS + 1];     // the row tile, rt[k][r] = A21(r, k)
     __shared__ int flag;
     const PanelItem it = items[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1013,21 +998,33 @@ __global__ __launch_bounds__(256) void k_panel(const PanelItem* __restrict__ ite
     GemmTask g{};
     if (rows) g = gtasks[it.trsm];
     potrf_tile<T>(ptasks[it.potrf], L, inv, status, a, x, it.count > 1 ? counters + it.potrf : nullptr,
-                  it.count, &flag, rows ? rt : nullptr, L + g.a_off, g.lda, g.M, g.K);
+                  it.count, &flag);
     if (!rows) return;
     T* C = L + g.a_off;         // in place (c_off == a_off)
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32, cc = lane & 15;
+    // A21(r, k) for the wave's rows wm + 16 i + cc and the 16 k steps, zero beyond M x K; all loads
+    // in one round trip
+    T ag[TS / 4][2];
+#pragma unroll
+    for (int s4 = 0; s4 < TS / 4; ++s4) {
+        const int kr = 4 * s4 + (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r = wm + 16 * i + cc;
+            ag[s4][i] = (r < g.M && kr < g.K) ? C[(int64_t)kr * g.lda + r] : T(0);
+        }
+    }
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll
     for (int ks = 0; ks < TS; ks += 4) {
         const int kr = ks + (lane >> 4);
         T af[2], bf[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = rt[kr][wm + 16 * i + cc];
+        for (int i = 0; i < 2; ++i) af[i] = ag[ks / 4][i];
 #pragma unroll
         for (int j = 0; j < 2; ++j) bf[j] = (T)x[kr][wn + 16 * j + cc];  // B(n, k) = X(n, k), rounded to T
 #pragma unroll
