@@ -225,7 +225,8 @@ def bench_batch(args):
         "config": {"workload": f"{args.batch} unit cubes, per-side n ~ U{{6..20}} hexes x 6 Kuhn tets, E log-uniform "
                                f"[1e3,1e6], nu 0.3, z=0 pinned, one block-diagonal system",
                    "free_dofs": 3 * nb - len(B["pins"]), "tets": len(tets), "factor_nnz": f.getFactorNNZ(),
-                   "factor_flops": flops, "schedule": f.scheduleStats(), "autotune": {
+                   "factor_flops": flops, "factor_flops_unrelaxed": f.flopBreakdown()["unrelaxed"],
+                   "schedule": f.scheduleStats(), "autotune": {
                        "fuse_degree": tune["fuse_degree"], "bucket_tiles": tune["bucket_tiles"],
                        "ms": {"fuse": {str(a): round(v, 3) for a, v in tune["ms"]["fuse"].items()},
                               "bucket_tiles": [{str(t): round(v, 3) for t, v in d.items()}
@@ -682,6 +683,7 @@ def bench_cube(args):
     # algorithmic GEMM flops of a step (supernodal count minus the diagonal-tile POTRF share),
     # apportioned to the classes by their executed tile flops
     potrf_flops = f.flopBreakdown()["potrf"]
+    unrelaxed_flops = f.flopBreakdown()["unrelaxed"]   # sum of c_j^2 over the columns of L
     exec_all = float(work[5] + work[6] + work[7] + work[KCLASSES.index("panel")])
     share = float(sum(work[c] for c in gcls)) / exec_all if exec_all > 0 else 1.0
     gemm_work = (flops - potrf_flops) * share * args.steps
@@ -789,7 +791,8 @@ def bench_cube(args):
                                + ("FP32 scaled supernodal Cholesky + FP64 refinement to 1e-10" if single
                                   else "FP64 supernodal Cholesky + solve/refine to 1e-10"),
                    "free_dofs": nred, "tets": ntets, "factor_nnz": nnz,
-                   "factor_flops": flops, "supernodes": st["num_supernodes"], "levels": st["num_levels"],
+                   "factor_flops": flops, "factor_flops_unrelaxed": unrelaxed_flops,
+                   "supernodes": st["num_supernodes"], "levels": st["num_levels"],
                    "he_layout": args.he_layout},
         "factor_tflops": factor_tflops, "t_assembly_s": (timed["t_zero_ms"] + timed["t_assembly_ms"]) / 1e3,
         "t_factor_s": timed["t_factor_s"], "t_solve_s": timed["t_solve_s"],

@@ -1818,15 +1818,15 @@ __device__ __forceinline__ T wave_partials(const T (*red)[TS], int r) {
     return v;
 }
 
-// Forward diagonal: y(rows of tile t) = X(rows, 0 .. 64(t+1)) x_b; lanes over rows, the NW waves
-// over interleaved columns, partial sums combined in a fixed order.  NW = 16 (1024 threads) for
-// launches of few tiles, where one workgroup's latency is the step's.
-template <typename T, int NW>
-__global__ __launch_bounds__(64 * NW) void k_sfdiag(const SolveTile* __restrict__ tiles,
+// Forward diagonal: y(rows of tile t) = X(rows, 0 .. 64(t+1)) x_b; lanes over rows, the four waves
+// over interleaved columns, partial sums combined in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sfdiag(const SolveTile* __restrict__ tiles,
                                                     const SolveBlock* __restrict__ blocks,
                                                     const T* __restrict__ Xb,
                                                     const T* __restrict__ xall, T* __restrict__ yall,
                                                     int64_t ldx) {
+    constexpr int NW = 4;
     const SolveTile st = tiles[blockIdx.x];
     const SolveBlock b = blocks[st.blk];
     const T* x = xall + blockIdx.y * ldx + b.x_off;
@@ -1844,11 +1844,12 @@ __global__ __launch_bounds__(64 * NW) void k_sfdiag(const SolveTile* __restrict_
 
 // Forward update: rows [r0, r0 + nr) below block b: x -= L(rows, b) y_b (rows of the supernode)
 // or u += L(rows, b) y_b (structure rows, the update vector passed to the parent).
-template <typename T, int NW>
-__global__ __launch_bounds__(64 * NW) void k_sfupd(const SolveRowTask* __restrict__ tasks, int64_t ntasks,
+template <typename T>
+__global__ __launch_bounds__(256) void k_sfupd(const SolveRowTask* __restrict__ tasks, int64_t ntasks,
                                                    const T* __restrict__ L,
                                                    const T* __restrict__ yall, T* __restrict__ xall,
                                                    int64_t ldx, T* __restrict__ uall, int64_t ldu) {
+    constexpr int NW = 4;
     const int64_t ti = xcd_chunk(blockIdx.x, gridDim.x);
     if (ti >= ntasks) return;                  // the whole workgroup leaves together
     const SolveRowTask t = tasks[ti];
@@ -2018,8 +2019,8 @@ __global__ __launch_bounds__(256) void k_sbpart(const SolveRowTask* __restrict__
     else sbpart_body<T, SOLVE_RC / 64>(t, L, src, part);
 }
 
-// Column passes of the backward diagonal: a wave's CP columns c + NW j per memory round trip (CP
-// = 4, or 2 for 16 row chunks, which keeps the kernel's registers at the 8-chunk size),
+// Column passes of the backward diagonal (NW waves): a wave's CP columns c + NW j per memory round
+// trip (CP = 4, or 2 for 16 row chunks, which keeps the kernel's registers at the 8-chunk size),
 // the rows in MZ chunks of 64; rows beyond nz read row 0 (in bounds), weighted by 0.
 template <typename T, int MZ, int NW>
 __device__ __forceinline__ void sbdiag_cols(const T* __restrict__ X, int nb, int c0, int ncol, int nz,
@@ -2046,10 +2047,9 @@ __device__ __forceinline__ void sbdiag_cols(const T* __restrict__ X, int nb, int
 }
 
 // Backward diagonal: x(columns of tile t) = X(rows >= 64t, columns)^T z, z = y_b - sum of the
-// block's partial sums (fixed order); one wave per column, lanes over rows.  NW = 16 (1024
-// threads: one column pass) for launches of few tiles.
-template <typename T, int NW>
-__global__ __launch_bounds__(64 * NW) void k_sbdiag(const SolveTile* __restrict__ tiles,
+// block's partial sums (fixed order); one wave per column, lanes over rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sbdiag(const SolveTile* __restrict__ tiles,
                                                     const SolveBlock* __restrict__ blocks,
                                                     const T* __restrict__ Xb,
                                                     const T* __restrict__ yall, T* __restrict__ xall,
@@ -2061,6 +2061,7 @@ __global__ __launch_bounds__(64 * NW) void k_sbdiag(const SolveTile* __restrict_
     T* x = xall + blockIdx.y * ldx + b.x_off;
     const T* part = partall + blockIdx.y * ldp;
     __shared__ T z[1024];
+    constexpr int NW = 4;
     const int tid = threadIdx.x;
     const int c0 = TS * st.t, ncol = min(TS, b.nb - c0), nz = b.nb - c0;
     for (int r = tid; r < nz; r += 64 * NW) {
@@ -2163,10 +2164,10 @@ static double row_task_bytes(const std::vector<SolveRowTask>& v, int64_t b, int6
     return n;
 }
 
-// Solve launches of fewer workgroups than this (per right-hand side) run the 16-wave forms of
-// the diagonal and forward-update kernels: with at most a few workgroups per CU the step's time is
-// one workgroup's latency, which more waves per workgroup shorten.  The choice depends on the
-// plan only, so results do not depend on the number of right-hand sides.
+// Solve steps of fewer tiles or row tasks than this (per right-hand side) run the quarter-tile
+// kernels: with at most a few workgroups per CU the step's time is one workgroup's latency, which
+// spreading its bytes over more CUs shortens.  The choice depends on the plan only, so results do
+// not depend on the number of right-hand sides.
 constexpr int64_t kSolveWideBelow = 512;
 
 // factor-order vector conversions around a scaled (or lower-precision) solve: y = (T)(x * s)
@@ -2794,7 +2795,7 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
                 if (d1 - d0 < kSolveWideBelow)
                     k_sfdiag_q<T><<<dim3(4 * g.x, g.y), 1024, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
                 else
-                    k_sfdiag<T, 4><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
+                    k_sfdiag<T><<<g, 256, 0, s>>>(fdt_.as<SolveTile>() + d0, blk, Xb, xp, Y, ldx);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
@@ -2806,7 +2807,7 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
                     k_sfupd_q<T><<<dim3(xcd_grid(4 * nt), (unsigned)nrhs), 1024, 0, s>>>(
                         fupd_.as<SolveRowTask>() + u0, nt, L, Y, xp, ldx, u, ldu);
                 else
-                    k_sfupd<T, 4><<<dim3(xcd_grid(nt), (unsigned)nrhs), 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, nt,
+                    k_sfupd<T><<<dim3(xcd_grid(nt), (unsigned)nrhs), 256, 0, s>>>(fupd_.as<SolveRowTask>() + u0, nt,
                                                                                      L, Y, xp, ldx, u, ldu);
                 LAUNCH_CHECK();
                 prof.end(s);
@@ -2840,7 +2841,7 @@ void FactorizerDev::solve_t(T* xp, int nrhs, void* stream, const SolveScratch* w
                     k_sbdiag_q<T><<<dim3(4 * g.x, g.y), 1024, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx,
                                                                      part, ldp, S.solve_block);
                 else
-                    k_sbdiag<T, 4><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
+                    k_sbdiag<T><<<g, 256, 0, s>>>(bdt_.as<SolveTile>() + d0, blk, Xb, Y, xp, ldx, part, ldp,
                                                      S.solve_block);
                 LAUNCH_CHECK();
                 prof.end(s);
