@@ -324,7 +324,7 @@ struct SymbolicOptions {
                             // (scalars) go to the fused level kernel; 0 disables it
     bool lookahead = true;  // split outer trailing updates: the next outer block's columns
                             // first, the rest beside the next outer block's steps
-    int panel_max = 256;    // a step whose TRSM needs at most this many row tiles (and POTRF
+    int panel_max = 384;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     int64_t n_scalar = -1;  // scalar length of the caller's matrix when it is not 3 nb (a scalar
                             // matrix grouped into blocks: scalars n_scalar .. 3 nb - 1 are padding)
