@@ -500,14 +500,19 @@ __global__ void k_diag_scale(const int32_t* __restrict__ diag_src, int64_t nbloc
 // ------------------------------------------------------------------------------------------
 // Extend-add: child's U (lower 3x3 blocks of column block kj) into the parent front.
 // ------------------------------------------------------------------------------------------
+// Four tasks per workgroup, one per wave: a CU holds a bounded number of workgroups, so
+// one-wave workgroups would leave most of its wave slots empty when tasks are short.
 template <typename T>
-__global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ tasks,
-                                               const int32_t* __restrict__ rel,
-                                               T* __restrict__ U, T* __restrict__ L) {
+__global__ __launch_bounds__(256) void k_extend(const ExtendTask* __restrict__ tasks, int64_t ntasks,
+                                                const int32_t* __restrict__ rel,
+                                                T* __restrict__ U, T* __restrict__ L) {
     // lane = one scalar row of the column block, so every load and store of a wave covers
     // consecutive addresses (child rows are contiguous; parent rows follow the increasing
     // relative indices); each parent entry still receives exactly one add per launch
-    const ExtendTask t = tasks[blockIdx.x];
+    const int64_t ti = 4 * (int64_t)blockIdx.x + (threadIdx.x >> 6);
+    if (ti >= ntasks) return;                  // the whole wave leaves together
+    const int lane = threadIdx.x & 63;
+    const ExtendTask t = tasks[ti];
     const int32_t* relc = rel + t.rel_off;
     const int kj = t.col_block;
     const int64_t Dc = 3 * (int64_t)t.dc;
@@ -530,7 +535,7 @@ __global__ __launch_bounds__(64) void k_extend(const ExtendTask* __restrict__ ta
     // of the next rows above these stores by itself)
     constexpr int UN = 4;
     const int64_t row1 = min(Dc, (int64_t)t.row0 + kExtendRows);
-    for (int64_t i0 = t.row0 + (int64_t)threadIdx.x; i0 < row1; i0 += 64 * UN) {
+    for (int64_t i0 = t.row0 + (int64_t)lane; i0 < row1; i0 += 64 * UN) {
         int64_t d[UN];
         int r[UN];
         bool ok[UN], diag[UN];
@@ -2551,7 +2556,7 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
         for (auto [a, b] : passes)
             if (b > a) {
                 prof.begin(KernelProfile::EXTEND, s);
-                k_extend<T><<<(unsigned)(b - a), 64, 0, s>>>(ext_.as<ExtendTask>() + a, rel_.as<int32_t>(), U, L);
+                k_extend<T><<<(unsigned)((b - a + 3) / 4), 256, 0, s>>>(ext_.as<ExtendTask>() + a, b - a, rel_.as<int32_t>(), U, L);
                 LAUNCH_CHECK();
                 prof.end(s);
             }
