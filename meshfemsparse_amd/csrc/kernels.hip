@@ -862,6 +862,49 @@ __device__ __forceinline__ void gemm_store_acc(const typename Mfma<T>::acc_t (&a
     }
 }
 
+// C (+)= alpha * acc for one wave's WM x WN quadrant through LDS: the quadrant goes to the wave's
+// own LDS region in NP parts of WN / NP columns, each written in the MFMA accumulator layout and
+// read back column by column, so the global loads and stores of C are whole column segments of WM
+// rows (the accumulator layout itself touches 16 columns per instruction), and every load of a
+// part is issued before its first store.  Same arithmetic as gemm_store_acc (cv + alpha * acc).
+// `region` holds WM * WN / NP elements; rows are XOR-swizzled by the column so that both the
+// layout writes and the column reads spread over the LDS banks.  Called by all waves together,
+// after the K loop's last barrier (the operand buffers that hold the regions are no longer read).
+template <typename T, int FM, int FN, int NP>
+__device__ __forceinline__ void gemm_store_lds(const typename Mfma<T>::acc_t (&acc)[FM][FN], T* __restrict__ C,
+                                               int64_t ldc, int M, int N, int wm, int wn, int lane,
+                                               T alpha, bool beta0, T* __restrict__ region) {
+    constexpr int FP = FN / NP, WM = 16 * FM, WC = 16 * FP, PER = WM * WC / 64;   // per part
+    static_assert(FN % NP == 0 && (WM == 32 || WM == 64), "epilogue shape");
+#pragma unroll
+    for (int h = 0; h < NP; ++h) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int jj = 0; jj < FP; ++jj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * i + Mfma<T>::row(lane, r), col = 16 * jj + (lane & 15);
+                    region[col * WM + (row ^ (col & 31))] = acc[i][h * FP + jj][r];
+                }
+        __syncthreads();
+        T cv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = lane + 64 * q, row = e % WM, col = e / WM;
+            const int grow = wm + row, gcol = wn + h * WC + col;
+            cv[q] = (!beta0 && grow < M && gcol < N) ? C[gcol * ldc + grow] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = lane + 64 * q, row = e % WM, col = e / WM;
+            const int grow = wm + row, gcol = wn + h * WC + col;
+            if (grow < M && gcol < N) C[gcol * ldc + grow] = cv[q] + alpha * region[col * WM + (row ^ (col & 31))];
+        }
+        __syncthreads();
+    }
+}
+
 // Raw buffer loads (resource word 3 for gfx9: 32-bit data format, no swizzle, stride 0).
 constexpr int kBufWord3 = 0x00020000;
 template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
@@ -964,7 +1007,14 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
     }
     const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
     const bool beta0 = (t.flags & GT_BETA0) != 0;
-    gemm_store_acc<T, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0);
+    // waves 0 and 1 keep their quadrant's parts in the A buffers, waves 2 and 3 in the B buffers;
+    // FP32 stores its 128x128 tiles in four parts (16 loads of C per lane in flight, which keeps
+    // the kernel within 128 registers: four workgroups per CU), the rest in two
+    constexpr int NP = (sizeof(T) == 4 && FN == 4) ? 4 : 2, RE = WM * WN / NP;
+    static_assert(2 * RE <= 2 * KC * (TM + PAD) && 2 * RE <= 2 * KC * (TN + PAD),
+                  "epilogue regions exceed the operand buffers");
+    T* region = (wave < 2 ? &As[0][0][0] : &Bs[0][0][0]) + (wave & 1) * RE;
+    gemm_store_lds<T, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region);
 }
 
 // (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
