@@ -875,7 +875,7 @@ __device__ __forceinline__ void gemm_store_lds(const typename Mfma<T>::acc_t (&a
                                                int64_t ldc, int M, int N, int wm, int wn, int lane,
                                                T alpha, bool beta0, T* __restrict__ region) {
     constexpr int FP = FN / NP, WM = 16 * FM, WC = 16 * FP, PER = WM * WC / 64;   // per part
-    static_assert(FN % NP == 0 && (WM == 32 || WM == 64), "epilogue shape");
+    static_assert(FN % NP == 0 && WM % 32 == 0 && PER * 64 == WM * WC, "epilogue shape");
 #pragma unroll
     for (int h = 0; h < NP; ++h) {
 #pragma unroll
@@ -1599,7 +1599,11 @@ __global__ __launch_bounds__(NT, 512 / NT) void k_gemm_bf(const GemmTask* __rest
         buf ^= 1;
     }
     const float alpha = (t.flags & GT_ALPHA_POS) ? 1.0f : -1.0f;
-    gemm_store_acc<float, FM, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, (t.flags & GT_BETA0) != 0);
+    // C through LDS one 16-column fragment per part (each wave's region in the operand buffers)
+    constexpr int RE = WM * 16;
+    static_assert((NT / 64) * RE * 4 <= 2 * STAGE * 2, "epilogue regions exceed the operand buffers");
+    gemm_store_lds<float, FM, FN, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, (t.flags & GT_BETA0) != 0,
+                                      reinterpret_cast<float*>(smem) + wave * RE);
 }
 
 // big256: the range's large tiles are 256x256 (Schur ranges of an analysis with schur_tile 256)
