@@ -806,7 +806,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // Workgroups are dispatched to the 8 XCDs round-robin by workgroup id.  Mode 1 gives XCD x the
 // x-th contiguous run of tasks; mode 2 deals runs of 8 consecutive tasks round-robin (tasks
 // beyond the last full round of 64 keep their position), so each run of 8 tiles shares one L2
-// while the XCDs still draw evenly from the whole launch.
+// while the XCDs still draw evenly from the whole launch; mode 3 deals runs of 64 (an 8 x 8
+// block of tiles in the task order of the schedule, whose A and B panels then share one L2;
+// tasks beyond the last full round of 512 keep their position).
 __device__ __forceinline__ int xcd_task(int b, int n, int mode) {
     if (mode == 1) {
         const int x = b & 7, j = b >> 3, q = n >> 3, r = n & 7;
@@ -815,6 +817,10 @@ __device__ __forceinline__ int xcd_task(int b, int n, int mode) {
     if (mode == 2 && b < (n & ~63)) {
         const int x = b & 7, j = b >> 3;
         return (((j >> 3) << 3) + x) * 8 + (j & 7);
+    }
+    if (mode == 3 && b < (n & ~511)) {
+        const int x = b & 7, j = b >> 3;
+        return (((j >> 6) << 3) + x) * 64 + (j & 63);
     }
     return b;
 }
@@ -1396,10 +1402,14 @@ static double gemm_noreuse_bytes(const Symbolic& S, const TaskRange& r, size_t f
 // tuning knobs: MFS_GEMM_V (1: k_gemm, 2: k_gemm2; default 1) for the
 // 128x128 tiles and MFS_GEMM_V64 for the 64x64 tiles (default: as MFS_GEMM_V), MFS_GEMM_KC2 and
 // MFS_GEMM_KC64 (k_gemm2's K chunk for the 128x128 and the 64x64 tiles: 16, 32 or 64; default 16),
-// MFS_GEMM_REMAP (task-to-XCD mode of xcd_task, default 0)
-static int gemm_remap() {
-    static const int v = getenv("MFS_GEMM_REMAP") ? atoi(getenv("MFS_GEMM_REMAP")) : 0;
-    return v;
+// MFS_GEMM_REMAP (task-to-XCD mode of xcd_task; default 2 for the FP64 and bfloat16 tiles, 3 for
+// the FP32 tiles: measured on the 64^3 step, modes 2 and 3 against 0 FP64 0.2824 -> 0.2802 /
+// 0.2809 s, FP32 0.1661 -> 0.1638 / 0.1631 s, BF16x3 0.1181 -> 0.1166 / 0.1169 s; mode 1 is
+// slower, FP64 0.2924 and FP32 0.1684)
+static int gemm_remap(int element_bytes) {
+    static const int v = getenv("MFS_GEMM_REMAP") ? atoi(getenv("MFS_GEMM_REMAP")) : -1;
+    if (v >= 0) return v;
+    return element_bytes == 4 ? 3 : 2;
 }
 static int gemm_version() {
     static const int v = getenv("MFS_GEMM_V") ? atoi(getenv("MFS_GEMM_V")) : 0;
@@ -1447,7 +1457,7 @@ static void launch_gemm2(int kc, unsigned grid, const GemmTask* tasks, T* L, T* 
 }
 template <typename T>
 static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
-    const int rm = gemm_remap();
+    const int rm = gemm_remap((int)sizeof(T));
     // default: the k-major buffer-load kernel for both precisions (the k-contiguous kernel remains
     // selectable)
     const int ver = gemm_version() ? gemm_version() : 1;
@@ -1609,7 +1619,7 @@ __global__ __launch_bounds__(NT, 512 / NT) void k_gemm_bf(const GemmTask* __rest
 // big256: the range's large tiles are 256x256 (Schur ranges of an analysis with schur_tile 256)
 static void launch_gemm_bf(const TaskRange& r, const GemmTask* tasks, float* L, float* U, const float* inv,
                            bool split, bool big256, hipStream_t s) {
-    const int rm = gemm_remap();
+    const int rm = gemm_remap(2);
     if (r.m > r.b && big256) {
         if (split) k_gemm_bf<256, 256, true, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
         else k_gemm_bf<256, 256, false, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
