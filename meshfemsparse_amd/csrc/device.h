@@ -131,6 +131,11 @@ public:
     int precision;
 
     void require(int t) const;
+    // the Schur-update tiles add the children's update-matrix blocks (FP64 and FP32 tile kernels,
+    // every tile of the analysis carrying its list) instead of the extend-add passes into U
+    bool pulls_active() const {
+        return precision < 2 && sym && sym->pulls_complete && !sym->pull_ents.empty();
+    }
 
 private:
     void install(std::unique_ptr<Symbolic> S);
@@ -156,6 +161,7 @@ private:
 
     DeviceArray L_, U_, inv_, status_;
     DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, fused_, uoff_, panel_, pcount_;
+    DeviceArray pull_;                     // children's update-matrix shares of the Schur tiles
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_, f2up_, up2i_, cmask_;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <climits>
+#include <cstdlib>
 
 #include "mfs_internal.h"
 
@@ -153,6 +154,30 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
         }
         children(true);
     };
+    // the Schur-update tiles' own lists of children's update-matrix blocks, added after the tile
+    // (child order), in place of the extend-add passes into U; MFS_SYRK_PULL=0 selects the passes,
+    // MFS_SYRK_PULL=2 makes an analysis without complete lists an error (replay checks)
+    const char* pe = getenv("MFS_SYRK_PULL");
+    const int pmode = pe ? atoi(pe) : 1;
+    const bool pull = S.pulls_complete && !S.pull_ents.empty() && pmode != 0;
+    if (pmode == 2 && !pull) throw Error(-1, "replay: the Schur tiles carry no complete children lists");
+    auto pull_tile = [&](const GemmTask& t) {
+        double* C = U.data() + t.c_off;
+        const int32_t n = t.flags >> GT_PULL_SHIFT;
+        for (int32_t e = t.pad; e < t.pad + n; ++e) {
+            const PullEnt& p = S.pull_ents[e];
+            const int32_t* rl = S.rel.data() + p.rel_off;
+            for (int32_t jb = p.jb0; jb < p.jb0 + p.njb; ++jb)
+                for (int32_t ib = std::max(jb, p.ib0); ib < p.ib0 + p.nib; ++ib)
+                    for (int cs = 0; cs < 3; ++cs)
+                        for (int r = (ib == jb ? cs : 0); r < 3; ++r) {
+                            const int64_t row = 3 * (int64_t)rl[ib] + r - p.base_r;
+                            const int64_t col = 3 * (int64_t)rl[jb] + cs - p.base_c;
+                            if (row < 0 || row >= t.M || col < 0 || col >= t.N) continue;
+                            C[col * t.ldc + row] += U[p.u_child + (int64_t)(3 * jb + cs) * p.dc3 + 3 * ib + r];
+                        }
+        }
+    };
     for (const LevelPlan& P : S.levels) {
         for (int64_t q = P.fused.first; q < P.fused.second; ++q) fused(S.fused_tasks[q]);
         for (const auto& pass : P.extend_passes)
@@ -164,9 +189,13 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
             for (int64_t q = P.update_steps[st].b; q < P.update_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
             for (int64_t q = P.rest_steps[st].b; q < P.rest_steps[st].e; ++q) gemm(S.gemm_tasks[q]);
         }
-        for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) gemm(S.gemm_tasks[q]);
-        for (const auto& pass : P.extend_u_passes)
-            for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
+        for (int64_t q = P.syrk.b; q < P.syrk.e; ++q) {
+            gemm(S.gemm_tasks[q]);
+            if (pull) pull_tile(S.gemm_tasks[q]);
+        }
+        if (!pull)
+            for (const auto& pass : P.extend_u_passes)
+                for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
     }
     return status == INT64_MAX ? -1 : status;
 }

@@ -876,10 +876,96 @@ __device__ __forceinline__ void gemm_store_acc(const typename Mfma<T>::acc_t (&a
 // `region` holds WM * WN / NP elements; rows are XOR-swizzled by the column so that both the
 // layout writes and the column reads spread over the LDS banks.  Called by all waves together,
 // after the K loop's last barrier (the operand buffers that hold the regions are no longer read).
+// Raw buffer loads (resource word 3 for gfx9: 32-bit data format, no swizzle, stride 0).
+constexpr int kBufWord3 = 0x00020000;
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+//
+// Schur-update tiles (beta = 0, C in the update arena) may also carry their children's update-matrix
+// blocks that land in them (`npull` PullEnt entries, child order).  The wave's values become
+// v = 0 + alpha * acc in registers, and each child's entries are added in child order: the same
+// additions, in the same order, as the extend-add passes into U after the Schur update.  Per entry
+// the wave's rows and columns are mapped to the child's scalar rows and columns (-1: none) through
+// LDS (`map`, WM + WN ints of the wave's epilogue region), then every fragment element with a
+// child entry in the child's lower triangle loads it (one fragment column of loads in flight).
+template <typename T, int FM, int FN>
+__device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][FN], T alpha,
+                                            int* __restrict__ map, const PullEnt* __restrict__ pe, int npull,
+                                            const int32_t* __restrict__ rel, const T* __restrict__ U,
+                                            int wm, int wn, int lane) {
+    constexpr int WM = 16 * FM, WN = 16 * FN;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0) + alpha * acc[i][j][r];
+    int* rmap = map;
+    int* cmap = map + WM;
+    for (int e = 0; e < npull; ++e) {
+        const PullEnt pu = pe[e];
+        const int32_t* rl = rel + pu.rel_off;
+        const int ib = pu.ib0 + lane, jb = pu.jb0 + lane;
+        // wave-relative scalar row / column of the block's first scalar (far out of range: no block)
+        const int rb = lane < pu.nib ? 3 * rl[ib] - pu.base_r - wm : -(1 << 20);
+        const int cb = lane < pu.njb ? 3 * rl[jb] - pu.base_c - wn : -(1 << 20);
+        __syncthreads();     // the previous entry's map reads are done (npull is uniform per tile)
+        for (int q = lane; q < WM + WN; q += 64) map[q] = -1;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            if (rb + r >= 0 && rb + r < WM) rmap[rb + r] = 3 * ib + r;
+            if (cb + r >= 0 && cb + r < WN) cmap[cb + r] = 3 * jb + r;
+        }
+        __syncthreads();
+        // buffer loads based at the child's matrix: 32-bit offsets, and an element outside the
+        // child's lower triangle reads the out-of-range offset `nrec`, which returns +0 (v holds
+        // 0 + alpha * acc, never -0, so adding +0 leaves it unchanged bit for bit)
+        const int nrec = pu.dc3 * pu.dc3 * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(U + pu.u_child), (short)0, nrec, kBufWord3);
+#ifndef MFS_PULL_FB
+#define MFS_PULL_FB 4
+#endif
+        constexpr int FB = FM < MFS_PULL_FB ? FM : MFS_PULL_FB;   // fragments of loads in flight
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int cj = cmap[16 * j + (lane & 15)];
+            const int cbase = cj * pu.dc3;
+#pragma unroll
+            for (int i0 = 0; i0 < FM; i0 += FB) {
+                T v[FB][4];
+#pragma unroll
+                for (int i = 0; i < FB; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int ri = rmap[16 * (i0 + i) + Mfma<T>::row(lane, r)];
+                        // child lower triangle: scalar row >= scalar column (cj >= 0 then implies ri >= 0)
+                        const int off = (cj >= 0 && ri >= cj) ? (cbase + ri) * (int)sizeof(T) : nrec;
+                        v[i][r] = buf_load<T>(rs, off, 0);
+                    }
+#pragma unroll
+                for (int i = 0; i < FB; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[i0 + i][j][r] += v[i][r];
+            }
+        }
+    }
+    __syncthreads();         // map reads done before the epilogue reuses the region
+}
+
+// (`final_vals`: acc already holds the values to store, as after apply_pulls)
 template <typename T, int FM, int FN, int NP>
 __device__ __forceinline__ void gemm_store_lds(const typename Mfma<T>::acc_t (&acc)[FM][FN], T* __restrict__ C,
                                                int64_t ldc, int M, int N, int wm, int wn, int lane,
-                                               T alpha, bool beta0, T* __restrict__ region) {
+                                               T alpha, bool beta0, T* __restrict__ region,
+                                               bool final_vals = false) {
     constexpr int FP = FN / NP, WM = 16 * FM, WC = 16 * FP, PER = WM * WC / 64;   // per part
     static_assert(FN % NP == 0 && WM % 32 == 0 && PER * 64 == WM * WC, "epilogue shape");
 #pragma unroll
@@ -905,21 +991,13 @@ __device__ __forceinline__ void gemm_store_lds(const typename Mfma<T>::acc_t (&a
         for (int q = 0; q < PER; ++q) {
             const int e = lane + 64 * q, row = e % WM, col = e / WM;
             const int grow = wm + row, gcol = wn + h * WC + col;
-            if (grow < M && gcol < N) C[gcol * ldc + grow] = cv[q] + alpha * region[col * WM + (row ^ (col & 31))];
+            const T rv = region[col * WM + (row ^ (col & 31))];
+            if (grow < M && gcol < N) C[gcol * ldc + grow] = final_vals ? rv : cv[q] + alpha * rv;
         }
         __syncthreads();
     }
 }
 
-// Raw buffer loads (resource word 3 for gfx9: 32-bit data format, no swizzle, stride 0).
-constexpr int kBufWord3 = 0x00020000;
-template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
 
 // One output tile of a task (256 threads; As/Bs: double-buffered operand chunks in LDS, rows padded
 // by PAD elements).  The operand loads are raw buffer loads: per K chunk one buffer resource based at
@@ -937,7 +1015,9 @@ template <typename T, int TM> constexpr int gemm_pad() { return sizeof(T) == 4 ?
 template <typename T, int TM, int TN, int KC, int PAD = gemm_pad<T, TM>()>
 __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, T* __restrict__ U,
                                             const T* __restrict__ inv, T (*__restrict__ As)[KC][TM + PAD],
-                                            T (*__restrict__ Bs)[KC][TN + PAD]) {
+                                            T (*__restrict__ Bs)[KC][TN + PAD],
+                                            const PullEnt* __restrict__ pulls = nullptr,
+                                            const int32_t* __restrict__ rel = nullptr) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int AST = 256 / TM, BST = 256 / TN;
@@ -1024,18 +1104,27 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
     static_assert(2 * RE <= 2 * KC * (TM + PAD) && 2 * RE <= 2 * KC * (TN + PAD),
                   "epilogue regions exceed the operand buffers");
     T* region = (wave < 2 ? &As[0][0][0] : &Bs[0][0][0]) + (wave & 1) * RE;
-    gemm_store_lds<T, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region);
+    const int npull = pulls ? (t.flags >> GT_PULL_SHIFT) : 0;
+    if (npull > 0)
+        apply_pulls<T, FM, FN>(acc, alpha, reinterpret_cast<int*>(region), pulls + t.pad, npull, rel, U, wm, wn,
+                               lane);
+    gemm_store_lds<T, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region, npull > 0);
 }
 
 // (second launch bound: at least 2 waves per SIMD, so two 128x128 workgroups share a CU)
-template <typename T, int TM, int TN, int KC>
+template <typename T, int TM, int TN, int KC, bool PULL = false>
 __global__ __launch_bounds__(256, 2) void k_gemm(const GemmTask* __restrict__ tasks,
                                                  T* __restrict__ L, T* __restrict__ U,
-                                                 const T* __restrict__ inv, int remap) {
+                                                 const T* __restrict__ inv, int remap,
+                                                 const PullEnt* __restrict__ pulls,
+                                                 const int32_t* __restrict__ rel) {
     constexpr int PAD = gemm_pad<T, TM>();
     __shared__ T As[2][KC][TM + PAD];
     __shared__ T Bs[2][KC][TN + PAD];
-    gemm_tile<T, TM, TN, KC>(tasks[xcd_task(blockIdx.x, gridDim.x, remap)], L, U, inv, As, Bs);
+    // (the pull epilogue only in the Schur-update instantiation: its registers would lower the
+    // occupancy of every other launch)
+    gemm_tile<T, TM, TN, KC>(tasks[xcd_task(blockIdx.x, gridDim.x, remap)], L, U, inv, As, Bs,
+                             PULL ? pulls : nullptr, rel);
 }
 
 // Fused diagonal tile + TRSM of one step: workgroup = one 64-row tile below the diagonal tile
@@ -1460,23 +1549,26 @@ static void launch_gemm2(int kc, unsigned grid, const GemmTask* tasks, T* L, T* 
     LAUNCH_CHECK();
 }
 template <typename T>
-static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s) {
+static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, const T* inv, hipStream_t s,
+                        const PullEnt* pulls = nullptr, const int32_t* rel = nullptr) {
     const int rm = gemm_remap((int)sizeof(T));
     // default: the k-major buffer-load kernel for both precisions (the k-contiguous kernel remains
     // selectable)
     const int ver = gemm_version() ? gemm_version() : 1;
     const int ver64 = gemm_version64() ? gemm_version64() : ver;
     if (r.m > r.b) {
-        if (ver == 2) launch_gemm2<T, 128>(gemm_kc2(), (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, s);
+        if (ver == 2 && !pulls) launch_gemm2<T, 128>(gemm_kc2(), (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, s);
         else {
-            k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
+            if (pulls) k_gemm<T, 128, 128, 16, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm, pulls, rel);
+            else k_gemm<T, 128, 128, 16><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm, nullptr, nullptr);
             LAUNCH_CHECK();
         }
     }
     if (r.e > r.m) {
-        if (ver64 == 2) launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
+        if (ver64 == 2 && !pulls) launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
         else {
-            k_gemm<T, 64, 64, 16><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
+            if (pulls) k_gemm<T, 64, 64, 16, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm, pulls, rel);
+            else k_gemm<T, 64, 64, 16><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm, nullptr, nullptr);
             LAUNCH_CHECK();
         }
     }
@@ -2524,6 +2616,7 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     fused_.upload(S->fused_tasks);
     uoff_.upload(S->u_off);
     gemm_.upload(S->gemm_tasks);
+    pull_.upload(S->pull_ents);
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
     invt_.upload(S->inv_tiles);
@@ -2589,6 +2682,9 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     }
     if (ev_inject) HIP_CHECK(hipEventRecord((hipEvent_t)ev_inject, s));
     const GemmTask* gt = gemm_.as<GemmTask>();
+    // the Schur-update tiles add the children's update-matrix blocks in their epilogue (in place
+    // of the extend-add passes into U) where every tile carries its list (FP64 and FP32 tiles)
+    const bool pulls = pulls_active();
     // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
     // tile kernel; TRSM and the diagonal tiles stay in the factor type
     // (`syrk`: the range is a Schur update U = -L21 L21^T, whose large tiles are S.schur_tile)
@@ -2600,7 +2696,8 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
                 return;
             }
         }
-        launch_gemm<T>(r, gt, L, U, inv, st);
+        if (syrk && pulls) launch_gemm<T>(r, gt, L, U, inv, st, pull_.as<PullEnt>(), rel_.as<int32_t>());
+        else launch_gemm<T>(r, gt, L, U, inv, st);
     };
     // lookahead: the rest of an outer trailing update runs on the side stream after the step's
     // TRSM; the stream waits for it before the next outer trailing update or the Schur update,
@@ -2703,7 +2800,7 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
             schur_gemm(P.syrk, s, true);
             prof.end(s);
         }
-        extend(P.extend_u_passes);
+        if (!pulls) extend(P.extend_u_passes);
     }
     // explicit inverses of the solve column blocks' diagonal parts
     if (!S.inv_tiles.empty()) {

@@ -110,6 +110,23 @@ struct GemmTask {           // C(MxN) = alpha * A(MxK) * B(NxK)^T + beta * C ; c
 };
 static_assert(sizeof(GemmTask) == 56, "GemmTask layout");
 enum : int32_t { GT_C_IN_U = 1, GT_BETA0 = 2, GT_ALPHA_POS = 4, GT_B_IN_INV = 8 };
+// A Schur-update tile (C in the update arena) may carry the children's update-matrix entries that
+// land in it: `pad` is the first of (flags >> GT_PULL_SHIFT) PullEnt entries, in child order.
+constexpr int GT_PULL_SHIFT = 8;
+
+// One child's share of a Schur-update tile of its parent p: the child's update-matrix blocks
+// (ib, jb) with ib in [ib0, ib0 + nib), jb in [jb0, jb0 + njb), ib >= jb, whose parent rows and
+// columns fall in the tile.  Tile-local scalar row of child block row ib, component r:
+// 3 rel[rel_off + ib] + r - base_r (columns likewise with base_c), base = 3 W_p + tile origin.
+struct PullEnt {
+    int64_t u_child;        // child's U offset in the update arena (ld = dc3)
+    int32_t dc3;            // 3 x child degree
+    int32_t rel_off;        // offset of the child's relative indices
+    int32_t ib0, nib;       // child block rows (nib <= 64)
+    int32_t jb0, njb;       // child block columns (njb <= 64)
+    int32_t base_r, base_c;
+};
+static_assert(sizeof(PullEnt) == 40, "PullEnt layout");
 
 struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], ld; inverse -> slot
     int64_t diag_off;
@@ -268,6 +285,8 @@ struct Symbolic {
     std::vector<PotrfTask> potrf_tasks;
     std::vector<PanelItem> panel_items;
     std::vector<GemmTask> gemm_tasks;
+    std::vector<PullEnt> pull_ents;            // children's update-matrix shares of the Schur tiles
+    bool pulls_complete = true;                // every Schur tile lists its shares (tiles <= 128)
     std::vector<FusedTask> fused_tasks;
     // solves
     int32_t solve_block = 1024;                // column-block width (scalars, multiple of 64)
@@ -326,6 +345,9 @@ struct SymbolicOptions {
                             // first, the rest beside the next outer block's steps
     int panel_max = 384;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
+    bool syrk_pull = true;  // Schur-update tiles list the children's update-matrix blocks that
+                            // land in them (added in the tile's epilogue instead of by the
+                            // extend-add passes into U; FP64 and FP32 tile kernels)
     int64_t n_scalar = -1;  // scalar length of the caller's matrix when it is not 3 nb (a scalar
                             // matrix grouped into blocks: scalars n_scalar .. 3 nb - 1 are padding)
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
@@ -342,7 +364,8 @@ struct SymbolicOptions {
 // thresholds), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
-// of the outer trailing updates), MFS_BUCKET_TILES ("t0,t1,t2": largest GEMM tile, 64 or 128, per
+// of the outer trailing updates), MFS_SYRK_PULL (0: extend-add passes into U instead of the
+// Schur tiles' children lists), MFS_BUCKET_TILES ("t0,t1,t2": largest GEMM tile, 64 or 128, per
 // supernode size bucket); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 

@@ -212,6 +212,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
     if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_LOOKAHEAD")) opt.lookahead = atoi(e) != 0;
+    if (const char* e = getenv("MFS_SYRK_PULL")) opt.syrk_pull = atoi(e) != 0;
     if (const char* e = getenv("MFS_BUCKET_TILES")) {
         int t0, t1, t2;
         if (sscanf(e, "%d,%d,%d", &t0, &t1, &t2) == 3) {
@@ -869,16 +870,62 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 P.trailing_step.push_back(trailing ? 1 : 0);
             }
         }
-        // Schur update U -= L21 L21^T (lower tiles), K = W
+        // Schur update U -= L21 L21^T (lower tiles), K = W.  Each tile also lists, in child order,
+        // the children's update-matrix blocks that land in it (the part of the extend-add into U
+        // that the tile's epilogue can add instead of the U extend-add passes)
         {
             std::vector<GemmTask> big, small;
+            auto attach_pulls = [&](GemmTask& g, int32_t s) {
+                const int64_t D = 3 * (int64_t)S.sdeg[s];
+                const int64_t rel0 = g.c_off - S.u_off[s];
+                const int32_t rr = (int32_t)(rel0 % D), cc = (int32_t)(rel0 / D);
+                const int32_t Wb = S.swidth[s];
+                const int32_t begin = (int32_t)S.pull_ents.size();
+                if (!opt.syrk_pull || g.M > 128 || g.N > 128) { S.pulls_complete = false; return; }
+                for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
+                    const int32_t c = S.children[q];
+                    const int32_t dc = S.sdeg[c];
+                    if (dc == 0) continue;
+                    const int32_t* rl = S.rel.data() + S.rel_off[c];
+                    // parent block rows/columns whose scalars meet the tile: [lo, hi)
+                    auto range = [&](int32_t o, int32_t n, int32_t& b0, int32_t& nb) {
+                        const int32_t lo = Wb + o / 3, hi = Wb + (o + n - 1) / 3 + 1;
+                        b0 = (int32_t)(std::lower_bound(rl, rl + dc, lo) - rl);
+                        nb = (int32_t)(std::lower_bound(rl, rl + dc, hi) - rl) - b0;
+                    };
+                    PullEnt e{};
+                    range(rr, g.M, e.ib0, e.nib);
+                    range(cc, g.N, e.jb0, e.njb);
+                    if (e.nib <= 0 || e.njb <= 0 || e.ib0 + e.nib - 1 < e.jb0) continue;   // nothing lower
+                    if (e.nib > 64 || e.njb > 64) throw Error(-1, "internal: Schur tile pull exceeds 64 blocks");
+                    // the tile kernel addresses a child's matrix with 32-bit byte offsets
+                    if (9 * (int64_t)dc * dc * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
+                        S.pulls_complete = false;
+                        return;
+                    }
+                    e.u_child = S.u_off[c];
+                    e.dc3 = 3 * dc;
+                    e.rel_off = (int32_t)S.rel_off[c];
+                    e.base_r = 3 * Wb + rr;
+                    e.base_c = 3 * Wb + cc;
+                    S.pull_ents.push_back(e);
+                }
+                const int32_t n = (int32_t)S.pull_ents.size() - begin;
+                if (n > 0) {
+                    g.pad = begin;
+                    g.flags |= n << GT_PULL_SHIFT;
+                }
+            };
             for (int32_t s : sl) {
                 const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
                 if (D == 0) continue;
                 const int64_t off = S.panel_off[s];
+                const size_t nb0 = big.size(), ns0 = small.size();
                 // region rows/cols are indices into L21 (0..D); A = P[W + r, 0..W), C = U[r, c]
                 emit_lower(big, small, off + W, ld, off + W, ld, S.u_off[s], D, 0, D, D, W, GT_C_IN_U | GT_BETA0,
                            big_ok(s), /*c_is_u=*/true);
+                for (size_t q = nb0; q < big.size(); ++q) attach_pulls(big[q], s);
+                for (size_t q = ns0; q < small.size(); ++q) attach_pulls(small[q], s);
             }
             P.syrk = push_tasks(S.gemm_tasks, big, small);
         }

@@ -157,6 +157,38 @@ def test_lookahead_matches_in_order_schedule(dev, outer, single, monkeypatch):
     assert np.array_equal(factor(1, graph=True), L0)
 
 
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("N,fuse", [(10, "0"), (14, "0"), (14, "300")])
+def test_schur_tile_children_lists_match_extend_add(dev, N, fuse, single, monkeypatch):
+    """The Schur-update tiles add the children's update-matrix blocks listed on them in their
+    epilogue, in child order, instead of the extend-add passes into U after the update: the same
+    additions in the same order, so the factor is bitwise the one of the passes, eagerly and
+    replayed as a captured graph."""
+    H, pins, _, _, _ = mesh_problem(dev, N)
+    monkeypatch.setenv("MFS_FUSE_DEGREE", fuse)
+
+    def factor(pull, graph=False):
+        monkeypatch.setenv("MFS_SYRK_PULL", str(pull))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+        f.factorizeSymbolic(H, pins)
+        if graph:
+            f.setGraphCapture(True)
+        f.factorizeNumeric(H)
+        Lp, Li, Lx = (np.asarray(a).copy() for a in f.exportFactorCSC())
+        if graph:
+            f.factorizeNumeric(H)          # a replay of the captured graph
+            assert np.array_equal(np.asarray(f.exportFactorCSC()[2]), Lx)
+        return (Lp, Li, Lx), f.scheduleStats()["extend_launches"]
+
+    (Lp0, Li0, Lx0), e0 = factor(0)
+    (Lp1, Li1, Lx1), e1 = factor(1)
+    assert e1 < e0                         # the passes into U are gone
+    assert np.array_equal(Lp0, Lp1) and np.array_equal(Li0, Li1)
+    assert np.array_equal(Lx0.view(np.uint8), Lx1.view(np.uint8))
+    (_, _, Lx2), _ = factor(1, graph=True)
+    assert np.array_equal(Lx2.view(np.uint8), Lx0.view(np.uint8))
+
+
 def test_fused_panel_step_reports_failing_pivot(dev, monkeypatch):
     """-A through the fused steps: the pivot failure is recorded once by the storing workgroup
     and raised like in the separate launches."""
