@@ -320,7 +320,8 @@ int mfs_schedule_stats(mfs_factorizer f, int64_t* out) {
         }
         for (const auto& r : P.update_steps) gemm_launches += (r.m > r.b) + (r.e > r.m);
         gemm_launches += (P.syrk.m > P.syrk.b) + (P.syrk.e > P.syrk.m);
-        extend_launches += (int64_t)P.extend_passes.size() +
+        extend_launches += (f->dev->panel_pulls_active() ? (P.panel_pulls.second > P.panel_pulls.first)
+                                                          : (int64_t)P.extend_passes.size()) +
                            (f->dev->pulls_active() ? 0 : (int64_t)P.extend_u_passes.size());
     }
     out[0] = (int64_t)S.fused_tasks.size();

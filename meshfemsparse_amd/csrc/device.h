@@ -133,9 +133,9 @@ public:
     void require(int t) const;
     // the Schur-update tiles add the children's update-matrix blocks (FP64 and FP32 tile kernels,
     // every tile of the analysis carrying its list) instead of the extend-add passes into U
-    bool pulls_active() const {
-        return precision < 2 && sym && sym->pulls_complete && !sym->pull_ents.empty();
-    }
+    bool pulls_active() const { return precision < 2 && sym && sym->pulls_complete; }
+    // the panel part of the extend-add through the panel tiles' children lists (every form)
+    bool panel_pulls_active() const { return sym && sym->panel_pulls_complete; }
 
 private:
     void install(std::unique_ptr<Symbolic> S);
@@ -161,7 +161,8 @@ private:
 
     DeviceArray L_, U_, inv_, status_;
     DeviceArray planent_, diagsrc_, scale_, xf_, rel_, ext_, potrf_, gemm_, fused_, uoff_, panel_, pcount_;
-    DeviceArray pull_;                     // children's update-matrix shares of the Schur tiles
+    DeviceArray pull_;                     // children's update-matrix shares of the Schur and panel tiles
+    DeviceArray ppull_;                    // panel tiles with children shares
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_, f2up_, up2i_, cmask_;

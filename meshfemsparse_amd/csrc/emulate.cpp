@@ -155,16 +155,15 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
         children(true);
     };
     // the Schur-update tiles' own lists of children's update-matrix blocks, added after the tile
-    // (child order), in place of the extend-add passes into U; MFS_SYRK_PULL=0 selects the passes,
-    // MFS_SYRK_PULL=2 makes an analysis without complete lists an error (replay checks)
-    const char* pe = getenv("MFS_SYRK_PULL");
+    // (child order), in place of the extend-add passes into U, and the panel tiles' lists in place
+    // of the passes into the panels; MFS_PULL=0 selects the passes, MFS_PULL=2 makes an analysis
+    // without complete lists an error (replay checks)
+    const char* pe = getenv("MFS_PULL");
     const int pmode = pe ? atoi(pe) : 1;
-    const bool pull = S.pulls_complete && !S.pull_ents.empty() && pmode != 0;
-    if (pmode == 2 && !pull) throw Error(-1, "replay: the Schur tiles carry no complete children lists");
-    auto pull_tile = [&](const GemmTask& t) {
-        double* C = U.data() + t.c_off;
-        const int32_t n = t.flags >> GT_PULL_SHIFT;
-        for (int32_t e = t.pad; e < t.pad + n; ++e) {
+    const bool pull = S.pulls_complete && pmode != 0, ppull = S.panel_pulls_complete && pmode != 0;
+    if (pmode == 2 && !(pull && ppull)) throw Error(-1, "replay: the tiles carry no complete children lists");
+    auto add_lists = [&](double* C, int64_t ldc, int M, int N, int32_t e0, int32_t n) {
+        for (int32_t e = e0; e < e0 + n; ++e) {
             const PullEnt& p = S.pull_ents[e];
             const int32_t* rl = S.rel.data() + p.rel_off;
             for (int32_t jb = p.jb0; jb < p.jb0 + p.njb; ++jb)
@@ -173,15 +172,25 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                         for (int r = (ib == jb ? cs : 0); r < 3; ++r) {
                             const int64_t row = 3 * (int64_t)rl[ib] + r - p.base_r;
                             const int64_t col = 3 * (int64_t)rl[jb] + cs - p.base_c;
-                            if (row < 0 || row >= t.M || col < 0 || col >= t.N) continue;
-                            C[col * t.ldc + row] += U[p.u_child + (int64_t)(3 * jb + cs) * p.dc3 + 3 * ib + r];
+                            if (row < 0 || row >= M || col < 0 || col >= N) continue;
+                            C[col * ldc + row] += U[p.u_child + (int64_t)(3 * jb + cs) * p.dc3 + 3 * ib + r];
                         }
         }
     };
+    auto pull_tile = [&](const GemmTask& t) {
+        add_lists(U.data() + t.c_off, t.ldc, t.M, t.N, t.pad, t.flags >> GT_PULL_SHIFT);
+    };
     for (const LevelPlan& P : S.levels) {
         for (int64_t q = P.fused.first; q < P.fused.second; ++q) fused(S.fused_tasks[q]);
-        for (const auto& pass : P.extend_passes)
-            for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
+        if (ppull) {
+            for (int64_t q = P.panel_pulls.first; q < P.panel_pulls.second; ++q) {
+                const PanelPullTask& t = S.panel_pull_tasks[q];
+                add_lists(L.data() + t.c_off, t.ldc, t.M, t.N, t.pull_begin, t.npull);
+            }
+        } else {
+            for (const auto& pass : P.extend_passes)
+                for (int64_t q = pass.first; q < pass.second; ++q) extend_one(S.extend_tasks[q]);
+        }
         for (size_t st = 0; st < P.potrf_steps.size(); ++st) {
             for (int64_t q = P.potrf_steps[st].first; q < P.potrf_steps[st].second; ++q)
                 potrf(S.potrf_tasks[q]);

@@ -960,6 +960,69 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
     __syncthreads();         // map reads done before the epilogue reuses the region
 }
 
+// The children's update-matrix blocks that land in a 64 x 64 tile of a parent's panel, added to
+// the tile in child order: the additions, in the order, of the extend-add passes into the panel.
+// Thread (row = tid % 64, columns 4 q + tid / 64) holds 16 entries of the tile; per child the
+// block rows and columns are mapped to the child's scalar rows and columns through LDS, and every
+// entry with a child entry in the child's lower triangle loads it and adds it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_pull(const PanelPullTask* __restrict__ tasks,
+                                                    const PullEnt* __restrict__ pulls,
+                                                    const int32_t* __restrict__ rel, T* __restrict__ L,
+                                                    const T* __restrict__ U) {
+    __shared__ int map[128];   // rows, then columns: child scalar index or -1
+    const PanelPullTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, row = tid & 63, cq = tid >> 6;
+    T* C = L + t.c_off;
+    T val[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = 4 * q + cq;
+        val[q] = (row < t.M && col < t.N) ? C[(int64_t)col * t.ldc + row] : T(0);
+    }
+    for (int e = 0; e < t.npull; ++e) {
+        const PullEnt pu = pulls[t.pull_begin + e];
+        const int32_t* rl = rel + pu.rel_off;
+        __syncthreads();                       // the previous entry's map reads are done
+        if (tid < 128) map[tid] = -1;
+        __syncthreads();
+        if (tid < pu.nib) {
+            const int ib = pu.ib0 + tid, rb = 3 * rl[ib] - pu.base_r;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (rb + r >= 0 && rb + r < 64) map[rb + r] = 3 * ib + r;
+        } else if (tid >= 64 && tid - 64 < pu.njb) {
+            const int jb = pu.jb0 + tid - 64, cb = 3 * rl[jb] - pu.base_c;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (cb + r >= 0 && cb + r < 64) map[64 + cb + r] = 3 * jb + r;
+        }
+        __syncthreads();
+        const int ri = map[row];
+        const int nrec = pu.dc3 * pu.dc3 * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(U + pu.u_child), (short)0, nrec, kBufWord3);
+        int cj[16];
+        T v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            cj[q] = map[64 + 4 * q + cq];
+            // child lower triangle: scalar row >= scalar column (cj >= 0 then implies ri >= 0)
+            const int off = (cj[q] >= 0 && ri >= cj[q]) ? (cj[q] * pu.dc3 + ri) * (int)sizeof(T) : nrec;
+            v[q] = buf_load<T>(rs, off, 0);
+        }
+        // only entries with a child entry are added (an injected -0 stays -0, as in the passes)
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (cj[q] >= 0 && ri >= cj[q]) val[q] += v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = 4 * q + cq;
+        if (row < t.M && col < t.N) C[(int64_t)col * t.ldc + row] = val[q];
+    }
+}
+
 // (`final_vals`: acc already holds the values to store, as after apply_pulls)
 template <typename T, int FM, int FN, int NP>
 __device__ __forceinline__ void gemm_store_lds(const typename Mfma<T>::acc_t (&acc)[FM][FN], T* __restrict__ C,
@@ -2617,6 +2680,7 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     uoff_.upload(S->u_off);
     gemm_.upload(S->gemm_tasks);
     pull_.upload(S->pull_ents);
+    ppull_.upload(S->panel_pull_tasks);
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
     invt_.upload(S->inv_tiles);
@@ -2736,7 +2800,18 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
             LAUNCH_CHECK();
             prof.end(s);
         }
-        extend(P.extend_passes);
+        if (panel_pulls_active()) {
+            const int64_t n = P.panel_pulls.second - P.panel_pulls.first;
+            if (n > 0) {
+                prof.begin(KernelProfile::EXTEND, s, 0.0, n);
+                k_panel_pull<T><<<(unsigned)n, 256, 0, s>>>(ppull_.as<PanelPullTask>() + P.panel_pulls.first,
+                                                            pull_.as<PullEnt>(), rel_.as<int32_t>(), L, U);
+                LAUNCH_CHECK();
+                prof.end(s);
+            }
+        } else {
+            extend(P.extend_passes);
+        }
         for (size_t t = 0; t < P.potrf_steps.size(); ++t) {
             auto [p0, p1] = P.potrf_steps[t];
             const TaskRange& tr = P.trsm_steps[t];

@@ -128,6 +128,18 @@ struct PullEnt {
 };
 static_assert(sizeof(PullEnt) == 40, "PullEnt layout");
 
+// A tile (at most 64 x 64) of a parent's panel, lower part, with the children's update-matrix
+// blocks that land in it in the panel's columns: the tile is L[c_off + c ldc + r], r < M, c < N;
+// PullEnt entries [pull_begin, pull_begin + npull) in child order, base = the tile's origin in
+// panel coordinates (scalar row 3 rel + r of the parent's block rows, own columns first).
+struct PanelPullTask {
+    int64_t c_off;
+    int32_t ldc, M, N;
+    int32_t pull_begin, npull;
+    int32_t pad;
+};
+static_assert(sizeof(PanelPullTask) == 32, "PanelPullTask layout");
+
 struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], ld; inverse -> slot
     int64_t diag_off;
     int32_t ld, kb;
@@ -221,6 +233,8 @@ struct LevelPlan {
     std::pair<int64_t,int64_t> fused{0, 0};                  // range in fused tasks (one launch)
     std::vector<std::pair<int64_t,int64_t>> extend_passes;   // ranges into extend tasks (into panels)
     std::vector<std::pair<int64_t,int64_t>> extend_u_passes; // after the Schur update (into U)
+    std::pair<int64_t,int64_t> panel_pulls{0, 0};            // the panel part as tile lists (one
+                                                             // launch, in place of extend_passes)
     std::vector<std::pair<int64_t,int64_t>> potrf_steps;     // per step: range in potrf tasks
     std::vector<TaskRange> trsm_steps;                       // per step: gemm tasks
     std::vector<std::pair<int64_t,int64_t>> panel_steps;     // per step: range in panel items (empty:
@@ -285,8 +299,11 @@ struct Symbolic {
     std::vector<PotrfTask> potrf_tasks;
     std::vector<PanelItem> panel_items;
     std::vector<GemmTask> gemm_tasks;
-    std::vector<PullEnt> pull_ents;            // children's update-matrix shares of the Schur tiles
+    std::vector<PullEnt> pull_ents;            // children's update-matrix shares of the Schur
+                                               // tiles and of the panel tiles
     bool pulls_complete = true;                // every Schur tile lists its shares (tiles <= 128)
+    std::vector<PanelPullTask> panel_pull_tasks;
+    bool panel_pulls_complete = true;          // the panel tiles list every panel share
     std::vector<FusedTask> fused_tasks;
     // solves
     int32_t solve_block = 1024;                // column-block width (scalars, multiple of 64)
@@ -345,9 +362,10 @@ struct SymbolicOptions {
                             // first, the rest beside the next outer block's steps
     int panel_max = 384;    // a step whose TRSM needs at most this many row tiles (and POTRF
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
-    bool syrk_pull = true;  // Schur-update tiles list the children's update-matrix blocks that
-                            // land in them (added in the tile's epilogue instead of by the
-                            // extend-add passes into U; FP64 and FP32 tile kernels)
+    bool pull_lists = true; // children lists in place of the extend-add passes: Schur-update
+                            // tiles list the children's update-matrix blocks that land in them
+                            // (added in the tile's epilogue; FP64 and FP32 tile kernels), and
+                            // 64 x 64 panel tiles those that land in the parent's panel
     int64_t n_scalar = -1;  // scalar length of the caller's matrix when it is not 3 nb (a scalar
                             // matrix grouped into blocks: scalars n_scalar .. 3 nb - 1 are padding)
     const std::vector<int32_t>* given_perm = nullptr;   // reuse this ordering (perm[new] = old)
@@ -364,8 +382,8 @@ struct SymbolicOptions {
 // thresholds), MFS_TILE_GROUP (GEMM tile-column group), MFS_FUSE_DEGREE
 // (fused level kernel bound), MFS_BIG_TILE (regions at least this large in both dimensions use
 // 128x128 tiles), MFS_PANEL_MAX (fused diagonal-tile + TRSM bound), MFS_LOOKAHEAD (0: no split
-// of the outer trailing updates), MFS_SYRK_PULL (0: extend-add passes into U instead of the
-// Schur tiles' children lists), MFS_BUCKET_TILES ("t0,t1,t2": largest GEMM tile, 64 or 128, per
+// of the outer trailing updates), MFS_PULL (0: extend-add passes instead of the Schur and panel
+// tiles' children lists), MFS_BUCKET_TILES ("t0,t1,t2": largest GEMM tile, 64 or 128, per
 // supernode size bucket); values are rounded down to multiples of 64 where required.
 SymbolicOptions options_from_env();
 

@@ -212,7 +212,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
     if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_LOOKAHEAD")) opt.lookahead = atoi(e) != 0;
-    if (const char* e = getenv("MFS_SYRK_PULL")) opt.syrk_pull = atoi(e) != 0;
+    if (const char* e = getenv("MFS_PULL")) opt.pull_lists = atoi(e) != 0;
     if (const char* e = getenv("MFS_BUCKET_TILES")) {
         int t0, t1, t2;
         if (sscanf(e, "%d,%d,%d", &t0, &t1, &t2) == 3) {
@@ -580,6 +580,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     S.u_off.assign(ns, -1);
     Arena arena;
     S.levels.resize(S.num_levels);
+    S.pulls_complete = S.panel_pulls_complete = opt.pull_lists;
     S.u_solve_off.assign(ns, 0);
     {
         int64_t acc = 0;
@@ -759,6 +760,53 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
         }
         };
         extend_passes(false, P.extend_passes);
+        // the same panel part as children lists of the parents' 64 x 64 panel tiles (lower part):
+        // per tile and child, the child's block rows and panel block columns that meet the tile
+        if (S.panel_pulls_complete) {
+            const int64_t t0 = (int64_t)S.panel_pull_tasks.size();
+            for (int32_t p : sl) {
+                const int32_t Wb = S.swidth[p], W = 3 * Wb, ld = 3 * (Wb + S.sdeg[p]);
+                bool any = false;      // a child with a column block in the panel (rel ascending)
+                for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
+                    const int32_t c = S.children[q];
+                    if (S.sdeg[c] > 0 && S.rel[S.rel_off[c]] < Wb) any = true;
+                    if (9 * (int64_t)S.sdeg[c] * S.sdeg[c] * (int64_t)sizeof(double) >= ((int64_t)1 << 31))
+                        S.panel_pulls_complete = false;   // 32-bit byte offsets into the child's matrix
+                }
+                if (!any || !S.panel_pulls_complete) continue;
+                for (int32_t c0 = 0; c0 < W; c0 += 64)
+                    for (int32_t r0 = c0; r0 < ld; r0 += 64) {
+                        const int32_t M = std::min(64, ld - r0), N = std::min(64, W - c0);
+                        const int32_t begin = (int32_t)S.pull_ents.size();
+                        for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
+                            const int32_t c = S.children[q];
+                            const int32_t dc = S.sdeg[c];
+                            if (dc == 0) continue;
+                            const int32_t* rl = S.rel.data() + S.rel_off[c];
+                            auto range = [&](int32_t o, int32_t n, int32_t& b0, int32_t& nb) {
+                                const int32_t lo = o / 3, hi = (o + n - 1) / 3 + 1;
+                                b0 = (int32_t)(std::lower_bound(rl, rl + dc, lo) - rl);
+                                nb = (int32_t)(std::lower_bound(rl, rl + dc, hi) - rl) - b0;
+                            };
+                            PullEnt e{};
+                            range(r0, M, e.ib0, e.nib);
+                            range(c0, N, e.jb0, e.njb);
+                            if (e.nib <= 0 || e.njb <= 0 || e.ib0 + e.nib - 1 < e.jb0) continue;
+                            e.u_child = S.u_off[c];
+                            e.dc3 = 3 * dc;
+                            e.rel_off = (int32_t)S.rel_off[c];
+                            e.base_r = r0;
+                            e.base_c = c0;
+                            S.pull_ents.push_back(e);
+                        }
+                        const int32_t n = (int32_t)S.pull_ents.size() - begin;
+                        if (n > 0)
+                            S.panel_pull_tasks.push_back(
+                                {S.panel_off[p] + (int64_t)c0 * ld + r0, ld, M, N, begin, n, 0});
+                    }
+            }
+            P.panel_pulls = {t0, (int64_t)S.panel_pull_tasks.size()};
+        }
         // release children's U (consumed by this level's extend-add; the U passes run at the end
         // of the level and nothing else is allocated in between)
         for (int32_t p : sl_all)
@@ -881,7 +929,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 const int32_t rr = (int32_t)(rel0 % D), cc = (int32_t)(rel0 / D);
                 const int32_t Wb = S.swidth[s];
                 const int32_t begin = (int32_t)S.pull_ents.size();
-                if (!opt.syrk_pull || g.M > 128 || g.N > 128) { S.pulls_complete = false; return; }
+                if (!S.pulls_complete || g.M > 128 || g.N > 128) { S.pulls_complete = false; return; }
                 for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
                     const int32_t c = S.children[q];
                     const int32_t dc = S.sdeg[c];

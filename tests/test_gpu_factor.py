@@ -157,19 +157,21 @@ def test_lookahead_matches_in_order_schedule(dev, outer, single, monkeypatch):
     assert np.array_equal(factor(1, graph=True), L0)
 
 
-@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("N,fuse", [(10, "0"), (14, "0"), (14, "300")])
-def test_schur_tile_children_lists_match_extend_add(dev, N, fuse, single, monkeypatch):
+def test_tile_children_lists_match_extend_add(dev, N, fuse, form, monkeypatch):
     """The Schur-update tiles add the children's update-matrix blocks listed on them in their
-    epilogue, in child order, instead of the extend-add passes into U after the update: the same
-    additions in the same order, so the factor is bitwise the one of the passes, eagerly and
-    replayed as a captured graph."""
+    epilogue (FP64 and FP32 forms), and the panel tiles those that land in the parents' panels
+    (every form), in child order, instead of the extend-add passes: the same additions in the
+    same order, so the factor is bitwise the one of the passes, eagerly and replayed as a captured
+    graph."""
     H, pins, _, _, _ = mesh_problem(dev, N)
     monkeypatch.setenv("MFS_FUSE_DEGREE", fuse)
 
     def factor(pull, graph=False):
-        monkeypatch.setenv("MFS_SYRK_PULL", str(pull))
-        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
+        monkeypatch.setenv("MFS_PULL", str(pull))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, form != "fp64",
+                                         "bf16x3" if form == "bf16x3" else None)
         f.factorizeSymbolic(H, pins)
         if graph:
             f.setGraphCapture(True)

@@ -298,11 +298,11 @@ def test_device_schedule_replay_matches_oracle(N, sigma, outer, fuse, big, monke
 
 
 @pytest.mark.parametrize("N,outer,fuse", [(5, None, 0), (8, 128, 0), (9, None, 96), (10, 64, 300)])
-def test_schur_tile_children_lists_match_extend_add(N, outer, fuse, monkeypatch):
-    """The children's update-matrix blocks listed on each Schur-update tile, added after the tile
-    in child order, give the same factor, bit for bit, as the extend-add passes into the update
-    matrices (the replay starts the update arena from NaNs, so an entry a list misses or adds
-    twice changes the factor)."""
+def test_tile_children_lists_match_extend_add(N, outer, fuse, monkeypatch):
+    """The children's update-matrix blocks listed on each Schur-update tile and each 64 x 64
+    panel tile, added to the tile in child order, give the same factor, bit for bit, as the
+    extend-add passes into the update matrices and panels (the replay starts the update arena
+    from NaNs, so an entry a list misses or adds twice changes the factor)."""
     if outer:
         monkeypatch.setenv("MFS_OUTER", str(outer))
     monkeypatch.setenv("MFS_FUSE_DEGREE", str(fuse))
@@ -312,9 +312,9 @@ def test_schur_tile_children_lists_match_extend_add(N, outer, fuse, monkeypatch)
     H = mfs.SystemAssembler(len(Xr)).blockSparsityPattern(tets)
     Ax = orc.assemble(H.Ap, H.Ai, H.n, tets, He)
     pins = wl.z0_pins(N)
-    monkeypatch.setenv("MFS_SYRK_PULL", "0")
+    monkeypatch.setenv("MFS_PULL", "0")
     _, (_, _, Lx0), fc0 = emulate(H, pins, Ax)
-    monkeypatch.setenv("MFS_SYRK_PULL", "2")
+    monkeypatch.setenv("MFS_PULL", "2")
     _, (_, _, Lx1), fc1 = emulate(H, pins, Ax)
     assert fc0 == fc1 == -1
     assert np.array_equal(Lx0.view(np.uint64), Lx1.view(np.uint64))
