@@ -19,13 +19,9 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
     // entry that was not written first shows up in the result
     std::vector<double> U(S.U_size, std::nan(""));
     int64_t status = INT64_MAX;
-    // injection
-    for (size_t q = 0; q < S.plan_src.size(); ++q) {
-        const int64_t dst = S.plan_dst[q] >> 2;
+    // injection: put(r, c, v) for the stored entries of plan entry q's 3x3 block
+    auto inject_entry = [&](size_t q, auto&& put) {
         const int mode = (int)(S.plan_dst[q] & 3);
-        const int32_t s = (int32_t)(std::upper_bound(S.panel_off.begin(), S.panel_off.end(), dst) -
-                                    S.panel_off.begin()) - 1;
-        const int64_t ld = 3 * ((int64_t)S.swidth[s] + S.sdeg[s]);
         const double* a = Ax + 9 * (int64_t)S.plan_src[q];
         const int mr = S.masked() ? S.comp_mask[S.plan_row[q]] : 0;
         const int mc = S.masked() ? S.comp_mask[S.plan_col[q]] : 0;
@@ -35,8 +31,15 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
                 double v = mode == 0 ? a[3 * c + r] : a[3 * r + c];
                 if (mode == 2 && r == c) v += sigma;
                 if (((mr >> r) | (mc >> c)) & 1) v = (mode == 2 && r == c) ? 1.0 : 0.0;   // masked: identity
-                L[dst + c * ld + r] = v;
+                put(r, c, v);
             }
+    };
+    for (size_t q = 0; q < S.plan_src.size(); ++q) {
+        const int64_t dst = S.plan_dst[q] >> 2;
+        const int32_t s = (int32_t)(std::upper_bound(S.panel_off.begin(), S.panel_off.end(), dst) -
+                                    S.panel_off.begin()) - 1;
+        const int64_t ld = 3 * ((int64_t)S.swidth[s] + S.sdeg[s]);
+        inject_entry(q, [&](int r, int c, double v) { L[dst + c * ld + r] = v; });
     }
     auto gemm = [&](const GemmTask& t) {
         const double* A = L.data() + t.a_off;
@@ -185,7 +188,18 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
         if (ppull) {
             for (int64_t q = P.panel_pulls.first; q < P.panel_pulls.second; ++q) {
                 const PanelPullTask& t = S.panel_pull_tasks[q];
-                add_lists(L.data() + t.c_off, t.ldc, t.M, t.N, t.pull_begin, t.npull);
+                // the tile rebuilt from its injection entries (zero elsewhere), then the lists
+                double* C = L.data() + t.c_off;
+                for (int c = 0; c < t.N; ++c)
+                    for (int r = 0; r < t.M; ++r) C[(int64_t)c * t.ldc + r] = 0.0;
+                for (int32_t i = t.inj_begin; i < t.inj_begin + t.ninj; ++i) {
+                    const TileInj& ti = S.tile_inj[i];
+                    inject_entry((size_t)ti.plan, [&](int r, int c, double v) {
+                        const int lr = ti.lr + r, lc = ti.lc + c;
+                        if (lr >= 0 && lr < t.M && lc >= 0 && lc < t.N) C[(int64_t)lc * t.ldc + lr] = v;
+                    });
+                }
+                add_lists(C, t.ldc, t.M, t.N, t.pull_begin, t.npull);
             }
         } else {
             for (const auto& pass : P.extend_passes)

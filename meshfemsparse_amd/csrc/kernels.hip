@@ -442,21 +442,17 @@ struct PlanEnt {
 // cmask (NULL: none) flags masked components per factor-order block (bit c): pinned scalars of a
 // partially pinned block and the padding of a scalar matrix; their rows and columns become the
 // identity, so the free scalars factor as the reduced matrix.
-template <typename T>
-__global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan, int64_t n,
-                                                const double* __restrict__ Ax,
-                                                const double* __restrict__ Bx, double sigma,
-                                                const double* __restrict__ scale,
-                                                const uint8_t* __restrict__ cmask,
-                                                T* __restrict__ L) {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const PlanEnt pe = plan[q];
+// The values of one injection entry: put(r, c, v) for every stored entry (r, c) of the 3x3 block
+// (a diagonal block's lower part), v = (A + sigma B or sigma I), symmetrically scaled, identity on
+// masked components, rounded to T once.
+template <typename T, typename Put>
+__device__ __forceinline__ void inject_block(const PlanEnt& pe, const double* __restrict__ Ax,
+                                             const double* __restrict__ Bx, double sigma,
+                                             const double* __restrict__ scale,
+                                             const uint8_t* __restrict__ cmask, Put put) {
     const int mode = (int)(pe.dstmode & 3);
-    T* d = L + (pe.dstmode >> 2);
     const double* a = Ax + 9 * (int64_t)pe.src;
     const double* bb = Bx ? Bx + 9 * (int64_t)pe.src : nullptr;
-    const int64_t ld = pe.ld;
     const int mr = cmask ? cmask[pe.rb] : 0, mc = cmask ? cmask[pe.cb] : 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -469,8 +465,22 @@ __global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan
             else if (mode == 2 && r == c) v += sigma;
             if (scale) v *= scale[3 * (int64_t)pe.rb + r] * scale[3 * (int64_t)pe.cb + c];
             if (((mr >> r) | (mc >> c)) & 1) v = (mode == 2 && r == c) ? 1.0 : 0.0;
-            d[c * ld + r] = (T)v;
+            put(r, c, (T)v);
         }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_inject(const PlanEnt* __restrict__ plan, int64_t n,
+                                                const double* __restrict__ Ax,
+                                                const double* __restrict__ Bx, double sigma,
+                                                const double* __restrict__ scale,
+                                                const uint8_t* __restrict__ cmask,
+                                                T* __restrict__ L) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const PlanEnt pe = plan[q];
+    T* d = L + (pe.dstmode >> 2);
+    inject_block<T>(pe, Ax, Bx, sigma, scale, cmask, [&](int r, int c, T v) { d[c * (int64_t)pe.ld + r] = v; });
 }
 
 // s_k = 1 / sqrt((A + shift)_kk) in factor order; a non-positive diagonal is a pivot failure
@@ -962,24 +972,42 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
 
 // The children's update-matrix blocks that land in a 64 x 64 tile of a parent's panel, added to
 // the tile in child order: the additions, in the order, of the extend-add passes into the panel.
-// Thread (row = tid % 64, columns 4 q + tid / 64) holds 16 entries of the tile; per child the
-// block rows and columns are mapped to the child's scalar rows and columns through LDS, and every
-// entry with a child entry in the child's lower triangle loads it and adds it.
+// The tile's starting values are rebuilt in LDS from its injection entries (zero elsewhere: the
+// values the fill and the injection left in L), so L is only written.  Thread (row = tid % 64,
+// columns 4 q + tid / 64) then holds 16 entries of the tile; per child the block rows and columns
+// are mapped to the child's scalar rows and columns through LDS, and every entry with a child
+// entry in the child's lower triangle loads it and adds it.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_pull(const PanelPullTask* __restrict__ tasks,
                                                     const PullEnt* __restrict__ pulls,
-                                                    const int32_t* __restrict__ rel, T* __restrict__ L,
-                                                    const T* __restrict__ U) {
+                                                    const int32_t* __restrict__ rel,
+                                                    const TileInj* __restrict__ tinj,
+                                                    const PlanEnt* __restrict__ plan,
+                                                    const double* __restrict__ Ax,
+                                                    const double* __restrict__ Bx, double sigma,
+                                                    const double* __restrict__ scale,
+                                                    const uint8_t* __restrict__ cmask,
+                                                    T* __restrict__ L, const T* __restrict__ U) {
+    __shared__ T tile[64 * 64];
     __shared__ int map[128];   // rows, then columns: child scalar index or -1
     const PanelPullTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, row = tid & 63, cq = tid >> 6;
     T* C = L + t.c_off;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tile[(4 * q + cq) * 64 + row] = T(0);
+    __syncthreads();
+    for (int i = tid; i < t.ninj; i += 256) {     // distinct blocks: no two threads share an entry
+        const TileInj ti = tinj[t.inj_begin + i];
+        const PlanEnt pe = plan[ti.plan];
+        inject_block<T>(pe, Ax, Bx, sigma, scale, cmask, [&](int r, int c, T v) {
+            const int lr = ti.lr + r, lc = ti.lc + c;
+            if (lr >= 0 && lr < 64 && lc >= 0 && lc < 64) tile[lc * 64 + lr] = v;
+        });
+    }
+    __syncthreads();
     T val[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int col = 4 * q + cq;
-        val[q] = (row < t.M && col < t.N) ? C[(int64_t)col * t.ldc + row] : T(0);
-    }
+    for (int q = 0; q < 16; ++q) val[q] = tile[(4 * q + cq) * 64 + row];
     for (int e = 0; e < t.npull; ++e) {
         const PullEnt pu = pulls[t.pull_begin + e];
         const int32_t* rl = rel + pu.rel_off;
@@ -2681,6 +2709,7 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     gemm_.upload(S->gemm_tasks);
     pull_.upload(S->pull_ents);
     ppull_.upload(S->panel_pull_tasks);
+    tinj_.upload(S->tile_inj);
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
     invt_.upload(S->inv_tiles);
@@ -2804,8 +2833,9 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
             const int64_t n = P.panel_pulls.second - P.panel_pulls.first;
             if (n > 0) {
                 prof.begin(KernelProfile::EXTEND, s, 0.0, n);
-                k_panel_pull<T><<<(unsigned)n, 256, 0, s>>>(ppull_.as<PanelPullTask>() + P.panel_pulls.first,
-                                                            pull_.as<PullEnt>(), rel_.as<int32_t>(), L, U);
+                k_panel_pull<T><<<(unsigned)n, 256, 0, s>>>(
+                    ppull_.as<PanelPullTask>() + P.panel_pulls.first, pull_.as<PullEnt>(), rel_.as<int32_t>(),
+                    tinj_.as<TileInj>(), planent_.as<PlanEnt>(), Ax, Bx, sigma, scale, cmask(), L, U);
                 LAUNCH_CHECK();
                 prof.end(s);
             }

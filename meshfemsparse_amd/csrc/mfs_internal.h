@@ -131,14 +131,25 @@ static_assert(sizeof(PullEnt) == 40, "PullEnt layout");
 // A tile (at most 64 x 64) of a parent's panel, lower part, with the children's update-matrix
 // blocks that land in it in the panel's columns: the tile is L[c_off + c ldc + r], r < M, c < N;
 // PullEnt entries [pull_begin, pull_begin + npull) in child order, base = the tile's origin in
-// panel coordinates (scalar row 3 rel + r of the parent's block rows, own columns first).
+// panel coordinates (scalar row 3 rel + r of the parent's block rows, own columns first).  The
+// tile is rebuilt from its matrix values (TileInj entries [inj_begin, inj_begin + ninj): the
+// injection entries whose 3x3 block meets it; every other entry of the tile is 0) plus the
+// children's blocks, so its previous contents are never read.
 struct PanelPullTask {
     int64_t c_off;
     int32_t ldc, M, N;
     int32_t pull_begin, npull;
+    int32_t inj_begin, ninj;
     int32_t pad;
 };
-static_assert(sizeof(PanelPullTask) == 32, "PanelPullTask layout");
+static_assert(sizeof(PanelPullTask) == 40, "PanelPullTask layout");
+// An injection entry of a panel tile: plan index and the block's first scalar row / column
+// relative to the tile's origin (-2 .. 63).
+struct TileInj {
+    int32_t plan;
+    int16_t lr, lc;
+};
+static_assert(sizeof(TileInj) == 8, "TileInj layout");
 
 struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], ld; inverse -> slot
     int64_t diag_off;
@@ -303,6 +314,7 @@ struct Symbolic {
                                                // tiles and of the panel tiles
     bool pulls_complete = true;                // every Schur tile lists its shares (tiles <= 128)
     std::vector<PanelPullTask> panel_pull_tasks;
+    std::vector<TileInj> tile_inj;             // injection entries of the panel tiles
     bool panel_pulls_complete = true;          // the panel tiles list every panel share
     std::vector<FusedTask> fused_tasks;
     // solves

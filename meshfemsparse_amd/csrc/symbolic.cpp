@@ -581,6 +581,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     Arena arena;
     S.levels.resize(S.num_levels);
     S.pulls_complete = S.panel_pulls_complete = opt.pull_lists;
+    std::vector<int32_t> tile_owner;   // per panel pull task: its supernode
     S.u_solve_off.assign(ns, 0);
     {
         int64_t acc = 0;
@@ -800,9 +801,11 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                             S.pull_ents.push_back(e);
                         }
                         const int32_t n = (int32_t)S.pull_ents.size() - begin;
-                        if (n > 0)
+                        if (n > 0) {
                             S.panel_pull_tasks.push_back(
-                                {S.panel_off[p] + (int64_t)c0 * ld + r0, ld, M, N, begin, n, 0});
+                                {S.panel_off[p] + (int64_t)c0 * ld + r0, ld, M, N, begin, n, 0, 0, 0});
+                            tile_owner.push_back(p);
+                        }
                     }
             }
             P.panel_pulls = {t0, (int64_t)S.panel_pull_tasks.size()};
@@ -1106,6 +1109,47 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     S.U_size = std::max<int64_t>(arena.peak, 32);
     S.inv_slots = std::max<int64_t>((int64_t)S.potrf_tasks.size(), 1);
     S.bpart_size = std::max<int64_t>(max_part, 1);
+
+    // ---- injection entries of the panel tiles (every 3x3 block that meets a tile) ----
+    if (S.panel_pulls_complete && !S.panel_pull_tasks.empty()) {
+        std::vector<std::vector<int64_t>> plan_of(ns);     // injection entries per supernode
+        for (size_t q = 0; q < S.plan_dst.size(); ++q) {
+            const int64_t dst = S.plan_dst[q] >> 2;
+            const int32_t s = (int32_t)(std::upper_bound(S.panel_off.begin(), S.panel_off.end(), dst) -
+                                        S.panel_off.begin()) - 1;
+            plan_of[s].push_back((int64_t)q);
+        }
+        std::vector<std::vector<TileInj>> lists(S.panel_pull_tasks.size());
+        std::vector<int32_t> tid;                           // per tile of a panel: task or -1
+        for (size_t a = 0; a < S.panel_pull_tasks.size();) {
+            const int32_t p = tile_owner[a];
+            size_t b = a;
+            while (b < S.panel_pull_tasks.size() && tile_owner[b] == p) ++b;
+            const int64_t ld = 3 * ((int64_t)S.swidth[p] + S.sdeg[p]);
+            const int32_t ntc = (3 * S.swidth[p] + 63) / 64, ntr = (int32_t)((ld + 63) / 64);
+            tid.assign((size_t)ntc * ntr, -1);
+            for (size_t q = a; q < b; ++q) {
+                const int64_t o = S.panel_pull_tasks[q].c_off - S.panel_off[p];
+                tid[(size_t)(o / ld / 64) * ntr + (size_t)(o % ld / 64)] = (int32_t)q;
+            }
+            for (int64_t q : plan_of[p]) {
+                const int64_t o = (S.plan_dst[q] >> 2) - S.panel_off[p];
+                const int32_t row = (int32_t)(o % ld), col = (int32_t)(o / ld);
+                for (int32_t tc = col / 64; tc <= (col + 2) / 64 && tc < ntc; ++tc)
+                    for (int32_t tr = row / 64; tr <= (row + 2) / 64 && tr < ntr; ++tr) {
+                        const int32_t k = tid[(size_t)tc * ntr + tr];
+                        if (k >= 0)
+                            lists[k].push_back({(int32_t)q, (int16_t)(row - 64 * tr), (int16_t)(col - 64 * tc)});
+                    }
+            }
+            a = b;
+        }
+        for (size_t k = 0; k < lists.size(); ++k) {
+            S.panel_pull_tasks[k].inj_begin = (int32_t)S.tile_inj.size();
+            S.panel_pull_tasks[k].ninj = (int32_t)lists[k].size();
+            S.tile_inj.insert(S.tile_inj.end(), lists[k].begin(), lists[k].end());
+        }
+    }
 
     // ---- symmetric SpMV map over the full pattern (block-CSR, both triangles) ----
     {
