@@ -131,9 +131,9 @@ public:
     int precision;
 
     void require(int t) const;
-    // the Schur-update tiles add the children's update-matrix blocks (FP64 and FP32 tile kernels,
-    // every tile of the analysis carrying its list) instead of the extend-add passes into U
-    bool pulls_active() const { return precision < 2 && sym && sym->pulls_complete; }
+    // the Schur-update tiles add the children's update-matrix blocks (every tile of the analysis
+    // carrying its list) instead of the extend-add passes into U
+    bool pulls_active() const { return sym && sym->pulls_complete; }
     // the panel part of the extend-add through the panel tiles' children lists (every form)
     bool panel_pulls_active() const { return sym && sym->panel_pulls_complete; }
 

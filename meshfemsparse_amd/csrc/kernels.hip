@@ -921,17 +921,25 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
     for (int e = 0; e < npull; ++e) {
         const PullEnt pu = pe[e];
         const int32_t* rl = rel + pu.rel_off;
-        const int ib = pu.ib0 + lane, jb = pu.jb0 + lane;
-        // wave-relative scalar row / column of the block's first scalar (far out of range: no block)
-        const int rb = lane < pu.nib ? 3 * rl[ib] - pu.base_r - wm : -(1 << 20);
-        const int cb = lane < pu.njb ? 3 * rl[jb] - pu.base_c - wn : -(1 << 20);
+        // wave-relative scalar row / column of each block's first scalar (lane b: blocks b, b + 64)
+        int rb[2], cb[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int b = lane + 64 * h;
+            rb[h] = b < pu.nib ? 3 * rl[pu.ib0 + b] - pu.base_r - wm : -(1 << 20);
+            cb[h] = b < pu.njb ? 3 * rl[pu.jb0 + b] - pu.base_c - wn : -(1 << 20);
+        }
         __syncthreads();     // the previous entry's map reads are done (npull is uniform per tile)
         for (int q = lane; q < WM + WN; q += 64) map[q] = -1;
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            if (rb + r >= 0 && rb + r < WM) rmap[rb + r] = 3 * ib + r;
-            if (cb + r >= 0 && cb + r < WN) cmap[cb + r] = 3 * jb + r;
+        for (int h = 0; h < 2; ++h) {
+            const int ib = pu.ib0 + lane + 64 * h, jb = pu.jb0 + lane + 64 * h;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (rb[h] + r >= 0 && rb[h] + r < WM) rmap[rb[h] + r] = 3 * ib + r;
+                if (cb[h] + r >= 0 && cb[h] + r < WN) cmap[cb[h] + r] = 3 * jb + r;
+            }
         }
         __syncthreads();
         // buffer loads based at the child's matrix: 32-bit offsets, and an element outside the
@@ -1677,10 +1685,12 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 // NT threads (2 x NT/128 waves): 256 for 128x128 and 64x64 tiles, 512 for 256x256 tiles (half the
 // operand loads and conversions per flop; its two LDS buffers fill a CU's 160 KB).
-template <int TM, int TN, bool SPLIT, int NT = 256>
+template <int TM, int TN, bool SPLIT, int NT = 256, bool PULL = false>
 __global__ __launch_bounds__(NT, 512 / NT) void k_gemm_bf(const GemmTask* __restrict__ tasks,
                                                           float* __restrict__ L, float* __restrict__ U,
-                                                          const float* __restrict__ inv, int remap) {
+                                                          const float* __restrict__ inv, int remap,
+                                                          const PullEnt* __restrict__ pulls,
+                                                          const int32_t* __restrict__ rel) {
     typedef f4 acc_t;
     constexpr int KC = 32, LDK = 40, NP = SPLIT ? 2 : 1;
     constexpr int WGY = NT / 128;                          // wave columns
@@ -1799,28 +1809,38 @@ __global__ __launch_bounds__(NT, 512 / NT) void k_gemm_bf(const GemmTask* __rest
     // C through LDS one 16-column fragment per part (each wave's region in the operand buffers)
     constexpr int RE = WM * 16;
     static_assert((NT / 64) * RE * 4 <= 2 * STAGE * 2, "epilogue regions exceed the operand buffers");
+    float* region = reinterpret_cast<float*>(smem) + wave * RE;
+    // a Schur-update tile's children lists (the Schur-update instantiation only)
+    const int npull = PULL ? (t.flags >> GT_PULL_SHIFT) : 0;
+    if (npull > 0)
+        apply_pulls<float, FM, FN>(acc, alpha, reinterpret_cast<int*>(region), pulls + t.pad, npull, rel, U, wm,
+                                   wn, lane);
     gemm_store_lds<float, FM, FN, FN>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, (t.flags & GT_BETA0) != 0,
-                                      reinterpret_cast<float*>(smem) + wave * RE);
+                                      region, npull > 0);
 }
 
 // big256: the range's large tiles are 256x256 (Schur ranges of an analysis with schur_tile 256)
+template <int TM, int TN, int NT = 256>
+static void launch_gemm_bf1(bool split, unsigned grid, const GemmTask* tasks, float* L, float* U, const float* inv,
+                            int rm, const PullEnt* pulls, const int32_t* rel, hipStream_t s) {
+    if (pulls) {
+        if (split) k_gemm_bf<TM, TN, true, NT, true><<<grid, NT, 0, s>>>(tasks, L, U, inv, rm, pulls, rel);
+        else k_gemm_bf<TM, TN, false, NT, true><<<grid, NT, 0, s>>>(tasks, L, U, inv, rm, pulls, rel);
+    } else {
+        if (split) k_gemm_bf<TM, TN, true, NT><<<grid, NT, 0, s>>>(tasks, L, U, inv, rm, nullptr, nullptr);
+        else k_gemm_bf<TM, TN, false, NT><<<grid, NT, 0, s>>>(tasks, L, U, inv, rm, nullptr, nullptr);
+    }
+    LAUNCH_CHECK();
+}
 static void launch_gemm_bf(const TaskRange& r, const GemmTask* tasks, float* L, float* U, const float* inv,
-                           bool split, bool big256, hipStream_t s) {
+                           bool split, bool big256, hipStream_t s, const PullEnt* pulls = nullptr,
+                           const int32_t* rel = nullptr) {
     const int rm = gemm_remap(2);
-    if (r.m > r.b && big256) {
-        if (split) k_gemm_bf<256, 256, true, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        else k_gemm_bf<256, 256, false, 512><<<(unsigned)(r.m - r.b), 512, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        LAUNCH_CHECK();
-    } else if (r.m > r.b) {
-        if (split) k_gemm_bf<128, 128, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        else k_gemm_bf<128, 128, false><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm);
-        LAUNCH_CHECK();
-    }
-    if (r.e > r.m) {
-        if (split) k_gemm_bf<64, 64, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
-        else k_gemm_bf<64, 64, false><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm);
-        LAUNCH_CHECK();
-    }
+    if (r.m > r.b && big256)
+        launch_gemm_bf1<256, 256, 512>(split, (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, pulls, rel, s);
+    else if (r.m > r.b)
+        launch_gemm_bf1<128, 128>(split, (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, pulls, rel, s);
+    if (r.e > r.m) launch_gemm_bf1<64, 64>(split, (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, pulls, rel, s);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2776,7 +2796,7 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     if (ev_inject) HIP_CHECK(hipEventRecord((hipEvent_t)ev_inject, s));
     const GemmTask* gt = gemm_.as<GemmTask>();
     // the Schur-update tiles add the children's update-matrix blocks in their epilogue (in place
-    // of the extend-add passes into U) where every tile carries its list (FP64 and FP32 tiles)
+    // of the extend-add passes into U) where every tile carries its list
     const bool pulls = pulls_active();
     // Schur updates (in-panel and update matrix) of the BF16 variants go through the bfloat16
     // tile kernel; TRSM and the diagonal tiles stay in the factor type
@@ -2785,7 +2805,8 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
         if constexpr (sizeof(T) == 4) {
             if (precision >= 2) {
                 launch_gemm_bf(r, gt, (float*)L, (float*)U, (const float*)inv, precision == 3,
-                               syrk && S.schur_tile == 256, st);
+                               syrk && S.schur_tile == 256, st, syrk && pulls ? pull_.as<PullEnt>() : nullptr,
+                               rel_.as<int32_t>());
                 return;
             }
         }

@@ -122,8 +122,8 @@ struct PullEnt {
     int64_t u_child;        // child's U offset in the update arena (ld = dc3)
     int32_t dc3;            // 3 x child degree
     int32_t rel_off;        // offset of the child's relative indices
-    int32_t ib0, nib;       // child block rows (nib <= 64)
-    int32_t jb0, njb;       // child block columns (njb <= 64)
+    int32_t ib0, nib;       // child block rows (nib <= 128)
+    int32_t jb0, njb;       // child block columns (njb <= 128)
     int32_t base_r, base_c;
 };
 static_assert(sizeof(PullEnt) == 40, "PullEnt layout");
@@ -312,7 +312,7 @@ struct Symbolic {
     std::vector<GemmTask> gemm_tasks;
     std::vector<PullEnt> pull_ents;            // children's update-matrix shares of the Schur
                                                // tiles and of the panel tiles
-    bool pulls_complete = true;                // every Schur tile lists its shares (tiles <= 128)
+    bool pulls_complete = true;                // every Schur tile lists its shares
     std::vector<PanelPullTask> panel_pull_tasks;
     std::vector<TileInj> tile_inj;             // injection entries of the panel tiles
     bool panel_pulls_complete = true;          // the panel tiles list every panel share
@@ -376,7 +376,7 @@ struct SymbolicOptions {
                             // tasks) runs as one fused diagonal-tile + TRSM launch; 0 disables it
     bool pull_lists = true; // children lists in place of the extend-add passes: Schur-update
                             // tiles list the children's update-matrix blocks that land in them
-                            // (added in the tile's epilogue; FP64 and FP32 tile kernels), and
+                            // (added in the tile's epilogue), and
                             // 64 x 64 panel tiles those that land in the parent's panel
     int64_t n_scalar = -1;  // scalar length of the caller's matrix when it is not 3 nb (a scalar
                             // matrix grouped into blocks: scalars n_scalar .. 3 nb - 1 are padding)

@@ -932,7 +932,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 const int32_t rr = (int32_t)(rel0 % D), cc = (int32_t)(rel0 / D);
                 const int32_t Wb = S.swidth[s];
                 const int32_t begin = (int32_t)S.pull_ents.size();
-                if (!S.pulls_complete || g.M > 128 || g.N > 128) { S.pulls_complete = false; return; }
+                if (!S.pulls_complete || g.M > 256 || g.N > 256) { S.pulls_complete = false; return; }
                 for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q) {
                     const int32_t c = S.children[q];
                     const int32_t dc = S.sdeg[c];
@@ -948,7 +948,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     range(rr, g.M, e.ib0, e.nib);
                     range(cc, g.N, e.jb0, e.njb);
                     if (e.nib <= 0 || e.njb <= 0 || e.ib0 + e.nib - 1 < e.jb0) continue;   // nothing lower
-                    if (e.nib > 64 || e.njb > 64) throw Error(-1, "internal: Schur tile pull exceeds 64 blocks");
+                    if (e.nib > 128 || e.njb > 128) throw Error(-1, "internal: Schur tile pull exceeds 128 blocks");
                     // the tile kernel addresses a child's matrix with 32-bit byte offsets
                     if (9 * (int64_t)dc * dc * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
                         S.pulls_complete = false;
