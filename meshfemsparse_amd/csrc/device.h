@@ -164,6 +164,7 @@ private:
     DeviceArray pull_;                     // children's update-matrix shares of the Schur and panel tiles
     DeviceArray ppull_;                    // panel tiles with children shares
     DeviceArray tinj_;                     // their injection entries
+    DeviceArray fillt_;                    // the panel tiles cleared before the injection
     DeviceArray gath_, sblk_, invt_, fdt_, bdt_, fupd_, bprt_, tslot_, Xb_;
     DeviceArray children_, usoff_, rows_, sdeg_, reloff_;
     DeviceArray spmv_ptr_, spmv_ent_, spmv_col_, f2p_, f2r_, r2p_, f2up_, up2i_, cmask_;

@@ -13,7 +13,24 @@ namespace mfs {
 int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::vector<double>& L,
                         std::vector<double>& inv) {
     const int TS = 64;
-    L.assign(S.L_size, 0.0);
+    // the Schur-update tiles' own lists of children's update-matrix blocks, added after the tile
+    // (child order), in place of the extend-add passes into U, and the panel tiles' lists in place
+    // of the passes into the panels; MFS_PULL=0 selects the passes, MFS_PULL=2 makes an analysis
+    // without complete lists an error (replay checks)
+    const char* pe = getenv("MFS_PULL");
+    const int pmode = pe ? atoi(pe) : 1;
+    const bool pull = S.pulls_complete && pmode != 0, ppull = S.panel_pulls_complete && pmode != 0;
+    if (pmode == 2 && !(pull && ppull)) throw Error(-1, "replay: the tiles carry no complete children lists");
+    if (ppull) {
+        // as on the device only the tiles the lists do not rebuild are cleared; the rest starts
+        // as NaN, so an entry read before its tile was rebuilt shows up in the result
+        L.assign(S.L_size, std::nan(""));
+        for (const FillTile& f : S.fill_tiles)
+            for (int c = 0; c < f.N; ++c)
+                for (int r = 0; r < f.M; ++r) L[f.off + (int64_t)c * f.ld + r] = 0.0;
+    } else {
+        L.assign(S.L_size, 0.0);
+    }
     inv.assign((size_t)S.inv_slots * TS * TS, 0.0);
     // the update arena is never cleared on the device: start from NaNs so that any read of an
     // entry that was not written first shows up in the result
@@ -157,14 +174,6 @@ int64_t emulate_numeric(const Symbolic& S, const double* Ax, double sigma, std::
         }
         children(true);
     };
-    // the Schur-update tiles' own lists of children's update-matrix blocks, added after the tile
-    // (child order), in place of the extend-add passes into U, and the panel tiles' lists in place
-    // of the passes into the panels; MFS_PULL=0 selects the passes, MFS_PULL=2 makes an analysis
-    // without complete lists an error (replay checks)
-    const char* pe = getenv("MFS_PULL");
-    const int pmode = pe ? atoi(pe) : 1;
-    const bool pull = S.pulls_complete && pmode != 0, ppull = S.panel_pulls_complete && pmode != 0;
-    if (pmode == 2 && !(pull && ppull)) throw Error(-1, "replay: the tiles carry no complete children lists");
     auto add_lists = [&](double* C, int64_t ldc, int M, int N, int32_t e0, int32_t n) {
         for (int32_t e = e0; e < e0 + n; ++e) {
             const PullEnt& p = S.pull_ents[e];

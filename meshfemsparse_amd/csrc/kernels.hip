@@ -123,6 +123,16 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
         p[i] = v;
 }
 __global__ void k_set_int(int* p, int v) { *p = v; }
+// Zero one tile per workgroup (thread: one row, 16 columns; coalesced column stores).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_tiles(const FillTile* __restrict__ tiles, T* __restrict__ L) {
+    const FillTile t = tiles[blockIdx.x];
+    const int row = threadIdx.x & 63, cq = threadIdx.x >> 6;
+    if (row >= t.M) return;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        if (4 * q + cq < t.N) L[t.off + (int64_t)(4 * q + cq) * t.ld + row] = T(0);
+}
 
 template <typename T>
 static void fill(T* p, int64_t n, T v, hipStream_t s) {
@@ -2730,6 +2740,7 @@ void FactorizerDev::install(std::unique_ptr<Symbolic> S) {
     pull_.upload(S->pull_ents);
     ppull_.upload(S->panel_pull_tasks);
     tinj_.upload(S->tile_inj);
+    fillt_.upload(S->fill_tiles);
     gath_.upload(S->gather_tasks);
     sblk_.upload(S->sblocks);
     invt_.upload(S->inv_tiles);
@@ -2775,7 +2786,15 @@ void FactorizerDev::enqueue_numeric(const double* Ax, double sigma, const double
     T* U = U_.as<T>();
     T* inv = inv_.as<T>();
     prof.begin(KernelProfile::FILL_L, s);
-    fill<T>(L, S.L_size, T(0), s);
+    if (panel_pulls_active()) {
+        // the panel tiles with children lists are rebuilt whole at their level: clear the rest
+        if (!S.fill_tiles.empty()) {
+            k_fill_tiles<T><<<(unsigned)S.fill_tiles.size(), 256, 0, s>>>(fillt_.as<FillTile>(), L);
+            LAUNCH_CHECK();
+        }
+    } else {
+        fill<T>(L, S.L_size, T(0), s);
+    }
     prof.end(s);
     k_set_int<<<1, 1, 0, s>>>(status_.as<int>(), INT_MAX);
     LAUNCH_CHECK();

@@ -150,6 +150,14 @@ struct TileInj {
     int16_t lr, lc;
 };
 static_assert(sizeof(TileInj) == 8, "TileInj layout");
+// A tile (at most 64 x 64) of a panel that no children list rebuilds: L[off + c ld + r] = 0 for
+// r < M, c < N at the start of a numeric factorization.
+struct FillTile {
+    int64_t off;
+    int32_t ld, M, N;
+    int32_t pad;
+};
+static_assert(sizeof(FillTile) == 24, "FillTile layout");
 
 struct PotrfTask {          // factor the kb x kb diagonal tile at L[diag_off], ld; inverse -> slot
     int64_t diag_off;
@@ -315,6 +323,8 @@ struct Symbolic {
     bool pulls_complete = true;                // every Schur tile lists its shares
     std::vector<PanelPullTask> panel_pull_tasks;
     std::vector<TileInj> tile_inj;             // injection entries of the panel tiles
+    std::vector<FillTile> fill_tiles;          // with panel lists: the panel tiles they do not
+                                               // rebuild (the only part of L cleared up front)
     bool panel_pulls_complete = true;          // the panel tiles list every panel share
     std::vector<FusedTask> fused_tasks;
     // solves

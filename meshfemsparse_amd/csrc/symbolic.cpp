@@ -1111,7 +1111,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
     S.bpart_size = std::max<int64_t>(max_part, 1);
 
     // ---- injection entries of the panel tiles (every 3x3 block that meets a tile) ----
-    if (S.panel_pulls_complete && !S.panel_pull_tasks.empty()) {
+    if (S.panel_pulls_complete) {
         std::vector<std::vector<int64_t>> plan_of(ns);     // injection entries per supernode
         for (size_t q = 0; q < S.plan_dst.size(); ++q) {
             const int64_t dst = S.plan_dst[q] >> 2;
@@ -1148,6 +1148,28 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             S.panel_pull_tasks[k].inj_begin = (int32_t)S.tile_inj.size();
             S.panel_pull_tasks[k].ninj = (int32_t)lists[k].size();
             S.tile_inj.insert(S.tile_inj.end(), lists[k].begin(), lists[k].end());
+        }
+        // every other 64 x 64 tile of every panel (the whole (W + D) x W rectangle) is cleared
+        std::vector<int64_t> tfirst(ns, 0), tcount(ns, 0);    // a supernode's tasks are contiguous
+        for (size_t a = S.panel_pull_tasks.size(); a-- > 0;) {
+            tfirst[tile_owner[a]] = (int64_t)a;
+            ++tcount[tile_owner[a]];
+        }
+        std::vector<char> pulled;
+        for (int32_t s = 0; s < ns; ++s) {
+            const int64_t ld = 3 * ((int64_t)S.swidth[s] + S.sdeg[s]);
+            const int32_t W = 3 * S.swidth[s], ntc = (W + 63) / 64, ntr = (int32_t)((ld + 63) / 64);
+            pulled.assign((size_t)ntc * ntr, 0);
+            for (int64_t a = tfirst[s]; a < tfirst[s] + tcount[s]; ++a) {
+                const int64_t o = S.panel_pull_tasks[a].c_off - S.panel_off[s];
+                pulled[(size_t)(o / ld / 64) * ntr + (size_t)(o % ld / 64)] = 1;
+            }
+            for (int32_t tc = 0; tc < ntc; ++tc)
+                for (int32_t tr = 0; tr < ntr; ++tr)
+                    if (!pulled[(size_t)tc * ntr + tr])
+                        S.fill_tiles.push_back({S.panel_off[s] + (int64_t)64 * tc * ld + 64 * tr, (int32_t)ld,
+                                                (int32_t)std::min<int64_t>(64, ld - 64 * tr),
+                                                std::min(64, W - 64 * tc), 0});
         }
     }
 
