@@ -561,7 +561,8 @@ def traffic_passes(args, precision):
             "k_gemm_bf": lambda n: n.startswith("void mfs::k_gemm_bf<"),
             "k_assemble": lambda n: "mfs::k_assemble" in n,
             "solve": lambda n: any(k in n for k in ("mfs::k_sf", "mfs::k_sb", "mfs::k_fgather", "mfs::k_bgather")),
-            "k_fill": lambda n: "mfs::k_fill<" in n, "k_extend": lambda n: "mfs::k_extend<" in n}
+            "k_fill": lambda n: "mfs::k_fill" in n, "k_extend": lambda n: "mfs::k_extend<" in n,
+            "k_panel_pull": lambda n: "mfs::k_panel_pull<" in n}
     out = {"scale_fetch": calib["FETCH_SIZE"], "scale_write": calib["WRITE_SIZE"]}
     for fam, pred in fams.items():
         rb = wb = dur = 0.0
