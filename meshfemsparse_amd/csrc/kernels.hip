@@ -952,12 +952,14 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
             }
         }
         __syncthreads();
-        // buffer loads based at the child's matrix: 32-bit offsets, and an element outside the
-        // child's lower triangle reads the out-of-range offset `nrec`, which returns +0 (v holds
-        // 0 + alpha * acc, never -0, so adding +0 leaves it unchanged bit for bit)
-        const int nrec = pu.dc3 * pu.dc3 * (int)sizeof(T);
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(U + pu.u_child), (short)0, nrec, kBufWord3);
+        // buffer loads based at the entry's first child column (32-bit offsets within its 3 njb
+        // columns), and an element outside the child's lower triangle reads the out-of-range
+        // offset `nrec`, which returns +0 (v holds 0 + alpha * acc, never -0, so adding +0 leaves
+        // it unchanged bit for bit)
+        const int c0s = 3 * pu.jb0;
+        const int nrec = 3 * pu.njb * pu.dc3 * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(U + pu.u_child + (int64_t)c0s * pu.dc3), (short)0, nrec, kBufWord3);
 #ifndef MFS_PULL_FB
 #define MFS_PULL_FB 4
 #endif
@@ -965,7 +967,7 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int cj = cmap[16 * j + (lane & 15)];
-            const int cbase = cj * pu.dc3;
+            const int cbase = (cj - c0s) * pu.dc3;
 #pragma unroll
             for (int i0 = 0; i0 < FM; i0 += FB) {
                 T v[FB][4];
@@ -1045,16 +1047,19 @@ __global__ __launch_bounds__(256) void k_panel_pull(const PanelPullTask* __restr
         }
         __syncthreads();
         const int ri = map[row];
-        const int nrec = pu.dc3 * pu.dc3 * (int)sizeof(T);
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(U + pu.u_child), (short)0, nrec, kBufWord3);
+        // buffer loads based at the entry's first child column (32-bit offsets within its 3 njb
+        // columns; an element without a child entry reads the out-of-range offset nrec)
+        const int c0s = 3 * pu.jb0;
+        const int nrec = 3 * pu.njb * pu.dc3 * (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(U + pu.u_child + (int64_t)c0s * pu.dc3), (short)0, nrec, kBufWord3);
         int cj[16];
         T v[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             cj[q] = map[64 + 4 * q + cq];
             // child lower triangle: scalar row >= scalar column (cj >= 0 then implies ri >= 0)
-            const int off = (cj[q] >= 0 && ri >= cj[q]) ? (cj[q] * pu.dc3 + ri) * (int)sizeof(T) : nrec;
+            const int off = (cj[q] >= 0 && ri >= cj[q]) ? ((cj[q] - c0s) * pu.dc3 + ri) * (int)sizeof(T) : nrec;
             v[q] = buf_load<T>(rs, off, 0);
         }
         // only entries with a child entry are added (an injected -0 stays -0, as in the passes)

@@ -771,10 +771,8 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 for (int32_t q = S.child_ptr[p]; q < S.child_ptr[p + 1]; ++q) {
                     const int32_t c = S.children[q];
                     if (S.sdeg[c] > 0 && S.rel[S.rel_off[c]] < Wb) any = true;
-                    if (9 * (int64_t)S.sdeg[c] * S.sdeg[c] * (int64_t)sizeof(double) >= ((int64_t)1 << 31))
-                        S.panel_pulls_complete = false;   // 32-bit byte offsets into the child's matrix
                 }
-                if (!any || !S.panel_pulls_complete) continue;
+                if (!any) continue;
                 for (int32_t c0 = 0; c0 < W; c0 += 64)
                     for (int32_t r0 = c0; r0 < ld; r0 += 64) {
                         const int32_t M = std::min(64, ld - r0), N = std::min(64, W - c0);
@@ -793,6 +791,9 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                             range(r0, M, e.ib0, e.nib);
                             range(c0, N, e.jb0, e.njb);
                             if (e.nib <= 0 || e.njb <= 0 || e.ib0 + e.nib - 1 < e.jb0) continue;
+                            // the kernel addresses the entry's 3 njb child columns with 32-bit offsets
+                            if (9 * (int64_t)e.njb * dc * (int64_t)sizeof(double) >= ((int64_t)1 << 31))
+                                S.panel_pulls_complete = false;
                             e.u_child = S.u_off[c];
                             e.dc3 = 3 * dc;
                             e.rel_off = (int32_t)S.rel_off[c];
@@ -949,8 +950,8 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                     range(cc, g.N, e.jb0, e.njb);
                     if (e.nib <= 0 || e.njb <= 0 || e.ib0 + e.nib - 1 < e.jb0) continue;   // nothing lower
                     if (e.nib > 128 || e.njb > 128) throw Error(-1, "internal: Schur tile pull exceeds 128 blocks");
-                    // the tile kernel addresses a child's matrix with 32-bit byte offsets
-                    if (9 * (int64_t)dc * dc * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
+                    // the tile kernel addresses the entry's 3 njb child columns with 32-bit offsets
+                    if (9 * (int64_t)e.njb * dc * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) {
                         S.pulls_complete = false;
                         return;
                     }
