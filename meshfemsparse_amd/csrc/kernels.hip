@@ -960,10 +960,7 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
         const int nrec = 3 * pu.njb * pu.dc3 * (int)sizeof(T);
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(U + pu.u_child + (int64_t)c0s * pu.dc3), (short)0, nrec, kBufWord3);
-#ifndef MFS_PULL_FB
-#define MFS_PULL_FB 4
-#endif
-        constexpr int FB = FM < MFS_PULL_FB ? FM : MFS_PULL_FB;   // fragments of loads in flight
+        constexpr int FB = FM < 4 ? FM : 4;   // fragment rows of loads in flight (a whole column)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int cj = cmap[16 * j + (lane & 15)];
