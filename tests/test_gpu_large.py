@@ -67,6 +67,7 @@ def test_newton_step_64cube_fp64(dev):
     print(f"64^3 FP64: nnz(L) {st[3]:.4e}, flops {fl[0]:.4e}, refinement steps {it}, relres {rr:.2e} "
           f"(oracle SpMV {ro:.2e})")
     assert rr <= 1e-10 and ro <= 1e-10
+    assert it == 0          # the FP64 factor solves to 1e-10 without refinement
     assert x[torch.from_numpy(pins).to(dev)].abs().max() == 0
 
 
