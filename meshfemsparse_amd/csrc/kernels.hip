@@ -1733,18 +1733,16 @@ static int gemm_remap(int element_bytes) {
     if (v >= 0) return v;
     return element_bytes == 4 ? 3 : 2;
 }
+// (read at every launch, so a process can compare the kernels)
 static int gemm_version() {
-    static const int v = getenv("MFS_GEMM_V") ? atoi(getenv("MFS_GEMM_V")) : 0;
-    return v;
+    const char* v = getenv("MFS_GEMM_V");
+    return v ? atoi(v) : 0;
 }
 static int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return v ? atoi(v) : dflt;
 }
-static int gemm_version64() {
-    static const int v = env_int("MFS_GEMM_V64", 0);
-    return v;
-}
+static int gemm_version64() { return env_int("MFS_GEMM_V64", 0); }
 static int gemm_kc2() {
     static const int v = env_int("MFS_GEMM_KC2", 16);
     return v;
