@@ -557,7 +557,7 @@ def traffic_passes(args, precision):
         if not cands:
             return None, "calibration copy not found"
         calib[counter] = (2 << 30) / (max(cands)[0] * 1024.0)
-    fams = {"k_gemm": lambda n: n.startswith("void mfs::k_gemm<"),
+    fams = {"k_gemm": lambda n: n.startswith("void mfs::k_gemm<") or n.startswith("void mfs::k_gemm_dl<"),
             "k_gemm_bf": lambda n: n.startswith("void mfs::k_gemm_bf<"),
             "k_assemble": lambda n: "mfs::k_assemble" in n,
             "solve": lambda n: any(k in n for k in ("mfs::k_sf", "mfs::k_sb", "mfs::k_fgather", "mfs::k_bgather")),
@@ -698,7 +698,9 @@ def bench_cube(args):
     peak = PEAK_TFLOPS["bf16" if bf else args.precision]
     kname = (f"k_gemm_bf ({'hi+lo split ' if args.precision == 'bf16x3' else ''}bfloat16 MFMA Schur updates, "
              f"FP32 accumulation; algorithmic flops)" if bf else
-             f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)")
+             (f"k_gemm_dl (FP64 MFMA tile GEMM, operands loaded straight into LDS: TRSM, in-panel update, "
+              f"Schur update)" if args.precision == "fp64" else
+              f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)"))
     roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None,
