@@ -1222,33 +1222,42 @@ __device__ __forceinline__ void gemm_tile(const GemmTask& t, T* __restrict__ L, 
     gemm_store_lds<T, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region, npull > 0);
 }
 
-// FP64 tile GEMM (128x128 or 64x64 tiles) with its operand chunks loaded straight into LDS
-// (buffer_load_dwordx4 ... lds: lane l of a wave writes its 16 bytes at the wave's LDS pointer +
-// 16 l), four stages of KC = 8 k rows in flight, no register staging.  Same task semantics,
-// fragment reads, MFMA order and epilogue arithmetic as gemm_tile<double, TM, TM, 16> (k ascending
-// in steps of 4).  One instruction fills a 128-double row (128x128 tiles: wave w loads k rows w and
-// w + 4 of each operand) or two 64-double rows (64x64 tiles, unpadded rows: wave w loads rows 2w
-// and 2w + 1).  A row of a chunk beyond K reads an out-of-range offset and lands as zeros.  Rows
-// beyond the tile's M (N) read the memory that follows in the operand array (they feed only
-// outputs that are never stored); a chunk's resource ends TM doubles after its last column's
-// start, which the factor arrays' allocation tails keep inside the allocation.  The stages are
-// separate LDS arrays and the chunk loop is unrolled by four, so the compiler sees that a stage's
-// loads never alias the stage being read and waits only where the code says.
-#define MFS_DL_STAGE(n) __shared__ __attribute__((aligned(16))) double As##n[8][LW], Bs##n[8][LW];
-template <int TM, bool PULL = false>
-__global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__ tasks, double* __restrict__ L,
-                                                    double* __restrict__ U, const double* __restrict__ inv,
-                                                    int remap, const PullEnt* __restrict__ pulls,
+// Tile GEMM (FP64 or FP32, 128x128 or 64x64 tiles) with its operand chunks loaded straight into
+// LDS (buffer_load_dwordx4 ... lds: lane l of a wave writes its 16 bytes at the wave's LDS pointer
+// + 16 l), four stages of KC k rows in flight, no register staging.  Same task semantics, fragment
+// reads, MFMA order and epilogue arithmetic as gemm_tile<T, TM, TM, 16> (k ascending in steps of
+// 4), so the results are the same bits.  One instruction fills 1024 bytes: one 128-double row
+// (FP64 128x128 tiles, rows padded by 16 doubles: wave w loads k rows w and w + 4 of each operand)
+// or RPI = 2 or 4 unpadded rows (wave w loads rows RPI w .. RPI w + RPI - 1).  FP32 rows hold their
+// 4-float groups XOR-swizzled by the row (float m of row r at m ^ 16 (r & 3)), which puts the
+// four rows of a fragment read in different banks as the padding does in the register-staged
+// kernel.  A row of a chunk beyond K reads an out-of-range offset and lands as zeros.  Rows beyond
+// the tile's M (N) read the memory that follows in the operand array (they feed only outputs that
+// are never stored); a chunk's resource ends TM elements after its last column's start, which the
+// factor arrays' allocation tails keep inside the allocation.  The stages are separate LDS arrays
+// and the chunk loop is unrolled by four, so the compiler sees that a stage's loads never alias
+// the stage being read and waits only where the code says.
+#define MFS_DL_STAGE(n) __shared__ __attribute__((aligned(16))) T As##n[KC][LW], Bs##n[KC][LW];
+template <typename T, int TM, bool PULL = false>
+__global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__ tasks, T* __restrict__ L,
+                                                    T* __restrict__ U, const T* __restrict__ inv, int remap,
+                                                    const PullEnt* __restrict__ pulls,
                                                     const int32_t* __restrict__ rel) {
-    typedef d4 acc_t;
+    typedef typename Mfma<T>::acc_t acc_t;
     static_assert(TM == 128 || TM == 64, "tile");
-    constexpr int KC = 8, LW = TM == 128 ? 144 : 64, WM = TM / 2, WN = TM / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int LPC = TM == 128 ? 4 : 2;            // loads per wave and chunk
+    constexpr int ES = (int)sizeof(T), RPI = 1024 / (TM * ES);              // rows per instruction
+    constexpr int KC = (ES == 4 && TM == 64) ? 16 : 8;
+    constexpr int IPW = KC / (4 * RPI);                                     // instructions per wave
+    constexpr bool PADDED = RPI == 1, SWZ = ES == 4;
+    constexpr int LW = PADDED ? TM + 16 : TM, WM = TM / 2, WN = TM / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int LPC = 2 * IPW;                                            // loads per wave and chunk
+    constexpr int GE = 16 / ES;                                             // elements per lane
+    static_assert(IPW >= 1 && IPW * 4 * RPI == KC, "chunk shape");
     MFS_DL_STAGE(0) MFS_DL_STAGE(1) MFS_DL_STAGE(2) MFS_DL_STAGE(3)
     const GemmTask t = tasks[xcd_task(blockIdx.x, gridDim.x, remap)];
-    const double* A = L + t.a_off;
-    const double* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
-    double* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
+    const T* A = L + t.a_off;
+    const T* B = (t.flags & GT_B_IN_INV) ? inv + t.b_off : L + t.b_off;
+    T* C = (t.flags & GT_C_IN_U) ? U + t.c_off : L + t.c_off;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave & 1) * WM, wn = (wave >> 1) * WN;
@@ -1259,46 +1268,51 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
     const int nch = (K + KC - 1) / KC;
-    auto issue = [&](int c, double (*as)[LW], double (*bs)[LW]) {
+    auto issue = [&](int c, T (*as)[LW], T (*bs)[LW]) {
         // (a chunk past the last one: every row out of range, an empty resource)
         const int k0 = min(c, nch - 1) * KC, kv = c < nch ? min(KC, K - k0) : 0;
         const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(A + (int64_t)k0 * lda), (short)0, kv > 0 ? ((kv - 1) * lda + TM) * 8 : 0, kBufWord3);
+            (void*)(A + (int64_t)k0 * lda), (short)0, kv > 0 ? ((kv - 1) * lda + TM) * ES : 0, kBufWord3);
         const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(B + (int64_t)k0 * ldb), (short)0, kv > 0 ? ((kv - 1) * ldb + TM) * 8 : 0, kBufWord3);
-        if constexpr (TM == 128) {
+            (void*)(B + (int64_t)k0 * ldb), (short)0, kv > 0 ? ((kv - 1) * ldb + TM) * ES : 0, kBufWord3);
+        if constexpr (PADDED) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < IPW; ++h) {
                 const int kk = wave + 4 * h;
-                const int sa = kk < kv ? kk * lda * 8 : 0x7fff0000, sb = kk < kv ? kk * ldb * 8 : 0x7fff0000;
+                const int sa = kk < kv ? kk * lda * ES : 0x7fff0000, sb = kk < kv ? kk * ldb * ES : 0x7fff0000;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rA, (__attribute__((address_space(3))) void*)&as[kk][0], 16, lane * 16, sa, 0, 0);
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rB, (__attribute__((address_space(3))) void*)&bs[kk][0], 16, lane * 16, sb, 0, 0);
             }
         } else {
-            const int kk = 2 * wave + (lane >> 5), m2 = (lane & 31) * 16;
-            const int va = kk < kv ? (kk * lda) * 8 + m2 : 0x7fff0000;
-            const int vb = kk < kv ? (kk * ldb) * 8 + m2 : 0x7fff0000;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rA, (__attribute__((address_space(3))) void*)&as[2 * wave][0], 16, va, 0, 0, 0);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                rB, (__attribute__((address_space(3))) void*)&bs[2 * wave][0], 16, vb, 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < IPW; ++h) {
+                const int r0 = RPI * (wave * IPW + h);                // first row of the instruction
+                const int kk = r0 + lane / (64 / RPI), g = lane % (64 / RPI);
+                const int m = (SWZ ? (g ^ (4 * (kk & 3))) : g) * GE;   // first element this lane fetches
+                const int va = kk < kv ? (kk * lda + m) * ES : 0x7fff0000;
+                const int vb = kk < kv ? (kk * ldb + m) * ES : 0x7fff0000;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rA, (__attribute__((address_space(3))) void*)&as[r0][0], 16, va, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rB, (__attribute__((address_space(3))) void*)&bs[r0][0], 16, vb, 0, 0, 0);
+            }
         }
     };
-    auto compute = [&](const double (*as)[LW], const double (*bs)[LW]) {
+    auto compute = [&](const T (*as)[LW], const T (*bs)[LW]) {
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
-            const int kr = ks + (lane >> 4), cc = lane & 15;
-            double af[FM], bf[FN];
+            const int kr = ks + (lane >> 4), cc = lane & 15, sx = SWZ ? 16 * (kr & 3) : 0;
+            T af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = as[kr][wm + 16 * i + cc];
+            for (int i = 0; i < FM; ++i) af[i] = as[kr][((wm + 16 * i) ^ sx) + cc];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bf[j] = bs[kr][wn + 16 * j + cc];
+            for (int j = 0; j < FN; ++j) bf[j] = bs[kr][((wn + 16 * j) ^ sx) + cc];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<double>::mma(af[i], bf[j], acc[i][j]);
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
         }
     };
     // chunk c: wait until this wave's loads of c are done (the two later chunks' loads remain:
@@ -1306,7 +1320,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__
     // LDS reads of chunk c - 1 have returned (the scheduler may not move them past this point),
     // barrier (every wave's loads of c have landed; every wave is done reading chunk c - 1), issue
     // chunk c + 3 into the stage of chunk c - 1, compute chunk c
-    auto step = [&](int c, double (*as)[LW], double (*bs)[LW], double (*an)[LW], double (*bn)[LW]) {
+    auto step = [&](int c, T (*as)[LW], T (*bs)[LW], T (*an)[LW], T (*bn)[LW]) {
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(0x0070 | (2 * LPC));   // vmcnt(2 LPC), lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
@@ -1325,18 +1339,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);   // the empty loads past the end have landed too
     __syncthreads();         // every wave is done with the operand stages
-    const double alpha = (t.flags & GT_ALPHA_POS) ? 1.0 : -1.0;
+    const T alpha = (t.flags & GT_ALPHA_POS) ? T(1) : T(-1);
     const bool beta0 = (t.flags & GT_BETA0) != 0;
-    // epilogue per wave in its own stage array: 128x128 tiles in four 64x16 parts (1024 of 1152
-    // doubles), 64x64 tiles in two 32x16 parts (512 of 512)
+    // epilogue per wave in its own stage array: 128x128 tiles in four 64x16 parts, 64x64 tiles in
+    // two 32x16 parts
     constexpr int NP = TM == 128 ? 4 : 2;
-    static_assert(WM * WN / NP <= 8 * LW, "epilogue region");
-    double* region = wave == 0 ? &As0[0][0] : wave == 1 ? &As1[0][0] : wave == 2 ? &Bs0[0][0] : &Bs1[0][0];
+    static_assert(WM * WN / NP <= KC * LW, "epilogue region");
+    T* region = wave == 0 ? &As0[0][0] : wave == 1 ? &As1[0][0] : wave == 2 ? &Bs0[0][0] : &Bs1[0][0];
     const int npull = PULL ? (t.flags >> GT_PULL_SHIFT) : 0;
     if (npull > 0)
-        apply_pulls<double, FM, FN>(acc, alpha, reinterpret_cast<int*>(region), pulls + t.pad, npull, rel, U, wm,
-                                    wn, lane);
-    gemm_store_lds<double, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region, npull > 0);
+        apply_pulls<T, FM, FN>(acc, alpha, reinterpret_cast<int*>(region), pulls + t.pad, npull, rel, U, wm, wn,
+                               lane);
+    gemm_store_lds<T, FM, FN, NP>(acc, C, t.ldc, t.M, t.N, wm, wn, lane, alpha, beta0, region, npull > 0);
 }
 #undef MFS_DL_STAGE
 
@@ -1786,13 +1800,9 @@ static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, c
     const int ver64 = gemm_version64() ? gemm_version64() : ver;
     if (r.m > r.b) {
         if (ver == 2 && !pulls) launch_gemm2<T, 128>(gemm_kc2(), (unsigned)(r.m - r.b), tasks + r.b, L, U, inv, rm, s);
-        else if (sizeof(T) == 8 && ver == 3) {
-            if (pulls)
-                k_gemm_dl<128, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, (double*)L, (double*)U,
-                                                                         (const double*)inv, rm, pulls, rel);
-            else
-                k_gemm_dl<128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, (double*)L, (double*)U,
-                                                                   (const double*)inv, rm, nullptr, nullptr);
+        else if (ver == 3) {
+            if (pulls) k_gemm_dl<T, 128, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm, pulls, rel);
+            else k_gemm_dl<T, 128><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm, nullptr, nullptr);
             LAUNCH_CHECK();
         } else {
             if (pulls) k_gemm<T, 128, 128, 16, true><<<(unsigned)(r.m - r.b), 256, 0, s>>>(tasks + r.b, L, U, inv, rm, pulls, rel);
@@ -1802,13 +1812,9 @@ static void launch_gemm(const TaskRange& r, const GemmTask* tasks, T* L, T* U, c
     }
     if (r.e > r.m) {
         if (ver64 == 2 && !pulls) launch_gemm2<T, 64>(gemm_kc64(), (unsigned)(r.e - r.m), tasks + r.m, L, U, inv, rm, s);
-        else if (sizeof(T) == 8 && ver64 == 3) {
-            if (pulls)
-                k_gemm_dl<64, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, (double*)L, (double*)U,
-                                                                        (const double*)inv, rm, pulls, rel);
-            else
-                k_gemm_dl<64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, (double*)L, (double*)U,
-                                                                  (const double*)inv, rm, nullptr, nullptr);
+        else if (ver64 == 3) {
+            if (pulls) k_gemm_dl<T, 64, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm, pulls, rel);
+            else k_gemm_dl<T, 64><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm, nullptr, nullptr);
             LAUNCH_CHECK();
         } else {
             if (pulls) k_gemm<T, 64, 64, 16, true><<<(unsigned)(r.e - r.m), 256, 0, s>>>(tasks + r.m, L, U, inv, rm, pulls, rel);

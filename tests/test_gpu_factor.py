@@ -191,19 +191,18 @@ def test_tile_children_lists_match_extend_add(dev, N, fuse, form, monkeypatch):
     assert np.array_equal(Lx2.view(np.uint8), Lx0.view(np.uint8))
 
 
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("N", [10, 16])
-def test_direct_load_tile_gemm_matches_register_staged(dev, N, monkeypatch):
-    """The FP64 tile GEMM that loads its operand chunks straight into LDS (the default) and the
-    one that stages them through registers (MFS_GEMM_V=1) issue the same MFMAs in the same order,
-    so they give the same factor, bit for bit, eagerly and replayed as a captured graph."""
+def test_direct_load_tile_gemm_matches_register_staged(dev, N, single, monkeypatch):
+    """The tile GEMM that loads its operand chunks straight into LDS (MFS_GEMM_V=3, the FP64
+    default) and the one that stages them through registers (MFS_GEMM_V=1, the FP32 default)
+    issue the same MFMAs in the same order, so they give the same factor, bit for bit, eagerly and
+    replayed as a captured graph."""
     H, pins, _, _, _ = mesh_problem(dev, N)
 
     def factor(v, graph=False):
-        if v:
-            monkeypatch.setenv("MFS_GEMM_V", str(v))
-        else:
-            monkeypatch.delenv("MFS_GEMM_V", raising=False)
-        f = mfs.make_cholesky_factorizer()
+        monkeypatch.setenv("MFS_GEMM_V", str(v))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single)
         f.factorizeSymbolic(H, pins)
         if graph:
             f.setGraphCapture(True)
@@ -215,8 +214,8 @@ def test_direct_load_tile_gemm_matches_register_staged(dev, N, monkeypatch):
         return Lx
 
     L1 = factor(1)
-    assert np.array_equal(factor(0).view(np.uint8), L1.view(np.uint8))
-    assert np.array_equal(factor(0, graph=True).view(np.uint8), L1.view(np.uint8))
+    assert np.array_equal(factor(3).view(np.uint8), L1.view(np.uint8))
+    assert np.array_equal(factor(3, graph=True).view(np.uint8), L1.view(np.uint8))
 
 
 def test_fused_panel_step_reports_failing_pivot(dev, monkeypatch):
