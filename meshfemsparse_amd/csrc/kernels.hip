@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_dl(const GemmTask* __restrict__
     const int nch = (K + KC - 1) / KC;
     auto issue = [&](int c, T (*as)[LW], T (*bs)[LW]) {
         // (a chunk past the last one: every row out of range, an empty resource)
-        const int k0 = min(c, nch - 1) * KC, kv = c < nch ? min(KC, K - k0) : 0;
+        const int k0 = max(0, min(c, nch - 1)) * KC, kv = c < nch ? min(KC, K - k0) : 0;
         const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(A + (int64_t)k0 * lda), (short)0, kv > 0 ? ((kv - 1) * lda + TM) * ES : 0, kBufWord3);
         const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
@@ -3761,7 +3761,19 @@ void test_gemm_nt(int M, int N, int K, T alpha, const T* A, int lda, const T* B,
     r.b = 0;
     r.m = TT == 128 ? (int64_t)tasks.size() : 0;
     r.e = (int64_t)tasks.size();
-    launch_gemm<T>(r, d.as<GemmTask>(), const_cast<T*>(A), C, B, s);
+    // the operands in arrays with the factor arrays' tails (the direct-to-LDS kernel reads up to a
+    // tile height past a column's start)
+    DeviceArray ap, bp;
+    const size_t ab = (size_t)std::max(K, 1) * lda * sizeof(T), bb = (size_t)std::max(K, 1) * ldb * sizeof(T);
+    ap.alloc(ab + 256 * sizeof(T));
+    bp.alloc(bb + 256 * sizeof(T));
+    HIP_CHECK(hipMemsetAsync(ap.p, 0, ap.bytes, s));
+    HIP_CHECK(hipMemsetAsync(bp.p, 0, bp.bytes, s));
+    if (K > 0) {
+        HIP_CHECK(hipMemcpyAsync(ap.p, A, (size_t)K * lda * sizeof(T), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(bp.p, B, (size_t)K * ldb * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    launch_gemm<T>(r, d.as<GemmTask>(), ap.as<T>(), C, bp.as<T>(), s);
     HIP_CHECK(hipStreamSynchronize(s));
 }
 void test_gemm_nt_bf16(int M, int N, int K, float alpha, const float* A, int lda, const float* B, int ldb,
