@@ -323,8 +323,8 @@ struct Symbolic {
     bool pulls_complete = true;                // every Schur tile lists its shares
     std::vector<PanelPullTask> panel_pull_tasks;
     std::vector<TileInj> tile_inj;             // injection entries of the panel tiles
-    std::vector<FillTile> fill_tiles;          // with panel lists: the panel tiles they do not
-                                               // rebuild (the only part of L cleared up front)
+    std::vector<FillTile> fill_tiles;          // with panel lists: the lower-part panel tiles
+                                               // they do not rebuild (cleared up front)
     bool panel_pulls_complete = true;          // the panel tiles list every panel share
     std::vector<FusedTask> fused_tasks;
     // solves

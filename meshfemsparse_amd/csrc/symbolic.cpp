@@ -1150,7 +1150,8 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
             S.panel_pull_tasks[k].ninj = (int32_t)lists[k].size();
             S.tile_inj.insert(S.tile_inj.end(), lists[k].begin(), lists[k].end());
         }
-        // every other 64 x 64 tile of every panel (the whole (W + D) x W rectangle) is cleared
+        // every other 64 x 64 tile of every panel's lower part is cleared (tiles wholly above the
+        // diagonal hold only entries above it, which no kernel reads)
         std::vector<int64_t> tfirst(ns, 0), tcount(ns, 0);    // a supernode's tasks are contiguous
         for (size_t a = S.panel_pull_tasks.size(); a-- > 0;) {
             tfirst[tile_owner[a]] = (int64_t)a;
@@ -1166,7 +1167,7 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 pulled[(size_t)(o / ld / 64) * ntr + (size_t)(o % ld / 64)] = 1;
             }
             for (int32_t tc = 0; tc < ntc; ++tc)
-                for (int32_t tr = 0; tr < ntr; ++tr)
+                for (int32_t tr = tc; tr < ntr; ++tr)
                     if (!pulled[(size_t)tc * ntr + tr])
                         S.fill_tiles.push_back({S.panel_off[s] + (int64_t)64 * tc * ld + 64 * tr, (int32_t)ld,
                                                 (int32_t)std::min<int64_t>(64, ld - 64 * tr),
