@@ -913,7 +913,8 @@ template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsr
 // additions, in the same order, as the extend-add passes into U after the Schur update.  Per entry
 // the wave's rows and columns are mapped to the child's scalar rows and columns (-1: none) through
 // LDS (`map`, WM + WN ints of the wave's epilogue region), then every fragment element with a
-// child entry in the child's lower triangle loads it (one fragment column of loads in flight).
+// child entry in the child's lower triangle loads it (up to four fragment rows of one fragment
+// column, 16 loads, in flight).
 template <typename T, int FM, int FN>
 __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][FN], T alpha,
                                             int* __restrict__ map, const PullEnt* __restrict__ pe, int npull,
@@ -989,8 +990,8 @@ __device__ __forceinline__ void apply_pulls(typename Mfma<T>::acc_t (&acc)[FM][F
 
 // The children's update-matrix blocks that land in a 64 x 64 tile of a parent's panel, added to
 // the tile in child order: the additions, in the order, of the extend-add passes into the panel.
-// The tile's starting values are rebuilt in LDS from its injection entries (zero elsewhere: the
-// values the fill and the injection left in L), so L is only written.  Thread (row = tid % 64,
+// The tile's starting values are rebuilt in LDS from its injection entries (zero elsewhere, as the
+// clear and the injection make the tiles without lists), so L is only written.  Thread (row = tid % 64,
 // columns 4 q + tid / 64) then holds 16 entries of the tile; per child the block rows and columns
 // are mapped to the child's scalar rows and columns through LDS, and every entry with a child
 // entry in the child's lower triangle loads it and adds it.
