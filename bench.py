@@ -700,7 +700,8 @@ def bench_cube(args):
              f"FP32 accumulation; algorithmic flops)" if bf else
              (f"k_gemm_dl (FP64 MFMA tile GEMM, operands loaded straight into LDS: TRSM, in-panel update, "
               f"Schur update)" if args.precision == "fp64" else
-              f"k_gemm ({args.precision.upper()} MFMA tile GEMM: TRSM, in-panel update, Schur update)"))
+              f"k_gemm_dl / k_gemm ({args.precision.upper()} MFMA tile GEMM, 128x128 tiles loaded straight into "
+              f"LDS: TRSM, in-panel update, Schur update)"))
     roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None,

@@ -194,10 +194,10 @@ def test_tile_children_lists_match_extend_add(dev, N, fuse, form, monkeypatch):
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("N", [10, 16])
 def test_direct_load_tile_gemm_matches_register_staged(dev, N, single, monkeypatch):
-    """The tile GEMM that loads its operand chunks straight into LDS (MFS_GEMM_V=3, the FP64
-    default) and the one that stages them through registers (MFS_GEMM_V=1, the FP32 default)
-    issue the same MFMAs in the same order, so they give the same factor, bit for bit, eagerly and
-    replayed as a captured graph."""
+    """The tile GEMM that loads its operand chunks straight into LDS (MFS_GEMM_V=3: every tile;
+    the default uses it for the 128x128 tiles, and for the 64x64 ones in FP64) and the one that
+    stages them through registers (MFS_GEMM_V=1: every tile) issue the same MFMAs in the same
+    order, so they give the same factor, bit for bit, eagerly and replayed as a captured graph."""
     H, pins, _, _, _ = mesh_problem(dev, N)
 
     def factor(v, graph=False):
