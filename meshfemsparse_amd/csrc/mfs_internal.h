@@ -368,6 +368,7 @@ struct SymbolicOptions {
     int tile = 64;          // diagonal tile / inner step width (scalars)
     int outer = 512;        // outer panel block: trailing update with K = outer
     int big_tile = 128;     // GEMM regions at least this large in both dims use 128x128 tiles
+    int split_big = 256;    // in-panel and next-block trailing updates with fewer 128x128 tiles use 64x64
     // Supernode size buckets by front size W + D (scalars): [0, 512), [512, 2048), [2048, inf).
     // bucket_tile[b] is the largest GEMM tile of the supernodes of bucket b: 128 (128x128 tiles
     // where the region is large enough; 256 for the bfloat16 Schur kernel) or 64 (64x64 tiles

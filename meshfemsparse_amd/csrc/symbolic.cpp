@@ -210,6 +210,7 @@ SymbolicOptions options_from_env() {
     if (const char* e = getenv("MFS_TILE_GROUP")) opt.tile_group = std::max(1, atoi(e));
     if (const char* e = getenv("MFS_FUSE_DEGREE")) opt.fuse_degree = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_BIG_TILE")) opt.big_tile = std::max(128, atoi(e));
+    if (const char* e = getenv("MFS_SPLIT_BIG")) opt.split_big = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_PANEL_MAX")) opt.panel_max = std::max(0, atoi(e));
     if (const char* e = getenv("MFS_LOOKAHEAD")) opt.lookahead = atoi(e) != 0;
     if (const char* e = getenv("MFS_PULL")) opt.pull_lists = atoi(e) != 0;
@@ -897,25 +898,41 @@ Symbolic analyze(const int64_t* Ap, const int64_t* Ai, int64_t nb, const int64_t
                 // step's rest range, which may run beside the next outer block's steps.
                 std::vector<GemmTask> big, small, rbig, rsmall;
                 bool trailing = false;
-                for (auto [s, slot] : active) {
-                    (void)slot;
-                    const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
-                    const int32_t k0 = t * T, kb = std::min(T, W - k0), k1 = k0 + kb;
-                    const int32_t J0 = (k0 / OB) * OB, Jend = std::min(W, J0 + OB);
-                    const int64_t off = S.panel_off[s];
-                    if (k1 < Jend) {
-                        const int32_t kb1 = std::min(T, W - k1);
-                        emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
-                                   k1, k1 + kb1, W + D, k1 - J0, 0, big_ok(s));
-                    } else if (Jend < W) {
-                        trailing = true;
-                        const int32_t split = opt.lookahead ? std::min(W, Jend + OB) : W;
-                        emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off, ld,
-                                   Jend, split, W + D, Jend - J0, 0, big_ok(s));
-                        if (split < W)
-                            emit_lower(rbig, rsmall, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off,
-                                       ld, split, W, W + D, Jend - J0, 0, big_ok(s));
+                auto emit_step = [&](bool allow_big) {
+                    for (auto [s, slot] : active) {
+                        (void)slot;
+                        const int32_t W = 3 * S.swidth[s], D = 3 * S.sdeg[s], ld = W + D;
+                        const int32_t k0 = t * T, kb = std::min(T, W - k0), k1 = k0 + kb;
+                        const int32_t J0 = (k0 / OB) * OB, Jend = std::min(W, J0 + OB);
+                        const int64_t off = S.panel_off[s];
+                        if (k1 < Jend) {
+                            const int32_t kb1 = std::min(T, W - k1);
+                            emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off,
+                                       ld, k1, k1 + kb1, W + D, k1 - J0, 0, allow_big && big_ok(s));
+                        } else if (Jend < W) {
+                            trailing = true;
+                            const int32_t split = opt.lookahead ? std::min(W, Jend + OB) : W;
+                            emit_lower(big, small, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld, off,
+                                       ld, Jend, split, W + D, Jend - J0, 0, allow_big && big_ok(s));
+                            if (split < W)
+                                emit_lower(rbig, rsmall, off + (int64_t)J0 * ld, ld, off + (int64_t)J0 * ld, ld,
+                                           off, ld, split, W, W + D, Jend - J0, 0, big_ok(s));
+                        }
                     }
+                };
+                emit_step(true);
+                // a critical-path launch of few 128x128 tiles leaves CUs idle or gives some CUs
+                // two tiles: it then runs on 64x64 tiles (four per 128x128 tile) instead.  The
+                // two tile kernels issue the same MFMAs per entry in the same order, so the factor
+                // is bitwise the same.  Measured on the 64^3 step (MFS_SPLIT_BIG, one box): below
+                // 256 tiles FP64 254.1 -> 253.3 ms, FP32 154.3 -> 153.7 ms; 128 to 1000 all
+                // land within 0.2 % of that.
+                if (!big.empty() && (int64_t)big.size() < opt.split_big) {
+                    big.clear();
+                    small.clear();
+                    rbig.clear();
+                    rsmall.clear();
+                    emit_step(false);
                 }
                 P.update_steps.push_back(push_tasks(S.gemm_tasks, big, small));
                 P.rest_steps.push_back(push_tasks(S.gemm_tasks, rbig, rsmall));

@@ -192,6 +192,27 @@ def test_tile_children_lists_match_extend_add(dev, N, fuse, form, monkeypatch):
 
 
 @pytest.mark.parametrize("single", [False, True])
+def test_small_critical_updates_on_64_tiles_bitwise(dev, single, monkeypatch):
+    """An update launch on the per-step critical path with few 128x128 tiles runs on 64x64 tiles
+    (MFS_SPLIT_BIG: the tile-count bound).  Either tile adds the same products in the same order,
+    so the factor is bitwise the one of the 128x128 tiles."""
+    H, pins, _, _, _ = mesh_problem(dev, 16)
+
+    def factor(split):
+        monkeypatch.setenv("MFS_SPLIT_BIG", str(split))
+        f = mfs.make_cholesky_factorizer(mfs.CholeskyProvider.GpuSupernodal, single, None)
+        f.factorizeSymbolic(H, pins)
+        f.factorizeNumeric(H)
+        return np.asarray(f.exportFactorCSC()[2]).copy()
+
+    # at 16^3 the top separator (867 columns) gets a next-outer-block trailing update of a few
+    # 128x128 tiles (512-column outer blocks), which the large bound splits
+    L0 = factor(0)
+    L1 = factor(100000)
+    assert np.array_equal(L0.view(np.uint8), L1.view(np.uint8))
+
+
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("N", [10, 16])
 def test_direct_load_tile_gemm_matches_register_staged(dev, N, single, monkeypatch):
     """The tile GEMM that loads its operand chunks straight into LDS (MFS_GEMM_V=3: every tile;
