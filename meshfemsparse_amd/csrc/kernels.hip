@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_assemble(const int32_t* __restrict__ se
     };
     // batches of 8 contributions: their loads issue together (past the end a batch re-reads the
     // last contribution and skips it), the values are added in ascending element order
-    constexpr int B = 8;
+    constexpr int B = 8;                       // 4: the same time; 16: 15 % slower
     if (staged) {
         for (int32_t k = b; k < e_; k += B) {
             uint32_t w[B];
